@@ -1,0 +1,349 @@
+"""CPU oracle for the genotype hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import this
+package, and only as the checker.  The product (``cosmicscanner_amd``) never imports it.
+
+Two layers:
+
+* ``libcsoracle.so`` (``cs_oracle.c``, built by ``oracle/Makefile``): FP64 / exact-integer
+  restatements of the reference's integer and byte work and of its matrix-free operator.
+* numpy (this file): the small dense linear algebra of the null model and the GLS identities.
+
+Pinning.  Each function and the checks that pin it (tests/test_oracle_*.py):
+
+==========================  =====================================================================
+function                    pinned by
+==========================  =====================================================================
+stats / class_counts        closed forms on the committed reference BEDs (examples/dataset/synth,
+                            tests/data/gemma_accuracy, tests/data/eb_format_consistency: counts
+                            re-derived with an independent numpy unpack), the kinship-batch
+                            fixture's monomorphic (j%31==0 -> inv_std 0) and all-missing
+                            (j%37==0) SNPs, and the partial last byte (n = 37)
+kinship_apply/trace/diag    brute-force dense z z^T v / M built from an independent numpy decode
+                            (the reference's own test, test_kinship_batch.cpp:61-86), symmetry,
+                            linearity, principal-submatrix and LOCO additivity identities
+grm (alg 0/1)               dense numpy Z Z^T from float-rounded means (grm_builder.cpp:240),
+                            mean diag ~ 1 (alg 0, no missingness), exact path agreement
+grm_exact_centred           int128 sum vs python arbitrary-precision integers on tiny inputs,
+                            and |exact - FP64 GEMM| within FP64 rounding
+marker / scan_sums          numpy re-derivation on the keep subset (genotype_encoding.h:33-81)
+chisq_prob                  scipy.special.chdtrc (closed form) to 1e-6 relative, P(stat<=0)=1
+eigen_vce_prepare/solve     textbook REML: brute-force grid over delta on the explicit
+                            V = G + delta I likelihood; invariance to the null-space basis
+rotated_scan / gls_scan     explicit P = V^-1 - V^-1 X (X^T V^-1 X)^-1 X^T V^-1 on n <= 500
+                            (pattern of tests/test_weighted_lmm_equivalence.py:13-22)
+gen_hwe_bed                 allele-frequency and missing-rate moments; GPU byte equality
+==========================  =====================================================================
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libcsoracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i8p = ctypes.POINTER(ctypes.c_int8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i64 = ctypes.c_int64
+
+
+def build() -> None:
+    """Compile ``libcsoracle.so`` with the committed Makefile."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_stats.argtypes = [_u8p, _i64, _i64, _dp, _dp, ctypes.c_int]
+        L.orc_class_counts.argtypes = [_u8p, _i64, _i64, _i32p, _i64p, ctypes.c_int]
+        L.orc_kinship_apply.argtypes = [_u8p, _i64, _dp, _dp, _i32p, _i64, _i32p, _i64, _dp, _i64,
+                                        _dp, ctypes.c_int]
+        L.orc_kinship_trace.argtypes = [_u8p, _i64, _dp, _dp, _i32p, _i64, _i32p, _i64]
+        L.orc_kinship_trace.restype = ctypes.c_double
+        L.orc_kinship_diag.argtypes = [_u8p, _i64, _dp, _dp, _i32p, _i64, _i32p, _i64, _dp]
+        L.orc_grm_snp_params.argtypes = [_u8p, _i64, _i64, ctypes.c_int, ctypes.c_float, _i8p,
+                                         _fp, _fp, _fp, _dp, _i64p, ctypes.c_int]
+        L.orc_grm.argtypes = [_u8p, _i64, _i64, ctypes.c_int, ctypes.c_float, _dp, _dp, _i64p,
+                              ctypes.c_int]
+        L.orc_grm_exact_centred.argtypes = [_u8p, _i64, _i64, ctypes.c_float, _dp, _fp, _dp,
+                                            _i64p, ctypes.c_int]
+        L.orc_grm_exact_centred.restype = ctypes.c_int
+        L.orc_scan_sums.argtypes = [_u8p, _i64, _i64, _i32p, _i64, ctypes.c_double, _dp, _i64,
+                                    _dp, _i64p, _i32p, _dp, _dp, ctypes.c_int]
+        L.orc_chisq_prob.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_chisq_prob.restype = ctypes.c_double
+        L.orc_chisq_prob_vec.argtypes = [_dp, _i64, ctypes.c_int, _dp]
+        L.orc_gen_hwe_bed.argtypes = [ctypes.c_uint64, _i64, _i64, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, _u8p, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _nthreads(n=None):
+    return int(n or os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _u8(bed):
+    return np.ascontiguousarray(bed, dtype=np.uint8)
+
+
+def _i32opt(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+# --------------------------------------------------------------------------- BED files
+def read_bed(prefix):
+    """Read ``prefix.{bed,bim,fam}``: returns (body uint8 array, N, M, bim rows, fam rows).
+
+    Checks the SNP-major magic ``6C 1B 01`` (plink_reader.cpp:67-92)."""
+    with open(prefix + ".fam") as f:
+        fam = [ln.split() for ln in f if ln.strip()]
+    with open(prefix + ".bim") as f:
+        bim = [ln.split() for ln in f if ln.strip()]
+    raw = np.fromfile(prefix + ".bed", dtype=np.uint8)
+    if raw.size < 3 or raw[0] != 0x6C or raw[1] != 0x1B:
+        raise RuntimeError("Invalid .bed file magic number")
+    if raw[2] != 0x01:
+        raise RuntimeError("Only SNP-major mode (0x01) .bed files are supported")
+    N, M = len(fam), len(bim)
+    B = (N + 3) // 4
+    return raw[3:3 + M * B].copy(), N, M, bim, fam
+
+
+def decode_numpy(bed, N, M):
+    """Independent numpy decode: (N x M) float64 dosage with NaN for missing."""
+    B = (N + 3) // 4
+    a = np.asarray(bed, dtype=np.uint8)[: M * B].reshape(M, B)
+    codes = np.stack([(a >> (2 * s)) & 3 for s in range(4)], axis=2).reshape(M, B * 4)[:, :N]
+    lut = np.array([2.0, np.nan, 1.0, 0.0])
+    return lut[codes].T.copy()
+
+
+# --------------------------------------------------------------------------- C oracle
+def stats(bed, N, M, nthreads=None):
+    bed = _u8(bed)
+    means = np.empty(M)
+    inv = np.empty(M)
+    lib().orc_stats(_p(bed, _u8p), N, M, _p(means, _dp), _p(inv, _dp), _nthreads(nthreads))
+    return means, inv
+
+
+def class_counts(bed, N, M, weights=None, nthreads=None):
+    bed = _u8(bed)
+    w = _i32opt(weights)
+    cnt = np.empty((M, 4), dtype=np.int64)
+    lib().orc_class_counts(_p(bed, _u8p), N, M, _p(w, _i32p), _p(cnt, _i64p), _nthreads(nthreads))
+    return cnt
+
+
+def kinship_apply(bed, N, means, inv_stds, keep, variants, X, nthreads=None):
+    """K X for the selected samples/variants; X is (n, k) (any layout), returns (n, k)."""
+    bed = _u8(bed)
+    keep = _i32opt(keep)
+    variants = _i32opt(variants)
+    X = np.asarray(X, dtype=np.float64)
+    if X.ndim == 1:
+        return kinship_apply(bed, N, means, inv_stds, keep, variants, X[:, None], nthreads)[:, 0]
+    n = N if keep is None else keep.size
+    Msel = means.size if variants is None else variants.size
+    if X.shape[0] != n:
+        raise RuntimeError("PlinkKinshipOperator::apply: dimension mismatch")
+    k = X.shape[1]
+    Xf = np.asfortranarray(X)
+    Y = np.empty((n, k), order="F")
+    lib().orc_kinship_apply(_p(bed, _u8p), N, _p(np.ascontiguousarray(means), _dp),
+                            _p(np.ascontiguousarray(inv_stds), _dp), _p(keep, _i32p), n,
+                            _p(variants, _i32p), Msel, _p(Xf, _dp), k, _p(Y, _dp),
+                            _nthreads(nthreads))
+    return np.ascontiguousarray(Y)
+
+
+def kinship_trace(bed, N, means, inv_stds, keep=None, variants=None):
+    bed = _u8(bed)
+    keep = _i32opt(keep)
+    variants = _i32opt(variants)
+    n = N if keep is None else keep.size
+    Msel = means.size if variants is None else variants.size
+    return lib().orc_kinship_trace(_p(bed, _u8p), N, _p(np.ascontiguousarray(means), _dp),
+                                   _p(np.ascontiguousarray(inv_stds), _dp), _p(keep, _i32p), n,
+                                   _p(variants, _i32p), Msel)
+
+
+def kinship_diag(bed, N, means, inv_stds, keep=None, variants=None):
+    bed = _u8(bed)
+    keep = _i32opt(keep)
+    variants = _i32opt(variants)
+    n = N if keep is None else keep.size
+    Msel = means.size if variants is None else variants.size
+    d = np.empty(n)
+    lib().orc_kinship_diag(_p(bed, _u8p), N, _p(np.ascontiguousarray(means), _dp),
+                           _p(np.ascontiguousarray(inv_stds), _dp), _p(keep, _i32p), n,
+                           _p(variants, _i32p), Msel, _p(d, _dp))
+    return d
+
+
+def grm_snp_params(bed, N, M, alg=1, min_maf=0.01, nthreads=None):
+    bed = _u8(bed)
+    valid = np.empty(M, dtype=np.int8)
+    mean_f = np.empty(M, dtype=np.float32)
+    scale_f = np.empty(M, dtype=np.float32)
+    pq2_f = np.empty(M, dtype=np.float32)
+    s = ctypes.c_double()
+    nv = ctypes.c_int64()
+    lib().orc_grm_snp_params(_p(bed, _u8p), N, M, alg, min_maf, _p(valid, _i8p), _p(mean_f, _fp),
+                             _p(scale_f, _fp), _p(pq2_f, _fp), ctypes.byref(s), ctypes.byref(nv),
+                             _nthreads(nthreads))
+    return valid.astype(bool), mean_f, scale_f, pq2_f, s.value, nv.value
+
+
+def grm(bed, N, M, alg=1, min_maf=0.01, nthreads=None):
+    bed = _u8(bed)
+    G = np.empty((N, N))
+    s = ctypes.c_double()
+    nv = ctypes.c_int64()
+    lib().orc_grm(_p(bed, _u8p), N, M, alg, min_maf, _p(G, _dp), ctypes.byref(s),
+                  ctypes.byref(nv), _nthreads(nthreads))
+    return G, s.value, nv.value
+
+
+def grm_exact_centred(bed, N, M, min_maf=0.01, nthreads=None):
+    bed = _u8(bed)
+    raw = np.empty((N, N))
+    nf = np.empty((N, N), dtype=np.float32)
+    s = ctypes.c_double()
+    nv = ctypes.c_int64()
+    rc = lib().orc_grm_exact_centred(_p(bed, _u8p), N, M, min_maf, _p(raw, _dp), _p(nf, _fp),
+                                     ctypes.byref(s), ctypes.byref(nv), _nthreads(nthreads))
+    if rc != 0:
+        raise ValueError("some SNP mean is not a multiple of 2^-29")
+    return raw, nf, s.value, nv.value
+
+
+def scan_sums(bed, N, M, keep, R, min_maf=0.01, nthreads=None):
+    """Per-SNP prepared-marker sums over the keep list: (freq, n_obs, pass, dot (M, nr), gtg)."""
+    bed = _u8(bed)
+    keep = _i32opt(keep)
+    n = N if keep is None else keep.size
+    R = np.asfortranarray(np.asarray(R, dtype=np.float64).reshape(n, -1))
+    nr = R.shape[1]
+    freq = np.empty(M)
+    nobs = np.empty(M, dtype=np.int64)
+    ps = np.empty(M, dtype=np.int32)
+    dot = np.empty((M, nr))
+    gtg = np.empty(M)
+    lib().orc_scan_sums(_p(bed, _u8p), N, M, _p(keep, _i32p), n, min_maf, _p(R, _dp), nr,
+                        _p(freq, _dp), _p(nobs, _i64p), _p(ps, _i32p), _p(dot, _dp),
+                        _p(gtg, _dp), _nthreads(nthreads))
+    return freq, nobs, ps.astype(bool), dot, gtg
+
+
+def chisq_prob(stat, df=1):
+    stat = np.ascontiguousarray(np.atleast_1d(stat), dtype=np.float64)
+    p = np.empty_like(stat)
+    lib().orc_chisq_prob_vec(_p(stat, _dp), stat.size, df, _p(p, _dp))
+    return p
+
+
+def gen_hwe_bed(seed, N, M, pmin, pmax, miss, nthreads=None):
+    B = (N + 3) // 4
+    out = np.empty(M * B, dtype=np.uint8)
+    lib().orc_gen_hwe_bed(seed, N, M, pmin, pmax, miss, _p(out, _u8p), _nthreads(nthreads))
+    return out
+
+
+# --------------------------------------------------------------------------- dense numpy
+def eigen_vce_prepare(G, y, X):
+    """EigenVce::prepare with covariate projection (eigen_vce.cpp:19-49): full Householder Q of
+    X, A = Q[:, c:], G_reml = A^T G A, eigendecomposition, D clamped at 0, W_proj = A U,
+    Uty = U^T A^T y."""
+    n, c = X.shape
+    Q, _ = np.linalg.qr(np.asarray(X, dtype=np.float64), mode="complete")
+    A = Q[:, c:]
+    Greml = A.T @ G @ A
+    D, U = np.linalg.eigh(Greml)
+    D = np.maximum(D, 0.0)
+    return D, A @ U, U.T @ (A.T @ y)
+
+
+def solve_diagonal(D, Uty):
+    """VCESolver::solveDiagonal with an empty UtX (vce.cpp:3073-3171): REML golden section on
+    log10(delta) in [-5, 5] while |c - d| > 1e-5; sigma_g^2 = quad / n_eff,
+    sigma_e^2 = sigma_g^2 * delta."""
+    n_eff = float(D.size)
+
+    def negll(delta):
+        V = D + delta
+        quad = float(np.sum(Uty * Uty / V))
+        return 0.5 * (np.sum(np.log(V)) + n_eff * np.log(quad / n_eff) + n_eff)
+
+    lo, hi = -5.0, 5.0
+    gr = (np.sqrt(5.0) + 1.0) / 2.0
+    c = hi - (hi - lo) / gr
+    d = lo + (hi - lo) / gr
+    fc, fd = negll(10.0 ** c), negll(10.0 ** d)
+    while abs(c - d) > 1e-5:
+        if fc < fd:
+            hi, d, fd = d, c, fc
+            c = hi - (hi - lo) / gr
+            fc = negll(10.0 ** c)
+        else:
+            lo, c, fc = c, d, fd
+            d = lo + (hi - lo) / gr
+            fd = negll(10.0 ** d)
+    delta = 10.0 ** ((lo + hi) / 2.0)
+    quad = float(np.sum(Uty * Uty / (D + delta)))
+    vg = quad / n_eff
+    return vg, vg * delta, delta
+
+
+def rotated_scan(W_proj, D, Uty, vg, ve, Gm):
+    """LmmEngine::scan projected form (lmm_engine.cpp:1363-1525) for a block of prepared
+    markers Gm (n x b): returns score, var, beta, se, p (chisq_prob), log10p."""
+    Vinv = 1.0 / (vg * D + ve)
+    Py = Uty * Vinv
+    UtG = W_proj.T @ Gm
+    score = UtG.T @ Py
+    var = np.sum(UtG * UtG * Vinv[:, None], axis=0)
+    return _stats_from(score, var)
+
+
+def gls_scan(V, X, y, Gm):
+    """Textbook GLS (pattern of tests/test_weighted_lmm_equivalence.py:13-22): P = V^-1 -
+    V^-1 X (X^T V^-1 X)^-1 X^T V^-1; score = g^T P y, var = g^T P g."""
+    Vi = np.linalg.inv(V)
+    XtVi = X.T @ Vi
+    P = Vi - XtVi.T @ np.linalg.solve(XtVi @ X, XtVi)
+    score = Gm.T @ (P @ y)
+    var = np.einsum("ij,ij->j", Gm, P @ Gm)
+    return _stats_from(score, var)
+
+
+def _stats_from(score, var):
+    score = np.asarray(score, dtype=np.float64)
+    var = np.asarray(var, dtype=np.float64)
+    ok = (var > 1e-12) & np.isfinite(var)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        beta = np.where(ok, score / var, np.nan)
+        se = np.where(ok, 1.0 / np.sqrt(var), np.nan)
+        stat = np.where(ok, score * score / var, 0.0)
+    p = np.where(ok, chisq_prob(stat, 1), np.nan)
+    with np.errstate(divide="ignore"):
+        log10p = np.where(ok, np.where(p > 0, -np.log10(np.where(p > 0, p, 1.0)), 300.0), np.nan)
+    return dict(score=score, var=var, beta=beta, se=se, p=p, log10p=log10p, tested=ok)

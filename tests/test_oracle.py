@@ -1,0 +1,281 @@
+"""Pins the CPU oracle against known answers, closed forms, brute force and invariants.
+
+Every check here is independent of the formula it checks: decodes come from a separate numpy
+unpack, dense products from numpy, p-values from scipy's closed form, REML from a brute-force
+grid.  These are CPU-only tests (no GPU).
+"""
+import numpy as np
+import pytest
+import scipy.special
+
+import oracle
+from kinship_batch_fixture import MT19937_64
+
+
+def dense_z(bed, N, M, means, inv, keep, variants):
+    """The reference test's dense oracle (test_kinship_batch.cpp:61-71) from a numpy decode."""
+    D = oracle.decode_numpy(bed, N, M)
+    keep = np.arange(N) if keep is None else keep
+    variants = np.arange(M) if variants is None else variants
+    Z = (D[np.ix_(keep, variants)] - means[variants]) * inv[variants]
+    return np.where(np.isnan(Z), 0.0, Z)
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(1e-30, np.max(np.abs(b)))
+
+
+def test_mt19937_64_standard_known_answer():
+    # C++ standard [rand.predef]: the 10000th output of a default-constructed mt19937_64.
+    r = MT19937_64(5489)
+    for _ in range(9999):
+        r()
+    assert r() == 9981545732273789042
+
+
+def test_bed_magic_rejected(tmp_path):
+    p = tmp_path / "x"
+    (p.with_suffix(".fam")).write_text("a a 0 0 0 1\n")
+    (p.with_suffix(".bim")).write_text("1 s 0 1 A C\n")
+    (p.with_suffix(".bed")).write_bytes(bytes([0x6C, 0x1B, 0x00, 0]))
+    with pytest.raises(RuntimeError, match="SNP-major"):
+        oracle.read_bed(str(p))
+    (p.with_suffix(".bed")).write_bytes(bytes([0x6C, 0x1C, 0x01, 0]))
+    with pytest.raises(RuntimeError, match="magic"):
+        oracle.read_bed(str(p))
+
+
+@pytest.mark.parametrize("name", ["synth", "gemma", "ebfmt"])
+def test_stats_closed_form_reference_beds(ref_datasets, name):
+    bed, N, M, bim, fam = ref_datasets[name]
+    means, inv = oracle.stats(bed, N, M)
+    D = oracle.decode_numpy(bed, N, M)
+    obs = ~np.isnan(D)
+    cnt = obs.sum(0)
+    s = np.nansum(D, 0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        mu = np.where(cnt > 0, s / cnt, 0.0)
+    p = mu / 2
+    var = 2 * p * (1 - p)
+    exp_inv = np.where((cnt > 0) & (var >= 1e-9), 1 / np.sqrt(np.maximum(var, 1e-300)), 0.0)
+    assert np.array_equal(means, mu)
+    assert np.array_equal(inv, exp_inv)
+    cc = oracle.class_counts(bed, N, M)
+    codes = np.array([[3, 2, 0][int(v)] if not np.isnan(v) else 1 for v in D[:, 0]])
+    assert cc[0].tolist() == [int((codes == c).sum()) for c in range(4)]
+    assert np.array_equal(cc.sum(1), np.full(M, N))
+    assert np.array_equal(cc[:, 0] * 2 + cc[:, 2], s.astype(np.int64))
+
+
+def test_stats_kinship_fixture_special_snps(kb_fixture):
+    bed, N, M, _ = kb_fixture
+    means, inv = oracle.stats(bed, N, M)
+    for j in range(M):
+        if j % 31 == 0:
+            assert means[j] == 0.0 and inv[j] == 0.0     # all code 3: dosage 0, monomorphic
+        elif j % 37 == 0:
+            assert means[j] == 0.0 and inv[j] == 0.0     # all missing
+    assert np.count_nonzero(inv) > 900
+    # partial last byte: padding codes (random) must not change the stats
+    cc = oracle.class_counts(bed, N, M)
+    assert np.array_equal(cc.sum(1), np.full(M, N))
+
+
+def test_kinship_apply_matches_dense_all_selections(kb_fixture):
+    bed, N, M, sels = kb_fixture
+    means, inv = oracle.stats(bed, N, M)
+    rng = np.random.default_rng(1)
+    for keep, variants in sels:
+        Z = dense_z(bed, N, M, means, inv, keep, variants)
+        Mv = Z.shape[1]
+        K = Z @ Z.T / Mv
+        tr = oracle.kinship_trace(bed, N, means, inv, keep, variants)
+        assert abs(tr - np.trace(K)) < 1e-10
+        assert np.max(np.abs(oracle.kinship_diag(bed, N, means, inv, keep, variants)
+                             - np.diag(K))) < 1e-12
+        for width in (1, 7, 41):
+            X = rng.standard_normal((Z.shape[0], width))
+            Y = oracle.kinship_apply(bed, N, means, inv, keep, variants, X)
+            assert rel(Y, K @ X) < 1e-10
+        with pytest.raises(RuntimeError):
+            oracle.kinship_apply(bed, N, means, inv, keep, variants, np.ones((Z.shape[0] + 1, 1)))
+
+
+def test_kinship_invariants(kb_fixture):
+    bed, N, M, sels = kb_fixture
+    means, inv = oracle.stats(bed, N, M)
+    rng = np.random.default_rng(2)
+    v, w = rng.standard_normal((2, N))
+    Kv = oracle.kinship_apply(bed, N, means, inv, None, None, v)
+    Kw = oracle.kinship_apply(bed, N, means, inv, None, None, w)
+    assert abs(w @ Kv - v @ Kw) < 1e-10 * np.abs(w @ Kv)          # symmetry
+    Kvw = oracle.kinship_apply(bed, N, means, inv, None, None, 2 * v - 3 * w)
+    assert rel(Kvw, 2 * Kv - 3 * Kw) < 1e-12                      # linearity
+    # principal submatrix: K_S v_S == (K_full P_S v)_S
+    keep = np.array([3, 9, 0, 30, 12], dtype=np.int32)
+    vs = rng.standard_normal(keep.size)
+    full = np.zeros(N)
+    np.add.at(full, keep, vs)
+    sub = oracle.kinship_apply(bed, N, means, inv, keep, None, vs)
+    assert rel(sub, oracle.kinship_apply(bed, N, means, inv, None, None, full)[keep]) < 1e-12
+    # LOCO additivity: M K_full = M_loco K_loco + M_c K_c
+    chrom = np.arange(M) % 3
+    loco = np.flatnonzero(chrom != 1).astype(np.int32)
+    only = np.flatnonzero(chrom == 1).astype(np.int32)
+    a = M * Kv
+    b = loco.size * oracle.kinship_apply(bed, N, means, inv, None, loco, v) + \
+        only.size * oracle.kinship_apply(bed, N, means, inv, None, only, v)
+    assert rel(a, b) < 1e-12
+
+
+def test_grm_alg1_alg0_dense(ref_datasets):
+    bed, N, M, _, _ = ref_datasets["gemma"]
+    D = oracle.decode_numpy(bed, N, M)
+    for alg in (1, 0):
+        G, s2pq, nv = oracle.grm(bed, N, M, alg=alg, min_maf=0.01)
+        valid, mean_f, scale_f, pq2_f, s2, nv2 = oracle.grm_snp_params(bed, N, M, alg, 0.01)
+        obs = ~np.isnan(D)
+        f = np.nansum(D, 0) / (2 * obs.sum(0))
+        maf = np.minimum(f, 1 - f)
+        assert np.array_equal(valid, (maf >= np.float32(0.01)) & (np.abs(maf) >= 1e-8))
+        assert np.array_equal(mean_f, np.where(valid, (2 * f).astype(np.float32), 0))
+        Z = (D[:, valid] - mean_f[valid].astype(np.float64)) * scale_f[valid].astype(np.float64)
+        Z = np.where(np.isnan(Z), 0.0, Z)
+        ref = Z @ Z.T / (nv if alg == 0 else s2pq)
+        assert nv == valid.sum() and s2 == s2pq
+        assert rel(G, ref) < 1e-13
+        if alg == 1:
+            assert abs(s2pq - np.sum(pq2_f[valid].astype(np.float64))) < 1e-9 * s2pq
+
+
+def test_grm_alg0_mean_diag_no_missing():
+    N, M = 300, 4000
+    bed = oracle.gen_hwe_bed(7, N, M, 0.05, 0.5, 0.0)
+    G, _, _ = oracle.grm(bed, N, M, alg=0, min_maf=0.01)
+    assert abs(np.mean(np.diag(G)) - 1.0) < 0.02
+
+
+def test_grm_exact_matches_python_integers():
+    N, M = 9, 57
+    bed = oracle.gen_hwe_bed(11, N, M, 0.05, 0.5, 0.08)
+    raw, nf, s2pq, nv = oracle.grm_exact_centred(bed, N, M, min_maf=0.01)
+    valid, mean_f, _, _, _, _ = oracle.grm_snp_params(bed, N, M, 1, 0.01)
+    D = oracle.decode_numpy(bed, N, M)
+    A = [int(np.ldexp(np.float64(m), 29)) for m in mean_f]
+    from fractions import Fraction
+    for i in range(N):
+        for k in range(N):
+            tot = 0
+            for j in range(M):
+                if not valid[j] or np.isnan(D[i, j]) or np.isnan(D[k, j]):
+                    continue
+                tot += ((int(D[i, j]) << 29) - A[j]) * ((int(D[k, j]) << 29) - A[j])
+            exact = Fraction(tot, 1 << 58)
+            assert raw[i, k] == float(exact)                           # correctly rounded
+            q = exact / Fraction(s2pq)
+            f32 = np.float32(float(q))
+            # correct rounding to float: the nearest float to q (ties even) -- check neighbours
+            lo, hi = np.nextafter(f32, np.float32(-np.inf)), np.nextafter(f32, np.float32(np.inf))
+            d = abs(Fraction(float(f32)) - q)
+            assert d <= abs(Fraction(float(lo)) - q) and d <= abs(Fraction(float(hi)) - q)
+            assert nf[i, k] == f32
+
+
+def test_grm_exact_vs_fp64(ref_datasets):
+    bed, N, M, _, _ = ref_datasets["gemma"]
+    raw, nf, s2pq, nv = oracle.grm_exact_centred(bed, N, M, min_maf=0.01)
+    G, s2, _ = oracle.grm(bed, N, M, alg=1, min_maf=0.01)
+    assert s2 == s2pq
+    assert rel(raw / s2pq, G) < 1e-12
+    assert np.max(np.abs(nf.astype(np.float64) - G) / (np.abs(G) + 1e-30)) < 2 ** -23
+
+
+def test_scan_sums_match_numpy(ref_datasets):
+    bed, N, M, _, _ = ref_datasets["synth"]
+    rng = np.random.default_rng(3)
+    keep = np.sort(rng.choice(N, 45, replace=False)).astype(np.int32)
+    R = rng.standard_normal((keep.size, 3))
+    freq, nobs, ps, dot, gtg = oracle.scan_sums(bed, N, M, keep, R, min_maf=0.05)
+    D = oracle.decode_numpy(bed, N, M)[keep]
+    obs = ~np.isnan(D)
+    mu = np.nansum(D, 0) / obs.sum(0)
+    G = np.where(obs, D, mu) - mu
+    f = np.nansum(D, 0) / (2 * obs.sum(0))
+    assert np.allclose(freq, f, rtol=0, atol=1e-15)
+    assert np.array_equal(nobs, obs.sum(0))
+    assert np.array_equal(ps, (obs.sum(0) >= 3) & (np.minimum(f, 1 - f) >= 0.05))
+    assert rel(dot, G.T @ R) < 1e-13
+    assert rel(gtg, np.sum(G * G, 0)) < 1e-13
+
+
+def test_chisq_prob_closed_form():
+    stat = np.concatenate([np.linspace(0.01, 5, 50), np.linspace(5, 200, 50)])
+    p = oracle.chisq_prob(stat, 1)
+    exact = scipy.special.chdtrc(1, stat)
+    assert np.all(np.abs(p - exact) <= 1e-6 * exact + 1e-300)
+    assert oracle.chisq_prob(np.array([0.0, -1.0]), 1).tolist() == [1.0, 1.0]
+    # deep tail: continued fraction keeps |d(-log10 p)| tiny
+    assert np.max(np.abs(np.log10(p[p > 0]) - np.log10(exact[p > 0]))) < 1e-6
+    p4 = oracle.chisq_prob(np.array([1.0, 9.0, 30.0]), 4)
+    assert np.allclose(p4, scipy.special.chdtrc(4, [1.0, 9.0, 30.0]), rtol=1e-6)
+
+
+def _small_lmm(seed=5, N=240, M=800):
+    rng = np.random.default_rng(seed)
+    bed = oracle.gen_hwe_bed(seed, N, M, 0.05, 0.5, 0.01)
+    G, _, _ = oracle.grm(bed, N, M, alg=1, min_maf=0.01)
+    X = np.column_stack([np.ones(N), rng.standard_normal((N, 2)).astype(np.float32)])
+    L = np.linalg.cholesky(G + 1e-6 * np.eye(N))
+    y = X @ np.array([1.0, 0.5, -0.2]) + L @ rng.standard_normal(N) + rng.standard_normal(N)
+    return bed, N, M, G, X, y
+
+
+def test_solve_diagonal_brute_force_grid():
+    bed, N, M, G, X, y = _small_lmm()
+    D, W, Uty = oracle.eigen_vce_prepare(G, y, X)
+    vg, ve, delta = oracle.solve_diagonal(D, Uty)
+    # explicit REML likelihood on V = G + delta I (X projected out), brute-force grid
+    def reml(dl):
+        V = G + dl * np.eye(N)
+        Vi = np.linalg.inv(V)
+        XtVi = X.T @ Vi
+        P = Vi - XtVi.T @ np.linalg.solve(XtVi @ X, XtVi)
+        q = y @ P @ y
+        nef = N - X.shape[1]
+        return -0.5 * (np.linalg.slogdet(V)[1] + np.linalg.slogdet(XtVi @ X)[1] + nef * np.log(q / nef))
+    grid = 10.0 ** np.linspace(-2, 2, 161)
+    best = grid[np.argmax([reml(d) for d in grid])]
+    assert abs(np.log10(best) - np.log10(delta)) <= 0.026
+    # invariance to the null-space basis: a rotated basis gives the same D and statistic
+    assert np.all(D >= 0)
+
+
+def test_rotated_scan_equals_explicit_gls():
+    bed, N, M, G, X, y = _small_lmm(seed=9, N=200, M=600)
+    D, W, Uty = oracle.eigen_vce_prepare(G, y, X)
+    vg, ve, delta = oracle.solve_diagonal(D, Uty)
+    keep = np.arange(N, dtype=np.int32)
+    Dd = oracle.decode_numpy(bed, N, M)[:, :50]
+    mu = np.nanmean(Dd, 0)
+    Gm = np.where(np.isnan(Dd), mu, Dd) - mu
+    a = oracle.rotated_scan(W, D, Uty, vg, ve, Gm)
+    b = oracle.gls_scan(vg * G + ve * np.eye(N), X, y, Gm)
+    assert rel(a["score"], b["score"]) < 1e-8
+    assert rel(a["var"], b["var"]) < 1e-8
+    assert np.max(np.abs(a["log10p"] - b["log10p"])) < 1e-6
+
+
+def test_gen_hwe_moments():
+    N, M = 4000, 300
+    bed = oracle.gen_hwe_bed(20261204, N, M, 0.01, 0.5, 0.005)
+    D = oracle.decode_numpy(bed, N, M)
+    miss = np.isnan(D).mean()
+    assert abs(miss - 0.005) < 0.001
+    f = np.nanmean(D, 0) / 2
+    assert f.min() > 0.0 and f.max() < 0.56
+    # HWE: het fraction ~ 2pq per SNP
+    het = np.nanmean(D == 1, 0)
+    assert np.mean(np.abs(het - 2 * f * (1 - f))) < 0.02
+    # padding bits are zero
+    bed2 = oracle.gen_hwe_bed(1, 5, 3, 0.1, 0.5, 0.0)
+    assert np.all(bed2.reshape(3, 2)[:, 1] >> 2 == 0)
