@@ -1,0 +1,296 @@
+"""MI355X-native genotype hot path of CosmicScanner: host-side mirror over the HIP C-ABI.
+
+The compute lives in ``libcsamd.so`` (hand-written HIP for gfx950, ``csrc/``); this package only
+binds its C entry points (``include/cosmicscanner_amd.h``) with ctypes and mirrors the
+reference's interfaces for the path:
+
+* :class:`PlinkReader`  -- ``libCosmicCore/src/plink_reader.h`` (load, stats, class counts), with
+  the packed matrix resident in HBM;
+* :class:`PlinkKinshipOperator` -- ``src/kinship_operator.h:19-93`` (apply / applyTranspose /
+  trace / getDiagonal / rows / cols / numVariants) and :func:`createKinshipOperator`.
+
+Importing fails loudly when the library has not been built: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcsamd.so")
+
+CS_OK = 0
+CS_ERR_INVALID_ARGUMENT = -1
+CS_ERR_RUNTIME = -2
+CS_PREC_INT8_L3 = 3
+CS_PREC_INT8_L2 = 2
+CS_PREC_FP32 = 32
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"cosmicscanner_amd: HIP library not built ({LIB_PATH}); run "
+                          "`python -m cosmicscanner_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "cs_last_error": ([], ctypes.c_char_p),
+        "cs_version": ([], ctypes.c_int),
+        "cs_bed_create": ([_u8p, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "cs_bed_create_from_file_image": ([_u8p, _i64, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "cs_bed_create_synthetic": ([ctypes.c_uint64, _i64, _i64, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.POINTER(_vp)], ctypes.c_int),
+        "cs_bed_destroy": ([_vp], ctypes.c_int),
+        "cs_bed_dims": ([_vp, _i64p, _i64p], ctypes.c_int),
+        "cs_bed_download": ([_vp, _u8p], ctypes.c_int),
+        "cs_bed_build_sample_major": ([_vp], ctypes.c_int),
+        "cs_bed_device_bytes": ([_vp, _i64p], ctypes.c_int),
+        "cs_bed_compute_stats": ([_vp], ctypes.c_int),
+        "cs_bed_get_stats": ([_vp, _dp, _dp], ctypes.c_int),
+        "cs_bed_class_counts": ([_vp, _i32p, _i64p], ctypes.c_int),
+        "cs_kinship_create": ([_vp, _i32p, _i64, _i32p, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "cs_kinship_destroy": ([_vp], ctypes.c_int),
+        "cs_kinship_rows": ([_vp, _i64p], ctypes.c_int),
+        "cs_kinship_num_variants": ([_vp, _i64p], ctypes.c_int),
+        "cs_kinship_set_precision": ([_vp, ctypes.c_int], ctypes.c_int),
+        "cs_kinship_apply": ([_vp, _dp, _i64, _i64, _dp, _i64], ctypes.c_int),
+        "cs_kinship_apply_device": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
+        "cs_kinship_trace": ([_vp, _dp], ctypes.c_int),
+        "cs_kinship_diag": ([_vp, _dp], ctypes.c_int),
+        "cs_kinship_last_timing": ([_vp, _dp], ctypes.c_int),
+    }
+    for name, (argt, rest) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = argt
+        f.restype = rest
+    _lib = L
+    return L
+
+
+def lib():
+    return _load()
+
+
+class CsError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc == CS_OK:
+        return
+    msg = lib().cs_last_error().decode()
+    if rc == CS_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def read_plink_text(prefix):
+    with open(prefix + ".fam") as f:
+        fam = [ln.split() for ln in f if ln.strip()]
+    with open(prefix + ".bim") as f:
+        bim = [ln.split() for ln in f if ln.strip()]
+    return bim, fam
+
+
+class PlinkReader:
+    """PLINK BED genotypes resident in HBM (PlinkReader, libCosmicCore/src/plink_reader.h:30-127).
+
+    Construct with :meth:`load` (prefix.{bed,bim,fam}), :meth:`from_body` (the bytes after the
+    magic) or :meth:`synthetic` (generated on the device)."""
+
+    def __init__(self, handle, n_samples, n_snps, bim=None, fam=None):
+        self._h = handle
+        self.num_samples = int(n_samples)
+        self.num_snps = int(n_snps)
+        self.bim = bim
+        self.fam = fam
+
+    # construction ------------------------------------------------------------------
+    @classmethod
+    def load(cls, prefix):
+        bim, fam = read_plink_text(prefix)
+        image = np.fromfile(prefix + ".bed", dtype=np.uint8)
+        h = _vp()
+        _check(lib().cs_bed_create_from_file_image(_p(image, _u8p), image.size, len(fam), len(bim),
+                                                   ctypes.byref(h)))
+        return cls(h, len(fam), len(bim), bim, fam)
+
+    @classmethod
+    def from_body(cls, body, n_samples, n_snps):
+        body = np.ascontiguousarray(body, dtype=np.uint8)
+        if body.size < n_snps * ((n_samples + 3) // 4):
+            raise RuntimeError("genotype buffer shorter than n_snps * ceil(n_samples/4)")
+        h = _vp()
+        _check(lib().cs_bed_create(_p(body, _u8p), n_samples, n_snps, ctypes.byref(h)))
+        return cls(h, n_samples, n_snps)
+
+    @classmethod
+    def synthetic(cls, seed, n_samples, n_snps, pmin, pmax, miss):
+        h = _vp()
+        _check(lib().cs_bed_create_synthetic(seed, n_samples, n_snps, pmin, pmax, miss,
+                                             ctypes.byref(h)))
+        return cls(h, n_samples, n_snps)
+
+    def close(self):
+        if self._h:
+            _check(lib().cs_bed_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # reference interface ----------------------------------------------------------
+    def getNumSamples(self):
+        return self.num_samples
+
+    def getNumSnps(self):
+        return self.num_snps
+
+    def getBytesPerSnp(self):
+        return (self.num_samples + 3) // 4
+
+    def computeStats(self, n_threads=1):
+        _check(lib().cs_bed_compute_stats(self._h))
+
+    def getMeans(self):
+        return self._stats()[0]
+
+    def getInvStds(self):
+        return self._stats()[1]
+
+    def _stats(self):
+        m = np.empty(self.num_snps)
+        s = np.empty(self.num_snps)
+        _check(lib().cs_bed_get_stats(self._h, _p(m, _dp), _p(s, _dp)))
+        return m, s
+
+    def class_counts(self, weights=None):
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.int32)
+        if w is not None and w.size != self.num_samples:
+            raise ValueError("weights must have one entry per sample")
+        out = np.empty((self.num_snps, 4), dtype=np.int64)
+        _check(lib().cs_bed_class_counts(self._h, _p(w, _i32p), _p(out, _i64p)))
+        return out
+
+    def download(self):
+        out = np.empty(self.num_snps * self.getBytesPerSnp(), dtype=np.uint8)
+        _check(lib().cs_bed_download(self._h, _p(out, _u8p)))
+        return out
+
+    def device_bytes(self):
+        v = ctypes.c_int64()
+        _check(lib().cs_bed_device_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+
+class PlinkKinshipOperator:
+    """Matrix-free K = Z Z^T / M over selected samples and variants (src/kinship_operator.h:19-93).
+
+    ``keep`` / ``variants`` default to all samples / all SNPs (empty list semantics of the
+    reference).  ``apply`` accepts a vector (n,) or a matrix (n, k)."""
+
+    def __init__(self, reader: PlinkReader, keep_indices=None, variant_indices=None):
+        self.reader = reader   # the reader must outlive the operator (kinship_operator.h:83)
+        keep = None if keep_indices is None or len(keep_indices) == 0 else \
+            np.ascontiguousarray(keep_indices, dtype=np.int32)
+        var = None if variant_indices is None or len(variant_indices) == 0 else \
+            np.ascontiguousarray(variant_indices, dtype=np.int32)
+        h = _vp()
+        _check(lib().cs_kinship_create(reader._h, _p(keep, _i32p), 0 if keep is None else keep.size,
+                                       _p(var, _i32p), 0 if var is None else var.size,
+                                       ctypes.byref(h)))
+        self._h = h
+        r = ctypes.c_int64()
+        _check(lib().cs_kinship_rows(h, ctypes.byref(r)))
+        self._rows = r.value
+        _check(lib().cs_kinship_num_variants(h, ctypes.byref(r)))
+        self._m = r.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib().cs_kinship_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rows(self):
+        return self._rows
+
+    def cols(self):
+        return self._rows
+
+    def numVariants(self):
+        return self._m
+
+    def isDense(self):
+        return False
+
+    def set_precision(self, mode):
+        _check(lib().cs_kinship_set_precision(self._h, int(mode)))
+
+    def apply(self, X, n_threads=1):
+        X = np.asarray(X, dtype=np.float64)
+        vec = X.ndim == 1
+        Xm = X[:, None] if vec else X
+        if Xm.shape[0] != self._rows:
+            raise RuntimeError("PlinkKinshipOperator::apply: dimension mismatch")
+        k = Xm.shape[1]
+        Xf = np.asfortranarray(Xm)
+        Y = np.empty((self._rows, k), order="F")
+        if k > 0:
+            _check(lib().cs_kinship_apply(self._h, _p(Xf, _dp), self._rows, k, _p(Y, _dp),
+                                          self._rows))
+        return Y[:, 0].copy() if vec else np.ascontiguousarray(Y)
+
+    apply_block = apply
+
+    def applyTranspose(self, v, n_threads=1):
+        return self.apply(v, n_threads)
+
+    def apply_device(self, dX_ptr, ldx, k, dY_ptr, ldy):
+        """Y = K X on device buffers (FP64, column-major) -- no host transfer."""
+        _check(lib().cs_kinship_apply_device(self._h, dX_ptr, ldx, k, dY_ptr, ldy))
+
+    def trace(self):
+        t = ctypes.c_double()
+        _check(lib().cs_kinship_trace(self._h, ctypes.byref(t)))
+        return t.value
+
+    def getDiagonal(self):
+        d = np.empty(self._rows)
+        _check(lib().cs_kinship_diag(self._h, _p(d, _dp)))
+        return d
+
+    def last_timing_ms(self):
+        t = np.empty(5)
+        _check(lib().cs_kinship_last_timing(self._h, _p(t, _dp)))
+        return t
+
+
+def createKinshipOperator(reader, keep_indices=None, variant_indices=None):
+    """Factory (src/kinship_operator.h:98-100, kinship_operator.cpp:370-374)."""
+    return PlinkKinshipOperator(reader, keep_indices, variant_indices)

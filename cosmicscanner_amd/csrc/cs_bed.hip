@@ -1,0 +1,431 @@
+// Resident packed genotypes: upload / synthetic generation, padded SNP-major layout, the
+// sample-major transpose, and kernel (a) -- decode + per-SNP statistics.
+//
+// HBM layout.  SNP-major: row j (SNP) holds Npad/4 bytes, sample i at byte i>>2, bits 2(i&3)
+// (the .bed order, plink_reader.cpp:84); rows are padded to Npad = roundup(N, 256) samples and
+// the matrix to Mpad = roundup(M, 256) rows.  Sample-major: row i (sample) holds Mpad/4 bytes,
+// SNP j at byte j>>2, bits 2(j&3).  Every padding slot holds code 01 (missing).
+#include <cmath>
+#include <cstring>
+
+#include "cs_common.h"
+
+namespace csamd {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+// ------------------------------------------------------------------------------ relayout
+// Copy `rows` raw rows (Braw bytes each) into the padded SNP-major matrix; slots of samples
+// >= N in the last byte become code 01.  The rest of each padded row was pre-filled with 0x55.
+__global__ void k_relayout(const uint8_t* __restrict__ raw, int64_t rows, int64_t Braw, int64_t N,
+                           uint8_t* __restrict__ dst, int64_t RS) {
+    const int64_t total = rows * Braw;
+    const int tail = (int)(N & 3);
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / Braw, b = e - r * Braw;
+        uint8_t v = raw[e];
+        if (tail && b == Braw - 1) {
+            const uint8_t keep_mask = (uint8_t)((1u << (2 * tail)) - 1u);
+            v = (uint8_t)((v & keep_mask) | (kMissingByte & ~keep_mask));
+        }
+        dst[r * RS + b] = v;
+    }
+}
+
+// Inverse for download: padded SNP-major -> raw rows, padding slots of the last byte zeroed.
+__global__ void k_unlayout(const uint8_t* __restrict__ src, int64_t RS, int64_t rows, int64_t Braw,
+                           int64_t N, uint8_t* __restrict__ raw) {
+    const int64_t total = rows * Braw;
+    const int tail = (int)(N & 3);
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / Braw, b = e - r * Braw;
+        uint8_t v = src[r * RS + b];
+        if (tail && b == Braw - 1) v &= (uint8_t)((1u << (2 * tail)) - 1u);
+        raw[e] = v;
+    }
+}
+
+// ------------------------------------------------------------------------------ synthetic
+// Counter-based HWE generator: byte-identical to the oracle's CPU generator (same integer
+// hash, same IEEE operations; the library is compiled with -ffp-contract=off).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t hash3(uint64_t seed, uint64_t j, uint64_t i) {
+    return mix64(mix64(mix64(seed + 0x9e3779b97f4a7c15ULL) ^ (j * 0xd1b54a32d192ed03ULL)) ^
+                 (i * 0x8cb92ba72f3d8dd7ULL + 0x632be59bd9b4e019ULL));
+}
+__device__ __forceinline__ double u01(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
+
+__global__ void k_synth(uint64_t seed, int64_t N, int64_t M, double pmin, double pmax,
+                        double miss, uint8_t* __restrict__ dst, int64_t RS) {
+    const int64_t Braw = (N + 3) / 4;
+    const int64_t total = M * Braw;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e / Braw, b = e - j * Braw;
+        const double p = pmin + (pmax - pmin) * u01(hash3(seed, (uint64_t)j, ~0ULL));
+        const double q = 1.0 - p;
+        const double p0 = q * q;
+        const double p1 = p0 + 2.0 * p * q;
+        uint8_t byte = 0;
+        for (int s = 0; s < 4; ++s) {
+            const int64_t i = b * 4 + s;
+            int code = 1;  // padding slots: missing
+            if (i < N) {
+                const uint64_t h = hash3(seed, (uint64_t)j, (uint64_t)i);
+                if (u01(h) < miss) {
+                    code = 1;
+                } else {
+                    const double v = u01(mix64(h ^ 0xa0761d6478bd642fULL));
+                    const int x = v < p0 ? 0 : (v < p1 ? 1 : 2);
+                    code = x == 2 ? 0 : (x == 1 ? 2 : 3);
+                }
+            }
+            byte |= (uint8_t)(code << (2 * s));
+        }
+        dst[j * RS + b] = byte;
+    }
+}
+
+// ------------------------------------------------------------------------------ transpose
+// SNP-major (Mpad x RS) -> sample-major (Npad x CS), 256 x 256 genotype tiles through LDS.
+__global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ snp, int64_t RS,
+                                                   uint8_t* __restrict__ smp, int64_t CS) {
+    __shared__ uint32_t tile[256][17];  // 256 SNP rows x 64 bytes (+1 dword pad vs conflicts)
+    const int t = threadIdx.x;
+    const int64_t j0 = (int64_t)blockIdx.y * 256;   // SNP tile
+    const int64_t s0 = (int64_t)blockIdx.x * 256;   // sample tile
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(snp + (j0 + t) * RS + s0 / 4);
+#pragma unroll
+    for (int d = 0; d < 16; ++d) tile[t][d] = src[d];
+    __syncthreads();
+    // thread t -> sample s0 + t; output bytes b = 0..63 cover SNPs j0 + 4b .. 4b+3
+    const int dw = t >> 4, sh = (t & 15) * 2;   // sample t lives in dword t/16, bits 2(t%16)
+    uint32_t out[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const uint32_t code = (tile[q * 16 + u][dw] >> sh) & 3u;
+            word |= code << (2 * u);
+        }
+        out[q] = word;
+    }
+    uint4* dst = reinterpret_cast<uint4*>(smp + (s0 + t) * CS + j0 / 4);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+}
+
+// ------------------------------------------------------------------------------ kernel (a)
+// All-sample statistics, one wave per SNP: popcounts of the three observed codes, then the
+// reference's FP64 formulae (plink_reader.cpp:205-234) -- correctly rounded IEEE division and
+// square root, no contraction, so the values are bit-identical to the CPU restatement.
+__global__ __launch_bounds__(256) void k_stats(const uint8_t* __restrict__ snp, int64_t RS,
+                                               int64_t M, int64_t N, int64_t* __restrict__ counts,
+                                               double* __restrict__ means,
+                                               double* __restrict__ inv_stds) {
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= M) return;
+    const uint4* row = reinterpret_cast<const uint4*>(snp + j * RS);
+    const int64_t nvec = RS / 16;
+    uint32_t c00 = 0, c10 = 0, c11 = 0;
+    for (int64_t v = lane; v < nvec; v += 64) {
+        const uint4 w4 = row[v];
+        const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t L = ws[q] & 0x55555555u, H = (ws[q] >> 1) & 0x55555555u;
+            c00 += __popc(~(L | H) & 0x55555555u);
+            c10 += __popc(H & ~L & 0x55555555u);
+            c11 += __popc(H & L);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        c00 += __shfl_xor(c00, off);
+        c10 += __shfl_xor(c10, off);
+        c11 += __shfl_xor(c11, off);
+    }
+    if (lane == 0) {
+        const int64_t obs = (int64_t)c00 + c10 + c11;
+        counts[j * 4 + 0] = c00;
+        counts[j * 4 + 1] = N - obs;
+        counts[j * 4 + 2] = c10;
+        counts[j * 4 + 3] = c11;
+        if (obs > 0) {
+            const double sum = 2.0 * (double)c00 + (double)c10;
+            const double mean = sum / (double)obs;
+            const double p = mean / 2.0;
+            const double var = 2.0 * p * (1.0 - p);
+            means[j] = mean;
+            inv_stds[j] = (var < 1e-9) ? 0.0 : 1.0 / sqrt(var);
+        } else {
+            means[j] = 0.0;
+            inv_stds[j] = 0.0;
+        }
+    }
+}
+
+// Weighted class counts, one wave per SNP; weights are int32 over Npad samples (0 on padding).
+__global__ __launch_bounds__(256) void k_wcounts(const uint8_t* __restrict__ snp, int64_t RS,
+                                                 int64_t M, const int32_t* __restrict__ w,
+                                                 int64_t* __restrict__ counts) {
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= M) return;
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(snp + j * RS);
+    const int64_t ndw = RS / 4;
+    long long c[4] = {0, 0, 0, 0};
+    for (int64_t d = lane; d < ndw; d += 64) {
+        const uint32_t word = row[d];
+        const int4* wv = reinterpret_cast<const int4*>(w + d * 16);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int4 ww = wv[q];
+            const int wq[4] = {ww.x, ww.y, ww.z, ww.w};
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const uint32_t code = (word >> (2 * (4 * q + s))) & 3u;
+                c[code] += wq[s];
+            }
+        }
+    }
+    for (int t = 0; t < 4; ++t)
+        for (int off = 32; off > 0; off >>= 1) c[t] += __shfl_xor(c[t], off);
+    if (lane == 0)
+        for (int t = 0; t < 4; ++t) counts[j * 4 + t] = c[t];
+}
+
+static int grid_for(int64_t total, int block = 256) {
+    int64_t g = (total + block - 1) / block;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+static void init_layout(cs_bed* b, int64_t N, int64_t M) {
+    if (N <= 0 || M <= 0) throw InvalidArgument("cs_bed: n_samples and n_snps must be positive");
+    CS_HIP(hipGetDevice(&b->device));
+    b->N = N;
+    b->M = M;
+    b->Braw = (N + 3) / 4;
+    b->Npad = round_up(N, kPad);
+    b->Mpad = round_up(M, kPad);
+    b->RS = b->Npad / 4;
+    b->CS = b->Mpad / 4;
+    CS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->snp.alloc((size_t)(b->Mpad * b->RS));
+    CS_HIP(hipMemsetAsync(b->snp.p, kMissingByte, b->snp.bytes, b->stream));
+}
+
+static void upload(cs_bed* b, const uint8_t* body) {
+    // Staged in chunks of whole rows through a pinned buffer and a device staging area.
+    const int64_t chunk_rows = std::max<int64_t>(1, (int64_t)(256ll << 20) / b->Braw);
+    const size_t chunk_bytes = (size_t)(chunk_rows * b->Braw);
+    DevBuf stage(chunk_bytes);
+    void* pinned = nullptr;
+    CS_HIP(hipHostMalloc(&pinned, chunk_bytes, hipHostMallocDefault));
+    try {
+        for (int64_t r0 = 0; r0 < b->M; r0 += chunk_rows) {
+            const int64_t rows = std::min(chunk_rows, b->M - r0);
+            const size_t nb = (size_t)(rows * b->Braw);
+            CS_HIP(hipStreamSynchronize(b->stream));
+            std::memcpy(pinned, body + r0 * b->Braw, nb);
+            CS_HIP(hipMemcpyAsync(stage.p, pinned, nb, hipMemcpyHostToDevice, b->stream));
+            k_relayout<<<grid_for(rows * b->Braw), 256, 0, b->stream>>>(
+                stage.as<uint8_t>(), rows, b->Braw, b->N, b->snp.as<uint8_t>() + r0 * b->RS, b->RS);
+            CS_HIP(hipGetLastError());
+        }
+        CS_HIP(hipStreamSynchronize(b->stream));
+    } catch (...) {
+        (void)hipHostFree(pinned);
+        throw;
+    }
+    CS_HIP(hipHostFree(pinned));
+}
+
+void ensure_sample_major(cs_bed* b) {
+    if (b->smp.p) return;
+    b->smp.alloc((size_t)(b->Npad * b->CS));
+    dim3 grid((unsigned)(b->Npad / 256), (unsigned)(b->Mpad / 256));
+    k_transpose<<<grid, 256, 0, b->stream>>>(b->snp.as<uint8_t>(), b->RS, b->smp.as<uint8_t>(), b->CS);
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipStreamSynchronize(b->stream));
+}
+
+void ensure_stats(cs_bed* b) {
+    if (b->have_stats) return;
+    b->counts.alloc((size_t)b->M * 4 * sizeof(int64_t));
+    b->means.alloc((size_t)b->M * sizeof(double));
+    b->inv_stds.alloc((size_t)b->M * sizeof(double));
+    k_stats<<<(unsigned)((b->M + 3) / 4), 256, 0, b->stream>>>(
+        b->snp.as<uint8_t>(), b->RS, b->M, b->N, b->counts.as<int64_t>(), b->means.as<double>(),
+        b->inv_stds.as<double>());
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipStreamSynchronize(b->stream));
+    b->have_stats = true;
+}
+
+void weighted_counts(cs_bed* b, const int32_t* d_w_npad, int64_t* d_counts, hipStream_t st) {
+    k_wcounts<<<(unsigned)((b->M + 3) / 4), 256, 0, st>>>(b->snp.as<uint8_t>(), b->RS, b->M,
+                                                         d_w_npad, d_counts);
+    CS_HIP(hipGetLastError());
+}
+
+}  // namespace csamd
+
+using namespace csamd;
+
+extern "C" {
+
+const char* cs_last_error(void) { return g_last_error.c_str(); }
+int cs_version(void) { return 100; }
+
+int cs_bed_create(const uint8_t* body, int64_t n_samples, int64_t n_snps, cs_bed** out) {
+    return guarded([&] {
+        if (!out) throw InvalidArgument("cs_bed_create: null output handle");
+        if (!body) throw RuntimeError("cs_bed_create: null genotype buffer");
+        auto* b = new cs_bed();
+        try {
+            init_layout(b, n_samples, n_snps);
+            upload(b, body);
+        } catch (...) {
+            if (b->stream) (void)hipStreamDestroy(b->stream);
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int cs_bed_create_from_file_image(const uint8_t* image, int64_t image_bytes, int64_t n_samples,
+                                  int64_t n_snps, cs_bed** out) {
+    return guarded([&] {
+        if (!image || image_bytes < 3) throw RuntimeError("Invalid .bed file (too small)");
+        if (image[0] != 0x6C || image[1] != 0x1B) throw RuntimeError("Invalid .bed file magic number");
+        if (image[2] != 0x01)
+            throw RuntimeError("Only SNP-major mode (0x01) .bed files are supported");
+        const int64_t need = 3 + n_snps * ((n_samples + 3) / 4);
+        if (image_bytes < need) throw RuntimeError("Invalid .bed file (truncated)");
+        int rc = cs_bed_create(image + 3, n_samples, n_snps, out);
+        if (rc != CS_OK) throw RuntimeError(cs_last_error());
+    });
+}
+
+int cs_bed_create_synthetic(uint64_t seed, int64_t n_samples, int64_t n_snps, double pmin,
+                            double pmax, double miss, cs_bed** out) {
+    return guarded([&] {
+        if (!out) throw InvalidArgument("cs_bed_create_synthetic: null output handle");
+        if (!(pmin >= 0.0 && pmax <= 1.0 && pmin <= pmax) || !(miss >= 0.0 && miss < 1.0))
+            throw InvalidArgument("cs_bed_create_synthetic: bad frequency or missing rate");
+        auto* b = new cs_bed();
+        try {
+            init_layout(b, n_samples, n_snps);
+            const int64_t total = n_snps * ((n_samples + 3) / 4);
+            k_synth<<<grid_for(total), 256, 0, b->stream>>>(seed, n_samples, n_snps, pmin, pmax, miss,
+                                                           b->snp.as<uint8_t>(), b->RS);
+            CS_HIP(hipGetLastError());
+            CS_HIP(hipStreamSynchronize(b->stream));
+        } catch (...) {
+            if (b->stream) (void)hipStreamDestroy(b->stream);
+            delete b;
+            throw;
+        }
+        *out = b;
+    });
+}
+
+int cs_bed_destroy(cs_bed* bed) {
+    return guarded([&] {
+        if (!bed) return;
+        if (bed->stream) (void)hipStreamSynchronize(bed->stream);
+        hipStream_t s = bed->stream;
+        delete bed;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int cs_bed_dims(const cs_bed* bed, int64_t* n_samples, int64_t* n_snps) {
+    return guarded([&] {
+        if (!bed) throw InvalidArgument("cs_bed_dims: null handle");
+        if (n_samples) *n_samples = bed->N;
+        if (n_snps) *n_snps = bed->M;
+    });
+}
+
+int cs_bed_download(cs_bed* bed, uint8_t* body) {
+    return guarded([&] {
+        if (!bed || !body) throw InvalidArgument("cs_bed_download: null argument");
+        DevBuf raw((size_t)(bed->M * bed->Braw));
+        k_unlayout<<<grid_for(bed->M * bed->Braw), 256, 0, bed->stream>>>(
+            bed->snp.as<uint8_t>(), bed->RS, bed->M, bed->Braw, bed->N, raw.as<uint8_t>());
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipMemcpyAsync(body, raw.p, (size_t)(bed->M * bed->Braw), hipMemcpyDeviceToHost,
+                              bed->stream));
+        CS_HIP(hipStreamSynchronize(bed->stream));
+    });
+}
+
+int cs_bed_build_sample_major(cs_bed* bed) {
+    return guarded([&] {
+        if (!bed) throw InvalidArgument("cs_bed_build_sample_major: null handle");
+        ensure_sample_major(bed);
+    });
+}
+
+int cs_bed_device_bytes(const cs_bed* bed, int64_t* bytes) {
+    return guarded([&] {
+        if (!bed || !bytes) throw InvalidArgument("cs_bed_device_bytes: null argument");
+        *bytes = (int64_t)(bed->snp.bytes + bed->smp.bytes + bed->counts.bytes + bed->means.bytes +
+                           bed->inv_stds.bytes);
+    });
+}
+
+int cs_bed_compute_stats(cs_bed* bed) {
+    return guarded([&] {
+        if (!bed) throw InvalidArgument("cs_bed_compute_stats: null handle");
+        ensure_stats(bed);
+    });
+}
+
+int cs_bed_get_stats(cs_bed* bed, double* means, double* inv_stds) {
+    return guarded([&] {
+        if (!bed) throw InvalidArgument("cs_bed_get_stats: null handle");
+        ensure_stats(bed);
+        if (means)
+            CS_HIP(hipMemcpy(means, bed->means.p, (size_t)bed->M * sizeof(double), hipMemcpyDeviceToHost));
+        if (inv_stds)
+            CS_HIP(hipMemcpy(inv_stds, bed->inv_stds.p, (size_t)bed->M * sizeof(double),
+                             hipMemcpyDeviceToHost));
+    });
+}
+
+int cs_bed_class_counts(cs_bed* bed, const int32_t* weights, int64_t* counts) {
+    return guarded([&] {
+        if (!bed || !counts) throw InvalidArgument("cs_bed_class_counts: null argument");
+        if (!weights) {
+            ensure_stats(bed);
+            CS_HIP(hipMemcpy(counts, bed->counts.p, (size_t)bed->M * 4 * sizeof(int64_t),
+                             hipMemcpyDeviceToHost));
+            return;
+        }
+        DevBuf w((size_t)bed->Npad * sizeof(int32_t));
+        DevBuf c((size_t)bed->M * 4 * sizeof(int64_t));
+        CS_HIP(hipMemsetAsync(w.p, 0, w.bytes, bed->stream));
+        CS_HIP(hipMemcpyAsync(w.p, weights, (size_t)bed->N * sizeof(int32_t), hipMemcpyHostToDevice,
+                              bed->stream));
+        weighted_counts(bed, w.as<int32_t>(), c.as<int64_t>(), bed->stream);
+        CS_HIP(hipMemcpyAsync(counts, c.p, c.bytes, hipMemcpyDeviceToHost, bed->stream));
+        CS_HIP(hipStreamSynchronize(bed->stream));
+    });
+}
+
+}  // extern "C"

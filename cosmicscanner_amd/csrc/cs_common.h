@@ -1,0 +1,110 @@
+// Shared definitions of the MI355X genotype library: error state, device handles, layouts.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cosmicscanner_amd.h"
+
+namespace csamd {
+
+// ---------------------------------------------------------------------------------------------
+// Errors: every C entry point catches these and turns them into a status code plus a
+// thread-local message (cs_last_error).  The C++ adapter rethrows with the same type and text.
+struct RuntimeError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct InvalidArgument : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+
+void set_error(const std::string& msg);
+void clear_error();
+
+#define CS_HIP(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::csamd::RuntimeError(std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                        " at " #call);                                     \
+    } while (0)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        clear_error();
+        f();
+        return CS_OK;
+    } catch (const InvalidArgument& e) {
+        set_error(e.what());
+        return CS_ERR_INVALID_ARGUMENT;
+    } catch (const RuntimeError& e) {
+        set_error(e.what());
+        return CS_ERR_RUNTIME;
+    } catch (const std::bad_alloc&) {
+        set_error("out of host memory");
+        return CS_ERR_RUNTIME;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return CS_ERR_RUNTIME;
+    }
+}
+
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------------------------
+// Device buffer (RAII over hipMalloc).
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 1;
+        CS_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void ensure(size_t n) {
+        if (n > bytes) alloc(n);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Padding granules.  Samples and SNPs are both padded to multiples of 256 so that every K block
+// of the MFMA kernels (256 packed elements = 64 bytes) and every row tile is whole; padding
+// slots hold code 01 (missing), so they contribute to no sum.
+constexpr int64_t kPad = 256;
+constexpr uint8_t kMissingByte = 0x55;
+
+}  // namespace csamd
+
+// The opaque handles of the C-ABI.
+struct cs_bed {
+    int device = 0;
+    int64_t N = 0, M = 0;      // samples (.fam rows), SNPs (.bim rows)
+    int64_t Braw = 0;          // ceil(N/4): bytes per SNP in the file
+    int64_t Npad = 0, Mpad = 0;
+    int64_t RS = 0;            // SNP-major row stride in bytes  (= Npad/4)
+    int64_t CS = 0;            // sample-major row stride in bytes (= Mpad/4)
+    csamd::DevBuf snp;         // Mpad x RS, SNP-major
+    csamd::DevBuf smp;         // Npad x CS, sample-major (built on demand)
+    csamd::DevBuf counts;      // M x 4 int64: all-sample class counts per code
+    csamd::DevBuf means;       // M double  (PlinkReader::getMeans)
+    csamd::DevBuf inv_stds;    // M double  (PlinkReader::getInvStds)
+    bool have_stats = false;
+    hipStream_t stream = nullptr;
+};

@@ -1,0 +1,667 @@
+// Kernel (b): the matrix-free kinship operator K = Z_sel Z_sel^T / M_sel on the resident packed
+// genotypes (PlinkKinshipOperator, src/kinship_operator.cpp:28-374).
+//
+// Factorisation (kinship_operator.cpp:101-135, :162-179):
+//   u_j = s_j (sum_i x~_ij v_i - mu_j sum_i o_ij v_i)         (Z^T V, missing excluded from both)
+//   y_i = sum_j x~_ij w_j - sum_j o_ij (mu_j w_j),  w_j = mult_j s_j u_j / M_sel
+// Samples: keep lists (any order, repeats) act through a scatter-add of V into the full sample
+// space and a gather of the result (K_sel is the principal submatrix of the full K).  Variants:
+// a per-SNP multiplicity (0 = not selected), so LOCO masks cost nothing extra.
+// Both sums run on int8 MFMA with the genotype operand exact and V / W as 2 or 3 signed 8-bit
+// limbs of a per-column fixed point; Z^T V reads the SNP-major copy, Z W the sample-major copy.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "cs_common.h"
+#include "cs_geno_mfma.h"
+
+namespace csamd {
+void ensure_stats(cs_bed* b);
+void ensure_sample_major(cs_bed* b);
+}  // namespace csamd
+
+struct cs_kinship {
+    cs_bed* bed = nullptr;
+    int64_t n = 0, Msel = 0;
+    bool identity = true;
+    std::vector<int32_t> keep;      // empty = identity
+    std::vector<int32_t> variants;  // empty = all
+    int prec = CS_PREC_INT8_L3;
+    hipStream_t stream = nullptr;
+    csamd::DevBuf d_keep, d_inv_off, d_inv_idx, d_mult;
+    // workspaces (grown on demand)
+    csamd::DevBuf dX, dY, Vt, BV, PQ1, W1, W2, B1, B2, PQ2, mx, scl, Vf, PQf, Wf;
+    csamd::DevBuf diag_tab, diag_full;
+    hipEvent_t ev[6] = {};
+    float last_ms[5] = {0, 0, 0, 0, 0};
+};
+
+namespace csamd {
+
+constexpr double kQmax3 = 127.0 * 65536.0;  // |q| bound for 3 balanced limbs (top limb <= 127)
+constexpr double kQmax2 = 127.0 * 256.0;
+
+// ------------------------------------------------------------------------------------------
+__global__ void k_scatter(const double* __restrict__ V, int64_t ldv, int64_t k,
+                          const int32_t* __restrict__ inv_off, const int32_t* __restrict__ inv_idx,
+                          int64_t N, int64_t Npad, double* __restrict__ Vt) {
+    const int64_t total = Npad * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = e / k, c = e - s * k;
+        double acc = 0.0;
+        if (s < N) {
+            if (inv_off) {
+                for (int32_t p = inv_off[s]; p < inv_off[s + 1]; ++p) acc += V[inv_idx[p] + c * ldv];
+            } else {
+                acc = V[s + c * ldv];
+            }
+        }
+        Vt[e] = acc;
+    }
+}
+
+// Column max |W| of a row-major [rows][k] matrix; one column per blockIdx.y.
+__global__ __launch_bounds__(256) void k_colmaxabs(const double* __restrict__ W, int64_t rows,
+                                                   int64_t k, unsigned long long* __restrict__ mx) {
+    __shared__ double red[256];
+    const int64_t c = blockIdx.y;
+    double m = 0.0;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows;
+         r += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(W[r * k + c]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(&mx[c], (unsigned long long)__double_as_longlong(red[0]));
+}
+
+// scale[c] = max/qmax (1 if the column is zero); inv[c] = 1/scale[c]
+__global__ void k_scales(const unsigned long long* __restrict__ mx, int64_t k, double qmax,
+                         double* __restrict__ scale, double* __restrict__ inv) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    const double m = __longlong_as_double((long long)mx[c]);
+    const double s = m > 0.0 ? m / qmax : 1.0;
+    scale[c] = s;
+    inv[c] = 1.0 / s;
+}
+
+// Fixed-point quantisation into L balanced signed 8-bit limbs, stored pre-permuted for the MFMA
+// B operand at columns limb*k + c.
+__global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, int L, int NT,
+                        double qmax, const double* __restrict__ inv, int8_t* __restrict__ B) {
+    const int64_t total = rows * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = e / k;
+        const int c = (int)(e - row * k);
+        double qd = rint(W[e] * inv[c]);
+        qd = fmin(fmax(qd, -qmax), qmax);
+        long long q = (long long)qd;
+        for (int l = 0; l < L; ++l) {
+            int d;
+            if (l == L - 1) {
+                d = (int)q;
+            } else {
+                d = (int)(((q + 128) & 255) - 128);
+                q = (q - d) >> 8;
+            }
+            B[bperm_offset(row, l * (int)k + c, NT)] = (int8_t)d;
+        }
+    }
+}
+
+// Z^T V epilogue: combine limbs, centre / scale per SNP, and form the two forward operands
+// W1 = w, W2 = mu * w with w = mult * s * u / M_sel.
+__global__ void k_epi_gt(const int32_t* __restrict__ PQ, int NC, int64_t M, int64_t Mpad, int64_t k,
+                         int L, const double* __restrict__ scaleV, const double* __restrict__ means,
+                         const double* __restrict__ inv_stds, const int32_t* __restrict__ mult,
+                         double inv_M, double* __restrict__ W1, double* __restrict__ W2) {
+    const int64_t total = Mpad * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e / k;
+        const int c = (int)(e - j * k);
+        double w1 = 0.0, w2 = 0.0;
+        if (j < M && mult[j] != 0 && inv_stds[j] != 0.0) {
+            long long P = 0, Q = 0;
+            const int32_t* row = PQ + j * 2 * NC;
+            for (int l = L - 1; l >= 0; --l) {
+                P = P * 256 + row[l * k + c];
+                Q = Q * 256 + row[NC + l * k + c];
+            }
+            const double s = inv_stds[j], mu = means[j];
+            const double u = ((double)P - mu * (double)Q) * s * scaleV[c];
+            const double w = (double)mult[j] * s * u * inv_M;
+            w1 = w;
+            w2 = mu * w;
+        }
+        W1[e] = w1;
+        W2[e] = w2;
+    }
+}
+
+// Z W epilogue: sum split partials and limbs, scale, gather the kept rows into Y (column-major).
+__global__ void k_epi_g(const int32_t* __restrict__ PQ, int nsplit, int NC, int64_t Npad, int64_t k,
+                        int L, const double* __restrict__ scaleW, const int32_t* __restrict__ keep,
+                        int64_t n, double* __restrict__ Y, int64_t ldy) {
+    const int64_t total = n * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / k;
+        const int c = (int)(e - r * k);
+        const int64_t s = keep ? keep[r] : r;
+        long long acc = 0;
+        for (int l = L - 1; l >= 0; --l) {
+            long long part = 0;
+            for (int sp = 0; sp < nsplit; ++sp) {
+                const int32_t* row = PQ + ((int64_t)sp * Npad + s) * 2 * NC;
+                part += (long long)row[l * k + c] - (long long)row[NC + l * k + c];
+            }
+            acc = acc * 256 + part;
+        }
+        Y[r + c * ldy] = (double)acc * scaleW[c];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Plain FP32 comparator: decode to FP32 and FMA on the vector ALUs, RHS tiles staged in LDS,
+// no MFMA and no split.  Block = 32 rows; 8 threads per row, each over 32 k-elements per 256-k
+// chunk, reduced by shuffles.  KC = number of columns (<= 32).
+template <int KC>
+__global__ __launch_bounds__(256) void k_rows_f32(const uint8_t* __restrict__ A, int64_t lda,
+                                                  int64_t rows, int64_t kpad,
+                                                  const float* __restrict__ Bx,
+                                                  const float* __restrict__ Bo,
+                                                  float* __restrict__ out) {
+    __shared__ float sx[256 * KC];
+    __shared__ float so[256 * KC];
+    const int t = threadIdx.x;
+    const int rloc = t >> 3, sub = t & 7;
+    const int64_t row = (int64_t)blockIdx.x * 32 + rloc;
+    float ax[KC], ao[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) { ax[c] = 0.f; ao[c] = 0.f; }
+    for (int64_t k0 = 0; k0 < kpad; k0 += 256) {
+        __syncthreads();
+        for (int e = t; e < 256 * KC; e += 256) {
+            sx[e] = Bx[k0 * KC + e];
+            so[e] = Bo[k0 * KC + e];
+        }
+        __syncthreads();
+        if (row < rows) {
+            const uint32_t* p = reinterpret_cast<const uint32_t*>(A + row * lda + k0 / 4) + sub * 2;
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+                const uint32_t w = p[d];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const uint32_t code = (w >> (2 * u)) & 3u;
+                    const float x = code == 0 ? 2.f : (code == 2 ? 1.f : 0.f);
+                    const float o = code == 1 ? 0.f : 1.f;
+                    const int kk = sub * 32 + d * 16 + u;
+#pragma unroll
+                    for (int c = 0; c < KC; ++c) {
+                        ax[c] = __builtin_fmaf(x, sx[kk * KC + c], ax[c]);
+                        ao[c] = __builtin_fmaf(o, so[kk * KC + c], ao[c]);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+        for (int off = 4; off > 0; off >>= 1) {
+            ax[c] += __shfl_xor(ax[c], off);
+            ao[c] += __shfl_xor(ao[c], off);
+        }
+    if (sub == 0 && row < rows) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            out[(row * 2) * KC + c] = ax[c];
+            out[(row * 2 + 1) * KC + c] = ao[c];
+        }
+    }
+}
+
+__global__ void k_to_f32_cols(const double* __restrict__ src, int64_t rows, int64_t k, int KC,
+                              float* __restrict__ dst) {
+    const int64_t total = rows * KC;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / KC;
+        const int c = (int)(e - r * KC);
+        dst[e] = c < k ? (float)src[r * k + c] : 0.f;
+    }
+}
+
+__global__ void k_epi_gt_f32(const float* __restrict__ PQ, int KC, int64_t M, int64_t Mpad,
+                             int64_t k, const double* __restrict__ means,
+                             const double* __restrict__ inv_stds, const int32_t* __restrict__ mult,
+                             float inv_M, float* __restrict__ W1, float* __restrict__ W2) {
+    const int64_t total = Mpad * KC;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e / KC;
+        const int c = (int)(e - j * KC);
+        float w1 = 0.f, w2 = 0.f;
+        if (j < M && c < k && mult[j] != 0 && inv_stds[j] != 0.0) {
+            const float s = (float)inv_stds[j], mu = (float)means[j];
+            const float u = (PQ[(j * 2) * KC + c] - mu * PQ[(j * 2 + 1) * KC + c]) * s;
+            const float w = (float)mult[j] * s * u * inv_M;
+            w1 = w;
+            w2 = mu * w;
+        }
+        W1[e] = w1;
+        W2[e] = w2;
+    }
+}
+
+__global__ void k_epi_g_f32(const float* __restrict__ PQ, int KC, int64_t k,
+                            const int32_t* __restrict__ keep, int64_t n, double* __restrict__ Y,
+                            int64_t ldy) {
+    const int64_t total = n * k;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / k;
+        const int c = (int)(e - r * k);
+        const int64_t s = keep ? keep[r] : r;
+        Y[r + c * ldy] = (double)(PQ[(s * 2) * KC + c] - PQ[(s * 2 + 1) * KC + c]);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// diag(K) over the sample-major copy: table T[j][code] = mult_j s_j^2 (x_code - mu_j)^2 (0 for
+// missing), one wave per sample row summing T over its SNPs in a fixed order.
+__global__ void k_diag_table(int64_t M, const double* __restrict__ means,
+                             const double* __restrict__ inv_stds, const int32_t* __restrict__ mult,
+                             double* __restrict__ T) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    const double s = inv_stds[j], mu = means[j], m = (double)mult[j];
+    const double x[4] = {2.0, 0.0, 1.0, 0.0};
+    for (int c = 0; c < 4; ++c) {
+        const double z = (x[c] - mu) * s;
+        T[j * 4 + c] = (c == 1 || s == 0.0 || mult[j] == 0) ? 0.0 : m * z * z;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, int64_t CS,
+                                              int64_t N, int64_t M, const double* __restrict__ T,
+                                              double inv_M, double* __restrict__ diag) {
+    const int lane = threadIdx.x & 63;
+    const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (s >= N) return;
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(smp + s * CS);
+    const int64_t ndw = (M + 15) / 16;
+    double acc = 0.0;
+    for (int64_t d = lane; d < ndw; d += 64) {
+        const uint32_t w = row[d];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int64_t j = d * 16 + u;
+            if (j < M) acc += T[j * 4 + ((w >> (2 * u)) & 3u)];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) diag[s] = acc * inv_M;
+}
+
+// ------------------------------------------------------------------------------------------
+static int grid_for(int64_t total, int block = 256) {
+    int64_t g = (total + block - 1) / block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
+}
+
+template <bool TWO_B>
+static void launch_mfma(int NTB, dim3 grid, hipStream_t st, const uint8_t* A, int64_t lda,
+                        int64_t rows_pad, int64_t kbps, int64_t kbt, int NT, const int8_t* Bx,
+                        const int8_t* Bo, int32_t* out) {
+    switch (NTB) {
+        case 1: k_geno_mfma<1, TWO_B><<<grid, 256, 0, st>>>(A, lda, rows_pad, kbps, kbt, NT, Bx, Bo, out); break;
+        case 2: k_geno_mfma<2, TWO_B><<<grid, 256, 0, st>>>(A, lda, rows_pad, kbps, kbt, NT, Bx, Bo, out); break;
+        case 3: k_geno_mfma<3, TWO_B><<<grid, 256, 0, st>>>(A, lda, rows_pad, kbps, kbt, NT, Bx, Bo, out); break;
+        default: k_geno_mfma<4, TWO_B><<<grid, 256, 0, st>>>(A, lda, rows_pad, kbps, kbt, NT, Bx, Bo, out); break;
+    }
+    CS_HIP(hipGetLastError());
+}
+
+// Column tiling of an int8 product with k right-hand sides in L limbs.
+struct Tiling {
+    int NT, NTB, NC;
+};
+static Tiling tiling(int64_t k, int L) {
+    int NT = (int)((L * k + 15) / 16);
+    int NTB = NT <= 4 ? NT : 4;
+    NT = (int)round_up(NT, NTB);
+    return Tiling{NT, NTB, NT * 16};
+}
+
+// Public helper for other translation units (scan): Z^T V int8 product on the SNP-major copy.
+// Vt: [Npad][k] fp64 in full sample space.  Output PQ [Mpad][2][NC] int32 and the V scales.
+void gt_product_int8(cs_bed* b, const double* Vt, int64_t k, int L, hipStream_t st, DevBuf& BV,
+                     DevBuf& mx, DevBuf& scl, DevBuf& PQ, Tiling& tl, hipEvent_t ev_mid) {
+    tl = tiling(k, L);
+    const double qmax = L == 3 ? kQmax3 : kQmax2;
+    mx.ensure((size_t)k * sizeof(unsigned long long));
+    scl.ensure((size_t)2 * k * sizeof(double));
+    CS_HIP(hipMemsetAsync(mx.p, 0, (size_t)k * sizeof(unsigned long long), st));
+    k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(Vt, b->Npad, k, mx.as<unsigned long long>());
+    k_scales<<<1, 256, 0, st>>>(mx.as<unsigned long long>(), k, qmax, scl.as<double>(),
+                                scl.as<double>() + k);
+    BV.ensure((size_t)(b->Npad * tl.NC));
+    CS_HIP(hipMemsetAsync(BV.p, 0, (size_t)(b->Npad * tl.NC), st));
+    k_quant<<<grid_for(b->Npad * k), 256, 0, st>>>(Vt, b->Npad, k, L, tl.NT, qmax,
+                                                  scl.as<double>() + k, BV.as<int8_t>());
+    CS_HIP(hipGetLastError());
+    if (ev_mid) CS_HIP(hipEventRecord(ev_mid, st));
+    PQ.ensure((size_t)b->Mpad * 2 * tl.NC * sizeof(int32_t));
+    dim3 grid((unsigned)(b->Mpad / 256), 1, (unsigned)(tl.NT / tl.NTB));
+    launch_mfma<false>(tl.NTB, grid, st, b->snp.as<uint8_t>(), b->RS, b->Mpad, b->Npad / 256,
+                       b->Npad / 256, tl.NT, BV.as<int8_t>(), BV.as<int8_t>(), PQ.as<int32_t>());
+}
+
+static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
+                       int64_t ldy) {
+    cs_bed* b = op->bed;
+    hipStream_t st = op->stream;
+    const int L = op->prec == CS_PREC_INT8_L2 ? 2 : 3;
+    const double qmax = L == 3 ? kQmax3 : kQmax2;
+    const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
+    const int32_t* inv_idx = op->identity ? nullptr : op->d_inv_idx.as<int32_t>();
+
+    CS_HIP(hipEventRecord(op->ev[0], st));
+    op->Vt.ensure((size_t)(b->Npad * k) * sizeof(double));
+    k_scatter<<<grid_for(b->Npad * k), 256, 0, st>>>(dX, ldx, k, inv_off, inv_idx, b->N, b->Npad,
+                                                     op->Vt.as<double>());
+    CS_HIP(hipGetLastError());
+    Tiling tl;
+    gt_product_int8(b, op->Vt.as<double>(), k, L, st, op->BV, op->mx, op->scl, op->PQ1, tl,
+                    op->ev[1]);
+    CS_HIP(hipEventRecord(op->ev[2], st));
+
+    op->W1.ensure((size_t)(b->Mpad * k) * sizeof(double));
+    op->W2.ensure((size_t)(b->Mpad * k) * sizeof(double));
+    k_epi_gt<<<grid_for(b->Mpad * k), 256, 0, st>>>(
+        op->PQ1.as<int32_t>(), tl.NC, b->M, b->Mpad, k, L, op->scl.as<double>(),
+        b->means.as<double>(), b->inv_stds.as<double>(), op->d_mult.as<int32_t>(),
+        1.0 / (double)op->Msel, op->W1.as<double>(), op->W2.as<double>());
+    CS_HIP(hipGetLastError());
+    // common scale for W1 and W2 (|W2| <= 2|W1|)
+    unsigned long long* mxw = op->mx.as<unsigned long long>();
+    CS_HIP(hipMemsetAsync(mxw, 0, (size_t)k * sizeof(unsigned long long), st));
+    k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(op->W1.as<double>(), b->Mpad, k, mxw);
+    k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, mxw);
+    double* sclW = op->scl.as<double>();  // V scales no longer needed past k_epi_gt
+    k_scales<<<1, 256, 0, st>>>(mxw, k, qmax, sclW, sclW + k);
+    op->B1.ensure((size_t)(b->Mpad * tl.NC));
+    op->B2.ensure((size_t)(b->Mpad * tl.NC));
+    CS_HIP(hipMemsetAsync(op->B1.p, 0, (size_t)(b->Mpad * tl.NC), st));
+    CS_HIP(hipMemsetAsync(op->B2.p, 0, (size_t)(b->Mpad * tl.NC), st));
+    k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W1.as<double>(), b->Mpad, k, L, tl.NT, qmax,
+                                                  sclW + k, op->B1.as<int8_t>());
+    k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, L, tl.NT, qmax,
+                                                  sclW + k, op->B2.as<int8_t>());
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipEventRecord(op->ev[3], st));
+
+    const int64_t kbt = b->Mpad / 256;
+    const int64_t rblocks = b->Npad / 256;
+    int64_t nsplit = std::max<int64_t>(1, (1024 + rblocks - 1) / rblocks);
+    nsplit = std::min<int64_t>(nsplit, kbt);
+    const int64_t kbps = (kbt + nsplit - 1) / nsplit;
+    nsplit = (kbt + kbps - 1) / kbps;
+    op->PQ2.ensure((size_t)(nsplit * b->Npad * 2 * tl.NC) * sizeof(int32_t));
+    dim3 grid((unsigned)rblocks, (unsigned)nsplit, (unsigned)(tl.NT / tl.NTB));
+    launch_mfma<true>(tl.NTB, grid, st, b->smp.as<uint8_t>(), b->CS, b->Npad, kbps, kbt, tl.NT,
+                      op->B1.as<int8_t>(), op->B2.as<int8_t>(), op->PQ2.as<int32_t>());
+    CS_HIP(hipEventRecord(op->ev[4], st));
+    k_epi_g<<<grid_for(op->n * k), 256, 0, st>>>(op->PQ2.as<int32_t>(), (int)nsplit, tl.NC, b->Npad,
+                                                k, L, sclW, op->identity ? nullptr : op->d_keep.as<int32_t>(),
+                                                op->n, dY, ldy);
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipEventRecord(op->ev[5], st));
+}
+
+template <int KC>
+static void apply_f32_cols(cs_kinship* op, int64_t k) {
+    cs_bed* b = op->bed;
+    hipStream_t st = op->stream;
+    // Vt (fp64 [Npad][k]) -> Vf (fp32 [Npad][KC])
+    op->Vf.ensure((size_t)(b->Npad * KC) * sizeof(float));
+    k_to_f32_cols<<<grid_for(b->Npad * KC), 256, 0, st>>>(op->Vt.as<double>(), b->Npad, k, KC,
+                                                         op->Vf.as<float>());
+    CS_HIP(hipEventRecord(op->ev[1], st));
+    op->PQf.ensure((size_t)(std::max(b->Mpad, b->Npad) * 2 * KC) * sizeof(float));
+    k_rows_f32<KC><<<(unsigned)(b->Mpad / 32), 256, 0, st>>>(
+        b->snp.as<uint8_t>(), b->RS, b->M, b->Npad, op->Vf.as<float>(), op->Vf.as<float>(),
+        op->PQf.as<float>());
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipEventRecord(op->ev[2], st));
+    op->Wf.ensure((size_t)(2 * b->Mpad * KC) * sizeof(float));
+    float* W1 = op->Wf.as<float>();
+    float* W2 = W1 + b->Mpad * KC;
+    k_epi_gt_f32<<<grid_for(b->Mpad * KC), 256, 0, st>>>(
+        op->PQf.as<float>(), KC, b->M, b->Mpad, k, b->means.as<double>(), b->inv_stds.as<double>(),
+        op->d_mult.as<int32_t>(), (float)(1.0 / (double)op->Msel), W1, W2);
+    CS_HIP(hipEventRecord(op->ev[3], st));
+    k_rows_f32<KC><<<(unsigned)(b->Npad / 32), 256, 0, st>>>(b->smp.as<uint8_t>(), b->CS, b->N,
+                                                             b->Mpad, W1, W2, op->PQf.as<float>());
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipEventRecord(op->ev[4], st));
+}
+
+static void apply_f32(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
+                      int64_t ldy) {
+    cs_bed* b = op->bed;
+    hipStream_t st = op->stream;
+    const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
+    const int32_t* inv_idx = op->identity ? nullptr : op->d_inv_idx.as<int32_t>();
+    // Column chunks of <= 16 (or 32) through the FP32 kernels.
+    for (int64_t c0 = 0; c0 < k; c0 += 32) {
+        const int64_t kc = std::min<int64_t>(32, k - c0);
+        CS_HIP(hipEventRecord(op->ev[0], st));
+        op->Vt.ensure((size_t)(b->Npad * kc) * sizeof(double));
+        k_scatter<<<grid_for(b->Npad * kc), 256, 0, st>>>(dX + c0 * ldx, ldx, kc, inv_off, inv_idx,
+                                                          b->N, b->Npad, op->Vt.as<double>());
+        int KC;
+        if (kc <= 1) { apply_f32_cols<1>(op, kc); KC = 1; }
+        else if (kc <= 4) { apply_f32_cols<4>(op, kc); KC = 4; }
+        else if (kc <= 16) { apply_f32_cols<16>(op, kc); KC = 16; }
+        else { apply_f32_cols<32>(op, kc); KC = 32; }
+        k_epi_g_f32<<<grid_for(op->n * kc), 256, 0, st>>>(
+            op->PQf.as<float>(), KC, kc, op->identity ? nullptr : op->d_keep.as<int32_t>(), op->n,
+            dY + c0 * ldy, ldy);
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipEventRecord(op->ev[5], st));
+    }
+}
+
+static void apply_device_impl(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
+                              int64_t ldy) {
+    if (k < 0) throw RuntimeError("PlinkKinshipOperator::apply: dimension mismatch");
+    if (k == 0) return;
+    if (ldx < op->n || ldy < op->n) throw RuntimeError("PlinkKinshipOperator::apply: dimension mismatch");
+    if (op->prec == CS_PREC_FP32) apply_f32(op, dX, ldx, k, dY, ldy);
+    else apply_int8(op, dX, ldx, k, dY, ldy);
+    CS_HIP(hipStreamSynchronize(op->stream));
+    for (int p = 0; p < 5; ++p) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, op->ev[p], op->ev[p + 1]) != hipSuccess) ms = 0.f;
+        op->last_ms[p] = ms;
+    }
+}
+
+}  // namespace csamd
+
+using namespace csamd;
+
+extern "C" {
+
+int cs_kinship_create(cs_bed* bed, const int32_t* keep, int64_t n_keep, const int32_t* variants,
+                      int64_t n_variants, cs_kinship** out) {
+    return guarded([&] {
+        if (!out) throw InvalidArgument("cs_kinship_create: null output handle");
+        if (!bed) throw RuntimeError("PlinkKinshipOperator: null reader");
+        const int64_t N = bed->N, M = bed->M;
+        auto* op = new cs_kinship();
+        try {
+            op->bed = bed;
+            if (keep && n_keep > 0) {
+                op->keep.assign(keep, keep + n_keep);
+                for (int32_t s : op->keep)
+                    if (s < 0 || s >= N) throw RuntimeError("PlinkKinshipOperator: sample index out of range");
+                bool id = n_keep == N;
+                for (int64_t i = 0; id && i < N; ++i) id = op->keep[i] == i;
+                op->identity = id;
+            }
+            if (variants && n_variants > 0) {
+                op->variants.assign(variants, variants + n_variants);
+                for (int32_t j : op->variants)
+                    if (j < 0 || j >= M) throw RuntimeError("PlinkKinshipOperator: variant index out of range");
+            }
+            op->n = op->keep.empty() ? N : (int64_t)op->keep.size();
+            op->Msel = op->variants.empty() ? M : (int64_t)op->variants.size();
+            if (op->Msel == 0) throw RuntimeError("PlinkKinshipOperator: no variants selected");
+            ensure_stats(bed);          // reader_->computeStats(1) (kinship_operator.cpp:67)
+            ensure_sample_major(bed);
+            CS_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+            for (auto& e : op->ev) CS_HIP(hipEventCreate(&e));
+            // variant multiplicities
+            std::vector<int32_t> mult((size_t)M, op->variants.empty() ? 1 : 0);
+            for (int32_t j : op->variants) mult[(size_t)j] += 1;
+            op->d_mult.alloc((size_t)M * sizeof(int32_t));
+            CS_HIP(hipMemcpy(op->d_mult.p, mult.data(), (size_t)M * sizeof(int32_t), hipMemcpyHostToDevice));
+            if (!op->identity) {
+                // keep list and its inverse (CSR from full sample -> positions r, ascending r)
+                op->d_keep.alloc((size_t)op->n * sizeof(int32_t));
+                CS_HIP(hipMemcpy(op->d_keep.p, op->keep.data(), (size_t)op->n * sizeof(int32_t),
+                                 hipMemcpyHostToDevice));
+                std::vector<int32_t> off((size_t)N + 1, 0), idx((size_t)op->n);
+                for (int32_t s : op->keep) off[(size_t)s + 1]++;
+                for (int64_t s = 0; s < N; ++s) off[(size_t)s + 1] += off[(size_t)s];
+                std::vector<int32_t> fill(off.begin(), off.end() - 1);
+                for (int64_t r = 0; r < op->n; ++r) idx[(size_t)fill[(size_t)op->keep[(size_t)r]]++] = (int32_t)r;
+                op->d_inv_off.alloc(off.size() * sizeof(int32_t));
+                op->d_inv_idx.alloc(idx.size() * sizeof(int32_t));
+                CS_HIP(hipMemcpy(op->d_inv_off.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+                CS_HIP(hipMemcpy(op->d_inv_idx.p, idx.data(), idx.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+            }
+        } catch (...) {
+            if (op->stream) (void)hipStreamDestroy(op->stream);
+            for (auto& e : op->ev) if (e) (void)hipEventDestroy(e);
+            delete op;
+            throw;
+        }
+        *out = op;
+    });
+}
+
+int cs_kinship_destroy(cs_kinship* op) {
+    return guarded([&] {
+        if (!op) return;
+        if (op->stream) (void)hipStreamSynchronize(op->stream);
+        for (auto& e : op->ev) if (e) (void)hipEventDestroy(e);
+        hipStream_t s = op->stream;
+        delete op;
+        if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int cs_kinship_rows(const cs_kinship* op, int64_t* rows) {
+    return guarded([&] {
+        if (!op || !rows) throw InvalidArgument("cs_kinship_rows: null argument");
+        *rows = op->n;
+    });
+}
+
+int cs_kinship_num_variants(const cs_kinship* op, int64_t* m) {
+    return guarded([&] {
+        if (!op || !m) throw InvalidArgument("cs_kinship_num_variants: null argument");
+        *m = op->Msel;
+    });
+}
+
+int cs_kinship_set_precision(cs_kinship* op, int mode) {
+    return guarded([&] {
+        if (!op) throw InvalidArgument("cs_kinship_set_precision: null handle");
+        if (mode != CS_PREC_INT8_L3 && mode != CS_PREC_INT8_L2 && mode != CS_PREC_FP32)
+            throw InvalidArgument("cs_kinship_set_precision: unknown mode");
+        op->prec = mode;
+    });
+}
+
+int cs_kinship_apply_device(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
+                            int64_t ldy) {
+    return guarded([&] {
+        if (!op) throw InvalidArgument("cs_kinship_apply_device: null handle");
+        if (k > 0 && (!dX || !dY)) throw InvalidArgument("cs_kinship_apply_device: null buffer");
+        apply_device_impl(op, dX, ldx, k, dY, ldy);
+    });
+}
+
+int cs_kinship_apply(cs_kinship* op, const double* X, int64_t ldx, int64_t k, double* Y,
+                     int64_t ldy) {
+    return guarded([&] {
+        if (!op) throw InvalidArgument("cs_kinship_apply: null handle");
+        if (k < 0 || (k > 0 && (ldx < op->n || ldy < op->n)))
+            throw RuntimeError("PlinkKinshipOperator::apply: dimension mismatch");
+        if (k == 0) return;
+        if (!X || !Y) throw InvalidArgument("cs_kinship_apply: null buffer");
+        const int64_t n = op->n;
+        op->dX.ensure((size_t)(n * k) * sizeof(double));
+        op->dY.ensure((size_t)(n * k) * sizeof(double));
+        CS_HIP(hipMemcpy2DAsync(op->dX.p, (size_t)n * sizeof(double), X, (size_t)ldx * sizeof(double),
+                                (size_t)n * sizeof(double), (size_t)k, hipMemcpyHostToDevice, op->stream));
+        apply_device_impl(op, op->dX.as<double>(), n, k, op->dY.as<double>(), n);
+        CS_HIP(hipMemcpy2DAsync(Y, (size_t)ldy * sizeof(double), op->dY.p, (size_t)n * sizeof(double),
+                                (size_t)n * sizeof(double), (size_t)k, hipMemcpyDeviceToHost, op->stream));
+        CS_HIP(hipStreamSynchronize(op->stream));
+    });
+}
+
+int cs_kinship_diag(cs_kinship* op, double* diag) {
+    return guarded([&] {
+        if (!op || !diag) throw InvalidArgument("cs_kinship_diag: null argument");
+        cs_bed* b = op->bed;
+        hipStream_t st = op->stream;
+        op->diag_tab.ensure((size_t)b->M * 4 * sizeof(double));
+        op->diag_full.ensure((size_t)b->Npad * sizeof(double));
+        k_diag_table<<<grid_for(b->M), 256, 0, st>>>(b->M, b->means.as<double>(), b->inv_stds.as<double>(),
+                                                    op->d_mult.as<int32_t>(), op->diag_tab.as<double>());
+        k_diag<<<(unsigned)((b->N + 3) / 4), 256, 0, st>>>(b->smp.as<uint8_t>(), b->CS, b->N, b->M,
+                                                          op->diag_tab.as<double>(),
+                                                          1.0 / (double)op->Msel, op->diag_full.as<double>());
+        CS_HIP(hipGetLastError());
+        std::vector<double> full((size_t)b->N);
+        CS_HIP(hipMemcpyAsync(full.data(), op->diag_full.p, (size_t)b->N * sizeof(double),
+                              hipMemcpyDeviceToHost, st));
+        CS_HIP(hipStreamSynchronize(st));
+        for (int64_t r = 0; r < op->n; ++r) diag[r] = full[(size_t)(op->keep.empty() ? r : op->keep[(size_t)r])];
+    });
+}
+
+int cs_kinship_trace(cs_kinship* op, double* trace) {
+    return guarded([&] {
+        if (!op || !trace) throw InvalidArgument("cs_kinship_trace: null argument");
+        std::vector<double> d((size_t)op->n);
+        if (cs_kinship_diag(op, d.data()) != CS_OK) throw RuntimeError(cs_last_error());
+        double t = 0.0;
+        for (double v : d) t += v;
+        *trace = t;
+    });
+}
+
+int cs_kinship_last_timing(const cs_kinship* op, double* ms5) {
+    return guarded([&] {
+        if (!op || !ms5) throw InvalidArgument("cs_kinship_last_timing: null argument");
+        for (int p = 0; p < 5; ++p) ms5[p] = op->last_ms[p];
+    });
+}
+
+}  // extern "C"

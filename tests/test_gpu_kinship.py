@@ -1,0 +1,120 @@
+"""GPU parity of kernels (a) decode/stats and (b) K.V against the CPU oracle, through the C-ABI.
+
+Mirrors the reference's own kinship test (tests/test_kinship_batch.cpp): the same n = 37 x m =
+1031 fixture (partial last byte, monomorphic and all-missing SNPs), the six sample / variant
+selections, RHS widths 1 / 7 / 41, trace, repeat determinism, empty RHS and dimension errors.
+
+Tolerances.  Integer work (class counts, generated bytes) and the FP64 statistics are compared
+bit for bit.  K.V in the default INT8_L3 mode quantises each RHS column to 23-bit fixed point (and
+the intermediate Z^T V likewise), so the bound is relative to the column maximum:
+max|Y - K X| <= 2e-6 * max|K X| (per column).  INT8_L2 (15-bit): 1e-3.  FP32: 2e-5.
+trace/diag are FP64: relative 1e-10 as in the reference test.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def csa():
+    import cosmicscanner_amd as csa
+    return csa
+
+
+def colrel(Y, R):
+    den = np.maximum(np.max(np.abs(R), axis=0), 1e-300)
+    return float(np.max(np.max(np.abs(Y - R), axis=0) / den))
+
+
+def test_stats_bitwise(csa, kb_fixture, ref_datasets):
+    bed, N, M, _ = kb_fixture
+    cases = [(bed, N, M)] + [(v[0], v[1], v[2]) for v in ref_datasets.values()]
+    for body, n, m in cases:
+        r = csa.PlinkReader.from_body(body, n, m)
+        means, inv = oracle.stats(body, n, m)
+        assert np.array_equal(r.getMeans(), means)
+        assert np.array_equal(r.getInvStds(), inv)
+        assert np.array_equal(r.class_counts(), oracle.class_counts(body, n, m))
+        w = np.random.default_rng(n).integers(0, 3, n).astype(np.int32)
+        assert np.array_equal(r.class_counts(w), oracle.class_counts(body, n, m, w))
+
+
+def test_load_file_and_magic(csa, tmp_path):
+    import os
+    prefix = os.path.join(os.path.dirname(__file__), "golden", "ref_synth", "synth")
+    r = csa.PlinkReader.load(prefix)
+    body, N, M, _, _ = oracle.read_bed(prefix)
+    assert (r.getNumSamples(), r.getNumSnps()) == (N, M)
+    assert np.array_equal(r.download(), body)
+    p = tmp_path / "bad"
+    (p.with_suffix(".fam")).write_text("a a 0 0 0 1\n")
+    (p.with_suffix(".bim")).write_text("1 s 0 1 A C\n")
+    (p.with_suffix(".bed")).write_bytes(bytes([0x6C, 0x1B, 0x00, 0]))
+    with pytest.raises(RuntimeError, match="SNP-major"):
+        csa.PlinkReader.load(str(p))
+
+
+@pytest.mark.parametrize("N,M", [(1001, 300), (256, 257), (37, 1031)])
+def test_synthetic_generator_bytes(csa, N, M):
+    r = csa.PlinkReader.synthetic(20261204, N, M, 0.01, 0.5, 0.005)
+    assert np.array_equal(r.download(), oracle.gen_hwe_bed(20261204, N, M, 0.01, 0.5, 0.005))
+
+
+def test_kinship_batch_selections(csa, kb_fixture):
+    bed, N, M, sels = kb_fixture
+    reader = csa.PlinkReader.from_body(bed, N, M)
+    means, inv = oracle.stats(bed, N, M)
+    rng = np.random.default_rng(8128)
+    for keep, variants in sels:
+        op = csa.createKinshipOperator(reader, keep, variants)
+        n = N if keep is None else keep.size
+        assert op.rows() == n and op.cols() == n
+        assert op.numVariants() == (M if variants is None else variants.size)
+        tr = oracle.kinship_trace(bed, N, means, inv, keep, variants)
+        assert abs(op.trace() - tr) <= 1e-10 * abs(tr)
+        d = oracle.kinship_diag(bed, N, means, inv, keep, variants)
+        assert np.max(np.abs(op.getDiagonal() - d)) <= 1e-10 * np.max(np.abs(d))
+        for width in (1, 7, 41):
+            X = rng.standard_normal((n, width))
+            ref = oracle.kinship_apply(bed, N, means, inv, keep, variants, X)
+            Y = op.apply(X)
+            assert colrel(Y, ref) < 2e-6, (keep is None, variants is None, width)
+            assert np.array_equal(op.apply(X), Y)                   # bitwise repeat
+            v = X[:, 0]
+            assert colrel(op.applyTranspose(v)[:, None], ref[:, :1]) < 2e-6
+        out = op.apply(np.zeros((n, 0)))
+        assert out.shape == (n, 0)
+        with pytest.raises(RuntimeError, match="dimension mismatch"):
+            op.apply(np.ones((n + 1, 1)))
+        op.close()
+
+
+def test_kinship_errors(csa, kb_fixture):
+    bed, N, M, _ = kb_fixture
+    reader = csa.PlinkReader.from_body(bed, N, M)
+    with pytest.raises(RuntimeError, match="sample index out of range"):
+        csa.PlinkKinshipOperator(reader, [0, N])
+    with pytest.raises(RuntimeError, match="variant index out of range"):
+        csa.PlinkKinshipOperator(reader, None, [M])
+
+
+@pytest.mark.parametrize("mode,tol", [(3, 2e-6), (2, 1e-3), (32, 2e-5)])
+def test_kinship_precision_modes_larger(csa, mode, tol):
+    N, M = 3000, 4100
+    body = oracle.gen_hwe_bed(77, N, M, 0.01, 0.5, 0.01)
+    reader = csa.PlinkReader.from_body(body, N, M)
+    means, inv = oracle.stats(body, N, M)
+    rng = np.random.default_rng(5)
+    keep = np.sort(rng.choice(N, 2200, replace=False)).astype(np.int32)
+    chrom = np.arange(M) * 22 // M
+    loco = np.flatnonzero(chrom != 3).astype(np.int32)
+    for kp, vp in [(None, None), (keep, loco)]:
+        op = csa.PlinkKinshipOperator(reader, kp, vp)
+        op.set_precision(mode)
+        X = rng.standard_normal((op.rows(), 16))
+        ref = oracle.kinship_apply(body, N, means, inv, kp, vp, X)
+        assert colrel(op.apply(X), ref) < tol
+        op.close()
