@@ -49,6 +49,11 @@ def _load():
     sig = {
         "cs_last_error": ([], ctypes.c_char_p),
         "cs_version": ([], ctypes.c_int),
+        "cs_dev_alloc": ([_i64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "cs_dev_free": ([_vp], ctypes.c_int),
+        "cs_dev_upload": ([_vp, _vp, _i64], ctypes.c_int),
+        "cs_dev_download": ([_vp, _vp, _i64], ctypes.c_int),
+        "cs_dev_synchronize": ([], ctypes.c_int),
         "cs_bed_create": ([_u8p, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
         "cs_bed_create_from_file_image": ([_u8p, _i64, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
         "cs_bed_create_synthetic": ([ctypes.c_uint64, _i64, _i64, ctypes.c_double, ctypes.c_double,
@@ -71,6 +76,11 @@ def _load():
         "cs_kinship_trace": ([_vp, _dp], ctypes.c_int),
         "cs_kinship_diag": ([_vp, _dp], ctypes.c_int),
         "cs_kinship_last_timing": ([_vp, _dp], ctypes.c_int),
+        "cs_scan_sums": ([_vp, _i32p, _i64, _dp, _i64, _i64, ctypes.c_double, _i32p, _i64,
+                          ctypes.c_int, _dp, _i64p, _i32p, _dp, _dp], ctypes.c_int),
+        "cs_lmm_scan": ([_vp, _i32p, _i64, _dp, _i64, _i64, ctypes.c_double, _i32p, _i64,
+                         ctypes.c_int, ctypes.c_void_p, _dp, _i64p, _i32p, _dp, _dp, _dp, _dp],
+                        ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
         f = getattr(L, name)
@@ -99,6 +109,49 @@ def _check(rc):
 
 def _p(a, t):
     return None if a is None else a.ctypes.data_as(t)
+
+
+class DeviceArray:
+    """A float64 column-major (n, k) buffer in HBM, owned by this object."""
+
+    def __init__(self, n, k=1):
+        self.n, self.k = int(n), int(k)
+        self.nbytes = self.n * self.k * 8
+        p = _vp()
+        _check(lib().cs_dev_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.asarray(a, dtype=np.float64)
+        a2 = a[:, None] if a.ndim == 1 else a
+        d = cls(a2.shape[0], a2.shape[1])
+        d.upload(a2)
+        return d
+
+    def upload(self, a):
+        f = np.asfortranarray(np.asarray(a, dtype=np.float64).reshape(self.n, self.k, order="A"))
+        _check(lib().cs_dev_upload(self.ptr, f.ctypes.data_as(_vp), self.nbytes))
+
+    def download(self):
+        out = np.empty((self.n, self.k), order="F")
+        _check(lib().cs_dev_download(out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return np.ascontiguousarray(out)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            _check(lib().cs_dev_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(lib().cs_dev_synchronize())
 
 
 def read_plink_text(prefix):
@@ -294,3 +347,63 @@ class PlinkKinshipOperator:
 def createKinshipOperator(reader, keep_indices=None, variant_indices=None):
     """Factory (src/kinship_operator.h:98-100, kinship_operator.cpp:370-374)."""
     return PlinkKinshipOperator(reader, keep_indices, variant_indices)
+
+
+# ----------------------------------------------------------------------------- scan (kernel c)
+class ScanModel(ctypes.Structure):
+    """Variance model of the fused scan (cs_scan_model in the C header)."""
+    _fields_ = [("n_score", ctypes.c_int64), ("alpha", ctypes.c_double),
+                ("quad_begin", ctypes.c_int64), ("quad_end", ctypes.c_int64),
+                ("quad_w", _dp), ("cov_begin", ctypes.c_int64), ("cov_end", ctypes.c_int64),
+                ("cov_C", _dp)]
+
+
+def _opt_i32(a):
+    return None if a is None or len(a) == 0 else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def scan_sums(reader, keep, R, min_maf=0.01, snps=None, prec=CS_PREC_INT8_L3):
+    """Prepared-marker sums per SNP: (freq, n_obs, pass, dot (n_snps, nr), gtg)."""
+    keep = _opt_i32(keep)
+    snps = _opt_i32(snps)
+    n = reader.num_samples if keep is None else keep.size
+    R = np.asfortranarray(np.asarray(R, dtype=np.float64).reshape(n, -1))
+    nr = R.shape[1]
+    ns = reader.num_snps if snps is None else snps.size
+    freq = np.empty(ns)
+    nobs = np.empty(ns, dtype=np.int64)
+    ps = np.empty(ns, dtype=np.int32)
+    dot = np.empty((ns, nr))
+    gtg = np.empty(ns)
+    _check(lib().cs_scan_sums(reader._h, _p(keep, _i32p), 0 if keep is None else keep.size,
+                              _p(R, _dp), n, nr, float(min_maf), _p(snps, _i32p),
+                              0 if snps is None else snps.size, int(prec), _p(freq, _dp),
+                              _p(nobs, _i64p), _p(ps, _i32p), _p(dot, _dp), _p(gtg, _dp)))
+    return freq, nobs, ps.astype(bool), dot, gtg
+
+
+def lmm_scan(reader, keep, R, n_score=1, alpha=0.0, quad=None, quad_w=None, cov=None, cov_C=None,
+             min_maf=0.01, snps=None, prec=CS_PREC_INT8_L3):
+    """Fused per-SNP scan; returns dict of freq, n_obs, pass, beta, se, p, log10p."""
+    keep = _opt_i32(keep)
+    snps = _opt_i32(snps)
+    n = reader.num_samples if keep is None else keep.size
+    R = np.asfortranarray(np.asarray(R, dtype=np.float64).reshape(n, -1))
+    nr = R.shape[1]
+    ns = reader.num_snps if snps is None else snps.size
+    qb, qe = quad if quad is not None else (0, 0)
+    cb, ce = cov if cov is not None else (0, 0)
+    qw = np.ascontiguousarray(quad_w if quad_w is not None else np.zeros(0), dtype=np.float64)
+    cC = np.ascontiguousarray(cov_C if cov_C is not None else np.zeros(0), dtype=np.float64)
+    m = ScanModel(n_score, float(alpha), qb, qe, _p(qw, _dp), cb, ce, _p(cC, _dp))
+    out = {k: np.empty((ns, n_score)) for k in ("beta", "se", "p", "log10p")}
+    freq = np.empty(ns)
+    nobs = np.empty(ns, dtype=np.int64)
+    ps = np.empty(ns, dtype=np.int32)
+    _check(lib().cs_lmm_scan(reader._h, _p(keep, _i32p), 0 if keep is None else keep.size,
+                             _p(R, _dp), n, nr, float(min_maf), _p(snps, _i32p),
+                             0 if snps is None else snps.size, int(prec), ctypes.byref(m),
+                             _p(freq, _dp), _p(nobs, _i64p), _p(ps, _i32p), _p(out["beta"], _dp),
+                             _p(out["se"], _dp), _p(out["p"], _dp), _p(out["log10p"], _dp)))
+    out.update(freq=freq, n_obs=nobs, passes=ps.astype(bool))
+    return out

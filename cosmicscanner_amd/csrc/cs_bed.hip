@@ -5,6 +5,7 @@
 // (the .bed order, plink_reader.cpp:84); rows are padded to Npad = roundup(N, 256) samples and
 // the matrix to Mpad = roundup(M, 256) rows.  Sample-major: row i (sample) holds Mpad/4 bytes,
 // SNP j at byte j>>2, bits 2(j&3).  Every padding slot holds code 01 (missing).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -288,6 +289,35 @@ extern "C" {
 
 const char* cs_last_error(void) { return g_last_error.c_str(); }
 int cs_version(void) { return 100; }
+
+int cs_dev_alloc(int64_t bytes, void** dptr) {
+    return guarded([&] {
+        if (!dptr || bytes < 0) throw InvalidArgument("cs_dev_alloc: bad argument");
+        CS_HIP(hipMalloc(dptr, (size_t)std::max<int64_t>(bytes, 1)));
+    });
+}
+
+int cs_dev_free(void* dptr) {
+    return guarded([&] {
+        if (dptr) CS_HIP(hipFree(dptr));
+    });
+}
+
+int cs_dev_upload(void* dst, const void* src, int64_t bytes) {
+    return guarded([&] {
+        if (bytes > 0) CS_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    });
+}
+
+int cs_dev_download(void* dst, const void* src, int64_t bytes) {
+    return guarded([&] {
+        if (bytes > 0) CS_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int cs_dev_synchronize(void) {
+    return guarded([&] { CS_HIP(hipDeviceSynchronize()); });
+}
 
 int cs_bed_create(const uint8_t* body, int64_t n_samples, int64_t n_snps, cs_bed** out) {
     return guarded([&] {

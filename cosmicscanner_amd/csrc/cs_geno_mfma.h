@@ -40,6 +40,11 @@ __global__ __launch_bounds__(256) void k_geno_mfma(const uint8_t* __restrict__ A
                                                    const int8_t* __restrict__ Bo,
                                                    int32_t* __restrict__ out) {
     constexpr int MT = 4;
+    constexpr int NB = TWO_B ? 2 : 1;
+    constexpr int TILE = 4 * NTB * 1024 * NB;   // B bytes per 256-k block (all steps, tiles)
+    constexpr int PER_T = TILE / 4096;          // uint4 per thread to stage one tile
+    __shared__ __attribute__((aligned(16))) int8_t sB[2][TILE];
+
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
     const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * MT);
@@ -49,31 +54,61 @@ __global__ __launch_bounds__(256) void k_geno_mfma(const uint8_t* __restrict__ A
     int64_t kb1 = kb0 + kb_per_split;
     if (kb1 > kb_total) kb1 = kb_total;
 
-    v4i accX[MT][NTB], accO[MT][NTB];
+    v4i acc[2 * MT * NTB];
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+    for (int i = 0; i < 2 * MT * NTB; ++i) acc[i] = v4i{0, 0, 0, 0};
+
+    // Global B of one block: [t][ntile (all NT)][lane][16]; the staged tile keeps only this
+    // block's NTB tiles: sB[(t*NTB + n)*1024 + lane*16] (+ 4*NTB*1024 for the o-operand).
+    auto stage_load = [&](int64_t kb, uint4 (&reg)[PER_T]) {
 #pragma unroll
-        for (int n = 0; n < NTB; ++n) {
-            accX[m][n] = v4i{0, 0, 0, 0};
-            accO[m][n] = v4i{0, 0, 0, 0};
+        for (int i = 0; i < PER_T; ++i) {
+            const int e = (i * 256 + threadIdx.x) * 16;         // byte in the staged tile
+            const int which = e / (4 * NTB * 1024);              // 0: x operand, 1: o operand
+            const int rem = e - which * (4 * NTB * 1024);
+            const int t = rem / (NTB * 1024);
+            const int rem2 = rem - t * (NTB * 1024);
+            const int n = rem2 >> 10, l16 = rem2 & 1023;
+            const int8_t* src = (which ? Bo : Bx) + (((kb * 4 + t) * NT + nt0 + n) << 10) + l16;
+            reg[i] = *reinterpret_cast<const uint4*>(src);
         }
-
-    const uint8_t* arow[MT];
+    };
+    auto stage_store = [&](int buf, const uint4 (&reg)[PER_T]) {
 #pragma unroll
-    for (int m = 0; m < MT; ++m) arow[m] = A + (row0 + 16 * m + r) * lda + g * 16;
+        for (int i = 0; i < PER_T; ++i)
+            *reinterpret_cast<uint4*>(&sB[buf][(i * 256 + threadIdx.x) * 16]) = reg[i];
+    };
 
+    const uint8_t* abase = A + (row0 + r) * lda + g * 16;
+    const int64_t astride = 16 * lda;
+    uint4 a[MT], an[MT];
+    uint4 breg[PER_T];
+    if (kb0 < kb1) {
+#pragma unroll
+        for (int m = 0; m < MT; ++m) a[m] = *reinterpret_cast<const uint4*>(abase + m * astride + kb0 * 64);
+        stage_load(kb0, breg);
+        stage_store(0, breg);
+    }
+    __syncthreads();
+    int buf = 0;
     for (int64_t kb = kb0; kb < kb1; ++kb) {
-        uint4 a[MT];
+        const bool more = kb + 1 < kb1;
+        if (more) {
 #pragma unroll
-        for (int m = 0; m < MT; ++m) a[m] = *reinterpret_cast<const uint4*>(arow[m] + kb * 64);
+            for (int m = 0; m < MT; ++m)
+                an[m] = *reinterpret_cast<const uint4*>(abase + m * astride + (kb + 1) * 64);
+            stage_load(kb + 1, breg);
+        }
+        const int8_t* tile = sB[buf];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             v4i bx[NTB], bo[NTB];
 #pragma unroll
             for (int n = 0; n < NTB; ++n) {
-                const int64_t off = ((((kb * 4 + t) * NT + nt0 + n) * 64 + lane) << 4);
-                bx[n] = *reinterpret_cast<const v4i*>(Bx + off);
-                if (TWO_B) bo[n] = *reinterpret_cast<const v4i*>(Bo + off);
+                bx[n] = *reinterpret_cast<const v4i*>(tile + ((t * NTB + n) << 10) + lane * 16);
+                if (TWO_B)
+                    bo[n] = *reinterpret_cast<const v4i*>(tile + 4 * NTB * 1024 +
+                                                          ((t * NTB + n) << 10) + lane * 16);
             }
 #pragma unroll
             for (int m = 0; m < MT; ++m) {
@@ -90,12 +125,20 @@ __global__ __launch_bounds__(256) void k_geno_mfma(const uint8_t* __restrict__ A
                                   (int)__builtin_amdgcn_perm(0u, kLutO, d3)};
 #pragma unroll
                 for (int n = 0; n < NTB; ++n) {
-                    accX[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(X, bx[n], accX[m][n], 0, 0, 0);
-                    accO[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[n] : bx[n],
-                                                                       accO[m][n], 0, 0, 0);
+                    v4i& cx = acc[(m * NTB + n) * 2];
+                    v4i& co = acc[(m * NTB + n) * 2 + 1];
+                    cx = __builtin_amdgcn_mfma_i32_16x16x64_i8(X, bx[n], cx, 0, 0, 0);
+                    co = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[n] : bx[n], co, 0, 0, 0);
                 }
             }
         }
+        if (more) {
+            stage_store(buf ^ 1, breg);
+#pragma unroll
+            for (int m = 0; m < MT; ++m) a[m] = an[m];
+        }
+        __syncthreads();
+        buf ^= 1;
     }
     // C layout of 16x16 MFMA: lane holds column (lane % 16), rows 4*(lane/16) + v, v = 0..3.
     const int NC = NT * 16;
@@ -108,8 +151,8 @@ __global__ __launch_bounds__(256) void k_geno_mfma(const uint8_t* __restrict__ A
                 const int64_t row = row0 + 16 * m + 4 * g + v;
                 const int col = (nt0 + n) * 16 + r;
                 int32_t* o = out + (((int64_t)split * rows_pad + row) * 2) * NC + col;
-                o[0] = accX[m][n][v];
-                o[NC] = accO[m][n][v];
+                o[0] = acc[(m * NTB + n) * 2][v];
+                o[NC] = acc[(m * NTB + n) * 2 + 1][v];
             }
 }
 

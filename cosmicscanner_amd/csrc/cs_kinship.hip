@@ -15,6 +15,7 @@
 
 #include "cs_common.h"
 #include "cs_geno_mfma.h"
+#include "cs_ops.h"
 
 namespace csamd {
 void ensure_stats(cs_bed* b);
@@ -39,8 +40,6 @@ struct cs_kinship {
 
 namespace csamd {
 
-constexpr double kQmax3 = 127.0 * 65536.0;  // |q| bound for 3 balanced limbs (top limb <= 127)
-constexpr double kQmax2 = 127.0 * 256.0;
 
 // ------------------------------------------------------------------------------------------
 __global__ void k_scatter(const double* __restrict__ V, int64_t ldv, int64_t k,
@@ -313,13 +312,9 @@ __global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, i
 }
 
 // ------------------------------------------------------------------------------------------
-static int grid_for(int64_t total, int block = 256) {
-    int64_t g = (total + block - 1) / block;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
-}
 
 template <bool TWO_B>
-static void launch_mfma(int NTB, dim3 grid, hipStream_t st, const uint8_t* A, int64_t lda,
+void launch_mfma(int NTB, dim3 grid, hipStream_t st, const uint8_t* A, int64_t lda,
                         int64_t rows_pad, int64_t kbps, int64_t kbt, int NT, const int8_t* Bx,
                         const int8_t* Bo, int32_t* out) {
     switch (NTB) {
@@ -331,34 +326,37 @@ static void launch_mfma(int NTB, dim3 grid, hipStream_t st, const uint8_t* A, in
     CS_HIP(hipGetLastError());
 }
 
-// Column tiling of an int8 product with k right-hand sides in L limbs.
-struct Tiling {
-    int NT, NTB, NC;
-};
-static Tiling tiling(int64_t k, int L) {
+Tiling tiling(int64_t k, int L) {
     int NT = (int)((L * k + 15) / 16);
     int NTB = NT <= 4 ? NT : 4;
     NT = (int)round_up(NT, NTB);
     return Tiling{NT, NTB, NT * 16};
 }
 
-// Public helper for other translation units (scan): Z^T V int8 product on the SNP-major copy.
-// Vt: [Npad][k] fp64 in full sample space.  Output PQ [Mpad][2][NC] int32 and the V scales.
-void gt_product_int8(cs_bed* b, const double* Vt, int64_t k, int L, hipStream_t st, DevBuf& BV,
-                     DevBuf& mx, DevBuf& scl, DevBuf& PQ, Tiling& tl, hipEvent_t ev_mid) {
-    tl = tiling(k, L);
+// Quantise the columns of a row-major [rows][k] fp64 matrix (rows = padded K extent) into the
+// permuted int8 limb layout of the MFMA B operand.  scl[0:k] = scales, scl[k:2k] = inverses.
+void quantize_cols(const double* W, int64_t rows, int64_t k, int L, const Tiling& tl,
+                   hipStream_t st, DevBuf& mx, DevBuf& scl, DevBuf& B) {
     const double qmax = L == 3 ? kQmax3 : kQmax2;
     mx.ensure((size_t)k * sizeof(unsigned long long));
     scl.ensure((size_t)2 * k * sizeof(double));
     CS_HIP(hipMemsetAsync(mx.p, 0, (size_t)k * sizeof(unsigned long long), st));
-    k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(Vt, b->Npad, k, mx.as<unsigned long long>());
-    k_scales<<<1, 256, 0, st>>>(mx.as<unsigned long long>(), k, qmax, scl.as<double>(),
+    k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(W, rows, k, mx.as<unsigned long long>());
+    k_scales<<<grid_for(k), 256, 0, st>>>(mx.as<unsigned long long>(), k, qmax, scl.as<double>(),
                                 scl.as<double>() + k);
-    BV.ensure((size_t)(b->Npad * tl.NC));
-    CS_HIP(hipMemsetAsync(BV.p, 0, (size_t)(b->Npad * tl.NC), st));
-    k_quant<<<grid_for(b->Npad * k), 256, 0, st>>>(Vt, b->Npad, k, L, tl.NT, qmax,
-                                                  scl.as<double>() + k, BV.as<int8_t>());
+    B.ensure((size_t)(rows * tl.NC));
+    CS_HIP(hipMemsetAsync(B.p, 0, (size_t)(rows * tl.NC), st));
+    k_quant<<<grid_for(rows * k), 256, 0, st>>>(W, rows, k, L, tl.NT, qmax, scl.as<double>() + k,
+                                               B.as<int8_t>());
     CS_HIP(hipGetLastError());
+}
+
+// Z^T V int8 product on the SNP-major copy.  Vt: [Npad][k] fp64 in full sample space.  Output
+// PQ [Mpad][2][NC] int32 and the V scales in scl.
+void gt_product_int8(cs_bed* b, const double* Vt, int64_t k, int L, hipStream_t st, DevBuf& BV,
+                     DevBuf& mx, DevBuf& scl, DevBuf& PQ, Tiling& tl, hipEvent_t ev_mid) {
+    tl = tiling(k, L);
+    quantize_cols(Vt, b->Npad, k, L, tl, st, mx, scl, BV);
     if (ev_mid) CS_HIP(hipEventRecord(ev_mid, st));
     PQ.ensure((size_t)b->Mpad * 2 * tl.NC * sizeof(int32_t));
     dim3 grid((unsigned)(b->Mpad / 256), 1, (unsigned)(tl.NT / tl.NTB));
@@ -398,7 +396,7 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(op->W1.as<double>(), b->Mpad, k, mxw);
     k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, mxw);
     double* sclW = op->scl.as<double>();  // V scales no longer needed past k_epi_gt
-    k_scales<<<1, 256, 0, st>>>(mxw, k, qmax, sclW, sclW + k);
+    k_scales<<<grid_for(k), 256, 0, st>>>(mxw, k, qmax, sclW, sclW + k);
     op->B1.ensure((size_t)(b->Mpad * tl.NC));
     op->B2.ensure((size_t)(b->Mpad * tl.NC));
     CS_HIP(hipMemsetAsync(op->B1.p, 0, (size_t)(b->Mpad * tl.NC), st));
@@ -665,3 +663,10 @@ int cs_kinship_last_timing(const cs_kinship* op, double* ms5) {
 }
 
 }  // extern "C"
+
+namespace csamd {
+template void launch_mfma<false>(int, dim3, hipStream_t, const uint8_t*, int64_t, int64_t, int64_t,
+                                 int64_t, int, const int8_t*, const int8_t*, int32_t*);
+template void launch_mfma<true>(int, dim3, hipStream_t, const uint8_t*, int64_t, int64_t, int64_t,
+                                int64_t, int, const int8_t*, const int8_t*, int32_t*);
+}  // namespace csamd

@@ -1,0 +1,349 @@
+// Kernel (c): the per-SNP LMM association scan on the resident packed genotypes.
+//
+// For every tested SNP j the reference prepares the marker on the keep subset
+// (genetics::prepareAssociationMarker, src/genetics/genotype_encoding.h:33-81: observed count,
+// A1 frequency, MAF filter with n_obs >= 3, mean-impute then centre) and then forms GLS sums of
+// the prepared marker g_j against projected vectors: score = g^T P y and a variance g^T P g
+// (lmm_engine.cpp:1363-1525 rotated form, :1269-1361 low-rank form).  Every one of those sums is
+// a dot product of g_j with a column of some n x nr matrix R, and
+//     g_j^T R_c = sum_s x~_sj R~_sc - mean_j sum_s o_sj R~_sc
+// (R~ = R scattered into the full sample space with keep multiplicities), i.e. the same exact
+// int8-MFMA Z^T V product as the kinship operator; g^T g comes from weighted class counts.
+// The FP64 epilogue assembles score, variance, beta = score/var, SE = var^-1/2 and the chi^2_1
+// tail with the reference's own series / continued fraction (lmm_engine.cpp:22-64).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cs_common.h"
+#include "cs_ops.h"
+
+namespace csamd {
+
+// gamma_q / chisq_prob (lmm_engine.cpp:22-64), restated for the device in FP64.
+__device__ double d_gamma_q(double a, double x) {
+    if (x < 0.0 || a <= 0.0) return 1.0;
+    if (x < a + 1.0) {
+        double ap = a, sum = 1.0 / a, del = sum;
+        for (int i = 0; i < 100; ++i) {
+            ap += 1.0;
+            del *= x / ap;
+            sum += del;
+            if (fabs(del) < fabs(sum) * 1e-7) return 1.0 - sum * exp(-x + a * log(x) - lgamma(a));
+        }
+    } else {
+        double b = x + 1.0 - a, c = 1.0 / 1e-30, d = 1.0 / b, h = d;
+        for (int i = 1; i <= 100; ++i) {
+            const double an = -i * (i - a);
+            b += 2.0;
+            d = an * d + b;
+            if (fabs(d) < 1e-30) d = 1e-30;
+            c = b + an / c;
+            if (fabs(c) < 1e-30) c = 1e-30;
+            d = 1.0 / d;
+            const double del = d * c;
+            h *= del;
+            if (fabs(del - 1.0) < 1e-7) return exp(-x + a * log(x) - lgamma(a)) * h;
+        }
+    }
+    return 0.0;
+}
+
+__device__ double d_chisq_prob(double stat, int df) {
+    if (stat <= 0) return 1.0;
+    return d_gamma_q(df / 2.0, stat / 2.0);
+}
+
+struct ModelDev {
+    int64_t n_score, quad_begin, quad_end, cov_begin, cov_end;
+    double alpha;
+    const double* quad_w;  // device
+    const double* cov_C;   // device
+};
+
+// One thread per requested SNP whose row falls in [row_lo, row_lo + rows): prepared-marker
+// statistics, the nr dot products, and (stats mode) the association statistics.
+__global__ void k_scan_epilogue(const int32_t* __restrict__ snps, int64_t n_snps, int64_t row_lo,
+                                int64_t rows, const int64_t* __restrict__ counts,
+                                const int32_t* __restrict__ PQ, int NC, int64_t nr, int L,
+                                const double* __restrict__ scale, double min_maf,
+                                double* __restrict__ freq, int64_t* __restrict__ n_obs,
+                                int32_t* __restrict__ pass, double* __restrict__ dot,
+                                double* __restrict__ gtg, ModelDev md, double* __restrict__ beta,
+                                double* __restrict__ se, double* __restrict__ p,
+                                double* __restrict__ log10p) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n_snps) return;
+    const int64_t j = snps[i];
+    if (j < row_lo || j >= row_lo + rows) return;
+    const int64_t* c = counts + j * 4;
+    const int64_t obs = c[0] + c[2] + c[3];
+    const double sx = 2.0 * (double)c[0] + (double)c[2];
+    double f = 0.0;
+    int ps = 0;
+    double mean = 0.0, g2 = 0.0;
+    if (obs > 0) {
+        f = sx / (2.0 * (double)obs);
+        const double maf = f < 1.0 - f ? f : 1.0 - f;
+        ps = (obs >= 3 && maf >= min_maf) ? 1 : 0;
+        mean = sx / (double)obs;
+        g2 = (double)c[0] * (2.0 - mean) * (2.0 - mean) + (double)c[2] * (1.0 - mean) * (1.0 - mean) +
+             (double)c[3] * mean * mean;
+    }
+    freq[i] = f;
+    n_obs[i] = obs;
+    pass[i] = ps;
+    if (gtg) gtg[i] = g2;
+    const int32_t* row = PQ + (j - row_lo) * 2 * NC;
+    auto dotc = [&](int64_t col) -> double {
+        long long P = 0, Q = 0;
+        for (int l = L - 1; l >= 0; --l) {
+            P = P * 256 + row[l * nr + col];
+            Q = Q * 256 + row[NC + l * nr + col];
+        }
+        return ((double)P - mean * (double)Q) * scale[col];
+    };
+    if (dot) {
+        for (int64_t col = 0; col < nr; ++col) dot[i * nr + col] = obs > 0 ? dotc(col) : 0.0;
+    }
+    if (!beta) return;
+    // variance shared by all score columns
+    double var = md.alpha * g2;
+    for (int64_t col = md.quad_begin; col < md.quad_end; ++col) {
+        const double v = dotc(col);
+        var += md.quad_w[col - md.quad_begin] * v * v;
+    }
+    const int64_t nc = md.cov_end - md.cov_begin;
+    if (nc > 0) {
+        double q[32];
+        for (int64_t a = 0; a < nc; ++a) q[a] = dotc(md.cov_begin + a);
+        double corr = 0.0;
+        for (int64_t a = 0; a < nc; ++a) {
+            double t = 0.0;
+            for (int64_t b2 = 0; b2 < nc; ++b2) t += md.cov_C[a * nc + b2] * q[b2];
+            corr += q[a] * t;
+        }
+        var -= corr;
+    }
+    const bool tested = ps && obs > 0 && var > 1e-12 && isfinite(var);
+    for (int64_t t = 0; t < md.n_score; ++t) {
+        const int64_t o = i * md.n_score + t;
+        if (!tested) {
+            beta[o] = se[o] = p[o] = log10p[o] = __builtin_nan("");
+            continue;
+        }
+        const double score = dotc(t);
+        const double sd = sqrt(var);
+        const double stat = score / sd;
+        const double pv = d_chisq_prob(stat * stat, 1);
+        beta[o] = score / var;
+        se[o] = 1.0 / sd;
+        p[o] = pv;
+        log10p[o] = pv > 0.0 ? -log10(pv) : 300.0;
+    }
+}
+
+struct ScanArgs {
+    const int32_t* keep = nullptr;
+    int64_t n = 0;
+    const double* R = nullptr;
+    int64_t ldr = 0, nr = 0;
+    double min_maf = 0.0;
+    const int32_t* snps = nullptr;
+    int64_t n_snps = 0;
+    int prec = CS_PREC_INT8_L3;
+    bool device_R = false;
+};
+
+struct ScanOut {
+    double *freq = nullptr, *dot = nullptr, *gtg = nullptr;
+    int64_t* n_obs = nullptr;
+    int32_t* pass = nullptr;
+    double *beta = nullptr, *se = nullptr, *p = nullptr, *log10p = nullptr;
+};
+
+static void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const ScanOut& o) {
+    if (a.nr < 1) throw InvalidArgument("scan: need at least one projected column");
+    ensure_stats(b);
+    const int64_t N = b->N, M = b->M;
+    const int64_t n = a.keep ? a.n : N;
+    if (a.ldr < n) throw RuntimeError("scan: dimension mismatch");
+    if (a.keep)
+        for (int64_t r = 0; r < n; ++r)
+            if (a.keep[r] < 0 || a.keep[r] >= N) throw RuntimeError("scan: sample index out of range");
+    std::vector<int32_t> all;
+    const int32_t* snps = a.snps;
+    int64_t ns = a.n_snps;
+    if (!snps) {
+        all.resize((size_t)M);
+        for (int64_t j = 0; j < M; ++j) all[(size_t)j] = (int32_t)j;
+        snps = all.data();
+        ns = M;
+    }
+    int64_t lo = M, hi = -1;
+    for (int64_t i = 0; i < ns; ++i) {
+        if (snps[i] < 0 || snps[i] >= M) throw RuntimeError("scan: variant index out of range");
+        lo = std::min<int64_t>(lo, snps[i]);
+        hi = std::max<int64_t>(hi, snps[i]);
+    }
+    if (ns == 0) return;
+    const int L = a.prec == CS_PREC_INT8_L2 ? 2 : 3;
+    hipStream_t st;
+    CS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    try {
+        // multiplicity weights and keep inverse
+        std::vector<int32_t> w((size_t)b->Npad, 0);
+        std::vector<int32_t> off((size_t)N + 1, 0), idx((size_t)n);
+        for (int64_t r = 0; r < n; ++r) w[(size_t)(a.keep ? a.keep[r] : r)] += 1;
+        for (int64_t s = 0; s < N; ++s) off[(size_t)s + 1] = off[(size_t)s] + w[(size_t)s];
+        {
+            std::vector<int32_t> fill(off.begin(), off.end() - 1);
+            for (int64_t r = 0; r < n; ++r) idx[(size_t)fill[(size_t)(a.keep ? a.keep[r] : r)]++] = (int32_t)r;
+        }
+        DevBuf dw(w.size() * 4), doff(off.size() * 4), didx(std::max<size_t>(1, idx.size()) * 4);
+        CS_HIP(hipMemcpyAsync(dw.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, st));
+        CS_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
+        if (!idx.empty()) CS_HIP(hipMemcpyAsync(didx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, st));
+        DevBuf counts((size_t)M * 4 * sizeof(int64_t));
+        weighted_counts(b, dw.as<int32_t>(), counts.as<int64_t>(), st);
+        // R into device (compact n x nr) then scatter to the full sample space
+        DevBuf dR;
+        const double* Rd = a.R;
+        int64_t ldr = a.ldr;
+        if (!a.device_R) {
+            dR.alloc((size_t)(n * a.nr) * sizeof(double));
+            CS_HIP(hipMemcpy2DAsync(dR.p, (size_t)n * sizeof(double), a.R, (size_t)a.ldr * sizeof(double),
+                                    (size_t)n * sizeof(double), (size_t)a.nr, hipMemcpyHostToDevice, st));
+            Rd = dR.as<double>();
+            ldr = n;
+        }
+        DevBuf Rt((size_t)(b->Npad * a.nr) * sizeof(double));
+        k_scatter<<<grid_for(b->Npad * a.nr), 256, 0, st>>>(Rd, ldr, a.nr, doff.as<int32_t>(),
+                                                            didx.as<int32_t>(), N, b->Npad,
+                                                            Rt.as<double>());
+        CS_HIP(hipGetLastError());
+        const Tiling tl = tiling(a.nr * 1, L);
+        DevBuf mx, scl, B;
+        quantize_cols(Rt.as<double>(), b->Npad, a.nr, L, tl, st, mx, scl, B);
+        // model to device
+        ModelDev md{};
+        DevBuf dqw, dC;
+        if (model) {
+            md.n_score = model->n_score;
+            md.alpha = model->alpha;
+            md.quad_begin = model->quad_begin;
+            md.quad_end = model->quad_end;
+            md.cov_begin = model->cov_begin;
+            md.cov_end = model->cov_end;
+            if (md.n_score < 1 || md.n_score > a.nr || md.quad_begin < 0 || md.quad_end > a.nr ||
+                md.quad_begin > md.quad_end || md.cov_begin < 0 || md.cov_end > a.nr ||
+                md.cov_begin > md.cov_end || md.cov_end - md.cov_begin > 32)
+                throw InvalidArgument("scan: bad model column ranges");
+            const int64_t nq = md.quad_end - md.quad_begin, nc = md.cov_end - md.cov_begin;
+            dqw.alloc((size_t)std::max<int64_t>(1, nq) * 8);
+            dC.alloc((size_t)std::max<int64_t>(1, nc * nc) * 8);
+            if (nq) CS_HIP(hipMemcpyAsync(dqw.p, model->quad_w, (size_t)nq * 8, hipMemcpyHostToDevice, st));
+            if (nc) CS_HIP(hipMemcpyAsync(dC.p, model->cov_C, (size_t)(nc * nc) * 8, hipMemcpyHostToDevice, st));
+            md.quad_w = dqw.as<double>();
+            md.cov_C = dC.as<double>();
+        }
+        // outputs on device
+        DevBuf dsn((size_t)ns * 4), dfreq((size_t)ns * 8), dnobs((size_t)ns * 8), dpass((size_t)ns * 4);
+        CS_HIP(hipMemcpyAsync(dsn.p, snps, (size_t)ns * 4, hipMemcpyHostToDevice, st));
+        DevBuf ddot, dgtg, dbeta, dse, dp, dl;
+        if (o.dot) ddot.alloc((size_t)(ns * a.nr) * 8);
+        if (o.gtg) dgtg.alloc((size_t)ns * 8);
+        const int64_t nsc = model ? model->n_score : 0;
+        if (model) {
+            dbeta.alloc((size_t)(ns * nsc) * 8);
+            dse.alloc((size_t)(ns * nsc) * 8);
+            dp.alloc((size_t)(ns * nsc) * 8);
+            dl.alloc((size_t)(ns * nsc) * 8);
+        }
+        // SNP-row chunks of the Z^T R product
+        const int64_t row_lo0 = lo / kPad * kPad;
+        const int64_t row_hi = round_up(hi + 1, kPad);
+        int64_t chunk = std::max<int64_t>(kPad, ((int64_t)(256ll << 20) / (8ll * tl.NC)) / kPad * kPad);
+        DevBuf PQ((size_t)std::min<int64_t>(chunk, row_hi - row_lo0) * 2 * tl.NC * sizeof(int32_t));
+        for (int64_t r0 = row_lo0; r0 < row_hi; r0 += chunk) {
+            const int64_t rows = std::min<int64_t>(chunk, row_hi - r0);
+            dim3 grid((unsigned)(rows / 256), 1, (unsigned)(tl.NT / tl.NTB));
+            launch_mfma<false>(tl.NTB, grid, st, b->snp.as<uint8_t>() + r0 * b->RS, b->RS, rows,
+                               b->Npad / 256, b->Npad / 256, tl.NT, B.as<int8_t>(), B.as<int8_t>(),
+                               PQ.as<int32_t>());
+            k_scan_epilogue<<<grid_for(ns), 256, 0, st>>>(
+                dsn.as<int32_t>(), ns, r0, rows, counts.as<int64_t>(), PQ.as<int32_t>(), tl.NC, a.nr, L,
+                scl.as<double>(), a.min_maf, dfreq.as<double>(), dnobs.as<int64_t>(), dpass.as<int32_t>(),
+                o.dot ? ddot.as<double>() : nullptr, o.gtg ? dgtg.as<double>() : nullptr, md,
+                model ? dbeta.as<double>() : nullptr, model ? dse.as<double>() : nullptr,
+                model ? dp.as<double>() : nullptr, model ? dl.as<double>() : nullptr);
+            CS_HIP(hipGetLastError());
+        }
+        auto d2h = [&](void* dst, const DevBuf& src, size_t bytes) {
+            if (dst) CS_HIP(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, st));
+        };
+        d2h(o.freq, dfreq, (size_t)ns * 8);
+        d2h(o.n_obs, dnobs, (size_t)ns * 8);
+        d2h(o.pass, dpass, (size_t)ns * 4);
+        if (o.dot) d2h(o.dot, ddot, (size_t)(ns * a.nr) * 8);
+        if (o.gtg) d2h(o.gtg, dgtg, (size_t)ns * 8);
+        if (model) {
+            d2h(o.beta, dbeta, (size_t)(ns * nsc) * 8);
+            d2h(o.se, dse, (size_t)(ns * nsc) * 8);
+            d2h(o.p, dp, (size_t)(ns * nsc) * 8);
+            d2h(o.log10p, dl, (size_t)(ns * nsc) * 8);
+        }
+        CS_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+        throw;
+    }
+    CS_HIP(hipStreamDestroy(st));
+}
+
+}  // namespace csamd
+
+using namespace csamd;
+
+extern "C" {
+
+int cs_scan_sums(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* R, int64_t ldr,
+                 int64_t nr, double min_maf, const int32_t* snps, int64_t n_snps, int prec,
+                 double* freq, int64_t* n_obs, int32_t* pass, double* dot, double* gtg) {
+    return guarded([&] {
+        if (!bed || !R) throw InvalidArgument("cs_scan_sums: null argument");
+        ScanArgs a;
+        a.keep = (keep && n_keep > 0) ? keep : nullptr;
+        a.n = a.keep ? n_keep : bed->N;
+        a.R = R; a.ldr = ldr; a.nr = nr; a.min_maf = min_maf;
+        a.snps = (snps && n_snps > 0) ? snps : nullptr;
+        a.n_snps = a.snps ? n_snps : bed->M;
+        a.prec = prec;
+        ScanOut o;
+        o.freq = freq; o.n_obs = n_obs; o.pass = pass; o.dot = dot; o.gtg = gtg;
+        run_scan(bed, a, nullptr, o);
+    });
+}
+
+int cs_lmm_scan(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* R, int64_t ldr,
+                int64_t nr, double min_maf, const int32_t* snps, int64_t n_snps, int prec,
+                const cs_scan_model* model, double* freq, int64_t* n_obs, int32_t* pass,
+                double* beta, double* se, double* p, double* log10p) {
+    return guarded([&] {
+        if (!bed || !R || !model) throw InvalidArgument("cs_lmm_scan: null argument");
+        ScanArgs a;
+        a.keep = (keep && n_keep > 0) ? keep : nullptr;
+        a.n = a.keep ? n_keep : bed->N;
+        a.R = R; a.ldr = ldr; a.nr = nr; a.min_maf = min_maf;
+        a.snps = (snps && n_snps > 0) ? snps : nullptr;
+        a.n_snps = a.snps ? n_snps : bed->M;
+        a.prec = prec;
+        ScanOut o;
+        o.freq = freq; o.n_obs = n_obs; o.pass = pass;
+        o.beta = beta; o.se = se; o.p = p; o.log10p = log10p;
+        run_scan(bed, a, model, o);
+    });
+}
+
+}  // extern "C"

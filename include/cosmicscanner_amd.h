@@ -40,6 +40,14 @@ const char* cs_last_error(void);
 /* Library version as major*10000 + minor*100 + patch. */
 int cs_version(void);
 
+/* ---- Device buffers (for device-resident right-hand sides and solver vectors; no reference
+ * counterpart: the reference keeps everything in host memory). */
+int cs_dev_alloc(int64_t bytes, void** dptr);
+int cs_dev_free(void* dptr);
+int cs_dev_upload(void* dst, const void* src, int64_t bytes);
+int cs_dev_download(void* dst, const void* src, int64_t bytes);
+int cs_dev_synchronize(void);
+
 /* ---- Genotypes (replaces PlinkReader::load/openBed, libCosmicCore/src/plink_reader.cpp:28-92,
  * with the packed matrix resident in HBM).  `body` is the .bed content after the 3 magic bytes:
  * n_snps rows of ceil(n_samples/4) bytes, SNP-major.  Padding slots of each row's last byte are
@@ -97,6 +105,43 @@ int cs_kinship_diag(cs_kinship* op, double* diag);
 /* Time of the last apply's kernels in ms per phase (0: quantize+scatter, 1: Z^T product,
  * 2: middle epilogue, 3: Z product, 4: final epilogue), from HIP events on the operator's stream. */
 int cs_kinship_last_timing(const cs_kinship* op, double* ms5);
+
+/* ---- Kernel (c): per-SNP association scan.
+ * Marker preparation as genetics::prepareAssociationMarker (src/genetics/genotype_encoding.h:33-81)
+ * on the keep list (lmm_engine.cpp:1500-1514): n_obs, A1 frequency, MAF pass (n_obs >= 3 and
+ * min(f,1-f) >= min_maf), mean-impute and centre -> g_j.  R is n x nr column-major (over the keep
+ * list).  snps: the SNPs to scan in output order (NULL/0 = all).
+ *
+ * cs_scan_sums: dot[i*nr + c] = g_{snps[i]}^T R_c and gtg[i] = g^T g (any output may be NULL). */
+int cs_scan_sums(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* R, int64_t ldr,
+                 int64_t nr, double min_maf, const int32_t* snps, int64_t n_snps, int prec,
+                 double* freq, int64_t* n_obs, int32_t* pass, double* dot, double* gtg);
+
+/* Variance model of the fused scan: with d_c = g^T R_c,
+ *   score_t = d_t                                  (t < n_score: one column per trait)
+ *   var     = alpha g^T g + sum_{c in [quad_begin, quad_end)} quad_w[c - quad_begin] d_c^2
+ *             - q^T cov_C q,   q = d[cov_begin:cov_end]  (at most 32 covariate columns)
+ * This covers the rotated dense form (LmmEngine::scan, lmm_engine.cpp:1430-1449: R = [P y,
+ * W_proj diag(V^-1/2)], alpha 0, unit weights), the low-rank Woodbury form (scanLowRank,
+ * lmm_engine.cpp:1297-1307: alpha = base, weights d_l - base on U, covariate block V^-1 X with
+ * cov_C = (X^T V^-1 X)^-1) and the PCG-projected form (alpha = calibration ratio). */
+typedef struct cs_scan_model {
+    int64_t n_score;
+    double alpha;
+    int64_t quad_begin, quad_end;
+    const double* quad_w;
+    int64_t cov_begin, cov_end;
+    const double* cov_C;
+} cs_scan_model;
+
+/* Fused scan: per SNP and trait beta = score/var, se = var^-1/2, p = chi^2_1 tail of
+ * score^2/var (gamma_q of lmm_engine.cpp:22-64), log10p = -log10 p (300 when p == 0).  SNPs
+ * failing the MAF filter or with var <= 1e-12 get NaN (reference: not written). Outputs
+ * n_snps x n_score row-major; freq / n_obs / pass per SNP. */
+int cs_lmm_scan(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* R, int64_t ldr,
+                int64_t nr, double min_maf, const int32_t* snps, int64_t n_snps, int prec,
+                const cs_scan_model* model, double* freq, int64_t* n_obs, int32_t* pass,
+                double* beta, double* se, double* p, double* log10p);
 
 #ifdef __cplusplus
 }

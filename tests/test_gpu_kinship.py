@@ -118,3 +118,16 @@ def test_kinship_precision_modes_larger(csa, mode, tol):
         ref = oracle.kinship_apply(body, N, means, inv, kp, vp, X)
         assert colrel(op.apply(X), ref) < tol
         op.close()
+
+
+def test_kinship_sketch_width_266(csa):
+    # The reference's randomized low-rank path applies K to n x (256 + 10) sketches
+    # (lmm_engine.cpp:915-946); widths above 256 columns exercise the per-column scale grid.
+    N, M = 700, 2300
+    body = oracle.gen_hwe_bed(31, N, M, 0.01, 0.5, 0.01)
+    reader = csa.PlinkReader.from_body(body, N, M)
+    means, inv = oracle.stats(body, N, M)
+    op = csa.PlinkKinshipOperator(reader)
+    X = np.random.default_rng(2).standard_normal((N, 266))
+    ref = oracle.kinship_apply(body, N, means, inv, None, None, X)
+    assert colrel(op.apply(X), ref) < 2e-6
