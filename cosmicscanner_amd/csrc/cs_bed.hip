@@ -1,15 +1,18 @@
 // Resident packed genotypes: upload / synthetic generation, padded SNP-major layout, the
 // sample-major transpose, and kernel (a) -- decode + per-SNP statistics.
 //
-// HBM layout.  SNP-major: row j (SNP) holds Npad/4 bytes, sample i at byte i>>2, bits 2(i&3)
-// (the .bed order, plink_reader.cpp:84); rows are padded to Npad = roundup(N, 256) samples and
-// the matrix to Mpad = roundup(M, 256) rows.  Sample-major: row i (sample) holds Mpad/4 bytes,
-// SNP j at byte j>>2, bits 2(j&3).  Every padding slot holds code 01 (missing).
+// HBM layout.  Two resident copies in MFMA tiles (cs_geno_mfma.h): SNP-major (row j = SNP,
+// sample i at byte i>>2, bits 2(i&3) of the row -- the .bed order, plink_reader.cpp:84) and
+// sample-major (row i = sample, SNP j at byte j>>2, bits 2(j&3)).  Samples are padded to
+// Npad = roundup(N, 256) and SNPs to Mpad = roundup(M, 256); every padding slot holds code 01
+// (missing), so it contributes to no sum.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 
 #include "cs_common.h"
+#include "cs_geno_mfma.h"
+#include "cs_ops.h"
 
 namespace csamd {
 
@@ -18,10 +21,10 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
 // ------------------------------------------------------------------------------ relayout
-// Copy `rows` raw rows (Braw bytes each) into the padded SNP-major matrix; slots of samples
-// >= N in the last byte become code 01.  The rest of each padded row was pre-filled with 0x55.
-__global__ void k_relayout(const uint8_t* __restrict__ raw, int64_t rows, int64_t Braw, int64_t N,
-                           uint8_t* __restrict__ dst, int64_t RS) {
+// Copy `rows` raw rows (Braw bytes each, starting at SNP r0) into the tiled SNP-major matrix;
+// slots of samples >= N in the last byte become code 01.  Everything else was pre-filled 0x55.
+__global__ void k_relayout(const uint8_t* __restrict__ raw, int64_t r0, int64_t rows, int64_t Braw,
+                           int64_t N, uint8_t* __restrict__ dst, int64_t KB) {
     const int64_t total = rows * Braw;
     const int tail = (int)(N & 3);
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
@@ -32,19 +35,19 @@ __global__ void k_relayout(const uint8_t* __restrict__ raw, int64_t rows, int64_
             const uint8_t keep_mask = (uint8_t)((1u << (2 * tail)) - 1u);
             v = (uint8_t)((v & keep_mask) | (kMissingByte & ~keep_mask));
         }
-        dst[r * RS + b] = v;
+        dst[tiled_offset(r0 + r, b, KB)] = v;
     }
 }
 
-// Inverse for download: padded SNP-major -> raw rows, padding slots of the last byte zeroed.
-__global__ void k_unlayout(const uint8_t* __restrict__ src, int64_t RS, int64_t rows, int64_t Braw,
+// Inverse for download: tiled SNP-major -> raw rows, padding slots of the last byte zeroed.
+__global__ void k_unlayout(const uint8_t* __restrict__ src, int64_t KB, int64_t rows, int64_t Braw,
                            int64_t N, uint8_t* __restrict__ raw) {
     const int64_t total = rows * Braw;
     const int tail = (int)(N & 3);
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = e / Braw, b = e - r * Braw;
-        uint8_t v = src[r * RS + b];
+        uint8_t v = src[tiled_offset(r, b, KB)];
         if (tail && b == Braw - 1) v &= (uint8_t)((1u << (2 * tail)) - 1u);
         raw[e] = v;
     }
@@ -65,7 +68,7 @@ __device__ __forceinline__ uint64_t hash3(uint64_t seed, uint64_t j, uint64_t i)
 __device__ __forceinline__ double u01(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
 
 __global__ void k_synth(uint64_t seed, int64_t N, int64_t M, double pmin, double pmax,
-                        double miss, uint8_t* __restrict__ dst, int64_t RS) {
+                        double miss, uint8_t* __restrict__ dst, int64_t KB) {
     const int64_t Braw = (N + 3) / 4;
     const int64_t total = M * Braw;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
@@ -91,23 +94,29 @@ __global__ void k_synth(uint64_t seed, int64_t N, int64_t M, double pmin, double
             }
             byte |= (uint8_t)(code << (2 * s));
         }
-        dst[j * RS + b] = byte;
+        dst[tiled_offset(j, b, KB)] = byte;
     }
 }
 
 // ------------------------------------------------------------------------------ transpose
-// SNP-major (Mpad x RS) -> sample-major (Npad x CS), 256 x 256 genotype tiles through LDS.
-__global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ snp, int64_t RS,
-                                                   uint8_t* __restrict__ smp, int64_t CS) {
+// Tiled SNP-major (Mpad x Npad) -> tiled sample-major (Npad x Mpad), 256 x 256 genotype blocks
+// through LDS.  blockIdx.x: 256-sample block, blockIdx.y: 256-SNP block.
+__global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ snp, int64_t KBs,
+                                                   uint8_t* __restrict__ smp, int64_t KBm) {
     __shared__ uint32_t tile[256][17];  // 256 SNP rows x 64 bytes (+1 dword pad vs conflicts)
     const int t = threadIdx.x;
-    const int64_t j0 = (int64_t)blockIdx.y * 256;   // SNP tile
-    const int64_t s0 = (int64_t)blockIdx.x * 256;   // sample tile
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(snp + (j0 + t) * RS + s0 / 4);
+    const int64_t j0 = (int64_t)blockIdx.y * 256;   // SNPs
+    const int64_t s0 = (int64_t)blockIdx.x * 256;   // samples
 #pragma unroll
-    for (int d = 0; d < 16; ++d) tile[t][d] = src[d];
+    for (int q = 0; q < 4; ++q) {
+        const uint4 v = *reinterpret_cast<const uint4*>(snp + tiled_offset(j0 + t, s0 / 4 + 16 * q, KBs));
+        tile[t][4 * q] = v.x;
+        tile[t][4 * q + 1] = v.y;
+        tile[t][4 * q + 2] = v.z;
+        tile[t][4 * q + 3] = v.w;
+    }
     __syncthreads();
-    // thread t -> sample s0 + t; output bytes b = 0..63 cover SNPs j0 + 4b .. 4b+3
+    // thread t -> sample s0 + t; output dword q covers SNPs j0 + 16q .. 16q+15
     const int dw = t >> 4, sh = (t & 15) * 2;   // sample t lives in dword t/16, bits 2(t%16)
     uint32_t out[16];
 #pragma unroll
@@ -120,49 +129,60 @@ __global__ __launch_bounds__(256) void k_transpose(const uint8_t* __restrict__ s
         }
         out[q] = word;
     }
-    uint4* dst = reinterpret_cast<uint4*>(smp + (s0 + t) * CS + j0 / 4);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) dst[q] = make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+    for (int q = 0; q < 4; ++q)
+        *reinterpret_cast<uint4*>(smp + tiled_offset(s0 + t, j0 / 4 + 16 * q, KBm)) =
+            make_uint4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
 }
 
 // ------------------------------------------------------------------------------ kernel (a)
-// All-sample statistics, one wave per SNP: popcounts of the three observed codes, then the
-// reference's FP64 formulae (plink_reader.cpp:205-234) -- correctly rounded IEEE division and
-// square root, no contraction, so the values are bit-identical to the CPU restatement.
-__global__ __launch_bounds__(256) void k_stats(const uint8_t* __restrict__ snp, int64_t RS,
+// All-sample statistics.  One wave per 64-SNP row tile streams the tile sequence (the access
+// pattern of the MFMA kernels); each lane counts the three observed codes of rows 16m + r over
+// its 16-byte chunk g, the four chunk lanes of a row are reduced by shuffles, and the
+// reference's FP64 formulae (plink_reader.cpp:205-234) run with correctly rounded IEEE division
+// and square root and no contraction, so values are bit-identical to the CPU restatement.
+__global__ __launch_bounds__(256) void k_stats(const uint8_t* __restrict__ snp, int64_t KB,
                                                int64_t M, int64_t N, int64_t* __restrict__ counts,
                                                double* __restrict__ means,
                                                double* __restrict__ inv_stds) {
-    const int lane = threadIdx.x & 63;
-    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (j >= M) return;
-    const uint4* row = reinterpret_cast<const uint4*>(snp + j * RS);
-    const int64_t nvec = RS / 16;
-    uint32_t c00 = 0, c10 = 0, c11 = 0;
-    for (int64_t v = lane; v < nvec; v += 64) {
-        const uint4 w4 = row[v];
-        const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tr * 64 >= M) return;
+    const uint8_t* base = snp + ((tr * KB) << 12) + lane * 16;
+    uint32_t c00[4] = {0, 0, 0, 0}, c10[4] = {0, 0, 0, 0}, c11[4] = {0, 0, 0, 0};
+    for (int64_t kb = 0; kb < KB; ++kb) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const uint32_t L = ws[q] & 0x55555555u, H = (ws[q] >> 1) & 0x55555555u;
-            c00 += __popc(~(L | H) & 0x55555555u);
-            c10 += __popc(H & ~L & 0x55555555u);
-            c11 += __popc(H & L);
+        for (int m = 0; m < 4; ++m) {
+            const uint4 w4 = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
+            const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t L = ws[q] & 0x55555555u, H = (ws[q] >> 1) & 0x55555555u;
+                c00[m] += __popc(~(L | H) & 0x55555555u);
+                c10[m] += __popc(H & ~L & 0x55555555u);
+                c11[m] += __popc(H & L);
+            }
         }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        c00 += __shfl_xor(c00, off);
-        c10 += __shfl_xor(c10, off);
-        c11 += __shfl_xor(c11, off);
-    }
-    if (lane == 0) {
-        const int64_t obs = (int64_t)c00 + c10 + c11;
-        counts[j * 4 + 0] = c00;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        for (int off = 16; off < 64; off <<= 1) {
+            c00[m] += __shfl_xor(c00[m], off);
+            c10[m] += __shfl_xor(c10[m], off);
+            c11[m] += __shfl_xor(c11[m], off);
+        }
+    if (g != 0) return;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int64_t j = tr * 64 + 16 * m + r;
+        if (j >= M) continue;
+        const int64_t obs = (int64_t)c00[m] + c10[m] + c11[m];
+        counts[j * 4 + 0] = c00[m];
         counts[j * 4 + 1] = N - obs;
-        counts[j * 4 + 2] = c10;
-        counts[j * 4 + 3] = c11;
+        counts[j * 4 + 2] = c10[m];
+        counts[j * 4 + 3] = c11[m];
         if (obs > 0) {
-            const double sum = 2.0 * (double)c00 + (double)c10;
+            const double sum = 2.0 * (double)c00[m] + (double)c10[m];
             const double mean = sum / (double)obs;
             const double p = mean / 2.0;
             const double var = 2.0 * p * (1.0 - p);
@@ -175,42 +195,54 @@ __global__ __launch_bounds__(256) void k_stats(const uint8_t* __restrict__ snp, 
     }
 }
 
-// Weighted class counts, one wave per SNP; weights are int32 over Npad samples (0 on padding).
-__global__ __launch_bounds__(256) void k_wcounts(const uint8_t* __restrict__ snp, int64_t RS,
+// Weighted class counts (keep multiplicities), tiled SNP-major; weights int32 over Npad samples.
+__global__ __launch_bounds__(256) void k_wcounts(const uint8_t* __restrict__ snp, int64_t KB,
                                                  int64_t M, const int32_t* __restrict__ w,
                                                  int64_t* __restrict__ counts) {
-    const int lane = threadIdx.x & 63;
-    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (j >= M) return;
-    const uint32_t* row = reinterpret_cast<const uint32_t*>(snp + j * RS);
-    const int64_t ndw = RS / 4;
-    long long c[4] = {0, 0, 0, 0};
-    for (int64_t d = lane; d < ndw; d += 64) {
-        const uint32_t word = row[d];
-        const int4* wv = reinterpret_cast<const int4*>(w + d * 16);
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tr * 64 >= M) return;
+    const uint8_t* base = snp + ((tr * KB) << 12) + lane * 16;
+    long long c[4][4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int4 ww = wv[q];
-            const int wq[4] = {ww.x, ww.y, ww.z, ww.w};
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const uint32_t code = (word >> (2 * (4 * q + s))) & 3u;
-                c[code] += wq[s];
+        for (int t = 0; t < 4; ++t) c[m][t] = 0;
+    for (int64_t kb = 0; kb < KB; ++kb) {
+        uint4 w4[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) w4[m] = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
+        const int4* wp = reinterpret_cast<const int4*>(w + kb * 256 + g * 64);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {       // dword q = samples 16q .. 16q+15 of the chunk
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {   // byte b of the dword = samples 16q + 4b .. +3
+                const int4 ww = wp[q * 4 + b];
+                const int wq[4] = {ww.x, ww.y, ww.z, ww.w};
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const uint32_t word = q == 0 ? w4[m].x : q == 1 ? w4[m].y : q == 2 ? w4[m].z : w4[m].w;
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) c[m][(word >> (8 * b + 2 * s)) & 3u] += wq[s];
+                }
             }
         }
     }
-    for (int t = 0; t < 4; ++t)
-        for (int off = 32; off > 0; off >>= 1) c[t] += __shfl_xor(c[t], off);
-    if (lane == 0)
-        for (int t = 0; t < 4; ++t) counts[j * 4 + t] = c[t];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            for (int off = 16; off < 64; off <<= 1) c[m][t] += __shfl_xor(c[m][t], off);
+    if (g != 0) return;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int64_t j = tr * 64 + 16 * m + r;
+        if (j >= M) continue;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) counts[j * 4 + t] = c[m][t];
+    }
 }
 
-static int grid_for(int64_t total, int block = 256) {
-    int64_t g = (total + block - 1) / block;
-    if (g > 65536) g = 65536;
-    if (g < 1) g = 1;
-    return (int)g;
-}
 
 static void init_layout(cs_bed* b, int64_t N, int64_t M) {
     if (N <= 0 || M <= 0) throw InvalidArgument("cs_bed: n_samples and n_snps must be positive");
@@ -220,10 +252,10 @@ static void init_layout(cs_bed* b, int64_t N, int64_t M) {
     b->Braw = (N + 3) / 4;
     b->Npad = round_up(N, kPad);
     b->Mpad = round_up(M, kPad);
-    b->RS = b->Npad / 4;
-    b->CS = b->Mpad / 4;
+    b->KBs = b->Npad / 256;
+    b->KBm = b->Mpad / 256;
     CS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    b->snp.alloc((size_t)(b->Mpad * b->RS));
+    b->snp.alloc((size_t)(b->Mpad * b->Npad / 4));
     CS_HIP(hipMemsetAsync(b->snp.p, kMissingByte, b->snp.bytes, b->stream));
 }
 
@@ -242,7 +274,7 @@ static void upload(cs_bed* b, const uint8_t* body) {
             std::memcpy(pinned, body + r0 * b->Braw, nb);
             CS_HIP(hipMemcpyAsync(stage.p, pinned, nb, hipMemcpyHostToDevice, b->stream));
             k_relayout<<<grid_for(rows * b->Braw), 256, 0, b->stream>>>(
-                stage.as<uint8_t>(), rows, b->Braw, b->N, b->snp.as<uint8_t>() + r0 * b->RS, b->RS);
+                stage.as<uint8_t>(), r0, rows, b->Braw, b->N, b->snp.as<uint8_t>(), b->KBs);
             CS_HIP(hipGetLastError());
         }
         CS_HIP(hipStreamSynchronize(b->stream));
@@ -255,9 +287,9 @@ static void upload(cs_bed* b, const uint8_t* body) {
 
 void ensure_sample_major(cs_bed* b) {
     if (b->smp.p) return;
-    b->smp.alloc((size_t)(b->Npad * b->CS));
+    b->smp.alloc((size_t)(b->Npad * b->Mpad / 4));
     dim3 grid((unsigned)(b->Npad / 256), (unsigned)(b->Mpad / 256));
-    k_transpose<<<grid, 256, 0, b->stream>>>(b->snp.as<uint8_t>(), b->RS, b->smp.as<uint8_t>(), b->CS);
+    k_transpose<<<grid, 256, 0, b->stream>>>(b->snp.as<uint8_t>(), b->KBs, b->smp.as<uint8_t>(), b->KBm);
     CS_HIP(hipGetLastError());
     CS_HIP(hipStreamSynchronize(b->stream));
 }
@@ -267,8 +299,8 @@ void ensure_stats(cs_bed* b) {
     b->counts.alloc((size_t)b->M * 4 * sizeof(int64_t));
     b->means.alloc((size_t)b->M * sizeof(double));
     b->inv_stds.alloc((size_t)b->M * sizeof(double));
-    k_stats<<<(unsigned)((b->M + 3) / 4), 256, 0, b->stream>>>(
-        b->snp.as<uint8_t>(), b->RS, b->M, b->N, b->counts.as<int64_t>(), b->means.as<double>(),
+    k_stats<<<(unsigned)(b->Mpad / 256), 256, 0, b->stream>>>(
+        b->snp.as<uint8_t>(), b->KBs, b->M, b->N, b->counts.as<int64_t>(), b->means.as<double>(),
         b->inv_stds.as<double>());
     CS_HIP(hipGetLastError());
     CS_HIP(hipStreamSynchronize(b->stream));
@@ -276,8 +308,8 @@ void ensure_stats(cs_bed* b) {
 }
 
 void weighted_counts(cs_bed* b, const int32_t* d_w_npad, int64_t* d_counts, hipStream_t st) {
-    k_wcounts<<<(unsigned)((b->M + 3) / 4), 256, 0, st>>>(b->snp.as<uint8_t>(), b->RS, b->M,
-                                                         d_w_npad, d_counts);
+    k_wcounts<<<(unsigned)(b->Mpad / 256), 256, 0, st>>>(b->snp.as<uint8_t>(), b->KBs, b->M,
+                                                        d_w_npad, d_counts);
     CS_HIP(hipGetLastError());
 }
 
@@ -361,7 +393,7 @@ int cs_bed_create_synthetic(uint64_t seed, int64_t n_samples, int64_t n_snps, do
             init_layout(b, n_samples, n_snps);
             const int64_t total = n_snps * ((n_samples + 3) / 4);
             k_synth<<<grid_for(total), 256, 0, b->stream>>>(seed, n_samples, n_snps, pmin, pmax, miss,
-                                                           b->snp.as<uint8_t>(), b->RS);
+                                                           b->snp.as<uint8_t>(), b->KBs);
             CS_HIP(hipGetLastError());
             CS_HIP(hipStreamSynchronize(b->stream));
         } catch (...) {
@@ -396,7 +428,7 @@ int cs_bed_download(cs_bed* bed, uint8_t* body) {
         if (!bed || !body) throw InvalidArgument("cs_bed_download: null argument");
         DevBuf raw((size_t)(bed->M * bed->Braw));
         k_unlayout<<<grid_for(bed->M * bed->Braw), 256, 0, bed->stream>>>(
-            bed->snp.as<uint8_t>(), bed->RS, bed->M, bed->Braw, bed->N, raw.as<uint8_t>());
+            bed->snp.as<uint8_t>(), bed->KBs, bed->M, bed->Braw, bed->N, raw.as<uint8_t>());
         CS_HIP(hipGetLastError());
         CS_HIP(hipMemcpyAsync(body, raw.p, (size_t)(bed->M * bed->Braw), hipMemcpyDeviceToHost,
                               bed->stream));

@@ -98,10 +98,10 @@ struct cs_bed {
     int64_t N = 0, M = 0;      // samples (.fam rows), SNPs (.bim rows)
     int64_t Braw = 0;          // ceil(N/4): bytes per SNP in the file
     int64_t Npad = 0, Mpad = 0;
-    int64_t RS = 0;            // SNP-major row stride in bytes  (= Npad/4)
-    int64_t CS = 0;            // sample-major row stride in bytes (= Mpad/4)
-    csamd::DevBuf snp;         // Mpad x RS, SNP-major
-    csamd::DevBuf smp;         // Npad x CS, sample-major (built on demand)
+    int64_t KBs = 0;           // SNP-major copy: 256-sample blocks per row tile (= Npad/256)
+    int64_t KBm = 0;           // sample-major copy: 256-SNP blocks per row tile (= Mpad/256)
+    csamd::DevBuf snp;         // Mpad x Npad genotypes, SNP-major, MFMA tiles (cs_geno_mfma.h)
+    csamd::DevBuf smp;         // Npad x Mpad genotypes, sample-major, MFMA tiles (on demand)
     csamd::DevBuf counts;      // M x 4 int64: all-sample class counts per code
     csamd::DevBuf means;       // M double  (PlinkReader::getMeans)
     csamd::DevBuf inv_stds;    // M double  (PlinkReader::getInvStds)

@@ -1,17 +1,24 @@
-// Kernel (b) core: the packed-genotype x int8-digit product on CDNA4 MFMA (16x16x64 i8).
+// Kernel (b) core: packed-genotype x int8-digit products on CDNA4 MFMA (v_mfma_i32_16x16x64_i8).
 //
-// out[split][row][type][col] = sum_k  op_type(code(row, k)) * B_type[k][col]
-// with op_0 = x~ (dosage 2/1/0, 0 for missing) and op_1 = o (1 if observed, else 0), over a packed
-// 2-bit matrix whose rows are contiguous along k (SNP-major: rows = SNPs, k = samples;
-// sample-major: rows = samples, k = SNPs).  All arithmetic is exact int32.
+// For a packed 2-bit matrix whose rows are contiguous along k (SNP-major: rows = SNPs, k =
+// samples; sample-major: rows = samples, k = SNPs) the kernel forms, exactly in int32,
+//     X[row][c] = sum_k x~(code(row,k)) Bx[k][c],   O[row][c] = sum_k o(code(row,k)) Bo[k][c]
+// with x~ = dosage of A1 (2/1/0, 0 for missing) and o = 1 if observed, for the L limbs of each
+// right-hand-side column, then combines the limbs in int64 and applies a fused epilogue.
 //
-// Operand layouts.  One wave owns 64 rows (4 M tiles of 16) and NTB N tiles of 16 columns.  Each
-// lane (group g = lane/16, r = lane%16) loads 16 packed bytes = 64 k-elements of its row per 256-k
-// block, and uses dword t of them at MFMA step t.  Inside a step the 16 operand bytes are ordered
-// byte(4s + q) <-> k = 64g + 16t + 4q + s, which makes the 2-bit -> int8 unpack a shift + mask
-// (d_s = (w >> 2s) & 0x03030303) plus one v_perm byte lookup per 4 genotypes.  The B digits are
-// stored pre-permuted the same way: [kblock][t][ntile][lane][16 bytes] (bperm_offset below), so
-// the same k pairs up in A and B whatever the hardware's k order inside an operand is.
+// HBM layout (both resident copies): "MFMA tiles".  Rows are grouped in tiles of 64 and k in
+// blocks of 256 (64 bytes); tile (rt, kb) is one contiguous 4 KB piece at (rt*KB + kb)*4096, and
+// inside it the byte of (row = 64 rt + 16 m + r, kbyte = 64 kb + 16 g + b) sits at
+// (m*64 + g*16 + r)*16 + b.  A wave owns one row tile and streams its tiles sequentially, each
+// lane loading 16 contiguous bytes per m-subtile, so every load instruction reads 1 KB of
+// contiguous memory.
+//
+// Operand order.  Lane (g = lane/16, r = lane%16) of m-subtile m supplies 16 packed bytes = 64
+// k-elements of row 16m + r and uses dword t of them at MFMA step t; inside a step the operand
+// bytes are ordered byte(4s + q) <-> k = 256 kb + 64 g + 16 t + 4 q + s, so unpacking is a shift
+// and mask (d_s = (w >> 2s) & 0x03030303) plus one v_perm byte lookup per 4 genotypes.  B digits
+// are stored pre-permuted the same way ([kb][t][tile][lane][16 bytes], bperm_offset), so A and B
+// pair the same k whatever the hardware's k order inside an operand.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -24,136 +31,184 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr uint32_t kLutX = 0x00010002u;
 constexpr uint32_t kLutO = 0x01010001u;
 
-__host__ __device__ inline int64_t bperm_offset(int64_t kidx, int col, int NT) {
+__host__ __device__ inline int64_t tiled_offset(int64_t row, int64_t kbyte, int64_t KB) {
+    const int64_t rt = row >> 6;
+    const int rr = (int)(row & 63), m = rr >> 4, r = rr & 15;
+    const int64_t kb = kbyte >> 6;
+    const int kk = (int)(kbyte & 63), g = kk >> 4, b = kk & 15;
+    return ((rt * KB + kb) << 12) + ((m * 64 + g * 16 + r) << 4) + b;
+}
+
+// B operand byte of k-element kidx, tile n (of NT), column c (0..15) within the tile.
+__host__ __device__ inline int64_t bperm_offset(int64_t kidx, int n, int c, int NT) {
     const int64_t kb = kidx >> 8;
-    const int r = (int)(kidx & 255);
-    const int g = r >> 6, t = (r >> 4) & 3, q = (r >> 2) & 3, s = r & 3;
-    const int n = col >> 4, c = col & 15;
+    const int rr = (int)(kidx & 255);
+    const int g = rr >> 6, t = (rr >> 4) & 3, q = (rr >> 2) & 3, s = rr & 3;
     return ((((kb * 4 + t) * NT + n) * 64 + g * 16 + c) << 4) + 4 * s + q;
 }
 
-template <int NTB, bool TWO_B>
-__global__ __launch_bounds__(256) void k_geno_mfma(const uint8_t* __restrict__ A, int64_t lda,
-                                                   int64_t rows_pad, int64_t kb_per_split,
-                                                   int64_t kb_total, int NT,
-                                                   const int8_t* __restrict__ Bx,
-                                                   const int8_t* __restrict__ Bo,
-                                                   int32_t* __restrict__ out) {
-    constexpr int MT = 4;
-    constexpr int NB = TWO_B ? 2 : 1;
-    constexpr int TILE = 4 * NTB * 1024 * NB;   // B bytes per 256-k block (all steps, tiles)
-    constexpr int PER_T = TILE / 4096;          // uint4 per thread to stage one tile
-    __shared__ __attribute__((aligned(16))) int8_t sB[2][TILE];
+enum { EPI_PQ = 0, EPI_KV_GT = 1, EPI_KV_G = 2 };
 
+struct GenoArgs {
+    const uint8_t* A;
+    int64_t KB;                // k blocks per row tile
+    int64_t rows;              // real rows (epilogue guard)
+    int64_t kb_per_split, kb_total;
+    int G;                     // groups of 16 right-hand sides (= gridDim.z)
+    const int8_t* Bx;
+    const int8_t* Bo;
+    int64_t k;                 // real number of right-hand sides
+    long long* pq;             // EPI_PQ: [row][2][16 G]
+    const double* scale;       // EPI_KV_GT: per-column scale of the quantised V
+    const double* means;
+    const double* inv_stds;
+    const int32_t* mult;       // variant multiplicities
+    double inv_M;
+    double* W1;                // EPI_KV_GT: [row][k] forward operands w and mu * w
+    double* W2;
+    unsigned long long* mx;    // EPI_KV_GT: per-column max(|w|, |mu w|) as double bits
+    long long* yint;           // EPI_KV_G: [row][k] int64, accumulated atomically over splits
+};
+
+template <int L, bool TWO_B, int EPI>
+__device__ __forceinline__ void geno_body(const GenoArgs& a) {
+    constexpr int MT = 4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
-    const int64_t row0 = ((int64_t)blockIdx.x * 4 + wave) * (16 * MT);
-    const int split = blockIdx.y;
-    const int nt0 = blockIdx.z * NTB;
-    const int64_t kb0 = (int64_t)split * kb_per_split;
-    int64_t kb1 = kb0 + kb_per_split;
-    if (kb1 > kb_total) kb1 = kb_total;
+    const int64_t tr = (int64_t)blockIdx.x * 4 + wave;
+    const int gz = blockIdx.z;
+    const int NT = a.G * L;
+    const int64_t kb0 = (int64_t)blockIdx.y * a.kb_per_split;
+    int64_t kb1 = kb0 + a.kb_per_split;
+    if (kb1 > a.kb_total) kb1 = a.kb_total;
 
-    v4i acc[2 * MT * NTB];
-#pragma unroll
-    for (int i = 0; i < 2 * MT * NTB; ++i) acc[i] = v4i{0, 0, 0, 0};
-
-    // Global B of one block: [t][ntile (all NT)][lane][16]; the staged tile keeps only this
-    // block's NTB tiles: sB[(t*NTB + n)*1024 + lane*16] (+ 4*NTB*1024 for the o-operand).
-    auto stage_load = [&](int64_t kb, uint4 (&reg)[PER_T]) {
-#pragma unroll
-        for (int i = 0; i < PER_T; ++i) {
-            const int e = (i * 256 + threadIdx.x) * 16;         // byte in the staged tile
-            const int which = e / (4 * NTB * 1024);              // 0: x operand, 1: o operand
-            const int rem = e - which * (4 * NTB * 1024);
-            const int t = rem / (NTB * 1024);
-            const int rem2 = rem - t * (NTB * 1024);
-            const int n = rem2 >> 10, l16 = rem2 & 1023;
-            const int8_t* src = (which ? Bo : Bx) + (((kb * 4 + t) * NT + nt0 + n) << 10) + l16;
-            reg[i] = *reinterpret_cast<const uint4*>(src);
-        }
-    };
-    auto stage_store = [&](int buf, const uint4 (&reg)[PER_T]) {
-#pragma unroll
-        for (int i = 0; i < PER_T; ++i)
-            *reinterpret_cast<uint4*>(&sB[buf][(i * 256 + threadIdx.x) * 16]) = reg[i];
-    };
-
-    const uint8_t* abase = A + (row0 + r) * lda + g * 16;
-    const int64_t astride = 16 * lda;
-    uint4 a[MT], an[MT];
-    uint4 breg[PER_T];
-    if (kb0 < kb1) {
-#pragma unroll
-        for (int m = 0; m < MT; ++m) a[m] = *reinterpret_cast<const uint4*>(abase + m * astride + kb0 * 64);
-        stage_load(kb0, breg);
-        stage_store(0, breg);
-    }
-    __syncthreads();
-    int buf = 0;
-    for (int64_t kb = kb0; kb < kb1; ++kb) {
-        const bool more = kb + 1 < kb1;
-        if (more) {
-#pragma unroll
-            for (int m = 0; m < MT; ++m)
-                an[m] = *reinterpret_cast<const uint4*>(abase + m * astride + (kb + 1) * 64);
-            stage_load(kb + 1, breg);
-        }
-        const int8_t* tile = sB[buf];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            v4i bx[NTB], bo[NTB];
-#pragma unroll
-            for (int n = 0; n < NTB; ++n) {
-                bx[n] = *reinterpret_cast<const v4i*>(tile + ((t * NTB + n) << 10) + lane * 16);
-                if (TWO_B)
-                    bo[n] = *reinterpret_cast<const v4i*>(tile + 4 * NTB * 1024 +
-                                                          ((t * NTB + n) << 10) + lane * 16);
-            }
-#pragma unroll
-            for (int m = 0; m < MT; ++m) {
-                const uint32_t w = t == 0 ? a[m].x : t == 1 ? a[m].y : t == 2 ? a[m].z : a[m].w;
-                const uint32_t d0 = w & 0x03030303u, d1 = (w >> 2) & 0x03030303u;
-                const uint32_t d2 = (w >> 4) & 0x03030303u, d3 = (w >> 6) & 0x03030303u;
-                const v4i X = v4i{(int)__builtin_amdgcn_perm(0u, kLutX, d0),
-                                  (int)__builtin_amdgcn_perm(0u, kLutX, d1),
-                                  (int)__builtin_amdgcn_perm(0u, kLutX, d2),
-                                  (int)__builtin_amdgcn_perm(0u, kLutX, d3)};
-                const v4i O = v4i{(int)__builtin_amdgcn_perm(0u, kLutO, d0),
-                                  (int)__builtin_amdgcn_perm(0u, kLutO, d1),
-                                  (int)__builtin_amdgcn_perm(0u, kLutO, d2),
-                                  (int)__builtin_amdgcn_perm(0u, kLutO, d3)};
-#pragma unroll
-                for (int n = 0; n < NTB; ++n) {
-                    v4i& cx = acc[(m * NTB + n) * 2];
-                    v4i& co = acc[(m * NTB + n) * 2 + 1];
-                    cx = __builtin_amdgcn_mfma_i32_16x16x64_i8(X, bx[n], cx, 0, 0, 0);
-                    co = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[n] : bx[n], co, 0, 0, 0);
-                }
-            }
-        }
-        if (more) {
-            stage_store(buf ^ 1, breg);
-#pragma unroll
-            for (int m = 0; m < MT; ++m) a[m] = an[m];
-        }
-        __syncthreads();
-        buf ^= 1;
-    }
-    // C layout of 16x16 MFMA: lane holds column (lane % 16), rows 4*(lane/16) + v, v = 0..3.
-    const int NC = NT * 16;
+    v4i accX[MT][L], accO[MT][L];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int n = 0; n < NTB; ++n)
+        for (int l = 0; l < L; ++l) {
+            accX[m][l] = v4i{0, 0, 0, 0};
+            accO[m][l] = v4i{0, 0, 0, 0};
+        }
+
+    if (kb0 < kb1) {
+        const uint8_t* abase = a.A + ((tr * a.KB) << 12) + lane * 16;
+        const int8_t* bxp = a.Bx + ((int64_t)(gz * L) << 10) + lane * 16;
+        const int8_t* bop = a.Bo + ((int64_t)(gz * L) << 10) + lane * 16;
+        const int64_t last_step = kb1 * 4 - 1;
+        uint4 av[MT], an[MT];
+        v4i bx[L], bo[L], bxn[L], bon[L];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t row = row0 + 16 * m + 4 * g + v;
-                const int col = (nt0 + n) * 16 + r;
-                int32_t* o = out + (((int64_t)split * rows_pad + row) * 2) * NC + col;
-                o[0] = acc[(m * NTB + n) * 2][v];
-                o[NC] = acc[(m * NTB + n) * 2 + 1][v];
+        for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const uint4*>(abase + (kb0 << 12) + m * 1024);
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            bx[l] = *reinterpret_cast<const v4i*>(bxp + ((kb0 * 4 * NT + l) << 10));
+            if (TWO_B) bo[l] = *reinterpret_cast<const v4i*>(bop + ((kb0 * 4 * NT + l) << 10));
+        }
+        for (int64_t kb = kb0; kb < kb1; ++kb) {
+            const int64_t kn = kb + 1 < kb1 ? kb + 1 : kb;
+#pragma unroll
+            for (int m = 0; m < MT; ++m) an[m] = *reinterpret_cast<const uint4*>(abase + (kn << 12) + m * 1024);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                int64_t sn = kb * 4 + t + 1;
+                if (sn > last_step) sn = last_step;
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    bxn[l] = *reinterpret_cast<const v4i*>(bxp + ((sn * NT + l) << 10));
+                    if (TWO_B) bon[l] = *reinterpret_cast<const v4i*>(bop + ((sn * NT + l) << 10));
+                }
+#pragma unroll
+                for (int m = 0; m < MT; ++m) {
+                    const uint32_t w = t == 0 ? av[m].x : t == 1 ? av[m].y : t == 2 ? av[m].z : av[m].w;
+                    const uint32_t d0 = w & 0x03030303u, d1 = (w >> 2) & 0x03030303u;
+                    const uint32_t d2 = (w >> 4) & 0x03030303u, d3 = (w >> 6) & 0x03030303u;
+                    const v4i X = v4i{(int)__builtin_amdgcn_perm(0u, kLutX, d0),
+                                      (int)__builtin_amdgcn_perm(0u, kLutX, d1),
+                                      (int)__builtin_amdgcn_perm(0u, kLutX, d2),
+                                      (int)__builtin_amdgcn_perm(0u, kLutX, d3)};
+                    const v4i O = v4i{(int)__builtin_amdgcn_perm(0u, kLutO, d0),
+                                      (int)__builtin_amdgcn_perm(0u, kLutO, d1),
+                                      (int)__builtin_amdgcn_perm(0u, kLutO, d2),
+                                      (int)__builtin_amdgcn_perm(0u, kLutO, d3)};
+#pragma unroll
+                    for (int l = 0; l < L; ++l) {
+                        accX[m][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(X, bx[l], accX[m][l], 0, 0, 0);
+                        accO[m][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[l] : bx[l],
+                                                                           accO[m][l], 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    bx[l] = bxn[l];
+                    if (TWO_B) bo[l] = bon[l];
+                }
             }
+#pragma unroll
+            for (int m = 0; m < MT; ++m) av[m] = an[m];
+        }
+    }
+
+    // Epilogue.  16x16 MFMA C layout: lane holds column r of its group, rows 4g + v (v = 0..3)
+    // of each m-subtile; the L limbs of one column sit in the same lane (tiles gz*L + l).
+    const int64_t rhs = (int64_t)gz * 16 + r;
+    const bool col_ok = rhs < a.k;
+    double wmax = 0.0;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t row = tr * 64 + 16 * m + 4 * g + v;
+            long long P = 0, Q = 0;
+#pragma unroll
+            for (int l = L - 1; l >= 0; --l) {
+                P = P * 256 + accX[m][l][v];
+                Q = Q * 256 + accO[m][l][v];
+            }
+            if (EPI == EPI_PQ) {
+                if (col_ok && row < a.rows) {
+                    const int64_t G16 = 16 * (int64_t)a.G;
+                    a.pq[(row * 2) * G16 + rhs] = P;
+                    a.pq[(row * 2 + 1) * G16 + rhs] = Q;
+                }
+            } else if (EPI == EPI_KV_GT) {
+                if (col_ok) {
+                    double w1 = 0.0, w2 = 0.0;
+                    if (row < a.rows && a.mult[row] != 0 && a.inv_stds[row] != 0.0) {
+                        const double s = a.inv_stds[row], mu = a.means[row];
+                        const double u = ((double)P - mu * (double)Q) * s * a.scale[rhs];
+                        const double w = (double)a.mult[row] * s * u * a.inv_M;
+                        w1 = w;
+                        w2 = mu * w;
+                    }
+                    a.W1[row * a.k + rhs] = w1;
+                    a.W2[row * a.k + rhs] = w2;
+                    wmax = fmax(wmax, fmax(fabs(w1), fabs(w2)));
+                }
+            } else {
+                if (col_ok && row < a.rows)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(a.yint + row * a.k + rhs),
+                              (unsigned long long)(P - Q));
+            }
+        }
+    if (EPI == EPI_KV_GT) {
+        wmax = fmax(wmax, __shfl_xor(wmax, 16));
+        wmax = fmax(wmax, __shfl_xor(wmax, 32));
+        if (g == 0 && col_ok && wmax > 0.0)
+            atomicMax(a.mx + rhs, (unsigned long long)__double_as_longlong(wmax));
+    }
+}
+
+// Entry points.  The two-operand form carries twice the B registers; without an explicit
+// occupancy target the compiler sinks its B prefetch loads next to their uses (every MFMA step
+// then waits on the outstanding A prefetch), so it is pinned to 2 waves per SIMD.
+template <int L, int EPI>
+__global__ __launch_bounds__(256) void k_geno1(GenoArgs a) {
+    geno_body<L, false, EPI>(a);
+}
+template <int L, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno2(GenoArgs a) {
+    geno_body<L, true, EPI>(a);
 }
 
 }  // namespace csamd

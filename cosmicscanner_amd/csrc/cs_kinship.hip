@@ -8,7 +8,8 @@
 // space and a gather of the result (K_sel is the principal submatrix of the full K).  Variants:
 // a per-SNP multiplicity (0 = not selected), so LOCO masks cost nothing extra.
 // Both sums run on int8 MFMA with the genotype operand exact and V / W as 2 or 3 signed 8-bit
-// limbs of a per-column fixed point; Z^T V reads the SNP-major copy, Z W the sample-major copy.
+// limbs of a per-column fixed point; Z^T V streams the SNP-major copy and writes w, mu w directly
+// from its epilogue; Z W streams the sample-major copy and accumulates exact int64 results.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -16,11 +17,6 @@
 #include "cs_common.h"
 #include "cs_geno_mfma.h"
 #include "cs_ops.h"
-
-namespace csamd {
-void ensure_stats(cs_bed* b);
-void ensure_sample_major(cs_bed* b);
-}  // namespace csamd
 
 struct cs_kinship {
     cs_bed* bed = nullptr;
@@ -32,14 +28,13 @@ struct cs_kinship {
     hipStream_t stream = nullptr;
     csamd::DevBuf d_keep, d_inv_off, d_inv_idx, d_mult;
     // workspaces (grown on demand)
-    csamd::DevBuf dX, dY, Vt, BV, PQ1, W1, W2, B1, B2, PQ2, mx, scl, Vf, PQf, Wf;
+    csamd::DevBuf dX, dY, Vt, BV, W1, W2, B1, B2, Yint, mxW, mx, scl, Vf, PQf, Wf;
     csamd::DevBuf diag_tab, diag_full;
     hipEvent_t ev[6] = {};
     float last_ms[5] = {0, 0, 0, 0, 0};
 };
 
 namespace csamd {
-
 
 // ------------------------------------------------------------------------------------------
 __global__ void k_scatter(const double* __restrict__ V, int64_t ldv, int64_t k,
@@ -91,7 +86,7 @@ __global__ void k_scales(const unsigned long long* __restrict__ mx, int64_t k, d
 }
 
 // Fixed-point quantisation into L balanced signed 8-bit limbs, stored pre-permuted for the MFMA
-// B operand at columns limb*k + c.
+// B operand: column c, limb l -> tile (c/16)*L + l, column c%16 of the tile.
 __global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, int L, int NT,
                         double qmax, const double* __restrict__ inv, int8_t* __restrict__ B) {
     const int64_t total = rows * k;
@@ -110,120 +105,75 @@ __global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, i
                 d = (int)(((q + 128) & 255) - 128);
                 q = (q - d) >> 8;
             }
-            B[bperm_offset(row, l * (int)k + c, NT)] = (int8_t)d;
+            B[bperm_offset(row, (c >> 4) * L + l, c & 15, NT)] = (int8_t)d;
         }
     }
 }
 
-// Z^T V epilogue: combine limbs, centre / scale per SNP, and form the two forward operands
-// W1 = w, W2 = mu * w with w = mult * s * u / M_sel.
-__global__ void k_epi_gt(const int32_t* __restrict__ PQ, int NC, int64_t M, int64_t Mpad, int64_t k,
-                         int L, const double* __restrict__ scaleV, const double* __restrict__ means,
-                         const double* __restrict__ inv_stds, const int32_t* __restrict__ mult,
-                         double inv_M, double* __restrict__ W1, double* __restrict__ W2) {
-    const int64_t total = Mpad * k;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = e / k;
-        const int c = (int)(e - j * k);
-        double w1 = 0.0, w2 = 0.0;
-        if (j < M && mult[j] != 0 && inv_stds[j] != 0.0) {
-            long long P = 0, Q = 0;
-            const int32_t* row = PQ + j * 2 * NC;
-            for (int l = L - 1; l >= 0; --l) {
-                P = P * 256 + row[l * k + c];
-                Q = Q * 256 + row[NC + l * k + c];
-            }
-            const double s = inv_stds[j], mu = means[j];
-            const double u = ((double)P - mu * (double)Q) * s * scaleV[c];
-            const double w = (double)mult[j] * s * u * inv_M;
-            w1 = w;
-            w2 = mu * w;
-        }
-        W1[e] = w1;
-        W2[e] = w2;
-    }
-}
-
-// Z W epilogue: sum split partials and limbs, scale, gather the kept rows into Y (column-major).
-__global__ void k_epi_g(const int32_t* __restrict__ PQ, int nsplit, int NC, int64_t Npad, int64_t k,
-                        int L, const double* __restrict__ scaleW, const int32_t* __restrict__ keep,
-                        int64_t n, double* __restrict__ Y, int64_t ldy) {
+// y = scale_c * yint[keep[r]][c] into the column-major output.
+__global__ void k_finalize(const long long* __restrict__ yint, int64_t k,
+                           const double* __restrict__ scale, const int32_t* __restrict__ keep,
+                           int64_t n, double* __restrict__ Y, int64_t ldy) {
     const int64_t total = n * k;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = e / k;
         const int c = (int)(e - r * k);
         const int64_t s = keep ? keep[r] : r;
-        long long acc = 0;
-        for (int l = L - 1; l >= 0; --l) {
-            long long part = 0;
-            for (int sp = 0; sp < nsplit; ++sp) {
-                const int32_t* row = PQ + ((int64_t)sp * Npad + s) * 2 * NC;
-                part += (long long)row[l * k + c] - (long long)row[NC + l * k + c];
-            }
-            acc = acc * 256 + part;
-        }
-        Y[r + c * ldy] = (double)acc * scaleW[c];
+        Y[r + c * ldy] = (double)yint[s * k + c] * scale[c];
     }
 }
 
 // ------------------------------------------------------------------------------------------
-// Plain FP32 comparator: decode to FP32 and FMA on the vector ALUs, RHS tiles staged in LDS,
-// no MFMA and no split.  Block = 32 rows; 8 threads per row, each over 32 k-elements per 256-k
-// chunk, reduced by shuffles.  KC = number of columns (<= 32).
+// Plain FP32 comparator: decode to FP32 and FMA on the vector ALUs (no MFMA, no split), over
+// the same tiled copies.  One wave per 64-row tile; lane (g, r) accumulates row 16m + r over its
+// 16-byte chunk g of every 256-k block, reading the right-hand sides (fp32 [kpad][KC]) from
+// cache; the four chunk lanes are reduced by shuffles.
 template <int KC>
-__global__ __launch_bounds__(256) void k_rows_f32(const uint8_t* __restrict__ A, int64_t lda,
-                                                  int64_t rows, int64_t kpad,
-                                                  const float* __restrict__ Bx,
+__global__ __launch_bounds__(256) void k_rows_f32(const uint8_t* __restrict__ A, int64_t KB,
+                                                  int64_t rows, const float* __restrict__ Bx,
                                                   const float* __restrict__ Bo,
                                                   float* __restrict__ out) {
-    __shared__ float sx[256 * KC];
-    __shared__ float so[256 * KC];
-    const int t = threadIdx.x;
-    const int rloc = t >> 3, sub = t & 7;
-    const int64_t row = (int64_t)blockIdx.x * 32 + rloc;
-    float ax[KC], ao[KC];
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint8_t* base = A + ((tr * KB) << 12) + lane * 16;
+    for (int m = 0; m < 4; ++m) {
+        float ax[KC], ao[KC];
 #pragma unroll
-    for (int c = 0; c < KC; ++c) { ax[c] = 0.f; ao[c] = 0.f; }
-    for (int64_t k0 = 0; k0 < kpad; k0 += 256) {
-        __syncthreads();
-        for (int e = t; e < 256 * KC; e += 256) {
-            sx[e] = Bx[k0 * KC + e];
-            so[e] = Bo[k0 * KC + e];
-        }
-        __syncthreads();
-        if (row < rows) {
-            const uint32_t* p = reinterpret_cast<const uint32_t*>(A + row * lda + k0 / 4) + sub * 2;
+        for (int c = 0; c < KC; ++c) { ax[c] = 0.f; ao[c] = 0.f; }
+        for (int64_t kb = 0; kb < KB; ++kb) {
+            const uint4 w4 = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
+            const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+            const int64_t k0 = kb * 256 + g * 64;
 #pragma unroll
-            for (int d = 0; d < 2; ++d) {
-                const uint32_t w = p[d];
-#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll 4
                 for (int u = 0; u < 16; ++u) {
-                    const uint32_t code = (w >> (2 * u)) & 3u;
+                    const uint32_t code = (ws[q] >> (2 * u)) & 3u;
                     const float x = code == 0 ? 2.f : (code == 2 ? 1.f : 0.f);
                     const float o = code == 1 ? 0.f : 1.f;
-                    const int kk = sub * 32 + d * 16 + u;
+                    const float* bx = Bx + (k0 + 16 * q + u) * KC;
+                    const float* bo = Bo + (k0 + 16 * q + u) * KC;
 #pragma unroll
                     for (int c = 0; c < KC; ++c) {
-                        ax[c] = __builtin_fmaf(x, sx[kk * KC + c], ax[c]);
-                        ao[c] = __builtin_fmaf(o, so[kk * KC + c], ao[c]);
+                        ax[c] = __builtin_fmaf(x, bx[c], ax[c]);
+                        ao[c] = __builtin_fmaf(o, bo[c], ao[c]);
                     }
                 }
+        }
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+            for (int off = 16; off < 64; off <<= 1) {
+                ax[c] += __shfl_xor(ax[c], off);
+                ao[c] += __shfl_xor(ao[c], off);
             }
-        }
-    }
+        const int64_t row = tr * 64 + 16 * m + r;
+        if (g == 0 && row < rows) {
 #pragma unroll
-    for (int c = 0; c < KC; ++c)
-        for (int off = 4; off > 0; off >>= 1) {
-            ax[c] += __shfl_xor(ax[c], off);
-            ao[c] += __shfl_xor(ao[c], off);
-        }
-    if (sub == 0 && row < rows) {
-#pragma unroll
-        for (int c = 0; c < KC; ++c) {
-            out[(row * 2) * KC + c] = ax[c];
-            out[(row * 2 + 1) * KC + c] = ao[c];
+            for (int c = 0; c < KC; ++c) {
+                out[(row * 2) * KC + c] = ax[c];
+                out[(row * 2 + 1) * KC + c] = ao[c];
+            }
         }
     }
 }
@@ -276,7 +226,8 @@ __global__ void k_epi_g_f32(const float* __restrict__ PQ, int KC, int64_t k,
 
 // ------------------------------------------------------------------------------------------
 // diag(K) over the sample-major copy: table T[j][code] = mult_j s_j^2 (x_code - mu_j)^2 (0 for
-// missing), one wave per sample row summing T over its SNPs in a fixed order.
+// missing), one wave per 64-sample tile summing T over the SNPs in a fixed order per lane, the
+// four chunk lanes of a sample reduced by shuffles.
 __global__ void k_diag_table(int64_t M, const double* __restrict__ means,
                              const double* __restrict__ inv_stds, const int32_t* __restrict__ mult,
                              double* __restrict__ T) {
@@ -290,78 +241,77 @@ __global__ void k_diag_table(int64_t M, const double* __restrict__ means,
     }
 }
 
-__global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, int64_t CS,
+__global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, int64_t KB,
                                               int64_t N, int64_t M, const double* __restrict__ T,
                                               double inv_M, double* __restrict__ diag) {
-    const int lane = threadIdx.x & 63;
-    const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (s >= N) return;
-    const uint32_t* row = reinterpret_cast<const uint32_t*>(smp + s * CS);
-    const int64_t ndw = (M + 15) / 16;
-    double acc = 0.0;
-    for (int64_t d = lane; d < ndw; d += 64) {
-        const uint32_t w = row[d];
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tr * 64 >= N) return;
+    const uint8_t* base = smp + ((tr * KB) << 12) + lane * 16;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t kb = 0; kb < KB; ++kb) {
+        const int64_t j0 = kb * 256 + g * 64;
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int64_t j = d * 16 + u;
-            if (j < M) acc += T[j * 4 + ((w >> (2 * u)) & 3u)];
+        for (int m = 0; m < 4; ++m) {
+            const uint4 w4 = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
+            const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int64_t j = j0 + 16 * q + u;
+                    if (j < M) acc[m] += T[j * 4 + ((ws[q] >> (2 * u)) & 3u)];
+                }
         }
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) diag[s] = acc * inv_M;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        for (int off = 16; off < 64; off <<= 1) acc[m] += __shfl_xor(acc[m], off);
+        const int64_t s = tr * 64 + 16 * m + r;
+        if (g == 0 && s < N) diag[s] = acc[m] * inv_M;
+    }
 }
 
 // ------------------------------------------------------------------------------------------
+template <int L, bool TWO_B, int EPI>
+static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a) {
+    if (TWO_B) k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
+    else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
+}
 
-template <bool TWO_B>
-void launch_mfma(int NTB, dim3 grid, hipStream_t st, const uint8_t* A, int64_t lda,
-                        int64_t rows_pad, int64_t kbps, int64_t kbt, int NT, const int8_t* Bx,
-                        const int8_t* Bo, int32_t* out) {
-    switch (NTB) {
-        case 1: k_geno_mfma<1, TWO_B><<<grid, 256, 0, st>>>(A, lda, rows_pad, kbps, kbt, NT, Bx, Bo, out); break;
-        case 2: k_geno_mfma<2, TWO_B><<<grid, 256, 0, st>>>(A, lda, rows_pad, kbps, kbt, NT, Bx, Bo, out); break;
-        case 3: k_geno_mfma<3, TWO_B><<<grid, 256, 0, st>>>(A, lda, rows_pad, kbps, kbt, NT, Bx, Bo, out); break;
-        default: k_geno_mfma<4, TWO_B><<<grid, 256, 0, st>>>(A, lda, rows_pad, kbps, kbt, NT, Bx, Bo, out); break;
+void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const GenoArgs& a) {
+#define CS_GENO_CASE(LL, TB, EE)                         \
+    if (L == LL && two_b == TB && epi == EE) {           \
+        launch_one<LL, TB, EE>(grid, st, a);             \
+        CS_HIP(hipGetLastError());                       \
+        return;                                          \
     }
-    CS_HIP(hipGetLastError());
+    CS_GENO_CASE(3, false, EPI_PQ)
+    CS_GENO_CASE(2, false, EPI_PQ)
+    CS_GENO_CASE(3, false, EPI_KV_GT)
+    CS_GENO_CASE(2, false, EPI_KV_GT)
+    CS_GENO_CASE(3, true, EPI_KV_G)
+    CS_GENO_CASE(2, true, EPI_KV_G)
+#undef CS_GENO_CASE
+    throw RuntimeError("launch_geno: unsupported configuration");
 }
 
-Tiling tiling(int64_t k, int L) {
-    int NT = (int)((L * k + 15) / 16);
-    int NTB = NT <= 4 ? NT : 4;
-    NT = (int)round_up(NT, NTB);
-    return Tiling{NT, NTB, NT * 16};
-}
-
-// Quantise the columns of a row-major [rows][k] fp64 matrix (rows = padded K extent) into the
-// permuted int8 limb layout of the MFMA B operand.  scl[0:k] = scales, scl[k:2k] = inverses.
-void quantize_cols(const double* W, int64_t rows, int64_t k, int L, const Tiling& tl,
-                   hipStream_t st, DevBuf& mx, DevBuf& scl, DevBuf& B) {
-    const double qmax = L == 3 ? kQmax3 : kQmax2;
+void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, const Tiling& tl,
+                   hipStream_t st, DevBuf& mx, DevBuf& scl, int8_t* const* B) {
+    const double qmax = tl.L == 3 ? kQmax3 : kQmax2;
     mx.ensure((size_t)k * sizeof(unsigned long long));
     scl.ensure((size_t)2 * k * sizeof(double));
     CS_HIP(hipMemsetAsync(mx.p, 0, (size_t)k * sizeof(unsigned long long), st));
-    k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(W, rows, k, mx.as<unsigned long long>());
+    for (int i = 0; i < nW; ++i)
+        k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(W[i], rows, k, mx.as<unsigned long long>());
     k_scales<<<grid_for(k), 256, 0, st>>>(mx.as<unsigned long long>(), k, qmax, scl.as<double>(),
-                                scl.as<double>() + k);
-    B.ensure((size_t)(rows * tl.NC));
-    CS_HIP(hipMemsetAsync(B.p, 0, (size_t)(rows * tl.NC), st));
-    k_quant<<<grid_for(rows * k), 256, 0, st>>>(W, rows, k, L, tl.NT, qmax, scl.as<double>() + k,
-                                               B.as<int8_t>());
+                                         scl.as<double>() + k);
+    for (int i = 0; i < nW; ++i) {
+        CS_HIP(hipMemsetAsync(B[i], 0, (size_t)(rows * tl.NT * 16), st));
+        k_quant<<<grid_for(rows * k), 256, 0, st>>>(W[i], rows, k, tl.L, tl.NT, qmax,
+                                                   scl.as<double>() + k, B[i]);
+    }
     CS_HIP(hipGetLastError());
-}
-
-// Z^T V int8 product on the SNP-major copy.  Vt: [Npad][k] fp64 in full sample space.  Output
-// PQ [Mpad][2][NC] int32 and the V scales in scl.
-void gt_product_int8(cs_bed* b, const double* Vt, int64_t k, int L, hipStream_t st, DevBuf& BV,
-                     DevBuf& mx, DevBuf& scl, DevBuf& PQ, Tiling& tl, hipEvent_t ev_mid) {
-    tl = tiling(k, L);
-    quantize_cols(Vt, b->Npad, k, L, tl, st, mx, scl, BV);
-    if (ev_mid) CS_HIP(hipEventRecord(ev_mid, st));
-    PQ.ensure((size_t)b->Mpad * 2 * tl.NC * sizeof(int32_t));
-    dim3 grid((unsigned)(b->Mpad / 256), 1, (unsigned)(tl.NT / tl.NTB));
-    launch_mfma<false>(tl.NTB, grid, st, b->snp.as<uint8_t>(), b->RS, b->Mpad, b->Npad / 256,
-                       b->Npad / 256, tl.NT, BV.as<int8_t>(), BV.as<int8_t>(), PQ.as<int32_t>());
 }
 
 static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
@@ -370,6 +320,7 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     hipStream_t st = op->stream;
     const int L = op->prec == CS_PREC_INT8_L2 ? 2 : 3;
     const double qmax = L == 3 ? kQmax3 : kQmax2;
+    const Tiling tl = tiling(k, L);
     const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
     const int32_t* inv_idx = op->identity ? nullptr : op->d_inv_idx.as<int32_t>();
 
@@ -378,50 +329,77 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     k_scatter<<<grid_for(b->Npad * k), 256, 0, st>>>(dX, ldx, k, inv_off, inv_idx, b->N, b->Npad,
                                                      op->Vt.as<double>());
     CS_HIP(hipGetLastError());
-    Tiling tl;
-    gt_product_int8(b, op->Vt.as<double>(), k, L, st, op->BV, op->mx, op->scl, op->PQ1, tl,
-                    op->ev[1]);
-    CS_HIP(hipEventRecord(op->ev[2], st));
+    op->BV.ensure((size_t)(b->Npad * tl.NT * 16));
+    {
+        const double* Ws[1] = {op->Vt.as<double>()};
+        int8_t* Bs[1] = {op->BV.as<int8_t>()};
+        quantize_cols(Ws, 1, b->Npad, k, tl, st, op->mx, op->scl, Bs);
+    }
+    CS_HIP(hipEventRecord(op->ev[1], st));
 
+    // Z^T V with the fused epilogue: W1 = w, W2 = mu w, and their per-column maxima.
     op->W1.ensure((size_t)(b->Mpad * k) * sizeof(double));
     op->W2.ensure((size_t)(b->Mpad * k) * sizeof(double));
-    k_epi_gt<<<grid_for(b->Mpad * k), 256, 0, st>>>(
-        op->PQ1.as<int32_t>(), tl.NC, b->M, b->Mpad, k, L, op->scl.as<double>(),
-        b->means.as<double>(), b->inv_stds.as<double>(), op->d_mult.as<int32_t>(),
-        1.0 / (double)op->Msel, op->W1.as<double>(), op->W2.as<double>());
-    CS_HIP(hipGetLastError());
-    // common scale for W1 and W2 (|W2| <= 2|W1|)
-    unsigned long long* mxw = op->mx.as<unsigned long long>();
-    CS_HIP(hipMemsetAsync(mxw, 0, (size_t)k * sizeof(unsigned long long), st));
-    k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(op->W1.as<double>(), b->Mpad, k, mxw);
-    k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, mxw);
-    double* sclW = op->scl.as<double>();  // V scales no longer needed past k_epi_gt
-    k_scales<<<grid_for(k), 256, 0, st>>>(mxw, k, qmax, sclW, sclW + k);
-    op->B1.ensure((size_t)(b->Mpad * tl.NC));
-    op->B2.ensure((size_t)(b->Mpad * tl.NC));
-    CS_HIP(hipMemsetAsync(op->B1.p, 0, (size_t)(b->Mpad * tl.NC), st));
-    CS_HIP(hipMemsetAsync(op->B2.p, 0, (size_t)(b->Mpad * tl.NC), st));
+    op->mxW.ensure((size_t)k * 8);
+    CS_HIP(hipMemsetAsync(op->mxW.p, 0, (size_t)k * 8, st));
+    GenoArgs a{};
+    a.A = b->snp.as<uint8_t>();
+    a.KB = b->KBs;
+    a.rows = b->M;
+    a.kb_total = a.kb_per_split = b->KBs;
+    a.G = tl.G;
+    a.Bx = a.Bo = op->BV.as<int8_t>();
+    a.k = k;
+    a.scale = op->scl.as<double>();
+    a.means = b->means.as<double>();
+    a.inv_stds = b->inv_stds.as<double>();
+    a.mult = op->d_mult.as<int32_t>();
+    a.inv_M = 1.0 / (double)op->Msel;
+    a.W1 = op->W1.as<double>();
+    a.W2 = op->W2.as<double>();
+    a.mx = op->mxW.as<unsigned long long>();
+    launch_geno(L, false, EPI_KV_GT, dim3((unsigned)(b->Mpad / 256), 1, (unsigned)tl.G), st, a);
+    CS_HIP(hipEventRecord(op->ev[2], st));
+
+    // common per-column scale of W1 and W2, their limbs
+    double* sclW = op->scl.as<double>();  // the V scales are no longer needed
+    k_scales<<<grid_for(k), 256, 0, st>>>(op->mxW.as<unsigned long long>(), k, qmax, sclW, sclW + k);
+    op->B1.ensure((size_t)(b->Mpad * tl.NT * 16));
+    op->B2.ensure((size_t)(b->Mpad * tl.NT * 16));
+    CS_HIP(hipMemsetAsync(op->B1.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
+    CS_HIP(hipMemsetAsync(op->B2.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
     k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W1.as<double>(), b->Mpad, k, L, tl.NT, qmax,
                                                   sclW + k, op->B1.as<int8_t>());
     k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, L, tl.NT, qmax,
                                                   sclW + k, op->B2.as<int8_t>());
     CS_HIP(hipGetLastError());
+    op->Yint.ensure((size_t)(b->Npad * k) * sizeof(long long));
+    CS_HIP(hipMemsetAsync(op->Yint.p, 0, (size_t)(b->Npad * k) * sizeof(long long), st));
     CS_HIP(hipEventRecord(op->ev[3], st));
 
-    const int64_t kbt = b->Mpad / 256;
+    // Z W over the sample-major copy, split along the SNPs, exact int64 accumulation.
+    const int64_t kbt = b->KBm;
     const int64_t rblocks = b->Npad / 256;
-    int64_t nsplit = std::max<int64_t>(1, (1024 + rblocks - 1) / rblocks);
+    int64_t nsplit = std::max<int64_t>(1, (1536 + rblocks * tl.G - 1) / (rblocks * tl.G));
     nsplit = std::min<int64_t>(nsplit, kbt);
     const int64_t kbps = (kbt + nsplit - 1) / nsplit;
     nsplit = (kbt + kbps - 1) / kbps;
-    op->PQ2.ensure((size_t)(nsplit * b->Npad * 2 * tl.NC) * sizeof(int32_t));
-    dim3 grid((unsigned)rblocks, (unsigned)nsplit, (unsigned)(tl.NT / tl.NTB));
-    launch_mfma<true>(tl.NTB, grid, st, b->smp.as<uint8_t>(), b->CS, b->Npad, kbps, kbt, tl.NT,
-                      op->B1.as<int8_t>(), op->B2.as<int8_t>(), op->PQ2.as<int32_t>());
+    GenoArgs c{};
+    c.A = b->smp.as<uint8_t>();
+    c.KB = b->KBm;
+    c.rows = b->N;
+    c.kb_total = kbt;
+    c.kb_per_split = kbps;
+    c.G = tl.G;
+    c.Bx = op->B1.as<int8_t>();
+    c.Bo = op->B2.as<int8_t>();
+    c.k = k;
+    c.yint = op->Yint.as<long long>();
+    launch_geno(L, true, EPI_KV_G, dim3((unsigned)rblocks, (unsigned)nsplit, (unsigned)tl.G), st, c);
     CS_HIP(hipEventRecord(op->ev[4], st));
-    k_epi_g<<<grid_for(op->n * k), 256, 0, st>>>(op->PQ2.as<int32_t>(), (int)nsplit, tl.NC, b->Npad,
-                                                k, L, sclW, op->identity ? nullptr : op->d_keep.as<int32_t>(),
-                                                op->n, dY, ldy);
+    k_finalize<<<grid_for(op->n * k), 256, 0, st>>>(op->Yint.as<long long>(), k, sclW,
+                                                   op->identity ? nullptr : op->d_keep.as<int32_t>(),
+                                                   op->n, dY, ldy);
     CS_HIP(hipGetLastError());
     CS_HIP(hipEventRecord(op->ev[5], st));
 }
@@ -430,15 +408,13 @@ template <int KC>
 static void apply_f32_cols(cs_kinship* op, int64_t k) {
     cs_bed* b = op->bed;
     hipStream_t st = op->stream;
-    // Vt (fp64 [Npad][k]) -> Vf (fp32 [Npad][KC])
     op->Vf.ensure((size_t)(b->Npad * KC) * sizeof(float));
     k_to_f32_cols<<<grid_for(b->Npad * KC), 256, 0, st>>>(op->Vt.as<double>(), b->Npad, k, KC,
                                                          op->Vf.as<float>());
     CS_HIP(hipEventRecord(op->ev[1], st));
     op->PQf.ensure((size_t)(std::max(b->Mpad, b->Npad) * 2 * KC) * sizeof(float));
-    k_rows_f32<KC><<<(unsigned)(b->Mpad / 32), 256, 0, st>>>(
-        b->snp.as<uint8_t>(), b->RS, b->M, b->Npad, op->Vf.as<float>(), op->Vf.as<float>(),
-        op->PQf.as<float>());
+    k_rows_f32<KC><<<(unsigned)(b->Mpad / 256), 256, 0, st>>>(
+        b->snp.as<uint8_t>(), b->KBs, b->M, op->Vf.as<float>(), op->Vf.as<float>(), op->PQf.as<float>());
     CS_HIP(hipGetLastError());
     CS_HIP(hipEventRecord(op->ev[2], st));
     op->Wf.ensure((size_t)(2 * b->Mpad * KC) * sizeof(float));
@@ -448,8 +424,8 @@ static void apply_f32_cols(cs_kinship* op, int64_t k) {
         op->PQf.as<float>(), KC, b->M, b->Mpad, k, b->means.as<double>(), b->inv_stds.as<double>(),
         op->d_mult.as<int32_t>(), (float)(1.0 / (double)op->Msel), W1, W2);
     CS_HIP(hipEventRecord(op->ev[3], st));
-    k_rows_f32<KC><<<(unsigned)(b->Npad / 32), 256, 0, st>>>(b->smp.as<uint8_t>(), b->CS, b->N,
-                                                             b->Mpad, W1, W2, op->PQf.as<float>());
+    k_rows_f32<KC><<<(unsigned)(b->Npad / 256), 256, 0, st>>>(b->smp.as<uint8_t>(), b->KBm, b->N, W1,
+                                                              W2, op->PQf.as<float>());
     CS_HIP(hipGetLastError());
     CS_HIP(hipEventRecord(op->ev[4], st));
 }
@@ -460,9 +436,8 @@ static void apply_f32(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, 
     hipStream_t st = op->stream;
     const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
     const int32_t* inv_idx = op->identity ? nullptr : op->d_inv_idx.as<int32_t>();
-    // Column chunks of <= 16 (or 32) through the FP32 kernels.
-    for (int64_t c0 = 0; c0 < k; c0 += 32) {
-        const int64_t kc = std::min<int64_t>(32, k - c0);
+    for (int64_t c0 = 0; c0 < k; c0 += 16) {
+        const int64_t kc = std::min<int64_t>(16, k - c0);
         CS_HIP(hipEventRecord(op->ev[0], st));
         op->Vt.ensure((size_t)(b->Npad * kc) * sizeof(double));
         k_scatter<<<grid_for(b->Npad * kc), 256, 0, st>>>(dX + c0 * ldx, ldx, kc, inv_off, inv_idx,
@@ -470,8 +445,7 @@ static void apply_f32(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, 
         int KC;
         if (kc <= 1) { apply_f32_cols<1>(op, kc); KC = 1; }
         else if (kc <= 4) { apply_f32_cols<4>(op, kc); KC = 4; }
-        else if (kc <= 16) { apply_f32_cols<16>(op, kc); KC = 16; }
-        else { apply_f32_cols<32>(op, kc); KC = 32; }
+        else { apply_f32_cols<16>(op, kc); KC = 16; }
         k_epi_g_f32<<<grid_for(op->n * kc), 256, 0, st>>>(
             op->PQf.as<float>(), KC, kc, op->identity ? nullptr : op->d_keep.as<int32_t>(), op->n,
             dY + c0 * ldy, ldy);
@@ -530,13 +504,11 @@ int cs_kinship_create(cs_bed* bed, const int32_t* keep, int64_t n_keep, const in
             ensure_sample_major(bed);
             CS_HIP(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
             for (auto& e : op->ev) CS_HIP(hipEventCreate(&e));
-            // variant multiplicities
             std::vector<int32_t> mult((size_t)M, op->variants.empty() ? 1 : 0);
             for (int32_t j : op->variants) mult[(size_t)j] += 1;
             op->d_mult.alloc((size_t)M * sizeof(int32_t));
             CS_HIP(hipMemcpy(op->d_mult.p, mult.data(), (size_t)M * sizeof(int32_t), hipMemcpyHostToDevice));
             if (!op->identity) {
-                // keep list and its inverse (CSR from full sample -> positions r, ascending r)
                 op->d_keep.alloc((size_t)op->n * sizeof(int32_t));
                 CS_HIP(hipMemcpy(op->d_keep.p, op->keep.data(), (size_t)op->n * sizeof(int32_t),
                                  hipMemcpyHostToDevice));
@@ -632,9 +604,9 @@ int cs_kinship_diag(cs_kinship* op, double* diag) {
         op->diag_full.ensure((size_t)b->Npad * sizeof(double));
         k_diag_table<<<grid_for(b->M), 256, 0, st>>>(b->M, b->means.as<double>(), b->inv_stds.as<double>(),
                                                     op->d_mult.as<int32_t>(), op->diag_tab.as<double>());
-        k_diag<<<(unsigned)((b->N + 3) / 4), 256, 0, st>>>(b->smp.as<uint8_t>(), b->CS, b->N, b->M,
-                                                          op->diag_tab.as<double>(),
-                                                          1.0 / (double)op->Msel, op->diag_full.as<double>());
+        k_diag<<<(unsigned)(b->Npad / 256), 256, 0, st>>>(b->smp.as<uint8_t>(), b->KBm, b->N, b->M,
+                                                         op->diag_tab.as<double>(),
+                                                         1.0 / (double)op->Msel, op->diag_full.as<double>());
         CS_HIP(hipGetLastError());
         std::vector<double> full((size_t)b->N);
         CS_HIP(hipMemcpyAsync(full.data(), op->diag_full.p, (size_t)b->N * sizeof(double),
@@ -663,10 +635,3 @@ int cs_kinship_last_timing(const cs_kinship* op, double* ms5) {
 }
 
 }  // extern "C"
-
-namespace csamd {
-template void launch_mfma<false>(int, dim3, hipStream_t, const uint8_t*, int64_t, int64_t, int64_t,
-                                 int64_t, int, const int8_t*, const int8_t*, int32_t*);
-template void launch_mfma<true>(int, dim3, hipStream_t, const uint8_t*, int64_t, int64_t, int64_t,
-                                int64_t, int, const int8_t*, const int8_t*, int32_t*);
-}  // namespace csamd

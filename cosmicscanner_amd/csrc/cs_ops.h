@@ -14,20 +14,25 @@ inline int grid_for(int64_t total, int block = 256) {
     return (int)g;
 }
 
-// Column tiling of an int8 product with k right-hand sides in L limbs: NT tiles of 16 columns,
-// NTB tiles per block (grid.z = NT / NTB), NC = 16 NT columns in the output.
+// Column tiling of an int8 product: k right-hand sides in groups of 16 (G = ceil(k/16) groups,
+// one per grid.z slice), each group as L limb tiles of 16 columns (NT = G*L tiles in B).
 struct Tiling {
-    int NT, NTB, NC;
+    int G, L, NT;
 };
-Tiling tiling(int64_t k, int L);
+inline Tiling tiling(int64_t k, int L) {
+    const int G = (int)((k + 15) / 16);
+    return Tiling{G, L, G * L};
+}
 
-void quantize_cols(const double* W, int64_t rows, int64_t k, int L, const Tiling& tl,
-                   hipStream_t st, DevBuf& mx, DevBuf& scl, DevBuf& B);
+// Quantise the columns of a row-major [rows][k] fp64 matrix (rows = padded k extent of the
+// product) into the permuted int8 limb layout of the MFMA B operand; a common per-column scale
+// over every matrix in W (all must have the same shape).  scl[0:k] = scales, scl[k:2k] = inverses.
+void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, const Tiling& tl,
+                   hipStream_t st, DevBuf& mx, DevBuf& scl, int8_t* const* B);
 
-template <bool TWO_B>
-void launch_mfma(int NTB, dim3 grid, hipStream_t st, const uint8_t* A, int64_t lda,
-                 int64_t rows_pad, int64_t kbps, int64_t kbt, int NT, const int8_t* Bx,
-                 const int8_t* Bo, int32_t* out);
+// Launch k_geno<L, TWO_B, EPI> (cs_geno_mfma.h) on the given grid.
+struct GenoArgs;
+void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const GenoArgs& a);
 
 void ensure_stats(cs_bed* b);
 void ensure_sample_major(cs_bed* b);

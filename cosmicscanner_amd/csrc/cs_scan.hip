@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "cs_common.h"
+#include "cs_geno_mfma.h"
 #include "cs_ops.h"
 
 namespace csamd {
@@ -65,7 +66,7 @@ struct ModelDev {
 // statistics, the nr dot products, and (stats mode) the association statistics.
 __global__ void k_scan_epilogue(const int32_t* __restrict__ snps, int64_t n_snps, int64_t row_lo,
                                 int64_t rows, const int64_t* __restrict__ counts,
-                                const int32_t* __restrict__ PQ, int NC, int64_t nr, int L,
+                                const long long* __restrict__ PQ, int64_t G16, int64_t nr,
                                 const double* __restrict__ scale, double min_maf,
                                 double* __restrict__ freq, int64_t* __restrict__ n_obs,
                                 int32_t* __restrict__ pass, double* __restrict__ dot,
@@ -94,14 +95,9 @@ __global__ void k_scan_epilogue(const int32_t* __restrict__ snps, int64_t n_snps
     n_obs[i] = obs;
     pass[i] = ps;
     if (gtg) gtg[i] = g2;
-    const int32_t* row = PQ + (j - row_lo) * 2 * NC;
+    const long long* row = PQ + (j - row_lo) * 2 * G16;
     auto dotc = [&](int64_t col) -> double {
-        long long P = 0, Q = 0;
-        for (int l = L - 1; l >= 0; --l) {
-            P = P * 256 + row[l * nr + col];
-            Q = Q * 256 + row[NC + l * nr + col];
-        }
-        return ((double)P - mean * (double)Q) * scale[col];
+        return ((double)row[col] - mean * (double)row[G16 + col]) * scale[col];
     };
     if (dot) {
         for (int64_t col = 0; col < nr; ++col) dot[i * nr + col] = obs > 0 ? dotc(col) : 0.0;
@@ -222,9 +218,13 @@ static void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, c
                                                             didx.as<int32_t>(), N, b->Npad,
                                                             Rt.as<double>());
         CS_HIP(hipGetLastError());
-        const Tiling tl = tiling(a.nr * 1, L);
-        DevBuf mx, scl, B;
-        quantize_cols(Rt.as<double>(), b->Npad, a.nr, L, tl, st, mx, scl, B);
+        const Tiling tl = tiling(a.nr, L);
+        DevBuf mx, scl, B((size_t)(b->Npad * tl.NT * 16));
+        {
+            const double* Ws[1] = {Rt.as<double>()};
+            int8_t* Bs[1] = {B.as<int8_t>()};
+            quantize_cols(Ws, 1, b->Npad, a.nr, tl, st, mx, scl, Bs);
+        }
         // model to device
         ModelDev md{};
         DevBuf dqw, dC;
@@ -263,16 +263,23 @@ static void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, c
         // SNP-row chunks of the Z^T R product
         const int64_t row_lo0 = lo / kPad * kPad;
         const int64_t row_hi = round_up(hi + 1, kPad);
-        int64_t chunk = std::max<int64_t>(kPad, ((int64_t)(256ll << 20) / (8ll * tl.NC)) / kPad * kPad);
-        DevBuf PQ((size_t)std::min<int64_t>(chunk, row_hi - row_lo0) * 2 * tl.NC * sizeof(int32_t));
+        const int64_t G16 = 16 * (int64_t)tl.G;
+        int64_t chunk = std::max<int64_t>(kPad, ((int64_t)(256ll << 20) / (16ll * G16)) / kPad * kPad);
+        DevBuf PQ((size_t)std::min<int64_t>(chunk, row_hi - row_lo0) * 2 * G16 * sizeof(long long));
         for (int64_t r0 = row_lo0; r0 < row_hi; r0 += chunk) {
             const int64_t rows = std::min<int64_t>(chunk, row_hi - r0);
-            dim3 grid((unsigned)(rows / 256), 1, (unsigned)(tl.NT / tl.NTB));
-            launch_mfma<false>(tl.NTB, grid, st, b->snp.as<uint8_t>() + r0 * b->RS, b->RS, rows,
-                               b->Npad / 256, b->Npad / 256, tl.NT, B.as<int8_t>(), B.as<int8_t>(),
-                               PQ.as<int32_t>());
+            GenoArgs ga{};
+            ga.A = b->snp.as<uint8_t>() + (((r0 / 64) * b->KBs) << 12);
+            ga.KB = b->KBs;
+            ga.rows = std::min<int64_t>(rows, M - r0);
+            ga.kb_total = ga.kb_per_split = b->KBs;
+            ga.G = tl.G;
+            ga.Bx = ga.Bo = B.as<int8_t>();
+            ga.k = a.nr;
+            ga.pq = PQ.as<long long>();
+            launch_geno(L, false, EPI_PQ, dim3((unsigned)(rows / 256), 1, (unsigned)tl.G), st, ga);
             k_scan_epilogue<<<grid_for(ns), 256, 0, st>>>(
-                dsn.as<int32_t>(), ns, r0, rows, counts.as<int64_t>(), PQ.as<int32_t>(), tl.NC, a.nr, L,
+                dsn.as<int32_t>(), ns, r0, rows, counts.as<int64_t>(), PQ.as<long long>(), G16, a.nr,
                 scl.as<double>(), a.min_maf, dfreq.as<double>(), dnobs.as<int64_t>(), dpass.as<int32_t>(),
                 o.dot ? ddot.as<double>() : nullptr, o.gtg ? dgtg.as<double>() : nullptr, md,
                 model ? dbeta.as<double>() : nullptr, model ? dse.as<double>() : nullptr,
