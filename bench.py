@@ -32,6 +32,19 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def profiled_traffic(key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (FETCH_SIZE +
+    WRITE_SIZE, calibrated against a plain copy kernel), or None if absent."""
+    path = os.path.join(ROOT, "profiles", "kv_apply_n50k_m500k_k16_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {key} "):
+                return float(line.rsplit(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(N, M_slice, k, seed):
     """Oracle K.V on a slice of the same workload: all N samples, the first M_slice SNPs."""
     import oracle
@@ -56,10 +69,22 @@ def main():
     ap.add_argument("--m", type=int, default=500_000)
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--prec", choices=["int8_l3", "int8_l2"], default="int8_l3")
-    ap.add_argument("--cpu-slice-snps", type=int, default=6000)
+    ap.add_argument("--cpu-slice-snps", type=int, default=250_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 2 GiB device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        import cosmicscanner_amd as csa
+        nbytes = 2 << 30
+        src = csa.DeviceArray(nbytes // 8)
+        dst = csa.DeviceArray(nbytes // 8)
+        for _ in range(3):
+            csa._check(csa.lib().cs_dev_copy(dst.ptr, src.ptr, nbytes))
+        print(json.dumps({"copy_probe_bytes": nbytes, "launches": 3}))
+        return 0
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
@@ -98,12 +123,13 @@ def main():
     gt_bytes = M * ((N + 3) // 4)
     g_bytes = N * ((M + 3) // 4)
     if phase[1] >= phase[3]:
-        name, kms, kbytes = "k_geno_mfma (Z^T V, SNP-major)", phase[1], gt_bytes
+        name, kms, kbytes, key = "k_geno1 (Z^T V, SNP-major)", phase[1], gt_bytes, "k_geno1"
     else:
-        name, kms, kbytes = "k_geno_mfma (Z W, sample-major)", phase[3], g_bytes
+        name, kms, kbytes, key = "k_geno2 (Z W, sample-major)", phase[3], g_bytes, "k_geno2"
     achieved = kbytes / (kms * 1e-3) / 1e9
+    traffic = profiled_traffic(key) if (N, M, k) == (50_000, 500_000, 16) else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=name,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
                     kernel_ms=round(kms, 4), apply_frac=round(gt_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
 
     fp32 = None

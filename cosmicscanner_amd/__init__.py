@@ -54,6 +54,7 @@ def _load():
         "cs_dev_upload": ([_vp, _vp, _i64], ctypes.c_int),
         "cs_dev_download": ([_vp, _vp, _i64], ctypes.c_int),
         "cs_dev_synchronize": ([], ctypes.c_int),
+        "cs_dev_copy": ([_vp, _vp, _i64], ctypes.c_int),
         "cs_bed_create": ([_u8p, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
         "cs_bed_create_from_file_image": ([_u8p, _i64, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
         "cs_bed_create_synthetic": ([ctypes.c_uint64, _i64, _i64, ctypes.c_double, ctypes.c_double,

@@ -51,7 +51,21 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cpp_tests()
     return LIB
+
+
+def build_cpp_tests() -> None:
+    """g++ build of the C++ adapter test (tests/cpp) against the library, binary next to it."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_kinship_batch_gpu.cpp")
+    if not os.path.exists(src):
+        return
+    out = os.path.join(ROOT, "tests", "cpp", "test_kinship_batch_gpu")
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src, "-o", out,
+           "-L" + HERE, "-lcsamd", "-Wl,-rpath," + HERE, "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ adapter test build failed:\n{r.stderr}")
 
 
 if __name__ == "__main__":

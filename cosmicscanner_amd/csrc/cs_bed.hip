@@ -20,6 +20,14 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 void clear_error() { g_last_error.clear(); }
 
+// Plain streaming copy (16 bytes per lane), the reference kernel for HBM byte counters.
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                int64_t n16) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------------------ relayout
 // Copy `rows` raw rows (Braw bytes each, starting at SNP r0) into the tiled SNP-major matrix;
 // slots of samples >= N in the last byte become code 01.  Everything else was pre-filled 0x55.
@@ -344,6 +352,16 @@ int cs_dev_upload(void* dst, const void* src, int64_t bytes) {
 int cs_dev_download(void* dst, const void* src, int64_t bytes) {
     return guarded([&] {
         if (bytes > 0) CS_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int cs_dev_copy(void* dst, const void* src, int64_t bytes) {
+    return guarded([&] {
+        if (bytes < 0 || bytes % 16) throw InvalidArgument("cs_dev_copy: bytes must be a multiple of 16");
+        if (bytes == 0) return;
+        k_copy16<<<8192, 256>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), bytes / 16);
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipDeviceSynchronize());
     });
 }
 

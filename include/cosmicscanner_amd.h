@@ -47,6 +47,8 @@ int cs_dev_free(void* dptr);
 int cs_dev_upload(void* dst, const void* src, int64_t bytes);
 int cs_dev_download(void* dst, const void* src, int64_t bytes);
 int cs_dev_synchronize(void);
+/* Plain device-to-device copy kernel (also the calibration reference for HBM byte counters). */
+int cs_dev_copy(void* dst, const void* src, int64_t bytes);
 
 /* ---- Genotypes (replaces PlinkReader::load/openBed, libCosmicCore/src/plink_reader.cpp:28-92,
  * with the packed matrix resident in HBM).  `body` is the .bed content after the 3 magic bytes:
