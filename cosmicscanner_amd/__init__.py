@@ -7,7 +7,10 @@ reference's interfaces for the path:
 * :class:`PlinkReader`  -- ``libCosmicCore/src/plink_reader.h`` (load, stats, class counts), with
   the packed matrix resident in HBM;
 * :class:`PlinkKinshipOperator` -- ``src/kinship_operator.h:19-93`` (apply / applyTranspose /
-  trace / getDiagonal / rows / cols / numVariants) and :func:`createKinshipOperator`.
+  trace / getDiagonal / rows / cols / numVariants) and :func:`createKinshipOperator`;
+* :class:`GrmBuilder` -- ``libCosmicCore/src/grm_builder.h:15-80`` (compute / save / getGrm /
+  getSum2pq / getTotalSnps) for the default VanRaden algorithm;
+* :func:`scan_sums` / :func:`lmm_scan` -- the per-SNP association scan.
 
 Importing fails loudly when the library has not been built: there is no CPU fallback.
 """
@@ -29,6 +32,7 @@ CS_PREC_INT8_L2 = 2
 CS_PREC_FP32 = 32
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -82,6 +86,9 @@ def _load():
         "cs_lmm_scan": ([_vp, _i32p, _i64, _dp, _i64, _i64, ctypes.c_double, _i32p, _i64,
                          ctypes.c_int, ctypes.c_void_p, _dp, _i64p, _i32p, _dp, _dp, _dp, _dp],
                         ctypes.c_int),
+        "cs_grm_build": ([_vp, ctypes.c_int, ctypes.c_float, _fp, _dp, _dp, _i64p], ctypes.c_int),
+        "cs_grm_build_device": ([_vp, ctypes.c_int, ctypes.c_float, _vp, _dp, _i64p], ctypes.c_int),
+        "cs_grm_last_timing": ([_dp, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
         f = getattr(L, name)
@@ -408,3 +415,90 @@ def lmm_scan(reader, keep, R, n_score=1, alpha=0.0, quad=None, quad_w=None, cov=
                              _p(out["se"], _dp), _p(out["p"], _dp), _p(out["log10p"], _dp)))
     out.update(freq=freq, n_obs=nobs, passes=ps.astype(bool))
     return out
+
+
+# ----------------------------------------------------------------------------- GRM (kernel b)
+class GrmBuilder:
+    """Dense GRM build (GrmBuilder, libCosmicCore/src/grm_builder.h:15-80; compute at
+    grm_builder.cpp:63-399, save at :1977-2026) for algorithm 1 (VanRaden), computed exactly on
+    the GPU: the lower triangle is the correctly rounded float32 of Z Z^T / sum_2pq, getGrm()
+    the FP64 matrix (correctly rounded raw Z Z^T divided by sum_2pq)."""
+
+    class Options:
+        def __init__(self, min_maf=0.01, algorithm=1, thread_num=1, block_size=2048):
+            self.min_maf = min_maf
+            self.algorithm = algorithm
+            self.thread_num = thread_num
+            self.block_size = block_size
+
+    def __init__(self, opts=None):
+        self.options = opts if opts is not None else GrmBuilder.Options()
+        self.tri = None
+        self.raw = None
+        self.total_sum_2pq = 0.0
+        self.total_valid_snps = 0
+        self._fam = None
+
+    def compute(self, reader, keep_indices=None, fp64=True):
+        """Build the GRM of all samples.  fp64=False skips the n x n FP64 copy (the float32
+        lower triangle for save() is always produced)."""
+        if keep_indices is not None and len(keep_indices) > 0:
+            raise RuntimeError("PlinkReader GRM calculation with subsetting not yet implemented. "
+                               "Use PGEN/BGEN or full dataset.")
+        if reader.num_samples == 0 or reader.num_snps == 0:
+            raise RuntimeError("Empty dataset provided to GrmBuilder")
+        n = reader.num_samples
+        tri = np.empty(n * (n + 1) // 2, dtype=np.float32)
+        raw = np.empty((n, n)) if fp64 else None
+        s = ctypes.c_double()
+        nv = ctypes.c_int64()
+        _check(lib().cs_grm_build(reader._h, int(self.options.algorithm), float(self.options.min_maf),
+                                  _p(tri, _fp), _p(raw, _dp), ctypes.byref(s), ctypes.byref(nv)))
+        self.tri, self.raw = tri, raw
+        self.total_sum_2pq, self.total_valid_snps = s.value, nv.value
+        self._fam = reader.fam
+        self._n = n
+
+    def getGrm(self):
+        if self.raw is None:
+            raise RuntimeError("GRM not computed with fp64=True")
+        return self.raw / self.total_sum_2pq if self.total_sum_2pq > 0 else self.raw.copy()
+
+    def getRaw(self):
+        return self.raw
+
+    def getLowerTriangle(self):
+        return self.tri
+
+    def getSum2pq(self):
+        return self.total_sum_2pq
+
+    def getTotalSnps(self):
+        return self.total_valid_snps
+
+    def last_timing_ms(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _check(lib().cs_grm_last_timing(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def save(self, out_prefix):
+        """GCTA files (grm_builder.cpp:1977-2026): .grm.id (FID<TAB>IID), .grm.bin (float32 lower
+        triangle, row-major, j <= i), .grm.N.bin (float32 SNP count of every pair)."""
+        if self.tri is None:
+            raise RuntimeError("GRM not computed")
+        fam = self._fam if self._fam is not None else [[str(i), str(i)] for i in range(self._n)]
+        with open(out_prefix + ".grm.id", "w") as f:
+            for row in fam:
+                f.write(f"{row[0]}\t{row[1]}\n")
+        self.tri.astype("<f4").tofile(out_prefix + ".grm.bin")
+        np.full(self.tri.size, np.float32(self.total_valid_snps), dtype="<f4").tofile(
+            out_prefix + ".grm.N.bin")
+
+
+def grm_build_device(reader, d_tri_ptr, min_maf=0.01):
+    """GRM lower triangle into a device buffer of n(n+1)/2 floats; returns (sum_2pq, n_valid)."""
+    s = ctypes.c_double()
+    nv = ctypes.c_int64()
+    _check(lib().cs_grm_build_device(reader._h, 1, float(min_maf), d_tri_ptr, ctypes.byref(s),
+                                     ctypes.byref(nv)))
+    return s.value, nv.value

@@ -1,0 +1,563 @@
+// Kernel (b), GRM build: the dense centred GRM of GrmBuilder::compute, algorithm 1 (VanRaden,
+// libCosmicCore/src/grm_builder.cpp:63-399), computed exactly on int8 MFMA.
+//
+// The reference centres each valid SNP by its float32-rounded mean mu_j = fl32(2 p_j)
+// (grm_builder.cpp:240) and accumulates G += Z Z^T in FP64 (:345), then divides by the block-order
+// FP64 sum of float32 2pq (:266-272, :392).  With A_j = mu_j 2^29 an integer (true whenever
+// mu_j >= 2^-6, i.e. every SNP passing the default MAF filter), z_ij = o_ij (x_ij - mu_j) is
+// exactly U_ij 2^-29 with U_ij = o_ij (2^29 x~_ij - A_j), so G_raw = 2^-58 I with the exact
+// 128-bit integer
+//   I_ik = 2^58 T1 - 2^29 (R_i + R_k) + S + 2^29 (C_ik + C_ki) - Q_i - Q_k + D_ik
+//   T1 = X~ X~^T,  R_i = sum_j A_j x~_ij,  S = sum_j A_j^2,  Q_i = sum_j A_j^2 m_ij,
+//   C_ik = sum_j x~_ij A_j m_kj,  D_ik = sum_j A_j^2 m_ij m_kj   (m = missing indicator).
+// T1 is one int8 MFMA product of packed genotypes with themselves; C and D (missing genotypes)
+// are int8 MFMA products with balanced signed 8-bit digits of A_j (4 digits) and A_j^2 (8
+// digits) folded into the missing-indicator operand; without missing genotypes they vanish.
+// The output is the correctly rounded FP64 of G_raw and the correctly rounded FP32 of
+// G_raw / sum_2pq (the GCTA .grm.bin value, grm_builder.cpp:1977-2026).
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "cs_common.h"
+#include "cs_geno_mfma.h"
+#include "cs_ops.h"
+
+namespace csamd {
+
+typedef __int128 i128;
+typedef unsigned __int128 u128;
+
+enum { OP_X = 0, OP_M = 1, OP_MD = 2 };
+enum { GE_T1 = 0, GE_C = 1, GE_D = 2 };
+
+constexpr uint32_t kLutM = 0x00000100u;    // code 01 -> 1
+constexpr uint32_t kLutMF = 0x0000FF00u;   // code 01 -> 0xFF
+constexpr int64_t kDigitChunkKB = 16384;   // 2^22 SNPs per digit-product launch
+
+struct GrmArgs {
+    const uint8_t* smp;    // masked tiled sample-major genotypes
+    int64_t KB;            // 256-SNP blocks per row tile
+    int64_t kb0, kb1;      // SNP-block range of this launch
+    const int2* blocks;    // (bi, bk) 128-row block pairs, bk <= bi
+    int64_t I0, P, N, Npad;
+    const uint8_t* digA;   // digit arrays [kb][t][g][16] (OP_MD operands)
+    const uint8_t* digB;
+    int shift;             // 8 * limb
+    int32_t* t1;           // panel accumulators, [P][Npad]
+    long long* c;
+    i128* d;
+};
+
+template <int OP>
+__device__ __forceinline__ v4i grm_operand(uint32_t w, uint4 dig) {
+    const uint32_t d0 = w & 0x03030303u, d1 = (w >> 2) & 0x03030303u;
+    const uint32_t d2 = (w >> 4) & 0x03030303u, d3 = (w >> 6) & 0x03030303u;
+    if (OP == OP_X)
+        return v4i{(int)__builtin_amdgcn_perm(0u, kLutX, d0), (int)__builtin_amdgcn_perm(0u, kLutX, d1),
+                   (int)__builtin_amdgcn_perm(0u, kLutX, d2), (int)__builtin_amdgcn_perm(0u, kLutX, d3)};
+    if (OP == OP_M)
+        return v4i{(int)__builtin_amdgcn_perm(0u, kLutM, d0), (int)__builtin_amdgcn_perm(0u, kLutM, d1),
+                   (int)__builtin_amdgcn_perm(0u, kLutM, d2), (int)__builtin_amdgcn_perm(0u, kLutM, d3)};
+    return v4i{(int)(__builtin_amdgcn_perm(0u, kLutMF, d0) & dig.x),
+               (int)(__builtin_amdgcn_perm(0u, kLutMF, d1) & dig.y),
+               (int)(__builtin_amdgcn_perm(0u, kLutMF, d2) & dig.z),
+               (int)(__builtin_amdgcn_perm(0u, kLutMF, d3) & dig.w)};
+}
+
+// One 128 x 128 lower-triangle block per workgroup; wave (wi, wk) owns the 64 x 64 quadrant
+// rows i0 = 128 bi + 64 wi, columns k0 = 128 bk + 64 wk, streaming all SNP blocks of both row
+// tiles (4 x 4 MFMA tiles per step).
+template <int AM, int BM, int EPI>
+__global__ __launch_bounds__(256) void k_grm_pass(GrmArgs a) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, r = lane & 15;
+    const int2 blk = a.blocks[blockIdx.x];
+    const int64_t i0 = (int64_t)blk.x * 128 + 64 * (wave >> 1);
+    const int64_t k0 = (int64_t)blk.y * 128 + 64 * (wave & 1);
+    if (k0 >= i0 + 64) return;   // quadrant strictly above the diagonal
+    v4i acc[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[m][n] = v4i{0, 0, 0, 0};
+    const uint8_t* abase = a.smp + (((i0 >> 6) * a.KB) << 12) + lane * 16;
+    const uint8_t* bbase = a.smp + (((k0 >> 6) * a.KB) << 12) + lane * 16;
+    uint4 av[4], bv[4], an[4], bn[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        av[m] = *reinterpret_cast<const uint4*>(abase + (a.kb0 << 12) + m * 1024);
+        bv[m] = *reinterpret_cast<const uint4*>(bbase + (a.kb0 << 12) + m * 1024);
+    }
+    for (int64_t kb = a.kb0; kb < a.kb1; ++kb) {
+        const int64_t kn = kb + 1 < a.kb1 ? kb + 1 : kb;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            an[m] = *reinterpret_cast<const uint4*>(abase + (kn << 12) + m * 1024);
+            bn[m] = *reinterpret_cast<const uint4*>(bbase + (kn << 12) + m * 1024);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            uint4 dA = uint4{0, 0, 0, 0}, dB = uint4{0, 0, 0, 0};
+            const int64_t doff = (((kb * 4 + t) * 4 + g) << 4);
+            if (AM == OP_MD) dA = *reinterpret_cast<const uint4*>(a.digA + doff);
+            if (BM == OP_MD) dB = *reinterpret_cast<const uint4*>(a.digB + doff);
+            v4i A[4], B[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t wa = t == 0 ? av[m].x : t == 1 ? av[m].y : t == 2 ? av[m].z : av[m].w;
+                const uint32_t wb = t == 0 ? bv[m].x : t == 1 ? bv[m].y : t == 2 ? bv[m].z : bv[m].w;
+                A[m] = grm_operand<AM>(wa, dA);
+                B[m] = grm_operand<BM>(wb, dB);
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int n = 0; n < 4; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[m], B[n], acc[m][n], 0, 0, 0);
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            av[m] = an[m];
+            bv[m] = bn[m];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t i = i0 + 16 * m + 4 * g + v;
+                const int64_t k = k0 + 16 * n + r;
+                if (i >= a.N || k >= a.N || k > i || i < a.I0 || i >= a.I0 + a.P) continue;
+                const int64_t idx = (i - a.I0) * a.Npad + k;
+                if (EPI == GE_T1) a.t1[idx] = acc[m][n][v];
+                else if (EPI == GE_C) a.c[idx] += (long long)acc[m][n][v] * (1LL << a.shift);
+                else a.d[idx] += (i128)acc[m][n][v] * ((i128)1 << a.shift);
+            }
+}
+
+// Invalid SNPs -> code 01 in a copy of the tiled sample-major matrix (keep mask: 2 bits per SNP,
+// 11 = valid).
+__global__ void k_mask_invalid(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16,
+                               int64_t KB, const uint32_t* __restrict__ keep) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n16;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        // tile (rt, kb): 256 chunks of 16 B; chunk = (m*64 + g*16 + r) -> SNPs 256 kb + 64 g ..
+        const int64_t tile = e >> 8;
+        const int chunk = (int)(e & 255);
+        const int64_t kb = tile % KB;
+        const int g = (chunk >> 4) & 3;
+        const int64_t q0 = (kb * 256 + g * 64) / 16;   // keep-mask dword of the chunk's first SNPs
+        uint4 v = src[e];
+        const uint32_t k0 = keep[q0], k1 = keep[q0 + 1], k2 = keep[q0 + 2], k3 = keep[q0 + 3];
+        v.x = (v.x & k0) | (0x55555555u & ~k0);
+        v.y = (v.y & k1) | (0x55555555u & ~k1);
+        v.z = (v.z & k2) | (0x55555555u & ~k2);
+        v.w = (v.w & k3) | (0x55555555u & ~k3);
+        dst[e] = v;
+    }
+}
+
+// R_i = sum_j A_j x~_ij (int64) and Q_i = sum_j A_j^2 m_ij (int128) per sample.
+__global__ __launch_bounds__(256) void k_grm_rowsums(const uint8_t* __restrict__ smp, int64_t KB,
+                                                     int64_t N, const long long* __restrict__ A,
+                                                     long long* __restrict__ R, i128* __restrict__ Q) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tr * 64 >= N) return;
+    const uint8_t* base = smp + ((tr * KB) << 12) + lane * 16;
+    long long rs[4] = {0, 0, 0, 0};
+    i128 qs[4] = {0, 0, 0, 0};
+    for (int64_t kb = 0; kb < KB; ++kb) {
+        const int64_t j0 = kb * 256 + g * 64;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const uint4 w4 = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
+            const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+            for (int q = 0; q < 4; ++q)
+                for (int u = 0; u < 16; ++u) {
+                    const long long aj = A[j0 + 16 * q + u];
+                    const uint32_t code = (ws[q] >> (2 * u)) & 3u;
+                    if (code == 1) qs[m] += (i128)aj * aj;
+                    else rs[m] += aj * (code == 0 ? 2 : (code == 2 ? 1 : 0));
+                }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        for (int off = 16; off < 64; off <<= 1) {
+            rs[m] += __shfl_xor(rs[m], off);
+            const unsigned long long lo = (unsigned long long)qs[m];
+            const long long hi = (long long)(qs[m] >> 64);
+            const unsigned long long lo2 = __shfl_xor(lo, off);
+            const long long hi2 = __shfl_xor(hi, off);
+            qs[m] += ((i128)hi2 << 64) + (i128)lo2;
+        }
+        const int64_t i = tr * 64 + 16 * m + r;
+        if (g == 0 && i < N) {
+            R[i] = rs[m];
+            Q[i] = qs[m];
+        }
+    }
+}
+
+__device__ __forceinline__ int top_bit(u128 a) {
+    const unsigned long long hi = (unsigned long long)(a >> 64), lo = (unsigned long long)a;
+    return hi ? 127 - __builtin_clzll(hi) : 63 - __builtin_clzll(lo);
+}
+
+// Correctly rounded double of an int128.
+__device__ double i128_to_f64_rn(i128 v) {
+    if (v == 0) return 0.0;
+    const bool neg = v < 0;
+    const u128 a = neg ? (u128)0 - (u128)v : (u128)v;
+    const int top = top_bit(a);
+    double res;
+    if (top <= 52) {
+        res = (double)(unsigned long long)a;
+    } else {
+        const int sh = top - 52;
+        u128 q = a >> sh;
+        const u128 rem = a & ((((u128)1) << sh) - 1);
+        const u128 half = ((u128)1) << (sh - 1);
+        if (rem > half || (rem == half && (q & 1))) q += 1;
+        res = ldexp((double)(unsigned long long)q, sh);
+    }
+    return neg ? -res : res;
+}
+
+// Correctly rounded float of (v 2^-58) / S with S = D 2^(se - 53), D a 53-bit integer.
+__device__ float i128_div_f32_rn(i128 v, unsigned long long D, int se) {
+    if (v == 0) return 0.0f;
+    const bool neg = v < 0;
+    const u128 a = neg ? (u128)0 - (u128)v : (u128)v;
+    int sh = 27 + 53 - top_bit(a);
+    if (sh < 0) sh = 0;
+    const u128 num = a << sh;
+    const u128 Q = num / D;
+    const bool sticky = (num % D) != 0;
+    const int drop = top_bit(Q) - 23;
+    u128 keepq = Q >> drop;
+    const u128 rem = Q & ((((u128)1) << drop) - 1);
+    const u128 half = ((u128)1) << (drop - 1);
+    if (rem > half || (rem == half && (sticky || (keepq & 1)))) keepq += 1;
+    const int e2 = drop - sh - 58 - (se - 53);
+    const float res = ldexpf((float)(unsigned long long)keepq, e2);
+    return neg ? -res : res;
+}
+
+struct CombineArgs {
+    int64_t I0, P, N, Npad;
+    const int32_t* t1;
+    const long long* c;
+    const i128* d;        // may be null (no missing genotypes)
+    const long long* R;
+    const i128* Q;
+    unsigned long long S_lo;
+    long long S_hi;
+    unsigned long long D53;
+    int se;
+    float* tri;           // lower triangle, GCTA order (i (i+1)/2 + k), device
+    double* raw;          // optional full square N x N (raw G, correctly rounded)
+};
+
+__global__ void k_grm_combine(CombineArgs a) {
+    const int64_t total = a.P * a.Npad;
+    const i128 S = ((i128)a.S_hi << 64) + (i128)a.S_lo;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t li = e / a.Npad, k = e - li * a.Npad;
+        const int64_t i = a.I0 + li;
+        if (i >= a.N || k > i) continue;
+        i128 I = (i128)a.t1[e] * ((i128)1 << 58) - (i128)(a.R[i] + a.R[k]) * (1LL << 29) + S - a.Q[i] - a.Q[k];
+        if (a.c) I += (i128)a.c[e] * (1LL << 29);
+        if (a.d) I += a.d[e];
+        if (a.tri) a.tri[i * (i + 1) / 2 + k] = i128_div_f32_rn(I, a.D53, a.se);
+        if (a.raw) {
+            const double g = ldexp(i128_to_f64_rn(I), -58);
+            a.raw[i * a.N + k] = g;
+            a.raw[k * a.N + i] = g;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------ host
+struct GrmParams {
+    std::vector<uint8_t> valid;
+    std::vector<long long> A;      // mu_j 2^29 (0 for invalid SNPs), Mpad entries
+    double sum_2pq = 0.0;
+    int64_t n_valid = 0;
+    bool any_missing = false;
+};
+
+// Per-SNP filter and float32 parameters of GrmBuilder::compute (grm_builder.cpp:167-272) from
+// the all-sample class counts.
+static GrmParams grm_params(cs_bed* b, float min_maf) {
+    GrmParams p;
+    std::vector<int64_t> cnt((size_t)b->M * 4);
+    CS_HIP(hipMemcpy(cnt.data(), b->counts.p, cnt.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    p.valid.assign((size_t)b->Mpad, 0);
+    p.A.assign((size_t)b->Mpad, 0);
+    std::vector<float> pq2((size_t)b->M, 0.0f);
+    for (int64_t j = 0; j < b->M; ++j) {
+        const int64_t* c = &cnt[(size_t)j * 4];
+        const long long n_obs = c[0] + c[2] + c[3];
+        const long long sum_x = 2 * c[0] + c[2];
+        if (n_obs == 0) continue;
+        const double freq = (double)sum_x / (2.0 * (double)n_obs);
+        const double maf = std::min(freq, 1.0 - freq);
+        if (maf < (double)min_maf || std::fabs(maf) < 1e-8) continue;
+        pq2[(size_t)j] = (float)(1.0f * 2.0 * freq * (1.0 - freq));
+        const float mean_f = (float)(freq * 2.0);
+        const double a = std::ldexp((double)mean_f, 29);
+        if (a != std::floor(a))
+            throw InvalidArgument("GRM: a SNP mean below 2^-6 is not a multiple of 2^-29; raise min_maf");
+        p.valid[(size_t)j] = 1;
+        p.A[(size_t)j] = (long long)a;
+        if (c[1] > 0) p.any_missing = true;
+    }
+    double total = 0.0;
+    for (int64_t blk = 0; blk < b->M; blk += 2048) {
+        double block = 0.0;
+        for (int64_t j = blk; j < std::min<int64_t>(blk + 2048, b->M); ++j)
+            if (p.valid[(size_t)j]) { block += pq2[(size_t)j]; p.n_valid++; }
+        total += block;
+    }
+    p.sum_2pq = total;
+    return p;
+}
+
+// Balanced signed 8-bit digits of a signed 128-bit value (little-endian), `nd` digits.
+static void digits128(i128 v, int nd, int8_t* out) {
+    for (int l = 0; l < nd; ++l) {
+        int d;
+        if (l == nd - 1) {
+            d = (int)v;
+        } else {
+            d = (int)(((long long)(v & 255) + 128) % 256) - 128;
+            if (d < -128) d += 256;
+            v = (v - d) >> 8;
+        }
+        out[l] = (int8_t)d;
+    }
+}
+
+}  // namespace csamd
+
+struct GrmTiming {
+    double ms_passes = 0, ms_combine = 0, ms_total = 0;
+    int passes = 0;
+};
+static thread_local GrmTiming g_grm_timing;
+
+using namespace csamd;
+
+static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, float* out_tri_dev,
+                    double* out_raw_host, double* sum_2pq, int64_t* n_valid) {
+    if (alg != 1) throw InvalidArgument("GRM: only algorithm 1 (VanRaden, the default) is built here");
+    ensure_stats(b);
+    ensure_sample_major(b);
+    GrmParams prm = grm_params(b, min_maf);
+    if (sum_2pq) *sum_2pq = prm.sum_2pq;
+    if (n_valid) *n_valid = prm.n_valid;
+    const int64_t N = b->N, Npad = b->Npad, Mpad = b->Mpad, KB = b->KBm;
+    if (Mpad > (int64_t(1) << 29))   // x~ x~ products reach 4 M in int32
+        throw InvalidArgument("GRM: more than 2^29 SNPs");
+    hipStream_t st;
+    CS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1, e2;
+    CS_HIP(hipEventCreate(&e0));
+    CS_HIP(hipEventCreate(&e1));
+    CS_HIP(hipEventCreate(&e2));
+    try {
+        // keep mask and masked copy
+        std::vector<uint32_t> keep((size_t)(Mpad / 16), 0);
+        for (int64_t j = 0; j < b->M; ++j)
+            if (prm.valid[(size_t)j]) keep[(size_t)(j / 16)] |= 3u << (2 * (j % 16));
+        DevBuf dkeep(keep.size() * 4), masked((size_t)(Npad * Mpad / 4));
+        CS_HIP(hipMemcpyAsync(dkeep.p, keep.data(), keep.size() * 4, hipMemcpyHostToDevice, st));
+        const int64_t n16 = Npad * Mpad / 64;
+        k_mask_invalid<<<grid_for(n16), 256, 0, st>>>(b->smp.as<uint4>(), masked.as<uint4>(), n16, KB,
+                                                     dkeep.as<uint32_t>());
+        CS_HIP(hipGetLastError());
+        // A_j, digits, row sums, S
+        DevBuf dA((size_t)Mpad * 8), dR((size_t)Npad * 8), dQ((size_t)Npad * 16);
+        CS_HIP(hipMemcpyAsync(dA.p, prm.A.data(), (size_t)Mpad * 8, hipMemcpyHostToDevice, st));
+        CS_HIP(hipMemsetAsync(dR.p, 0, dR.bytes, st));
+        CS_HIP(hipMemsetAsync(dQ.p, 0, dQ.bytes, st));
+        k_grm_rowsums<<<(unsigned)(Npad / 256), 256, 0, st>>>(masked.as<uint8_t>(), KB, N, dA.as<long long>(),
+                                                             dR.as<long long>(), dQ.as<i128>());
+        CS_HIP(hipGetLastError());
+        i128 S = 0;
+        for (long long a : prm.A) S += (i128)a * a;
+        const int nC = prm.any_missing ? 4 : 0, nD = prm.any_missing ? 8 : 0;
+        // digit arrays in the operand order [kb][t][g][16]: element k = 256 kb + 64 g + 16 t + 4 q + s
+        // at byte 4 s + q.
+        std::vector<uint8_t> dig((size_t)(nC + nD) * Mpad, 0);
+        for (int64_t j = 0; j < Mpad; ++j) {
+            if (!prm.A[(size_t)j]) continue;
+            int8_t dd[8];
+            const int64_t kb = j >> 8, rr = j & 255, g = rr >> 6, t = (rr >> 4) & 3, q = (rr >> 2) & 3, s = rr & 3;
+            const int64_t off = (((kb * 4 + t) * 4 + g) << 4) + 4 * s + q;
+            digits128((i128)prm.A[(size_t)j], 4, dd);
+            for (int l = 0; l < nC; ++l) dig[(size_t)l * Mpad + off] = (uint8_t)dd[l];
+            digits128((i128)prm.A[(size_t)j] * prm.A[(size_t)j], 8, dd);
+            for (int l = 0; l < nD; ++l) dig[(size_t)(nC + l) * Mpad + off] = (uint8_t)dd[l];
+        }
+        DevBuf ddig(std::max<size_t>(1, dig.size()));
+        if (!dig.empty()) CS_HIP(hipMemcpyAsync(ddig.p, dig.data(), dig.size(), hipMemcpyHostToDevice, st));
+        // S2pq as D * 2^(se-53)
+        int se = 0;
+        const double fm = std::frexp(prm.sum_2pq > 0 ? prm.sum_2pq : 1.0, &se);
+        const unsigned long long D53 = (unsigned long long)std::ldexp(fm, 53);
+        // output
+        DevBuf tri_own;
+        float* tri = out_tri_dev;
+        if (!tri) {
+            tri_own.alloc((size_t)(N * (N + 1) / 2) * sizeof(float));
+            tri = tri_own.as<float>();
+        }
+        DevBuf raw_dev;
+        if (out_raw_host) raw_dev.alloc((size_t)(N * N) * sizeof(double));
+        // panels of P rows (multiple of 128) bounded to ~8 GB of accumulators
+        // (CSAMD_GRM_PANEL_BYTES overrides the bound)
+        size_t panel_bytes = 8ull << 30;
+        if (const char* env = std::getenv("CSAMD_GRM_PANEL_BYTES")) panel_bytes = std::strtoull(env, nullptr, 10);
+        const size_t per_row = (size_t)Npad * (4 + (nC ? 8 : 0) + (nD ? 16 : 0));
+        int64_t P = std::max<int64_t>(128, (int64_t)(panel_bytes / per_row) / 128 * 128);
+        P = std::min<int64_t>(P, round_up(Npad, 128));
+        DevBuf t1((size_t)(P * Npad) * 4), cacc(nC ? (size_t)(P * Npad) * 8 : 1),
+            dacc(nD ? (size_t)(P * Npad) * 16 : 1);
+        float ms_pass = 0.f, ms_comb = 0.f;
+        int passes = 0;
+        for (int64_t I0 = 0; I0 < Npad; I0 += P) {
+            const int64_t rows = std::min<int64_t>(P, Npad - I0);
+            std::vector<int2> blocks;
+            for (int64_t bi = I0 / 128; bi < (I0 + rows) / 128; ++bi)
+                for (int64_t bk = 0; bk <= bi; ++bk) blocks.push_back(int2{(int)bi, (int)bk});
+            DevBuf dblocks(blocks.size() * sizeof(int2));
+            CS_HIP(hipMemcpyAsync(dblocks.p, blocks.data(), blocks.size() * sizeof(int2),
+                                  hipMemcpyHostToDevice, st));
+            if (nC) CS_HIP(hipMemsetAsync(cacc.p, 0, (size_t)(rows * Npad) * 8, st));
+            if (nD) CS_HIP(hipMemsetAsync(dacc.p, 0, (size_t)(rows * Npad) * 16, st));
+            GrmArgs ga{};
+            ga.smp = masked.as<uint8_t>();
+            ga.KB = KB;
+            ga.blocks = dblocks.as<int2>();
+            ga.I0 = I0;
+            ga.P = rows;
+            ga.N = N;
+            ga.Npad = Npad;
+            ga.t1 = t1.as<int32_t>();
+            ga.c = cacc.as<long long>();
+            ga.d = dacc.as<i128>();
+            const unsigned nb = (unsigned)blocks.size();
+            CS_HIP(hipEventRecord(e0, st));
+            ga.kb0 = 0;
+            ga.kb1 = KB;
+            k_grm_pass<OP_X, OP_X, GE_T1><<<nb, 256, 0, st>>>(ga);
+            ++passes;
+            // digit products accumulate |x~ d| <= 256 per SNP in int32: chunks of <= 2^22 SNPs
+            for (int64_t c0 = 0; c0 < KB; c0 += kDigitChunkKB) {
+                GrmArgs gc = ga;
+                gc.kb0 = c0;
+                gc.kb1 = std::min<int64_t>(KB, c0 + kDigitChunkKB);
+                for (int l = 0; l < nC; ++l) {
+                    GrmArgs g2 = gc;
+                    g2.shift = 8 * l;
+                    g2.digA = g2.digB = ddig.as<uint8_t>() + (size_t)l * Mpad;
+                    k_grm_pass<OP_X, OP_MD, GE_C><<<nb, 256, 0, st>>>(g2);
+                    k_grm_pass<OP_MD, OP_X, GE_C><<<nb, 256, 0, st>>>(g2);
+                    passes += 2;
+                }
+                for (int l = 0; l < nD; ++l) {
+                    GrmArgs g2 = gc;
+                    g2.shift = 8 * l;
+                    g2.digB = ddig.as<uint8_t>() + (size_t)(nC + l) * Mpad;
+                    k_grm_pass<OP_M, OP_MD, GE_D><<<nb, 256, 0, st>>>(g2);
+                    ++passes;
+                }
+            }
+            CS_HIP(hipGetLastError());
+            CS_HIP(hipEventRecord(e1, st));
+            CombineArgs ca{};
+            ca.I0 = I0;
+            ca.P = rows;
+            ca.N = N;
+            ca.Npad = Npad;
+            ca.t1 = t1.as<int32_t>();
+            ca.c = nC ? cacc.as<long long>() : nullptr;
+            ca.d = nD ? dacc.as<i128>() : nullptr;
+            ca.R = dR.as<long long>();
+            ca.Q = dQ.as<i128>();
+            ca.S_lo = (unsigned long long)S;
+            ca.S_hi = (long long)(S >> 64);
+            ca.D53 = D53;
+            ca.se = se;
+            ca.tri = tri;
+            ca.raw = out_raw_host ? raw_dev.as<double>() : nullptr;
+            k_grm_combine<<<grid_for(rows * Npad), 256, 0, st>>>(ca);
+            CS_HIP(hipGetLastError());
+            CS_HIP(hipEventRecord(e2, st));
+            CS_HIP(hipEventSynchronize(e2));
+            float a1 = 0, a2 = 0;
+            CS_HIP(hipEventElapsedTime(&a1, e0, e1));
+            CS_HIP(hipEventElapsedTime(&a2, e1, e2));
+            ms_pass += a1;
+            ms_comb += a2;
+        }
+        if (out_tri_host)
+            CS_HIP(hipMemcpyAsync(out_tri_host, tri, (size_t)(N * (N + 1) / 2) * sizeof(float),
+                                  hipMemcpyDeviceToHost, st));
+        if (out_raw_host)
+            CS_HIP(hipMemcpyAsync(out_raw_host, raw_dev.p, (size_t)(N * N) * sizeof(double),
+                                  hipMemcpyDeviceToHost, st));
+        CS_HIP(hipStreamSynchronize(st));
+        g_grm_timing.ms_passes = ms_pass;
+        g_grm_timing.ms_combine = ms_comb;
+        g_grm_timing.passes = passes;
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+        (void)hipStreamDestroy(st);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    CS_HIP(hipStreamDestroy(st));
+}
+
+extern "C" {
+
+int cs_grm_build(cs_bed* bed, int alg, float min_maf, float* grm_tri_f32, double* grm_raw_f64,
+                 double* sum_2pq, int64_t* n_valid) {
+    return guarded([&] {
+        if (!bed) throw InvalidArgument("cs_grm_build: null handle");
+        grm_run(bed, alg, min_maf, grm_tri_f32, nullptr, grm_raw_f64, sum_2pq, n_valid);
+    });
+}
+
+int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float* d_grm_tri_f32, double* sum_2pq,
+                        int64_t* n_valid) {
+    return guarded([&] {
+        if (!bed || !d_grm_tri_f32) throw InvalidArgument("cs_grm_build_device: null argument");
+        grm_run(bed, alg, min_maf, nullptr, d_grm_tri_f32, nullptr, sum_2pq, n_valid);
+    });
+}
+
+int cs_grm_last_timing(double* ms_passes, double* ms_combine, int* passes) {
+    return guarded([&] {
+        if (ms_passes) *ms_passes = g_grm_timing.ms_passes;
+        if (ms_combine) *ms_combine = g_grm_timing.ms_combine;
+        if (passes) *passes = g_grm_timing.passes;
+    });
+}
+
+}  // extern "C"

@@ -145,6 +145,29 @@ int cs_lmm_scan(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* 
                 const cs_scan_model* model, double* freq, int64_t* n_obs, int32_t* pass,
                 double* beta, double* se, double* p, double* log10p);
 
+/* ---- Kernel (b): dense GRM build.
+ * GrmBuilder::compute(const PlinkReader&) (libCosmicCore/src/grm_builder.cpp:63-399) for
+ * algorithm 1 (VanRaden, the default, --make-grm-alg 1): per-SNP filter on all samples
+ * (min(f,1-f) >= min_maf and > 1e-8, :167-257), float32 mean 2p and float32 2pq, centred
+ * z = x - mean (missing -> 0), G = Z Z^T / sum_2pq with sum_2pq the FP64 sum of the float32 2pq in
+ * 2048-SNP block order (:266-272, :392).  Computed exactly in integers on int8 MFMA:
+ *   grm_tri_f32: n(n+1)/2 floats, lower triangle row-major (i, j <= i) -- the GCTA .grm.bin
+ *                order written by GrmBuilder::save (:1977-2026); each value is the correctly
+ *                rounded float of the exact G_ij.  May be NULL.
+ *   grm_raw_f64: n x n doubles, the correctly rounded FP64 of Z Z^T (before normalisation,
+ *                symmetric).  May be NULL.
+ *   sum_2pq / n_valid: the normaliser and the number of SNPs used (.grm.N.bin holds n_valid).
+ * alg != 1 -> CS_ERR_INVALID_ARGUMENT; a kept SNP whose float32 mean is below 2^-6 and not a
+ * multiple of 2^-29 (only possible for min_maf < 2^-7) -> CS_ERR_INVALID_ARGUMENT. */
+int cs_grm_build(cs_bed* bed, int alg, float min_maf, float* grm_tri_f32, double* grm_raw_f64,
+                 double* sum_2pq, int64_t* n_valid);
+/* Same with the lower triangle written to device memory (n(n+1)/2 floats). */
+int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float* d_grm_tri_f32,
+                        double* sum_2pq, int64_t* n_valid);
+/* Kernel time of the last GRM build (all int8 products; the int128 combine) and the number of
+ * int8 product launches, from HIP events on the build's stream. */
+int cs_grm_last_timing(double* ms_products, double* ms_combine, int* launches);
+
 #ifdef __cplusplus
 }
 #endif
