@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+INT8_PEAK_TOPS = 5000.0
 
 
 def log(msg):
@@ -60,6 +61,30 @@ def cpu_baseline(N, M_slice, k, seed):
                        f"of the same generator ({dt:.1f} s)")
 
 
+def grm_leg(csa, reader, N, M):
+    """Exact centred GRM (alg 1) of the whole workload into a device-resident GCTA triangle.
+
+    tops_full_square = 2 n^2 m / wall time (the customary full-square op count; the build computes
+    only the lower triangle).  The MFMA roofline counts the int8 MFMA ops actually issued by the
+    product kernels (T1 plus the digit products that carry missing genotypes) over their time."""
+    nt = N * (N + 1) // 2
+    d_tri = csa.DeviceArray((nt + 1) // 2)
+    csa.synchronize()
+    t0 = time.perf_counter()
+    s2pq, nv = csa.grm_build_device(reader, d_tri.ptr)
+    csa.synchronize()
+    dt = time.perf_counter() - t0
+    ms_prod, ms_comb, launches, ops_issued = csa.GrmBuilder().last_timing_ms()
+    achieved = ops_issued / (ms_prod * 1e-3) / 1e12
+    d_tri.close()
+    return dict(ms=dt * 1e3, ms_products=ms_prod, ms_combine=ms_comb, launches=launches,
+                int8_ops_issued=ops_issued,
+                sum_2pq=s2pq, n_valid=nv, tops_full_square=2.0 * N * N * M / dt / 1e12,
+                roofline=dict(bound="mfma", achieved=round(achieved, 1), peak=INT8_PEAK_TOPS,
+                              unit="TOP/s (int8)", frac=round(achieved / INT8_PEAK_TOPS, 4),
+                              kernel="k_grm_pass (all launches)"))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +97,7 @@ def main():
     ap.add_argument("--cpu-slice-snps", type=int, default=250_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-grm", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 2 GiB device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -132,6 +158,15 @@ def main():
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
                     kernel_ms=round(kms, 4), apply_frac=round(gt_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
 
+    # the same apply through the host-buffer entry point (PCIe transfers of X and Y included)
+    Xh = np.asfortranarray(np.random.default_rng(7).standard_normal((N, k)))
+    op.apply(Xh)
+    t1 = time.perf_counter()
+    for _ in range(3):
+        op.apply(Xh)
+    host_rate = N * M / ((time.perf_counter() - t1) / 3)
+    log(f"K.V host buffers: {host_rate:.3e} genotypes/s")
+
     fp32 = None
     if not args.no_fp32:
         op.set_precision(csa.CS_PREC_FP32)
@@ -144,6 +179,12 @@ def main():
         fp32_dt = (time.perf_counter() - t1) / reps
         fp32 = N * M / fp32_dt
         log(f"K.V plain FP32: {fp32_dt * 1e3:.2f} ms/step")
+
+    grm = None
+    if not args.no_grm:
+        grm = grm_leg(csa, reader, N, M)
+        log(f"GRM: {grm['ms']:.1f} ms, {grm['tops_full_square']:.1f} TOPS (2 n^2 m), "
+            f"int8 products {grm['ms_products']:.1f} ms in {grm['launches']} launches")
 
     cpu = None
     if not args.no_cpu:
@@ -166,9 +207,11 @@ def main():
         "data": "synthetic HWE genotypes generated on device (p~U(0.01,0.5), 0.5% missing), random RHS",
         "config": {"workload": "GRM build + matrix-free K.v, n=50,000 x m=500,000 (K.V apply, k=16)",
                    "n": N, "m": M, "k": k, "precision": args.prec},
+        "host_buffer_genotypes_per_s": host_rate,
         "fp32_gpu_genotypes_per_s": fp32,
         "speedup_vs_fp32_gpu": (value / fp32) if fp32 else None,
         "roofline": roofline,
+        "grm": grm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

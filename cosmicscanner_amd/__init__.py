@@ -88,7 +88,7 @@ def _load():
                         ctypes.c_int),
         "cs_grm_build": ([_vp, ctypes.c_int, ctypes.c_float, _fp, _dp, _dp, _i64p], ctypes.c_int),
         "cs_grm_build_device": ([_vp, ctypes.c_int, ctypes.c_float, _vp, _dp, _i64p], ctypes.c_int),
-        "cs_grm_last_timing": ([_dp, _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "cs_grm_last_timing": ([_dp, _dp, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
         f = getattr(L, name)
@@ -477,9 +477,11 @@ class GrmBuilder:
         return self.total_valid_snps
 
     def last_timing_ms(self):
-        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-        _check(lib().cs_grm_last_timing(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return a.value, b.value, c.value
+        """(ms of the int8 products, ms of the combine, product launches, int8 MFMA ops issued)."""
+        a, b, c, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().cs_grm_last_timing(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                        ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
 
     def save(self, out_prefix):
         """GCTA files (grm_builder.cpp:1977-2026): .grm.id (FID<TAB>IID), .grm.bin (float32 lower

@@ -349,7 +349,7 @@ static void digits128(i128 v, int nd, int8_t* out) {
 }  // namespace csamd
 
 struct GrmTiming {
-    double ms_passes = 0, ms_combine = 0, ms_total = 0;
+    double ms_passes = 0, ms_combine = 0, ms_total = 0, mfma_ops = 0;
     int passes = 0;
 };
 static thread_local GrmTiming g_grm_timing;
@@ -434,6 +434,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, floa
             dacc(nD ? (size_t)(P * Npad) * 16 : 1);
         float ms_pass = 0.f, ms_comb = 0.f;
         int passes = 0;
+        double mfma_ops = 0;
         for (int64_t I0 = 0; I0 < Npad; I0 += P) {
             const int64_t rows = std::min<int64_t>(P, Npad - I0);
             std::vector<int2> blocks;
@@ -456,6 +457,9 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, floa
             ga.c = cacc.as<long long>();
             ga.d = dacc.as<i128>();
             const unsigned nb = (unsigned)blocks.size();
+            int64_t quadrants = 0;   // 64 x 64 wave tiles on or below the diagonal
+            for (const int2& bl : blocks) quadrants += bl.x == bl.y ? 3 : 4;
+            const int passes0 = passes;
             CS_HIP(hipEventRecord(e0, st));
             ga.kb0 = 0;
             ga.kb1 = KB;
@@ -484,6 +488,8 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, floa
             }
             CS_HIP(hipGetLastError());
             CS_HIP(hipEventRecord(e1, st));
+            // int8 MFMA ops issued: T1 over all SNP blocks plus each digit product over all blocks
+            mfma_ops += (double)(passes - passes0) * (double)quadrants * 64.0 * 64.0 * (double)Mpad * 2.0;
             CombineArgs ca{};
             ca.I0 = I0;
             ca.P = rows;
@@ -520,6 +526,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, floa
         g_grm_timing.ms_passes = ms_pass;
         g_grm_timing.ms_combine = ms_comb;
         g_grm_timing.passes = passes;
+        g_grm_timing.mfma_ops = mfma_ops;
     } catch (...) {
         (void)hipStreamSynchronize(st);
         (void)hipEventDestroy(e0);
@@ -552,11 +559,12 @@ int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float* d_grm_tri_f3
     });
 }
 
-int cs_grm_last_timing(double* ms_passes, double* ms_combine, int* passes) {
+int cs_grm_last_timing(double* ms_passes, double* ms_combine, int* passes, double* mfma_ops) {
     return guarded([&] {
         if (ms_passes) *ms_passes = g_grm_timing.ms_passes;
         if (ms_combine) *ms_combine = g_grm_timing.ms_combine;
         if (passes) *passes = g_grm_timing.passes;
+        if (mfma_ops) *mfma_ops = g_grm_timing.mfma_ops;
     });
 }
 

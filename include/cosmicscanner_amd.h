@@ -164,9 +164,10 @@ int cs_grm_build(cs_bed* bed, int alg, float min_maf, float* grm_tri_f32, double
 /* Same with the lower triangle written to device memory (n(n+1)/2 floats). */
 int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float* d_grm_tri_f32,
                         double* sum_2pq, int64_t* n_valid);
-/* Kernel time of the last GRM build (all int8 products; the int128 combine) and the number of
- * int8 product launches, from HIP events on the build's stream. */
-int cs_grm_last_timing(double* ms_products, double* ms_combine, int* launches);
+/* Kernel time of the last GRM build on this thread (all int8 products; the int128 combine), the
+ * number of int8 product launches and the int8 MFMA ops they issued, from HIP events on the
+ * build's stream. */
+int cs_grm_last_timing(double* ms_products, double* ms_combine, int* launches, double* mfma_ops);
 
 #ifdef __cplusplus
 }
