@@ -33,6 +33,7 @@ CS_PREC_FP32 = 32
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
+_f32p = _fp
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -89,6 +90,20 @@ def _load():
         "cs_grm_build": ([_vp, ctypes.c_int, ctypes.c_float, _fp, _dp, _dp, _i64p], ctypes.c_int),
         "cs_grm_build_device": ([_vp, ctypes.c_int, ctypes.c_float, _vp, _dp, _i64p], ctypes.c_int),
         "cs_grm_last_timing": ([_dp, _dp, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
+        "cs_grm_build_raw": ([_vp, ctypes.c_int, ctypes.c_float, _i32p, _i64, _vp, ctypes.c_int, _dp,
+                              _dp, _i64p], ctypes.c_int),
+        "cs_grm_loco_subset": ([_vp, _vp, _i64, _i32p, _i64, ctypes.c_double, ctypes.c_double, _vp],
+                               ctypes.c_int),
+        "cs_lmm_create": ([_dp, _f32p, _i64, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "cs_lmm_destroy": ([_vp], ctypes.c_int),
+        "cs_lmm_prepare_dense": ([_vp, _vp, _i64, ctypes.c_int], ctypes.c_int),
+        "cs_lmm_null_brent": ([_vp, _dp, _dp, _dp], ctypes.c_int),
+        "cs_lmm_set_variance": ([_vp, ctypes.c_double, ctypes.c_double], ctypes.c_int),
+        "cs_lmm_get_eigen": ([_vp, _dp, _dp, _dp, _i64], ctypes.c_int),
+        "cs_lmm_scan_rotated": ([_vp, _vp, _i32p, _i64, _i32p, _i64, ctypes.c_double, ctypes.c_int,
+                                 _dp, _i64p, _i32p, _dp, _dp, _dp, _dp], ctypes.c_int),
+        "cs_gwas_dense_run": ([ctypes.c_char_p, _dp, _dp, _i64, _i64, ctypes.c_void_p,
+                               ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
         f = getattr(L, name)
@@ -504,3 +519,157 @@ def grm_build_device(reader, d_tri_ptr, min_maf=0.01):
     _check(lib().cs_grm_build_device(reader._h, 1, float(min_maf), d_tri_ptr, ctypes.byref(s),
                                      ctypes.byref(nv)))
     return s.value, nv.value
+
+
+# ----------------------------------------------------------------------- dense null model + scan
+class DenseLmm:
+    """EigenVce (libCosmicCore/src/eigen_vce.cpp:19-113) + LmmEngine::scan
+    (src/lmm_engine.cpp:1363-1525) on the device, with covariate projection.
+
+    y: kept-sample phenotype (n,); X: (n, c) design including the intercept (cast to float32 as
+    the reference's Eigen::MatrixXf)."""
+
+    def __init__(self, y, X):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        X = np.asfortranarray(np.asarray(X, dtype=np.float32).reshape(y.size, -1))
+        self.n, self.c = y.size, X.shape[1]
+        h = _vp()
+        _check(lib().cs_lmm_create(_p(y, _dp), _p(X, _f32p), self.n, self.n, self.c, ctypes.byref(h)))
+        self._h = h
+        self.sigma2_g = self.sigma2_e = self.delta = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib().cs_lmm_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, G):
+        """EigenVce::prepare with an n x n GRM (numpy array or DeviceArray)."""
+        if isinstance(G, DeviceArray):
+            _check(lib().cs_lmm_prepare_dense(self._h, G.ptr, G.n, 1))
+        else:
+            Gf = np.asfortranarray(np.asarray(G, dtype=np.float64))
+            if Gf.shape != (self.n, self.n):
+                raise RuntimeError("EigenVce::prepare: Dimension mismatch")
+            _check(lib().cs_lmm_prepare_dense(self._h, Gf.ctypes.data_as(_vp), self.n, 0))
+
+    def runNullModel(self):
+        a, b, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().cs_lmm_null_brent(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(d)))
+        self.sigma2_g, self.sigma2_e, self.delta = a.value, b.value, d.value
+        return True
+
+    def set_variance(self, sigma2_g, sigma2_e):
+        _check(lib().cs_lmm_set_variance(self._h, float(sigma2_g), float(sigma2_e)))
+        self.sigma2_g, self.sigma2_e = sigma2_g, sigma2_e
+
+    def eigen(self, with_w=False):
+        ne = self.n - self.c
+        D = np.empty(ne)
+        Uty = np.empty(ne)
+        W = np.empty((self.n, ne), order="F") if with_w else None
+        _check(lib().cs_lmm_get_eigen(self._h, _p(D, _dp), _p(Uty, _dp), _p(W, _dp), self.n))
+        return (D, Uty, W) if with_w else (D, Uty)
+
+    def scan(self, reader, keep, min_maf=0.01, snps=None, prec=CS_PREC_INT8_L3):
+        keep = _opt_i32(keep)
+        snps = _opt_i32(snps)
+        ns = reader.num_snps if snps is None else snps.size
+        out = {k: np.empty(ns) for k in ("freq", "beta", "se", "p", "log10p")}
+        nobs = np.empty(ns, dtype=np.int64)
+        ps = np.empty(ns, dtype=np.int32)
+        _check(lib().cs_lmm_scan_rotated(self._h, reader._h, _p(keep, _i32p),
+                                         0 if keep is None else keep.size, _p(snps, _i32p),
+                                         0 if snps is None else snps.size, float(min_maf), int(prec),
+                                         _p(out["freq"], _dp), _p(nobs, _i64p), _p(ps, _i32p),
+                                         _p(out["beta"], _dp), _p(out["se"], _dp), _p(out["p"], _dp),
+                                         _p(out["log10p"], _dp)))
+        out.update(n_obs=nobs, passes=ps.astype(bool))
+        return out
+
+
+def grm_build_raw(reader, d_raw, min_maf=0.01, snps=None):
+    """Raw Z Z^T over a SNP subset into an N x N DeviceArray; returns (S_block, S_seq, n_valid)."""
+    snps = _opt_i32(snps)
+    a, b, nv = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+    _check(lib().cs_grm_build_raw(reader._h, 1, float(min_maf), _p(snps, _i32p),
+                                  0 if snps is None else snps.size, d_raw.ptr, 1, ctypes.byref(a),
+                                  ctypes.byref(b), ctypes.byref(nv)))
+    return a.value, b.value, nv.value
+
+
+def grm_loco_subset(d_raw, d_raw_chr, N, keep, S_global, S_chr, d_out):
+    keep = _opt_i32(keep)
+    _check(lib().cs_grm_loco_subset(d_raw.ptr, None if d_raw_chr is None else d_raw_chr.ptr, N,
+                                    _p(keep, _i32p), 0 if keep is None else keep.size,
+                                    float(S_global), float(S_chr), d_out.ptr))
+
+
+# ------------------------------------------------------------------------------- GWAS driver
+class GwasOptions(ctypes.Structure):
+    _fields_ = [("min_maf", ctypes.c_double), ("no_loco", ctypes.c_int),
+                ("scan_chr", ctypes.c_char_p), ("prec", ctypes.c_int)]
+
+
+class GwasSummary(ctypes.Structure):
+    _fields_ = [("n_samples", ctypes.c_int64), ("n_snps_grm", ctypes.c_int64),
+                ("n_tested", ctypes.c_int64), ("sum_2pq", ctypes.c_double),
+                ("loco", ctypes.c_int32), ("n_models", ctypes.c_int32),
+                ("model_label", (ctypes.c_char * 16) * 64),
+                ("sigma2_g", ctypes.c_double * 64), ("sigma2_e", ctypes.c_double * 64),
+                ("ms_grm", ctypes.c_double), ("ms_null", ctypes.c_double),
+                ("ms_scan", ctypes.c_double)]
+
+
+def read_pheno(path, fam, mpheno=1, missing=-9.0):
+    """Phenotype column `mpheno` (1-based after FID IID) aligned to the .fam rows
+    (PhenoReader::load / align, libCosmicCore/src/pheno_reader.cpp:13-63,139-160): NA / na / NaN /
+    nan / NAN, unparsable values and the missing code -9 become NaN; absent samples are NaN."""
+    vals = {}
+    with open(path) as f:
+        for ln in f:
+            t = ln.split()
+            if len(t) < 2 + mpheno:
+                continue
+            s = t[1 + mpheno]
+            if s in ("NA", "na", "NaN", "nan", "NAN"):
+                v = float("nan")
+            else:
+                try:
+                    v = float(s)
+                except ValueError:
+                    v = float("nan")
+            if v == v and abs(v - missing) < 1e-6:
+                v = float("nan")
+            vals[(t[0], t[1])] = v
+    return np.array([vals.get((r[0], r[1]), float("nan")) for r in fam])
+
+
+def gwas_dense_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=False, scan_chr=None,
+                   prec=CS_PREC_INT8_L3):
+    """The default dense PLINK GWAS of GwasScanner::runLocoScan end to end on the GPU; writes
+    out_prefix.assoc.txt / .scan.meta.txt / .cosmic.sumstats and returns the run summary."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    N = y.size
+    if cov is None:
+        covf, nc = None, 0
+    else:
+        covf = np.asfortranarray(np.asarray(cov, dtype=np.float64).reshape(N, -1))
+        nc = covf.shape[1]
+    o = GwasOptions(float(min_maf), int(bool(no_loco)),
+                    None if scan_chr is None else str(scan_chr).encode(), int(prec))
+    sm = GwasSummary()
+    _check(lib().cs_gwas_dense_run(str(bfile).encode(), _p(y, _dp), _p(covf, _dp), N, nc,
+                                   ctypes.byref(o), str(out_prefix).encode(), ctypes.byref(sm)))
+    k = sm.n_models
+    return dict(n_samples=sm.n_samples, n_snps_grm=sm.n_snps_grm, n_tested=sm.n_tested,
+                sum_2pq=sm.sum_2pq, loco=bool(sm.loco),
+                models=[(sm.model_label[i].value.decode(), sm.sigma2_g[i], sm.sigma2_e[i])
+                        for i in range(k)],
+                ms_grm=sm.ms_grm, ms_null=sm.ms_null, ms_scan=sm.ms_scan)

@@ -1,6 +1,7 @@
 """Build recipe of the HIP library ``libcsamd.so`` (gfx950 only, in-tree).
 
-``python -m cosmicscanner_amd.build`` compiles every ``csrc/*.hip`` with hipcc and links the
+``python -m cosmicscanner_amd.build`` compiles every ``csrc/*.hip`` (kernels) and ``csrc/*.cpp``
+(host driver code over the C-ABI) with hipcc and links the
 shared library next to this file.  ``__graft_entry__.build()`` calls :func:`build`.
 """
 from __future__ import annotations
@@ -39,7 +40,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

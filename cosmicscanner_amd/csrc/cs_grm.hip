@@ -289,21 +289,33 @@ __global__ void k_grm_combine(CombineArgs a) {
 struct GrmParams {
     std::vector<uint8_t> valid;
     std::vector<long long> A;      // mu_j 2^29 (0 for invalid SNPs), Mpad entries
-    double sum_2pq = 0.0;
+    double sum_2pq = 0.0;          // float 2pq summed in 2048-SNP block order (compute, :266-272)
+    double sum_2pq_seq = 0.0;      // ... summed SNP by SNP (computeSplit's per-chromosome sums)
     int64_t n_valid = 0;
+    int64_t kb_lo = 0, kb_hi = 0;  // 256-SNP blocks holding the valid SNPs
     bool any_missing = false;
 };
 
 // Per-SNP filter and float32 parameters of GrmBuilder::compute (grm_builder.cpp:167-272) from
-// the all-sample class counts.
-static GrmParams grm_params(cs_bed* b, float min_maf) {
+// the all-sample class counts; `snps` (optional) restricts the SNP set, as a chromosome does in
+// GrmBuilder::computeSplit (grm_builder.cpp:506-600, 684-713).
+static GrmParams grm_params(cs_bed* b, float min_maf, const int32_t* snps, int64_t n_snps) {
     GrmParams p;
     std::vector<int64_t> cnt((size_t)b->M * 4);
     CS_HIP(hipMemcpy(cnt.data(), b->counts.p, cnt.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
     p.valid.assign((size_t)b->Mpad, 0);
     p.A.assign((size_t)b->Mpad, 0);
+    std::vector<uint8_t> sel;
+    if (snps) {
+        sel.assign((size_t)b->M, 0);
+        for (int64_t i = 0; i < n_snps; ++i) {
+            if (snps[i] < 0 || snps[i] >= b->M) throw RuntimeError("GRM: variant index out of range");
+            sel[(size_t)snps[i]] = 1;
+        }
+    }
     std::vector<float> pq2((size_t)b->M, 0.0f);
     for (int64_t j = 0; j < b->M; ++j) {
+        if (snps && !sel[(size_t)j]) continue;
         const int64_t* c = &cnt[(size_t)j * 4];
         const long long n_obs = c[0] + c[2] + c[3];
         const long long sum_x = 2 * c[0] + c[2];
@@ -320,14 +332,24 @@ static GrmParams grm_params(cs_bed* b, float min_maf) {
         p.A[(size_t)j] = (long long)a;
         if (c[1] > 0) p.any_missing = true;
     }
-    double total = 0.0;
+    double total = 0.0, seq = 0.0;
+    int64_t lo = b->M, hi = -1;
     for (int64_t blk = 0; blk < b->M; blk += 2048) {
         double block = 0.0;
         for (int64_t j = blk; j < std::min<int64_t>(blk + 2048, b->M); ++j)
-            if (p.valid[(size_t)j]) { block += pq2[(size_t)j]; p.n_valid++; }
+            if (p.valid[(size_t)j]) {
+                block += pq2[(size_t)j];
+                seq += pq2[(size_t)j];
+                p.n_valid++;
+                lo = std::min(lo, j);
+                hi = std::max(hi, j);
+            }
         total += block;
     }
     p.sum_2pq = total;
+    p.sum_2pq_seq = seq;
+    p.kb_lo = hi >= 0 ? lo / 256 : 0;
+    p.kb_hi = hi >= 0 ? hi / 256 + 1 : 0;
     return p;
 }
 
@@ -356,14 +378,26 @@ static thread_local GrmTiming g_grm_timing;
 
 using namespace csamd;
 
-static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, float* out_tri_dev,
-                    double* out_raw_host, double* sum_2pq, int64_t* n_valid) {
+struct GrmOut {
+    float* tri_host = nullptr;     // n(n+1)/2 GCTA lower triangle
+    float* tri_dev = nullptr;
+    double* raw_host = nullptr;    // n x n raw Z Z^T, symmetric
+    double* raw_dev = nullptr;
+    double sum_2pq = 0.0, sum_2pq_seq = 0.0;
+    int64_t n_valid = 0;
+};
+
+static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int64_t n_snps, GrmOut& out) {
     if (alg != 1) throw InvalidArgument("GRM: only algorithm 1 (VanRaden, the default) is built here");
     ensure_stats(b);
     ensure_sample_major(b);
-    GrmParams prm = grm_params(b, min_maf);
-    if (sum_2pq) *sum_2pq = prm.sum_2pq;
-    if (n_valid) *n_valid = prm.n_valid;
+    GrmParams prm = grm_params(b, min_maf, snps, n_snps);
+    out.sum_2pq = prm.sum_2pq;
+    out.sum_2pq_seq = prm.sum_2pq_seq;
+    out.n_valid = prm.n_valid;
+    float* out_tri_host = out.tri_host;
+    float* out_tri_dev = out.tri_dev;
+    double* out_raw_host = out.raw_host;
     const int64_t N = b->N, Npad = b->Npad, Mpad = b->Mpad, KB = b->KBm;
     if (Mpad > (int64_t(1) << 29))   // x~ x~ products reach 4 M in int32
         throw InvalidArgument("GRM: more than 2^29 SNPs");
@@ -417,12 +451,16 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, floa
         // output
         DevBuf tri_own;
         float* tri = out_tri_dev;
-        if (!tri) {
+        if (!tri && out_tri_host) {
             tri_own.alloc((size_t)(N * (N + 1) / 2) * sizeof(float));
             tri = tri_own.as<float>();
         }
-        DevBuf raw_dev;
-        if (out_raw_host) raw_dev.alloc((size_t)(N * N) * sizeof(double));
+        DevBuf raw_own;
+        double* raw = out.raw_dev;
+        if (!raw && out_raw_host) {
+            raw_own.alloc((size_t)(N * N) * sizeof(double));
+            raw = raw_own.as<double>();
+        }
         // panels of P rows (multiple of 128) bounded to ~8 GB of accumulators
         // (CSAMD_GRM_PANEL_BYTES overrides the bound)
         size_t panel_bytes = 8ull << 30;
@@ -461,15 +499,15 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, floa
             for (const int2& bl : blocks) quadrants += bl.x == bl.y ? 3 : 4;
             const int passes0 = passes;
             CS_HIP(hipEventRecord(e0, st));
-            ga.kb0 = 0;
-            ga.kb1 = KB;
+            ga.kb0 = prm.kb_lo;
+            ga.kb1 = prm.kb_hi;
             k_grm_pass<OP_X, OP_X, GE_T1><<<nb, 256, 0, st>>>(ga);
             ++passes;
             // digit products accumulate |x~ d| <= 256 per SNP in int32: chunks of <= 2^22 SNPs
-            for (int64_t c0 = 0; c0 < KB; c0 += kDigitChunkKB) {
+            for (int64_t c0 = prm.kb_lo; c0 < prm.kb_hi; c0 += kDigitChunkKB) {
                 GrmArgs gc = ga;
                 gc.kb0 = c0;
-                gc.kb1 = std::min<int64_t>(KB, c0 + kDigitChunkKB);
+                gc.kb1 = std::min<int64_t>(prm.kb_hi, c0 + kDigitChunkKB);
                 for (int l = 0; l < nC; ++l) {
                     GrmArgs g2 = gc;
                     g2.shift = 8 * l;
@@ -489,7 +527,8 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, floa
             CS_HIP(hipGetLastError());
             CS_HIP(hipEventRecord(e1, st));
             // int8 MFMA ops issued: T1 over all SNP blocks plus each digit product over all blocks
-            mfma_ops += (double)(passes - passes0) * (double)quadrants * 64.0 * 64.0 * (double)Mpad * 2.0;
+            mfma_ops += (double)(passes - passes0) * (double)quadrants * 64.0 * 64.0 *
+                        (double)((prm.kb_hi - prm.kb_lo) * 256) * 2.0;
             CombineArgs ca{};
             ca.I0 = I0;
             ca.P = rows;
@@ -505,7 +544,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, floa
             ca.D53 = D53;
             ca.se = se;
             ca.tri = tri;
-            ca.raw = out_raw_host ? raw_dev.as<double>() : nullptr;
+            ca.raw = raw;
             k_grm_combine<<<grid_for(rows * Npad), 256, 0, st>>>(ca);
             CS_HIP(hipGetLastError());
             CS_HIP(hipEventRecord(e2, st));
@@ -520,7 +559,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float* out_tri_host, floa
             CS_HIP(hipMemcpyAsync(out_tri_host, tri, (size_t)(N * (N + 1) / 2) * sizeof(float),
                                   hipMemcpyDeviceToHost, st));
         if (out_raw_host)
-            CS_HIP(hipMemcpyAsync(out_raw_host, raw_dev.p, (size_t)(N * N) * sizeof(double),
+            CS_HIP(hipMemcpyAsync(out_raw_host, raw, (size_t)(N * N) * sizeof(double),
                                   hipMemcpyDeviceToHost, st));
         CS_HIP(hipStreamSynchronize(st));
         g_grm_timing.ms_passes = ms_pass;
@@ -547,7 +586,12 @@ int cs_grm_build(cs_bed* bed, int alg, float min_maf, float* grm_tri_f32, double
                  double* sum_2pq, int64_t* n_valid) {
     return guarded([&] {
         if (!bed) throw InvalidArgument("cs_grm_build: null handle");
-        grm_run(bed, alg, min_maf, grm_tri_f32, nullptr, grm_raw_f64, sum_2pq, n_valid);
+        GrmOut o;
+        o.tri_host = grm_tri_f32;
+        o.raw_host = grm_raw_f64;
+        grm_run(bed, alg, min_maf, nullptr, 0, o);
+        if (sum_2pq) *sum_2pq = o.sum_2pq;
+        if (n_valid) *n_valid = o.n_valid;
     });
 }
 
@@ -555,7 +599,25 @@ int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float* d_grm_tri_f3
                         int64_t* n_valid) {
     return guarded([&] {
         if (!bed || !d_grm_tri_f32) throw InvalidArgument("cs_grm_build_device: null argument");
-        grm_run(bed, alg, min_maf, nullptr, d_grm_tri_f32, nullptr, sum_2pq, n_valid);
+        GrmOut o;
+        o.tri_dev = d_grm_tri_f32;
+        grm_run(bed, alg, min_maf, nullptr, 0, o);
+        if (sum_2pq) *sum_2pq = o.sum_2pq;
+        if (n_valid) *n_valid = o.n_valid;
+    });
+}
+
+int cs_grm_build_raw(cs_bed* bed, int alg, float min_maf, const int32_t* snps, int64_t n_snps,
+                     double* raw_f64, int raw_on_device, double* sum_2pq_block,
+                     double* sum_2pq_seq, int64_t* n_valid) {
+    return guarded([&] {
+        if (!bed || !raw_f64) throw InvalidArgument("cs_grm_build_raw: null argument");
+        GrmOut o;
+        (raw_on_device ? o.raw_dev : o.raw_host) = raw_f64;
+        grm_run(bed, alg, min_maf, (snps && n_snps > 0) ? snps : nullptr, n_snps, o);
+        if (sum_2pq_block) *sum_2pq_block = o.sum_2pq;
+        if (sum_2pq_seq) *sum_2pq_seq = o.sum_2pq_seq;
+        if (n_valid) *n_valid = o.n_valid;
     });
 }
 

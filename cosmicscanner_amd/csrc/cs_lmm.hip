@@ -1,0 +1,377 @@
+// Dense LMM null model on the device and the rotated per-SNP scan that consumes it.
+//
+//   EigenVce::prepare with covariate projection (libCosmicCore/src/eigen_vce.cpp:19-49):
+//     full Householder Q of X (n x c), A = Q[:, c:], G_reml = A^T G A, eigendecomposition
+//     G_reml = U diag(D) U^T (D clamped at 0), W_proj = A U, Uty = U^T A^T y.   rocSOLVER / rocBLAS.
+//   VCESolver::solveDiagonal (libCosmicCore/src/vce.cpp:3073-3171), the default "brent" null
+//     solver: REML golden section on log10(delta) in [-5, 5] while |c - d| > 1e-5; one workgroup
+//     evaluates every likelihood with a fixed-order reduction.
+//   LmmEngine::scan (src/lmm_engine.cpp:1363-1525): score = (W_proj^T g)^T (V^-1 Uty),
+//     var = sum_l (W_proj^T g)_l^2 / (sigma_g D_l + sigma_e), through the fused scan kernel with
+//     R = [W_proj V^-1 Uty, W_proj] and quadratic weights V^-1.
+//   The dense LOCO GRM of GwasScanner::runLocoScan (src/gwas_scanner.cpp:481-495).
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "cs_common.h"
+#include "cs_ops.h"
+
+namespace csamd {
+
+#define CS_BLAS(call)                                                                   \
+    do {                                                                                \
+        rocblas_status s_ = (call);                                                     \
+        if (s_ != rocblas_status_success)                                               \
+            throw ::csamd::RuntimeError(std::string("rocBLAS/rocSOLVER error ") +      \
+                                        rocblas_status_to_string(s_) + " at " #call);   \
+    } while (0)
+
+__global__ void k_clamp_nonneg(double* __restrict__ d, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n && d[i] < 0.0) d[i] = 0.0;
+}
+
+// Two sums over the eigen-system for one delta: sum log(D + delta), sum Uty^2 / (D + delta);
+// each thread strides, then a fixed tree -- the same order on every call.
+__device__ void reml_sums(const double* __restrict__ D, const double* __restrict__ Uty, int64_t ne,
+                          double delta, double* sh, double& slog, double& squad) {
+    double a = 0.0, b = 0.0;
+    for (int64_t i = threadIdx.x; i < ne; i += blockDim.x) {
+        const double v = D[i] + delta;
+        a += log(v);
+        b += Uty[i] * Uty[i] / v;
+    }
+    sh[threadIdx.x] = a;
+    sh[blockDim.x + threadIdx.x] = b;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sh[threadIdx.x] += sh[threadIdx.x + w];
+            sh[blockDim.x + threadIdx.x] += sh[blockDim.x + threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    slog = sh[0];
+    squad = sh[blockDim.x];
+    __syncthreads();
+}
+
+// solveDiagonal with an empty UtX (covariates projected out): n_eff = ne,
+// -logL = 0.5 (sum log V + n_eff log(quad / n_eff) + n_eff).  out = {sigma_g, sigma_e, delta, evals}.
+__global__ __launch_bounds__(256) void k_reml_golden(const double* __restrict__ D,
+                                                     const double* __restrict__ Uty, int64_t ne,
+                                                     double* __restrict__ out) {
+    __shared__ double sh[512];
+    const double n_eff = (double)ne;
+    int evals = 0;
+    auto negll = [&](double delta) -> double {
+        double slog, squad;
+        reml_sums(D, Uty, ne, delta, sh, slog, squad);
+        ++evals;
+        return 0.5 * (slog + n_eff * log(squad / n_eff) + n_eff);
+    };
+    double lo = -5.0, hi = 5.0;
+    const double gr = (sqrt(5.0) + 1.0) / 2.0;
+    double c = hi - (hi - lo) / gr;
+    double d = lo + (hi - lo) / gr;
+    double fc = negll(pow(10.0, c));
+    double fd = negll(pow(10.0, d));
+    while (fabs(c - d) > 1e-5) {
+        if (fc < fd) {
+            hi = d;
+            d = c;
+            fd = fc;
+            c = hi - (hi - lo) / gr;
+            fc = negll(pow(10.0, c));
+        } else {
+            lo = c;
+            c = d;
+            fc = fd;
+            d = lo + (hi - lo) / gr;
+            fd = negll(pow(10.0, d));
+        }
+    }
+    const double best = pow(10.0, (lo + hi) / 2.0);
+    double slog, squad;
+    reml_sums(D, Uty, ne, best, sh, slog, squad);
+    if (threadIdx.x == 0) {
+        const double vg = squad / n_eff;
+        out[0] = vg;
+        out[1] = vg * best;
+        out[2] = best;
+        out[3] = evals;
+    }
+}
+
+// P_y = Uty / (sigma_g D + sigma_e) and the quadratic weights 1 / (sigma_g D + sigma_e)
+// (lmm_engine.cpp:1372-1373, 1404).
+__global__ void k_vinv(const double* __restrict__ D, const double* __restrict__ Uty, int64_t ne,
+                       double sg, double se, double* __restrict__ vinv, double* __restrict__ py) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ne) return;
+    const double v = 1.0 / (sg * D[i] + se);
+    vinv[i] = v;
+    py[i] = Uty[i] * v;
+}
+
+// Dense LOCO GRM over the keep list, in the reference's operation order:
+// G_global = raw / S (GrmBuilder's normalisation), G_sub = G_global[keep, keep], and with
+// S_chr > 0 and S_loco = S - S_chr > 1e-6: G_sub * (S / S_loco) - raw_chr[keep, keep] / S_loco.
+__global__ void k_loco(const double* __restrict__ raw, const double* __restrict__ raw_chr, int64_t N,
+                       const int32_t* __restrict__ keep, int64_t n, double S, double S_loco,
+                       bool loco, double* __restrict__ out) {
+    const double ratio = S / S_loco;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % n, j = e / n;
+        const int64_t src = (int64_t)keep[j] * N + keep[i];
+        const double g = raw[src] / S;
+        out[e] = loco ? g * ratio - raw_chr[src] / S_loco : g;
+    }
+}
+
+}  // namespace csamd
+
+using namespace csamd;
+
+struct cs_lmm {
+    int64_t n = 0, c = 0, ne = 0;
+    rocblas_handle blas = nullptr;
+    hipStream_t st = nullptr;
+    DevBuf Q;        // n x n: full Householder Q of X (A = Q[:, c:]); unused when c == 0
+    DevBuf y;        // n
+    DevBuf G;        // staging of a host G (n x n)
+    DevBuf T;        // n x ne work
+    DevBuf U;        // ne x ne eigenvectors
+    DevBuf D, E, Uty, t, vinv, py;
+    DevBuf R;        // n x (1 + ne): [W_proj P_y, W_proj]
+    DevBuf info, res;
+    bool prepared = false, have_null = false;
+    double sg = NAN, se = NAN, delta = NAN;
+    int evals = 0;
+    ~cs_lmm() {
+        if (blas) (void)rocblas_destroy_handle(blas);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+static void lmm_prepare(cs_lmm* h, const double* G, int64_t ldg, bool g_dev) {
+    const int64_t n = h->n, ne = h->ne, c = h->c;
+    if (ldg < n) throw RuntimeError("EigenVce::prepare: Dimension mismatch");
+    const double one = 1.0, zero = 0.0;
+    const double* Gd = G;
+    if (!g_dev) {
+        h->G.ensure((size_t)(n * n) * 8);
+        CS_HIP(hipMemcpy2DAsync(h->G.p, (size_t)n * 8, G, (size_t)ldg * 8, (size_t)n * 8, (size_t)n,
+                                hipMemcpyHostToDevice, h->st));
+        Gd = h->G.as<double>();
+        ldg = n;
+    }
+    h->U.ensure((size_t)(ne * ne) * 8);
+    h->T.ensure((size_t)(n * ne) * 8);
+    h->D.ensure((size_t)ne * 8);
+    h->E.ensure((size_t)ne * 8);
+    h->Uty.ensure((size_t)ne * 8);
+    h->t.ensure((size_t)ne * 8);
+    h->R.ensure((size_t)(n * (ne + 1)) * 8);
+    double* U = h->U.as<double>();
+    double* W = h->R.as<double>() + n;
+    if (c > 0) {
+        const double* A = h->Q.as<double>() + c * n;
+        // G_reml = A^T (G A)
+        CS_BLAS(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)ne,
+                              (int)n, &one, Gd, (int)ldg, A, (int)n, &zero, h->T.as<double>(), (int)n));
+        CS_BLAS(rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, (int)ne,
+                              (int)ne, (int)n, &one, A, (int)n, h->T.as<double>(), (int)n, &zero, U,
+                              (int)ne));
+        CS_BLAS(rocblas_dgemv(h->blas, rocblas_operation_transpose, (int)n, (int)ne, &one, A, (int)n,
+                              h->y.as<double>(), 1, &zero, h->t.as<double>(), 1));
+    } else {
+        CS_HIP(hipMemcpy2DAsync(U, (size_t)n * 8, Gd, (size_t)ldg * 8, (size_t)n * 8, (size_t)n,
+                                hipMemcpyDeviceToDevice, h->st));
+        CS_HIP(hipMemcpyAsync(h->t.p, h->y.p, (size_t)n * 8, hipMemcpyDeviceToDevice, h->st));
+    }
+    CS_BLAS(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)ne, U, (int)ne,
+                             h->D.as<double>(), h->E.as<double>(), h->info.as<rocblas_int>()));
+    k_clamp_nonneg<<<grid_for(ne), 256, 0, h->st>>>(h->D.as<double>(), ne);
+    CS_HIP(hipGetLastError());
+    // Uty = U^T (A^T y); W_proj = A U
+    CS_BLAS(rocblas_dgemv(h->blas, rocblas_operation_transpose, (int)ne, (int)ne, &one, U, (int)ne,
+                          h->t.as<double>(), 1, &zero, h->Uty.as<double>(), 1));
+    if (c > 0)
+        CS_BLAS(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)ne,
+                              (int)ne, &one, h->Q.as<double>() + c * n, (int)n, U, (int)ne, &zero, W,
+                              (int)n));
+    else
+        CS_HIP(hipMemcpyAsync(W, U, (size_t)(n * n) * 8, hipMemcpyDeviceToDevice, h->st));
+    rocblas_int info = 0;
+    CS_HIP(hipMemcpyAsync(&info, h->info.p, sizeof(info), hipMemcpyDeviceToHost, h->st));
+    CS_HIP(hipStreamSynchronize(h->st));
+    if (info != 0) throw RuntimeError("EigenVce::prepare: eigendecomposition did not converge");
+    h->prepared = true;
+    h->have_null = false;
+}
+
+extern "C" {
+
+int cs_lmm_create(const double* y, const float* X, int64_t ldx, int64_t n, int64_t c, cs_lmm** out) {
+    return guarded([&] {
+        if (!y || !out || (c > 0 && !X)) throw InvalidArgument("cs_lmm_create: null argument");
+        if (n < 2 || c < 0 || c >= n || (c > 0 && ldx < n))
+            throw RuntimeError("EigenVce::prepare: Dimension mismatch");
+        auto h = std::make_unique<cs_lmm>();
+        h->n = n;
+        h->c = c;
+        h->ne = n - c;
+        CS_HIP(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+        CS_BLAS(rocblas_create_handle(&h->blas));
+        CS_BLAS(rocblas_set_stream(h->blas, h->st));
+        h->info.alloc(sizeof(rocblas_int));
+        h->res.alloc(4 * sizeof(double));
+        h->y.alloc((size_t)n * 8);
+        CS_HIP(hipMemcpyAsync(h->y.p, y, (size_t)n * 8, hipMemcpyHostToDevice, h->st));
+        if (c > 0) {
+            // X is float32 (univariate_model.cpp:368-372); the QR runs on X.cast<double>()
+            std::vector<double> Xd((size_t)(n * c));
+            for (int64_t j = 0; j < c; ++j)
+                for (int64_t i = 0; i < n; ++i) Xd[(size_t)(j * n + i)] = (double)X[j * ldx + i];
+            h->Q.alloc((size_t)(n * n) * 8);
+            DevBuf tau((size_t)c * 8);
+            CS_HIP(hipMemcpyAsync(h->Q.p, Xd.data(), Xd.size() * 8, hipMemcpyHostToDevice, h->st));
+            CS_BLAS(rocsolver_dgeqrf(h->blas, (int)n, (int)c, h->Q.as<double>(), (int)n, tau.as<double>()));
+            CS_BLAS(rocsolver_dorgqr(h->blas, (int)n, (int)n, (int)c, h->Q.as<double>(), (int)n,
+                                     tau.as<double>()));
+            CS_HIP(hipStreamSynchronize(h->st));
+        }
+        CS_HIP(hipStreamSynchronize(h->st));
+        *out = h.release();
+    });
+}
+
+int cs_lmm_destroy(cs_lmm* h) {
+    return guarded([&] { delete h; });
+}
+
+int cs_lmm_prepare_dense(cs_lmm* h, const double* G, int64_t ldg, int g_on_device) {
+    return guarded([&] {
+        if (!h || !G) throw InvalidArgument("cs_lmm_prepare_dense: null argument");
+        lmm_prepare(h, G, ldg, g_on_device != 0);
+    });
+}
+
+int cs_lmm_null_brent(cs_lmm* h, double* sigma2_g, double* sigma2_e, double* delta) {
+    return guarded([&] {
+        if (!h) throw InvalidArgument("cs_lmm_null_brent: null handle");
+        if (!h->prepared) throw RuntimeError("null model: prepare() has not run");
+        k_reml_golden<<<1, 256, 0, h->st>>>(h->D.as<double>(), h->Uty.as<double>(), h->ne,
+                                            h->res.as<double>());
+        CS_HIP(hipGetLastError());
+        double r[4];
+        CS_HIP(hipMemcpyAsync(r, h->res.p, sizeof(r), hipMemcpyDeviceToHost, h->st));
+        CS_HIP(hipStreamSynchronize(h->st));
+        h->sg = r[0];
+        h->se = r[1];
+        h->delta = r[2];
+        h->evals = (int)r[3];
+        // null_delta = sigma_e / sigma_g (eigen_vce.cpp:105)
+        h->have_null = true;
+        if (sigma2_g) *sigma2_g = h->sg;
+        if (sigma2_e) *sigma2_e = h->se;
+        if (delta) *delta = h->se / h->sg;
+    });
+}
+
+int cs_lmm_set_variance(cs_lmm* h, double sigma2_g, double sigma2_e) {
+    return guarded([&] {
+        if (!h) throw InvalidArgument("cs_lmm_set_variance: null handle");
+        if (!h->prepared) throw RuntimeError("null model: prepare() has not run");
+        h->sg = sigma2_g;
+        h->se = sigma2_e;
+        h->have_null = true;
+    });
+}
+
+int cs_lmm_get_eigen(cs_lmm* h, double* D, double* Uty, double* W_proj, int64_t ldw) {
+    return guarded([&] {
+        if (!h) throw InvalidArgument("cs_lmm_get_eigen: null handle");
+        if (!h->prepared) throw RuntimeError("null model: prepare() has not run");
+        if (D) CS_HIP(hipMemcpyAsync(D, h->D.p, (size_t)h->ne * 8, hipMemcpyDeviceToHost, h->st));
+        if (Uty) CS_HIP(hipMemcpyAsync(Uty, h->Uty.p, (size_t)h->ne * 8, hipMemcpyDeviceToHost, h->st));
+        if (W_proj) {
+            if (ldw < h->n) throw RuntimeError("cs_lmm_get_eigen: dimension mismatch");
+            CS_HIP(hipMemcpy2DAsync(W_proj, (size_t)ldw * 8, h->R.as<double>() + h->n, (size_t)h->n * 8,
+                                    (size_t)h->n * 8, (size_t)h->ne, hipMemcpyDeviceToHost, h->st));
+        }
+        CS_HIP(hipStreamSynchronize(h->st));
+    });
+}
+
+int cs_lmm_scan_rotated(cs_lmm* h, cs_bed* bed, const int32_t* keep, int64_t n_keep,
+                        const int32_t* snps, int64_t n_snps, double min_maf, int prec, double* freq,
+                        int64_t* n_obs, int32_t* pass, double* beta, double* se, double* p,
+                        double* log10p) {
+    return guarded([&] {
+        if (!h || !bed) throw InvalidArgument("cs_lmm_scan_rotated: null argument");
+        if (!h->have_null) throw RuntimeError("scan: the null model has not been fitted");
+        const int64_t n = h->n, ne = h->ne;
+        if ((keep && n_keep > 0 ? n_keep : bed->N) != n) throw RuntimeError("scan: dimension mismatch");
+        h->vinv.ensure((size_t)ne * 8);
+        h->py.ensure((size_t)ne * 8);
+        k_vinv<<<grid_for(ne), 256, 0, h->st>>>(h->D.as<double>(), h->Uty.as<double>(), ne, h->sg, h->se,
+                                                h->vinv.as<double>(), h->py.as<double>());
+        CS_HIP(hipGetLastError());
+        const double one = 1.0, zero = 0.0;
+        CS_BLAS(rocblas_dgemv(h->blas, rocblas_operation_none, (int)n, (int)ne, &one,
+                              h->R.as<double>() + n, (int)n, h->py.as<double>(), 1, &zero,
+                              h->R.as<double>(), 1));
+        std::vector<double> w((size_t)ne);
+        CS_HIP(hipMemcpyAsync(w.data(), h->vinv.p, (size_t)ne * 8, hipMemcpyDeviceToHost, h->st));
+        CS_HIP(hipStreamSynchronize(h->st));
+        cs_scan_model m{1, 0.0, 1, 1 + ne, w.data(), 0, 0, nullptr};
+        ScanArgs a;
+        a.keep = (keep && n_keep > 0) ? keep : nullptr;
+        a.n = n;
+        a.R = h->R.as<double>();
+        a.ldr = n;
+        a.nr = 1 + ne;
+        a.min_maf = min_maf;
+        a.snps = (snps && n_snps > 0) ? snps : nullptr;
+        a.n_snps = a.snps ? n_snps : bed->M;
+        a.prec = prec;
+        a.device_R = true;
+        ScanOut o;
+        o.freq = freq; o.n_obs = n_obs; o.pass = pass;
+        o.beta = beta; o.se = se; o.p = p; o.log10p = log10p;
+        run_scan(bed, a, &m, o);
+    });
+}
+
+int cs_grm_loco_subset(const double* d_raw, const double* d_raw_chr, int64_t N, const int32_t* keep,
+                       int64_t n_keep, double S_global, double S_chr, double* d_out) {
+    return guarded([&] {
+        if (!d_raw || !d_out) throw InvalidArgument("cs_grm_loco_subset: null argument");
+        std::vector<int32_t> all;
+        if (!keep || n_keep <= 0) {
+            all.resize((size_t)N);
+            for (int64_t i = 0; i < N; ++i) all[(size_t)i] = (int32_t)i;
+            keep = all.data();
+            n_keep = N;
+        }
+        for (int64_t i = 0; i < n_keep; ++i)
+            if (keep[i] < 0 || keep[i] >= N) throw RuntimeError("LOCO GRM: sample index out of range");
+        const double S_loco = S_global - S_chr;
+        const bool loco = d_raw_chr && S_chr > 0.0 && S_loco > 1e-6;
+        DevBuf dk((size_t)n_keep * 4);
+        CS_HIP(hipMemcpy(dk.p, keep, (size_t)n_keep * 4, hipMemcpyHostToDevice));
+        k_loco<<<grid_for(n_keep * n_keep), 256>>>(d_raw, d_raw_chr, N, dk.as<int32_t>(), n_keep, S_global,
+                                                   S_loco, loco, d_out);
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipDeviceSynchronize());
+    });
+}
+
+}  // extern "C"

@@ -38,6 +38,27 @@ void ensure_stats(cs_bed* b);
 void ensure_sample_major(cs_bed* b);
 void weighted_counts(cs_bed* b, const int32_t* d_w_npad, int64_t* d_counts, hipStream_t st);
 
+// Per-SNP scan over the keep list (cs_scan.hip): marker preparation, g^T R in int8 MFMA, and --
+// with a model -- the fused statistics.  R is host memory, or device memory when device_R.
+struct ScanArgs {
+    const int32_t* keep = nullptr;
+    int64_t n = 0;
+    const double* R = nullptr;
+    int64_t ldr = 0, nr = 0;
+    double min_maf = 0.0;
+    const int32_t* snps = nullptr;
+    int64_t n_snps = 0;
+    int prec = CS_PREC_INT8_L3;
+    bool device_R = false;
+};
+struct ScanOut {
+    double *freq = nullptr, *dot = nullptr, *gtg = nullptr;
+    int64_t* n_obs = nullptr;
+    int32_t* pass = nullptr;
+    double *beta = nullptr, *se = nullptr, *p = nullptr, *log10p = nullptr;
+};
+void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const ScanOut& o);
+
 __global__ void k_scatter(const double* __restrict__ V, int64_t ldv, int64_t k,
                           const int32_t* __restrict__ inv_off, const int32_t* __restrict__ inv_idx,
                           int64_t N, int64_t Npad, double* __restrict__ Vt);

@@ -139,26 +139,7 @@ __global__ void k_scan_epilogue(const int32_t* __restrict__ snps, int64_t n_snps
     }
 }
 
-struct ScanArgs {
-    const int32_t* keep = nullptr;
-    int64_t n = 0;
-    const double* R = nullptr;
-    int64_t ldr = 0, nr = 0;
-    double min_maf = 0.0;
-    const int32_t* snps = nullptr;
-    int64_t n_snps = 0;
-    int prec = CS_PREC_INT8_L3;
-    bool device_R = false;
-};
-
-struct ScanOut {
-    double *freq = nullptr, *dot = nullptr, *gtg = nullptr;
-    int64_t* n_obs = nullptr;
-    int32_t* pass = nullptr;
-    double *beta = nullptr, *se = nullptr, *p = nullptr, *log10p = nullptr;
-};
-
-static void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const ScanOut& o) {
+void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const ScanOut& o) {
     if (a.nr < 1) throw InvalidArgument("scan: need at least one projected column");
     ensure_stats(b);
     const int64_t N = b->N, M = b->M;

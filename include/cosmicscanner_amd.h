@@ -34,6 +34,7 @@ extern "C" {
 
 typedef struct cs_bed cs_bed;
 typedef struct cs_kinship cs_kinship;
+typedef struct cs_lmm cs_lmm;
 
 /* Thread-local message of the last failed call on this thread ("" if none). */
 const char* cs_last_error(void);
@@ -164,10 +165,81 @@ int cs_grm_build(cs_bed* bed, int alg, float min_maf, float* grm_tri_f32, double
 /* Same with the lower triangle written to device memory (n(n+1)/2 floats). */
 int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float* d_grm_tri_f32,
                         double* sum_2pq, int64_t* n_valid);
+/* Raw Z Z^T (the correctly rounded FP64 of the exact integer product, n x n, symmetric) over a
+ * SNP subset -- a chromosome's per_chrom_raw in GrmBuilder::computeSplit
+ * (grm_builder.cpp:401-785, accumulated at :703-713) -- with the same per-SNP filter and float32
+ * means as cs_grm_build.  snps NULL/0 = all SNPs.  raw_f64 is host memory, or device memory when
+ * raw_on_device != 0.  sum_2pq_block: the float 2pq of the selected valid SNPs summed in 2048-SNP
+ * block order (the global normaliser, :266-272); sum_2pq_seq: the same summed SNP by SNP (the
+ * per-chromosome per_chrom_sum_2pq, :718-724). */
+int cs_grm_build_raw(cs_bed* bed, int alg, float min_maf, const int32_t* snps, int64_t n_snps,
+                     double* raw_f64, int raw_on_device, double* sum_2pq_block,
+                     double* sum_2pq_seq, int64_t* n_valid);
 /* Kernel time of the last GRM build on this thread (all int8 products; the int128 combine), the
  * number of int8 product launches and the int8 MFMA ops they issued, from HIP events on the
  * build's stream. */
 int cs_grm_last_timing(double* ms_products, double* ms_combine, int* launches, double* mfma_ops);
+
+/* ---- Dense null model and rotated scan (the default --make-grm path of
+ * GwasScanner::runLocoScan, src/gwas_scanner.cpp:74-525).
+ *
+ * cs_lmm_create: y (n) and covariates X (n x c, float32 column-major with leading dimension ldx,
+ * intercept included -- univariate_model.cpp:368-372) over the kept samples.  With c > 0 the
+ * covariates are projected out (EigenVce with use_covariate_projection, the default): the full
+ * Householder Q of X is formed once and reused by every prepare. */
+int cs_lmm_create(const double* y, const float* X, int64_t ldx, int64_t n, int64_t c, cs_lmm** out);
+int cs_lmm_destroy(cs_lmm* lmm);
+/* EigenVce::prepare (libCosmicCore/src/eigen_vce.cpp:19-49) for an n x n GRM G over the kept
+ * samples (host memory, or device memory when g_on_device != 0): G_reml = A^T G A,
+ * eigendecomposition (ascending, clamped at 0), W_proj = A U, Uty = U^T A^T y. */
+int cs_lmm_prepare_dense(cs_lmm* lmm, const double* G, int64_t ldg, int g_on_device);
+/* EigenVce::runNullModel with the default Brent solver = VCESolver::solveDiagonal
+ * (libCosmicCore/src/vce.cpp:3073-3171): golden section on log10(delta) in [-5, 5], tol 1e-5.
+ * Returns sigma_g^2, sigma_e^2 and delta = sigma_e^2 / sigma_g^2 (any may be NULL). */
+int cs_lmm_null_brent(cs_lmm* lmm, double* sigma2_g, double* sigma2_e, double* delta);
+/* Use given variance components instead of fitting them. */
+int cs_lmm_set_variance(cs_lmm* lmm, double sigma2_g, double sigma2_e);
+/* D (n-c), Uty (n-c) and W_proj (n x (n-c), leading dimension ldw); any may be NULL. */
+int cs_lmm_get_eigen(cs_lmm* lmm, double* D, double* Uty, double* W_proj, int64_t ldw);
+/* LmmEngine::scan (src/lmm_engine.cpp:1363-1525) with the fitted null model: per SNP in `snps`
+ * (NULL/0 = all), over the keep list (the rows of y / X / G, in that order): the prepared marker
+ * g, score = (W_proj^T g)^T V^-1 Uty, var = sum_l (W_proj^T g)_l^2 / (sigma_g D_l + sigma_e),
+ * beta, se, p, log10p as cs_lmm_scan (NaN when not tested). */
+int cs_lmm_scan_rotated(cs_lmm* lmm, cs_bed* bed, const int32_t* keep, int64_t n_keep,
+                        const int32_t* snps, int64_t n_snps, double min_maf, int prec, double* freq,
+                        int64_t* n_obs, int32_t* pass, double* beta, double* se, double* p,
+                        double* log10p);
+/* Dense LOCO GRM over the keep list from device raw GRMs (src/gwas_scanner.cpp:481-495):
+ * G = raw / S_global restricted to keep x keep; when d_raw_chr is non-NULL, S_chr > 0 and
+ * S_loco = S_global - S_chr > 1e-6: G * (S_global / S_loco) - raw_chr[keep, keep] / S_loco.
+ * d_raw / d_raw_chr: N x N device; d_out: n_keep x n_keep device; keep NULL/0 = all samples. */
+int cs_grm_loco_subset(const double* d_raw, const double* d_raw_chr, int64_t N, const int32_t* keep,
+                       int64_t n_keep, double S_global, double S_chr, double* d_out);
+
+/* ---- GWAS driver: the default dense PLINK path of GwasScanner::runLocoScan
+ * (src/gwas_scanner.cpp:74-525) end to end on the GPU.  Reads bfile.{bed,bim,fam}; y has one
+ * entry per .fam row (NaN = missing), cov is N x n_cov column-major (leading dimension ldc, NaN =
+ * missing; an intercept is added).  Writes out_prefix.assoc.txt, .scan.meta.txt and
+ * .cosmic.sumstats in the reference's formats.  LOCO over the .bim chromosome labels in
+ * lexicographic order when there is more than one and no_loco is 0. */
+typedef struct cs_gwas_options {
+    double min_maf;        /* --maf, default 0.01 */
+    int no_loco;           /* --no-loco */
+    const char* scan_chr;  /* scan only this chromosome (NULL = all) */
+    int prec;              /* scan precision, CS_PREC_INT8_L3 by default */
+} cs_gwas_options;
+#define CS_GWAS_MAX_MODELS 64
+typedef struct cs_gwas_summary {
+    int64_t n_samples, n_snps_grm, n_tested;
+    double sum_2pq;
+    int32_t loco, n_models;
+    char model_label[CS_GWAS_MAX_MODELS][16];   /* "GLOBAL" or the chromosome label */
+    double sigma2_g[CS_GWAS_MAX_MODELS], sigma2_e[CS_GWAS_MAX_MODELS];
+    double ms_grm, ms_null, ms_scan;            /* wall time of the stages */
+} cs_gwas_summary;
+int cs_gwas_dense_run(const char* bfile, const double* y, const double* cov, int64_t ldc,
+                      int64_t n_cov, const cs_gwas_options* opt, const char* out_prefix,
+                      cs_gwas_summary* summary);
 
 #ifdef __cplusplus
 }

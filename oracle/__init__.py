@@ -347,3 +347,82 @@ def _stats_from(score, var):
     with np.errstate(divide="ignore"):
         log10p = np.where(ok, np.where(p > 0, -np.log10(np.where(p > 0, p, 1.0)), 300.0), np.nan)
     return dict(score=score, var=var, beta=beta, se=se, p=p, log10p=log10p, tested=ok)
+
+
+# ------------------------------------------------------------------------ dense GWAS pipeline
+def sequential_sum_2pq(bed, N, M, min_maf=0.01, snps=None):
+    """computeSplit's per-chromosome normaliser: the float 2pq of the valid SNPs summed one SNP at
+    a time in double (grm_builder.cpp:718-724)."""
+    valid, _, _, pq2, _, nv = grm_snp_params(bed, N, M, 1, min_maf)
+    idx = range(M) if snps is None else sorted(snps)
+    s = 0.0
+    for j in idx:
+        if valid[j]:
+            s += float(pq2[j])
+    return s
+
+
+def select_snps(bed, N, snps):
+    """Body bytes of a SNP subset (rows of the SNP-major matrix)."""
+    B = (N + 3) // 4
+    a = np.asarray(bed, dtype=np.uint8).reshape(-1, B)
+    return np.ascontiguousarray(a[np.asarray(snps, dtype=np.int64)]).reshape(-1)
+
+
+def gwas_dense(bed, N, M, chroms, y, cov=None, min_maf=0.01, no_loco=False, scan_chr=None):
+    """GwasScanner::runLocoScan's default dense PLINK path (src/gwas_scanner.cpp:74-525) restated
+    on the CPU: DataFilter keep (finite y and float32 covariates), X = [1, cov] as float32, exact
+    GRM and per-chromosome raw GRMs, LOCO formula (:481-495), EigenVce::prepare + solveDiagonal,
+    rotated scan per chromosome (chromosome labels in lexicographic order).
+
+    Returns (models [(label, vg, ve)], rows [(snp_index, freq, beta, se, p, log10p)]) with rows
+    in output order and untested SNPs dropped."""
+    y = np.asarray(y, dtype=np.float64)
+    covf = None if cov is None else np.asarray(cov, dtype=np.float64).reshape(N, -1)
+    ok = ~np.isnan(y)
+    if covf is not None:
+        ok &= ~np.isnan(covf.astype(np.float32)).any(axis=1)
+    keep = np.nonzero(ok)[0].astype(np.int32)
+    n = keep.size
+    if n < 10:
+        raise RuntimeError("Too few valid samples.")
+    X = np.ones((n, 1 + (0 if covf is None else covf.shape[1])), dtype=np.float32)
+    if covf is not None:
+        X[:, 1:] = covf[keep].astype(np.float32)
+    Xd = X.astype(np.float64)
+    ys = y[keep]
+    raw, _, S, _ = grm_exact_centred(bed, N, M, min_maf)
+    G_global = raw / S
+    labels = sorted(set(chroms))
+    use_loco = len(labels) > 1 and not no_loco
+    models, rows = [], []
+
+    def run(Gs, snps, label):
+        D, W, Uty = eigen_vce_prepare(Gs, ys, Xd)
+        vg, ve, _ = solve_diagonal(D, Uty)
+        models.append((label, vg, ve))
+        Vinv = 1.0 / (vg * D + ve)
+        R = np.column_stack([W @ (Uty * Vinv), W])
+        freq, _, ps, dot, _ = scan_sums(bed, N, M, keep, R, min_maf)
+        st = _stats_from(dot[:, 0], (dot[:, 1:] ** 2) @ Vinv)
+        for j in snps:
+            if ps[j] and st["tested"][j]:
+                rows.append((j, freq[j], st["beta"][j], st["se"][j], st["p"][j], st["log10p"][j]))
+
+    Gsub = G_global[np.ix_(keep, keep)]
+    if not use_loco:
+        snps = [j for j in range(M) if scan_chr is None or chroms[j] == scan_chr]
+        run(Gsub, snps, scan_chr or "GLOBAL")
+    else:
+        for c in labels:
+            if scan_chr is not None and c != scan_chr:
+                continue
+            snps = [j for j in range(M) if chroms[j] == c]
+            raw_c = grm_exact_centred(select_snps(bed, N, snps), N, len(snps), min_maf)[0]
+            S_chr = sequential_sum_2pq(bed, N, M, min_maf, snps)
+            S_loco = S - S_chr
+            G = Gsub
+            if S_chr > 0.0 and S_loco > 1e-6:
+                G = Gsub * (S / S_loco) - raw_c[np.ix_(keep, keep)] / S_loco
+            run(G, snps, c)
+    return models, rows

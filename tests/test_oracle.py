@@ -279,3 +279,52 @@ def test_gen_hwe_moments():
     # padding bits are zero
     bed2 = oracle.gen_hwe_bed(1, 5, 3, 0.1, 0.5, 0.0)
     assert np.all(bed2.reshape(3, 2)[:, 1] >> 2 == 0)
+
+
+def test_loco_formula_equals_complement_grm(ref_datasets):
+    # gwas_scanner.cpp:481-495: G * (S / S_loco) - raw_c / S_loco must equal the GRM of the other
+    # chromosomes' SNPs, (raw - raw_c) / S_loco -- computed here from scratch on the complement.
+    body, N, M, bim, _ = ref_datasets["gemma"]
+    chroms = [b[0] for b in bim]
+    raw, _, S, _ = oracle.grm_exact_centred(body, N, M)
+    for c in sorted(set(chroms)):
+        on = [j for j in range(M) if chroms[j] == c]
+        off = [j for j in range(M) if chroms[j] != c]
+        raw_c = oracle.grm_exact_centred(oracle.select_snps(body, N, on), N, len(on))[0]
+        raw_o, _, S_o, _ = oracle.grm_exact_centred(oracle.select_snps(body, N, off), N, len(off))
+        S_c = oracle.sequential_sum_2pq(body, N, M, 0.01, on)
+        G_loco = (raw / S) * (S / (S - S_c)) - raw_c / (S - S_c)
+        assert abs((S - S_c) - S_o) <= 1e-12 * S
+        assert rel(G_loco, raw_o / S_o) < 1e-12
+
+
+def test_gwas_dense_reference_readme_row_count(ref_datasets):
+    # README.md:63: the bundled synth fixture (60 x 180) gives 180 association rows.
+    import os
+    import cosmicscanner_amd as csa
+    body, N, M, bim, fam = ref_datasets["synth"]
+    prefix = os.path.join(os.path.dirname(__file__), "golden", "ref_synth", "synth")
+    y = csa.read_pheno(prefix[:-5] + "quantitative.pheno", fam)
+    models, rows = oracle.gwas_dense(body, N, M, [b[0] for b in bim], y)
+    assert len(rows) == 180
+    assert [m[0] for m in models] == ["1", "2"]
+
+
+def test_gwas_dense_gemma_truth(ref_datasets):
+    # tests/data/gemma_accuracy/gemma_accuracy.truth.txt: simulated causal SNPs and effects; the
+    # two large effects are among the top hits with the right sign, the third has the right sign.
+    import os
+    import cosmicscanner_amd as csa
+    body, N, M, bim, fam = ref_datasets["gemma"]
+    prefix = os.path.join(os.path.dirname(__file__), "golden", "ref_gemma_accuracy", "gemma_accuracy")
+    y = csa.read_pheno(prefix + ".pheno", fam)
+    _, rows = oracle.gwas_dense(body, N, M, [b[0] for b in bim], y)
+    by_id = {bim[r[0]][1]: r for r in rows}
+    truth = {}
+    for ln in open(prefix + ".truth.txt").read().splitlines()[1:]:
+        k, v = ln.split()
+        truth[k] = float(v)
+    rank = [bim[r[0]][1] for r in sorted(rows, key=lambda r: r[4])]
+    for snp, b in truth.items():
+        assert np.sign(by_id[snp][2]) == np.sign(b)
+    assert rank.index("rs000013") < 5 and rank.index("rs000212") < 5
