@@ -102,6 +102,18 @@ def _load():
         "cs_lmm_get_eigen": ([_vp, _dp, _dp, _dp, _i64], ctypes.c_int),
         "cs_lmm_scan_rotated": ([_vp, _vp, _i32p, _i64, _i32p, _i64, ctypes.c_double, ctypes.c_int,
                                  _dp, _i64p, _i32p, _dp, _dp, _dp, _dp], ctypes.c_int),
+        "cs_pcg_solve": ([_vp, ctypes.c_double, ctypes.c_double, _vp, _i64, _i64, _vp, _i64,
+                          ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                          ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
+        "cs_pcg_solve_cols": ([_vp, _dp, _dp, _vp, _i64, _i64, _vp, _i64, ctypes.c_double,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                               _dp], ctypes.c_int),
+        "cs_markers_device": ([_vp, _i32p, _i64, _i32p, _i64, ctypes.c_double, _vp, _i64, _i32p, _dp],
+                              ctypes.c_int),
+        "cs_pcg_projected_scan": ([_vp, _vp, _i32p, _i64, _dp, _i64, _i64, _f32p, _i64, _i64,
+                                   ctypes.c_double, ctypes.c_double, _i32p, _i64, _i32p, _i64,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp,
+                                   _i64p, _i32p, _dp, _dp, _dp, _dp, ctypes.c_void_p], ctypes.c_int),
         "cs_gwas_dense_run": ([ctypes.c_char_p, _dp, _dp, _i64, _i64, ctypes.c_void_p,
                                ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
     }
@@ -365,6 +377,18 @@ class PlinkKinshipOperator:
         t = np.empty(5)
         _check(lib().cs_kinship_last_timing(self._h, _p(t, _dp)))
         return t
+
+
+def pcg_solve(op, sigma_g, sigma_e, dB, dX, rel_tol=1e-6, max_iter=200, precond=1, warm_start=False):
+    """(sigma_g K + sigma_e I) X = B on device buffers (DeviceArray, n x k); returns
+    (iterations, true relative residual per column)."""
+    k = dB.k
+    it = ctypes.c_int()
+    res = np.empty(k)
+    _check(lib().cs_pcg_solve(op._h, float(sigma_g), float(sigma_e), dB.ptr, dB.n, k, dX.ptr, dX.n,
+                              float(rel_tol), int(max_iter), int(precond), int(bool(warm_start)),
+                              ctypes.byref(it), _p(res, _dp)))
+    return it.value, res
 
 
 def createKinshipOperator(reader, keep_indices=None, variant_indices=None):
@@ -673,3 +697,50 @@ def gwas_dense_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=False, 
                 models=[(sm.model_label[i].value.decode(), sm.sigma2_g[i], sm.sigma2_e[i])
                         for i in range(k)],
                 ms_grm=sm.ms_grm, ms_null=sm.ms_null, ms_scan=sm.ms_scan)
+
+
+# --------------------------------------------------------------------- PCG-projected scan
+class PscanInfo(ctypes.Structure):
+    _fields_ = [("gamma", ctypes.c_double), ("n_calibration", ctypes.c_int64),
+                ("pcg_iterations", ctypes.c_int), ("max_rel_residual", ctypes.c_double)]
+
+
+def markers_device(reader, keep, snps, d_G, min_maf=0.01):
+    """Prepared markers of `snps` over `keep` into d_G (DeviceArray n x len(snps)); returns (pass, freq)."""
+    keep = _opt_i32(keep)
+    snps = np.ascontiguousarray(snps, dtype=np.int32)
+    ps = np.empty(snps.size, dtype=np.int32)
+    fr = np.empty(snps.size)
+    _check(lib().cs_markers_device(reader._h, _p(keep, _i32p), 0 if keep is None else keep.size,
+                                   _p(snps, _i32p), snps.size, float(min_maf), d_G.ptr, d_G.n,
+                                   _p(ps, _i32p), _p(fr, _dp)))
+    return ps.astype(bool), fr
+
+
+def pcg_projected_scan(op, reader, keep, Y, X, sigma_g, sigma_e, cal_snps, snps=None, min_maf=0.01,
+                       rel_tol=1e-6, max_iter=200, prec=CS_PREC_INT8_L3):
+    """PCG-projected GLS scan with kinship operator `op` (see cs_pcg_projected_scan)."""
+    keep = _opt_i32(keep)
+    snps = _opt_i32(snps)
+    n = op.rows()
+    Yf = np.asfortranarray(np.asarray(Y, dtype=np.float64).reshape(n, -1))
+    Xf = np.asfortranarray(np.asarray(X, dtype=np.float32).reshape(n, -1))
+    T, c = Yf.shape[1], Xf.shape[1]
+    cal = np.ascontiguousarray(cal_snps, dtype=np.int32)
+    ns = reader.num_snps if snps is None else snps.size
+    out = {k: np.empty((ns, T)) for k in ("beta", "se", "p", "log10p")}
+    freq = np.empty(ns)
+    nobs = np.empty(ns, dtype=np.int64)
+    ps = np.empty(ns, dtype=np.int32)
+    info = PscanInfo()
+    _check(lib().cs_pcg_projected_scan(op._h, reader._h, _p(keep, _i32p), 0 if keep is None else keep.size,
+                                       _p(Yf, _dp), n, T, _p(Xf, _f32p), n, c, float(sigma_g),
+                                       float(sigma_e), _p(cal, _i32p), cal.size, _p(snps, _i32p),
+                                       0 if snps is None else snps.size, float(min_maf), float(rel_tol),
+                                       int(max_iter), int(prec), _p(freq, _dp), _p(nobs, _i64p),
+                                       _p(ps, _i32p), _p(out["beta"], _dp), _p(out["se"], _dp),
+                                       _p(out["p"], _dp), _p(out["log10p"], _dp), ctypes.byref(info)))
+    out.update(freq=freq, n_obs=nobs, passes=ps.astype(bool), gamma=info.gamma,
+               n_calibration=info.n_calibration, pcg_iterations=info.pcg_iterations,
+               max_rel_residual=info.max_rel_residual)
+    return out

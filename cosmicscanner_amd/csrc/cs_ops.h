@@ -58,6 +58,10 @@ struct ScanOut {
     double *beta = nullptr, *se = nullptr, *p = nullptr, *log10p = nullptr;
 };
 void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const ScanOut& o);
+// Prepared markers of `snps` over the keep list (NULL = all samples) into dG (n x ns, device).
+void markers_device(cs_bed* b, const int32_t* keep, int64_t n_keep, const int32_t* snps, int64_t ns,
+                    double min_maf, double* dG, int64_t ldg, int32_t* pass_out, double* freq_out,
+                    hipStream_t st);
 
 __global__ void k_scatter(const double* __restrict__ V, int64_t ldv, int64_t k,
                           const int32_t* __restrict__ inv_off, const int32_t* __restrict__ inv_idx,

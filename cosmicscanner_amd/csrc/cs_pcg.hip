@@ -1,0 +1,283 @@
+// Multi-right-hand-side preconditioned conjugate gradients on the device for
+//   (sigma_g K + sigma_e I) X = B,
+// K the matrix-free kinship operator (cs_kinship).  Each column runs the textbook PCG recurrence
+// of MatrixFreePCG::solve (libCosmicCore/src/matrix_free_pcg.h:20-66) -- x += alpha p,
+// r -= alpha A p, z = M^-1 r, p = z + beta p -- and all columns share one K.V apply per
+// iteration.  Convergence per column on the relative residual ||r|| / ||b|| <= tol (the reference
+// tests the absolute ||r||; the relative form is what the scan pipeline needs across phenotype
+// scales).  Preconditioner: Jacobi, diag(A) = sigma_g diag(K) + sigma_e, or the reference's scalar
+// 1 / (sigma_g + sigma_e) (lmm_engine.cpp:306-313).  Dots and norms are FP64 with a fixed-order
+// two-stage reduction, so a solve is deterministic.
+#include <cmath>
+#include <utility>
+#include <vector>
+
+#include "cs_common.h"
+#include "cs_ops.h"
+
+namespace csamd {
+
+constexpr int kDotBlocks = 128;
+
+// partial[c * kDotBlocks + blk] = sum over this block's rows of A[:, c] * B[:, c]
+__global__ __launch_bounds__(256) void k_col_dots(const double* __restrict__ A, int64_t lda,
+                                                  const double* __restrict__ B, int64_t ldb, int64_t n,
+                                                  double* __restrict__ partial) {
+    __shared__ double sh[256];
+    const int c = blockIdx.y;
+    const int64_t chunk = (n + kDotBlocks - 1) / kDotBlocks;
+    const int64_t r0 = blockIdx.x * chunk, r1 = min(n, r0 + chunk);
+    double s = 0.0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) s += A[c * lda + r] * B[c * ldb + r];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[c * kDotBlocks + blockIdx.x] = sh[0];
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ partial, int64_t k, double* __restrict__ out) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= k) return;
+    double s = 0.0;
+    for (int b = 0; b < kDotBlocks; ++b) s += partial[c * kDotBlocks + b];
+    out[c] = s;
+}
+
+// r = b - (ax ? A x : 0), z = Minv r, p = z  (x zeroed when not warm)
+__global__ void k_pcg_start(const double* __restrict__ Bm, int64_t ldb, double* __restrict__ X,
+                            int64_t ldx, const double* __restrict__ AX, bool warm,
+                            const double* __restrict__ minv, int64_t n, int64_t k,
+                            double* __restrict__ R, double* __restrict__ Z, double* __restrict__ P) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * k;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n, i = e - c * n;
+        double r = Bm[c * ldb + i];
+        if (warm) r -= AX[e];
+        else X[c * ldx + i] = 0.0;
+        const double z = minv[e] * r;
+        R[e] = r;
+        Z[e] = z;
+        P[e] = z;
+    }
+}
+
+// AP[:, c] = sigma_g[c] KP[:, c] + sigma_e[c] P[:, c]
+__global__ void k_axpby(const double* __restrict__ KP, const double* __restrict__ P,
+                        const double* __restrict__ sg, const double* __restrict__ se, int64_t n, int64_t k,
+                        double* __restrict__ AP) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * k;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n;
+        AP[e] = sg[c] * KP[e] + se[c] * P[e];
+    }
+}
+
+// D = B - AX  (B with leading dimension ldb)
+__global__ void k_residual(const double* __restrict__ Bm, int64_t ldb, const double* __restrict__ AX,
+                           int64_t n, int64_t k, double* __restrict__ D) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * k;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n, i = e - c * n;
+        D[e] = Bm[c * ldb + i] - AX[e];
+    }
+}
+
+// alpha = rz / pAp on active columns: x += alpha p, r -= alpha Ap, z = Minv r
+__global__ void k_pcg_step(double* __restrict__ X, int64_t ldx, double* __restrict__ R,
+                           double* __restrict__ Z, const double* __restrict__ P,
+                           const double* __restrict__ AP, const double* __restrict__ minv,
+                           const double* __restrict__ rz, const double* __restrict__ pap,
+                           const int32_t* __restrict__ active, int64_t n, int64_t k) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * k;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n, i = e - c * n;
+        if (!active[c]) continue;
+        const double alpha = rz[c] / pap[c];
+        X[c * ldx + i] += alpha * P[e];
+        const double r = R[e] - alpha * AP[e];
+        R[e] = r;
+        Z[e] = minv[e] * r;
+    }
+}
+
+// beta = rz_new / rz_old on active columns: p = z + beta p
+__global__ void k_pcg_dir(double* __restrict__ P, const double* __restrict__ Z,
+                          const double* __restrict__ rz_new, const double* __restrict__ rz_old,
+                          const int32_t* __restrict__ active, int64_t n, int64_t k) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * k;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n;
+        if (!active[c]) continue;
+        P[e] = Z[e] + (rz_new[c] / rz_old[c]) * P[e];
+    }
+}
+
+}  // namespace csamd
+
+using namespace csamd;
+
+namespace {
+
+struct Dots {
+    DevBuf partial, out;
+    std::vector<double> host;
+    void init(int64_t k) {
+        partial.alloc((size_t)(k * kDotBlocks) * 8);
+        out.alloc((size_t)k * 8);
+        host.resize((size_t)k);
+    }
+    // out[c] = A[:, c] . B[:, c]
+    void run(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t k,
+             hipStream_t st, bool to_host) {
+        k_col_dots<<<dim3(kDotBlocks, (unsigned)k), 256, 0, st>>>(A, lda, B, ldb, n, partial.as<double>());
+        k_reduce_partials<<<grid_for(k), 256, 0, st>>>(partial.as<double>(), k, out.as<double>());
+        CS_HIP(hipGetLastError());
+        if (to_host) {
+            CS_HIP(hipMemcpyAsync(host.data(), out.p, (size_t)k * 8, hipMemcpyDeviceToHost, st));
+            CS_HIP(hipStreamSynchronize(st));
+        }
+    }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { CS_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+    ~Stream() { if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } }
+};
+
+void kinship_apply(cs_kinship* K, const double* dX, int64_t n, int64_t k, double* dY) {
+    const int rc = cs_kinship_apply_device(K, dX, n, k, dY, n);
+    if (rc == CS_ERR_INVALID_ARGUMENT) throw InvalidArgument(cs_last_error());
+    if (rc != CS_OK) throw RuntimeError(cs_last_error());
+}
+
+}  // namespace
+
+static void pcg_run(cs_kinship* K, const double* sg_c, const double* se_c, const double* dB, int64_t ldb,
+                    int64_t k, double* dX, int64_t ldx, double rel_tol, int max_iter, int precond,
+                    int warm_start, int* iterations, double* rel_residual) {
+    if (!K || (k > 0 && (!dB || !dX))) throw InvalidArgument("cs_pcg_solve: null argument");
+    if (k < 0 || max_iter < 0 || !(rel_tol >= 0.0)) throw InvalidArgument("cs_pcg_solve: bad argument");
+    for (int64_t c = 0; c < k; ++c)
+        if (!(se_c[c] > 0.0) || !(sg_c[c] >= 0.0))
+            throw InvalidArgument("cs_pcg_solve: need sigma_g >= 0 and sigma_e > 0");
+    int64_t n = 0;
+    if (cs_kinship_rows(K, &n) != CS_OK) throw RuntimeError(cs_last_error());
+    if (ldb < n || ldx < n) throw RuntimeError("cs_pcg_solve: dimension mismatch");
+    if (iterations) *iterations = 0;
+    if (k == 0) return;
+    Stream st;
+    // preconditioner, one column per right-hand side
+    std::vector<double> minv((size_t)(n * k));
+    std::vector<double> d;
+    if (precond == 1) {
+        d.resize((size_t)n);
+        if (cs_kinship_diag(K, d.data()) != CS_OK) throw RuntimeError(cs_last_error());
+    } else if (precond != 0) {
+        throw InvalidArgument("cs_pcg_solve: precond must be 0 (scalar) or 1 (Jacobi)");
+    }
+    for (int64_t c = 0; c < k; ++c)
+        for (int64_t i = 0; i < n; ++i)
+            minv[(size_t)(c * n + i)] = precond == 1 ? 1.0 / (sg_c[c] * d[(size_t)i] + se_c[c])
+                                                     : 1.0 / (sg_c[c] + se_c[c]);
+    const size_t nk = (size_t)(n * k) * 8;
+    DevBuf dminv(nk), R(nk), Z(nk), P(nk), KP(nk), AP(nk), active((size_t)k * 4), dsg((size_t)k * 8),
+        dse((size_t)k * 8);
+    Dots bb, rz, rz_new, pap, rr;
+    bb.init(k); rz.init(k); rz_new.init(k); pap.init(k); rr.init(k);
+    CS_HIP(hipMemcpyAsync(dminv.p, minv.data(), nk, hipMemcpyHostToDevice, st.s));
+    CS_HIP(hipMemcpyAsync(dsg.p, sg_c, (size_t)k * 8, hipMemcpyHostToDevice, st.s));
+    CS_HIP(hipMemcpyAsync(dse.p, se_c, (size_t)k * 8, hipMemcpyHostToDevice, st.s));
+    auto apply_A = [&](const double* src) {   // AP = A src (src: n x k compact)
+        CS_HIP(hipStreamSynchronize(st.s));
+        kinship_apply(K, src, n, k, KP.as<double>());
+        k_axpby<<<grid_for(n * k), 256, 0, st.s>>>(KP.as<double>(), src, dsg.as<double>(), dse.as<double>(),
+                                                   n, k, AP.as<double>());
+        CS_HIP(hipGetLastError());
+    };
+    if (warm_start) {
+        CS_HIP(hipMemcpy2DAsync(P.p, (size_t)n * 8, dX, (size_t)ldx * 8, (size_t)n * 8, (size_t)k,
+                                hipMemcpyDeviceToDevice, st.s));
+        apply_A(P.as<double>());
+    }
+    k_pcg_start<<<grid_for(n * k), 256, 0, st.s>>>(dB, ldb, dX, ldx, AP.as<double>(), warm_start != 0,
+                                                   dminv.as<double>(), n, k, R.as<double>(), Z.as<double>(),
+                                                   P.as<double>());
+    CS_HIP(hipGetLastError());
+    bb.run(dB, ldb, dB, ldb, n, k, st.s, true);
+    rr.run(R.as<double>(), n, R.as<double>(), n, n, k, st.s, true);
+    rz.run(R.as<double>(), n, Z.as<double>(), n, n, k, st.s, false);
+    std::vector<int32_t> act((size_t)k);
+    auto update_active = [&]() {
+        int64_t live = 0;
+        for (int64_t c = 0; c < k; ++c) {
+            const double b2 = bb.host[(size_t)c];
+            const bool done = b2 == 0.0 || std::sqrt(rr.host[(size_t)c] / b2) <= rel_tol;
+            act[(size_t)c] = done ? 0 : 1;
+            live += act[(size_t)c];
+        }
+        CS_HIP(hipMemcpyAsync(active.p, act.data(), (size_t)k * 4, hipMemcpyHostToDevice, st.s));
+        CS_HIP(hipStreamSynchronize(st.s));
+        return live;
+    };
+    int it = 0;
+    int64_t live = update_active();
+    while (live > 0 && it < max_iter) {
+        apply_A(P.as<double>());
+        pap.run(P.as<double>(), n, AP.as<double>(), n, n, k, st.s, false);
+        k_pcg_step<<<grid_for(n * k), 256, 0, st.s>>>(dX, ldx, R.as<double>(), Z.as<double>(), P.as<double>(),
+                                                      AP.as<double>(), dminv.as<double>(), rz.out.as<double>(),
+                                                      pap.out.as<double>(), active.as<int32_t>(), n, k);
+        rz_new.run(R.as<double>(), n, Z.as<double>(), n, n, k, st.s, false);
+        k_pcg_dir<<<grid_for(n * k), 256, 0, st.s>>>(P.as<double>(), Z.as<double>(), rz_new.out.as<double>(),
+                                                     rz.out.as<double>(), active.as<int32_t>(), n, k);
+        CS_HIP(hipGetLastError());
+        std::swap(rz.out.p, rz_new.out.p);
+        std::swap(rz.out.bytes, rz_new.out.bytes);
+        rr.run(R.as<double>(), n, R.as<double>(), n, n, k, st.s, true);
+        ++it;
+        live = update_active();
+    }
+    if (iterations) *iterations = it;
+    if (rel_residual) {
+        // true residual ||B - A X|| / ||B||, one more apply
+        CS_HIP(hipMemcpy2DAsync(P.p, (size_t)n * 8, dX, (size_t)ldx * 8, (size_t)n * 8, (size_t)k,
+                                hipMemcpyDeviceToDevice, st.s));
+        apply_A(P.as<double>());
+        k_residual<<<grid_for(n * k), 256, 0, st.s>>>(dB, ldb, AP.as<double>(), n, k, Z.as<double>());
+        CS_HIP(hipGetLastError());
+        rr.run(Z.as<double>(), n, Z.as<double>(), n, n, k, st.s, true);
+        for (int64_t c = 0; c < k; ++c) {
+            const double b2 = bb.host[(size_t)c];
+            rel_residual[c] = b2 > 0.0 ? std::sqrt(rr.host[(size_t)c] / b2) : 0.0;
+        }
+    }
+    CS_HIP(hipStreamSynchronize(st.s));
+}
+
+extern "C" {
+
+int cs_pcg_solve(cs_kinship* K, double sigma_g, double sigma_e, const double* dB, int64_t ldb, int64_t k,
+                 double* dX, int64_t ldx, double rel_tol, int max_iter, int precond, int warm_start,
+                 int* iterations, double* rel_residual) {
+    return guarded([&] {
+        const std::vector<double> sg((size_t)std::max<int64_t>(k, 0), sigma_g), se(sg.size(), sigma_e);
+        pcg_run(K, sg.data(), se.data(), dB, ldb, k, dX, ldx, rel_tol, max_iter, precond, warm_start,
+                iterations, rel_residual);
+    });
+}
+
+int cs_pcg_solve_cols(cs_kinship* K, const double* sigma_g, const double* sigma_e, const double* dB,
+                      int64_t ldb, int64_t k, double* dX, int64_t ldx, double rel_tol, int max_iter,
+                      int precond, int warm_start, int* iterations, double* rel_residual) {
+    return guarded([&] {
+        if (k > 0 && (!sigma_g || !sigma_e)) throw InvalidArgument("cs_pcg_solve_cols: null argument");
+        pcg_run(K, sigma_g, sigma_e, dB, ldb, k, dX, ldx, rel_tol, max_iter, precond, warm_start,
+                iterations, rel_residual);
+    });
+}
+
+}  // extern "C"

@@ -290,11 +290,97 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
     CS_HIP(hipStreamDestroy(st));
 }
 
+// Prepared markers (genotype_encoding.h:61-79) as dense columns over the keep list: x - mean for
+// observed genotypes, 0 for missing (mean imputation then centring); columns of SNPs that fail
+// the filter are zero.
+__global__ void k_markers(const uint8_t* __restrict__ snp, int64_t KBs, const int32_t* __restrict__ keep,
+                          int64_t n, const int32_t* __restrict__ snps, int64_t ns,
+                          const double* __restrict__ mean, const int32_t* __restrict__ pass,
+                          double* __restrict__ G, int64_t ldg) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * ns;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n, r = e - c * n;
+        double v = 0.0;
+        if (pass[c]) {
+            const int64_t s = keep ? keep[r] : r;
+            const int code = (snp[tiled_offset(snps[c], s >> 2, KBs)] >> (2 * (s & 3))) & 3;
+            if (code != 1) v = (code == 0 ? 2.0 : code == 2 ? 1.0 : 0.0) - mean[c];
+        }
+        G[c * ldg + r] = v;
+    }
+}
+
+void markers_device(cs_bed* b, const int32_t* keep, int64_t n_keep, const int32_t* snps, int64_t ns,
+                    double min_maf, double* dG, int64_t ldg, int32_t* pass_out, double* freq_out,
+                    hipStream_t st) {
+    ensure_stats(b);
+    const int64_t N = b->N, M = b->M;
+    const int64_t n = keep ? n_keep : N;
+    if (ldg < n) throw RuntimeError("markers: dimension mismatch");
+    for (int64_t r = 0; keep && r < n; ++r)
+        if (keep[r] < 0 || keep[r] >= N) throw RuntimeError("markers: sample index out of range");
+    for (int64_t i = 0; i < ns; ++i)
+        if (snps[i] < 0 || snps[i] >= M) throw RuntimeError("markers: variant index out of range");
+    if (ns == 0) return;
+    std::vector<int32_t> w((size_t)b->Npad, 0);
+    for (int64_t r = 0; r < n; ++r) w[(size_t)(keep ? keep[r] : r)] += 1;
+    DevBuf dw(w.size() * 4), counts((size_t)M * 32), dk((size_t)std::max<int64_t>(1, keep ? n : 1) * 4),
+        dsn((size_t)ns * 4), dmean((size_t)ns * 8), dpass((size_t)ns * 4);
+    CS_HIP(hipMemcpyAsync(dw.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, st));
+    weighted_counts(b, dw.as<int32_t>(), counts.as<int64_t>(), st);
+    std::vector<int64_t> cnt((size_t)M * 4);
+    CS_HIP(hipMemcpyAsync(cnt.data(), counts.p, cnt.size() * 8, hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+    std::vector<double> mean((size_t)ns, 0.0);
+    std::vector<int32_t> ps((size_t)ns, 0);
+    for (int64_t i = 0; i < ns; ++i) {
+        const int64_t* c = &cnt[(size_t)snps[i] * 4];
+        const int64_t obs = c[0] + c[2] + c[3];
+        const double sx = 2.0 * (double)c[0] + (double)c[2];
+        double f = 0.0;
+        if (obs > 0) {
+            f = sx / (2.0 * (double)obs);
+            const double maf = f < 1.0 - f ? f : 1.0 - f;
+            ps[(size_t)i] = (obs >= 3 && maf >= min_maf) ? 1 : 0;
+            mean[(size_t)i] = sx / (double)obs;
+        }
+        if (freq_out) freq_out[i] = f;
+        if (pass_out) pass_out[i] = ps[(size_t)i];
+    }
+    if (keep) CS_HIP(hipMemcpyAsync(dk.p, keep, (size_t)n * 4, hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(dsn.p, snps, (size_t)ns * 4, hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(dmean.p, mean.data(), (size_t)ns * 8, hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(dpass.p, ps.data(), (size_t)ns * 4, hipMemcpyHostToDevice, st));
+    k_markers<<<grid_for(n * ns), 256, 0, st>>>(b->snp.as<uint8_t>(), b->KBs, keep ? dk.as<int32_t>() : nullptr,
+                                                n, dsn.as<int32_t>(), ns, dmean.as<double>(),
+                                                dpass.as<int32_t>(), dG, ldg);
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace csamd
 
 using namespace csamd;
 
 extern "C" {
+
+int cs_markers_device(cs_bed* bed, const int32_t* keep, int64_t n_keep, const int32_t* snps,
+                      int64_t n_snps, double min_maf, double* d_G, int64_t ldg, int32_t* pass,
+                      double* freq) {
+    return guarded([&] {
+        if (!bed || (n_snps > 0 && (!snps || !d_G))) throw InvalidArgument("cs_markers_device: null argument");
+        hipStream_t st;
+        CS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        try {
+            markers_device(bed, (keep && n_keep > 0) ? keep : nullptr, n_keep, snps, n_snps, min_maf, d_G, ldg,
+                           pass, freq, st);
+        } catch (...) {
+            (void)hipStreamDestroy(st);
+            throw;
+        }
+        CS_HIP(hipStreamDestroy(st));
+    });
+}
 
 int cs_scan_sums(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* R, int64_t ldr,
                  int64_t nr, double min_maf, const int32_t* snps, int64_t n_snps, int prec,

@@ -180,6 +180,51 @@ int cs_grm_build_raw(cs_bed* bed, int alg, float min_maf, const int32_t* snps, i
  * build's stream. */
 int cs_grm_last_timing(double* ms_products, double* ms_combine, int* launches, double* mfma_ops);
 
+/* ---- Matrix-free solves.
+ * Multi-RHS PCG for (sigma_g K + sigma_e I) X = B with the kinship operator K
+ * (MatrixFreePCG::solve, libCosmicCore/src/matrix_free_pcg.h:20-66): the columns share one K.V
+ * apply per iteration and converge independently on ||r|| / ||b|| <= rel_tol (max_iter
+ * iterations at most).  precond 0: scalar 1 / (sigma_g + sigma_e) (lmm_engine.cpp:306-313);
+ * 1: Jacobi with diag(K).  dB, dX: device n x k column-major; warm_start != 0 starts from the
+ * given dX.  iterations: iterations run; rel_residual (k, may be NULL): the true residual
+ * ||B - A X|| / ||B|| per column, from one more apply. */
+int cs_pcg_solve(cs_kinship* K, double sigma_g, double sigma_e, const double* dB, int64_t ldb,
+                 int64_t k, double* dX, int64_t ldx, double rel_tol, int max_iter, int precond,
+                 int warm_start, int* iterations, double* rel_residual);
+/* Same with per-column variance components (sigma_g[c], sigma_e[c] for column c). */
+int cs_pcg_solve_cols(cs_kinship* K, const double* sigma_g, const double* sigma_e, const double* dB,
+                      int64_t ldb, int64_t k, double* dX, int64_t ldx, double rel_tol, int max_iter,
+                      int precond, int warm_start, int* iterations, double* rel_residual);
+
+/* Prepared markers (genotype_encoding.h:61-79 on the keep list: mean-imputed, centred; zero
+ * columns for SNPs failing n_obs >= 3 / MAF) of `snps` as dense device columns d_G (n x n_snps,
+ * leading dimension ldg); pass / freq per SNP (host, may be NULL). */
+int cs_markers_device(cs_bed* bed, const int32_t* keep, int64_t n_keep, const int32_t* snps,
+                      int64_t n_snps, double min_maf, double* d_G, int64_t ldg, int32_t* pass,
+                      double* freq);
+
+/* PCG-projected scan with one (e.g. LOCO) kinship operator K over the keep list:
+ * V = sigma_g K + sigma_e I, one multi-RHS PCG solve V [Z_Y, Z_X, Z_C] = [Y, X, G_cal],
+ * P y = Z_Y - Z_X (X^T Z_X)^-1 X^T Z_Y (the GLS projection of LmmEngine::scan,
+ * src/lmm_engine.cpp:1363-1525), calibration gamma = mean_cal g^T P g / g~^T g~ (g~ = g projected
+ * off X), and per SNP and trait: score = g^T P y_t, var = gamma g~^T g~, beta, se, p, log10p
+ * (n_snps x T row-major; NaN when not tested).  Y: n x T FP64 (ldy); X: n x c float32 (ldx, the
+ * intercept included, 1 <= c <= 32).  cal_snps: calibration SNPs (those failing the filter are
+ * skipped). */
+typedef struct cs_pscan_info {
+    double gamma;
+    int64_t n_calibration;
+    int pcg_iterations;
+    double max_rel_residual;
+} cs_pscan_info;
+int cs_pcg_projected_scan(cs_kinship* K, cs_bed* bed, const int32_t* keep, int64_t n_keep,
+                          const double* Y, int64_t ldy, int64_t T, const float* X, int64_t ldx,
+                          int64_t c, double sigma_g, double sigma_e, const int32_t* cal_snps,
+                          int64_t n_cal, const int32_t* snps, int64_t n_snps, double min_maf,
+                          double rel_tol, int max_iter, int prec, double* freq, int64_t* n_obs,
+                          int32_t* pass, double* beta, double* se, double* p, double* log10p,
+                          cs_pscan_info* info);
+
 /* ---- Dense null model and rotated scan (the default --make-grm path of
  * GwasScanner::runLocoScan, src/gwas_scanner.cpp:74-525).
  *

@@ -426,3 +426,61 @@ def gwas_dense(bed, N, M, chroms, y, cov=None, min_maf=0.01, no_loco=False, scan
                 G = Gsub * (S / S_loco) - raw_c[np.ix_(keep, keep)] / S_loco
             run(G, snps, c)
     return models, rows
+
+
+# ------------------------------------------------------------- matrix-free (PCG-projected) path
+def dense_kinship(bed, N, M, keep=None, variants=None):
+    """K = Z_sel Z_sel^T / M_sel as a dense matrix (the operator applied to the identity)."""
+    means, inv = stats(bed, N, M)
+    n = N if keep is None else len(keep)
+    return kinship_apply(bed, N, means, inv, keep, variants, np.eye(n))
+
+
+def prepared_markers(bed, N, M, keep, snps, min_maf=0.01):
+    """genotype_encoding.h:61-79 on the keep list from an independent numpy decode: (G n x s, pass)."""
+    D = decode_numpy(bed, N, M)
+    keep = np.arange(N) if keep is None else np.asarray(keep)
+    Xs = D[np.ix_(keep, np.asarray(snps))]
+    obs = (~np.isnan(Xs)).sum(0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        mean = np.nansum(Xs, 0) / obs
+    f = np.where(obs > 0, mean / 2.0, 0.0)
+    ok = (obs >= 3) & (np.minimum(f, 1.0 - f) >= min_maf)
+    G = np.where(np.isnan(Xs), 0.0, Xs - mean)
+    G[:, ~ok] = 0.0
+    return G, ok
+
+
+def pcg_projected_scan(bed, N, M, keep, variants, Y, X, sigma_g, sigma_e, cal_snps, min_maf=0.01):
+    """The projected scan with exact dense solves: V = sigma_g K + sigma_e I, P y, calibration
+    gamma over cal_snps, var = gamma g~^T g~.  Returns (stats per trait dict of (M,) arrays,
+    gamma); SNPs failing the filter or var <= 1e-12 are NaN."""
+    K = dense_kinship(bed, N, M, keep, variants)
+    n = K.shape[0]
+    Y = np.asarray(Y, dtype=np.float64).reshape(n, -1)
+    Xd = np.asarray(X, dtype=np.float32).astype(np.float64).reshape(n, -1)
+    V = sigma_g * K + sigma_e * np.eye(n)
+    Gc, okc = prepared_markers(bed, N, M, keep, cal_snps, min_maf)
+    Gc = Gc[:, okc]
+    Z = np.linalg.solve(V, np.column_stack([Y, Xd, Gc]))
+    T, c = Y.shape[1], Xd.shape[1]
+    ZY, ZX, ZC = Z[:, :T], Z[:, T:T + c], Z[:, T + c:]
+    C = np.linalg.inv(Xd.T @ ZX)
+    PY = ZY - ZX @ (C @ (Xd.T @ ZY))
+    XtXi = np.linalg.inv(Xd.T @ Xd)
+    q = Xd.T @ Gc
+    gpg = np.einsum("ij,ij->j", Gc, ZC) - np.einsum("ij,ik,kj->j", Xd.T @ ZC, C, Xd.T @ ZC)
+    gtg = np.einsum("ij,ij->j", Gc, Gc) - np.einsum("ij,ik,kj->j", q, XtXi, q)
+    use = (gtg > 0) & (gtg > 1e-8 * np.einsum("ij,ij->j", Gc, Gc))
+    gamma = float(np.mean(gpg[use] / gtg[use]))
+    freq, _, ps, dot, g2 = scan_sums(bed, N, M, keep, np.column_stack([PY, Xd]), min_maf)
+    qs = dot[:, T:]
+    var = gamma * (g2 - np.einsum("ij,jk,ik->i", qs, XtXi, qs))
+    out = []
+    for t in range(T):
+        st = _stats_from(dot[:, t], var)
+        bad = ~(ps.astype(bool) & st["tested"])
+        for key in ("beta", "se", "p", "log10p"):
+            st[key] = np.where(bad, np.nan, st[key])
+        out.append(st)
+    return out, gamma, freq

@@ -328,3 +328,32 @@ def test_gwas_dense_gemma_truth(ref_datasets):
     for snp, b in truth.items():
         assert np.sign(by_id[snp][2]) == np.sign(b)
     assert rank.index("rs000013") < 5 and rank.index("rs000212") < 5
+
+
+def test_projected_scan_reduces_to_ols_without_kinship():
+    # sigma_g = 0: V = sigma_e I, so g^T P g = g~^T g~ / sigma_e exactly -> gamma = 1 / sigma_e, and
+    # the calibrated scan is exactly the textbook GLS (here OLS) scan of the same markers.
+    N, M = 120, 300
+    body = oracle.gen_hwe_bed(9, N, M, 0.05, 0.5, 0.02)
+    rng = np.random.default_rng(0)
+    X = np.column_stack([np.ones(N), rng.standard_normal(N)]).astype(np.float32)
+    y = rng.standard_normal(N)
+    st, gamma, _ = oracle.pcg_projected_scan(body, N, M, None, None, y, X, 0.0, 0.7, np.arange(0, 40))
+    assert abs(gamma * 0.7 - 1.0) < 1e-12
+    G, ok = oracle.prepared_markers(body, N, M, None, np.arange(M))
+    ref = oracle.gls_scan(0.7 * np.eye(N), X.astype(np.float64), y, G[:, ok])
+    lp = st[0]["log10p"][ok]
+    m = ~np.isnan(lp)
+    assert np.array_equal(m, ref["tested"])
+    assert np.max(np.abs(lp[m] - ref["log10p"][m])) < 1e-9
+    assert np.max(np.abs(st[0]["beta"][ok][m] - ref["beta"][m])) < 1e-9
+
+
+def test_prepared_markers_match_scan_sums():
+    N, M = 77, 200
+    body = oracle.gen_hwe_bed(5, N, M, 0.0, 0.5, 0.05)
+    keep = np.array([3, 1, 1, 40, 76, 20, 9, 9, 9, 50, 60, 61], dtype=np.int32)
+    G, ok = oracle.prepared_markers(body, N, M, keep, np.arange(M), 0.05)
+    _, _, ps, _, gtg = oracle.scan_sums(body, N, M, keep, np.ones((keep.size, 1)), 0.05)
+    assert np.array_equal(ok, ps.astype(bool))
+    assert np.max(np.abs(np.sum(G * G, 0)[ok] - gtg[ok])) < 1e-9
