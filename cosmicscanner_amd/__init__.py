@@ -114,6 +114,13 @@ def _load():
                                    ctypes.c_double, ctypes.c_double, _i32p, _i64, _i32p, _i64,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp,
                                    _i64p, _i32p, _dp, _dp, _dp, _dp, ctypes.c_void_p], ctypes.c_int),
+        "cs_lowrank_prepare": ([_vp, _dp, _f32p, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
+                                ctypes.c_void_p], ctypes.c_int),
+        "cs_lowrank_destroy": ([_vp], ctypes.c_int),
+        "cs_lowrank_get": ([_vp, _i64p, _dp, _dp, _i64, _dp, _dp, _dp], ctypes.c_int),
+        "cs_lowrank_scan": ([_vp, _vp, _i32p, _i64, _i32p, _i64, ctypes.c_double, ctypes.c_int, _dp, _i64p,
+                             _i32p, _dp, _dp, _dp, _dp], ctypes.c_int),
         "cs_gwas_dense_run": ([ctypes.c_char_p, _dp, _dp, _i64, _i64, ctypes.c_void_p,
                                ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
     }
@@ -744,3 +751,68 @@ def pcg_projected_scan(op, reader, keep, Y, X, sigma_g, sigma_e, cal_snps, snps=
                n_calibration=info.n_calibration, pcg_iterations=info.pcg_iterations,
                max_rel_residual=info.max_rel_residual)
     return out
+
+
+# ------------------------------------------------------------------- low-rank matrix-free path
+class LowRankDiag(ctypes.Structure):
+    _fields_ = [("requested_rank", ctypes.c_int), ("effective_rank", ctypes.c_int),
+                ("sketch_size", ctypes.c_int), ("total_rank", ctypes.c_int),
+                ("residual_eigenvalue", ctypes.c_double), ("trace", ctypes.c_double),
+                ("trace_fraction", ctypes.c_double), ("relative_frobenius_error", ctypes.c_double),
+                ("trace_threshold_met", ctypes.c_int), ("fell_back", ctypes.c_int),
+                ("sigma2_g", ctypes.c_double), ("sigma2_e", ctypes.c_double), ("delta", ctypes.c_double),
+                ("decomposition_seconds", ctypes.c_double), ("estimated_peak_bytes", ctypes.c_int64)]
+
+
+class LowRankLmm:
+    """Randomized low-rank null model on a kinship operator + scanLowRank
+    (src/lmm_engine.cpp:897-1095, 1269-1361; src/matrix_free_vce.cpp:32-235)."""
+
+    def __init__(self, op, y, X, rank=256, oversample=10, seed=42, trace_threshold=0.95, min_rank=1,
+                 no_fallback=False):
+        n = op.rows()
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        Xf = np.asfortranarray(np.asarray(X, dtype=np.float32).reshape(n, -1))
+        self.n, self.c = n, Xf.shape[1]
+        h = _vp()
+        d = LowRankDiag()
+        _check(lib().cs_lowrank_prepare(op._h, _p(y, _dp), _p(Xf, _f32p), n, self.c, int(rank), int(oversample),
+                                        int(seed), float(trace_threshold), int(min_rank), int(bool(no_fallback)),
+                                        ctypes.byref(h), ctypes.byref(d)))
+        self._h = h
+        self.diag = {f[0]: getattr(d, f[0]) for f in LowRankDiag._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib().cs_lowrank_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get(self, with_u=False):
+        r = ctypes.c_int64()
+        _check(lib().cs_lowrank_get(self._h, ctypes.byref(r), None, None, 0, None, None, None))
+        lam = np.empty(r.value)
+        U = np.empty((self.n, r.value), order="F") if with_u else None
+        r0, sg, se = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().cs_lowrank_get(self._h, None, _p(lam, _dp), _p(U, _dp), self.n, ctypes.byref(r0),
+                                    ctypes.byref(sg), ctypes.byref(se)))
+        return dict(rank=r.value, lam=lam, U=U, r0=r0.value, sg=sg.value, se=se.value)
+
+    def scan(self, reader, keep, snps=None, min_maf=0.01, prec=CS_PREC_INT8_L3):
+        keep = _opt_i32(keep)
+        snps = _opt_i32(snps)
+        ns = reader.num_snps if snps is None else snps.size
+        out = {k: np.empty(ns) for k in ("freq", "beta", "se", "p", "log10p")}
+        nobs = np.empty(ns, dtype=np.int64)
+        ps = np.empty(ns, dtype=np.int32)
+        _check(lib().cs_lowrank_scan(self._h, reader._h, _p(keep, _i32p), 0 if keep is None else keep.size,
+                                     _p(snps, _i32p), 0 if snps is None else snps.size, float(min_maf), int(prec),
+                                     _p(out["freq"], _dp), _p(nobs, _i64p), _p(ps, _i32p), _p(out["beta"], _dp),
+                                     _p(out["se"], _dp), _p(out["p"], _dp), _p(out["log10p"], _dp)))
+        out.update(n_obs=nobs, passes=ps.astype(bool))
+        return out

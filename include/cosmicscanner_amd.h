@@ -35,6 +35,7 @@ extern "C" {
 typedef struct cs_bed cs_bed;
 typedef struct cs_kinship cs_kinship;
 typedef struct cs_lmm cs_lmm;
+typedef struct cs_lowrank cs_lowrank;
 
 /* Thread-local message of the last failed call on this thread ("" if none). */
 const char* cs_last_error(void);
@@ -224,6 +225,35 @@ int cs_pcg_projected_scan(cs_kinship* K, cs_bed* bed, const int32_t* keep, int64
                           double rel_tol, int max_iter, int prec, double* freq, int64_t* n_obs,
                           int32_t* pass, double* beta, double* se, double* p, double* log10p,
                           cs_pscan_info* info);
+
+/* ---- Randomized low-rank null model on the kinship operator (the matrix-free GWAS path).
+ * LmmEngine::prepareLowRankMatrixFree (src/lmm_engine.cpp:897-1095): Omega from
+ * std::mt19937_64(seed) + std::normal_distribution<double> (as the reference), K Omega, two power
+ * iterations, B = Q^T K Q, eigendecomposition, the leading `rank` pairs, trace-fraction rank
+ * selection (trace_threshold, min_rank), residual eigenvalue, REML (81-point grid on
+ * log eta in [-12, 12] + golden section).  When the threshold is not met and no_fallback == 0 the
+ * full rank is kept -- GwasScanner's fallback to MatrixFreeVce (src/matrix_free_vce.cpp:32-235),
+ * the same computation with the threshold off.  y: n phenotypes over the operator's rows;
+ * X: n x c float32 (intercept included). */
+typedef struct cs_lowrank_diag {
+    int requested_rank, effective_rank, sketch_size, total_rank;
+    double residual_eigenvalue, trace, trace_fraction, relative_frobenius_error;
+    int trace_threshold_met, fell_back;
+    double sigma2_g, sigma2_e, delta, decomposition_seconds;
+    int64_t estimated_peak_bytes;
+} cs_lowrank_diag;
+int cs_lowrank_prepare(cs_kinship* K, const double* y, const float* X, int64_t ldx, int64_t c, int rank,
+                       int oversample, int64_t seed, double trace_threshold, int min_rank,
+                       int no_fallback, cs_lowrank** out, cs_lowrank_diag* diag);
+int cs_lowrank_destroy(cs_lowrank* lr);
+/* Selected rank, eigenvalues (rank), U (n x rank, ldu), r0, sigma_g^2, sigma_e^2 (any may be NULL). */
+int cs_lowrank_get(cs_lowrank* lr, int64_t* rank, double* lambda, double* U, int64_t ldu, double* r0,
+                   double* sigma2_g, double* sigma2_e);
+/* LmmEngine::scanLowRank (src/lmm_engine.cpp:1269-1361) over the keep list (the operator's rows):
+ * outputs as cs_lmm_scan (NaN when not tested). */
+int cs_lowrank_scan(cs_lowrank* lr, cs_bed* bed, const int32_t* keep, int64_t n_keep, const int32_t* snps,
+                    int64_t n_snps, double min_maf, int prec, double* freq, int64_t* n_obs, int32_t* pass,
+                    double* beta, double* se, double* p, double* log10p);
 
 /* ---- Dense null model and rotated scan (the default --make-grm path of
  * GwasScanner::runLocoScan, src/gwas_scanner.cpp:74-525).

@@ -484,3 +484,187 @@ def pcg_projected_scan(bed, N, M, keep, variants, Y, X, sigma_g, sigma_e, cal_sn
             st[key] = np.where(bad, np.nan, st[key])
         out.append(st)
     return out, gamma, freq
+
+
+# ------------------------------------------------------------------ standard-library generators
+class MT19937_64:
+    """std::mt19937_64 ([rand.predef] parameters); pinned by the standard's 10000th-output
+    known answer."""
+
+    NN, MM = 312, 156
+    MATRIX_A = 0xB5026F5AA96619E9
+    UM, LM = 0xFFFFFFFF80000000, 0x7FFFFFFF
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed):
+        self.mt = [0] * self.NN
+        self.mt[0] = seed & self.MASK
+        for i in range(1, self.NN):
+            prev = self.mt[i - 1]
+            self.mt[i] = (6364136223846793005 * (prev ^ (prev >> 62)) + i) & self.MASK
+        self.mti = self.NN
+
+    def _twist(self):
+        mt = self.mt
+        for i in range(self.NN):
+            x = (mt[i] & self.UM) | (mt[(i + 1) % self.NN] & self.LM)
+            xa = x >> 1
+            if x & 1:
+                xa ^= self.MATRIX_A
+            mt[i] = mt[(i + self.MM) % self.NN] ^ xa
+        self.mti = 0
+
+    def __call__(self):
+        if self.mti >= self.NN:
+            self._twist()
+        x = self.mt[self.mti]
+        self.mti += 1
+        x ^= (x >> 29) & 0x5555555555555555
+        x ^= (x << 17) & 0x71D67FFFEDA60000
+        x ^= (x << 37) & 0xFFF7EEE000000000
+        x ^= x >> 43
+        return x & self.MASK
+
+
+
+class StdNormal:
+    """libstdc++'s std::normal_distribution<double> over a 64-bit engine: Marsaglia's polar method
+    with the second variate saved, uniforms from std::generate_canonical<double, 53> (one 64-bit
+    draw divided by 2^64, clamped below 1).  Pinned by known answers of this image's standard
+    library (tests/golden/std_normal_mt64.txt, made by tests/golden/make_std_normal.cpp)."""
+
+    def __init__(self, engine):
+        self.eng = engine
+        self.saved = None
+
+    def _u(self):
+        v = float(self.eng()) / 18446744073709551616.0
+        return v if v < 1.0 else float(np.nextafter(1.0, 0.0))
+
+    def __call__(self):
+        if self.saved is not None:
+            v, self.saved = self.saved, None
+            return v
+        while True:
+            x = 2.0 * self._u() - 1.0
+            y = 2.0 * self._u() - 1.0
+            r2 = x * x + y * y
+            if not (r2 > 1.0 or r2 == 0.0):
+                break
+        import math
+        mult = math.sqrt(-2.0 * math.log(r2) / r2)
+        self.saved = x * mult
+        return y * mult
+
+
+def sketch_omega(seed, n, s):
+    """The reference's Omega (n x s), filled column by column (lmm_engine.cpp:918-921)."""
+    g = StdNormal(MT19937_64(seed))
+    out = np.empty((n, s))
+    for j in range(s):
+        for i in range(n):
+            out[i, j] = g()
+    return out
+
+
+def lowrank_prepare(K, y, X, rank, oversample=10, seed=42, trace_threshold=0.95, min_rank=1,
+                    no_fallback=False):
+    """prepareLowRankMatrixFree (src/lmm_engine.cpp:897-1095) / MatrixFreeVce fallback on a dense
+    K, with the reference's dense R^-1 algebra.  Returns dict(U, lam, r0, sg, se, fell_back, ...)."""
+    n = K.shape[0]
+    X = np.asarray(X, dtype=np.float32).astype(np.float64).reshape(n, -1)
+    y = np.asarray(y, dtype=np.float64)
+    s = min(n, rank + max(0, oversample))
+    sample = K @ sketch_omega(seed, n, s)
+    for _ in range(2):
+        q, _ = np.linalg.qr(sample)
+        sample = K @ q
+    q, _ = np.linalg.qr(sample)
+    w, v = np.linalg.eigh(q.T @ (K @ q))
+    lead = np.maximum(0.0, w[::-1][:rank])
+    U = q @ v[:, ::-1][:, :rank]
+    tr = float(np.trace(K))
+    sel, met = rank, trace_threshold <= 0.0
+    if trace_threshold > 0.0 and tr > 0.0:
+        ret = 0.0
+        first = min(rank, max(1, min_rank))
+        for j in range(rank):
+            ret += lead[j]
+            if j + 1 >= first and ret / tr >= trace_threshold:
+                sel, met = j + 1, True
+                break
+    fell_back = (not met) and (not no_fallback)
+    if fell_back:
+        sel = rank
+    lam, U = lead[:sel], U[:, :sel]
+    r0 = max(0.0, (tr - lam.sum()) / (n - sel))
+    p = X.shape[1]
+    df = n - p
+
+    def rinv(eta, rhs):
+        base = 1.0 / (1.0 + eta * r0)
+        corr = 1.0 / (1.0 + eta * lam) - base
+        return base * rhs + U @ (corr[:, None] * (U.T @ rhs))
+
+    def objective(log_eta):
+        eta = np.exp(log_eta)
+        info = X.T @ rinv(eta, X)
+        try:
+            L = np.linalg.cholesky(info)
+        except np.linalg.LinAlgError:
+            return np.inf, None
+        beta = np.linalg.solve(info, X.T @ rinv(eta, y[:, None])[:, 0])
+        res = y - X @ beta
+        qf = float(res @ rinv(eta, res[:, None])[:, 0])
+        if not (qf > 0.0) or not np.isfinite(qf):
+            return np.inf, None
+        logdet_r = (n - sel) * np.log1p(eta * r0) + np.sum(np.log(1.0 + eta * lam))
+        return logdet_r + 2.0 * np.sum(np.log(np.diag(L))) + df * np.log(qf / df), qf
+
+    grid = 81
+    vals = [objective(-12.0 + 24.0 * i / (grid - 1))[0] for i in range(grid)]
+    bi = 0
+    for i in range(1, grid):
+        if vals[i] < vals[bi]:
+            bi = i
+    left = -12.0 + 24.0 * max(0, bi - 1) / (grid - 1)
+    right = -12.0 + 24.0 * min(grid - 1, bi + 1) / (grid - 1)
+    g = (np.sqrt(5.0) - 1.0) / 2.0
+    c, d = right - g * (right - left), left + g * (right - left)
+    fc, fd = objective(c)[0], objective(d)[0]
+    for _ in range(80):
+        if not (right - left > 1e-8):
+            break
+        if fc < fd:
+            right, d, fd = d, c, fc
+            c = right - g * (right - left)
+            fc = objective(c)[0]
+        else:
+            left, c, fc = c, d, fd
+            d = left + g * (right - left)
+            fd = objective(d)[0]
+    le = 0.5 * (left + right)
+    _, qf = objective(le)
+    se = qf / df
+    return dict(U=U, lam=lam, r0=r0, sg=np.exp(le) * se, se=se, fell_back=fell_back, rank=sel,
+                trace=tr, X=X, y=y)
+
+
+def lowrank_scan(m, bed, N, M, keep, min_maf=0.01):
+    """scanLowRank (src/lmm_engine.cpp:1269-1361) with the explicit V^-1 of model m (dense)."""
+    U, lam, r0, sg, se, X, y = m["U"], m["lam"], m["r0"], m["sg"], m["se"], m["X"], m["y"]
+    n = U.shape[0]
+    base = 1.0 / max(1e-12, sg * r0 + se)
+    Vi = base * np.eye(n) + U @ np.diag(1.0 / (sg * lam + se) - base) @ U.T
+    info = X.T @ Vi @ X
+    beta = np.linalg.solve(info, X.T @ (Vi @ y))
+    py = Vi @ (y - X @ beta)
+    G, ok = prepared_markers(bed, N, M, keep, np.arange(M), min_maf)
+    VG = Vi @ G
+    score = G.T @ py
+    XVG = X.T @ VG
+    var = np.einsum("ij,ij->j", G, VG) - np.einsum("ij,ik,kj->j", XVG, np.linalg.inv(info), XVG)
+    st = _stats_from(score, var)
+    for key in ("beta", "se", "p", "log10p"):
+        st[key] = np.where(ok & st["tested"], st[key], np.nan)
+    return st

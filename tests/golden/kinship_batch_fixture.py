@@ -16,40 +16,7 @@ N_SNPS = 1031
 SEED = 8128
 
 
-class MT19937_64:
-    NN, MM = 312, 156
-    MATRIX_A = 0xB5026F5AA96619E9
-    UM, LM = 0xFFFFFFFF80000000, 0x7FFFFFFF
-    MASK = (1 << 64) - 1
-
-    def __init__(self, seed):
-        self.mt = [0] * self.NN
-        self.mt[0] = seed & self.MASK
-        for i in range(1, self.NN):
-            prev = self.mt[i - 1]
-            self.mt[i] = (6364136223846793005 * (prev ^ (prev >> 62)) + i) & self.MASK
-        self.mti = self.NN
-
-    def _twist(self):
-        mt = self.mt
-        for i in range(self.NN):
-            x = (mt[i] & self.UM) | (mt[(i + 1) % self.NN] & self.LM)
-            xa = x >> 1
-            if x & 1:
-                xa ^= self.MATRIX_A
-            mt[i] = mt[(i + self.MM) % self.NN] ^ xa
-        self.mti = 0
-
-    def __call__(self):
-        if self.mti >= self.NN:
-            self._twist()
-        x = self.mt[self.mti]
-        self.mti += 1
-        x ^= (x >> 29) & 0x5555555555555555
-        x ^= (x << 17) & 0x71D67FFFEDA60000
-        x ^= (x << 37) & 0xFFF7EEE000000000
-        x ^= x >> 43
-        return x & self.MASK
+from oracle import MT19937_64  # noqa: E402  (std::mt19937_64, restated in the oracle)
 
 
 def make_bed():

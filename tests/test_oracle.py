@@ -357,3 +357,50 @@ def test_prepared_markers_match_scan_sums():
     _, _, ps, _, gtg = oracle.scan_sums(body, N, M, keep, np.ones((keep.size, 1)), 0.05)
     assert np.array_equal(ok, ps.astype(bool))
     assert np.max(np.abs(np.sum(G * G, 0)[ok] - gtg[ok])) < 1e-9
+
+
+def test_std_normal_known_answers():
+    # tests/golden/std_normal_mt64.txt: this image's libstdc++ std::normal_distribution<double>
+    # over std::mt19937_64 (the reference's sketch generator), made by make_std_normal.cpp.
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "std_normal_mt64.txt")
+    blocks, cur = {}, None
+    for ln in open(path).read().splitlines():
+        if ln.startswith("seed"):
+            cur = int(ln.split()[1])
+            blocks[cur] = []
+        else:
+            blocks[cur].append(float(ln))
+    for seed, vals in blocks.items():
+        g = oracle.StdNormal(oracle.MT19937_64(seed))
+        assert [g() for _ in range(len(vals))] == vals
+    om = oracle.sketch_omega(42, 3, 2)
+    assert om[0, 0] == blocks[42][0] and om[2, 0] == blocks[42][2] and om[0, 1] == blocks[42][3]
+
+
+def test_lowrank_exact_when_rank_covers_kinship():
+    # With K of rank < sketch size the sketch is exact: the leading eigenpairs are K's own, r0 = 0
+    # beyond them, and the low-rank REML equals the dense REML on the same model.
+    rng = np.random.default_rng(2)
+    n, q = 60, 8
+    Zr = rng.standard_normal((n, q))
+    K = Zr @ Zr.T / q
+    X = np.column_stack([np.ones(n), rng.standard_normal(n)]).astype(np.float32)
+    y = rng.standard_normal(n) + 0.5 * Zr[:, 0]
+    m = oracle.lowrank_prepare(K, y, X, rank=q, oversample=4, trace_threshold=0.0)
+    w = np.linalg.eigvalsh(K)[::-1][:q]
+    assert np.max(np.abs(m["lam"] - w)) < 1e-10 * w[0]
+    assert abs(m["r0"]) < 1e-10
+    # REML with V = sg K + se I directly (brute-force grid around the optimum)
+    Xd = X.astype(np.float64)
+
+    def reml(sg, se):
+        V = sg * K + se * np.eye(n)
+        Vi = np.linalg.inv(V)
+        XVX = Xd.T @ Vi @ Xd
+        P = Vi - Vi @ Xd @ np.linalg.solve(XVX, Xd.T @ Vi)
+        return -0.5 * (np.linalg.slogdet(V)[1] + np.linalg.slogdet(XVX)[1] + y @ P @ y)
+    best = reml(m["sg"], m["se"])
+    for fg in (0.9, 1.1):
+        for fe in (0.9, 1.1):
+            assert reml(m["sg"] * fg, m["se"] * fe) <= best + 1e-9
