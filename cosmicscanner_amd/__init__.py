@@ -121,6 +121,8 @@ def _load():
         "cs_lowrank_get": ([_vp, _i64p, _dp, _dp, _i64, _dp, _dp, _dp], ctypes.c_int),
         "cs_lowrank_scan": ([_vp, _vp, _i32p, _i64, _i32p, _i64, ctypes.c_double, ctypes.c_int, _dp, _i64p,
                              _i32p, _dp, _dp, _dp, _dp], ctypes.c_int),
+        "cs_gwas_matrix_free_run": ([ctypes.c_char_p, _dp, _dp, _i64, _i64, ctypes.c_void_p,
+                                     ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
         "cs_gwas_dense_run": ([ctypes.c_char_p, _dp, _dp, _i64, _i64, ctypes.c_void_p,
                                ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
     }
@@ -680,6 +682,46 @@ def read_pheno(path, fam, mpheno=1, missing=-9.0):
                 v = float("nan")
             vals[(t[0], t[1])] = v
     return np.array([vals.get((r[0], r[1]), float("nan")) for r in fam])
+
+
+class GwasMfOptions(ctypes.Structure):
+    _fields_ = [("min_maf", ctypes.c_double), ("no_loco", ctypes.c_int), ("scan_chr", ctypes.c_char_p),
+                ("prec", ctypes.c_int), ("mode", ctypes.c_int), ("max_rank", ctypes.c_int),
+                ("oversample", ctypes.c_int), ("seed", ctypes.c_int64), ("trace_threshold", ctypes.c_double),
+                ("min_rank", ctypes.c_int), ("no_fallback", ctypes.c_int), ("error_tol", ctypes.c_double),
+                ("sigma2_g", ctypes.c_double), ("sigma2_e", ctypes.c_double), ("pcg_tol", ctypes.c_double),
+                ("pcg_max_iter", ctypes.c_int), ("n_calibration", ctypes.c_int)]
+
+
+def _summary_dict(sm):
+    return dict(n_samples=sm.n_samples, n_snps_grm=sm.n_snps_grm, n_tested=sm.n_tested,
+                sum_2pq=sm.sum_2pq, loco=bool(sm.loco),
+                models=[(sm.model_label[i].value.decode(), sm.sigma2_g[i], sm.sigma2_e[i])
+                        for i in range(sm.n_models)],
+                ms_grm=sm.ms_grm, ms_null=sm.ms_null, ms_scan=sm.ms_scan)
+
+
+def gwas_matrix_free_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=False, scan_chr=None,
+                         mode="lowrank", max_rank=256, oversample=10, seed=42, trace_threshold=0.95,
+                         min_rank=1, no_fallback=False, error_tol=1.0, sigma2_g=0.0, sigma2_e=0.0,
+                         pcg_tol=1e-6, pcg_max_iter=200, n_calibration=32, prec=CS_PREC_INT8_L3):
+    """The matrix-free GWAS (--fastlmm --fastlmm-matrix-free) on the GPU; mode "lowrank" (the
+    reference path) or "pcg" (PCG-projected scan)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    N = y.size
+    if cov is None:
+        covf, nc = None, 0
+    else:
+        covf = np.asfortranarray(np.asarray(cov, dtype=np.float64).reshape(N, -1))
+        nc = covf.shape[1]
+    o = GwasMfOptions(float(min_maf), int(bool(no_loco)), None if scan_chr is None else str(scan_chr).encode(),
+                      int(prec), {"lowrank": 0, "pcg": 1}[mode], int(max_rank), int(oversample), int(seed),
+                      float(trace_threshold), int(min_rank), int(bool(no_fallback)), float(error_tol),
+                      float(sigma2_g), float(sigma2_e), float(pcg_tol), int(pcg_max_iter), int(n_calibration))
+    sm = GwasSummary()
+    _check(lib().cs_gwas_matrix_free_run(str(bfile).encode(), _p(y, _dp), _p(covf, _dp), N, nc, ctypes.byref(o),
+                                         str(out_prefix).encode(), ctypes.byref(sm)))
+    return _summary_dict(sm)
 
 
 def gwas_dense_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=False, scan_chr=None,

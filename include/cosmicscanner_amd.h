@@ -315,6 +315,36 @@ typedef struct cs_gwas_summary {
 int cs_gwas_dense_run(const char* bfile, const double* y, const double* cov, int64_t ldc,
                       int64_t n_cov, const cs_gwas_options* opt, const char* out_prefix,
                       cs_gwas_summary* summary);
+/* The matrix-free path (--fastlmm --fastlmm-matrix-free, src/gwas_scanner.cpp:194-198,257-372):
+ * GRM variants = inv_std != 0 and all-sample MAF >= min_maf (and >= 1e-8); per LOCO chromosome a
+ * kinship operator without that chromosome.  mode 0 (the reference path): randomized low-rank REML
+ * (cs_lowrank_prepare with max_rank / oversample / seed / trace_threshold / min_rank /
+ * no_fallback; a relative Frobenius error above error_tol also falls back to the full rank) and
+ * scanLowRank; writes .low_rank.meta.tsv as well.  mode 1: PCG-projected scan
+ * (cs_pcg_projected_scan) with sigma2_g / sigma2_e, or -- when sigma2_e <= 0 -- the variance
+ * components of one global low-rank REML; n_calibration calibration SNPs per chromosome drawn with
+ * std::mt19937_64(seed). */
+typedef struct cs_gwas_mf_options {
+    double min_maf;        /* 0.01 */
+    int no_loco;
+    const char* scan_chr;
+    int prec;              /* CS_PREC_INT8_L3 */
+    int mode;              /* 0 low-rank (reference), 1 PCG-projected */
+    int max_rank;          /* --fastlmm-max-rank, 256 */
+    int oversample;        /* --lmm-low-rank-oversample, 10 */
+    int64_t seed;          /* --lmm-low-rank-seed, 42 */
+    double trace_threshold;/* --fastlmm-trace-threshold, 0.95 */
+    int min_rank;          /* --fastlmm-min-rank, 1 */
+    int no_fallback;       /* --fastlmm-no-fallback */
+    double error_tol;      /* --lmm-low-rank-error-tol, 1.0 */
+    double sigma2_g, sigma2_e;   /* mode 1: fixed variance components (sigma2_e <= 0: estimate) */
+    double pcg_tol;        /* mode 1: relative residual, 1e-6 */
+    int pcg_max_iter;      /* mode 1: 200 */
+    int n_calibration;     /* mode 1: calibration SNPs per chromosome, 32 */
+} cs_gwas_mf_options;
+int cs_gwas_matrix_free_run(const char* bfile, const double* y, const double* cov, int64_t ldc,
+                            int64_t n_cov, const cs_gwas_mf_options* opt, const char* out_prefix,
+                            cs_gwas_summary* summary);
 
 #ifdef __cplusplus
 }

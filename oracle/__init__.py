@@ -668,3 +668,46 @@ def lowrank_scan(m, bed, N, M, keep, min_maf=0.01):
     for key in ("beta", "se", "p", "log10p"):
         st[key] = np.where(ok & st["tested"], st[key], np.nan)
     return st
+
+
+def gwas_matrix_free(bed, N, M, chroms, y, cov=None, min_maf=0.01, no_loco=False, max_rank=256,
+                     oversample=10, seed=42, trace_threshold=0.95, min_rank=1, no_fallback=False,
+                     error_tol=1.0):
+    """GwasScanner::runLocoScan's matrix-free path (src/gwas_scanner.cpp:257-372) restated with dense
+    LOCO kinship matrices: returns (models [(label, vg, ve)], rows [(snp, freq, beta, se, p, log10p)])."""
+    y = np.asarray(y, dtype=np.float64)
+    covf = None if cov is None else np.asarray(cov, dtype=np.float64).reshape(N, -1)
+    ok = ~np.isnan(y)
+    if covf is not None:
+        ok &= ~np.isnan(covf.astype(np.float32)).any(axis=1)
+    keep = np.nonzero(ok)[0].astype(np.int32)
+    n = keep.size
+    X = np.ones((n, 1 + (0 if covf is None else covf.shape[1])), dtype=np.float32)
+    if covf is not None:
+        X[:, 1:] = covf[keep].astype(np.float32)
+    means, inv = stats(bed, N, M)
+    f = means / 2.0
+    maf = np.minimum(f, 1.0 - f)
+    grm_vars = [j for j in range(M) if inv[j] != 0.0 and not (maf[j] < min_maf or abs(maf[j]) < 1e-8)]
+    labels = sorted(set(chroms))
+    use_loco = len(labels) > 1 and not no_loco
+    eff = max(1, min(max_rank, n - 1))
+    models, rows = [], []
+    for c in (labels if use_loco else [None]):
+        var = [j for j in grm_vars if c is None or chroms[j] != c]
+        K = dense_kinship(bed, N, M, keep, np.asarray(var, dtype=np.int32))
+        m = lowrank_prepare(K, y[keep], X, eff, oversample, seed, trace_threshold, min_rank, no_fallback)
+        tr = m["trace"]
+        lam, r0, k = m["lam"], m["r0"], m["rank"]
+        sq = float(np.sum(lam * lam)) + (n - k) * r0 * r0
+        cross = r0 * tr + float(np.sum((lam - r0) * lam))
+        err = np.sqrt(max(0.0, tr * tr + sq - 2.0 * cross)) / tr
+        if not m["fell_back"] and err > error_tol:
+            m = lowrank_prepare(K, y[keep], X, eff, oversample, seed, 0.0, min_rank, True)
+        models.append((c or "GLOBAL", m["sg"], m["se"]))
+        st = lowrank_scan(m, bed, N, M, keep, min_maf)
+        freq = scan_sums(bed, N, M, keep, np.ones((n, 1)), min_maf)[0]
+        for j in range(M):
+            if (c is None or chroms[j] == c) and not np.isnan(st["p"][j]):
+                rows.append((j, freq[j], st["beta"][j], st["se"][j], st["p"][j], st["log10p"][j]))
+    return models, rows

@@ -191,3 +191,56 @@ def test_gwas_reference_fixtures(csa, ref_datasets, tmp_path, name, pheno):
     else:
         rank = [r[1] for r in sorted(rg, key=lambda r: float(r[9]))]
         assert rank.index("rs000013") < 5 and rank.index("rs000212") < 5
+
+
+@pytest.mark.parametrize("max_rank,threshold", [(64, 0.95), (64, 0.05)])
+def test_gwas_matrix_free_lowrank(csa, synth_gwas, tmp_path, max_rank, threshold):
+    d = synth_gwas
+    out = str(tmp_path / "mf")
+    sm = csa.gwas_matrix_free_run(d["prefix"], d["y"], out, cov=d["cov"], max_rank=max_rank,
+                                  trace_threshold=threshold)
+    models, rows = oracle.gwas_matrix_free(d["body"], d["N"], d["M"], d["chroms"], d["y"], d["cov"],
+                                           max_rank=max_rank, trace_threshold=threshold)
+    assert [m[0] for m in sm["models"]] == ["1", "10", "2"]
+    for (lg, vg, ve), (lo, vgo, veo) in zip(sm["models"], models):
+        assert lg == lo and abs(vg / vgo - 1) <= 1e-4 and abs(ve / veo - 1) <= 1e-4
+    _, rg = read_assoc(out + ".assoc.txt")
+    assert len(rg) == len(rows)
+    ids = [f"rs{j:06d}" for j in range(d["M"])]
+    assert [r[1] for r in rg] == [ids[r[0]] for r in rows]
+    for g, o in zip(rg, rows):
+        assert g[5] == f"{o[1]:.6g}"
+        assert abs(float(g[10]) - o[5]) <= 1e-3
+        assert abs(float(g[7]) - o[2]) <= 1e-3 * o[3] + 2e-5 * abs(o[2])
+    meta = open(out + ".low_rank.meta.tsv").read().splitlines()
+    assert meta[0].split("\t")[0] == "SCOPE" and [m.split("\t")[0] for m in meta[1:]] == ["1", "10", "2"]
+    fb = {m.split("\t")[-1] for m in meta[1:]}
+    assert fb == ({"trace_threshold"} if threshold > 0.5 else {"none"})
+
+
+def test_gwas_matrix_free_pcg(csa, synth_gwas, tmp_path):
+    d = synth_gwas
+    out = str(tmp_path / "pcg")
+    sm = csa.gwas_matrix_free_run(d["prefix"], d["y"], out, cov=d["cov"], mode="pcg", sigma2_g=0.4, sigma2_e=0.6,
+                                  pcg_tol=1e-8, n_calibration=24)
+    assert sm["loco"] and len(sm["models"]) == 3
+    _, rg = read_assoc(out + ".assoc.txt")
+    # each chromosome against the oracle's exact-solve restatement of the projected scan
+    ok = ~np.isnan(d["y"]) & ~np.isnan(d["cov"]).any(1)
+    keep = np.nonzero(ok)[0].astype(np.int32)
+    X = np.column_stack([np.ones(keep.size), d["cov"][keep]]).astype(np.float32)
+    means, inv = oracle.stats(d["body"], d["N"], d["M"])
+    f = means / 2
+    grm = [j for j in range(d["M"]) if inv[j] != 0 and min(f[j], 1 - f[j]) >= 0.01]
+    got = {r[1]: float(r[10]) for r in rg}
+    for chrom in ("1", "10", "2"):
+        snps = np.array([j for j in range(d["M"]) if d["chroms"][j] == chrom], dtype=np.int32)
+        var = np.array([j for j in grm if d["chroms"][j] != chrom], dtype=np.int32)
+        # the driver draws its calibration SNPs with std::shuffle (not restated here), so gamma
+        # differs by the calibration sample: compare the ordering of the calibrated scan
+        st, gamma, _ = oracle.pcg_projected_scan(d["body"], d["N"], d["M"], keep, var, d["y"][keep], X, 0.4, 0.6,
+                                                 snps[:24])
+        lo = st[0]["log10p"][snps]
+        lg = np.array([got.get(f"rs{j:06d}", np.nan) for j in snps])
+        m = ~np.isnan(lo) & ~np.isnan(lg)
+        assert np.corrcoef(lo[m], lg[m])[0, 1] > 0.999
