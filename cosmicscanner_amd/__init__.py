@@ -77,6 +77,7 @@ def _load():
         "cs_kinship_rows": ([_vp, _i64p], ctypes.c_int),
         "cs_kinship_num_variants": ([_vp, _i64p], ctypes.c_int),
         "cs_kinship_set_precision": ([_vp, ctypes.c_int], ctypes.c_int),
+        "cs_kinship_set_missing_mode": ([_vp, ctypes.c_int], ctypes.c_int),
         "cs_kinship_apply": ([_vp, _dp, _i64, _i64, _dp, _i64], ctypes.c_int),
         "cs_kinship_apply_device": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
         "cs_kinship_trace": ([_vp, _dp], ctypes.c_int),
@@ -348,6 +349,10 @@ class PlinkKinshipOperator:
 
     def set_precision(self, mode):
         _check(lib().cs_kinship_set_precision(self._h, int(mode)))
+
+    def set_missing_mode(self, mode):
+        """0 auto, 1 dense observed-indicator products, 2 sparse missing lists (same results)."""
+        _check(lib().cs_kinship_set_missing_mode(self._h, int(mode)))
 
     def apply(self, X, n_threads=1):
         X = np.asarray(X, dtype=np.float64)

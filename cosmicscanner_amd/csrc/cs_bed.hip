@@ -321,6 +321,125 @@ void weighted_counts(cs_bed* b, const int32_t* d_w_npad, int64_t* d_counts, hipS
     CS_HIP(hipGetLastError());
 }
 
+// Missing-genotype lists.  One wave per row of a tiled matrix (SNP-major: rows are SNPs and the
+// elements samples; sample-major: the reverse).  Lane l of an iteration covers k-block 16 it + l/4,
+// chunk l % 4: 64 consecutive elements, so lane order is element order.
+__global__ __launch_bounds__(256) void k_miss_count(const uint8_t* __restrict__ A, int64_t KB, int64_t rows,
+                                                   int64_t limit, int32_t* __restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;
+    const int64_t rt = row >> 6;
+    const int m = (int)((row & 63) >> 4), r = (int)(row & 15);
+    int c = 0;
+    for (int64_t kb0 = 0; kb0 < KB; kb0 += 16) {
+        const int64_t kb = kb0 + (lane >> 2);
+        const int g = lane & 3;
+        if (kb < KB) {
+            const uint4 w4 = *reinterpret_cast<const uint4*>(A + ((rt * KB + kb) << 12) + ((m * 64 + g * 16 + r) << 4));
+            const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+            const int64_t e0 = kb * 256 + g * 64;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                // code 01 <=> low bit set and high bit clear
+                const uint32_t w = ws[t];
+                uint32_t miss = (w & 0x55555555u) & ~((w >> 1) & 0x55555555u);
+                const int64_t base = e0 + 16 * t;
+                if (base + 16 > limit) {
+                    const int64_t keep = limit - base;
+                    miss = keep <= 0 ? 0u : (keep >= 16 ? miss : (miss & ((1u << (2 * keep)) - 1u)));
+                }
+                c += __builtin_popcount(miss);
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) cnt[row] = c;
+}
+
+__global__ __launch_bounds__(256) void k_miss_fill(const uint8_t* __restrict__ A, int64_t KB, int64_t rows,
+                                                  int64_t limit, const int64_t* __restrict__ off,
+                                                  int32_t* __restrict__ idx) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;
+    const int64_t rt = row >> 6;
+    const int m = (int)((row & 63) >> 4), r = (int)(row & 15);
+    int64_t pos = off[row];
+    for (int64_t kb0 = 0; kb0 < KB; kb0 += 16) {
+        const int64_t kb = kb0 + (lane >> 2);
+        const int g = lane & 3;
+        uint32_t ws[4] = {0u, 0u, 0u, 0u};
+        const int64_t e0 = kb * 256 + g * 64;
+        if (kb < KB) {
+            const uint4 w4 = *reinterpret_cast<const uint4*>(A + ((rt * KB + kb) << 12) + ((m * 64 + g * 16 + r) << 4));
+            ws[0] = w4.x; ws[1] = w4.y; ws[2] = w4.z; ws[3] = w4.w;
+        }
+        uint32_t miss[4];
+        int c = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            uint32_t mm = (ws[t] & 0x55555555u) & ~((ws[t] >> 1) & 0x55555555u);
+            const int64_t base = e0 + 16 * t;
+            if (kb >= KB) mm = 0u;
+            else if (base + 16 > limit) {
+                const int64_t keep = limit - base;
+                mm = keep <= 0 ? 0u : (keep >= 16 ? mm : (mm & ((1u << (2 * keep)) - 1u)));
+            }
+            miss[t] = mm;
+            c += __builtin_popcount(mm);
+        }
+        // exclusive prefix over the wave
+        int x = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(x, d);
+            if (lane >= d) x += y;
+        }
+        const int total = __shfl(x, 63);
+        int64_t p = pos + (x - c);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            uint32_t mm = miss[t];
+            while (mm) {
+                const int bit = __builtin_ctz(mm);
+                idx[p++] = (int32_t)(e0 + 16 * t + (bit >> 1));
+                mm &= mm - 1u;
+            }
+        }
+        pos += total;
+    }
+}
+
+static void build_lists(const uint8_t* A, int64_t KB, int64_t rows, int64_t limit, DevBuf& off, DevBuf& idx,
+                        int64_t& nnz, hipStream_t st) {
+    DevBuf cnt((size_t)std::max<int64_t>(1, rows) * 4);
+    const unsigned nb = (unsigned)((rows + 3) / 4);
+    k_miss_count<<<nb, 256, 0, st>>>(A, KB, rows, limit, cnt.as<int32_t>());
+    CS_HIP(hipGetLastError());
+    std::vector<int32_t> hc((size_t)rows);
+    CS_HIP(hipMemcpyAsync(hc.data(), cnt.p, (size_t)rows * 4, hipMemcpyDeviceToHost, st));
+    CS_HIP(hipStreamSynchronize(st));
+    std::vector<int64_t> ho((size_t)rows + 1, 0);
+    for (int64_t i = 0; i < rows; ++i) ho[(size_t)i + 1] = ho[(size_t)i] + hc[(size_t)i];
+    nnz = ho[(size_t)rows];
+    off.alloc(ho.size() * 8);
+    idx.alloc((size_t)std::max<int64_t>(1, nnz) * 4);
+    CS_HIP(hipMemcpyAsync(off.p, ho.data(), ho.size() * 8, hipMemcpyHostToDevice, st));
+    k_miss_fill<<<nb, 256, 0, st>>>(A, KB, rows, limit, off.as<int64_t>(), idx.as<int32_t>());
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipStreamSynchronize(st));
+}
+
+void ensure_missing_lists(cs_bed* b) {
+    if (b->miss_nnz >= 0) return;
+    ensure_sample_major(b);
+    int64_t n1 = 0, n2 = 0;
+    build_lists(b->snp.as<uint8_t>(), b->KBs, b->M, b->N, b->miss_snp_off, b->miss_snp_idx, n1, b->stream);
+    build_lists(b->smp.as<uint8_t>(), b->KBm, b->N, b->M, b->miss_smp_off, b->miss_smp_idx, n2, b->stream);
+    if (n1 != n2) throw RuntimeError("missing-genotype lists disagree between the two layouts");
+    b->miss_nnz = n1;
+}
+
 }  // namespace csamd
 
 using namespace csamd;

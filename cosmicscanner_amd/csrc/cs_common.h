@@ -106,5 +106,9 @@ struct cs_bed {
     csamd::DevBuf means;       // M double  (PlinkReader::getMeans)
     csamd::DevBuf inv_stds;    // M double  (PlinkReader::getInvStds)
     bool have_stats = false;
+    // Missing genotypes as CSR lists (built on demand): per SNP the missing samples, per sample
+    // the missing SNPs, both in increasing order; real rows and columns only.
+    csamd::DevBuf miss_snp_off, miss_snp_idx, miss_smp_off, miss_smp_idx;
+    int64_t miss_nnz = -1;     // -1: lists not built
     hipStream_t stream = nullptr;
 };

@@ -68,9 +68,13 @@ struct GenoArgs {
     double* W2;
     unsigned long long* mx;    // EPI_KV_GT: per-column max(|w|, |mu w|) as double bits
     long long* yint;           // EPI_KV_G: [row][k] int64, accumulated atomically over splits
+    const long long* qext;     // ONLY_X: O^T B per [row][16 G] (exact, from the missing lists)
 };
 
-template <int L, bool TWO_B, int EPI>
+// ONLY_X: only the genotype product X~^T B runs on MFMA.  The observed-indicator product O^T B
+// (O = 1 - missing) is then supplied exactly as a.qext (column sums minus the missing entries)
+// for EPI_PQ / EPI_KV_GT, and is added by the caller after the kernel for EPI_KV_G.
+template <int L, bool TWO_B, int EPI, bool ONLY_X = false>
 __device__ __forceinline__ void geno_body(const GenoArgs& a) {
     constexpr int MT = 4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -134,8 +138,9 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a) {
 #pragma unroll
                     for (int l = 0; l < L; ++l) {
                         accX[m][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(X, bx[l], accX[m][l], 0, 0, 0);
-                        accO[m][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[l] : bx[l],
-                                                                           accO[m][l], 0, 0, 0);
+                        if (!ONLY_X)
+                            accO[m][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[l] : bx[l],
+                                                                               accO[m][l], 0, 0, 0);
                     }
                 }
 #pragma unroll
@@ -163,8 +168,9 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a) {
 #pragma unroll
             for (int l = L - 1; l >= 0; --l) {
                 P = P * 256 + accX[m][l][v];
-                Q = Q * 256 + accO[m][l][v];
+                if (!ONLY_X) Q = Q * 256 + accO[m][l][v];
             }
+            if (ONLY_X && EPI != EPI_KV_G && col_ok && row < a.rows) Q = a.qext[row * 16 * (int64_t)a.G + rhs];
             if (EPI == EPI_PQ) {
                 if (col_ok && row < a.rows) {
                     const int64_t G16 = 16 * (int64_t)a.G;
@@ -188,7 +194,7 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a) {
             } else {
                 if (col_ok && row < a.rows)
                     atomicAdd(reinterpret_cast<unsigned long long*>(a.yint + row * a.k + rhs),
-                              (unsigned long long)(P - Q));
+                              (unsigned long long)(ONLY_X ? P : P - Q));
             }
         }
     if (EPI == EPI_KV_GT) {
@@ -205,6 +211,11 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a) {
 template <int L, int EPI>
 __global__ __launch_bounds__(256) void k_geno1(GenoArgs a) {
     geno_body<L, false, EPI>(a);
+}
+// Genotype-only product (the sparse observed-indicator form).
+template <int L, int EPI>
+__global__ __launch_bounds__(256) void k_genox(GenoArgs a) {
+    geno_body<L, false, EPI, true>(a);
 }
 template <int L, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno2(GenoArgs a) {

@@ -32,10 +32,13 @@ void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, cons
 
 // Launch k_geno<L, TWO_B, EPI> (cs_geno_mfma.h) on the given grid.
 struct GenoArgs;
-void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const GenoArgs& a);
+void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const GenoArgs& a,
+                 bool only_x = false);
 
 void ensure_stats(cs_bed* b);
 void ensure_sample_major(cs_bed* b);
+// CSR lists of the missing genotypes (both orientations); idempotent.
+void ensure_missing_lists(cs_bed* b);
 void weighted_counts(cs_bed* b, const int32_t* d_w_npad, int64_t* d_counts, hipStream_t st);
 
 // Per-SNP scan over the keep list (cs_scan.hip): marker preparation, g^T R in int8 MFMA, and --

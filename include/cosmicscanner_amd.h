@@ -95,6 +95,13 @@ int cs_kinship_destroy(cs_kinship* op);
 int cs_kinship_rows(const cs_kinship* op, int64_t* rows);
 int cs_kinship_num_variants(const cs_kinship* op, int64_t* m);
 int cs_kinship_set_precision(cs_kinship* op, int mode);
+/* How the int8 operator treats O = observed-genotype indicators (same results either way, both
+ * exact integer sums): 0 auto (= dense; the list gathers cost more than they save at 0.5 %
+ * missing, n = 50k, m = 500k), 1 dense (an O product on the
+ * MFMA units next to each genotype product), 2 sparse (genotype products only; O^T V and
+ * O (mu W) from column sums minus the missing entries, from lists built once per genotype
+ * handle). */
+int cs_kinship_set_missing_mode(cs_kinship* op, int mode);
 /* apply / apply_block / applyTranspose (kinship_operator.cpp:211-304): Y = K X for X n x k
  * column-major with leading dimension ldx; Y may alias X; k == 0 is allowed.  ldx < n or
  * n_threads-like misuse -> CS_ERR_RUNTIME "dimension mismatch". */
