@@ -124,6 +124,13 @@ def _load():
                              _i32p, _dp, _dp, _dp, _dp], ctypes.c_int),
         "cs_gwas_matrix_free_run": ([ctypes.c_char_p, _dp, _dp, _i64, _i64, ctypes.c_void_p,
                                      ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
+        "cs_server_create": ([_vp, _i32p, _i64, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "cs_server_destroy": ([_vp], ctypes.c_int),
+        "cs_server_submit": ([_vp, ctypes.c_void_p, _i64p], ctypes.c_int),
+        "cs_server_step": ([_vp, _i32p, _i32p, _i32p], ctypes.c_int),
+        "cs_server_result": ([_vp, _i64, _i32p, _dp, _i64p, _i32p, _dp, _dp, _dp, _dp, ctypes.c_void_p],
+                             ctypes.c_int),
+        "cs_server_release": ([_vp, _i64], ctypes.c_int),
         "cs_gwas_dense_run": ([ctypes.c_char_p, _dp, _dp, _i64, _i64, ctypes.c_void_p,
                                ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
     }
@@ -863,3 +870,100 @@ class LowRankLmm:
                                      _p(out["se"], _dp), _p(out["p"], _dp), _p(out["log10p"], _dp)))
         out.update(n_obs=nobs, passes=ps.astype(bool))
         return out
+
+
+# ------------------------------------------------------------------------------ serving loop
+class _Request(ctypes.Structure):
+    _fields_ = [("keep", _i32p), ("n_keep", ctypes.c_int64), ("Y", _dp), ("ldy", ctypes.c_int64),
+                ("n_traits", ctypes.c_int64), ("X", _f32p), ("ldx", ctypes.c_int64), ("n_cov", ctypes.c_int64),
+                ("sigma2_g", ctypes.c_double), ("sigma2_e", ctypes.c_double), ("cal_snps", _i32p),
+                ("n_cal", ctypes.c_int64), ("snps", _i32p), ("n_snps", ctypes.c_int64),
+                ("min_maf", ctypes.c_double), ("rel_tol", ctypes.c_double), ("max_iter", ctypes.c_int),
+                ("prec", ctypes.c_int)]
+
+
+class _ReqStats(ctypes.Structure):
+    _fields_ = [("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("gamma", ctypes.c_double), ("n_calibration", ctypes.c_int64),
+                ("n_columns", ctypes.c_int64)]
+
+
+class ScanServer:
+    """Continuously batched scan requests over one resident genotype matrix (see cs_server_*)."""
+
+    def __init__(self, reader, variants=None, max_columns=256, hbm_budget_bytes=0):
+        self.reader = reader
+        var = _opt_i32(variants)
+        h = _vp()
+        _check(lib().cs_server_create(reader._h, _p(var, _i32p), 0 if var is None else var.size, int(max_columns),
+                                      int(hbm_budget_bytes), ctypes.byref(h)))
+        self._h = h
+        self._meta = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib().cs_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, Y, X, sigma2_g, sigma2_e, cal_snps, keep=None, snps=None, min_maf=0.01, rel_tol=1e-6,
+               max_iter=200, prec=CS_PREC_INT8_L3):
+        keep_a = _opt_i32(keep)
+        n = self.reader.num_samples if keep_a is None else keep_a.size
+        Yf = np.asfortranarray(np.asarray(Y, dtype=np.float64).reshape(n, -1))
+        Xf = np.asfortranarray(np.asarray(X, dtype=np.float32).reshape(n, -1))
+        cal = np.ascontiguousarray(cal_snps, dtype=np.int32)
+        sn = _opt_i32(snps)
+        r = _Request(_p(keep_a, _i32p), 0 if keep_a is None else keep_a.size, _p(Yf, _dp), n, Yf.shape[1],
+                     _p(Xf, _f32p), n, Xf.shape[1], float(sigma2_g), float(sigma2_e), _p(cal, _i32p), cal.size,
+                     _p(sn, _i32p), 0 if sn is None else sn.size, float(min_maf), float(rel_tol), int(max_iter),
+                     int(prec))
+        rid = ctypes.c_int64()
+        _check(lib().cs_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
+        self._meta[rid.value] = (Yf.shape[1], self.reader.num_snps if sn is None else sn.size)
+        return rid.value
+
+    def step(self):
+        a, p, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().cs_server_step(self._h, ctypes.byref(a), ctypes.byref(p), ctypes.byref(c)))
+        return a.value, p.value, c.value
+
+    def run_until_idle(self, max_steps=100000):
+        for _ in range(max_steps):
+            a, p, _ = self.step()
+            if a == 0 and p == 0:
+                return
+        raise RuntimeError("server did not drain")
+
+    def status(self, rid):
+        st = ctypes.c_int32()
+        s = _ReqStats()
+        _check(lib().cs_server_result(self._h, rid, ctypes.byref(st), None, None, None, None, None, None, None,
+                                      ctypes.byref(s)))
+        return st.value, {f[0]: getattr(s, f[0]) for f in _ReqStats._fields_}
+
+    def result(self, rid):
+        T, ns = self._meta[rid]
+        out = {k: np.empty((ns, T)) for k in ("beta", "se", "p", "log10p")}
+        freq = np.empty(ns)
+        nobs = np.empty(ns, dtype=np.int64)
+        ps = np.empty(ns, dtype=np.int32)
+        st = ctypes.c_int32()
+        s = _ReqStats()
+        _check(lib().cs_server_result(self._h, rid, ctypes.byref(st), _p(freq, _dp), _p(nobs, _i64p), _p(ps, _i32p),
+                                      _p(out["beta"], _dp), _p(out["se"], _dp), _p(out["p"], _dp),
+                                      _p(out["log10p"], _dp), ctypes.byref(s)))
+        if st.value != 2:
+            raise RuntimeError(f"request {rid} not finished (status {st.value})")
+        out.update(freq=freq, n_obs=nobs, passes=ps.astype(bool),
+                   stats={f[0]: getattr(s, f[0]) for f in _ReqStats._fields_})
+        return out
+
+    def release(self, rid):
+        _check(lib().cs_server_release(self._h, rid))
+        self._meta.pop(rid, None)

@@ -36,6 +36,7 @@ typedef struct cs_bed cs_bed;
 typedef struct cs_kinship cs_kinship;
 typedef struct cs_lmm cs_lmm;
 typedef struct cs_lowrank cs_lowrank;
+typedef struct cs_server cs_server;
 
 /* Thread-local message of the last failed call on this thread ("" if none). */
 const char* cs_last_error(void);
@@ -261,6 +262,48 @@ int cs_lowrank_get(cs_lowrank* lr, int64_t* rank, double* lambda, double* U, int
 int cs_lowrank_scan(cs_lowrank* lr, cs_bed* bed, const int32_t* keep, int64_t n_keep, const int32_t* snps,
                     int64_t n_snps, double min_maf, int prec, double* freq, int64_t* n_obs, int32_t* pass,
                     double* beta, double* se, double* p, double* log10p);
+
+/* ---- Serving loop: scan requests over cohort subsets of one resident genotype matrix, run as
+ * continuously batched multi-RHS PCG columns (one K.V apply per iteration for every admitted
+ * request; the kinship matrix of a cohort is the restriction of the all-sample operator since the
+ * standardisation uses all-sample statistics) and a PCG-projected fused scan per finished request
+ * (cs_pcg_projected_scan's estimator).  A request's results do not depend on which other
+ * requests share the batch.  No reference counterpart: the reference runs one phenotype per
+ * process. */
+typedef struct cs_request {
+    const int32_t* keep; int64_t n_keep;     /* cohort (distinct samples; NULL/0 = all) */
+    const double* Y; int64_t ldy; int64_t n_traits;   /* n x T phenotypes over the cohort */
+    const float* X; int64_t ldx; int64_t n_cov;       /* n x c covariates incl. intercept (1..32) */
+    double sigma2_g, sigma2_e;               /* variance components of the null model */
+    const int32_t* cal_snps; int64_t n_cal;  /* calibration SNPs */
+    const int32_t* snps; int64_t n_snps;     /* SNPs to scan (NULL/0 = all) */
+    double min_maf;
+    double rel_tol;                          /* PCG relative residual (<= 0: 1e-6) */
+    int max_iter;                            /* (<= 0: 200) */
+    int prec;                                /* (0: CS_PREC_INT8_L3) */
+} cs_request;
+typedef struct cs_request_stats {
+    double t_submit, t_admit, t_done;        /* seconds since the server was created */
+    int iterations;
+    double gamma;
+    int64_t n_calibration, n_columns;
+} cs_request_stats;
+/* max_columns: PCG column slots (N x max_columns x 8 doubles of HBM); hbm_budget_bytes: bound on
+ * the per-request buffers of admitted requests (<= 0: unbounded). */
+int cs_server_create(cs_bed* bed, const int32_t* variants, int64_t n_variants, int64_t max_columns,
+                     int64_t hbm_budget_bytes, cs_server** out);
+int cs_server_destroy(cs_server* srv);
+int cs_server_submit(cs_server* srv, const cs_request* req, int64_t* id);
+/* One scheduler iteration: admit pending requests that fit, one PCG iteration over all active
+ * columns, finish the requests whose columns converged. */
+int cs_server_step(cs_server* srv, int32_t* n_active, int32_t* n_pending, int32_t* n_completed);
+/* status: 0 pending, 1 active, 2 done, -1 failed.  Asking for outputs of a failed request gives
+ * CS_ERR_RUNTIME with its message.  When done, outputs as cs_pcg_projected_scan (any may be
+ * NULL). */
+int cs_server_result(cs_server* srv, int64_t id, int32_t* status, double* freq, int64_t* n_obs,
+                     int32_t* pass, double* beta, double* se, double* p, double* log10p,
+                     cs_request_stats* stats);
+int cs_server_release(cs_server* srv, int64_t id);
 
 /* ---- Dense null model and rotated scan (the default --make-grm path of
  * GwasScanner::runLocoScan, src/gwas_scanner.cpp:74-525).
