@@ -85,6 +85,83 @@ def grm_leg(csa, reader, N, M):
                               kernel="k_grm_pass (all launches)"))
 
 
+def scan_leg(csa, reader, N, M, T=16):
+    """PCG-projected LMM scan of T phenotype replicates over the whole workload (one multi-RHS
+    PCG solve of T + 4 covariate + 16 calibration columns, then one fused scan pass), and the
+    fused scan pass alone.  genotypes/s = n m / time of one pass over all SNPs (all T traits)."""
+    rng = np.random.default_rng(17)
+    X = np.column_stack([np.ones(N), rng.standard_normal((N, 3))]).astype(np.float32)
+    Y = rng.standard_normal((N, T))
+    cal = np.sort(rng.choice(M, 16, replace=False)).astype(np.int32)
+    op = csa.PlinkKinshipOperator(reader)
+    csa.synchronize()
+    t0 = time.perf_counter()
+    out = csa.pcg_projected_scan(op, reader, None, Y, X, 0.4, 0.6, cal, rel_tol=1e-6, max_iter=200)
+    csa.synchronize()
+    total = time.perf_counter() - t0
+    # the fused scan pass alone: R = [P y_1..T, X]
+    R = np.asfortranarray(rng.standard_normal((N, T + 4)))
+    cov_C = np.eye(4) * 1e-6
+    csa.lmm_scan(reader, None, R, n_score=T, alpha=1.0, cov=(T, T + 4), cov_C=cov_C)
+    t1 = time.perf_counter()
+    csa.lmm_scan(reader, None, R, n_score=T, alpha=1.0, cov=(T, T + 4), cov_C=cov_C)
+    scan_s = time.perf_counter() - t1
+    return dict(traits=T, covariates=4, pcg_iterations=out["pcg_iterations"], gamma=out["gamma"],
+                max_rel_residual=out["max_rel_residual"], total_ms=total * 1e3, scan_pass_ms=scan_s * 1e3,
+                scan_genotypes_per_s=N * M / scan_s, scan_trait_genotypes_per_s=N * M * T / scan_s,
+                end_to_end_trait_genotypes_per_s=N * M * T / total)
+
+
+def serving_leg(csa, n=100_000, m=1_000_000, n_requests=12, rate=5.0, seed=20261206):
+    """Poisson stream of scan requests (rate req/s) over a resident n x m matrix; each request: a
+    random cohort (log-uniform 10k..n), 1-64 traits, 0-10 covariates (+ intercept), h2 ~
+    U(0.1, 0.8), 16 calibration SNPs, a scan of all m SNPs.  Latency = completion - arrival."""
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    reader = csa.PlinkReader.synthetic(seed, n, m, 0.05, 0.5, 0.005)
+    srv = csa.ScanServer(reader, max_columns=512)
+    setup = time.perf_counter() - t0
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    reqs = []
+    for _ in range(n_requests):
+        nc = int(np.exp(rng.uniform(np.log(10_000), np.log(n))))
+        keep = np.sort(rng.choice(n, nc, replace=False)).astype(np.int32)
+        T = int(rng.integers(1, 65))
+        c = int(rng.integers(0, 11)) + 1
+        X = np.column_stack([np.ones(nc)] + [rng.standard_normal(nc) for _ in range(c - 1)]).astype(np.float32)
+        Y = rng.standard_normal((nc, T))
+        h2 = float(rng.uniform(0.1, 0.8))
+        cal = np.sort(rng.choice(m, 16, replace=False)).astype(np.int32)
+        reqs.append(dict(Y=Y, X=X, sigma2_g=h2, sigma2_e=1.0 - h2, cal_snps=cal, keep=keep))
+    ids, submitted, done = {}, 0, {}
+    genos = 0.0
+    start = time.perf_counter()
+    while len(done) < n_requests:
+        now = time.perf_counter() - start
+        while submitted < n_requests and arrivals[submitted] <= now:
+            ids[srv.submit(**reqs[submitted])] = submitted
+            submitted += 1
+        if srv.step() == (0, 0, 0) and submitted < n_requests:
+            time.sleep(max(0.0, min(0.01, arrivals[submitted] - (time.perf_counter() - start))))
+        for rid, i in list(ids.items()):
+            if rid in done:
+                continue
+            st, info = srv.status(rid)
+            if st == 2:
+                done[rid] = time.perf_counter() - start - arrivals[i]
+                genos += reqs[i]["keep"].size * m * reqs[i]["Y"].shape[1]
+                srv.release(rid)
+            elif st == -1:
+                raise RuntimeError(f"request {i} failed")
+    wall = time.perf_counter() - start
+    lat = np.array(sorted(done.values()))
+    srv.close()
+    reader.close()
+    return dict(workload=f"n={n} x m={m} resident, {n_requests} requests, Poisson {rate} req/s",
+                p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                wall_s=wall, trait_genotypes_per_s=genos / wall, setup_s=setup)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +175,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-grm", action="store_true")
+    ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 2 GiB device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -186,6 +265,19 @@ def main():
         log(f"GRM: {grm['ms']:.1f} ms, {grm['tops_full_square']:.1f} TOPS (2 n^2 m), "
             f"int8 products {grm['ms_products']:.1f} ms in {grm['launches']} launches")
 
+    scan = None
+    if not args.no_scan:
+        scan = scan_leg(csa, reader, N, M)
+        log(f"projected LMM scan: {scan['total_ms']:.1f} ms end to end ({scan['pcg_iterations']} PCG iterations), "
+            f"scan pass {scan['scan_pass_ms']:.1f} ms = {scan['scan_genotypes_per_s']:.3e} genotypes/s")
+
+    serving = None
+    if not args.no_serving:
+        op.close()
+        serving = serving_leg(csa)
+        log(f"serving: p50 {serving['p50_latency_s']:.2f} s, p99 {serving['p99_latency_s']:.2f} s, "
+            f"{serving['trait_genotypes_per_s']:.3e} trait-genotypes/s")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(N, args.cpu_slice_snps, k, seed)
@@ -212,6 +304,8 @@ def main():
         "speedup_vs_fp32_gpu": (value / fp32) if fp32 else None,
         "roofline": roofline,
         "grm": grm,
+        "lmm_scan": scan,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

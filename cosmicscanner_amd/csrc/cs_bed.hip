@@ -203,55 +203,6 @@ __global__ __launch_bounds__(256) void k_stats(const uint8_t* __restrict__ snp, 
     }
 }
 
-// Weighted class counts (keep multiplicities), tiled SNP-major; weights int32 over Npad samples.
-__global__ __launch_bounds__(256) void k_wcounts(const uint8_t* __restrict__ snp, int64_t KB,
-                                                 int64_t M, const int32_t* __restrict__ w,
-                                                 int64_t* __restrict__ counts) {
-    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-    const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tr * 64 >= M) return;
-    const uint8_t* base = snp + ((tr * KB) << 12) + lane * 16;
-    long long c[4][4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) c[m][t] = 0;
-    for (int64_t kb = 0; kb < KB; ++kb) {
-        uint4 w4[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) w4[m] = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
-        const int4* wp = reinterpret_cast<const int4*>(w + kb * 256 + g * 64);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {       // dword q = samples 16q .. 16q+15 of the chunk
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {   // byte b of the dword = samples 16q + 4b .. +3
-                const int4 ww = wp[q * 4 + b];
-                const int wq[4] = {ww.x, ww.y, ww.z, ww.w};
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const uint32_t word = q == 0 ? w4[m].x : q == 1 ? w4[m].y : q == 2 ? w4[m].z : w4[m].w;
-#pragma unroll
-                    for (int s = 0; s < 4; ++s) c[m][(word >> (8 * b + 2 * s)) & 3u] += wq[s];
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int t = 0; t < 4; ++t)
-            for (int off = 16; off < 64; off <<= 1) c[m][t] += __shfl_xor(c[m][t], off);
-    if (g != 0) return;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const int64_t j = tr * 64 + 16 * m + r;
-        if (j >= M) continue;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) counts[j * 4 + t] = c[m][t];
-    }
-}
-
-
 static void init_layout(cs_bed* b, int64_t N, int64_t M) {
     if (N <= 0 || M <= 0) throw InvalidArgument("cs_bed: n_samples and n_snps must be positive");
     CS_HIP(hipGetDevice(&b->device));
@@ -315,10 +266,96 @@ void ensure_stats(cs_bed* b) {
     b->have_stats = true;
 }
 
-void weighted_counts(cs_bed* b, const int32_t* d_w_npad, int64_t* d_counts, hipStream_t st) {
-    k_wcounts<<<(unsigned)(b->Mpad / 256), 256, 0, st>>>(b->snp.as<uint8_t>(), b->KBs, b->M,
-                                                        d_w_npad, d_counts);
+// Weighted class counts from popcounts.  The weights (non-negative multiplicities) are split into
+// binary planes, each packed like a genotype row (bit 2i set where sample i's plane bit is 1);
+// for a 16-genotype word with low bits lo = w & 0x55.. and high bits hi = (w >> 1) & 0x55..:
+// n11 = popc(hi & lo & pm), n10 = popc(hi & pm) - n11, n01 = popc(lo & pm) - n11,
+// n00 = popc(pm) - n01 - n10 - n11, each weighted by 2^plane.
+__global__ __launch_bounds__(256) void k_wcounts_pc(const uint8_t* __restrict__ snp, int64_t KB, int64_t M,
+                                                    const uint32_t* __restrict__ planes, int nplanes,
+                                                    int64_t* __restrict__ counts) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tr * 64 >= M) return;
+    const uint8_t* base = snp + ((tr * KB) << 12) + lane * 16;
+    long long c[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) c[m][t] = 0;
+    for (int64_t kb = 0; kb < KB; ++kb) {
+        uint4 w4[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) w4[m] = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
+        for (int p = 0; p < nplanes; ++p) {
+            const uint4 pm4 = *reinterpret_cast<const uint4*>(planes + ((p * KB + kb) * 4 + g) * 4);
+            const uint32_t pm[4] = {pm4.x, pm4.y, pm4.z, pm4.w};
+            int tot = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) tot += __builtin_popcount(pm[q]);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                int h = 0, l = 0, hl = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t wd = q == 0 ? w4[m].x : q == 1 ? w4[m].y : q == 2 ? w4[m].z : w4[m].w;
+                    const uint32_t lo = wd & 0x55555555u, hi = (wd >> 1) & 0x55555555u;
+                    h += __builtin_popcount(hi & pm[q]);
+                    l += __builtin_popcount(lo & pm[q]);
+                    hl += __builtin_popcount(hi & lo & pm[q]);
+                }
+                const long long sc = 1LL << p;
+                c[m][3] += (long long)hl * sc;
+                c[m][2] += (long long)(h - hl) * sc;
+                c[m][1] += (long long)(l - hl) * sc;
+                c[m][0] += (long long)(tot - h - l + hl) * sc;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            for (int off = 16; off < 64; off <<= 1) c[m][t] += __shfl_xor(c[m][t], off);
+    if (g != 0) return;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int64_t j = tr * 64 + 16 * m + r;
+        if (j >= M) continue;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) counts[j * 4 + t] = c[m][t];
+    }
+}
+
+void weighted_counts(cs_bed* b, const int32_t* w, int64_t n_w, int64_t* d_counts, hipStream_t st) {
+    if (n_w > b->Npad) throw InvalidArgument("class counts: more weights than samples");
+    int32_t wmax = 0;
+    for (int64_t i = 0; i < n_w; ++i) {
+        if (w[i] < 0) throw InvalidArgument("class counts: negative weight");
+        wmax = std::max(wmax, w[i]);
+    }
+    int np = 0;
+    while (np < 31 && (wmax >> np) > 0) ++np;
+    if (np == 0) {
+        CS_HIP(hipMemsetAsync(d_counts, 0, (size_t)b->M * 32, st));
+        return;
+    }
+    // planes[p][kb][g][q]: word q of chunk g of k-block kb covers samples 256 kb + 64 g + 16 q + u
+    const int64_t KB = b->KBs;
+    std::vector<uint32_t> planes((size_t)(np * KB * 16), 0u);
+    for (int64_t i = 0; i < n_w; ++i) {
+        const int32_t wi = w[i];
+        if (!wi) continue;
+        const int64_t kb = i >> 8, rr = i & 255, g = rr >> 6, q = (rr >> 4) & 3, u = rr & 15;
+        for (int p = 0; p < np; ++p)
+            if ((wi >> p) & 1) planes[(size_t)(((p * KB + kb) * 4 + g) * 4 + q)] |= 1u << (2 * u);
+    }
+    DevBuf dp(planes.size() * 4);
+    CS_HIP(hipMemcpyAsync(dp.p, planes.data(), planes.size() * 4, hipMemcpyHostToDevice, st));
+    k_wcounts_pc<<<(unsigned)(b->Mpad / 256), 256, 0, st>>>(b->snp.as<uint8_t>(), KB, b->M, dp.as<uint32_t>(), np,
+                                                           d_counts);
     CS_HIP(hipGetLastError());
+    CS_HIP(hipStreamSynchronize(st));
 }
 
 // Missing-genotype lists.  One wave per row of a tiled matrix (SNP-major: rows are SNPs and the
@@ -616,12 +653,8 @@ int cs_bed_class_counts(cs_bed* bed, const int32_t* weights, int64_t* counts) {
                              hipMemcpyDeviceToHost));
             return;
         }
-        DevBuf w((size_t)bed->Npad * sizeof(int32_t));
         DevBuf c((size_t)bed->M * 4 * sizeof(int64_t));
-        CS_HIP(hipMemsetAsync(w.p, 0, w.bytes, bed->stream));
-        CS_HIP(hipMemcpyAsync(w.p, weights, (size_t)bed->N * sizeof(int32_t), hipMemcpyHostToDevice,
-                              bed->stream));
-        weighted_counts(bed, w.as<int32_t>(), c.as<int64_t>(), bed->stream);
+        weighted_counts(bed, weights, bed->N, c.as<int64_t>(), bed->stream);
         CS_HIP(hipMemcpyAsync(counts, c.p, c.bytes, hipMemcpyDeviceToHost, bed->stream));
         CS_HIP(hipStreamSynchronize(bed->stream));
     });

@@ -74,41 +74,54 @@ struct GenoArgs {
 // ONLY_X: only the genotype product X~^T B runs on MFMA.  The observed-indicator product O^T B
 // (O = 1 - missing) is then supplied exactly as a.qext (column sums minus the missing entries)
 // for EPI_PQ / EPI_KV_GT, and is added by the caller after the kernel for EPI_KV_G.
-template <int L, bool TWO_B, int EPI, bool ONLY_X = false>
+//
+// MT x GZ: a wave covers MT 16-row subtiles and GZ groups of 16 right-hand sides; every decoded
+// genotype fragment feeds GZ L (x2) MFMAs.  Grid: x = padded rows / (64 MT), y = k splits,
+// z = ceil(G / GZ).
+template <int L, bool TWO_B, int EPI, bool ONLY_X = false, int MT = 4, int GZ = 1>
 __device__ __forceinline__ void geno_body(const GenoArgs& a) {
-    constexpr int MT = 4;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
-    const int64_t tr = (int64_t)blockIdx.x * 4 + wave;
-    const int gz = blockIdx.z;
+    const int64_t rbase = ((int64_t)blockIdx.x * 4 + wave) * (16 * MT);   // first row of the wave
+    const int64_t tr = rbase >> 6;
+    const int m0 = (int)((rbase & 63) >> 4);
+    const int gz0 = blockIdx.z * GZ;
     const int NT = a.G * L;
     const int64_t kb0 = (int64_t)blockIdx.y * a.kb_per_split;
     int64_t kb1 = kb0 + a.kb_per_split;
     if (kb1 > a.kb_total) kb1 = a.kb_total;
 
-    v4i accX[MT][L], accO[MT][L];
+    v4i accX[MT][GZ][L], accO[MT][GZ][L];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int l = 0; l < L; ++l) {
-            accX[m][l] = v4i{0, 0, 0, 0};
-            accO[m][l] = v4i{0, 0, 0, 0};
-        }
+        for (int j = 0; j < GZ; ++j)
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                accX[m][j][l] = v4i{0, 0, 0, 0};
+                accO[m][j][l] = v4i{0, 0, 0, 0};
+            }
+    bool gok[GZ];
+#pragma unroll
+    for (int j = 0; j < GZ; ++j) gok[j] = gz0 + j < a.G;
 
     if (kb0 < kb1) {
-        const uint8_t* abase = a.A + ((tr * a.KB) << 12) + lane * 16;
-        const int8_t* bxp = a.Bx + ((int64_t)(gz * L) << 10) + lane * 16;
-        const int8_t* bop = a.Bo + ((int64_t)(gz * L) << 10) + lane * 16;
+        const uint8_t* abase = a.A + ((tr * a.KB) << 12) + m0 * 1024 + lane * 16;
+        const int8_t* bxp = a.Bx + ((int64_t)(gz0 * L) << 10) + lane * 16;
+        const int8_t* bop = a.Bo + ((int64_t)(gz0 * L) << 10) + lane * 16;
         const int64_t last_step = kb1 * 4 - 1;
         uint4 av[MT], an[MT];
-        v4i bx[L], bo[L], bxn[L], bon[L];
+        v4i bx[GZ][L], bo[GZ][L], bxn[GZ][L], bon[GZ][L];
 #pragma unroll
         for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const uint4*>(abase + (kb0 << 12) + m * 1024);
 #pragma unroll
-        for (int l = 0; l < L; ++l) {
-            bx[l] = *reinterpret_cast<const v4i*>(bxp + ((kb0 * 4 * NT + l) << 10));
-            if (TWO_B) bo[l] = *reinterpret_cast<const v4i*>(bop + ((kb0 * 4 * NT + l) << 10));
-        }
+        for (int j = 0; j < GZ; ++j)
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                if (!gok[j]) continue;
+                bx[j][l] = *reinterpret_cast<const v4i*>(bxp + ((kb0 * 4 * NT + j * L + l) << 10));
+                if (TWO_B) bo[j][l] = *reinterpret_cast<const v4i*>(bop + ((kb0 * 4 * NT + j * L + l) << 10));
+            }
         for (int64_t kb = kb0; kb < kb1; ++kb) {
             const int64_t kn = kb + 1 < kb1 ? kb + 1 : kb;
 #pragma unroll
@@ -118,10 +131,13 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a) {
                 int64_t sn = kb * 4 + t + 1;
                 if (sn > last_step) sn = last_step;
 #pragma unroll
-                for (int l = 0; l < L; ++l) {
-                    bxn[l] = *reinterpret_cast<const v4i*>(bxp + ((sn * NT + l) << 10));
-                    if (TWO_B) bon[l] = *reinterpret_cast<const v4i*>(bop + ((sn * NT + l) << 10));
-                }
+                for (int j = 0; j < GZ; ++j)
+#pragma unroll
+                    for (int l = 0; l < L; ++l) {
+                        if (!gok[j]) continue;
+                        bxn[j][l] = *reinterpret_cast<const v4i*>(bxp + ((sn * NT + j * L + l) << 10));
+                        if (TWO_B) bon[j][l] = *reinterpret_cast<const v4i*>(bop + ((sn * NT + j * L + l) << 10));
+                    }
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
                     const uint32_t w = t == 0 ? av[m].x : t == 1 ? av[m].y : t == 2 ? av[m].z : av[m].w;
@@ -136,18 +152,24 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a) {
                                       (int)__builtin_amdgcn_perm(0u, kLutO, d2),
                                       (int)__builtin_amdgcn_perm(0u, kLutO, d3)};
 #pragma unroll
-                    for (int l = 0; l < L; ++l) {
-                        accX[m][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(X, bx[l], accX[m][l], 0, 0, 0);
-                        if (!ONLY_X)
-                            accO[m][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[l] : bx[l],
-                                                                               accO[m][l], 0, 0, 0);
+                    for (int j = 0; j < GZ; ++j) {
+                        if (!gok[j]) continue;
+#pragma unroll
+                        for (int l = 0; l < L; ++l) {
+                            accX[m][j][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(X, bx[j][l], accX[m][j][l], 0, 0, 0);
+                            if (!ONLY_X)
+                                accO[m][j][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[j][l] : bx[j][l],
+                                                                                      accO[m][j][l], 0, 0, 0);
+                        }
                     }
                 }
 #pragma unroll
-                for (int l = 0; l < L; ++l) {
-                    bx[l] = bxn[l];
-                    if (TWO_B) bo[l] = bon[l];
-                }
+                for (int j = 0; j < GZ; ++j)
+#pragma unroll
+                    for (int l = 0; l < L; ++l) {
+                        bx[j][l] = bxn[j][l];
+                        if (TWO_B) bo[j][l] = bon[j][l];
+                    }
             }
 #pragma unroll
             for (int m = 0; m < MT; ++m) av[m] = an[m];
@@ -156,70 +178,73 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a) {
 
     // Epilogue.  16x16 MFMA C layout: lane holds column r of its group, rows 4g + v (v = 0..3)
     // of each m-subtile; the L limbs of one column sit in the same lane (tiles gz*L + l).
-    const int64_t rhs = (int64_t)gz * 16 + r;
-    const bool col_ok = rhs < a.k;
-    double wmax = 0.0;
 #pragma unroll
-    for (int m = 0; m < MT; ++m)
+    for (int j = 0; j < GZ; ++j) {
+        const int64_t rhs = (int64_t)(gz0 + j) * 16 + r;
+        const bool col_ok = gok[j] && rhs < a.k;
+        double wmax = 0.0;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int64_t row = tr * 64 + 16 * m + 4 * g + v;
-            long long P = 0, Q = 0;
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
-            for (int l = L - 1; l >= 0; --l) {
-                P = P * 256 + accX[m][l][v];
-                if (!ONLY_X) Q = Q * 256 + accO[m][l][v];
-            }
-            if (ONLY_X && EPI != EPI_KV_G && col_ok && row < a.rows) Q = a.qext[row * 16 * (int64_t)a.G + rhs];
-            if (EPI == EPI_PQ) {
-                if (col_ok && row < a.rows) {
-                    const int64_t G16 = 16 * (int64_t)a.G;
-                    a.pq[(row * 2) * G16 + rhs] = P;
-                    a.pq[(row * 2 + 1) * G16 + rhs] = Q;
+            for (int v = 0; v < 4; ++v) {
+                const int64_t row = rbase + 16 * m + 4 * g + v;
+                long long P = 0, Q = 0;
+#pragma unroll
+                for (int l = L - 1; l >= 0; --l) {
+                    P = P * 256 + accX[m][j][l][v];
+                    if (!ONLY_X) Q = Q * 256 + accO[m][j][l][v];
                 }
-            } else if (EPI == EPI_KV_GT) {
-                if (col_ok) {
-                    double w1 = 0.0, w2 = 0.0;
-                    if (row < a.rows && a.mult[row] != 0 && a.inv_stds[row] != 0.0) {
-                        const double s = a.inv_stds[row], mu = a.means[row];
-                        const double u = ((double)P - mu * (double)Q) * s * a.scale[rhs];
-                        const double w = (double)a.mult[row] * s * u * a.inv_M;
-                        w1 = w;
-                        w2 = mu * w;
+                if (ONLY_X && EPI != EPI_KV_G && col_ok && row < a.rows) Q = a.qext[row * 16 * (int64_t)a.G + rhs];
+                if (EPI == EPI_PQ) {
+                    if (col_ok && row < a.rows) {
+                        const int64_t G16 = 16 * (int64_t)a.G;
+                        a.pq[(row * 2) * G16 + rhs] = P;
+                        a.pq[(row * 2 + 1) * G16 + rhs] = Q;
                     }
-                    a.W1[row * a.k + rhs] = w1;
-                    a.W2[row * a.k + rhs] = w2;
-                    wmax = fmax(wmax, fmax(fabs(w1), fabs(w2)));
+                } else if (EPI == EPI_KV_GT) {
+                    if (col_ok) {
+                        double w1 = 0.0, w2 = 0.0;
+                        if (row < a.rows && a.mult[row] != 0 && a.inv_stds[row] != 0.0) {
+                            const double s = a.inv_stds[row], mu = a.means[row];
+                            const double u = ((double)P - mu * (double)Q) * s * a.scale[rhs];
+                            const double w = (double)a.mult[row] * s * u * a.inv_M;
+                            w1 = w;
+                            w2 = mu * w;
+                        }
+                        a.W1[row * a.k + rhs] = w1;
+                        a.W2[row * a.k + rhs] = w2;
+                        wmax = fmax(wmax, fmax(fabs(w1), fabs(w2)));
+                    }
+                } else {
+                    if (col_ok && row < a.rows)
+                        atomicAdd(reinterpret_cast<unsigned long long*>(a.yint + row * a.k + rhs),
+                                  (unsigned long long)(ONLY_X ? P : P - Q));
                 }
-            } else {
-                if (col_ok && row < a.rows)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(a.yint + row * a.k + rhs),
-                              (unsigned long long)(ONLY_X ? P : P - Q));
             }
+        if (EPI == EPI_KV_GT) {
+            wmax = fmax(wmax, __shfl_xor(wmax, 16));
+            wmax = fmax(wmax, __shfl_xor(wmax, 32));
+            if (g == 0 && col_ok && wmax > 0.0)
+                atomicMax(a.mx + rhs, (unsigned long long)__double_as_longlong(wmax));
         }
-    if (EPI == EPI_KV_GT) {
-        wmax = fmax(wmax, __shfl_xor(wmax, 16));
-        wmax = fmax(wmax, __shfl_xor(wmax, 32));
-        if (g == 0 && col_ok && wmax > 0.0)
-            atomicMax(a.mx + rhs, (unsigned long long)__double_as_longlong(wmax));
     }
 }
 
 // Entry points.  The two-operand form carries twice the B registers; without an explicit
 // occupancy target the compiler sinks its B prefetch loads next to their uses (every MFMA step
 // then waits on the outstanding A prefetch), so it is pinned to 2 waves per SIMD.
-template <int L, int EPI>
+template <int L, int EPI, int MT = 4, int GZ = 1>
 __global__ __launch_bounds__(256) void k_geno1(GenoArgs a) {
-    geno_body<L, false, EPI>(a);
+    geno_body<L, false, EPI, false, MT, GZ>(a);
 }
 // Genotype-only product (the sparse observed-indicator form).
 template <int L, int EPI>
 __global__ __launch_bounds__(256) void k_genox(GenoArgs a) {
     geno_body<L, false, EPI, true>(a);
 }
-template <int L, int EPI>
+template <int L, int EPI, int MT = 4, int GZ = 1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno2(GenoArgs a) {
-    geno_body<L, true, EPI>(a);
+    geno_body<L, true, EPI, false, MT, GZ>(a);
 }
 
 }  // namespace csamd

@@ -323,8 +323,16 @@ __global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, i
 }
 
 // ------------------------------------------------------------------------------------------
+// grid as the callers give it: x = padded rows / 256, y = splits, z = G.  With two or more
+// column groups each wave takes 32 rows and two groups (half the genotype passes and decodes).
 template <int L, bool TWO_B, int EPI>
 static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a) {
+    if (a.G >= 2) {
+        const dim3 g2(grid.x * 2, grid.y, (unsigned)((a.G + 1) / 2));
+        if (TWO_B) k_geno2<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
+        else k_geno1<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
+        return;
+    }
     if (TWO_B) k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
     else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
 }

@@ -177,12 +177,11 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
             std::vector<int32_t> fill(off.begin(), off.end() - 1);
             for (int64_t r = 0; r < n; ++r) idx[(size_t)fill[(size_t)(a.keep ? a.keep[r] : r)]++] = (int32_t)r;
         }
-        DevBuf dw(w.size() * 4), doff(off.size() * 4), didx(std::max<size_t>(1, idx.size()) * 4);
-        CS_HIP(hipMemcpyAsync(dw.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, st));
+        DevBuf doff(off.size() * 4), didx(std::max<size_t>(1, idx.size()) * 4);
         CS_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
         if (!idx.empty()) CS_HIP(hipMemcpyAsync(didx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, st));
         DevBuf counts((size_t)M * 4 * sizeof(int64_t));
-        weighted_counts(b, dw.as<int32_t>(), counts.as<int64_t>(), st);
+        weighted_counts(b, w.data(), (int64_t)w.size(), counts.as<int64_t>(), st);
         // R into device (compact n x nr) then scatter to the full sample space
         DevBuf dR;
         const double* Rd = a.R;
@@ -324,10 +323,9 @@ void markers_device(cs_bed* b, const int32_t* keep, int64_t n_keep, const int32_
     if (ns == 0) return;
     std::vector<int32_t> w((size_t)b->Npad, 0);
     for (int64_t r = 0; r < n; ++r) w[(size_t)(keep ? keep[r] : r)] += 1;
-    DevBuf dw(w.size() * 4), counts((size_t)M * 32), dk((size_t)std::max<int64_t>(1, keep ? n : 1) * 4),
+    DevBuf counts((size_t)M * 32), dk((size_t)std::max<int64_t>(1, keep ? n : 1) * 4),
         dsn((size_t)ns * 4), dmean((size_t)ns * 8), dpass((size_t)ns * 4);
-    CS_HIP(hipMemcpyAsync(dw.p, w.data(), w.size() * 4, hipMemcpyHostToDevice, st));
-    weighted_counts(b, dw.as<int32_t>(), counts.as<int64_t>(), st);
+    weighted_counts(b, w.data(), (int64_t)w.size(), counts.as<int64_t>(), st);
     std::vector<int64_t> cnt((size_t)M * 4);
     CS_HIP(hipMemcpyAsync(cnt.data(), counts.p, cnt.size() * 8, hipMemcpyDeviceToHost, st));
     CS_HIP(hipStreamSynchronize(st));
