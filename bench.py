@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 INT8_PEAK_TOPS = 5000.0
+FP16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
@@ -83,6 +84,27 @@ def grm_leg(csa, reader, N, M):
                 roofline=dict(bound="mfma", achieved=round(achieved, 1), peak=INT8_PEAK_TOPS,
                               unit="TOP/s (int8)", frac=round(achieved / INT8_PEAK_TOPS, 4),
                               kernel="k_grm_pass (all launches)"))
+
+
+def grm_std_leg(csa, reader, N, M):
+    """Standardised GRM (alg 0: FP16 hi/lo split products on MFMA, FP32 accumulation flushed to
+    FP64) of the whole workload into a device-resident GCTA triangle.  The roofline counts the
+    FP16 MFMA flops issued (3 split products per lower-triangle wave tile) over the product time."""
+    nt = N * (N + 1) // 2
+    d_tri = csa.DeviceArray((nt + 1) // 2)
+    csa.synchronize()
+    t0 = time.perf_counter()
+    s2pq, nv = csa.grm_build_device(reader, d_tri.ptr, alg=0)
+    csa.synchronize()
+    dt = time.perf_counter() - t0
+    ms_prod, ms_fin, launches, flops = csa.GrmBuilder().last_timing_ms()
+    achieved = flops / (ms_prod * 1e-3) / 1e12
+    d_tri.close()
+    return dict(ms=dt * 1e3, ms_products=ms_prod, ms_final=ms_fin, launches=launches,
+                fp16_flops_issued=flops, n_valid=nv, tops_full_square=2.0 * N * N * M / dt / 1e12,
+                roofline=dict(bound="mfma", achieved=round(achieved, 1), peak=FP16_PEAK_TFLOPS,
+                              unit="TFLOP/s (fp16)", frac=round(achieved / FP16_PEAK_TFLOPS, 4),
+                              kernel="k_grm_std (all launches)"))
 
 
 def scan_leg(csa, reader, N, M, T=16):
@@ -264,6 +286,10 @@ def main():
         grm = grm_leg(csa, reader, N, M)
         log(f"GRM: {grm['ms']:.1f} ms, {grm['tops_full_square']:.1f} TOPS (2 n^2 m), "
             f"int8 products {grm['ms_products']:.1f} ms in {grm['launches']} launches")
+        grm["standardised"] = grm_std_leg(csa, reader, N, M)
+        gs = grm["standardised"]
+        log(f"standardised GRM: {gs['ms']:.1f} ms, {gs['tops_full_square']:.1f} TOPS (2 n^2 m), "
+            f"fp16 products {gs['ms_products']:.1f} ms = {gs['roofline']['achieved']} TFLOP/s issued")
 
     scan = None
     if not args.no_scan:

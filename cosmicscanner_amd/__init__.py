@@ -480,9 +480,11 @@ def lmm_scan(reader, keep, R, n_score=1, alpha=0.0, quad=None, quad_w=None, cov=
 # ----------------------------------------------------------------------------- GRM (kernel b)
 class GrmBuilder:
     """Dense GRM build (GrmBuilder, libCosmicCore/src/grm_builder.h:15-80; compute at
-    grm_builder.cpp:63-399, save at :1977-2026) for algorithm 1 (VanRaden), computed exactly on
-    the GPU: the lower triangle is the correctly rounded float32 of Z Z^T / sum_2pq, getGrm()
-    the FP64 matrix (correctly rounded raw Z Z^T divided by sum_2pq)."""
+    grm_builder.cpp:63-399, save at :1977-2026).  Algorithm 1 (VanRaden, the default) is computed
+    exactly on int8 MFMA: the lower triangle is the correctly rounded float32 of Z Z^T / sum_2pq,
+    getGrm() the FP64 matrix (correctly rounded raw Z Z^T divided by sum_2pq).  Algorithms 0 and 3
+    (standardised, G = Z Z^T / M) run as FP16 hi/lo split products on MFMA with FP32 accumulation
+    flushed to FP64 every 16384 SNPs."""
 
     class Options:
         def __init__(self, min_maf=0.01, algorithm=1, thread_num=1, block_size=2048):
@@ -522,6 +524,8 @@ class GrmBuilder:
     def getGrm(self):
         if self.raw is None:
             raise RuntimeError("GRM not computed with fp64=True")
+        if self.options.algorithm in (0, 3):
+            return self.raw / self.total_valid_snps if self.total_valid_snps > 0 else self.raw.copy()
         return self.raw / self.total_sum_2pq if self.total_sum_2pq > 0 else self.raw.copy()
 
     def getRaw(self):
@@ -557,11 +561,11 @@ class GrmBuilder:
             out_prefix + ".grm.N.bin")
 
 
-def grm_build_device(reader, d_tri_ptr, min_maf=0.01):
+def grm_build_device(reader, d_tri_ptr, min_maf=0.01, alg=1):
     """GRM lower triangle into a device buffer of n(n+1)/2 floats; returns (sum_2pq, n_valid)."""
     s = ctypes.c_double()
     nv = ctypes.c_int64()
-    _check(lib().cs_grm_build_device(reader._h, 1, float(min_maf), d_tri_ptr, ctypes.byref(s),
+    _check(lib().cs_grm_build_device(reader._h, int(alg), float(min_maf), d_tri_ptr, ctypes.byref(s),
                                      ctypes.byref(nv)))
     return s.value, nv.value
 
@@ -659,7 +663,7 @@ def grm_loco_subset(d_raw, d_raw_chr, N, keep, S_global, S_chr, d_out):
 # ------------------------------------------------------------------------------- GWAS driver
 class GwasOptions(ctypes.Structure):
     _fields_ = [("min_maf", ctypes.c_double), ("no_loco", ctypes.c_int),
-                ("scan_chr", ctypes.c_char_p), ("prec", ctypes.c_int)]
+                ("scan_chr", ctypes.c_char_p), ("prec", ctypes.c_int), ("grm_alg", ctypes.c_int)]
 
 
 class GwasSummary(ctypes.Structure):
@@ -737,9 +741,10 @@ def gwas_matrix_free_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=F
 
 
 def gwas_dense_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=False, scan_chr=None,
-                   prec=CS_PREC_INT8_L3):
+                   prec=CS_PREC_INT8_L3, grm_alg=1):
     """The default dense PLINK GWAS of GwasScanner::runLocoScan end to end on the GPU; writes
-    out_prefix.assoc.txt / .scan.meta.txt / .cosmic.sumstats and returns the run summary."""
+    out_prefix.assoc.txt / .scan.meta.txt / .cosmic.sumstats and returns the run summary.
+    grm_alg is --make-grm-alg: 1 (VanRaden, exact) or 0 / 3 (standardised, FP16-split MFMA)."""
     y = np.ascontiguousarray(y, dtype=np.float64)
     N = y.size
     if cov is None:
@@ -748,7 +753,7 @@ def gwas_dense_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=False, 
         covf = np.asfortranarray(np.asarray(cov, dtype=np.float64).reshape(N, -1))
         nc = covf.shape[1]
     o = GwasOptions(float(min_maf), int(bool(no_loco)),
-                    None if scan_chr is None else str(scan_chr).encode(), int(prec))
+                    None if scan_chr is None else str(scan_chr).encode(), int(prec), int(grm_alg))
     sm = GwasSummary()
     _check(lib().cs_gwas_dense_run(str(bfile).encode(), _p(y, _dp), _p(covf, _dp), N, nc,
                                    ctypes.byref(o), str(out_prefix).encode(), ctypes.byref(sm)))

@@ -285,6 +285,151 @@ __global__ void k_grm_combine(CombineArgs a) {
     }
 }
 
+// ------------------------------------------------------------- standardised GRM (algorithms 0, 3)
+// GrmBuilder::compute with --make-grm-alg 0/3 (grm_builder.cpp:241-243, 300-304, 386-387): per
+// valid SNP z = (x - mu_j) s_j with the float32 mean mu_j = fl32(2p) and the float32 scale
+// s_j = fl32(1/sqrt(2p(1-p))), missing -> 0, G = Z Z^T / M.  The three non-zero values of each
+// SNP are exact in FP64 and split into FP16 hi + lo (z = hi + lo + O(2^-22 z)); the product is
+// Z_hi Z_hi^T + Z_hi Z_lo^T + Z_lo Z_hi^T on FP16 MFMA with FP32 accumulation, flushed into an
+// FP64 accumulator every chunk of SNPs.
+//
+// One 256 x 256 lower-triangle block per workgroup of 8 waves.  Per 32-SNP step the block
+// decodes its 512 sample rows (256 of the row block, 256 of the column block) from the packed
+// sample-major tiles into LDS through per-SNP lookup tables (v_perm on the 2-bit codes), double
+// buffered; wave (wr, wc) multiplies rows 64 wr .. +64 by columns 128 wc .. +128.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kStdRow = 144;                 // bytes per decoded row: 32 hi + 32 lo FP16 + 16 pad
+constexpr int kStdBuf = 512 * kStdRow;       // one buffer: 512 rows
+constexpr int64_t kStdChunkSteps = 512;      // 32-SNP steps per FP32 accumulation chunk
+
+struct StdArgs {
+    const uint8_t* smp;    // tiled sample-major genotypes
+    int64_t KB;            // 256-SNP blocks per row tile
+    int64_t s0, s1;        // 32-SNP step range of this launch
+    const int2* blocks;    // (bi, bk) 256-row block pairs, bk <= bi
+    const uint4* tab;      // per SNP: {hi(c0)|hi(c1)<<16, hi(c2)|hi(c3)<<16, lo.., lo..}
+    int64_t I0, P, N, Npad;
+    double* acc;           // panel accumulator [P][Npad]
+};
+
+__device__ __forceinline__ uint32_t std_sel(uint32_t c, bool high) {
+    // v_perm selector taking FP16 entry c of an 8-byte table into the low (or high) half, zero
+    // elsewhere (selector byte 0x0c gives 0x00)
+    return high ? c * 0x02020000u + 0x01000c0cu : c * 0x0202u + 0x0c0c0100u;
+}
+
+__device__ __forceinline__ void std_decode(const uint8_t* __restrict__ src, const uint4* tq, uint8_t* dst) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t w = src[(i >> 1) * 1024 + (i & 1) * 128];
+        const uint32_t s0 = std_sel(w & 3u, false), s1 = std_sel((w >> 2) & 3u, true);
+        const uint32_t s2 = std_sel((w >> 4) & 3u, false), s3 = std_sel(w >> 6, true);
+        uint2 hi, lo;
+        hi.x = __builtin_amdgcn_perm(tq[0].y, tq[0].x, s0) | __builtin_amdgcn_perm(tq[1].y, tq[1].x, s1);
+        hi.y = __builtin_amdgcn_perm(tq[2].y, tq[2].x, s2) | __builtin_amdgcn_perm(tq[3].y, tq[3].x, s3);
+        lo.x = __builtin_amdgcn_perm(tq[0].w, tq[0].z, s0) | __builtin_amdgcn_perm(tq[1].w, tq[1].z, s1);
+        lo.y = __builtin_amdgcn_perm(tq[2].w, tq[2].z, s2) | __builtin_amdgcn_perm(tq[3].w, tq[3].z, s3);
+        uint8_t* row = dst + (i * 8) * kStdRow;
+        *reinterpret_cast<uint2*>(row) = hi;
+        *reinterpret_cast<uint2*>(row + 64) = lo;
+    }
+}
+
+__global__ __launch_bounds__(512) void k_grm_std(StdArgs a) {
+    __shared__ __align__(16) uint8_t lds[2 * kStdBuf];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int2 blk = a.blocks[blockIdx.x];
+    // decode role: byte column db (4 SNPs) of local rows wave*64 + drl + 8 i, i = 0..7
+    const int db = lane & 7, drl = lane >> 3;
+    const int64_t rbase = wave < 4 ? (int64_t)blk.x * 256 + wave * 64 : (int64_t)blk.y * 256 + (wave - 4) * 64;
+    const uint8_t* tile = a.smp + (((rbase >> 6) * a.KB) << 12) + (drl << 4) + db;
+    const int dst_off = (wave * 64 + drl) * kStdRow + 8 * db;
+    // MFMA role
+    const int wr = wave >> 1, wc = wave & 1, g = lane >> 4, r = lane & 15;
+    const bool active = !(blk.x == blk.y && 128 * wc >= 64 * wr + 64);
+    f32x4 acc[4][8];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 8; ++n) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto step_src = [&](int64_t s) {
+        return tile + (((s >> 3)) << 12) + ((((s & 7) >> 1) * 16) << 4) + (s & 1) * 8;
+    };
+    uint4 tq[4];
+    {
+        const uint4* tp = a.tab + a.s0 * 32 + 4 * db;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tq[q] = tp[q];
+        std_decode(step_src(a.s0), tq, lds + dst_off);
+    }
+    __syncthreads();
+    const int aoff = (64 * wr + r) * kStdRow + 16 * g;
+    const int boff = (256 + 128 * wc + r) * kStdRow + 16 * g;
+    for (int64_t s = a.s0; s < a.s1; ++s) {
+        const int cur = (int)((s - a.s0) & 1);
+        const bool more = s + 1 < a.s1;
+        if (more) {
+            const uint4* tp = a.tab + (s + 1) * 32 + 4 * db;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) tq[q] = tp[q];
+        }
+        if (active) {
+            const uint8_t* buf = lds + cur * kStdBuf;
+            f16x8 ah[4], al[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                ah[m] = *reinterpret_cast<const f16x8*>(buf + aoff + 16 * m * kStdRow);
+                al[m] = *reinterpret_cast<const f16x8*>(buf + aoff + 16 * m * kStdRow + 64);
+            }
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                const f16x8 bh = *reinterpret_cast<const f16x8*>(buf + boff + 16 * n * kStdRow);
+                const f16x8 bl = *reinterpret_cast<const f16x8*>(buf + boff + 16 * n * kStdRow + 64);
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bh, acc[m][n], 0, 0, 0);
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[m], bl, acc[m][n], 0, 0, 0);
+                    acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[m], bh, acc[m][n], 0, 0, 0);
+                }
+            }
+        }
+        if (more) std_decode(step_src(s + 1), tq, lds + (cur ^ 1) * kStdBuf + dst_off);
+        __syncthreads();
+    }
+    if (!active) return;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 8; ++n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t i = (int64_t)blk.x * 256 + 64 * wr + 16 * m + 4 * g + v;
+                const int64_t k = (int64_t)blk.y * 256 + 128 * wc + 16 * n + r;
+                if (i >= a.N || k > i || i < a.I0 || i >= a.I0 + a.P) continue;
+                a.acc[(i - a.I0) * a.Npad + k] += (double)acc[m][n][v];
+            }
+}
+
+// Panel output of the standardised GRM: raw Z Z^T (FP64, both triangles) and fl32(raw / M).
+__global__ void k_grm_std_final(const double* __restrict__ acc, int64_t I0, int64_t P, int64_t N, int64_t Npad,
+                                double M, float* __restrict__ tri, double* __restrict__ raw) {
+    const int64_t total = P * Npad;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t li = e / Npad, k = e - li * Npad;
+        const int64_t i = I0 + li;
+        if (i >= N || k > i) continue;
+        const double g = acc[e];
+        if (tri) tri[i * (i + 1) / 2 + k] = (float)(g / M);
+        if (raw) {
+            raw[i * N + k] = g;
+            raw[k * N + i] = g;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------ host
 struct GrmParams {
     std::vector<uint8_t> valid;
@@ -387,8 +532,181 @@ struct GrmOut {
     int64_t n_valid = 0;
 };
 
+// Per-SNP FP16 hi/lo lookup tables of the standardised GRM (grm_builder.cpp:167-257, 283-305):
+// filter and float32 mean/scale as GrmBuilder::compute computes them for algorithms 0 and 3.
+static std::vector<uint4> grm_std_tables(cs_bed* b, float min_maf, const int32_t* snps, int64_t n_snps,
+                                         GrmOut& out, int64_t& s_lo, int64_t& s_hi) {
+    std::vector<int64_t> cnt((size_t)b->M * 4);
+    CS_HIP(hipMemcpy(cnt.data(), b->counts.p, cnt.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> sel;
+    if (snps) {
+        sel.assign((size_t)b->M, 0);
+        for (int64_t i = 0; i < n_snps; ++i) {
+            if (snps[i] < 0 || snps[i] >= b->M) throw RuntimeError("GRM: variant index out of range");
+            sel[(size_t)snps[i]] = 1;
+        }
+    }
+    std::vector<uint4> tab((size_t)b->Mpad, uint4{0, 0, 0, 0});
+    auto h16 = [](double v) -> uint32_t {
+        const _Float16 h = (_Float16)v;
+        uint16_t u;
+        std::memcpy(&u, &h, 2);
+        return u;
+    };
+    double total = 0.0, block = 0.0, seq = 0.0;
+    int64_t nv = 0, lo = b->M, hi = -1;
+    for (int64_t j = 0; j < b->M; ++j) {
+        if (j % 2048 == 0) {
+            total += block;
+            block = 0.0;
+        }
+        if (snps && !sel[(size_t)j]) continue;
+        const int64_t* c = &cnt[(size_t)j * 4];
+        const long long n_obs = c[0] + c[2] + c[3];
+        const long long sum_x = 2 * c[0] + c[2];
+        if (n_obs == 0) continue;
+        const double freq = (double)sum_x / (2.0 * (double)n_obs);
+        const double maf = std::min(freq, 1.0 - freq);
+        if (maf < (double)min_maf || std::fabs(maf) < 1e-8) continue;
+        const float pq2f = (float)(1.0f * 2.0 * freq * (1.0 - freq));
+        block += pq2f;
+        seq += pq2f;
+        const double mean = (double)(float)(freq * 2.0);
+        const double pq2 = 2.0 * freq * (1.0 - freq);
+        const double scale = (double)(float)(pq2 > 1e-9 ? 1.0 / std::sqrt(pq2) : 0.0);
+        const double z[4] = {(2.0 - mean) * scale, 0.0, (1.0 - mean) * scale, (0.0 - mean) * scale};
+        uint32_t zh[4], zl[4];
+        for (int q = 0; q < 4; ++q) {
+            zh[q] = h16(z[q]);
+            _Float16 hh;
+            uint16_t u = (uint16_t)zh[q];
+            std::memcpy(&hh, &u, 2);
+            zl[q] = h16(z[q] - (double)hh);
+        }
+        tab[(size_t)j] = uint4{zh[0] | zh[1] << 16, zh[2] | zh[3] << 16, zl[0] | zl[1] << 16, zl[2] | zl[3] << 16};
+        ++nv;
+        lo = std::min(lo, j);
+        hi = std::max(hi, j);
+    }
+    total += block;
+    out.sum_2pq = total;
+    out.sum_2pq_seq = seq;
+    out.n_valid = nv;
+    s_lo = hi >= 0 ? lo / 32 : 0;
+    s_hi = hi >= 0 ? hi / 32 + 1 : 0;
+    return tab;
+}
+
+static void grm_std_run(cs_bed* b, float min_maf, const int32_t* snps, int64_t n_snps, GrmOut& out) {
+    ensure_stats(b);
+    ensure_sample_major(b);
+    int64_t s_lo = 0, s_hi = 0;
+    std::vector<uint4> tab = grm_std_tables(b, min_maf, snps, n_snps, out, s_lo, s_hi);
+    const int64_t N = b->N, Npad = b->Npad, KB = b->KBm;
+    hipStream_t st;
+    CS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1, e2;
+    CS_HIP(hipEventCreate(&e0));
+    CS_HIP(hipEventCreate(&e1));
+    CS_HIP(hipEventCreate(&e2));
+    try {
+        DevBuf dtab(tab.size() * sizeof(uint4));
+        CS_HIP(hipMemcpyAsync(dtab.p, tab.data(), dtab.bytes, hipMemcpyHostToDevice, st));
+        DevBuf tri_own;
+        float* tri = out.tri_dev;
+        if (!tri && out.tri_host) {
+            tri_own.alloc((size_t)(N * (N + 1) / 2) * sizeof(float));
+            tri = tri_own.as<float>();
+        }
+        DevBuf raw_own;
+        double* raw = out.raw_dev;
+        if (!raw && out.raw_host) {
+            raw_own.alloc((size_t)(N * N) * sizeof(double));
+            raw = raw_own.as<double>();
+        }
+        size_t panel_bytes = 8ull << 30;
+        if (const char* env = std::getenv("CSAMD_GRM_PANEL_BYTES")) panel_bytes = std::strtoull(env, nullptr, 10);
+        int64_t P = std::max<int64_t>(256, (int64_t)(panel_bytes / ((size_t)Npad * 8)) / 256 * 256);
+        P = std::min<int64_t>(P, Npad);
+        DevBuf acc((size_t)(P * Npad) * 8);
+        float ms_pass = 0.f, ms_fin = 0.f;
+        int launches = 0;
+        double ops = 0;
+        const double Mv = (double)out.n_valid;
+        for (int64_t I0 = 0; I0 < Npad; I0 += P) {
+            const int64_t rows = std::min<int64_t>(P, Npad - I0);
+            std::vector<int2> blocks;
+            int64_t wave_tiles = 0;   // 64 x 128 wave tiles issuing MFMAs
+            for (int64_t bi = I0 / 256; bi < (I0 + rows) / 256; ++bi)
+                for (int64_t bk = 0; bk <= bi; ++bk) {
+                    blocks.push_back(int2{(int)bi, (int)bk});
+                    wave_tiles += bi == bk ? 6 : 8;
+                }
+            DevBuf dblocks(blocks.size() * sizeof(int2));
+            CS_HIP(hipMemcpyAsync(dblocks.p, blocks.data(), dblocks.bytes, hipMemcpyHostToDevice, st));
+            CS_HIP(hipMemsetAsync(acc.p, 0, (size_t)(rows * Npad) * 8, st));
+            CS_HIP(hipEventRecord(e0, st));
+            StdArgs sa{};
+            sa.smp = b->smp.as<uint8_t>();
+            sa.KB = KB;
+            sa.blocks = dblocks.as<int2>();
+            sa.tab = dtab.as<uint4>();
+            sa.I0 = I0;
+            sa.P = rows;
+            sa.N = N;
+            sa.Npad = Npad;
+            sa.acc = acc.as<double>();
+            for (int64_t c0 = s_lo; c0 < s_hi; c0 += kStdChunkSteps) {
+                sa.s0 = c0;
+                sa.s1 = std::min<int64_t>(s_hi, c0 + kStdChunkSteps);
+                k_grm_std<<<(unsigned)blocks.size(), 512, 0, st>>>(sa);
+                ++launches;
+            }
+            CS_HIP(hipGetLastError());
+            CS_HIP(hipEventRecord(e1, st));
+            ops += (double)wave_tiles * 3.0 * 64.0 * 128.0 * 2.0 * (double)((s_hi - s_lo) * 32);
+            k_grm_std_final<<<grid_for(rows * Npad), 256, 0, st>>>(acc.as<double>(), I0, rows, N, Npad,
+                                                                  Mv > 0 ? Mv : 1.0, tri, raw);
+            CS_HIP(hipGetLastError());
+            CS_HIP(hipEventRecord(e2, st));
+            CS_HIP(hipEventSynchronize(e2));
+            float a1 = 0, a2 = 0;
+            CS_HIP(hipEventElapsedTime(&a1, e0, e1));
+            CS_HIP(hipEventElapsedTime(&a2, e1, e2));
+            ms_pass += a1;
+            ms_fin += a2;
+        }
+        if (out.tri_host)
+            CS_HIP(hipMemcpyAsync(out.tri_host, tri, (size_t)(N * (N + 1) / 2) * sizeof(float),
+                                  hipMemcpyDeviceToHost, st));
+        if (out.raw_host)
+            CS_HIP(hipMemcpyAsync(out.raw_host, raw, (size_t)(N * N) * sizeof(double), hipMemcpyDeviceToHost, st));
+        CS_HIP(hipStreamSynchronize(st));
+        g_grm_timing.ms_passes = ms_pass;
+        g_grm_timing.ms_combine = ms_fin;
+        g_grm_timing.passes = launches;
+        g_grm_timing.mfma_ops = ops;
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+        (void)hipStreamDestroy(st);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    CS_HIP(hipStreamDestroy(st));
+}
+
 static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int64_t n_snps, GrmOut& out) {
-    if (alg != 1) throw InvalidArgument("GRM: only algorithm 1 (VanRaden, the default) is built here");
+    if (alg == 0 || alg == 3) {
+        grm_std_run(b, min_maf, snps, n_snps, out);
+        return;
+    }
+    if (alg != 1)
+        throw InvalidArgument("GRM: algorithms 0/3 (standardised, Yang) and 1 (VanRaden) are built here");
     ensure_stats(b);
     ensure_sample_major(b);
     GrmParams prm = grm_params(b, min_maf, snps, n_snps);

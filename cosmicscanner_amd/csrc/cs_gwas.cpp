@@ -279,7 +279,7 @@ extern "C" int cs_gwas_dense_run(const char* bfile, const double* y, const doubl
     return csamd::guarded([&] {
         if (!bfile || !y || !out_prefix || (n_cov > 0 && !cov))
             throw InvalidArgument("cs_gwas_dense_run: null argument");
-        cs_gwas_options o{0.01, 0, nullptr, CS_PREC_INT8_L3};
+        cs_gwas_options o{0.01, 0, nullptr, CS_PREC_INT8_L3, 1};
         if (opt) o = *opt;
         const std::string out(out_prefix);
         Inputs in;
@@ -294,7 +294,9 @@ extern "C" int cs_gwas_dense_run(const char* bfile, const double* y, const doubl
         DevMem raw(N * N * 8), gsub(n * n * 8);
         double S_global = 0.0;
         int64_t nv = 0;
-        chk(cs_grm_build_raw(in.bed.h, 1, (float)o.min_maf, nullptr, 0, raw.d(), 1, &S_global, nullptr, &nv));
+        const bool std_alg = o.grm_alg == 0 || o.grm_alg == 3;
+        chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, nullptr, 0, raw.d(), 1, &S_global, nullptr, &nv));
+        if (std_alg) S_global = (double)nv;
         ms_grm += ms_since(t0);
         LmmHandle lmm;
         chk(cs_lmm_create(in.y.data(), in.X.data(), n, n, in.c, &lmm.h));
@@ -338,8 +340,10 @@ extern "C" int cs_gwas_dense_run(const char* bfile, const double* y, const doubl
                 const std::vector<int32_t> snps = chrom_snps(in, chrom);
                 auto t1 = std::chrono::steady_clock::now();
                 double S_chr = 0.0;
-                chk(cs_grm_build_raw(in.bed.h, 1, (float)o.min_maf, snps.data(), (int64_t)snps.size(), raw_chr.d(), 1,
-                                     nullptr, &S_chr, nullptr));
+                int64_t nv_chr = 0;
+                chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, snps.data(), (int64_t)snps.size(),
+                                     raw_chr.d(), 1, nullptr, &S_chr, &nv_chr));
+                if (std_alg) S_chr = (double)nv_chr;
                 chk(cs_grm_loco_subset(raw.d(), raw_chr.d(), N, in.keep.data(), n, S_global, S_chr, gsub.d()));
                 ms_grm += ms_since(t1);
                 run_scan_with_grm(snps, chrom);

@@ -352,6 +352,8 @@ typedef struct cs_gwas_options {
     int no_loco;           /* --no-loco */
     const char* scan_chr;  /* scan only this chromosome (NULL = all) */
     int prec;              /* scan precision, CS_PREC_INT8_L3 by default */
+    int grm_alg;           /* --make-grm-alg: 1 (VanRaden, default) or 0 / 3 (standardised; the
+                              LOCO scaling factors are then SNP counts, gwas_scanner.cpp:189,481) */
 } cs_gwas_options;
 #define CS_GWAS_MAX_MODELS 64
 typedef struct cs_gwas_summary {

@@ -369,14 +369,15 @@ def select_snps(bed, N, snps):
     return np.ascontiguousarray(a[np.asarray(snps, dtype=np.int64)]).reshape(-1)
 
 
-def gwas_dense(bed, N, M, chroms, y, cov=None, min_maf=0.01, no_loco=False, scan_chr=None):
+def gwas_dense(bed, N, M, chroms, y, cov=None, min_maf=0.01, no_loco=False, scan_chr=None, grm_alg=1):
     """GwasScanner::runLocoScan's default dense PLINK path (src/gwas_scanner.cpp:74-525) restated
     on the CPU: DataFilter keep (finite y and float32 covariates), X = [1, cov] as float32, exact
     GRM and per-chromosome raw GRMs, LOCO formula (:481-495), EigenVce::prepare + solveDiagonal,
     rotated scan per chromosome (chromosome labels in lexicographic order).
 
     Returns (models [(label, vg, ve)], rows [(snp_index, freq, beta, se, p, log10p)]) with rows
-    in output order and untested SNPs dropped."""
+    in output order and untested SNPs dropped.  grm_alg 0 / 3: the standardised GRM (FP64) with the
+    SNP counts as the LOCO scaling factors (gwas_scanner.cpp:189-191, 481-483)."""
     y = np.asarray(y, dtype=np.float64)
     covf = None if cov is None else np.asarray(cov, dtype=np.float64).reshape(N, -1)
     ok = ~np.isnan(y)
@@ -391,8 +392,13 @@ def gwas_dense(bed, N, M, chroms, y, cov=None, min_maf=0.01, no_loco=False, scan
         X[:, 1:] = covf[keep].astype(np.float32)
     Xd = X.astype(np.float64)
     ys = y[keep]
-    raw, _, S, _ = grm_exact_centred(bed, N, M, min_maf)
-    G_global = raw / S
+    std_alg = grm_alg in (0, 3)
+    if std_alg:
+        G_global, _, S = grm(bed, N, M, 0, min_maf)
+        raw = G_global * S
+    else:
+        raw, _, S, _ = grm_exact_centred(bed, N, M, min_maf)
+        G_global = raw / S
     labels = sorted(set(chroms))
     use_loco = len(labels) > 1 and not no_loco
     models, rows = [], []
@@ -418,8 +424,12 @@ def gwas_dense(bed, N, M, chroms, y, cov=None, min_maf=0.01, no_loco=False, scan
             if scan_chr is not None and c != scan_chr:
                 continue
             snps = [j for j in range(M) if chroms[j] == c]
-            raw_c = grm_exact_centred(select_snps(bed, N, snps), N, len(snps), min_maf)[0]
-            S_chr = sequential_sum_2pq(bed, N, M, min_maf, snps)
+            if std_alg:
+                Gc, _, S_chr = grm(select_snps(bed, N, snps), N, len(snps), 0, min_maf)
+                raw_c = Gc * S_chr
+            else:
+                raw_c = grm_exact_centred(select_snps(bed, N, snps), N, len(snps), min_maf)[0]
+                S_chr = sequential_sum_2pq(bed, N, M, min_maf, snps)
             S_loco = S - S_chr
             G = Gsub
             if S_chr > 0.0 and S_loco > 1e-6:

@@ -12,6 +12,13 @@ Cases: the reference's committed BED fixtures, the kinship-batch fixture (partia
 monomorphic and all-missing SNPs), device-generated HWE data with and without missing genotypes
 at sample counts that are not multiples of the 128-row block, the multi-panel path, and the
 GCTA writer (GrmBuilder::save, grm_builder.cpp:1977-2026).
+
+The standardised GRM (algorithms 0 and 3, grm_builder.cpp:241-243, 386-387) runs as FP16 hi/lo
+split products with FP32 accumulation (flushed to FP64 every 16384 SNPs).  Its bound against the
+oracle's FP64 restatement is max|G_gpu - G_fp64| <= STD_TOL * max diag(G_fp64): the split alone
+contributes ~5e-8, the FP32 accumulation the rest (about 1.4e-6 at n=400, m=6000 in an emulation
+of the same rounding; it grows with the chunk length, about chunk / sqrt(m) x 1e-7 of the diagonal,
+so 1-3e-6 for the 16384-SNP chunks at m <= 40000 and below 1e-6 at m = 500000).  The valid-SNP count and sum_2pq are compared exactly.
 """
 import os
 
@@ -99,8 +106,8 @@ def test_grm_device_output(csa):
 def test_grm_errors(csa, kb_fixture):
     bed, N, M, _ = kb_fixture
     r = csa.PlinkReader.from_body(bed, N, M)
-    with pytest.raises(ValueError, match="algorithm 1"):
-        csa.GrmBuilder(csa.GrmBuilder.Options(algorithm=0)).compute(r)
+    with pytest.raises(ValueError, match="algorithms 0/3"):
+        csa.GrmBuilder(csa.GrmBuilder.Options(algorithm=2)).compute(r)
     with pytest.raises(RuntimeError, match="subsetting"):
         csa.GrmBuilder().compute(r, keep_indices=[0, 1])
     with pytest.raises(RuntimeError, match="not computed"):
@@ -122,3 +129,67 @@ def test_grm_save_gcta(csa, ref_datasets, tmp_path):
     assert nb.size == N * (N + 1) // 2 and np.all(nb == np.float32(nv))
     ids = open(out + ".grm.id").read().splitlines()
     assert ids == [f"{row[0]}\t{row[1]}" for row in fam]
+
+
+# ------------------------------------------------------------------ standardised GRM (alg 0, 3)
+STD_TOL = 8e-6
+
+
+def check_std(csa, reader, body, N, M, min_maf=0.01, alg=0):
+    b = csa.GrmBuilder(csa.GrmBuilder.Options(min_maf=min_maf, algorithm=alg))
+    b.compute(reader)
+    G64, s64, nv64 = oracle.grm(body, N, M, 0, min_maf)
+    assert b.getTotalSnps() == nv64
+    assert b.getSum2pq() == s64
+    G = b.getGrm()
+    scale = np.max(np.abs(np.diag(G64)))
+    err = np.max(np.abs(G - G64))
+    print(f"  standardised GRM max|dG| = {err:.3e} ({err / scale:.3e} of max diag)", flush=True)
+    assert err <= STD_TOL * scale
+    assert np.array_equal(G, G.T)
+    tri = b.getLowerTriangle()
+    assert np.array_equal(tri, lower(G).astype(np.float32))
+    return b, err / scale
+
+
+def test_grm_std_reference_fixtures(csa, ref_datasets):
+    for name, (body, N, M, _, _) in ref_datasets.items():
+        print(f"standardised GRM fixture {name}: N={N} M={M}", flush=True)
+        check_std(csa, csa.PlinkReader.from_body(body, N, M), body, N, M)
+
+
+def test_grm_std_kinship_fixture(csa, kb_fixture):
+    bed, N, M, _ = kb_fixture
+    check_std(csa, csa.PlinkReader.from_body(bed, N, M), bed, N, M, alg=3)
+
+
+@pytest.mark.parametrize("N,M,miss", [(301, 1000, 0.02), (700, 3000, 0.005), (517, 2200, 0.0)])
+def test_grm_std_synthetic(csa, N, M, miss):
+    print(f"standardised GRM synthetic N={N} M={M} miss={miss}", flush=True)
+    r = csa.PlinkReader.synthetic(191 + N, N, M, 0.01, 0.5, miss)
+    check_std(csa, r, r.download(), N, M)
+
+
+def test_grm_std_two_chunks_and_panels(csa, monkeypatch):
+    # 40000 SNPs span three FP32 accumulation chunks; a panel bound of one 256-row block forces
+    # the row-panel loop
+    N, M = 600, 40000
+    monkeypatch.setenv("CSAMD_GRM_PANEL_BYTES", "1")
+    r = csa.PlinkReader.synthetic(17, N, M, 0.01, 0.5, 0.01)
+    check_std(csa, r, r.download(), N, M)
+
+
+def test_grm_std_min_maf_and_device(csa):
+    N, M = 280, 1500
+    r = csa.PlinkReader.synthetic(23, N, M, 0.0, 0.5, 0.01)
+    body = r.download()
+    for maf in (0.05, 0.2):
+        check_std(csa, r, body, N, M, min_maf=maf)
+    nt = N * (N + 1) // 2
+    d = csa.DeviceArray((nt + 1) // 2)
+    s2pq, nv = csa.grm_build_device(r, d.ptr, alg=0)
+    tri = d.download().reshape(-1).view(np.float32)[:nt]
+    b = csa.GrmBuilder(csa.GrmBuilder.Options(algorithm=0))
+    b.compute(r, fp64=False)
+    assert np.array_equal(tri, b.getLowerTriangle())
+    assert (s2pq, nv) == (b.getSum2pq(), b.getTotalSnps())

@@ -11,6 +11,9 @@ Tolerances.
 * Scan statistics (int8 MFMA products of 23-bit fixed-point projections, FP64 epilogue):
   |d log10 P| <= 1e-4, |d beta| <= 1e-5 SE, SE relative 1e-5.  Printed .assoc.txt values carry the
   reference's 6 significant digits on top (5e-6 relative); FREQ prints identically.
+* With the standardised GRM (--make-grm-alg 0: FP16-split MFMA products, FP32 accumulation, GRM
+  entries within ~8e-6 of FP64) the bounds are the path's stated ones: variance components 1e-4
+  relative, |d log10 P| <= 1e-3, |d beta| <= 1e-3 SE, SE relative 1e-4.
 """
 import os
 
@@ -172,6 +175,28 @@ def test_gwas_options(csa, synth_gwas, tmp_path):
     compare_rows([f"rs{j:06d}" for j in range(d["M"])], rg2, rows2)
     with pytest.raises(RuntimeError, match="Too few valid samples"):
         csa.gwas_dense_run(d["prefix"], np.full(d["N"], np.nan), str(tmp_path / "x"))
+
+
+def test_gwas_standardised_grm(csa, synth_gwas, tmp_path):
+    d = synth_gwas
+    out = str(tmp_path / "alg0")
+    sm = csa.gwas_dense_run(d["prefix"], d["y"], out, cov=d["cov"], grm_alg=0)
+    models, rows = oracle.gwas_dense(d["body"], d["N"], d["M"], d["chroms"], d["y"], d["cov"],
+                                     grm_alg=0)
+    assert sm["loco"] and [m[0] for m in sm["models"]] == ["1", "10", "2"]
+    for (lg, vg, ve), (lo, vgo, veo) in zip(sm["models"], models):
+        assert lg == lo and abs(vg - vgo) <= 1e-4 * vgo and abs(ve - veo) <= 1e-4 * veo
+    _, rg = read_assoc(out + ".assoc.txt")
+    ids = [f"rs{j:06d}" for j in range(d["M"])]
+    assert [r[1] for r in rg] == [ids[r[0]] for r in rows]
+    worst = 0.0
+    for r, (_, freq, beta, se, p, l10) in zip(rg, rows):
+        assert r[5] == f"{freq:.6g}"
+        b, s_, lp = float(r[7]), float(r[8]), float(r[10])
+        assert abs(b - beta) <= 1e-3 * se and abs(s_ - se) <= 1e-4 * se
+        assert abs(lp - l10) <= 1e-3
+        worst = max(worst, abs(lp - l10))
+    print(f"  standardised-GRM scan: max |d log10 P| = {worst:.2e}", flush=True)
 
 
 @pytest.mark.parametrize("name,pheno", [("synth", "ref_synth/quantitative.pheno"),
