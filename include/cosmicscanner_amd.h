@@ -167,8 +167,13 @@ int cs_lmm_scan(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* 
  *   grm_raw_f64: n x n doubles, the correctly rounded FP64 of Z Z^T (before normalisation,
  *                symmetric).  May be NULL.
  *   sum_2pq / n_valid: the normaliser and the number of SNPs used (.grm.N.bin holds n_valid).
- * alg != 1 -> CS_ERR_INVALID_ARGUMENT; a kept SNP whose float32 mean is below 2^-6 and not a
- * multiple of 2^-29 (only possible for min_maf < 2^-7) -> CS_ERR_INVALID_ARGUMENT. */
+ * A kept SNP whose float32 mean is below 2^-6 and not a multiple of 2^-29 (only possible for
+ * min_maf < 2^-7) -> CS_ERR_INVALID_ARGUMENT.
+ * alg 0 or 3 (Yang / GCTA default, grm_builder.cpp:241-243, 386-387): z = (x - fl32(2p)) *
+ * fl32(1/sqrt(2p(1-p))), G = Z Z^T / n_valid, computed as FP16 hi/lo split products on MFMA with
+ * FP32 accumulation flushed to FP64 every 16384 SNPs (entries within ~1e-6 of the FP64 value
+ * relative to the diagonal); grm_raw_f64 is then that FP64 Z Z^T and grm_tri_f32 fl32(raw / n_valid).
+ * Other algorithms -> CS_ERR_INVALID_ARGUMENT. */
 int cs_grm_build(cs_bed* bed, int alg, float min_maf, float* grm_tri_f32, double* grm_raw_f64,
                  double* sum_2pq, int64_t* n_valid);
 /* Same with the lower triangle written to device memory (n(n+1)/2 floats). */
