@@ -133,6 +133,19 @@ def _load():
         "cs_server_release": ([_vp, _i64], ctypes.c_int),
         "cs_gwas_dense_run": ([ctypes.c_char_p, _dp, _dp, _i64, _i64, ctypes.c_void_p,
                                ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
+        "cs_mvlmm_create": ([_dp, _i64, _i64, _i64, _f32p, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "cs_mvlmm_destroy": ([_vp], ctypes.c_int),
+        "cs_mvlmm_prepare_dense": ([_vp, _vp, _i64, ctypes.c_int], ctypes.c_int),
+        "cs_mvlmm_null": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                           ctypes.c_double, _i32p, _i32p, _i32p], ctypes.c_int),
+        "cs_mvlmm_get": ([_vp, _dp, _dp, _dp, _dp, _dp], ctypes.c_int),
+        "cs_mvlmm_set_variance": ([_vp, _dp, _dp], ctypes.c_int),
+        "cs_mvlmm_get_eigen": ([_vp, _dp, _dp, _dp, _i64], ctypes.c_int),
+        "cs_mvlmm_scan": ([_vp, _vp, _i32p, _i64, _i32p, _i64, _dp, _dp, _dp, _dp, _dp, _i32p], ctypes.c_int),
+        "cs_gwas_mv_run": ([ctypes.c_char_p, _dp, _i64, _i64, _dp, _i64, _i64, ctypes.c_void_p,
+                            ctypes.c_char_p, ctypes.c_void_p], ctypes.c_int),
+        "cs_grm_loco_subset_scaled": ([_vp, _vp, _i64, _i32p, _i64, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, _vp], ctypes.c_int),
     }
     for name, (argt, rest) in sig.items():
         f = getattr(L, name)
@@ -972,3 +985,140 @@ class ScanServer:
     def release(self, rid):
         _check(lib().cs_server_release(self._h, rid))
         self._meta.pop(rid, None)
+
+
+# ----------------------------------------------------------------------- multivariate LMM
+class MvLmm:
+    """MvEigenVce (libCosmicCore/src/eigen_vce.cpp:496-910) + MvLmmEngine::scan
+    (src/mvlmm_engine.cpp:210-394, 756-931) on the device, with covariate projection.
+
+    Y: (n, d) kept-sample phenotypes (1 <= d <= 6); X: (n, c) design including the intercept
+    (float32, as the reference's covariate matrix)."""
+
+    def __init__(self, Y, X):
+        Y = np.asfortranarray(np.asarray(Y, dtype=np.float64))
+        if Y.ndim == 1:
+            Y = Y.reshape(-1, 1, order="F")
+        self.n, self.d = Y.shape
+        X = np.asfortranarray(np.asarray(X, dtype=np.float32).reshape(self.n, -1))
+        self.c = X.shape[1]
+        h = _vp()
+        _check(lib().cs_mvlmm_create(_p(Y, _dp), self.n, self.n, self.d, _p(X, _f32p), self.n, self.c,
+                                     ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib().cs_mvlmm_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self, G):
+        """MvEigenVce::prepare with an n x n GRM (numpy array or DeviceArray)."""
+        if isinstance(G, DeviceArray):
+            _check(lib().cs_mvlmm_prepare_dense(self._h, G.ptr, G.n, 1))
+        else:
+            Gf = np.asfortranarray(np.asarray(G, dtype=np.float64))
+            if Gf.shape != (self.n, self.n):
+                raise RuntimeError("MvEigenVce::prepare: Dimension mismatch")
+            _check(lib().cs_mvlmm_prepare_dense(self._h, Gf.ctypes.data_as(_vp), self.n, 0))
+
+    def runNullModel(self, solver="em", max_iter=1000, tol=1e-8, em_min_iter=10, ai_max_iter=50,
+                     ai_tol=1e-8):
+        """EM (default) or "aireml"; returns (converged, em_iterations, ai_iterations)."""
+        conv, em_it, ai_it = (np.zeros(1, dtype=np.int32) for _ in range(3))
+        _check(lib().cs_mvlmm_null(self._h, 1 if solver == "aireml" else 0, int(max_iter), float(tol),
+                                   int(em_min_iter), int(ai_max_iter), float(ai_tol), _p(conv, _i32p),
+                                   _p(em_it, _i32p), _p(ai_it, _i32p)))
+        return bool(conv[0]), int(em_it[0]), int(ai_it[0])
+
+    def variance(self):
+        """(Vg, Ve, Vg_se, Ve_se, logl); the standard errors are NaN when not computed."""
+        d = self.d
+        out = [np.empty((d, d)) for _ in range(4)]
+        ll = np.empty(1)
+        _check(lib().cs_mvlmm_get(self._h, *[_p(a, _dp) for a in out], _p(ll, _dp)))
+        return (*out, float(ll[0]))
+
+    def set_variance(self, Vg, Ve):
+        Vg = np.ascontiguousarray(Vg, dtype=np.float64)
+        Ve = np.ascontiguousarray(Ve, dtype=np.float64)
+        _check(lib().cs_mvlmm_set_variance(self._h, _p(Vg, _dp), _p(Ve, _dp)))
+
+    def eigen(self, with_w=False):
+        ne = self.n - self.c
+        D = np.empty(ne)
+        Uty = np.empty((ne, self.d), order="F")
+        W = np.empty((self.n, ne), order="F") if with_w else None
+        _check(lib().cs_mvlmm_get_eigen(self._h, _p(D, _dp), _p(Uty, _dp), _p(W, _dp), self.n))
+        return (D, Uty, W) if with_w else (D, Uty)
+
+    def scan(self, reader, keep=None, snps=None):
+        keep = _opt_i32(keep)
+        snps = _opt_i32(snps)
+        ns = reader.num_snps if snps is None else snps.size
+        d = self.d
+        freq, wald, p = np.empty(ns), np.empty(ns), np.empty(ns)
+        beta = np.empty((ns, d))
+        vbeta = np.empty((ns, d, d))
+        status = np.empty(ns, dtype=np.int32)
+        _check(lib().cs_mvlmm_scan(self._h, reader._h, _p(keep, _i32p), 0 if keep is None else keep.size,
+                                   _p(snps, _i32p), 0 if snps is None else snps.size, _p(freq, _dp),
+                                   _p(beta, _dp), _p(vbeta, _dp), _p(wald, _dp), _p(p, _dp),
+                                   _p(status, _i32p)))
+        return dict(freq=freq, beta=beta, vbeta=vbeta, wald=wald, p=p, status=status)
+
+
+def grm_loco_subset_scaled(d_raw, d_raw_chr, N, keep, norm, norm_chr, S, S_chr, d_out):
+    """LOCO GRM of the multivariate driver (multivariate_model.cpp:1040-1063) into d_out."""
+    keep = _opt_i32(keep)
+    _check(lib().cs_grm_loco_subset_scaled(d_raw.ptr, None if d_raw_chr is None else d_raw_chr.ptr, N,
+                                           _p(keep, _i32p), 0 if keep is None else keep.size,
+                                           float(norm), float(norm_chr), float(S), float(S_chr), d_out.ptr))
+
+
+class GwasMvOptions(ctypes.Structure):
+    _fields_ = [("min_maf", ctypes.c_double), ("no_loco", ctypes.c_int), ("scan_chr", ctypes.c_char_p),
+                ("grm_alg", ctypes.c_int), ("solver", ctypes.c_int), ("max_iter", ctypes.c_int),
+                ("tol", ctypes.c_double), ("em_min_iter", ctypes.c_int), ("ai_max_iter", ctypes.c_int),
+                ("ai_tol", ctypes.c_double)]
+
+
+class GwasMvSummary(ctypes.Structure):
+    _fields_ = [("n_samples", ctypes.c_int64), ("n_traits", ctypes.c_int64), ("n_tested", ctypes.c_int64),
+                ("loco", ctypes.c_int32), ("n_models", ctypes.c_int32),
+                ("model_label", (ctypes.c_char * 16) * 64),
+                ("vg", (ctypes.c_double * 36) * 64), ("ve", (ctypes.c_double * 36) * 64),
+                ("ms_grm", ctypes.c_double), ("ms_null", ctypes.c_double), ("ms_scan", ctypes.c_double)]
+
+
+def gwas_mv_run(bfile, Y, out_prefix, cov=None, min_maf=0.01, no_loco=False, scan_chr=None, grm_alg=1,
+                solver="em", max_iter=1000, tol=1e-8, em_min_iter=10, ai_max_iter=50, ai_tol=1e-8):
+    """MultivariateModel::run's fixed-null Wald GWAS end to end on the GPU (Y: N x d over the .fam
+    rows, NaN = missing); writes out_prefix.assoc.mv.txt and returns the run summary."""
+    Y = np.asfortranarray(np.asarray(Y, dtype=np.float64))
+    if Y.ndim == 1:
+        Y = Y.reshape(-1, 1, order="F")
+    N, d = Y.shape
+    if cov is None:
+        covf, nc = None, 0
+    else:
+        covf = np.asfortranarray(np.asarray(cov, dtype=np.float64).reshape(N, -1))
+        nc = covf.shape[1]
+    o = GwasMvOptions(float(min_maf), int(bool(no_loco)), None if scan_chr is None else str(scan_chr).encode(),
+                      int(grm_alg), 1 if solver == "aireml" else 0, int(max_iter), float(tol), int(em_min_iter),
+                      int(ai_max_iter), float(ai_tol))
+    sm = GwasMvSummary()
+    _check(lib().cs_gwas_mv_run(str(bfile).encode(), _p(Y, _dp), N, d, _p(covf, _dp), N, nc, ctypes.byref(o),
+                                str(out_prefix).encode(), ctypes.byref(sm)))
+    k = sm.n_models
+    return dict(n_samples=sm.n_samples, n_traits=sm.n_traits, n_tested=sm.n_tested, loco=bool(sm.loco),
+                models=[(sm.model_label[i].value.decode(),
+                         np.array(sm.vg[i][:d * d]).reshape(d, d), np.array(sm.ve[i][:d * d]).reshape(d, d))
+                        for i in range(k)],
+                ms_grm=sm.ms_grm, ms_null=sm.ms_null, ms_scan=sm.ms_scan)

@@ -113,12 +113,16 @@ struct Inputs {
     int64_t N = 0, M = 0, n = 0, c = 0;
     BedHandle bed;
     std::vector<int32_t> keep;
-    std::vector<double> y;
+    std::vector<double> y;         // n x d, column-major
     std::vector<float> X;          // n x c, column-major, intercept first
     std::set<std::string> labels;
 };
 
-void load_inputs(Inputs& in, const char* bfile, const double* y, const double* cov, int64_t ldc, int64_t n_cov) {
+// y: N x d phenotypes (column-major, ldy); a sample is kept when every trait and covariate is
+// finite (the multivariate filter of MultivariateModel::loadCovariates,
+// src/models/multivariate_model.cpp:830-867, is the same rule over d traits).
+void load_inputs(Inputs& in, const char* bfile, const double* y, const double* cov, int64_t ldc, int64_t n_cov,
+                 int64_t d = 1, int64_t ldy = 0) {
     const std::string prefix(bfile);
     in.bim = read_bim(prefix + ".bim");
     std::ifstream fb(prefix + ".bed", std::ios::binary);
@@ -134,8 +138,12 @@ void load_inputs(Inputs& in, const char* bfile, const double* y, const double* c
     in.M = (int64_t)in.bim.size();
     chk(cs_bed_create_from_file_image(image.data(), (int64_t)image.size(), in.N, in.M, &in.bed.h));
     if (n_cov > 0 && ldc < in.N) throw RuntimeError("DataFilter: X and y dimension mismatch");
+    if (d > 1 && ldy < in.N) throw RuntimeError("multivariate: Y dimension mismatch");
     for (int64_t i = 0; i < in.N; ++i) {
-        bool k = !std::isnan(y[i]);
+        bool k = true;
+        for (int64_t t = 0; t < d; ++t)
+            if (!std::isfinite(y[t * ldy + i])) k = false;
+        if (d == 1) k = !std::isnan(y[i]);
         for (int64_t j = 0; k && j < n_cov; ++j)
             if (std::isnan((float)cov[j * ldc + i])) k = false;
         if (k) in.keep.push_back((int32_t)i);
@@ -143,10 +151,10 @@ void load_inputs(Inputs& in, const char* bfile, const double* y, const double* c
     in.n = (int64_t)in.keep.size();
     if (in.n < 10) throw RuntimeError("Too few valid samples.");
     in.c = n_cov + 1;
-    in.y.resize((size_t)in.n);
+    in.y.resize((size_t)(in.n * d));
     in.X.resize((size_t)(in.n * in.c));
     for (int64_t r = 0; r < in.n; ++r) {
-        in.y[(size_t)r] = y[in.keep[(size_t)r]];
+        for (int64_t t = 0; t < d; ++t) in.y[(size_t)(t * in.n + r)] = y[t * ldy + in.keep[(size_t)r]];
         in.X[(size_t)r] = 1.0f;
         for (int64_t j = 0; j < n_cov; ++j)
             in.X[(size_t)((j + 1) * in.n + r)] = (float)cov[j * ldc + in.keep[(size_t)r]];
@@ -503,5 +511,162 @@ extern "C" int cs_gwas_matrix_free_run(const char* bfile, const double* y, const
         const int64_t n_tested = write_outputs(out, results, n, use_loco);
         fill_summary(summary, in, (int64_t)grm_vars.size(), (double)M, use_loco, n_tested, model_labels, vg_all,
                      ve_all, 0.0, ms_null, ms_scan);
+    });
+}
+
+// ---------------------------------------------------------------------------------------------
+// Multivariate driver: MultivariateModel::run's fixed-null Wald path
+// (src/models/multivariate_model.cpp:870-1110) over the library's C-ABI.
+namespace {
+
+using MvHandle = Handle<cs_mvlmm, cs_mvlmm_destroy>;
+
+// std::sort comparator of the LOCO chromosome order (multivariate_model.cpp:1022-1024).
+bool chrom_less(const std::string& a, const std::string& b) {
+    try {
+        return std::stoi(a) < std::stoi(b);
+    } catch (...) {
+        return a < b;
+    }
+}
+
+void write_mv_header(const std::string& path, int64_t d) {
+    std::ofstream f(path);
+    if (!f) throw RuntimeError("Cannot create " + path);
+    f << "CHR\tSNP\tBP\tA1\tA2\tFREQ\tWALD_STAT\tP";
+    for (int64_t t = 0; t < d; ++t) f << "\tBETA_" << (t + 1) << "\tSE_" << (t + 1);
+    f << "\n";
+}
+
+// MultivariateModel::writeResults (multivariate_output.cpp:42-70): rows with a finite P.
+int64_t append_mv_rows(const std::string& path, const Inputs& in, const std::vector<int32_t>& snps, int64_t d,
+                       const std::vector<double>& freq, const std::vector<double>& beta,
+                       const std::vector<double>& vbeta, const std::vector<double>& wald,
+                       const std::vector<double>& p) {
+    std::ofstream f(path, std::ios::app);
+    int64_t rows = 0;
+    for (size_t i = 0; i < snps.size(); ++i) {
+        if (std::isnan(p[i])) continue;
+        const BimRow& b = in.bim[(size_t)snps[i]];
+        f << b.chrom << '\t' << b.id << '\t' << b.bp << '\t' << b.alt << '\t' << b.ref << '\t' << freq[i] << '\t'
+          << wald[i] << '\t' << p[i];
+        for (int64_t t = 0; t < d; ++t) {
+            const double var = vbeta[(i * d + t) * d + t];
+            if (!std::isfinite(var) || var < 0.0)
+                throw RuntimeError("mvLMM covariance TRUE_SINGULARITY for SNP " + b.id + ": invalid trait variance");
+            f << '\t' << beta[i * d + t] << '\t' << std::sqrt(var);
+        }
+        f << '\n';
+        ++rows;
+    }
+    return rows;
+}
+
+}  // namespace
+
+extern "C" int cs_gwas_mv_run(const char* bfile, const double* Y, int64_t ldy, int64_t n_traits, const double* cov,
+                              int64_t ldc, int64_t n_cov, const cs_gwas_mv_options* opt, const char* out_prefix,
+                              cs_gwas_mv_summary* summary) {
+    return csamd::guarded([&] {
+        if (!bfile || !Y || !out_prefix || (n_cov > 0 && !cov))
+            throw InvalidArgument("cs_gwas_mv_run: null argument");
+        if (n_traits < 1 || n_traits > 6) throw InvalidArgument("multivariate LMM: 1 to 6 traits are supported");
+        cs_gwas_mv_options o{0.01, 0, nullptr, 1, 0, 1000, 1e-8, 10, 50, 1e-8};
+        if (opt) o = *opt;
+        const int64_t d = n_traits;
+        Inputs in;
+        load_inputs(in, bfile, Y, cov, ldc, n_cov, d, ldy);
+        const int64_t N = in.N, M = in.M, n = in.n;
+        const std::string scan_chr = o.scan_chr ? std::string(o.scan_chr) : std::string();
+        const std::string path = std::string(out_prefix) + ".assoc.mv.txt";
+        const bool std_alg = o.grm_alg == 0 || o.grm_alg == 3;
+        auto t0 = std::chrono::steady_clock::now();
+        double ms_grm = 0, ms_null = 0, ms_scan = 0;
+        DevMem raw(N * N * 8), gsub(n * n * 8);
+        double S_global = 0.0;
+        int64_t nv = 0;
+        chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, nullptr, 0, raw.d(), 1, &S_global, nullptr, &nv));
+        const double norm_global = std_alg ? (double)nv : S_global;
+        ms_grm += ms_since(t0);
+        write_mv_header(path, d);
+        std::vector<std::string> labels;
+        std::vector<std::vector<double>> vgs, ves;
+        int64_t n_tested = 0;
+        auto fit_and_scan = [&](const std::vector<int32_t>& snps, const std::string& label, bool loco_model) {
+            auto t1 = std::chrono::steady_clock::now();
+            MvHandle mv;
+            chk(cs_mvlmm_create(in.y.data(), n, n, d, in.X.data(), n, in.c, &mv.h));
+            chk(cs_mvlmm_prepare_dense(mv.h, gsub.d(), n, 1));
+            int conv = 0;
+            chk(cs_mvlmm_null(mv.h, o.solver, o.max_iter, o.tol, o.em_min_iter, o.ai_max_iter, o.ai_tol, &conv,
+                              nullptr, nullptr));
+            if (!conv) {
+                if (loco_model)
+                    throw RuntimeError("Multivariate LOCO null model did not converge for chromosome " + label +
+                                       "; stopping association scan. Check the model/data or increase --reml-maxit.");
+                throw RuntimeError("Multivariate null model did not converge; no association scan was performed. "
+                                   "Check the model/data or increase --reml-maxit.");
+            }
+            std::vector<double> vg((size_t)(d * d)), ve((size_t)(d * d));
+            chk(cs_mvlmm_get(mv.h, vg.data(), ve.data(), nullptr, nullptr, nullptr));
+            ms_null += ms_since(t1);
+            labels.push_back(label);
+            vgs.push_back(vg);
+            ves.push_back(ve);
+            t1 = std::chrono::steady_clock::now();
+            const size_t ns = snps.size();
+            std::vector<double> freq(ns), beta(ns * d), vbeta(ns * d * d), wald(ns), p(ns);
+            std::vector<int32_t> status(ns);
+            if (ns > 0)
+                chk(cs_mvlmm_scan(mv.h, in.bed.h, in.keep.data(), n, snps.data(), (int64_t)ns, freq.data(),
+                                  beta.data(), vbeta.data(), wald.data(), p.data(), status.data()));
+            ms_scan += ms_since(t1);
+            n_tested += append_mv_rows(path, in, snps, d, freq, beta, vbeta, wald, p);
+        };
+        const bool use_loco = !(o.no_loco || in.labels.size() <= 1);
+        if (!use_loco) {
+            chk(cs_grm_loco_subset_scaled(raw.d(), nullptr, N, in.keep.data(), n, norm_global, 1.0, S_global, 0.0,
+                                          gsub.d()));
+            std::vector<int32_t> snps;
+            for (int64_t j = 0; j < M; ++j)
+                if (scan_chr.empty() || in.bim[(size_t)j].chrom == scan_chr) snps.push_back((int32_t)j);
+            fit_and_scan(snps, "GLOBAL", false);
+        } else {
+            std::vector<std::string> order(in.labels.begin(), in.labels.end());
+            std::sort(order.begin(), order.end(), chrom_less);
+            DevMem raw_chr(N * N * 8);
+            for (const std::string& chrom : order) {
+                if (!scan_chr.empty() && chrom != scan_chr) continue;
+                const std::vector<int32_t> snps = chrom_snps(in, chrom);
+                auto t1 = std::chrono::steady_clock::now();
+                double S_chr = 0.0;
+                int64_t nv_chr = 0;
+                chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, snps.data(), (int64_t)snps.size(),
+                                     raw_chr.d(), 1, &S_chr, nullptr, &nv_chr));
+                const double norm_chr = std_alg ? (double)nv_chr : S_chr;
+                chk(cs_grm_loco_subset_scaled(raw.d(), raw_chr.d(), N, in.keep.data(), n, norm_global, norm_chr,
+                                              S_global, S_chr, gsub.d()));
+                ms_grm += ms_since(t1);
+                fit_and_scan(snps, chrom, true);
+            }
+        }
+        if (summary) {
+            std::memset(summary, 0, sizeof(*summary));
+            summary->n_samples = n;
+            summary->n_traits = d;
+            summary->n_tested = n_tested;
+            summary->loco = use_loco ? 1 : 0;
+            summary->n_models = (int32_t)labels.size();
+            for (size_t i = 0; i < labels.size() && i < CS_GWAS_MAX_MODELS; ++i) {
+                std::strncpy(summary->model_label[i], labels[i].c_str(), 15);
+                for (int64_t k = 0; k < d * d; ++k) {
+                    summary->vg[i][k] = vgs[i][(size_t)k];
+                    summary->ve[i][k] = ves[i][(size_t)k];
+                }
+            }
+            summary->ms_grm = ms_grm;
+            summary->ms_null = ms_null;
+            summary->ms_scan = ms_scan;
+        }
     });
 }

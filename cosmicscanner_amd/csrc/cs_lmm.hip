@@ -134,6 +134,20 @@ __global__ void k_loco(const double* __restrict__ raw, const double* __restrict_
     }
 }
 
+// LOCO GRM of the multivariate driver (src/models/multivariate_model.cpp:1040-1063):
+// (raw / norm) w1 - (raw_chr / norm_chr) w2, w1 = S / S_loco, w2 = S_chr / S_loco.
+__global__ void k_loco_scaled(const double* __restrict__ raw, const double* __restrict__ raw_chr, int64_t N,
+                              const int32_t* __restrict__ keep, int64_t n, double norm, double norm_chr,
+                              double w1, double w2, bool loco, double* __restrict__ out) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e % n, j = e / n;
+        const int64_t src = (int64_t)keep[j] * N + keep[i];
+        const double g = raw[src] / norm;
+        out[e] = loco ? g * w1 - (raw_chr[src] / norm_chr) * w2 : g;
+    }
+}
+
 }  // namespace csamd
 
 using namespace csamd;
@@ -369,6 +383,33 @@ int cs_grm_loco_subset(const double* d_raw, const double* d_raw_chr, int64_t N, 
         CS_HIP(hipMemcpy(dk.p, keep, (size_t)n_keep * 4, hipMemcpyHostToDevice));
         k_loco<<<grid_for(n_keep * n_keep), 256>>>(d_raw, d_raw_chr, N, dk.as<int32_t>(), n_keep, S_global,
                                                    S_loco, loco, d_out);
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipDeviceSynchronize());
+    });
+}
+
+int cs_grm_loco_subset_scaled(const double* d_raw, const double* d_raw_chr, int64_t N, const int32_t* keep,
+                              int64_t n_keep, double norm, double norm_chr, double S, double S_chr,
+                              double* d_out) {
+    return guarded([&] {
+        if (!d_raw || !d_out) throw InvalidArgument("cs_grm_loco_subset_scaled: null argument");
+        std::vector<int32_t> all;
+        if (!keep || n_keep <= 0) {
+            all.resize((size_t)N);
+            for (int64_t i = 0; i < N; ++i) all[(size_t)i] = (int32_t)i;
+            keep = all.data();
+            n_keep = N;
+        }
+        for (int64_t i = 0; i < n_keep; ++i)
+            if (keep[i] < 0 || keep[i] >= N) throw RuntimeError("LOCO GRM: sample index out of range");
+        const double S_loco = S - S_chr;
+        const bool loco = d_raw_chr && S_chr > 0.0 && S_loco > 1e-6;
+        DevBuf dk((size_t)n_keep * 4);
+        CS_HIP(hipMemcpy(dk.p, keep, (size_t)n_keep * 4, hipMemcpyHostToDevice));
+        k_loco_scaled<<<grid_for(n_keep * n_keep), 256>>>(d_raw, d_raw_chr, N, dk.as<int32_t>(), n_keep,
+                                                          norm > 0 ? norm : 1.0, norm_chr > 0 ? norm_chr : 1.0,
+                                                          loco ? S / S_loco : 1.0, loco ? S_chr / S_loco : 0.0,
+                                                          loco, d_out);
         CS_HIP(hipGetLastError());
         CS_HIP(hipDeviceSynchronize());
     });

@@ -18,42 +18,9 @@
 #include "cs_common.h"
 #include "cs_geno_mfma.h"
 #include "cs_ops.h"
+#include "cs_stats.h"
 
 namespace csamd {
-
-// gamma_q / chisq_prob (lmm_engine.cpp:22-64), restated for the device in FP64.
-__device__ double d_gamma_q(double a, double x) {
-    if (x < 0.0 || a <= 0.0) return 1.0;
-    if (x < a + 1.0) {
-        double ap = a, sum = 1.0 / a, del = sum;
-        for (int i = 0; i < 100; ++i) {
-            ap += 1.0;
-            del *= x / ap;
-            sum += del;
-            if (fabs(del) < fabs(sum) * 1e-7) return 1.0 - sum * exp(-x + a * log(x) - lgamma(a));
-        }
-    } else {
-        double b = x + 1.0 - a, c = 1.0 / 1e-30, d = 1.0 / b, h = d;
-        for (int i = 1; i <= 100; ++i) {
-            const double an = -i * (i - a);
-            b += 2.0;
-            d = an * d + b;
-            if (fabs(d) < 1e-30) d = 1e-30;
-            c = b + an / c;
-            if (fabs(c) < 1e-30) c = 1e-30;
-            d = 1.0 / d;
-            const double del = d * c;
-            h *= del;
-            if (fabs(del - 1.0) < 1e-7) return exp(-x + a * log(x) - lgamma(a)) * h;
-        }
-    }
-    return 0.0;
-}
-
-__device__ double d_chisq_prob(double stat, int df) {
-    if (stat <= 0) return 1.0;
-    return d_gamma_q(df / 2.0, stat / 2.0);
-}
 
 struct ModelDev {
     int64_t n_score, quad_begin, quad_end, cov_begin, cov_end;

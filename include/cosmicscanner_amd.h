@@ -35,6 +35,7 @@ extern "C" {
 typedef struct cs_bed cs_bed;
 typedef struct cs_kinship cs_kinship;
 typedef struct cs_lmm cs_lmm;
+typedef struct cs_mvlmm cs_mvlmm;
 typedef struct cs_lowrank cs_lowrank;
 typedef struct cs_server cs_server;
 
@@ -402,6 +403,84 @@ typedef struct cs_gwas_mf_options {
 int cs_gwas_matrix_free_run(const char* bfile, const double* y, const double* cov, int64_t ldc,
                             int64_t n_cov, const cs_gwas_mf_options* opt, const char* out_prefix,
                             cs_gwas_summary* summary);
+
+/* ---- Multivariate LMM (d traits, 1 <= d <= 6): MvEigenVce + MvLmmEngine with covariate
+ * projection (the defaults of MultivariateModel, src/models/multivariate_model.cpp:870-1110).
+ *
+ * cs_mvlmm_create: Y (n x d, column-major, ldy) and covariates X (n x c float32, intercept
+ * included) over the kept samples; the full Householder Q of X is formed once. */
+int cs_mvlmm_create(const double* Y, int64_t ldy, int64_t n, int64_t d, const float* X, int64_t ldx,
+                    int64_t c, cs_mvlmm** out);
+int cs_mvlmm_destroy(cs_mvlmm* mv);
+/* MvEigenVce::prepare (libCosmicCore/src/eigen_vce.cpp:496-549) for an n x n GRM over the kept
+ * samples (host memory, or device memory when g_on_device != 0): A^T G A, eigendecomposition
+ * (clamped at 0), W_proj = A U, Uty = U^T A^T Y, start values Vg = Ve = Cov(Y) / 2 (Cov with the
+ * divisor n - c - 1 of the projected sample count, as the reference). */
+int cs_mvlmm_prepare_dense(cs_mvlmm* mv, const double* G, int64_t ldg, int g_on_device);
+/* MvEigenVce::runNullModel (eigen_vce.cpp:1031-1066): solver 0 = EM (updateEM until the
+ * normalised squared change is below tol, at most max_iter iterations; converged = 0 otherwise,
+ * which the reference turns into an error), solver 1 = runAireml (:631-910: EM warm-up of at
+ * least em_min_iter iterations, eigenvalue clipping, AI-REML with step halving, ai_max_iter,
+ * ai_tol; standard errors from the final AI matrix unless it stopped on ai_tol).  Reference
+ * defaults: max_iter 1000 (--reml-maxit), tol 1e-8, em_min_iter 10, ai_max_iter 50, ai_tol 1e-8. */
+int cs_mvlmm_null(cs_mvlmm* mv, int solver, int max_iter, double tol, int em_min_iter, int ai_max_iter,
+                  double ai_tol, int* converged, int* em_iterations, int* ai_iterations);
+/* Vg, Ve, their standard errors (NaN when not computed) as d x d row-major, and the REML
+ * log-likelihood (calcLogLikelihood, eigen_vce.cpp:617-632); any may be NULL. */
+int cs_mvlmm_get(cs_mvlmm* mv, double* Vg, double* Ve, double* Vg_se, double* Ve_se, double* logl);
+int cs_mvlmm_set_variance(cs_mvlmm* mv, const double* Vg, const double* Ve);
+/* D (n-c), Uty ((n-c) x d column-major), W_proj (n x (n-c), leading dimension ldw); any may be NULL. */
+int cs_mvlmm_get_eigen(cs_mvlmm* mv, double* D, double* Uty, double* W_proj, int64_t ldw);
+/* MvLmmEngine::scan(PlinkReader&) (src/mvlmm_engine.cpp:756-931, processBlock :210-394) with the
+ * fitted (or set) null model, over the keep list (the rows of Y / X / G, in that order): per SNP
+ * the centred marker (mean of the observed genotypes of the kept samples, missing -> 0; no MAF
+ * filter), s = W_proj^T g, H = sum_c s_c^2 Sigma_c^-1, r = sum_c s_c Sigma_c^-1 y_c, beta = H^-1 r
+ * (symmetric eigendecomposition, singular below 128 eps d max(1, |lambda|max)), Wald = beta^T H
+ * beta, p = chi^2_d tail, vbeta = H^-1 conditioned as conditionCovariance
+ * (src/mvlmm_numerics.h:27-66).  Outputs per requested SNP: freq (mean / 2), beta (n_snps x d),
+ * vbeta (n_snps x d x d), wald, p; status 0 / 1 / 2 (OK / NUMERICAL_ROUNDOFF /
+ * BOUNDARY_SOLUTION) when tested, -1 (NaN outputs) when not.  The reference reads the first n
+ * rows of the .fam instead of the kept rows (mvlmm_engine.cpp:760-766, 858-866); both agree when
+ * every sample is kept. */
+int cs_mvlmm_scan(cs_mvlmm* mv, cs_bed* bed, const int32_t* keep, int64_t n_keep, const int32_t* snps,
+                  int64_t n_snps, double* freq, double* beta, double* vbeta, double* wald, double* p,
+                  int32_t* status);
+
+/* ---- Multivariate GWAS driver: MultivariateModel::run's fixed-null Wald path
+ * (src/models/multivariate_model.cpp:870-1110) end to end on the GPU.  Y is N x d (one row per
+ * .fam sample, column-major, ldy; NaN = missing), cov as cs_gwas_dense_run.  GRM with grm_alg;
+ * LOCO over the chromosomes in numeric order (non-numeric labels compared as strings) unless
+ * no_loco or a single chromosome, with G_loco = G_norm (S / S_loco) - G_chr,norm (S_chr / S_loco)
+ * and S, S_chr the sums of 2pq (:1040-1063).  Writes out_prefix.assoc.mv.txt (CHR SNP BP A1 A2
+ * FREQ WALD_STAT P BETA_t SE_t ...; multivariate_output.cpp:42-70). */
+typedef struct cs_gwas_mv_options {
+    double min_maf;        /* GRM SNP filter, 0.01 */
+    int no_loco;
+    const char* scan_chr;  /* NULL = all */
+    int grm_alg;           /* 1 */
+    int solver;            /* 0 EM (default), 1 AI-REML (--mv-vce-solver aireml) */
+    int max_iter;          /* --reml-maxit, 1000 */
+    double tol;            /* --reml-tol, 1e-8 */
+    int em_min_iter;       /* 10 */
+    int ai_max_iter;       /* 50 */
+    double ai_tol;         /* 1e-8 */
+} cs_gwas_mv_options;
+typedef struct cs_gwas_mv_summary {
+    int64_t n_samples, n_traits, n_tested;
+    int32_t loco, n_models;
+    char model_label[CS_GWAS_MAX_MODELS][16];
+    double vg[CS_GWAS_MAX_MODELS][36], ve[CS_GWAS_MAX_MODELS][36];   /* d x d row-major */
+    double ms_grm, ms_null, ms_scan;
+} cs_gwas_mv_summary;
+int cs_gwas_mv_run(const char* bfile, const double* Y, int64_t ldy, int64_t n_traits, const double* cov,
+                   int64_t ldc, int64_t n_cov, const cs_gwas_mv_options* opt, const char* out_prefix,
+                   cs_gwas_mv_summary* summary);
+/* LOCO GRM of the multivariate driver (multivariate_model.cpp:1040-1063) over the keep list from
+ * device raw GRMs: G = raw / norm restricted to keep x keep; with d_raw_chr non-NULL, S_chr > 0 and
+ * S_loco = S - S_chr > 1e-6: (raw / norm) (S / S_loco) - (raw_chr / norm_chr) (S_chr / S_loco). */
+int cs_grm_loco_subset_scaled(const double* d_raw, const double* d_raw_chr, int64_t N, const int32_t* keep,
+                              int64_t n_keep, double norm, double norm_chr, double S, double S_chr,
+                              double* d_out);
 
 #ifdef __cplusplus
 }

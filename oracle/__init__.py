@@ -33,6 +33,12 @@ eigen_vce_prepare/solve     textbook REML: brute-force grid over delta on the ex
 rotated_scan / gls_scan     explicit P = V^-1 - V^-1 X (X^T V^-1 X)^-1 X^T V^-1 on n <= 500
                             (pattern of tests/test_weighted_lmm_equivalence.py:13-22)
 gen_hwe_bed                 allele-frequency and missing-rate moments; GPU byte equality
+mv_prepare / mv_null        d = 1: AI-REML reaches solve_diagonal's REML optimum; the AI score is
+  / mv_ai / mv_logl         the central-difference gradient of mv_logl; mv_logl equals the
+                            explicit Kronecker-V likelihood; EM never decreases it
+mv_scan                     d = 1: equals rotated_scan (chi^2, beta, SE); d = 2: brute-force GLS
+                            with V = Vg (x) K + Ve (x) I in the projected space
+gwas_mv                     composition of the above with grm / grm_exact_centred / select_snps
 ==========================  =====================================================================
 """
 from __future__ import annotations
@@ -720,4 +726,277 @@ def gwas_matrix_free(bed, N, M, chroms, y, cov=None, min_maf=0.01, no_loco=False
         for j in range(M):
             if (c is None or chroms[j] == c) and not np.isnan(st["p"][j]):
                 rows.append((j, freq[j], st["beta"][j], st["se"][j], st["p"][j], st["log10p"][j]))
+    return models, rows
+
+
+# ------------------------------------------------------------------- multivariate LMM (C5)
+def mv_prepare(G, Y, X):
+    """MvEigenVce::prepare with covariate projection (eigen_vce.cpp:496-549): A = Q[:, c:] of the
+    full QR of X, A^T G A eigendecomposition (D clamped at 0), W_proj = A U, Uty = U^T A^T Y and
+    the start values Vg = Ve = Cov(Y) / 2, Cov divided by (n - c) - 1 (the projected count,
+    :541-545)."""
+    Y = np.asarray(Y, dtype=np.float64).reshape(len(Y), -1)
+    n, c = X.shape
+    Q, _ = np.linalg.qr(np.asarray(X, dtype=np.float64), mode="complete")
+    A = Q[:, c:]
+    D, U = np.linalg.eigh(A.T @ G @ A)
+    D = np.maximum(D, 0.0)
+    Yc = Y - Y.mean(axis=0)
+    cov = Yc.T @ Yc / ((n - c) - 1.0)
+    return D, A @ U, U.T @ (A.T @ Y), 0.5 * cov, 0.5 * cov.copy()
+
+
+def _mv_pairs(d):
+    return [(j, k) for j in range(d) for k in range(j, d)]
+
+
+def mv_em_update(D, Uty, Vg, Ve):
+    """MvEigenVce::updateEM without covariates (eigen_vce.cpp:551-608)."""
+    ne = D.size
+    V = D[:, None, None] * Vg + Ve
+    P = np.linalg.inv(V)
+    Wg = D[:, None, None] * np.einsum("ab,ibc->iac", Vg, P)
+    u = np.einsum("iab,ib->ia", Wg, Uty)
+    Sg_i = u[:, :, None] * u[:, None, :] / np.where(D > 1e-12, D, 1.0)[:, None, None] + Vg - Wg @ Vg
+    Sg_i = np.where((D > 1e-12)[:, None, None], Sg_i, Vg)
+    We = np.einsum("ab,ibc->iac", Ve, P)
+    e = np.einsum("iab,ib->ia", We, Uty)
+    Se_i = e[:, :, None] * e[:, None, :] + Ve - We @ Ve
+    return Sg_i.sum(axis=0) / ne, Se_i.sum(axis=0) / ne
+
+
+def mv_logl(D, Uty, Vg, Ve):
+    """MvEigenVce::calcLogLikelihood (eigen_vce.cpp:617-632): -0.5 sum (log|V_i| + y_i^T V_i^-1 y_i),
+    V_i = D_i Vg + Ve + 1e-8 I."""
+    d = Vg.shape[0]
+    V = D[:, None, None] * Vg + Ve + 1e-8 * np.eye(d)
+    sign, ld = np.linalg.slogdet(V)
+    if np.any(sign <= 0):
+        return -np.inf
+    x = np.linalg.solve(V, Uty[:, :, None])[:, :, 0]
+    return float(-0.5 * np.sum(ld + np.sum(Uty * x, axis=1)))
+
+
+def mv_ai(D, Uty, Vg, Ve):
+    """Score and average-information matrix of runAireml (eigen_vce.cpp:688-740), symmetrised."""
+    d = Vg.shape[0]
+    pairs = _mv_pairs(d)
+    vs = len(pairs)
+    V = D[:, None, None] * Vg + Ve + 1e-8 * np.eye(d)
+    Vi = np.linalg.inv(V)
+    v = np.einsum("iab,ib->ia", Vi, Uty)
+    score = np.zeros(2 * vs)
+    AI = np.zeros((2 * vs, 2 * vs))
+
+    def wb(p, q, x):
+        return v[:, p] * Vi[:, x, p] if p == q else v[:, q] * Vi[:, x, p] + v[:, p] * Vi[:, x, q]
+
+    for a, (j, k) in enumerate(pairs):
+        vjvk = v[:, j] * v[:, k]
+        tr = Vi[:, j, j] if j == k else 2.0 * Vi[:, j, k]
+        quad = vjvk if j == k else 2.0 * vjvk
+        score[a] = np.sum(0.5 * D * (quad - tr))
+        score[a + vs] = np.sum(0.5 * (quad - tr))
+        for b in range(a, vs):
+            p, q = pairs[b]
+            qv = v[:, j] * wb(p, q, j) if j == k else v[:, j] * wb(p, q, k) + v[:, k] * wb(p, q, j)
+            qv = 0.5 * qv
+            AI[a, b] += np.sum(D * D * qv)
+            AI[a, vs + b] += np.sum(D * qv)
+            AI[vs + a, b] += np.sum(D * qv)
+            AI[vs + a, vs + b] += np.sum(qv)
+    return 0.5 * (AI + AI.T), score
+
+
+def _near_pd(M):
+    w, V = np.linalg.eigh(M)
+    if w.min() <= 1e-10:
+        return (V * np.maximum(w, 1e-10)) @ V.T
+    return M
+
+
+def mv_null(D, Uty, Vg, Ve, solver="em", max_iter=1000, tol=1e-8, em_min_iter=10, ai_max_iter=50,
+            ai_tol=1e-8):
+    """MvEigenVce::runNullModel: EM (:1040-1066) or runAireml (:631-910).  Returns
+    (Vg, Ve, converged, em_iterations, ai_iterations)."""
+    d = Vg.shape[0]
+    Vg, Ve = Vg.copy(), Ve.copy()
+    denom = max(1.0, 2.0 * d * d)
+    if solver != "aireml":
+        for it in range(max_iter):
+            Vg_n, Ve_n = mv_em_update(D, Uty, Vg, Ve)
+            err = (np.sum((Vg_n - Vg) ** 2) + np.sum((Ve_n - Ve) ** 2)) / denom
+            Vg, Ve = Vg_n, Ve_n
+            if err < tol:
+                return Vg, Ve, True, it + 1, 0
+        return Vg, Ve, False, max_iter, 0
+    pairs = _mv_pairs(d)
+    vs = len(pairs)
+    em_it = 0
+    for em_it in range(max_iter):
+        Vg_n, Ve_n = mv_em_update(D, Uty, Vg, Ve)
+        err = (np.sum((Vg_n - Vg) ** 2) + np.sum((Ve_n - Ve) ** 2)) / denom
+        Vg, Ve = Vg_n, Ve_n
+        if err < tol and em_it >= em_min_iter:
+            break
+    Vg, Ve = _near_pd(Vg), _near_pd(Ve)
+    logl_old = mv_logl(D, Uty, Vg, Ve)
+    logl0, Vg0, Ve0 = logl_old, Vg.copy(), Ve.copy()
+    ai_it = 0
+    if not np.isfinite(logl_old):
+        return Vg, Ve, True, em_it + 1, 0
+    for it in range(ai_max_iter):
+        AI, score = mv_ai(D, Uty, Vg, Ve)
+        AI[np.diag_indices_from(AI)] *= 1.0 + 1e-8
+        delta = np.linalg.solve(AI, score)
+        step, accepted = 1.0, False
+        for _ in range(12):
+            Vg_t, Ve_t = Vg.copy(), Ve.copy()
+            for a, (j, k) in enumerate(pairs):
+                Vg_t[j, k] += step * delta[a]
+                Vg_t[k, j] = Vg_t[j, k]
+                Ve_t[j, k] += step * delta[a + vs]
+                Ve_t[k, j] = Ve_t[j, k]
+            Vg_t, Ve_t = _near_pd(Vg_t), _near_pd(Ve_t)
+            ln = mv_logl(D, Uty, Vg_t, Ve_t)
+            if np.isfinite(ln) and ln >= logl_old - 1e-10:
+                accepted = True
+                break
+            step *= 0.5
+        if not accepted:
+            break
+        Vg, Ve = Vg_t, Ve_t
+        ai_it = it + 1
+        logl_old = mv_logl(D, Uty, Vg, Ve)
+        if np.linalg.norm(delta) / np.sqrt(2 * vs) < ai_tol:
+            return Vg, Ve, True, em_it + 1, ai_it
+    if logl_old < logl0 + 1e-8:
+        Vg, Ve = Vg0, Ve0
+    return Vg, Ve, True, em_it + 1, ai_it
+
+
+def mv_scan(bed, N, M, keep, W, D, Uty, Vg, Ve, snps=None):
+    """MvLmmEngine::scan (src/mvlmm_engine.cpp:756-931, processBlock :210-394) over the keep list:
+    centred marker (mean of the observed genotypes, missing -> 0, no MAF filter), s = W^T g,
+    H = sum s^2 Sigma^-1, r = sum s Sigma^-1 y, invertInformation / Wald / chi^2_d /
+    conditionCovariance (src/mvlmm_numerics.h:27-66).  Returns freq, beta (ns, d),
+    vbeta (ns, d, d), wald, p, status (-1 = not tested)."""
+    d = Vg.shape[0]
+    keep = np.arange(N) if keep is None else np.asarray(keep)
+    snps = np.arange(M) if snps is None else np.asarray(snps)
+    Dg = decode_numpy(bed, N, M)[keep][:, snps]
+    obs = ~np.isnan(Dg)
+    cnt = obs.sum(axis=0)
+    mean = np.where(cnt > 0, np.nansum(Dg, axis=0) / np.maximum(cnt, 1), 0.0)
+    Gm = np.where(obs, Dg - mean, 0.0)
+    S = W.T @ Gm                                        # ne x ns
+    Sinv = np.linalg.inv(D[:, None, None] * Vg + Ve)     # ne x d x d
+    b = np.einsum("iab,ib->ia", Sinv, Uty)               # ne x d
+    H = np.einsum("is,iab->sab", S * S, Sinv)
+    r = S.T @ b
+    ns = snps.size
+    out = dict(freq=mean / 2.0, beta=np.full((ns, d), np.nan), vbeta=np.full((ns, d, d), np.nan),
+               wald=np.full(ns, np.nan), p=np.full(ns, np.nan), status=np.full(ns, -1, dtype=np.int32))
+    eps = np.finfo(float).eps
+    for i in range(ns):
+        Hs = 0.5 * (H[i] + H[i].T)
+        if not np.all(np.isfinite(Hs)):
+            continue
+        w, V = np.linalg.eigh(Hs)
+        if not w.min() > 128 * eps * max(1.0, np.abs(w).max()) * d:
+            continue
+        C = (V / w) @ V.T
+        C = 0.5 * (C + C.T)
+        beta = C @ r[i]
+        wald = float(beta @ Hs @ beta)
+        if not np.isfinite(wald) or wald < 0:
+            continue
+        pv = float(chisq_prob(wald, d)[0])
+        cw, CV = np.linalg.eigh(C)
+        tol = 128 * eps * max(1.0, np.abs(cw).max()) * d
+        st = 0
+        if cw.min() < -tol:
+            continue
+        if cw.min() < 0:
+            C = (CV * np.maximum(cw, 0.0)) @ CV.T
+            C = 0.5 * (C + C.T)
+            st = 1
+        elif cw.min() <= tol:
+            st = 2
+        out["beta"][i], out["vbeta"][i], out["wald"][i], out["p"][i], out["status"][i] = beta, C, wald, pv, st
+    return out
+
+
+def _chrom_key(c):
+    try:
+        return (0, int(c), "")
+    except ValueError:
+        return (1, 0, c)
+
+
+def gwas_mv(bed, N, M, chroms, Y, cov=None, min_maf=0.01, no_loco=False, scan_chr=None, grm_alg=1,
+            solver="em", max_iter=1000, tol=1e-8, em_min_iter=10, ai_max_iter=50, ai_tol=1e-8):
+    """MultivariateModel::run's fixed-null Wald path (src/models/multivariate_model.cpp:870-1110):
+    keep = finite Y rows and covariates, X = [1, cov] float32, GRM (alg 1 exact / alg 0 FP64),
+    LOCO in numeric chromosome order with G_norm (S / S_loco) - G_chr,norm (S_chr / S_loco)
+    (S, S_chr sums of 2pq), null model, scan per model.  Returns (models [(label, Vg, Ve)],
+    rows [(snp, freq, wald, p, beta (d,), se (d,))])."""
+    Y = np.asarray(Y, dtype=np.float64).reshape(N, -1)
+    d = Y.shape[1]
+    covf = None if cov is None else np.asarray(cov, dtype=np.float64).reshape(N, -1)
+    ok = np.all(np.isfinite(Y), axis=1)
+    if covf is not None:
+        ok &= np.all(np.isfinite(covf.astype(np.float32).astype(np.float64)), axis=1)
+    keep = np.nonzero(ok)[0].astype(np.int32)
+    n = keep.size
+    if n < 10:
+        raise RuntimeError("Too few valid samples.")
+    X = np.ones((n, 1 + (0 if covf is None else covf.shape[1])), dtype=np.float32)
+    if covf is not None:
+        X[:, 1:] = covf[keep].astype(np.float32)
+    Xd = X.astype(np.float64)
+    Ys = Y[keep]
+    std_alg = grm_alg in (0, 3)
+
+    def grm_norm(b, m):
+        if std_alg:
+            G, s2, nv = grm(b, N, m, 0, min_maf)
+            return G, s2
+        raw, _, s2, _ = grm_exact_centred(b, N, m, min_maf)
+        return (raw / s2 if s2 > 0 else raw), s2
+
+    G_global, S = grm_norm(bed, M)
+    Gk = G_global[np.ix_(keep, keep)]
+    labels = sorted(set(chroms), key=_chrom_key)
+    models, rows = [], []
+
+    def run(Gs, snps, label):
+        D, W, Uty, Vg0, Ve0 = mv_prepare(Gs, Ys, Xd)
+        Vg, Ve, conv, _, _ = mv_null(D, Uty, Vg0, Ve0, solver, max_iter, tol, em_min_iter, ai_max_iter, ai_tol)
+        if not conv:
+            raise RuntimeError("Multivariate null model did not converge")
+        models.append((label, Vg, Ve))
+        if not snps:
+            return
+        o = mv_scan(bed, N, M, keep, W, D, Uty, Vg, Ve, snps)
+        for i, j in enumerate(snps):
+            if np.isnan(o["p"][i]):
+                continue
+            rows.append((j, o["freq"][i], o["wald"][i], o["p"][i], o["beta"][i],
+                         np.sqrt(np.diagonal(o["vbeta"][i]))))
+
+    if no_loco or len(labels) <= 1:
+        snps = [j for j in range(M) if scan_chr is None or chroms[j] == scan_chr]
+        run(Gk, snps, "GLOBAL")
+    else:
+        for c in labels:
+            if scan_chr is not None and c != scan_chr:
+                continue
+            snps = [j for j in range(M) if chroms[j] == c]
+            Gc, S_c = grm_norm(select_snps(bed, N, snps), len(snps))
+            S_loco = S - S_c
+            G = Gk
+            if S_c > 0 and S_loco > 1e-6:
+                G = Gk * (S / S_loco) - Gc[np.ix_(keep, keep)] * (S_c / S_loco)
+            run(G, snps, c)
     return models, rows

@@ -404,3 +404,102 @@ def test_lowrank_exact_when_rank_covers_kinship():
     for fg in (0.9, 1.1):
         for fe in (0.9, 1.1):
             assert reml(m["sg"] * fg, m["se"] * fe) <= best + 1e-9
+
+
+# ------------------------------------------------------------------ multivariate (C5) oracle pins
+def _mv_case(N=300, M=1200, seed=3, d=1):
+    bed = oracle.gen_hwe_bed(seed, N, M, 0.05, 0.5, 0.01)
+    raw, _, S, _ = oracle.grm_exact_centred(bed, N, M)
+    rng = np.random.default_rng(seed)
+    X = np.column_stack([np.ones(N), rng.standard_normal(N)]).astype(np.float32).astype(np.float64)
+    Dm = oracle.decode_numpy(bed, N, M)
+    Dm = np.where(np.isnan(Dm), np.nanmean(Dm, 0), Dm)
+    g = (Dm[:, :60] - Dm[:, :60].mean(0)) @ rng.normal(0, 0.15, (60, d))
+    Y = X[:, 1:2] * 0.3 + g + rng.normal(0, 1, (N, d))
+    return bed, N, M, raw / S, X, Y
+
+
+def test_mv_oracle_reduces_to_univariate():
+    """d = 1: AI-REML reaches the univariate REML optimum of solveDiagonal, and the multivariate
+    Wald, beta and Var(beta) equal the rotated scan's chi^2, beta and SE^2."""
+    bed, N, M, G, X, Y = _mv_case()
+    D, W, Uty, Vg0, Ve0 = oracle.mv_prepare(G, Y, X)
+    Vg, Ve, conv, _, ai_it = oracle.mv_null(D, Uty, Vg0, Ve0, "aireml")
+    D1, W1, U1 = oracle.eigen_vce_prepare(G, Y[:, 0], X)
+    vg, ve, _ = oracle.solve_diagonal(D1, U1)
+    assert conv and ai_it > 0
+    assert abs(Vg[0, 0] - vg) <= 2e-5 * vg and abs(Ve[0, 0] - ve) <= 2e-5 * ve
+    o = oracle.mv_scan(bed, N, M, None, W, D, Uty, np.array([[0.3]]), np.array([[0.8]]), np.arange(80))
+    Dg = oracle.decode_numpy(bed, N, M)[:, :80]
+    obs = ~np.isnan(Dg)
+    Gm = np.where(obs, Dg - np.nansum(Dg, 0) / obs.sum(0), 0.0)
+    st = oracle.rotated_scan(W1, D1, U1, 0.3, 0.8, Gm)
+    chi = (st["beta"] / st["se"]) ** 2
+    assert rel(o["wald"], chi) < 1e-12
+    assert rel(o["beta"][:, 0], st["beta"]) < 1e-12
+    assert rel(np.sqrt(o["vbeta"][:, 0, 0]), st["se"]) < 1e-12
+    assert rel(o["p"], st["p"]) < 1e-12
+
+
+def test_mv_oracle_kronecker_gls():
+    """The per-eigen-sample scan equals brute-force GLS with V = Vg (x) K + Ve (x) I in the
+    projected space (explicit (n-c) d x (n-c) d inverse)."""
+    N, M = 60, 40
+    bed = oracle.gen_hwe_bed(5, N, M, 0.1, 0.5, 0.02)
+    raw, _, S, _ = oracle.grm_exact_centred(bed, N, M)
+    G = raw / S
+    rng = np.random.default_rng(7)
+    X = np.column_stack([np.ones(N), rng.standard_normal(N)]).astype(np.float32).astype(np.float64)
+    Y = rng.normal(0, 1, (N, 2))
+    D, W, Uty, _, _ = oracle.mv_prepare(G, Y, X)
+    Vg = np.array([[0.5, 0.2], [0.2, 0.4]])
+    Ve = np.array([[0.7, -0.1], [-0.1, 0.9]])
+    o = oracle.mv_scan(bed, N, M, None, W, D, Uty, Vg, Ve, np.arange(12))
+    Q, _ = np.linalg.qr(X, mode="complete")
+    A = Q[:, 2:]
+    ne = N - 2
+    Vi = np.linalg.inv(np.kron(Vg, A.T @ G @ A) + np.kron(Ve, np.eye(ne)))
+    yv = (A.T @ Y).T.reshape(-1)
+    Dg = oracle.decode_numpy(bed, N, M)[:, :12]
+    obs = ~np.isnan(Dg)
+    gm = np.where(obs, Dg - np.nansum(Dg, 0) / obs.sum(0), 0.0)
+    for j in range(12):
+        Gd = np.kron(np.eye(2), (A.T @ gm[:, j]).reshape(-1, 1))
+        H = Gd.T @ Vi @ Gd
+        beta = np.linalg.solve(H, Gd.T @ Vi @ yv)
+        assert abs(beta @ H @ beta - o["wald"][j]) <= 1e-11 * o["wald"][j]
+        assert np.max(np.abs(beta - o["beta"][j])) <= 1e-12 * np.max(np.abs(beta)) + 1e-15
+        assert np.max(np.abs(np.linalg.inv(H) - o["vbeta"][j])) <= 1e-11 * np.max(np.abs(o["vbeta"][j]))
+
+
+def test_mv_oracle_score_is_gradient_and_em_ascends():
+    """The AI-REML score is the gradient of the REML log-likelihood (central differences on the
+    packed Vg / Ve entries), the log-likelihood equals the explicit Kronecker form, and EM never
+    decreases it."""
+    bed, N, M, G, X, Y = _mv_case(N=200, M=800, seed=9, d=2)
+    D, W, Uty, Vg, Ve = oracle.mv_prepare(G, Y, X)
+    Vg = Vg + 0.05 * np.eye(2)
+    AI, score = oracle.mv_ai(D, Uty, Vg, Ve)
+    pairs = [(0, 0), (0, 1), (1, 1)]
+    h = 1e-6
+    for a, (j, k) in enumerate(pairs):
+        for which, off in ((0, 0), (1, 3)):
+            P = [Vg.copy(), Ve.copy()]
+            M1 = [Vg.copy(), Ve.copy()]
+            P[which][j, k] += h
+            P[which][k, j] = P[which][j, k]
+            M1[which][j, k] -= h
+            M1[which][k, j] = M1[which][j, k]
+            fd = (oracle.mv_logl(D, Uty, *P) - oracle.mv_logl(D, Uty, *M1)) / (2 * h)
+            assert abs(fd - score[a + off]) <= 1e-5 * max(1.0, abs(fd))
+    ne = D.size
+    V = np.kron(Vg, np.diag(D)) + np.kron(Ve + 1e-8 * np.eye(2), np.eye(ne))
+    yv = Uty.T.reshape(-1)
+    s, ld = np.linalg.slogdet(V)
+    assert abs(-0.5 * (ld + yv @ np.linalg.solve(V, yv)) - oracle.mv_logl(D, Uty, Vg, Ve)) < 1e-8 * ne
+    ll = oracle.mv_logl(D, Uty, Vg, Ve)
+    for _ in range(20):
+        Vg, Ve = oracle.mv_em_update(D, Uty, Vg, Ve)
+        nl = oracle.mv_logl(D, Uty, Vg, Ve)
+        assert nl >= ll - 1e-9 * abs(ll)
+        ll = nl
