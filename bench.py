@@ -134,6 +134,90 @@ def scan_leg(csa, reader, N, M, T=16):
                 end_to_end_trait_genotypes_per_s=N * M * T / total)
 
 
+def mv_leg(csa, n=32_000, m=300_000, d=4, seed=20261207, n_check=32):
+    """Multi-trait LMM (the C5 workload): d = 4 correlated traits, Vg ~ Wishart(4, 0.3 I),
+    Ve ~ Wishart(4, 0.7 I), 3 records per individual averaged, 500 causal SNPs; standardised GRM
+    (alg 0, FP16-split MFMA), MvEigenVce prepare (rocSOLVER QR + dsyevd), AI-REML, and the
+    multivariate Wald scan of all m SNPs.  The scan is checked against an FP64 recomputation
+    (W_proj^T g in FP64 on the host) for n_check SNPs."""
+    rng = np.random.default_rng(seed)
+    out = dict(workload=f"n={n} x m={m}, {d} traits (3 records each, averaged)")
+    t0 = time.perf_counter()
+    reader = csa.PlinkReader.synthetic(seed, n, m, 0.05, 0.5, 0.01)
+    A = rng.normal(0, np.sqrt(0.3), (4, d))
+    B = rng.normal(0, np.sqrt(0.7), (4, d))
+    Vg, Ve = A.T @ A, B.T @ B
+    causal = np.sort(rng.choice(m, 500, replace=False)).astype(np.int32)
+    dZ = csa.DeviceArray(n, causal.size)
+    csa.markers_device(reader, None, causal, dZ, min_maf=0.0)
+    Z = dZ.download()
+    dZ.close()
+    Z /= np.maximum(Z.std(axis=0), 1e-12)
+    g = Z @ (rng.standard_normal((causal.size, d)) @ np.linalg.cholesky(Vg).T) / np.sqrt(causal.size)
+    e = np.mean([rng.standard_normal((n, d)) @ np.linalg.cholesky(Ve).T for _ in range(3)], axis=0)
+    X = np.column_stack([np.ones(n), rng.standard_normal((n, 2))]).astype(np.float32)
+    Y = X[:, 1:].astype(np.float64) @ rng.normal(0, 0.2, (2, d)) + g + e
+    out["setup_s"] = time.perf_counter() - t0
+    csa.synchronize()
+    t0 = time.perf_counter()
+    raw = csa.DeviceArray(n, n)
+    S, _, nv = csa.grm_build_raw(reader, raw, min_maf=0.01, alg=0)
+    G = csa.DeviceArray(n, n)
+    csa.grm_loco_subset_scaled(raw, None, n, None, nv, 1.0, S, 0.0, G)
+    raw.close()
+    csa.synchronize()
+    out["grm_s"] = time.perf_counter() - t0
+    log(f"mv: standardised GRM {out['grm_s']:.1f} s")
+    t0 = time.perf_counter()
+    mv = csa.MvLmm(Y, X)
+    mv.prepare(G)
+    G.close()
+    out["prepare_s"] = time.perf_counter() - t0
+    log(f"mv: prepare (QR, A^T G A, dsyevd, W) {out['prepare_s']:.1f} s")
+    t0 = time.perf_counter()
+    conv, em_it, ai_it = mv.runNullModel("aireml")
+    out["null_s"] = time.perf_counter() - t0
+    Vg_hat, Ve_hat, _, _, ll = mv.variance()
+    out.update(em_iterations=em_it, ai_iterations=ai_it, converged=conv,
+               vg_rel_err_vs_truth=float(np.linalg.norm(Vg_hat - Vg) / np.linalg.norm(Vg)),
+               ve_rel_err_vs_truth=float(np.linalg.norm(Ve_hat - Ve) / np.linalg.norm(Ve)))
+    log(f"mv: AI-REML {out['null_s']:.2f} s (EM {em_it}, AI {ai_it})")
+    csa.synchronize()
+    t0 = time.perf_counter()
+    res = mv.scan(reader)
+    dt = time.perf_counter() - t0
+    out.update(scan_s=dt, scan_genotypes_per_s=n * m / dt, n_tested=int(np.sum(res["status"] >= 0)),
+               rotation_fp16_tflops_issued=3 * 2.0 * m * n * (n - 3) / dt / 1e12)
+    log(f"mv: scan of {m} SNPs {dt:.2f} s ({out['scan_genotypes_per_s']:.3e} genotypes/s)")
+    # FP64 recomputation for n_check SNPs
+    snps = np.sort(rng.choice(m, n_check, replace=False)).astype(np.int32)
+    dG = csa.DeviceArray(n, n_check)
+    csa.markers_device(reader, None, snps, dG, min_maf=0.0)
+    Gm = dG.download()
+    dG.close()
+    D, Uty, W = mv.eigen(with_w=True)
+    Sr = W.T @ Gm
+    del W
+    Sinv = np.linalg.inv(D[:, None, None] * Vg_hat + Ve_hat)
+    H = np.einsum("is,iab->sab", Sr * Sr, Sinv)
+    r = Sr.T @ np.einsum("iab,ib->ia", Sinv, Uty)
+    beta = np.linalg.solve(H, r[:, :, None])[:, :, 0]
+    wald = np.einsum("sa,sab,sb->s", beta, H, beta)
+    se = np.sqrt(np.diagonal(np.linalg.inv(H), axis1=1, axis2=2))
+    from scipy.special import chdtrc
+    lp = -np.log10(chdtrc(d, wald))
+    lp_gpu = -np.log10(res["p"][snps])
+    out.update(check_snps=n_check,
+               max_abs_dlog10p_vs_fp64=float(np.max(np.abs(lp_gpu - lp))),
+               max_dbeta_over_se_vs_fp64=float(np.max(np.abs(res["beta"][snps] - beta) / se)),
+               max_rel_dwald_vs_fp64=float(np.max(np.abs(res["wald"][snps] - wald) / (wald + 1.0))))
+    log(f"mv: vs FP64 on {n_check} SNPs: |dlog10P| {out['max_abs_dlog10p_vs_fp64']:.2e}, "
+        f"|dbeta|/SE {out['max_dbeta_over_se_vs_fp64']:.2e}")
+    mv.close()
+    reader.close()
+    return out
+
+
 def serving_leg(csa, n=100_000, m=1_000_000, n_requests=12, rate=5.0, seed=20261206):
     """Poisson stream of scan requests (rate req/s) over a resident n x m matrix; each request: a
     random cohort (log-uniform 10k..n), 1-64 traits, 0-10 covariates (+ intercept), h2 ~
@@ -199,6 +283,11 @@ def main():
     ap.add_argument("--no-grm", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--mv", action="store_true", help="also run the multi-trait LMM leg (C5 workload)")
+    # the dense eigen path (MvEigenVce::prepare) is bounded by rocSOLVER's 32-bit interfaces at
+    # n <= 32767 individuals; the C5 workload names 50,000
+    ap.add_argument("--mv-n", type=int, default=32_000)
+    ap.add_argument("--mv-m", type=int, default=300_000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 2 GiB device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -304,6 +393,11 @@ def main():
         log(f"serving: p50 {serving['p50_latency_s']:.2f} s, p99 {serving['p99_latency_s']:.2f} s, "
             f"{serving['trait_genotypes_per_s']:.3e} trait-genotypes/s")
 
+    mvres = None
+    if args.mv:
+        op.close()
+        mvres = mv_leg(csa, n=args.mv_n, m=args.mv_m)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(N, args.cpu_slice_snps, k, seed)
@@ -332,6 +426,7 @@ def main():
         "grm": grm,
         "lmm_scan": scan,
         "serving": serving,
+        "multivariate": mvres,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -656,11 +656,11 @@ class DenseLmm:
         return out
 
 
-def grm_build_raw(reader, d_raw, min_maf=0.01, snps=None):
+def grm_build_raw(reader, d_raw, min_maf=0.01, snps=None, alg=1):
     """Raw Z Z^T over a SNP subset into an N x N DeviceArray; returns (S_block, S_seq, n_valid)."""
     snps = _opt_i32(snps)
     a, b, nv = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
-    _check(lib().cs_grm_build_raw(reader._h, 1, float(min_maf), _p(snps, _i32p),
+    _check(lib().cs_grm_build_raw(reader._h, int(alg), float(min_maf), _p(snps, _i32p),
                                   0 if snps is None else snps.size, d_raw.ptr, 1, ctypes.byref(a),
                                   ctypes.byref(b), ctypes.byref(nv)))
     return a.value, b.value, nv.value

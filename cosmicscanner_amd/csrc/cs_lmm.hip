@@ -237,6 +237,7 @@ int cs_lmm_create(const double* y, const float* X, int64_t ldx, int64_t n, int64
         if (!y || !out || (c > 0 && !X)) throw InvalidArgument("cs_lmm_create: null argument");
         if (n < 2 || c < 0 || c >= n || (c > 0 && ldx < n))
             throw RuntimeError("EigenVce::prepare: Dimension mismatch");
+        check_dense_eigen_size(n, "cs_lmm_create");
         auto h = std::make_unique<cs_lmm>();
         h->n = n;
         h->c = c;

@@ -1036,6 +1036,7 @@ int cs_mvlmm_create(const double* Y, int64_t ldy, int64_t n, int64_t d, const fl
         if (d < 1 || d > kMvMaxD) throw InvalidArgument("multivariate LMM: 1 to 6 traits are supported");
         if (n < 2 || c < 0 || c >= n || ldy < n || (c > 0 && ldx < n))
             throw RuntimeError("MvEigenVce::prepare: Dimension mismatch");
+        check_dense_eigen_size(n, "cs_mvlmm_create");
         auto h = std::make_unique<cs_mvlmm>();
         h->n = n;
         h->c = c;

@@ -3,6 +3,9 @@ import ctypes
 import os
 import re
 
+import numpy as np
+import pytest
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -33,3 +36,16 @@ def test_error_path_without_gpu_calls():
     # null handle -> invalid argument, message set, no device work involved
     assert L.cs_kinship_rows(None, None) == csa.CS_ERR_INVALID_ARGUMENT
     assert b"null" in L.cs_last_error()
+
+
+def test_dense_eigen_size_guard_without_gpu_calls():
+    """The dense eigen paths refuse n > 32767 before any device call (rocSOLVER's 32-bit
+    dorgqr / dsyevd interfaces cannot address larger n x n work arrays)."""
+    import cosmicscanner_amd as csa
+    n = 40000
+    y = np.zeros(n)
+    X = np.ones((n, 1), dtype=np.float32)
+    with pytest.raises(ValueError, match="32767"):
+        csa.DenseLmm(y, X)
+    with pytest.raises(ValueError, match="32767"):
+        csa.MvLmm(np.zeros((n, 2)), X)
