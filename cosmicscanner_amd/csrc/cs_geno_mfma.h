@@ -78,8 +78,18 @@ struct GenoArgs {
 // MT x GZ: a wave covers MT 16-row subtiles and GZ groups of 16 right-hand sides; every decoded
 // genotype fragment feeds GZ L (x2) MFMAs.  Grid: x = padded rows / (64 MT), y = k splits,
 // z = ceil(G / GZ).
-template <int L, bool TWO_B, int EPI, bool ONLY_X = false, int MT = 4, int GZ = 1>
-__device__ __forceinline__ void geno_body(const GenoArgs& a) {
+// LDSB (one group of 16 right-hand sides only): the workgroup's four waves share the B
+// fragments of each 256-k block through LDS (double-buffered, staged one block ahead), so a
+// workgroup reads each B byte once instead of once per wave.
+template <int L, bool TWO_B>
+struct GenoLds {
+    static constexpr int kChunksPerOp = 4 * L * 64;                     // 16-byte chunks per k block
+    static constexpr int kChunks = (TWO_B ? 2 : 1) * kChunksPerOp;
+    static constexpr int kPerThread = kChunks / 256;
+};
+
+template <int L, bool TWO_B, int EPI, bool ONLY_X = false, int MT = 4, int GZ = 1, bool LDSB = false>
+__device__ __forceinline__ void geno_body(const GenoArgs& a, uint4* ldsb = nullptr) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
     const int64_t rbase = ((int64_t)blockIdx.x * 4 + wave) * (16 * MT);   // first row of the wave
@@ -112,29 +122,64 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a) {
         const int64_t last_step = kb1 * 4 - 1;
         uint4 av[MT], an[MT];
         v4i bx[GZ][L], bo[GZ][L], bxn[GZ][L], bon[GZ][L];
+        using LD = GenoLds<L, TWO_B>;
+        uint4 stg[LD::kPerThread];
+        auto stage_load = [&](int64_t kb) {
+#pragma unroll
+            for (int i = 0; i < LD::kPerThread; ++i) {
+                const int c = (int)threadIdx.x + 256 * i;
+                const int op = c / LD::kChunksPerOp, w = c - op * LD::kChunksPerOp;
+                const uint4* src = reinterpret_cast<const uint4*>(op == 0 ? a.Bx : a.Bo);
+                stg[i] = src[kb * LD::kChunksPerOp + w];
+            }
+        };
+        auto stage_store = [&](int buf) {
+#pragma unroll
+            for (int i = 0; i < LD::kPerThread; ++i) ldsb[buf * LD::kChunks + (int)threadIdx.x + 256 * i] = stg[i];
+        };
 #pragma unroll
         for (int m = 0; m < MT; ++m) av[m] = *reinterpret_cast<const uint4*>(abase + (kb0 << 12) + m * 1024);
+        if (LDSB) {
+            stage_load(kb0);
+            stage_store(0);
+            __syncthreads();
+        } else {
 #pragma unroll
-        for (int j = 0; j < GZ; ++j)
+            for (int j = 0; j < GZ; ++j)
 #pragma unroll
-            for (int l = 0; l < L; ++l) {
-                if (!gok[j]) continue;
-                bx[j][l] = *reinterpret_cast<const v4i*>(bxp + ((kb0 * 4 * NT + j * L + l) << 10));
-                if (TWO_B) bo[j][l] = *reinterpret_cast<const v4i*>(bop + ((kb0 * 4 * NT + j * L + l) << 10));
-            }
+                for (int l = 0; l < L; ++l) {
+                    if (!gok[j]) continue;
+                    bx[j][l] = *reinterpret_cast<const v4i*>(bxp + ((kb0 * 4 * NT + j * L + l) << 10));
+                    if (TWO_B) bo[j][l] = *reinterpret_cast<const v4i*>(bop + ((kb0 * 4 * NT + j * L + l) << 10));
+                }
+        }
         for (int64_t kb = kb0; kb < kb1; ++kb) {
             const int64_t kn = kb + 1 < kb1 ? kb + 1 : kb;
+            const int cur = (int)((kb - kb0) & 1);
+            if (LDSB && kb + 1 < kb1) stage_load(kb + 1);
 #pragma unroll
             for (int m = 0; m < MT; ++m) an[m] = *reinterpret_cast<const uint4*>(abase + (kn << 12) + m * 1024);
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
+                if (LDSB) {
+                    const uint4* lb = ldsb + cur * LD::kChunks;
+#pragma unroll
+                    for (int l = 0; l < L; ++l) {
+                        const uint4 x = lb[(t * L + l) * 64 + lane];
+                        bx[0][l] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+                        if (TWO_B) {
+                            const uint4 o = lb[LD::kChunksPerOp + (t * L + l) * 64 + lane];
+                            bo[0][l] = v4i{(int)o.x, (int)o.y, (int)o.z, (int)o.w};
+                        }
+                    }
+                }
                 int64_t sn = kb * 4 + t + 1;
                 if (sn > last_step) sn = last_step;
 #pragma unroll
                 for (int j = 0; j < GZ; ++j)
 #pragma unroll
                     for (int l = 0; l < L; ++l) {
-                        if (!gok[j]) continue;
+                        if (LDSB || !gok[j]) continue;
                         bxn[j][l] = *reinterpret_cast<const v4i*>(bxp + ((sn * NT + j * L + l) << 10));
                         if (TWO_B) bon[j][l] = *reinterpret_cast<const v4i*>(bop + ((sn * NT + j * L + l) << 10));
                     }
@@ -163,16 +208,22 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a) {
                         }
                     }
                 }
+                if (!LDSB) {
 #pragma unroll
-                for (int j = 0; j < GZ; ++j)
+                    for (int j = 0; j < GZ; ++j)
 #pragma unroll
-                    for (int l = 0; l < L; ++l) {
-                        bx[j][l] = bxn[j][l];
-                        if (TWO_B) bo[j][l] = bon[j][l];
-                    }
+                        for (int l = 0; l < L; ++l) {
+                            bx[j][l] = bxn[j][l];
+                            if (TWO_B) bo[j][l] = bon[j][l];
+                        }
+                }
             }
 #pragma unroll
             for (int m = 0; m < MT; ++m) av[m] = an[m];
+            if (LDSB) {
+                if (kb + 1 < kb1) stage_store(cur ^ 1);
+                __syncthreads();
+            }
         }
     }
 
@@ -245,6 +296,17 @@ __global__ __launch_bounds__(256) void k_genox(GenoArgs a) {
 template <int L, int EPI, int MT = 4, int GZ = 1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno2(GenoArgs a) {
     geno_body<L, true, EPI, false, MT, GZ>(a);
+}
+// One group of 16 right-hand sides with the B fragments shared through LDS (GenoLds).
+template <int L, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno2s(GenoArgs a) {
+    __shared__ uint4 ldsb[2 * GenoLds<L, true>::kChunks];
+    geno_body<L, true, EPI, false, 4, 1, true>(a, ldsb);
+}
+template <int L, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno1s(GenoArgs a) {
+    __shared__ uint4 ldsb[2 * GenoLds<L, false>::kChunks];
+    geno_body<L, false, EPI, false, 4, 1, true>(a, ldsb);
 }
 
 }  // namespace csamd

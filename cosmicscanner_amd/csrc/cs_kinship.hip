@@ -333,8 +333,18 @@ static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a) {
         else k_geno1<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
         return;
     }
-    if (TWO_B) k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
-    else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
+    // one group: B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
+    static const bool ldsb = [] {
+        const char* e = std::getenv("CSAMD_GENO_LDSB");
+        return !(e && e[0] == '0');
+    }();
+    if (TWO_B) {
+        if (ldsb) k_geno2s<L, EPI><<<grid, 256, 0, st>>>(a);
+        else k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
+    } else {
+        if (ldsb) k_geno1s<L, EPI><<<grid, 256, 0, st>>>(a);
+        else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
+    }
 }
 
 void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const GenoArgs& a, bool only_x) {
@@ -493,10 +503,28 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     // Z W over the sample-major copy, split along the SNPs, exact int64 accumulation.
     const int64_t kbt = b->KBm;
     const int64_t rblocks = b->Npad / 256;
-    int64_t nsplit = std::max<int64_t>(1, (1536 + rblocks * tl.G - 1) / (rblocks * tl.G));
-    nsplit = std::min<int64_t>(nsplit, kbt);
-    const int64_t kbps = (kbt + nsplit - 1) / nsplit;
-    nsplit = (kbt + kbps - 1) / kbps;
+    // SNP splits: at least ~1536 workgroups, and a count that fills the last round of the
+    // device's resident workgroups (two per CU at two waves per SIMD) as fully as possible.
+    const int64_t gz = tl.G >= 2 ? 2 * ((tl.G + 1) / 2) : 1;   // launch_one's z and x factor
+    const int64_t per_split = rblocks * gz;
+    static const int64_t slots = [] {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        return 2 * (int64_t)std::max(cus, 1);
+    }();
+    const int64_t base = std::min<int64_t>(std::max<int64_t>(1, (1536 + per_split - 1) / per_split), kbt);
+    int64_t nsplit = base, kbps = (kbt + base - 1) / base;
+    double best = -1.0;
+    for (int64_t cand = base; cand <= std::min<int64_t>(kbt, 2 * base); ++cand) {
+        const int64_t kp = (kbt + cand - 1) / cand, ns = (kbt + kp - 1) / kp;
+        const int64_t blocks = per_split * ns;
+        const double eff = (double)blocks / (double)(((blocks + slots - 1) / slots) * slots);
+        if (eff > best + 1e-9) {
+            best = eff;
+            nsplit = ns;
+            kbps = kp;
+        }
+    }
     GenoArgs c{};
     c.A = b->smp.as<uint8_t>();
     c.KB = b->KBm;
