@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     geno_body<L, true, EPI, false, 4, 1, true>(a, ldsb);
 }
 template <int L, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno1s(GenoArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_geno1s(GenoArgs a) {
     __shared__ uint4 ldsb[2 * GenoLds<L, false>::kChunks];
     geno_body<L, false, EPI, false, 4, 1, true>(a, ldsb);
 }
