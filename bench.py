@@ -339,9 +339,9 @@ def main():
     gt_bytes = M * ((N + 3) // 4)
     g_bytes = N * ((M + 3) // 4)
     if phase[1] >= phase[3]:
-        name, kms, kbytes, key = "k_geno1 (Z^T V, SNP-major)", phase[1], gt_bytes, "k_geno1"
+        name, kms, kbytes, key = "k_geno1s (Z^T V, SNP-major)", phase[1], gt_bytes, "k_geno1s"
     else:
-        name, kms, kbytes, key = "k_geno2 (Z W, sample-major)", phase[3], g_bytes, "k_geno2"
+        name, kms, kbytes, key = "k_geno2s (Z W, sample-major)", phase[3], g_bytes, "k_geno2s"
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = profiled_traffic(key) if (N, M, k) == (50_000, 500_000, 16) else None
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
