@@ -134,6 +134,69 @@ def scan_leg(csa, reader, N, M, T=16):
                 end_to_end_trait_genotypes_per_s=N * M * T / total)
 
 
+# GRCh38 autosome lengths (Mb), the chromosome proportions of the LOCO workload
+CHROM_MB = [248.96, 242.19, 198.30, 190.22, 181.54, 170.81, 159.35, 145.14, 138.39, 133.80, 135.09,
+            133.28, 114.36, 107.04, 101.99, 90.34, 83.26, 80.37, 58.62, 64.44, 46.71, 50.82]
+
+
+def loco_leg(csa, n=200_000, m=600_000, T=16, seed=20261205):
+    """LOCO matrix-free PCG + LMM scan (the C3 workload): m SNPs over 22 chromosomes in proportion
+    to their lengths, 20 causal SNPs per chromosome, h2 = 0.4, T phenotype replicates with
+    independent effects and noise; per chromosome one kinship operator without that chromosome,
+    one multi-RHS PCG solve (T traits + 3 covariates + 16 calibration SNPs, relative residual
+    1e-6) and the fused scan of that chromosome's SNPs.  genotypes/s = n m / total time."""
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    reader = csa.PlinkReader.synthetic(seed, n, m, 0.01, 0.5, 0.005)
+    bounds = np.concatenate([[0], np.round(np.cumsum(CHROM_MB) / np.sum(CHROM_MB) * m).astype(np.int64)])
+    bounds[-1] = m
+    causal = np.concatenate([rng.choice(np.arange(bounds[c], bounds[c + 1]), 20, replace=False)
+                             for c in range(22)]).astype(np.int32)
+    causal.sort()
+    dZ = csa.DeviceArray(n, causal.size)
+    csa.markers_device(reader, None, causal, dZ, min_maf=0.0)
+    Z = dZ.download()
+    dZ.close()
+    Z /= np.maximum(Z.std(axis=0), 1e-12)
+    X = np.column_stack([np.ones(n), rng.standard_normal((n, 2))]).astype(np.float32)
+    G = Z @ rng.standard_normal((causal.size, T))
+    G *= np.sqrt(0.4) / G.std(axis=0)
+    Y = G + np.sqrt(0.6) * rng.standard_normal((n, T))
+    del Z, G
+    means, inv_stds = reader.getMeans(), reader.getInvStds()
+    f = means / 2.0
+    grm_ok = (inv_stds != 0) & (np.minimum(f, 1 - f) >= 0.01)
+    setup = time.perf_counter() - t0
+    csa.synchronize()
+    t0 = time.perf_counter()
+    iters, resid, gammas, tested = [], [], [], 0
+    for c in range(22):
+        lo, hi = bounds[c], bounds[c + 1]
+        idx = np.arange(m)
+        loco = idx[grm_ok & ((idx < lo) | (idx >= hi))].astype(np.int32)
+        snps = np.arange(lo, hi, dtype=np.int32)
+        cal = np.sort(rng.choice(snps, 16, replace=False)).astype(np.int32)
+        op = csa.PlinkKinshipOperator(reader, None, loco)
+        out = csa.pcg_projected_scan(op, reader, None, Y, X, 0.4, 0.6, cal, snps=snps, rel_tol=1e-6,
+                                     max_iter=200)
+        op.close()
+        iters.append(out["pcg_iterations"])
+        resid.append(out["max_rel_residual"])
+        gammas.append(out["gamma"])
+        tested += int(np.sum(out["passes"]))
+    csa.synchronize()
+    total = time.perf_counter() - t0
+    reader.close()
+    res = dict(workload=f"n={n} x m={m}, 22 LOCO chromosomes, {T} traits, PCG rel. residual 1e-6",
+               setup_s=setup, total_s=total, genotypes_per_s=n * m / total,
+               trait_genotypes_per_s=n * m * T / total, pcg_iterations=iters,
+               max_rel_residual=float(np.max(resid)), gamma_range=[float(np.min(gammas)), float(np.max(gammas))],
+               n_tested=tested)
+    log(f"LOCO leg: {total:.1f} s for 22 chromosomes, {res['genotypes_per_s']:.3e} genotypes/s, "
+        f"PCG iterations {min(iters)}-{max(iters)}")
+    return res
+
+
 def mv_leg(csa, n=32_000, m=300_000, d=4, seed=20261207, n_check=32):
     """Multi-trait LMM (the C5 workload): d = 4 correlated traits, Vg ~ Wishart(4, 0.3 I),
     Ve ~ Wishart(4, 0.7 I), 3 records per individual averaged, 500 causal SNPs; standardised GRM
@@ -284,6 +347,7 @@ def main():
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--mv", action="store_true", help="also run the multi-trait LMM leg (C5 workload)")
+    ap.add_argument("--loco", action="store_true", help="also run the LOCO PCG + scan leg (C3 workload)")
     # the dense eigen path (MvEigenVce::prepare) is bounded by rocSOLVER's 32-bit interfaces at
     # n <= 32767 individuals; the C5 workload names 50,000
     ap.add_argument("--mv-n", type=int, default=32_000)
@@ -393,6 +457,11 @@ def main():
         log(f"serving: p50 {serving['p50_latency_s']:.2f} s, p99 {serving['p99_latency_s']:.2f} s, "
             f"{serving['trait_genotypes_per_s']:.3e} trait-genotypes/s")
 
+    locores = None
+    if args.loco:
+        op.close()
+        locores = loco_leg(csa)
+
     mvres = None
     if args.mv:
         op.close()
@@ -427,6 +496,7 @@ def main():
         "lmm_scan": scan,
         "serving": serving,
         "multivariate": mvres,
+        "loco_scan": locores,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -30,6 +30,7 @@ CS_ERR_RUNTIME = -2
 CS_PREC_INT8_L3 = 3
 CS_PREC_INT8_L2 = 2
 CS_PREC_FP32 = 32
+CS_PREC_INT8_L4 = 4
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)

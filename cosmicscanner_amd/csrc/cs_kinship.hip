@@ -327,23 +327,30 @@ __global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, i
 // column groups each wave takes 32 rows and two groups (half the genotype passes and decodes).
 template <int L, bool TWO_B, int EPI>
 static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a) {
-    if (a.G >= 2) {
-        const dim3 g2(grid.x * 2, grid.y, (unsigned)((a.G + 1) / 2));
-        if (TWO_B) k_geno2<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
-        else k_geno1<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
-        return;
-    }
-    // one group: B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
-    static const bool ldsb = [] {
-        const char* e = std::getenv("CSAMD_GENO_LDSB");
-        return !(e && e[0] == '0');
-    }();
-    if (TWO_B) {
-        if (ldsb) k_geno2s<L, EPI><<<grid, 256, 0, st>>>(a);
-        else k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
-    } else {
-        if (ldsb) k_geno1s<L, EPI><<<grid, 256, 0, st>>>(a);
+    if constexpr (L == 4) {
+        // four limbs: the LDS-shared and two-group forms exceed the register file (they spill),
+        // so every group count takes the four-tile form, one group per z slice
+        if (TWO_B) k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
         else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
+    } else {
+        if (a.G >= 2) {
+            const dim3 g2(grid.x * 2, grid.y, (unsigned)((a.G + 1) / 2));
+            if (TWO_B) k_geno2<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
+            else k_geno1<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
+            return;
+        }
+        // one group: B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
+        static const bool ldsb = [] {
+            const char* e = std::getenv("CSAMD_GENO_LDSB");
+            return !(e && e[0] == '0');
+        }();
+        if (TWO_B) {
+            if (ldsb) k_geno2s<L, EPI><<<grid, 256, 0, st>>>(a);
+            else k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
+        } else {
+            if (ldsb) k_geno1s<L, EPI><<<grid, 256, 0, st>>>(a);
+            else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
+        }
     }
 }
 
@@ -377,13 +384,16 @@ void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const Ge
     CS_GENO_CASE(2, false, EPI_KV_GT)
     CS_GENO_CASE(3, true, EPI_KV_G)
     CS_GENO_CASE(2, true, EPI_KV_G)
+    CS_GENO_CASE(4, false, EPI_PQ)
+    CS_GENO_CASE(4, false, EPI_KV_GT)
+    CS_GENO_CASE(4, true, EPI_KV_G)
 #undef CS_GENO_CASE
     throw RuntimeError("launch_geno: unsupported configuration");
 }
 
 void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, const Tiling& tl,
                    hipStream_t st, DevBuf& mx, DevBuf& scl, int8_t* const* B) {
-    const double qmax = tl.L == 3 ? kQmax3 : kQmax2;
+    const double qmax = limbs_qmax(tl.L);
     mx.ensure((size_t)k * sizeof(unsigned long long));
     scl.ensure((size_t)2 * k * sizeof(double));
     CS_HIP(hipMemsetAsync(mx.p, 0, (size_t)k * sizeof(unsigned long long), st));
@@ -406,7 +416,7 @@ static bool use_sparse_missing(cs_kinship* op) {
     // Measured at n = 50k, m = 500k, 0.5 % missing, k = 16: the genotype-only MFMA kernels take
     // 1.77 + 1.88 ms against 2.09 + 2.84 ms for the dense pair, but the two list gathers take
     // 2.7 + 3.5 ms, so auto keeps the dense form.
-    if (op->missing_mode != 2) return false;
+    if (op->missing_mode != 2 || op->prec == CS_PREC_INT8_L4) return false;   // int32 plain operand
     ensure_missing_lists(b);
     return true;
 }
@@ -415,8 +425,8 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
                        int64_t ldy) {
     cs_bed* b = op->bed;
     hipStream_t st = op->stream;
-    const int L = op->prec == CS_PREC_INT8_L2 ? 2 : 3;
-    const double qmax = L == 3 ? kQmax3 : kQmax2;
+    const int L = prec_limbs(op->prec);
+    const double qmax = limbs_qmax(L);
     const Tiling tl = tiling(k, L);
     const int64_t K16 = 16 * (int64_t)tl.G;
     const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
@@ -505,7 +515,8 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     const int64_t rblocks = b->Npad / 256;
     // SNP splits: at least ~1536 workgroups, and a count that fills the last round of the
     // device's resident workgroups (two per CU at two waves per SIMD) as fully as possible.
-    const int64_t gz = tl.G >= 2 ? 2 * ((tl.G + 1) / 2) : 1;   // launch_one's z and x factor
+    // launch_one's z and x factor
+    const int64_t gz = L == 4 ? tl.G : (tl.G >= 2 ? 2 * ((tl.G + 1) / 2) : 1);
     const int64_t per_split = rblocks * gz;
     static const int64_t slots = [] {
         int dev = 0, cus = 256;
@@ -710,11 +721,17 @@ int cs_kinship_num_variants(const cs_kinship* op, int64_t* m) {
 int cs_kinship_set_precision(cs_kinship* op, int mode) {
     return guarded([&] {
         if (!op) throw InvalidArgument("cs_kinship_set_precision: null handle");
-        if (mode != CS_PREC_INT8_L3 && mode != CS_PREC_INT8_L2 && mode != CS_PREC_FP32)
+        if (mode != CS_PREC_INT8_L3 && mode != CS_PREC_INT8_L2 && mode != CS_PREC_INT8_L4 && mode != CS_PREC_FP32)
             throw InvalidArgument("cs_kinship_set_precision: unknown mode");
         op->prec = mode;
     });
 }
+
+}  // extern "C"
+
+int csamd::kinship_get_precision(const cs_kinship* op) { return op->prec; }
+
+extern "C" {
 
 int cs_kinship_set_missing_mode(cs_kinship* op, int mode) {
     return guarded([&] {

@@ -6,6 +6,11 @@ namespace csamd {
 
 constexpr double kQmax3 = 127.0 * 65536.0;  // |q| bound for 3 balanced limbs (top limb <= 127)
 constexpr double kQmax2 = 127.0 * 256.0;
+constexpr double kQmax4 = 127.0 * 16777216.0;   // 4 limbs: 31-bit fixed point
+inline int prec_limbs(int prec) { return prec == CS_PREC_INT8_L2 ? 2 : prec == CS_PREC_INT8_L4 ? 4 : 3; }
+inline double limbs_qmax(int L) { return L == 2 ? kQmax2 : L == 4 ? kQmax4 : kQmax3; }
+// Current operand precision of a kinship operator (CS_PREC_*).
+int kinship_get_precision(const cs_kinship* op);
 
 inline int grid_for(int64_t total, int block = 256) {
     int64_t g = (total + block - 1) / block;

@@ -156,9 +156,9 @@ void kinship_apply(cs_kinship* K, const double* dX, int64_t n, int64_t k, double
 
 }  // namespace
 
-static void pcg_run(cs_kinship* K, const double* sg_c, const double* se_c, const double* dB, int64_t ldb,
-                    int64_t k, double* dX, int64_t ldx, double rel_tol, int max_iter, int precond,
-                    int warm_start, int* iterations, double* rel_residual) {
+static void pcg_core(cs_kinship* K, const double* sg_c, const double* se_c, const double* dB, int64_t ldb,
+                     int64_t k, double* dX, int64_t ldx, double rel_tol, int max_iter, int precond,
+                     int warm_start, int* iterations, double* rel_residual) {
     if (!K || (k > 0 && (!dB || !dX))) throw InvalidArgument("cs_pcg_solve: null argument");
     if (k < 0 || max_iter < 0 || !(rel_tol >= 0.0)) throw InvalidArgument("cs_pcg_solve: bad argument");
     for (int64_t c = 0; c < k; ++c)
@@ -256,6 +256,33 @@ static void pcg_run(cs_kinship* K, const double* sg_c, const double* se_c, const
         }
     }
     CS_HIP(hipStreamSynchronize(st.s));
+}
+
+// PCG with the operator's precision.  With INT8_L3 operands (23-bit right-hand sides) the true
+// residual ||B - A X|| / ||B|| levels off near 1e-6-1e-5 at biobank sizes (each apply re-quantises
+// its input), so for tolerances below 1e-5 the solve continues as a warm-started PCG with 31-bit
+// operands (INT8_L4) from the INT8_L3 solution: the restart recomputes the residual exactly at
+// that precision (residual replacement) and iterates until it meets rel_tol.  The operator's
+// precision is restored on return.
+static void pcg_run(cs_kinship* K, const double* sg_c, const double* se_c, const double* dB, int64_t ldb,
+                    int64_t k, double* dX, int64_t ldx, double rel_tol, int max_iter, int precond,
+                    int warm_start, int* iterations, double* rel_residual) {
+    const int prec = K ? kinship_get_precision(K) : CS_PREC_INT8_L3;
+    if (prec != CS_PREC_INT8_L3 || !(rel_tol < 1e-5)) {
+        pcg_core(K, sg_c, se_c, dB, ldb, k, dX, ldx, rel_tol, max_iter, precond, warm_start, iterations,
+                 rel_residual);
+        return;
+    }
+    int it1 = 0, it2 = 0;
+    pcg_core(K, sg_c, se_c, dB, ldb, k, dX, ldx, rel_tol, max_iter, precond, warm_start, &it1, nullptr);
+    struct Restore {
+        cs_kinship* K;
+        ~Restore() { (void)cs_kinship_set_precision(K, CS_PREC_INT8_L3); }
+    } restore{K};
+    if (cs_kinship_set_precision(K, CS_PREC_INT8_L4) != CS_OK) throw RuntimeError(cs_last_error());
+    pcg_core(K, sg_c, se_c, dB, ldb, k, dX, ldx, rel_tol, std::max(0, max_iter - it1), precond, 1, &it2,
+             rel_residual);
+    if (iterations) *iterations = it1 + it2;
 }
 
 extern "C" {

@@ -131,7 +131,7 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
         hi = std::max<int64_t>(hi, snps[i]);
     }
     if (ns == 0) return;
-    const int L = a.prec == CS_PREC_INT8_L2 ? 2 : 3;
+    const int L = prec_limbs(a.prec);
     hipStream_t st;
     CS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     try {

@@ -27,9 +27,12 @@ extern "C" {
  * INT8_L3: int8 MFMA, exact genotype operand, right-hand sides as 3 signed 8-bit limbs
  *          (23-bit fixed point per column), int32 accumulation -- the default.
  * INT8_L2: as above with 2 limbs (15-bit fixed point per column).
+ * INT8_L4: as above with 4 limbs (31-bit fixed point per column); the PCG's continuation
+ *          below rel_tol 1e-5 runs on it.
  * FP32:    plain FP32 VALU kernels (decode to FP32, FMA, no MFMA, no split) -- the comparator. */
 #define CS_PREC_INT8_L3 3
 #define CS_PREC_INT8_L2 2
+#define CS_PREC_INT8_L4 4
 #define CS_PREC_FP32 32
 
 typedef struct cs_bed cs_bed;
@@ -202,7 +205,11 @@ int cs_grm_last_timing(double* ms_products, double* ms_combine, int* launches, d
  * iterations at most).  precond 0: scalar 1 / (sigma_g + sigma_e) (lmm_engine.cpp:306-313);
  * 1: Jacobi with diag(K).  dB, dX: device n x k column-major; warm_start != 0 starts from the
  * given dX.  iterations: iterations run; rel_residual (k, may be NULL): the true residual
- * ||B - A X|| / ||B|| per column, from one more apply. */
+ * ||B - A X|| / ||B|| per column, from one more apply.  With an INT8_L3 operator and
+ * rel_tol < 1e-5 the solve continues, once the INT8_L3 iteration has met rel_tol, as a
+ * warm-started PCG on INT8_L4 operands (the 23-bit operands leave the true residual near
+ * 1e-6-1e-5); iterations counts both phases, the true residual is the INT8_L4 one and the
+ * operator is back at INT8_L3 on return. */
 int cs_pcg_solve(cs_kinship* K, double sigma_g, double sigma_e, const double* dB, int64_t ldb,
                  int64_t k, double* dX, int64_t ldx, double rel_tol, int max_iter, int precond,
                  int warm_start, int* iterations, double* rel_residual);

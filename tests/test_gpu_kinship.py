@@ -7,7 +7,8 @@ selections, RHS widths 1 / 7 / 41, trace, repeat determinism, empty RHS and dime
 Tolerances.  Integer work (class counts, generated bytes) and the FP64 statistics are compared
 bit for bit.  K.V in the default INT8_L3 mode quantises each RHS column to 23-bit fixed point (and
 the intermediate Z^T V likewise), so the bound is relative to the column maximum:
-max|Y - K X| <= 2e-6 * max|K X| (per column).  INT8_L2 (15-bit): 1e-3.  FP32: 2e-5.
+max|Y - K X| <= 2e-6 * max|K X| (per column).  INT8_L2 (15-bit): 1e-3.  INT8_L4 (31-bit): 3e-8.
+FP32: 2e-5.
 trace/diag are FP64: relative 1e-10 as in the reference test.
 """
 import numpy as np
@@ -101,7 +102,7 @@ def test_kinship_errors(csa, kb_fixture):
         csa.PlinkKinshipOperator(reader, None, [M])
 
 
-@pytest.mark.parametrize("mode,tol", [(3, 2e-6), (2, 1e-3), (32, 2e-5)])
+@pytest.mark.parametrize("mode,tol", [(3, 2e-6), (2, 1e-3), (32, 2e-5), (4, 3e-8)])
 def test_kinship_precision_modes_larger(csa, mode, tol):
     N, M = 3000, 4100
     body = oracle.gen_hwe_bed(77, N, M, 0.01, 0.5, 0.01)
@@ -114,9 +115,11 @@ def test_kinship_precision_modes_larger(csa, mode, tol):
     for kp, vp in [(None, None), (keep, loco)]:
         op = csa.PlinkKinshipOperator(reader, kp, vp)
         op.set_precision(mode)
-        X = rng.standard_normal((op.rows(), 16))
+        X = rng.standard_normal((op.rows(), 41 if mode == 4 else 16))   # INT8_L4: three groups
         ref = oracle.kinship_apply(body, N, means, inv, kp, vp, X)
-        assert colrel(op.apply(X), ref) < tol
+        err = colrel(op.apply(X), ref)
+        print(f"mode {mode}: column-relative error {err:.2e}", flush=True)
+        assert err < tol
         op.close()
 
 

@@ -107,3 +107,27 @@ def test_projected_scan_vs_oracle(csa, data):
     lp = out["log10p"][:, 0][okm]
     mm = ~np.isnan(lp) & ex["tested"]
     print(f"calibrated vs exact GLS: max |d log10 P| = {np.max(np.abs(lp[mm] - ex['log10p'][mm])):.3e}")
+
+
+def test_pcg_refines_below_operator_floor(csa, data):
+    """Tolerances under 1e-5 continue with 31-bit operands: the exact residual (dense FP64 K)
+    meets rel_tol, and the operator is back at its own precision afterwards (bitwise apply)."""
+    N, M, body, keep, loco = data["N"], data["M"], data["body"], data["keep"], data["loco"]
+    r = csa.PlinkReader.from_body(body, N, M)
+    op = csa.PlinkKinshipOperator(r, keep, loco)
+    n = keep.size
+    K = oracle.dense_kinship(body, N, M, keep, loco)
+    rng = np.random.default_rng(9)
+    B = rng.standard_normal((n, 5))
+    probe = rng.standard_normal((n, 3))
+    before = op.apply(probe)
+    for tol in (1e-6, 1e-7):
+        dX = csa.DeviceArray(n, 5)
+        it, res = csa.pcg_solve(op, 0.7, 0.3, csa.DeviceArray.from_host(B), dX, rel_tol=tol, max_iter=400)
+        X = dX.download()
+        exact = np.linalg.norm(B - (0.7 * K @ X + 0.3 * X), axis=0) / np.linalg.norm(B, axis=0)
+        print(f"tol {tol:.0e}: {it} iterations, reported {res.max():.2e}, exact residual {exact.max():.2e}",
+              flush=True)
+        assert res.max() <= tol and exact.max() <= 1.5 * tol
+    assert np.array_equal(op.apply(probe), before)
+    op.close()
