@@ -22,10 +22,13 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace csamd {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ v4i as_v4i(const v4i& x) { return x; }
+__device__ __forceinline__ v4i as_v4i(const uint4& x) { return v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w}; }
 
 // byte c of the LUT = value for code c: x~ = {2, 0, 1, 0}, o = {1, 0, 1, 1}
 constexpr uint32_t kLutX = 0x00010002u;
@@ -56,7 +59,7 @@ struct GenoArgs {
     int64_t kb_per_split, kb_total;
     int G;                     // groups of 16 right-hand sides (= gridDim.z)
     const int8_t* Bx;
-    const int8_t* Bo;
+    const int8_t* Bo;          // EPI_KV_G with two operands: the digits of -(mu W)
     int64_t k;                 // real number of right-hand sides
     long long* pq;             // EPI_PQ: [row][2][16 G]
     const double* scale;       // EPI_KV_GT: per-column scale of the quantised V
@@ -89,7 +92,15 @@ struct GenoLds {
 };
 
 template <int L, bool TWO_B, int EPI, bool ONLY_X = false, int MT = 4, int GZ = 1, bool LDSB = false>
-__device__ __forceinline__ void geno_body(const GenoArgs& a, uint4* ldsb = nullptr) {
+__device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nullptr) {
+    // B staging type: native vectors for the two-operand form (a HIP_vector_type array there is
+    // copied through a stack object), the HIP vector type for the one-operand form (fewer
+    // registers at three waves per SIMD).
+    using ST = typename std::conditional<TWO_B, v4i, uint4>::type;
+    ST* ldsb = static_cast<ST*>(lds_raw);
+    // Z W needs only X~ W1 - O (mu W): with the second operand stored negated, both products
+    // accumulate into one set of int32 registers (exact while |sum| < 2^31: 384 per k-element).
+    constexpr bool FUSE = TWO_B && EPI == EPI_KV_G && !ONLY_X;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
     const int64_t rbase = ((int64_t)blockIdx.x * 4 + wave) * (16 * MT);   // first row of the wave
@@ -123,13 +134,13 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, uint4* ldsb = nullp
         uint4 av[MT], an[MT];
         v4i bx[GZ][L], bo[GZ][L], bxn[GZ][L], bon[GZ][L];
         using LD = GenoLds<L, TWO_B>;
-        uint4 stg[LD::kPerThread];
+        ST stg[LD::kPerThread];
         auto stage_load = [&](int64_t kb) {
 #pragma unroll
             for (int i = 0; i < LD::kPerThread; ++i) {
                 const int c = (int)threadIdx.x + 256 * i;
                 const int op = c / LD::kChunksPerOp, w = c - op * LD::kChunksPerOp;
-                const uint4* src = reinterpret_cast<const uint4*>(op == 0 ? a.Bx : a.Bo);
+                const ST* src = reinterpret_cast<const ST*>(op == 0 ? a.Bx : a.Bo);
                 stg[i] = src[kb * LD::kChunksPerOp + w];
             }
         };
@@ -162,14 +173,12 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, uint4* ldsb = nullp
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 if (LDSB) {
-                    const uint4* lb = ldsb + cur * LD::kChunks;
+                    const ST* lb = ldsb + cur * LD::kChunks;
 #pragma unroll
                     for (int l = 0; l < L; ++l) {
-                        const uint4 x = lb[(t * L + l) * 64 + lane];
-                        bx[0][l] = v4i{(int)x.x, (int)x.y, (int)x.z, (int)x.w};
+                        bx[0][l] = as_v4i(lb[(t * L + l) * 64 + lane]);
                         if (TWO_B) {
-                            const uint4 o = lb[LD::kChunksPerOp + (t * L + l) * 64 + lane];
-                            bo[0][l] = v4i{(int)o.x, (int)o.y, (int)o.z, (int)o.w};
+                            bo[0][l] = as_v4i(lb[LD::kChunksPerOp + (t * L + l) * 64 + lane]);
                         }
                     }
                 }
@@ -202,9 +211,14 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, uint4* ldsb = nullp
 #pragma unroll
                         for (int l = 0; l < L; ++l) {
                             accX[m][j][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(X, bx[j][l], accX[m][j][l], 0, 0, 0);
-                            if (!ONLY_X)
+                            if (!ONLY_X && !FUSE)
                                 accO[m][j][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[j][l] : bx[j][l],
                                                                                       accO[m][j][l], 0, 0, 0);
+                        }
+                        if (FUSE) {
+#pragma unroll
+                            for (int l = 0; l < L; ++l)
+                                accX[m][j][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, bo[j][l], accX[m][j][l], 0, 0, 0);
                         }
                     }
                 }
@@ -243,7 +257,7 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, uint4* ldsb = nullp
 #pragma unroll
                 for (int l = L - 1; l >= 0; --l) {
                     P = P * 256 + accX[m][j][l][v];
-                    if (!ONLY_X) Q = Q * 256 + accO[m][j][l][v];
+                    if (!ONLY_X && !FUSE) Q = Q * 256 + accO[m][j][l][v];
                 }
                 if (ONLY_X && EPI != EPI_KV_G && col_ok && row < a.rows) Q = a.qext[row * 16 * (int64_t)a.G + rhs];
                 if (EPI == EPI_PQ) {
@@ -269,7 +283,7 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, uint4* ldsb = nullp
                 } else {
                     if (col_ok && row < a.rows)
                         atomicAdd(reinterpret_cast<unsigned long long*>(a.yint + row * a.k + rhs),
-                                  (unsigned long long)(ONLY_X ? P : P - Q));
+                                  (unsigned long long)(ONLY_X || FUSE ? P : P - Q));
                 }
             }
         if (EPI == EPI_KV_GT) {
@@ -297,10 +311,11 @@ template <int L, int EPI, int MT = 4, int GZ = 1>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno2(GenoArgs a) {
     geno_body<L, true, EPI, false, MT, GZ>(a);
 }
-// One group of 16 right-hand sides with the B fragments shared through LDS (GenoLds).
+// One group of 16 right-hand sides with the B fragments shared through LDS (GenoLds).  For Z W
+// the two products share one accumulator set (FUSE), which leaves room for 3 waves per SIMD.
 template <int L, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno2s(GenoArgs a) {
-    __shared__ uint4 ldsb[2 * GenoLds<L, true>::kChunks];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_geno2s(GenoArgs a) {
+    __shared__ v4i ldsb[2 * GenoLds<L, true>::kChunks];
     geno_body<L, true, EPI, false, 4, 1, true>(a, ldsb);
 }
 template <int L, int EPI>

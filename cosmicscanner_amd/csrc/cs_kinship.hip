@@ -90,15 +90,16 @@ __global__ void k_scales(const unsigned long long* __restrict__ mx, int64_t k, d
 
 // Fixed-point quantisation into L balanced signed 8-bit limbs, stored pre-permuted for the MFMA
 // B operand: column c, limb l -> tile (c/16)*L + l, column c%16 of the tile.
+// sign = -1 quantises -W (rint and the clamp are symmetric, so the digits give exactly -q(W)).
 __global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, int L, int NT,
                         double qmax, const double* __restrict__ inv, int8_t* __restrict__ B,
-                        int32_t* __restrict__ plain = nullptr, int64_t K16 = 0) {
+                        int32_t* __restrict__ plain = nullptr, int64_t K16 = 0, double sign = 1.0) {
     const int64_t total = rows * k;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t row = e / k;
         const int c = (int)(e - row * k);
-        double qd = rint(W[e] * inv[c]);
+        double qd = rint(sign * W[e] * inv[c]);
         qd = fmin(fmax(qd, -qmax), qmax);
         long long q = (long long)qd;
         if (plain) plain[row * K16 + c] = (int32_t)q;
@@ -502,8 +503,9 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     } else {
         op->B2.ensure((size_t)(b->Mpad * tl.NT * 16));
         CS_HIP(hipMemsetAsync(op->B2.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
+        // negated: the Z W kernel accumulates X~ W1 and O (-mu W) into the same registers
         k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, L, tl.NT, qmax,
-                                                      sclW + k, op->B2.as<int8_t>());
+                                                      sclW + k, op->B2.as<int8_t>(), nullptr, 0, -1.0);
     }
     CS_HIP(hipGetLastError());
     op->Yint.ensure((size_t)(b->Npad * k) * sizeof(long long));
@@ -514,16 +516,23 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     const int64_t kbt = b->KBm;
     const int64_t rblocks = b->Npad / 256;
     // SNP splits: at least ~1536 workgroups, and a count that fills the last round of the
-    // device's resident workgroups (two per CU at two waves per SIMD) as fully as possible.
+    // device's resident workgroups as fully as possible.
     // launch_one's z and x factor
     const int64_t gz = L == 4 ? tl.G : (tl.G >= 2 ? 2 * ((tl.G + 1) / 2) : 1);
     const int64_t per_split = rblocks * gz;
-    static const int64_t slots = [] {
-        int dev = 0, cus = 256;
-        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        return 2 * (int64_t)std::max(cus, 1);
+    static const int64_t cus = [] {
+        int dev = 0, n = 256;
+        if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
+        return (int64_t)std::max(n, 1);
     }();
-    const int64_t base = std::min<int64_t>(std::max<int64_t>(1, (1536 + per_split - 1) / per_split), kbt);
+    // resident workgroups per CU: three for the one-group LDS kernel, two otherwise
+    const int64_t slots = (tl.G == 1 && L != 4 ? 3 : 2) * cus;
+    // int32 accumulators: |x~ d1 + o d2| <= 2*128 + 128 per SNP, so a split holds at most
+    // (2^31 - 1) / 384 SNPs (21845 blocks of 256)
+    const int64_t kbps_max = ((int64_t)INT32_MAX / 384) / 256;
+    const int64_t base = std::min<int64_t>(
+        std::max<int64_t>(std::max<int64_t>(1, (1536 + per_split - 1) / per_split), (kbt + kbps_max - 1) / kbps_max),
+        kbt);
     int64_t nsplit = base, kbps = (kbt + base - 1) / base;
     double best = -1.0;
     for (int64_t cand = base; cand <= std::min<int64_t>(kbt, 2 * base); ++cand) {
