@@ -58,6 +58,11 @@ struct GenoArgs {
     int64_t rows;              // real rows (epilogue guard)
     int64_t kb_per_split, kb_total;
     int G;                     // groups of 16 right-hand sides (= gridDim.z)
+    // flat_tiles > 0: a 1-D grid in XCD-aware order (one group per workgroup).  Workgroup b runs
+    // on XCD b % 8; its tile T = 8 (s / G) + b % 8 and group s % G (s = b / 8), so the G groups
+    // of one (row block, split) tile run back to back on one XCD and share the A tiles in its
+    // L2.  T = y * tiles_x + x; workgroups with T >= flat_tiles exit at once.
+    int64_t flat_tiles, tiles_x;
     const int8_t* Bx;
     const int8_t* Bo;          // EPI_KV_G with two operands: the digits of -(mu W)
     int64_t k;                 // real number of right-hand sides
@@ -81,9 +86,9 @@ struct GenoArgs {
 // MT x GZ: a wave covers MT 16-row subtiles and GZ groups of 16 right-hand sides; every decoded
 // genotype fragment feeds GZ L (x2) MFMAs.  Grid: x = padded rows / (64 MT), y = k splits,
 // z = ceil(G / GZ).
-// LDSB (one group of 16 right-hand sides only): the workgroup's four waves share the B
-// fragments of each 256-k block through LDS (double-buffered, staged one block ahead), so a
-// workgroup reads each B byte once instead of once per wave.
+// LDSB (one group of 16 right-hand sides per workgroup, GZ = 1): the workgroup's four waves
+// share the B fragments of each 256-k block through LDS (double-buffered, staged one block
+// ahead), so a workgroup reads each B byte once instead of once per wave.
 template <int L, bool TWO_B>
 struct GenoLds {
     static constexpr int kChunksPerOp = 4 * L * 64;                     // 16-byte chunks per k block
@@ -101,14 +106,25 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
     // Z W needs only X~ W1 - O (mu W): with the second operand stored negated, both products
     // accumulate into one set of int32 registers (exact while |sum| < 2^31: 384 per k-element).
     constexpr bool FUSE = TWO_B && EPI == EPI_KV_G && !ONLY_X;
+    int64_t bx = blockIdx.x, by = blockIdx.y;
+    int bz = blockIdx.z;
+    if (a.flat_tiles > 0) {
+        const int64_t b = blockIdx.x, s = b >> 3;
+        const int64_t q = s / a.G;
+        bz = (int)(s - q * a.G);
+        const int64_t T = q * 8 + (b & 7);
+        if (T >= a.flat_tiles) return;   // whole workgroup, before any barrier
+        by = T / a.tiles_x;
+        bx = T - by * a.tiles_x;
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
-    const int64_t rbase = ((int64_t)blockIdx.x * 4 + wave) * (16 * MT);   // first row of the wave
+    const int64_t rbase = (bx * 4 + wave) * (16 * MT);   // first row of the wave
     const int64_t tr = rbase >> 6;
     const int m0 = (int)((rbase & 63) >> 4);
-    const int gz0 = blockIdx.z * GZ;
+    const int gz0 = bz * GZ;
     const int NT = a.G * L;
-    const int64_t kb0 = (int64_t)blockIdx.y * a.kb_per_split;
+    const int64_t kb0 = by * a.kb_per_split;
     int64_t kb1 = kb0 + a.kb_per_split;
     if (kb1 > a.kb_total) kb1 = a.kb_total;
 
@@ -140,8 +156,9 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
             for (int i = 0; i < LD::kPerThread; ++i) {
                 const int c = (int)threadIdx.x + 256 * i;
                 const int op = c / LD::kChunksPerOp, w = c - op * LD::kChunksPerOp;
+                const int t = w / (L * 64), rem = w - t * (L * 64);   // step t, limb tiles of group gz0
                 const ST* src = reinterpret_cast<const ST*>(op == 0 ? a.Bx : a.Bo);
-                stg[i] = src[kb * LD::kChunksPerOp + w];
+                stg[i] = src[((kb * 4 + t) * NT + gz0 * L) * 64 + rem];
             }
         };
         auto stage_store = [&](int buf) {

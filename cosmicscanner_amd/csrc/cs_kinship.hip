@@ -324,8 +324,9 @@ __global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, i
 }
 
 // ------------------------------------------------------------------------------------------
-// grid as the callers give it: x = padded rows / 256, y = splits, z = G.  With two or more
-// column groups each wave takes 32 rows and two groups (half the genotype passes and decodes).
+// grid as the callers give it: x = padded rows / 256, y = splits, z = G.  The LDS kernels take
+// one group per workgroup (with two or more groups on a 1-D XCD-aware grid); without LDS
+// sharing (CSAMD_GENO_LDSB=0) two or more groups run with each wave on 32 rows and two groups.
 template <int L, bool TWO_B, int EPI>
 static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a) {
     if constexpr (L == 4) {
@@ -334,17 +335,29 @@ static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a) {
         if (TWO_B) k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
         else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
     } else {
+        // B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
+        static const bool ldsb = [] {
+            const char* e = std::getenv("CSAMD_GENO_LDSB");
+            return !(e && e[0] == '0');
+        }();
+        if (a.G >= 2 && ldsb) {
+            // one group per workgroup, 1-D grid in XCD-aware order: the groups of one tile run
+            // together on one XCD and read its A tiles from HBM about once
+            GenoArgs f = a;
+            f.tiles_x = grid.x;
+            f.flat_tiles = (int64_t)grid.x * grid.y;
+            const int64_t blocks = (f.flat_tiles + 7) / 8 * 8 * a.G;
+            if (blocks > 0x7fffffff) throw RuntimeError("launch_geno: grid too large");
+            if (TWO_B) k_geno2s<L, EPI><<<dim3((unsigned)blocks), 256, 0, st>>>(f);
+            else k_geno1s<L, EPI><<<dim3((unsigned)blocks), 256, 0, st>>>(f);
+            return;
+        }
         if (a.G >= 2) {
             const dim3 g2(grid.x * 2, grid.y, (unsigned)((a.G + 1) / 2));
             if (TWO_B) k_geno2<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
             else k_geno1<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
             return;
         }
-        // one group: B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
-        static const bool ldsb = [] {
-            const char* e = std::getenv("CSAMD_GENO_LDSB");
-            return !(e && e[0] == '0');
-        }();
         if (TWO_B) {
             if (ldsb) k_geno2s<L, EPI><<<grid, 256, 0, st>>>(a);
             else k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
@@ -517,16 +530,15 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     const int64_t rblocks = b->Npad / 256;
     // SNP splits: at least ~1536 workgroups, and a count that fills the last round of the
     // device's resident workgroups as fully as possible.
-    // launch_one's z and x factor
-    const int64_t gz = L == 4 ? tl.G : (tl.G >= 2 ? 2 * ((tl.G + 1) / 2) : 1);
-    const int64_t per_split = rblocks * gz;
+    // workgroups per split: one per (row block, group) in launch_one
+    const int64_t per_split = rblocks * tl.G;
     static const int64_t cus = [] {
         int dev = 0, n = 256;
         if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev);
         return (int64_t)std::max(n, 1);
     }();
-    // resident workgroups per CU: three for the one-group LDS kernel, two otherwise
-    const int64_t slots = (tl.G == 1 && L != 4 ? 3 : 2) * cus;
+    // resident workgroups per CU: three for the LDS kernel, two for the four-limb one
+    const int64_t slots = (L != 4 ? 3 : 2) * cus;
     // int32 accumulators: |x~ d1 + o d2| <= 2*128 + 128 per SNP, so a split holds at most
     // (2^31 - 1) / 384 SNPs (21845 blocks of 256)
     const int64_t kbps_max = ((int64_t)INT32_MAX / 384) / 256;
