@@ -329,7 +329,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     geno_body<L, true, EPI, false, MT, GZ>(a);
 }
 // One group of 16 right-hand sides with the B fragments shared through LDS (GenoLds).  For Z W
-// the two products share one accumulator set (FUSE), which leaves room for 3 waves per SIMD.
+// the two products share one accumulator set (FUSE), which leaves room for 3 waves per SIMD
+// (two with four limbs).
 template <int L, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_geno2s(GenoArgs a) {
     __shared__ v4i ldsb[2 * GenoLds<L, true>::kChunks];
@@ -337,6 +338,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 }
 template <int L, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_geno1s(GenoArgs a) {
+    __shared__ uint4 ldsb[2 * GenoLds<L, false>::kChunks];
+    geno_body<L, false, EPI, false, 4, 1, true>(a, ldsb);
+}
+// Four limbs (two accumulator sets of 4 x 4 tiles) fit only at two waves per SIMD.
+template <int L, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_geno1s2(GenoArgs a) {
     __shared__ uint4 ldsb[2 * GenoLds<L, false>::kChunks];
     geno_body<L, false, EPI, false, 4, 1, true>(a, ldsb);
 }

@@ -329,42 +329,39 @@ __global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, i
 // sharing (CSAMD_GENO_LDSB=0) two or more groups run with each wave on 32 rows and two groups.
 template <int L, bool TWO_B, int EPI>
 static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a) {
-    if constexpr (L == 4) {
-        // four limbs: the LDS-shared and two-group forms exceed the register file (they spill),
-        // so every group count takes the four-tile form, one group per z slice
-        if (TWO_B) k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
-        else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
-    } else {
-        // B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
-        static const bool ldsb = [] {
-            const char* e = std::getenv("CSAMD_GENO_LDSB");
-            return !(e && e[0] == '0');
-        }();
-        if (a.G >= 2 && ldsb) {
+    // B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
+    static const bool ldsb = [] {
+        const char* e = std::getenv("CSAMD_GENO_LDSB");
+        return !(e && e[0] == '0');
+    }();
+    if (ldsb) {
+        GenoArgs f = a;
+        dim3 g = grid;
+        if (a.G >= 2) {
             // one group per workgroup, 1-D grid in XCD-aware order: the groups of one tile run
             // together on one XCD and read its A tiles from HBM about once
-            GenoArgs f = a;
             f.tiles_x = grid.x;
             f.flat_tiles = (int64_t)grid.x * grid.y;
             const int64_t blocks = (f.flat_tiles + 7) / 8 * 8 * a.G;
             if (blocks > 0x7fffffff) throw RuntimeError("launch_geno: grid too large");
-            if (TWO_B) k_geno2s<L, EPI><<<dim3((unsigned)blocks), 256, 0, st>>>(f);
-            else k_geno1s<L, EPI><<<dim3((unsigned)blocks), 256, 0, st>>>(f);
-            return;
+            g = dim3((unsigned)blocks);
         }
-        if (a.G >= 2) {
-            const dim3 g2(grid.x * 2, grid.y, (unsigned)((a.G + 1) / 2));
-            if (TWO_B) k_geno2<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
-            else k_geno1<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
-            return;
-        }
-        if (TWO_B) {
-            if (ldsb) k_geno2s<L, EPI><<<grid, 256, 0, st>>>(a);
-            else k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
-        } else {
-            if (ldsb) k_geno1s<L, EPI><<<grid, 256, 0, st>>>(a);
-            else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
-        }
+        if constexpr (TWO_B) k_geno2s<L, EPI><<<g, 256, 0, st>>>(f);
+        else if constexpr (L == 4) k_geno1s2<L, EPI><<<g, 256, 0, st>>>(f);
+        else k_geno1s<L, EPI><<<g, 256, 0, st>>>(f);
+        return;
+    }
+    if constexpr (L == 4) {
+        // four limbs: the two-group form exceeds the register file, so one group per z slice
+        if (TWO_B) k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
+        else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
+    } else if (a.G >= 2) {
+        const dim3 g2(grid.x * 2, grid.y, (unsigned)((a.G + 1) / 2));
+        if (TWO_B) k_geno2<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
+        else k_geno1<L, EPI, 2, 2><<<g2, 256, 0, st>>>(a);
+    } else {
+        if (TWO_B) k_geno2<L, EPI><<<grid, 256, 0, st>>>(a);
+        else k_geno1<L, EPI><<<grid, 256, 0, st>>>(a);
     }
 }
 
