@@ -139,14 +139,11 @@ CHROM_MB = [248.96, 242.19, 198.30, 190.22, 181.54, 170.81, 159.35, 145.14, 138.
             133.28, 114.36, 107.04, 101.99, 90.34, 83.26, 80.37, 58.62, 64.44, 46.71, 50.82]
 
 
-def loco_leg(csa, n=200_000, m=600_000, T=16, seed=20261205):
-    """LOCO matrix-free PCG + LMM scan (the C3 workload): m SNPs over 22 chromosomes in proportion
-    to their lengths, 20 causal SNPs per chromosome, h2 = 0.4, T phenotype replicates with
-    independent effects and noise; per chromosome one kinship operator without that chromosome,
-    one multi-RHS PCG solve (T traits + 3 covariates + 16 calibration SNPs, relative residual
-    1e-6) and the fused scan of that chromosome's SNPs.  genotypes/s = n m / total time."""
+def c3_setup(csa, n, m, T, seed):
+    """The C3 genotypes and phenotypes: m SNPs over 22 chromosomes in proportion to their
+    lengths, 20 causal SNPs per chromosome, h2 = 0.4, T phenotype replicates with independent
+    effects and noise; X = [1, 2 N(0,1)]."""
     rng = np.random.default_rng(seed)
-    t0 = time.perf_counter()
     reader = csa.PlinkReader.synthetic(seed, n, m, 0.01, 0.5, 0.005)
     bounds = np.concatenate([[0], np.round(np.cumsum(CHROM_MB) / np.sum(CHROM_MB) * m).astype(np.int64)])
     bounds[-1] = m
@@ -166,6 +163,16 @@ def loco_leg(csa, n=200_000, m=600_000, T=16, seed=20261205):
     means, inv_stds = reader.getMeans(), reader.getInvStds()
     f = means / 2.0
     grm_ok = (inv_stds != 0) & (np.minimum(f, 1 - f) >= 0.01)
+    return rng, reader, bounds, X, Y, grm_ok
+
+
+def loco_leg(csa, n=200_000, m=600_000, T=16, seed=20261205):
+    """LOCO matrix-free PCG + LMM scan (the C3 workload, c3_setup): per chromosome one kinship
+    operator without that chromosome, one multi-RHS PCG solve (T traits + 3 covariates + 16
+    calibration SNPs, relative residual 1e-6) and the fused scan of that chromosome's SNPs.
+    genotypes/s = n m / total time."""
+    t0 = time.perf_counter()
+    rng, reader, bounds, X, Y, grm_ok = c3_setup(csa, n, m, T, seed)
     setup = time.perf_counter() - t0
     csa.synchronize()
     t0 = time.perf_counter()
@@ -194,6 +201,45 @@ def loco_leg(csa, n=200_000, m=600_000, T=16, seed=20261205):
                n_tested=tested)
     log(f"LOCO leg: {total:.1f} s for 22 chromosomes, {res['genotypes_per_s']:.3e} genotypes/s, "
         f"PCG iterations {min(iters)}-{max(iters)}")
+    return res
+
+
+def lowrank_leg(csa, n=200_000, m=600_000, seed=20261205, rank=256, oversample=10):
+    """The reference's live matrix-free GWAS path (--fastlmm-matrix-free, low-rank mode) on the C3
+    genotypes (c3_setup, first phenotype replicate): per LOCO chromosome a kinship operator without
+    that chromosome, the randomized low-rank null model (rank 256 + 10 oversampling, two power
+    iterations, REML) and scanLowRank of that chromosome's SNPs.  genotypes/s = n m / total."""
+    t0 = time.perf_counter()
+    _, reader, bounds, X, Y, grm_ok = c3_setup(csa, n, m, 16, seed)
+    y = np.ascontiguousarray(Y[:, 0])
+    setup = time.perf_counter() - t0
+    csa.synchronize()
+    t0 = time.perf_counter()
+    t_null, ranks, h2, tested = 0.0, [], [], 0
+    for c in range(22):
+        lo, hi = bounds[c], bounds[c + 1]
+        idx = np.arange(m)
+        loco = idx[grm_ok & ((idx < lo) | (idx >= hi))].astype(np.int32)
+        t1 = time.perf_counter()
+        op = csa.PlinkKinshipOperator(reader, None, loco)
+        lr = csa.LowRankLmm(op, y, X, rank=rank, oversample=oversample)
+        t_null += time.perf_counter() - t1
+        out = lr.scan(reader, None, np.arange(lo, hi, dtype=np.int32))
+        d = lr.diag
+        ranks.append(int(d["effective_rank"]))
+        h2.append(d["sigma2_g"] / (d["sigma2_g"] + d["sigma2_e"]))
+        tested += int(np.sum(out["passes"]))
+        lr.close()
+        op.close()
+    csa.synchronize()
+    total = time.perf_counter() - t0
+    reader.close()
+    res = dict(workload=f"n={n} x m={m}, 22 LOCO chromosomes, 1 trait, low-rank rank {rank} + {oversample}",
+               setup_s=setup, total_s=total, null_model_s=t_null, genotypes_per_s=n * m / total,
+               effective_rank=[min(ranks), max(ranks)], h2_range=[float(np.min(h2)), float(np.max(h2))],
+               n_tested=tested)
+    log(f"low-rank leg: {total:.1f} s for 22 chromosomes (null models {t_null:.1f} s), "
+        f"{res['genotypes_per_s']:.3e} genotypes/s, h2 {res['h2_range'][0]:.3f}-{res['h2_range'][1]:.3f}")
     return res
 
 
@@ -348,6 +394,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--mv", action="store_true", help="also run the multi-trait LMM leg (C5 workload)")
     ap.add_argument("--loco", action="store_true", help="also run the LOCO PCG + scan leg (C3 workload)")
+    ap.add_argument("--lowrank", action="store_true",
+                    help="also run the low-rank matrix-free GWAS leg (the reference path, C3 genotypes)")
     # the dense eigen path (MvEigenVce::prepare) is bounded by rocSOLVER's 32-bit interfaces at
     # n <= 32767 individuals; the C5 workload names 50,000
     ap.add_argument("--mv-n", type=int, default=32_000)
@@ -462,6 +510,11 @@ def main():
         op.close()
         locores = loco_leg(csa)
 
+    lowres = None
+    if args.lowrank:
+        op.close()
+        lowres = lowrank_leg(csa)
+
     mvres = None
     if args.mv:
         op.close()
@@ -497,6 +550,7 @@ def main():
         "serving": serving,
         "multivariate": mvres,
         "loco_scan": locores,
+        "lowrank_scan": lowres,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

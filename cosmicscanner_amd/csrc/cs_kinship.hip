@@ -292,6 +292,27 @@ __global__ void k_diag_table(int64_t M, const double* __restrict__ means,
     }
 }
 
+// Per-SNP trace terms mult_j s_j^2 sum_c n_c (x_c - mu_j)^2 from class counts [M][4].
+__global__ void k_trace_terms(const int64_t* __restrict__ counts, int64_t M, const double* __restrict__ means,
+                              const double* __restrict__ inv_stds, const int32_t* __restrict__ mult,
+                              double* __restrict__ t) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    const double s = inv_stds[j], mu = means[j];
+    if (s == 0.0 || mult[j] == 0) {
+        t[j] = 0.0;
+        return;
+    }
+    const double x[4] = {2.0, 0.0, 1.0, 0.0};
+    double acc = 0.0;
+    for (int c = 0; c < 4; ++c) {
+        if (c == 1) continue;   // missing
+        const double z = (x[c] - mu) * s;
+        acc += (double)counts[j * 4 + c] * (z * z);
+    }
+    t[j] = (double)mult[j] * acc;
+}
+
 __global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, int64_t KB,
                                               int64_t N, int64_t M, const double* __restrict__ T,
                                               double inv_M, double* __restrict__ diag) {
@@ -809,14 +830,30 @@ int cs_kinship_diag(cs_kinship* op, double* diag) {
     });
 }
 
+// trace(K) as the reference sums it (kinship_operator.cpp:306-343): per selected SNP the sum of
+// z^2 over the kept samples, then over the SNPs.  The per-SNP sum comes from the class counts of
+// the kept samples (keep multiplicities as weights): sum_c n_c(j) (x_c - mu_j)^2 s_j^2.
 int cs_kinship_trace(cs_kinship* op, double* trace) {
     return guarded([&] {
         if (!op || !trace) throw InvalidArgument("cs_kinship_trace: null argument");
-        std::vector<double> d((size_t)op->n);
-        if (cs_kinship_diag(op, d.data()) != CS_OK) throw RuntimeError(cs_last_error());
-        double t = 0.0;
-        for (double v : d) t += v;
-        *trace = t;
+        cs_bed* b = op->bed;
+        hipStream_t st = op->stream;
+        std::vector<int32_t> w((size_t)b->N, 0);
+        if (op->keep.empty()) std::fill(w.begin(), w.end(), 1);
+        else
+            for (int32_t s : op->keep) w[(size_t)s] += 1;
+        DevBuf counts((size_t)b->M * 4 * sizeof(int64_t)), terms((size_t)b->M * sizeof(double));
+        weighted_counts(b, w.data(), (int64_t)w.size(), counts.as<int64_t>(), st);
+        k_trace_terms<<<grid_for(b->M), 256, 0, st>>>(counts.as<int64_t>(), b->M, b->means.as<double>(),
+                                                     b->inv_stds.as<double>(), op->d_mult.as<int32_t>(),
+                                                     terms.as<double>());
+        CS_HIP(hipGetLastError());
+        std::vector<double> t((size_t)b->M);
+        CS_HIP(hipMemcpyAsync(t.data(), terms.p, (size_t)b->M * sizeof(double), hipMemcpyDeviceToHost, st));
+        CS_HIP(hipStreamSynchronize(st));
+        double total = 0.0;
+        for (double v : t) total += v;
+        *trace = total / (double)op->Msel;
     });
 }
 

@@ -10,7 +10,8 @@
 //   `rank` eigenpairs U = Q v (descending), trace-fraction rank selection, residual eigenvalue
 //   r0 = (tr K - sum lambda) / (n - rank), REML over log eta on an 81-point grid on [-12, 12] then
 //   golden section (80 iterations, 1e-8), sigma_e = qf / df, sigma_g = eta sigma_e.
-//   K.V on the GPU (int8 MFMA operator), QR / eigen / GEMMs with rocSOLVER / rocBLAS.  The REML
+//   K.V on the GPU (int8 MFMA operator), CholeskyQR2 (Householder when ill-conditioned), eigen
+//   and GEMMs with rocSOLVER / rocBLAS.  The REML
 //   objective is evaluated from the small projections U^T X, U^T y, X^T X, X^T y, y^T y (exact
 //   algebra of the reference's R^-1 = base I + U diag(1/(1+eta lambda) - base) U^T).
 // LmmEngine::scanLowRank (src/lmm_engine.cpp:1269-1361): V^-1 = base I + U diag(corr) U^T with
@@ -26,6 +27,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <vector>
 
@@ -113,10 +115,53 @@ void kv(cs_kinship* K, const double* dX, int64_t n, int64_t k, double* dY) {
     if (rc != CS_OK) throw RuntimeError(cs_last_error());
 }
 
-// Thin Q of an n x s matrix in place (Householder QR).
-void thin_q(cs_lowrank* h, double* A, int64_t n, int64_t s, DevBuf& tau) {
+// Thin Q of an n x s matrix in place.  CholeskyQR2: twice G = A^T A (syrk), G = L L^T (potrf),
+// A <- A L^-T (trsm) -- GEMM-shaped work -- unless the Cholesky fails or its diagonal spans more
+// than 1e6 (cond(A) above ~1e6, where A^T A loses too much), then Householder QR (geqrf + orgqr)
+// of the current A.  Every orthonormal basis of the sample spans the same subspace, so the
+// eigenpairs of Q^T K Q and U = Q V do not depend on the choice.
+void householder_q(cs_lowrank* h, double* A, int64_t n, int64_t s, DevBuf& tau) {
     CS_BLAS3(rocsolver_dgeqrf(h->blas, (int)n, (int)s, A, (int)n, tau.as<double>()));
     CS_BLAS3(rocsolver_dorgqr(h->blas, (int)n, (int)s, (int)s, A, (int)n, tau.as<double>()));
+}
+
+void thin_q(cs_lowrank* h, double* A, int64_t n, int64_t s, DevBuf& tau) {
+    const double one = 1.0, zero = 0.0;
+    DevBuf G((size_t)(s * s) * 8), info(sizeof(rocblas_int));
+    std::vector<double> dg((size_t)s);
+    for (int pass = 0; pass < 2; ++pass) {
+        CS_BLAS3(rocblas_dsyrk(h->blas, rocblas_fill_lower, rocblas_operation_transpose, (int)s, (int)n, &one, A,
+                               (int)n, &zero, G.as<double>(), (int)s));
+        CS_BLAS3(rocsolver_dpotrf(h->blas, rocblas_fill_lower, (int)s, G.as<double>(), (int)s,
+                                  info.as<rocblas_int>()));
+        rocblas_int inf = 0;
+        CS_HIP(hipMemcpyAsync(&inf, info.p, sizeof(inf), hipMemcpyDeviceToHost, h->st));
+        CS_HIP(hipMemcpy2DAsync(dg.data(), 8, G.p, (size_t)(s + 1) * 8, 8, (size_t)s, hipMemcpyDeviceToHost, h->st));
+        CS_HIP(hipStreamSynchronize(h->st));
+        double lo = std::numeric_limits<double>::infinity(), hi = 0.0;
+        for (double v : dg) {
+            lo = std::min(lo, std::fabs(v));
+            hi = std::max(hi, std::fabs(v));
+        }
+        if (inf != 0 || !(lo > 0.0) || !(hi <= 1e6 * lo)) {
+            householder_q(h, A, n, s, tau);
+            return;
+        }
+        CS_BLAS3(rocblas_dtrsm(h->blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                               rocblas_diagonal_non_unit, (int)n, (int)s, &one, G.as<double>(), (int)s, A, (int)n));
+    }
+}
+
+// Omega of the last prepare (host copy): LOCO runs ask for the same seed and size once per
+// chromosome, and drawing n s normals serially takes ~1.6 s at n = 200k, s = 266.
+struct OmegaCache {
+    std::mutex mu;
+    int64_t seed = 0, n = -1, s = -1;
+    std::vector<double> v;
+};
+OmegaCache& omega_cache() {
+    static OmegaCache c;
+    return c;
 }
 
 // REML objective of the residual-augmented low-rank model at log(eta) (lmm_engine.cpp:1001-1023),
@@ -190,18 +235,24 @@ int cs_lowrank_prepare(cs_kinship* K, const double* y, const float* X, int64_t l
         const int64_t s = std::min<int64_t>(n, rank + std::max(0, oversample));
         h->sketch = s;
         // Omega, column by column, from the standard generators (as the reference fills it)
-        std::vector<double> omega((size_t)(n * s));
-        {
-            std::mt19937_64 gen((uint64_t)seed);
-            std::normal_distribution<double> normal(0.0, 1.0);
-            for (int64_t j = 0; j < s; ++j)
-                for (int64_t i = 0; i < n; ++i) omega[(size_t)(j * n + i)] = normal(gen);
-        }
         DevBuf A((size_t)(n * s) * 8), Yb((size_t)(n * s) * 8), tau((size_t)s * 8);
-        CS_HIP(hipMemcpyAsync(A.p, omega.data(), omega.size() * 8, hipMemcpyHostToDevice, h->st));
-        CS_HIP(hipStreamSynchronize(h->st));
-        omega.clear();
-        omega.shrink_to_fit();
+        {
+            OmegaCache& oc = omega_cache();
+            std::lock_guard<std::mutex> lock(oc.mu);
+            if (oc.seed != seed || oc.n != n || oc.s != s) {
+                oc.n = -1;
+                oc.v.assign((size_t)(n * s), 0.0);
+                std::mt19937_64 gen((uint64_t)seed);
+                std::normal_distribution<double> normal(0.0, 1.0);
+                for (int64_t j = 0; j < s; ++j)
+                    for (int64_t i = 0; i < n; ++i) oc.v[(size_t)(j * n + i)] = normal(gen);
+                oc.seed = seed;
+                oc.n = n;
+                oc.s = s;
+            }
+            CS_HIP(hipMemcpyAsync(A.p, oc.v.data(), oc.v.size() * 8, hipMemcpyHostToDevice, h->st));
+            CS_HIP(hipStreamSynchronize(h->st));
+        }
         kv(K, A.as<double>(), n, s, Yb.as<double>());            // sample = K Omega
         for (int it = 0; it < 2; ++it) {
             thin_q(h.get(), Yb.as<double>(), n, s, tau);

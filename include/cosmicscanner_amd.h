@@ -255,7 +255,8 @@ int cs_pcg_projected_scan(cs_kinship* K, cs_bed* bed, const int32_t* keep, int64
  * log eta in [-12, 12] + golden section).  When the threshold is not met and no_fallback == 0 the
  * full rank is kept -- GwasScanner's fallback to MatrixFreeVce (src/matrix_free_vce.cpp:32-235),
  * the same computation with the threshold off.  y: n phenotypes over the operator's rows;
- * X: n x c float32 (intercept included). */
+ * X: n x c float32 (intercept included).  Omega of the last call (n x sketch doubles) is kept in
+ * host memory for the next call with the same seed and size (one call per LOCO chromosome). */
 typedef struct cs_lowrank_diag {
     int requested_rank, effective_rank, sketch_size, total_rank;
     double residual_eigenvalue, trace, trace_fraction, relative_frobenius_error;
