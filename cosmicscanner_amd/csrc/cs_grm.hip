@@ -68,62 +68,67 @@ __device__ __forceinline__ v4i grm_operand(uint32_t w, uint4 dig) {
 }
 
 // One 128 x 128 lower-triangle block per workgroup; wave (wi, wk) owns the 64 x 64 quadrant
-// rows i0 = 128 bi + 64 wi, columns k0 = 128 bk + 64 wk, streaming all SNP blocks of both row
-// tiles (4 x 4 MFMA tiles per step).
+// rows i0 = 128 bi + 64 wi, columns k0 = 128 bk + 64 wk (4 x 4 MFMA tiles per step).  The block
+// needs four 64-row operand tiles (A rows of bi, B rows of bk); wave w decodes tile w once per
+// 64-k step into LDS (double-buffered, one barrier per step) and every wave reads its A and B
+// operands from there, so each tile is loaded and decoded once instead of twice.  Quadrants
+// above the diagonal still decode their tile but skip the MFMAs and the epilogue.
 template <int AM, int BM, int EPI>
-__global__ __launch_bounds__(256) void k_grm_pass(GrmArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_grm_pass(GrmArgs a) {
+    __shared__ v4i lds[2][4][4][64];   // [buffer][tile][m-subtile][lane], 32 KiB
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
     const int2 blk = a.blocks[blockIdx.x];
-    const int64_t i0 = (int64_t)blk.x * 128 + 64 * (wave >> 1);
-    const int64_t k0 = (int64_t)blk.y * 128 + 64 * (wave & 1);
-    if (k0 >= i0 + 64) return;   // quadrant strictly above the diagonal
+    const int wi = wave >> 1, wk = wave & 1;
+    const int64_t i0 = (int64_t)blk.x * 128 + 64 * wi;
+    const int64_t k0 = (int64_t)blk.y * 128 + 64 * wk;
+    const bool active = k0 < i0 + 64;   // quadrant on or below the diagonal
+    const bool sideA = wave < 2;
+    const int64_t trow = sideA ? (int64_t)blk.x * 128 + 64 * wave : (int64_t)blk.y * 128 + 64 * (wave - 2);
+    const uint8_t* tbase = a.smp + (((trow >> 6) * a.KB) << 12) + lane * 16;
+    const uint8_t* dig = sideA ? a.digA : a.digB;
+    const bool use_dig = sideA ? AM == OP_MD : BM == OP_MD;
     v4i acc[4][4];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int n = 0; n < 4; ++n) acc[m][n] = v4i{0, 0, 0, 0};
-    const uint8_t* abase = a.smp + (((i0 >> 6) * a.KB) << 12) + lane * 16;
-    const uint8_t* bbase = a.smp + (((k0 >> 6) * a.KB) << 12) + lane * 16;
-    uint4 av[4], bv[4], an[4], bn[4];
+    uint4 cv[4], cn[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        av[m] = *reinterpret_cast<const uint4*>(abase + (a.kb0 << 12) + m * 1024);
-        bv[m] = *reinterpret_cast<const uint4*>(bbase + (a.kb0 << 12) + m * 1024);
-    }
+    for (int m = 0; m < 4; ++m) cv[m] = *reinterpret_cast<const uint4*>(tbase + (a.kb0 << 12) + m * 1024);
+    int buf = 0;
     for (int64_t kb = a.kb0; kb < a.kb1; ++kb) {
         const int64_t kn = kb + 1 < a.kb1 ? kb + 1 : kb;
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            an[m] = *reinterpret_cast<const uint4*>(abase + (kn << 12) + m * 1024);
-            bn[m] = *reinterpret_cast<const uint4*>(bbase + (kn << 12) + m * 1024);
-        }
+        for (int m = 0; m < 4; ++m) cn[m] = *reinterpret_cast<const uint4*>(tbase + (kn << 12) + m * 1024);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            uint4 dA = uint4{0, 0, 0, 0}, dB = uint4{0, 0, 0, 0};
-            const int64_t doff = (((kb * 4 + t) * 4 + g) << 4);
-            if (AM == OP_MD) dA = *reinterpret_cast<const uint4*>(a.digA + doff);
-            if (BM == OP_MD) dB = *reinterpret_cast<const uint4*>(a.digB + doff);
-            v4i A[4], B[4];
+            uint4 d = uint4{0, 0, 0, 0};
+            if (use_dig) d = *reinterpret_cast<const uint4*>(dig + (((kb * 4 + t) * 4 + g) << 4));
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-                const uint32_t wa = t == 0 ? av[m].x : t == 1 ? av[m].y : t == 2 ? av[m].z : av[m].w;
-                const uint32_t wb = t == 0 ? bv[m].x : t == 1 ? bv[m].y : t == 2 ? bv[m].z : bv[m].w;
-                A[m] = grm_operand<AM>(wa, dA);
-                B[m] = grm_operand<BM>(wb, dB);
+                const uint32_t w = t == 0 ? cv[m].x : t == 1 ? cv[m].y : t == 2 ? cv[m].z : cv[m].w;
+                lds[buf][wave][m][lane] = sideA ? grm_operand<AM>(w, d) : grm_operand<BM>(w, d);
             }
+            __syncthreads();
+            if (active) {
+                v4i B[4];
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
+                for (int n = 0; n < 4; ++n) B[n] = lds[buf][2 + wk][n][lane];
 #pragma unroll
-                for (int n = 0; n < 4; ++n)
-                    acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[m], B[n], acc[m][n], 0, 0, 0);
+                for (int m = 0; m < 4; ++m) {
+                    const v4i A = lds[buf][wi][m][lane];
+#pragma unroll
+                    for (int n = 0; n < 4; ++n)
+                        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[n], acc[m][n], 0, 0, 0);
+                }
+            }
+            buf ^= 1;
         }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            av[m] = an[m];
-            bv[m] = bn[m];
-        }
+        for (int m = 0; m < 4; ++m) cv[m] = cn[m];
     }
+    if (!active) return;
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
