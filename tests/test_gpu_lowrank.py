@@ -53,3 +53,33 @@ def test_lowrank_vs_oracle(csa, rank, threshold, no_fallback):
     t = ~np.isnan(lo)
     assert np.max(np.abs(lp[t] - lo[t])) <= 1e-3
     assert np.max(np.abs(out["beta"][t] - st["beta"][snps][t]) / st["se"][snps][t]) <= 1e-3
+
+
+def test_lowrank_rank_deficient_sketch(csa):
+    """Fewer selected SNPs (30) than sketch columns (40 + 10): K Omega is rank-deficient, so the
+    sketch orthonormalisation meets a singular (or 1e6-ill-conditioned) Gram matrix.  The leading
+    30 eigenvalues still match the oracle's and the rest vanish."""
+    N, M = 460, 2600
+    body = oracle.gen_hwe_bed(31, N, M, 0.05, 0.5, 0.01)
+    rng = np.random.default_rng(6)
+    keep = np.sort(rng.choice(N, 420, replace=False)).astype(np.int32)
+    variants = np.arange(100, 130, dtype=np.int32)
+    n = keep.size
+    X = np.column_stack([np.ones(n), rng.standard_normal((n, 2))]).astype(np.float32)
+    K = oracle.dense_kinship(body, N, M, keep, variants)
+    y = 0.3 * X[:, 1] + rng.standard_normal(n)
+    r = csa.PlinkReader.from_body(body, N, M)
+    op = csa.PlinkKinshipOperator(r, keep, variants)
+    lr = csa.LowRankLmm(op, y, X, rank=40, oversample=10, seed=42, trace_threshold=0.0)
+    m = oracle.lowrank_prepare(K, y, X, 40, 10, 42, 0.0, 1, False)
+    got = lr.get()
+    top = np.linalg.eigvalsh(K)[::-1][:30]
+    print(f"rank {got['rank']}: lam[29] {got['lam'][29]:.3e}, lam[30] {got['lam'][30]:.3e}, "
+          f"oracle lam[30] {m['lam'][30]:.3e}", flush=True)
+    assert got["rank"] == 40
+    assert np.max(np.abs(got["lam"][:30] - m["lam"][:30])) <= 1e-5 * m["lam"][0]
+    assert np.max(np.abs(got["lam"][:30] - top)) <= 1e-5 * top[0]
+    assert np.max(np.abs(got["lam"][30:])) <= 1e-5 * top[0]
+    assert np.isfinite(got["sg"]) and np.isfinite(got["se"])
+    lr.close()
+    op.close()
