@@ -96,12 +96,16 @@ struct GenoLds {
     static constexpr int kPerThread = kChunks / 256;
 };
 
-template <int L, bool TWO_B, int EPI, bool ONLY_X = false, int MT = 4, int GZ = 1, bool LDSB = false>
+template <int L, bool TWO_B, int EPI, bool ONLY_X = false, int MT = 4, int GZ = 1, bool LDSB = false, bool PK = false>
 __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nullptr) {
-    // B staging type: native vectors for the two-operand form (a HIP_vector_type array there is
-    // copied through a stack object), the HIP vector type for the one-operand form (fewer
-    // registers at three waves per SIMD).
-    using ST = typename std::conditional<TWO_B, v4i, uint4>::type;
+    // PK: the L limbs of up to 16 / L columns share one 16-column tile (column c, limb l in tile
+    // column c L + l), so each decoded fragment feeds one MFMA per operand instead of L.
+    constexpr int LT = PK ? 1 : L;   // limb tiles per group
+    static_assert(!PK || GZ == 1, "packed limbs use one group");
+    // B staging type: native vectors for the two-operand and packed forms (a HIP_vector_type
+    // array there is copied through a stack object), the HIP vector type for the one-operand
+    // form (fewer registers at three waves per SIMD).
+    using ST = typename std::conditional<TWO_B || PK, v4i, uint4>::type;
     ST* ldsb = static_cast<ST*>(lds_raw);
     // Z W needs only X~ W1 - O (mu W): with the second operand stored negated, both products
     // accumulate into one set of int32 registers (exact while |sum| < 2^31: 384 per k-element).
@@ -123,42 +127,42 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
     const int64_t tr = rbase >> 6;
     const int m0 = (int)((rbase & 63) >> 4);
     const int gz0 = bz * GZ;
-    const int NT = a.G * L;
+    const int NT = a.G * LT;
     const int64_t kb0 = by * a.kb_per_split;
     int64_t kb1 = kb0 + a.kb_per_split;
     if (kb1 > a.kb_total) kb1 = a.kb_total;
 
-    v4i accX[MT][GZ][L], accO[MT][GZ][L];
+    v4i accX[MT][GZ][LT], accO[MT][GZ][LT];
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int j = 0; j < GZ; ++j)
 #pragma unroll
-            for (int l = 0; l < L; ++l) {
+            for (int l = 0; l < LT; ++l) {
                 accX[m][j][l] = v4i{0, 0, 0, 0};
                 accO[m][j][l] = v4i{0, 0, 0, 0};
             }
     bool gok[GZ];
 #pragma unroll
-    for (int j = 0; j < GZ; ++j) gok[j] = gz0 + j < a.G;
+    for (int j = 0; j < GZ; ++j) gok[j] = PK || gz0 + j < a.G;   // packed: one group, always valid
 
     if (kb0 < kb1) {
         const uint8_t* abase = a.A + ((tr * a.KB) << 12) + m0 * 1024 + lane * 16;
-        const int8_t* bxp = a.Bx + ((int64_t)(gz0 * L) << 10) + lane * 16;
-        const int8_t* bop = a.Bo + ((int64_t)(gz0 * L) << 10) + lane * 16;
+        const int8_t* bxp = a.Bx + ((int64_t)(gz0 * LT) << 10) + lane * 16;
+        const int8_t* bop = a.Bo + ((int64_t)(gz0 * LT) << 10) + lane * 16;
         const int64_t last_step = kb1 * 4 - 1;
         uint4 av[MT], an[MT];
-        v4i bx[GZ][L], bo[GZ][L], bxn[GZ][L], bon[GZ][L];
-        using LD = GenoLds<L, TWO_B>;
+        v4i bx[GZ][LT], bo[GZ][LT], bxn[GZ][LT], bon[GZ][LT];
+        using LD = GenoLds<LT, TWO_B>;
         ST stg[LD::kPerThread];
         auto stage_load = [&](int64_t kb) {
 #pragma unroll
             for (int i = 0; i < LD::kPerThread; ++i) {
                 const int c = (int)threadIdx.x + 256 * i;
                 const int op = c / LD::kChunksPerOp, w = c - op * LD::kChunksPerOp;
-                const int t = w / (L * 64), rem = w - t * (L * 64);   // step t, limb tiles of group gz0
+                const int t = w / (LT * 64), rem = w - t * (LT * 64);   // step t, limb tiles of group gz0
                 const ST* src = reinterpret_cast<const ST*>(op == 0 ? a.Bx : a.Bo);
-                stg[i] = src[((kb * 4 + t) * NT + gz0 * L) * 64 + rem];
+                stg[i] = src[((kb * 4 + t) * NT + gz0 * LT) * 64 + rem];
             }
         };
         auto stage_store = [&](int buf) {
@@ -175,10 +179,10 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
 #pragma unroll
             for (int j = 0; j < GZ; ++j)
 #pragma unroll
-                for (int l = 0; l < L; ++l) {
+                for (int l = 0; l < LT; ++l) {
                     if (!gok[j]) continue;
-                    bx[j][l] = *reinterpret_cast<const v4i*>(bxp + ((kb0 * 4 * NT + j * L + l) << 10));
-                    if (TWO_B) bo[j][l] = *reinterpret_cast<const v4i*>(bop + ((kb0 * 4 * NT + j * L + l) << 10));
+                    bx[j][l] = *reinterpret_cast<const v4i*>(bxp + ((kb0 * 4 * NT + j * LT + l) << 10));
+                    if (TWO_B) bo[j][l] = *reinterpret_cast<const v4i*>(bop + ((kb0 * 4 * NT + j * LT + l) << 10));
                 }
         }
         for (int64_t kb = kb0; kb < kb1; ++kb) {
@@ -192,10 +196,10 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
                 if (LDSB) {
                     const ST* lb = ldsb + cur * LD::kChunks;
 #pragma unroll
-                    for (int l = 0; l < L; ++l) {
-                        bx[0][l] = as_v4i(lb[(t * L + l) * 64 + lane]);
+                    for (int l = 0; l < LT; ++l) {
+                        bx[0][l] = as_v4i(lb[(t * LT + l) * 64 + lane]);
                         if (TWO_B) {
-                            bo[0][l] = as_v4i(lb[LD::kChunksPerOp + (t * L + l) * 64 + lane]);
+                            bo[0][l] = as_v4i(lb[LD::kChunksPerOp + (t * LT + l) * 64 + lane]);
                         }
                     }
                 }
@@ -204,10 +208,10 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
 #pragma unroll
                 for (int j = 0; j < GZ; ++j)
 #pragma unroll
-                    for (int l = 0; l < L; ++l) {
+                    for (int l = 0; l < LT; ++l) {
                         if (LDSB || !gok[j]) continue;
-                        bxn[j][l] = *reinterpret_cast<const v4i*>(bxp + ((sn * NT + j * L + l) << 10));
-                        if (TWO_B) bon[j][l] = *reinterpret_cast<const v4i*>(bop + ((sn * NT + j * L + l) << 10));
+                        bxn[j][l] = *reinterpret_cast<const v4i*>(bxp + ((sn * NT + j * LT + l) << 10));
+                        if (TWO_B) bon[j][l] = *reinterpret_cast<const v4i*>(bop + ((sn * NT + j * LT + l) << 10));
                     }
 #pragma unroll
                 for (int m = 0; m < MT; ++m) {
@@ -226,7 +230,7 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
                     for (int j = 0; j < GZ; ++j) {
                         if (!gok[j]) continue;
 #pragma unroll
-                        for (int l = 0; l < L; ++l) {
+                        for (int l = 0; l < LT; ++l) {
                             accX[m][j][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(X, bx[j][l], accX[m][j][l], 0, 0, 0);
                             if (!ONLY_X && !FUSE)
                                 accO[m][j][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, TWO_B ? bo[j][l] : bx[j][l],
@@ -234,7 +238,7 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
                         }
                         if (FUSE) {
 #pragma unroll
-                            for (int l = 0; l < L; ++l)
+                            for (int l = 0; l < LT; ++l)
                                 accX[m][j][l] = __builtin_amdgcn_mfma_i32_16x16x64_i8(O, bo[j][l], accX[m][j][l], 0, 0, 0);
                         }
                     }
@@ -243,7 +247,7 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
 #pragma unroll
                     for (int j = 0; j < GZ; ++j)
 #pragma unroll
-                        for (int l = 0; l < L; ++l) {
+                        for (int l = 0; l < LT; ++l) {
                             bx[j][l] = bxn[j][l];
                             if (TWO_B) bo[j][l] = bon[j][l];
                         }
@@ -259,11 +263,12 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
     }
 
     // Epilogue.  16x16 MFMA C layout: lane holds column r of its group, rows 4g + v (v = 0..3)
-    // of each m-subtile; the L limbs of one column sit in the same lane (tiles gz*L + l).
+    // of each m-subtile; the L limbs of one column sit in the same lane (tiles gz*L + l), or with
+    // PK in lanes c L + l of the same 16-lane group (gathered by the lane of limb 0).
 #pragma unroll
     for (int j = 0; j < GZ; ++j) {
-        const int64_t rhs = (int64_t)(gz0 + j) * 16 + r;
-        const bool col_ok = gok[j] && rhs < a.k;
+        const int64_t rhs = PK ? (int64_t)(r / L) : (int64_t)(gz0 + j) * 16 + r;
+        const bool col_ok = PK ? (r % L == 0 && rhs < a.k) : (gok[j] && rhs < a.k);
         double wmax = 0.0;
 #pragma unroll
         for (int m = 0; m < MT; ++m)
@@ -271,10 +276,18 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
             for (int v = 0; v < 4; ++v) {
                 const int64_t row = rbase + 16 * m + 4 * g + v;
                 long long P = 0, Q = 0;
+                if (PK) {
 #pragma unroll
-                for (int l = L - 1; l >= 0; --l) {
-                    P = P * 256 + accX[m][j][l][v];
-                    if (!ONLY_X && !FUSE) Q = Q * 256 + accO[m][j][l][v];
+                    for (int l = L - 1; l >= 0; --l) {
+                        P = P * 256 + __shfl(accX[m][j][0][v], lane + l);
+                        if (!ONLY_X && !FUSE) Q = Q * 256 + __shfl(accO[m][j][0][v], lane + l);
+                    }
+                } else {
+#pragma unroll
+                    for (int l = L - 1; l >= 0; --l) {
+                        P = P * 256 + accX[m][j][l][v];
+                        if (!ONLY_X && !FUSE) Q = Q * 256 + accO[m][j][l][v];
+                    }
                 }
                 if (ONLY_X && EPI != EPI_KV_G && col_ok && row < a.rows) Q = a.qext[row * 16 * (int64_t)a.G + rhs];
                 if (EPI == EPI_PQ) {
@@ -330,16 +343,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 // One group of 16 right-hand sides with the B fragments shared through LDS (GenoLds).  For Z W
 // the two products share one accumulator set (FUSE), which leaves room for 3 waves per SIMD
-// (two with four limbs).
-template <int L, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_geno2s(GenoArgs a) {
-    __shared__ v4i ldsb[2 * GenoLds<L, true>::kChunks];
-    geno_body<L, true, EPI, false, 4, 1, true>(a, ldsb);
+// (two with four limbs).  Packed limbs (PK, k L <= 16) need a quarter of the registers and run
+// four waves per SIMD.
+template <int L, int EPI, bool PK = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PK ? 4 : 3, PK ? 4 : 3))) void
+k_geno2s(GenoArgs a) {
+    __shared__ v4i ldsb[2 * GenoLds<PK ? 1 : L, true>::kChunks];
+    geno_body<L, true, EPI, false, 4, 1, true, PK>(a, ldsb);
 }
-template <int L, int EPI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_geno1s(GenoArgs a) {
-    __shared__ uint4 ldsb[2 * GenoLds<L, false>::kChunks];
-    geno_body<L, false, EPI, false, 4, 1, true>(a, ldsb);
+template <int L, int EPI, bool PK = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PK ? 4 : 3, PK ? 4 : 3))) void
+k_geno1s(GenoArgs a) {
+    __shared__ typename std::conditional<PK, v4i, uint4>::type ldsb[2 * GenoLds<PK ? 1 : L, false>::kChunks];
+    geno_body<L, false, EPI, false, 4, 1, true, PK>(a, ldsb);
 }
 // Four limbs (two accumulator sets of 4 x 4 tiles) fit only at two waves per SIMD.
 template <int L, int EPI>

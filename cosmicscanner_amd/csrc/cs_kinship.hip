@@ -91,9 +91,11 @@ __global__ void k_scales(const unsigned long long* __restrict__ mx, int64_t k, d
 // Fixed-point quantisation into L balanced signed 8-bit limbs, stored pre-permuted for the MFMA
 // B operand: column c, limb l -> tile (c/16)*L + l, column c%16 of the tile.
 // sign = -1 quantises -W (rint and the clamp are symmetric, so the digits give exactly -q(W)).
+// packed: limb l of column c in column c L + l of tile 0 (the packed tiling).
 __global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, int L, int NT,
                         double qmax, const double* __restrict__ inv, int8_t* __restrict__ B,
-                        int32_t* __restrict__ plain = nullptr, int64_t K16 = 0, double sign = 1.0) {
+                        int32_t* __restrict__ plain = nullptr, int64_t K16 = 0, double sign = 1.0,
+                        int packed = 0) {
     const int64_t total = rows * k;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
@@ -112,7 +114,8 @@ __global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, i
                 d = (int)(((q + 128) & 255) - 128);
                 q = (q - d) >> 8;
             }
-            B[bperm_offset(row, (c >> 4) * L + l, c & 15, NT)] = (int8_t)d;
+            if (packed) B[bperm_offset(row, 0, c * L + l, NT)] = (int8_t)d;
+            else B[bperm_offset(row, (c >> 4) * L + l, c & 15, NT)] = (int8_t)d;
         }
     }
 }
@@ -348,13 +351,24 @@ __global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, i
 // grid as the callers give it: x = padded rows / 256, y = splits, z = G.  The LDS kernels take
 // one group per workgroup (with two or more groups on a 1-D XCD-aware grid); without LDS
 // sharing (CSAMD_GENO_LDSB=0) two or more groups run with each wave on 32 rows and two groups.
-template <int L, bool TWO_B, int EPI>
-static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a) {
-    // B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
-    static const bool ldsb = [] {
+bool geno_lds_enabled() {
+    static const bool on = [] {
         const char* e = std::getenv("CSAMD_GENO_LDSB");
         return !(e && e[0] == '0');
     }();
+    return on;
+}
+
+template <int L, bool TWO_B, int EPI>
+static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a, bool packed) {
+    // B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
+    const bool ldsb = geno_lds_enabled();
+    if (packed) {
+        if (!ldsb || a.G != 1) throw RuntimeError("launch_geno: packed limbs need the LDS kernels and one group");
+        if (TWO_B) k_geno2s<L, EPI, true><<<grid, 256, 0, st>>>(a);
+        else k_geno1s<L, EPI, true><<<grid, 256, 0, st>>>(a);
+        return;
+    }
     if (ldsb) {
         GenoArgs f = a;
         dim3 g = grid;
@@ -386,9 +400,10 @@ static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a) {
     }
 }
 
-void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const GenoArgs& a, bool only_x) {
+void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const GenoArgs& a, bool only_x,
+                 bool packed) {
     if (only_x) {
-        if (two_b) throw RuntimeError("launch_geno: the genotype-only product has one operand");
+        if (two_b || packed) throw RuntimeError("launch_geno: the genotype-only product has one unpacked operand");
 #define CS_GENOX_CASE(LL, EE)                                \
         if (L == LL && epi == EE) {                          \
             k_genox<LL, EE><<<grid, 256, 0, st>>>(a);        \
@@ -406,7 +421,7 @@ void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const Ge
     }
 #define CS_GENO_CASE(LL, TB, EE)                         \
     if (L == LL && two_b == TB && epi == EE) {           \
-        launch_one<LL, TB, EE>(grid, st, a);             \
+        launch_one<LL, TB, EE>(grid, st, a, packed);     \
         CS_HIP(hipGetLastError());                       \
         return;                                          \
     }
@@ -436,7 +451,7 @@ void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, cons
     for (int i = 0; i < nW; ++i) {
         CS_HIP(hipMemsetAsync(B[i], 0, (size_t)(rows * tl.NT * 16), st));
         k_quant<<<grid_for(rows * k), 256, 0, st>>>(W[i], rows, k, tl.L, tl.NT, qmax,
-                                                   scl.as<double>() + k, B[i]);
+                                                   scl.as<double>() + k, B[i], nullptr, 0, 1.0, tl.packed);
     }
     CS_HIP(hipGetLastError());
 }
@@ -459,11 +474,11 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     hipStream_t st = op->stream;
     const int L = prec_limbs(op->prec);
     const double qmax = limbs_qmax(L);
-    const Tiling tl = tiling(k, L);
+    const bool sparse = use_sparse_missing(op);
+    const Tiling tl = tiling(k, L, !sparse);   // the genotype-only kernels take unpacked operands
     const int64_t K16 = 16 * (int64_t)tl.G;
     const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
     const int32_t* inv_idx = op->identity ? nullptr : op->d_inv_idx.as<int32_t>();
-    const bool sparse = use_sparse_missing(op);
 
     CS_HIP(hipEventRecord(op->ev[0], st));
     op->Vt.ensure((size_t)(b->Npad * k) * sizeof(double));
@@ -514,7 +529,7 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     a.W2 = op->W2.as<double>();
     a.mx = op->mxW.as<unsigned long long>();
     a.qext = sparse ? op->qext.as<long long>() : nullptr;
-    launch_geno(L, false, EPI_KV_GT, dim3((unsigned)(b->Mpad / 256), 1, (unsigned)tl.G), st, a, sparse);
+    launch_geno(L, false, EPI_KV_GT, dim3((unsigned)(b->Mpad / 256), 1, (unsigned)tl.G), st, a, sparse, tl.packed);
     CS_HIP(hipEventRecord(op->ev[2], st));
 
     // common per-column scale of W1 and W2, their limbs
@@ -523,7 +538,7 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     op->B1.ensure((size_t)(b->Mpad * tl.NT * 16));
     CS_HIP(hipMemsetAsync(op->B1.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
     k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W1.as<double>(), b->Mpad, k, L, tl.NT, qmax,
-                                                  sclW + k, op->B1.as<int8_t>());
+                                                  sclW + k, op->B1.as<int8_t>(), nullptr, 0, 1.0, tl.packed);
     if (sparse) {
         op->B2p.ensure((size_t)(b->Mpad * K16) * 4);
         op->S2.ensure((size_t)K16 * 8);
@@ -536,7 +551,7 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
         CS_HIP(hipMemsetAsync(op->B2.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
         // negated: the Z W kernel accumulates X~ W1 and O (-mu W) into the same registers
         k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, L, tl.NT, qmax,
-                                                      sclW + k, op->B2.as<int8_t>(), nullptr, 0, -1.0);
+                                                      sclW + k, op->B2.as<int8_t>(), nullptr, 0, -1.0, tl.packed);
     }
     CS_HIP(hipGetLastError());
     op->Yint.ensure((size_t)(b->Npad * k) * sizeof(long long));
@@ -589,7 +604,8 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     if (sparse)
         launch_geno(L, false, EPI_KV_G, dim3((unsigned)rblocks, (unsigned)nsplit, (unsigned)tl.G), st, c, true);
     else
-        launch_geno(L, true, EPI_KV_G, dim3((unsigned)rblocks, (unsigned)nsplit, (unsigned)tl.G), st, c);
+        launch_geno(L, true, EPI_KV_G, dim3((unsigned)rblocks, (unsigned)nsplit, (unsigned)tl.G), st, c, false,
+                    tl.packed);
     CS_HIP(hipEventRecord(op->ev[4], st));
     if (sparse) {
         k_sparse_yadd<<<grid_for(b->N * k), 256, 0, st>>>(b->miss_smp_off.as<int64_t>(), b->miss_smp_idx.as<int32_t>(),

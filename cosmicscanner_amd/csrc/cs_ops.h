@@ -31,12 +31,20 @@ inline void check_dense_eigen_size(int64_t n, const char* who) {
                               "(rocSOLVER's 32-bit indexing of the n x n work arrays); use the matrix-free path");
 }
 
+// Whether the genotype products run the LDS-shared kernels (CSAMD_GENO_LDSB unset or not 0).
+bool geno_lds_enabled();
+
+// Operand tiling of k right-hand sides with L limbs: G groups of 16 columns and NT = G L limb
+// tiles, or -- packed, when k L <= 16 and allow_pack (LDS kernels only) -- one tile holding limb
+// l of column c in tile column c L + l.
 struct Tiling {
     int G, L, NT;
+    bool packed;
 };
-inline Tiling tiling(int64_t k, int L) {
+inline Tiling tiling(int64_t k, int L, bool allow_pack = true) {
+    if (allow_pack && k >= 1 && k * L <= 16 && geno_lds_enabled()) return Tiling{1, L, 1, true};
     const int G = (int)((k + 15) / 16);
-    return Tiling{G, L, G * L};
+    return Tiling{G, L, G * L, false};
 }
 
 // Quantise the columns of a row-major [rows][k] fp64 matrix (rows = padded k extent of the
@@ -48,7 +56,7 @@ void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, cons
 // Launch k_geno<L, TWO_B, EPI> (cs_geno_mfma.h) on the given grid.
 struct GenoArgs;
 void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const GenoArgs& a,
-                 bool only_x = false);
+                 bool only_x = false, bool packed = false);
 
 void ensure_stats(cs_bed* b);
 void ensure_sample_major(cs_bed* b);

@@ -224,7 +224,7 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
             ga.Bx = ga.Bo = B.as<int8_t>();
             ga.k = a.nr;
             ga.pq = PQ.as<long long>();
-            launch_geno(L, false, EPI_PQ, dim3((unsigned)(rows / 256), 1, (unsigned)tl.G), st, ga);
+            launch_geno(L, false, EPI_PQ, dim3((unsigned)(rows / 256), 1, (unsigned)tl.G), st, ga, false, tl.packed);
             k_scan_epilogue<<<grid_for(ns), 256, 0, st>>>(
                 dsn.as<int32_t>(), ns, r0, rows, counts.as<int64_t>(), PQ.as<long long>(), G16, a.nr,
                 scl.as<double>(), a.min_maf, dfreq.as<double>(), dnobs.as<int64_t>(), dpass.as<int32_t>(),

@@ -123,6 +123,27 @@ def test_kinship_precision_modes_larger(csa, mode, tol):
         op.close()
 
 
+@pytest.mark.parametrize("mode,tol", [(2, 1e-3), (3, 2e-6), (4, 3e-8)])
+def test_kinship_packed_limbs_small_k(csa, mode, tol):
+    """Up to 16 / L right-hand sides share one MFMA tile (limb l of column c in tile column
+    c L + l); widths on both sides of that limit, against the oracle."""
+    N, M = 1500, 2300
+    body = oracle.gen_hwe_bed(78, N, M, 0.01, 0.5, 0.01)
+    reader = csa.PlinkReader.from_body(body, N, M)
+    means, inv = oracle.stats(body, N, M)
+    rng = np.random.default_rng(8)
+    keep = rng.choice(N, 1300, replace=True).astype(np.int32)
+    op = csa.PlinkKinshipOperator(reader, keep, None)
+    op.set_precision(mode)
+    for k in sorted({1, 16 // mode, 16 // mode + 1}):
+        X = rng.standard_normal((op.rows(), k))
+        ref = oracle.kinship_apply(body, N, means, inv, keep, None, X)
+        err = colrel(op.apply(X), ref)
+        print(f"mode {mode} k {k}: column-relative error {err:.2e}", flush=True)
+        assert err < tol
+    op.close()
+
+
 def test_kinship_sketch_width_266(csa):
     # The reference's randomized low-rank path applies K to n x (256 + 10) sketches
     # (lmm_engine.cpp:915-946); widths above 256 columns exercise the per-column scale grid.
