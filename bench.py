@@ -469,6 +469,22 @@ def main():
     host_rate = N * M / ((time.perf_counter() - t1) / 3)
     log(f"K.V host buffers: {host_rate:.3e} genotypes/s")
 
+    # few right-hand sides (limbs packed into one MFMA tile when k L <= 16), device buffers
+    small_k = {}
+    for ks in (1, 4):
+        if ks >= k:
+            continue
+        for _ in range(2):
+            op.apply_device(X.ptr, N, ks, Y.ptr, N)
+        csa.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(10):
+            op.apply_device(X.ptr, N, ks, Y.ptr, N)
+        csa.synchronize()
+        small_k[f"k{ks}_ms"] = (time.perf_counter() - t1) / 10 * 1e3
+    if small_k:
+        log("K.V few columns: " + ", ".join(f"{key} {v:.3f}" for key, v in small_k.items()))
+
     fp32 = None
     if not args.no_fp32:
         op.set_precision(csa.CS_PREC_FP32)
@@ -542,6 +558,7 @@ def main():
         "config": {"workload": "GRM build + matrix-free K.v, n=50,000 x m=500,000 (K.V apply, k=16)",
                    "n": N, "m": M, "k": k, "precision": args.prec},
         "host_buffer_genotypes_per_s": host_rate,
+        "small_k_apply_ms": small_k or None,
         "fp32_gpu_genotypes_per_s": fp32,
         "speedup_vs_fp32_gpu": (value / fp32) if fp32 else None,
         "roofline": roofline,
