@@ -33,6 +33,7 @@ struct cs_kinship {
     csamd::DevBuf dX, dY, Vt, BV, W1, W2, B1, B2, Yint, mxW, mx, scl, Vf, PQf, Wf;
     csamd::DevBuf Vp, B2p, S1, S2, qext;   // sparse-missing form: plain quantised operands, sums
     csamd::DevBuf diag_tab, diag_full;
+    std::vector<double> diag_cache;  // diag(K) over the rows, once computed (fixed by keep / variants)
     hipEvent_t ev[6] = {};
     float last_ms[5] = {0, 0, 0, 0, 0};
 };
@@ -280,8 +281,9 @@ __global__ void k_epi_g_f32(const float* __restrict__ PQ, int KC, int64_t k,
 
 // ------------------------------------------------------------------------------------------
 // diag(K) over the sample-major copy: table T[j][code] = mult_j s_j^2 (x_code - mu_j)^2 (0 for
-// missing), one wave per 64-sample tile summing T over the SNPs in a fixed order per lane, the
-// four chunk lanes of a sample reduced by shuffles.
+// missing), one wave per 64-sample tile summing T over the SNPs in a fixed order per lane (the
+// k-block's table rows staged in LDS for the workgroup), the four chunk lanes of a sample reduced
+// by shuffles.  The grid covers the padded samples (whole tiles; padding codes are "missing").
 __global__ void k_diag_table(int64_t M, const double* __restrict__ means,
                              const double* __restrict__ inv_stds, const int32_t* __restrict__ mult,
                              double* __restrict__ T) {
@@ -319,25 +321,29 @@ __global__ void k_trace_terms(const int64_t* __restrict__ counts, int64_t M, con
 __global__ __launch_bounds__(256) void k_diag(const uint8_t* __restrict__ smp, int64_t KB,
                                               int64_t N, int64_t M, const double* __restrict__ T,
                                               double inv_M, double* __restrict__ diag) {
+    __shared__ double tab[256 * 4];   // T of the k-block's 256 SNPs, shared by the four waves
     const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
     const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tr * 64 >= N) return;
     const uint8_t* base = smp + ((tr * KB) << 12) + lane * 16;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int64_t kb = 0; kb < KB; ++kb) {
-        const int64_t j0 = kb * 256 + g * 64;
+        uint4 w4[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) w4[m] = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
+        const int64_t js = kb * 256 + threadIdx.x;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) tab[threadIdx.x * 4 + c] = js < M ? T[js * 4 + c] : 0.0;
+        __syncthreads();
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            const uint4 w4 = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
-            const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+            const uint32_t ws[4] = {w4[m].x, w4[m].y, w4[m].z, w4[m].w};
 #pragma unroll
             for (int q = 0; q < 4; ++q)
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const int64_t j = j0 + 16 * q + u;
-                    if (j < M) acc[m] += T[j * 4 + ((ws[q] >> (2 * u)) & 3u)];
-                }
+                for (int u = 0; u < 16; ++u)
+                    acc[m] += tab[(g * 64 + 16 * q + u) * 4 + ((ws[q] >> (2 * u)) & 3u)];
         }
+        __syncthreads();
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -828,6 +834,10 @@ int cs_kinship_apply(cs_kinship* op, const double* X, int64_t ldx, int64_t k, do
 int cs_kinship_diag(cs_kinship* op, double* diag) {
     return guarded([&] {
         if (!op || !diag) throw InvalidArgument("cs_kinship_diag: null argument");
+        if (!op->diag_cache.empty()) {
+            std::copy(op->diag_cache.begin(), op->diag_cache.end(), diag);
+            return;
+        }
         cs_bed* b = op->bed;
         hipStream_t st = op->stream;
         op->diag_tab.ensure((size_t)b->M * 4 * sizeof(double));
@@ -843,6 +853,7 @@ int cs_kinship_diag(cs_kinship* op, double* diag) {
                               hipMemcpyDeviceToHost, st));
         CS_HIP(hipStreamSynchronize(st));
         for (int64_t r = 0; r < op->n; ++r) diag[r] = full[(size_t)(op->keep.empty() ? r : op->keep[(size_t)r])];
+        op->diag_cache.assign(diag, diag + op->n);
     });
 }
 

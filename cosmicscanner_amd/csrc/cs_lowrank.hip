@@ -31,6 +31,7 @@
 #include <random>
 #include <vector>
 
+#include "cs_blas.h"
 #include "cs_common.h"
 #include "cs_ops.h"
 
@@ -88,15 +89,6 @@ __global__ void k_scale_cols_add(const double* __restrict__ base_src, double bas
         out[e] += base * base_src[e];
 }
 
-// C[e] = sum over the nb partial products (fixed order).
-__global__ void k_sum_parts(const double* __restrict__ part, int64_t nb, int64_t len, double* __restrict__ C) {
-    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (e >= len) return;
-    double acc = 0.0;
-    for (int64_t b = 0; b < nb; ++b) acc += part[b * len + e];
-    C[e] = acc;
-}
-
 // M = (L^-1)^T as a full s x s matrix (zeros below the diagonal) from the lower L^-1 in Li.
 __global__ void k_upper_from_lower_t(const double* __restrict__ Li, int64_t s, double* __restrict__ M) {
     const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -142,33 +134,12 @@ void householder_q(cs_lowrank* h, double* A, int64_t n, int64_t s, DevBuf& tau) 
     CS_BLAS3(rocsolver_dorgqr(h->blas, (int)n, (int)s, (int)s, A, (int)n, tau.as<double>()));
 }
 
-// C (s1 x s2, ldc = s1) = A^T B for tall A (n x s1) and B (n x s2), column-major with ld n.
-// rocBLAS tiles only the small s1 x s2 output, so the long n dimension is split into chunks:
-// one strided-batched GEMM of the chunk products, then a fixed-order sum of the partials.
-void gemm_tn_tall(cs_lowrank* h, const double* A, const double* B, int64_t n, int64_t s1, int64_t s2, double* C) {
-    const double one = 1.0, zero = 0.0;
-    const int64_t chunk = std::max<int64_t>(1024, (n + 255) / 256);
-    const int64_t full = n / chunk, nb = (n + chunk - 1) / chunk;
-    DevBuf part((size_t)(std::max<int64_t>(nb, 1) * s1 * s2) * 8);
-    if (full > 0)
-        CS_BLAS3(rocblas_dgemm_strided_batched(h->blas, rocblas_operation_transpose, rocblas_operation_none, (int)s1,
-                                               (int)s2, (int)chunk, &one, A, (int)n, chunk, B, (int)n, chunk, &zero,
-                                               part.as<double>(), (int)s1, s1 * s2, (int)full));
-    if (nb > full)
-        CS_BLAS3(rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, (int)s1, (int)s2,
-                               (int)(n - full * chunk), &one, A + full * chunk, (int)n, B + full * chunk, (int)n,
-                               &zero, part.as<double>() + full * s1 * s2, (int)s1));
-    k_sum_parts<<<grid_for(s1 * s2), 256, 0, h->st>>>(part.as<double>(), nb, s1 * s2, C);
-    CS_HIP(hipGetLastError());
-    CS_HIP(hipStreamSynchronize(h->st));   // part is freed on return
-}
-
 void thin_q(cs_lowrank* h, double* A, int64_t n, int64_t s, DevBuf& tau, double* scratch) {
     const double one = 1.0, zero = 0.0;
     DevBuf G((size_t)(s * s) * 8), Mt((size_t)(s * s) * 8), info(sizeof(rocblas_int));
     std::vector<double> dg((size_t)s);
     for (int pass = 0; pass < 2; ++pass) {
-        gemm_tn_tall(h, A, A, n, s, s, G.as<double>());
+        gemm_tn_tall(h->blas, h->st, A, A, n, s, s, G.as<double>());
         CS_BLAS3(rocsolver_dpotrf(h->blas, rocblas_fill_lower, (int)s, G.as<double>(), (int)s,
                                   info.as<rocblas_int>()));
         rocblas_int inf = 0;
@@ -310,7 +281,7 @@ int cs_lowrank_prepare(cs_kinship* K, const double* y, const float* X, int64_t l
         kv(K, Yb.as<double>(), n, s, A.as<double>());            // kq
         const double one = 1.0, zero = 0.0;
         DevBuf B((size_t)(s * s) * 8), ev((size_t)s * 8), E((size_t)s * 8), info(sizeof(rocblas_int));
-        gemm_tn_tall(h.get(), Yb.as<double>(), A.as<double>(), n, s, s, B.as<double>());
+        gemm_tn_tall(h->blas, h->st, Yb.as<double>(), A.as<double>(), n, s, s, B.as<double>());
         CS_BLAS3(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)s, B.as<double>(), (int)s,
                                   ev.as<double>(), E.as<double>(), info.as<rocblas_int>()));
         std::vector<double> evals((size_t)s);
@@ -364,8 +335,8 @@ int cs_lowrank_prepare(cs_kinship* K, const double* y, const float* X, int64_t l
         CS_HIP(hipMemcpyAsync(h->y.p, y, (size_t)n * 8, hipMemcpyHostToDevice, h->st));
         const int64_t r = selected;
         DevBuf UtXd((size_t)(r * std::max<int64_t>(c, 1)) * 8), Utyd((size_t)r * 8);
-        if (c > 0) gemm_tn_tall(h.get(), h->U.as<double>(), h->X.as<double>(), n, r, c, UtXd.as<double>());
-        gemm_tn_tall(h.get(), h->U.as<double>(), h->y.as<double>(), n, r, 1, Utyd.as<double>());
+        if (c > 0) gemm_tn_tall(h->blas, h->st, h->U.as<double>(), h->X.as<double>(), n, r, c, UtXd.as<double>());
+        gemm_tn_tall(h->blas, h->st, h->U.as<double>(), h->y.as<double>(), n, r, 1, Utyd.as<double>());
         h->UtX.resize((size_t)(r * c));
         h->Uty.resize((size_t)r);
         if (c > 0) CS_HIP(hipMemcpyAsync(h->UtX.data(), UtXd.p, (size_t)(r * c) * 8, hipMemcpyDeviceToHost, h->st));

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <vector>
 
+#include "cs_blas.h"
 #include "cs_common.h"
 #include "cs_ops.h"
 
@@ -104,16 +105,12 @@ void pscan_finish(cs_bed* bed, const int32_t* keep, int64_t n_keep, int64_t n, i
                   double* freq, int64_t* n_obs, int32_t* pass, double* beta, double* se, double* p, double* log10p,
                   double* gamma_out, int64_t* ncal_out, hipStream_t st, rocblas_handle blas) {
     const int64_t k = T + c + L;
-    const double one = 1.0, zero = 0.0, mone = -1.0;
+    const double one = 1.0, mone = -1.0;
     const double* dX = B + T * n;
     DevBuf XtZ((size_t)(c * k) * 8), XtG((size_t)(c * std::max<int64_t>(1, L)) * 8), XtX((size_t)(c * c) * 8);
-    CS_BLAS2(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, (int)c, (int)k, (int)n, &one, dX,
-                           (int)n, Z, (int)n, &zero, XtZ.as<double>(), (int)c));
-    CS_BLAS2(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, (int)c, (int)c, (int)n, &one, dX,
-                           (int)n, dX, (int)n, &zero, XtX.as<double>(), (int)c));
-    if (L > 0)
-        CS_BLAS2(rocblas_dgemm(blas, rocblas_operation_transpose, rocblas_operation_none, (int)c, (int)L, (int)n, &one,
-                               dX, (int)n, B + (T + c) * n, (int)n, &zero, XtG.as<double>(), (int)c));
+    gemm_tn_tall(blas, st, dX, Z, n, c, k, XtZ.as<double>());
+    gemm_tn_tall(blas, st, dX, dX, n, c, c, XtX.as<double>());
+    if (L > 0) gemm_tn_tall(blas, st, dX, B + (T + c) * n, n, c, L, XtG.as<double>());
     std::vector<double> hXtZ((size_t)(c * k)), hXtX((size_t)(c * c)), hXtG((size_t)(c * std::max<int64_t>(1, L)));
     std::vector<double> gZg((size_t)std::max<int64_t>(1, L)), gg((size_t)std::max<int64_t>(1, L));
     CS_BLAS2(rocblas_set_pointer_mode(blas, rocblas_pointer_mode_host));
