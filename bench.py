@@ -397,7 +397,7 @@ def main():
     ap.add_argument("--lowrank", action="store_true",
                     help="also run the low-rank matrix-free GWAS leg (the reference path, C3 genotypes)")
     # the dense eigen path (MvEigenVce::prepare) is bounded by rocSOLVER's 32-bit interfaces at
-    # n <= 32767 individuals; the C5 workload names 50,000
+    # n <= 32766 individuals; the C5 workload names 50,000
     ap.add_argument("--mv-n", type=int, default=32_000)
     ap.add_argument("--mv-m", type=int, default=300_000)
     ap.add_argument("--copy-probe", action="store_true",

@@ -23,11 +23,12 @@ inline int grid_for(int64_t total, int block = 256) {
 // one per grid.z slice), each group as L limb tiles of 16 columns (NT = G*L tiles in B).
 // rocSOLVER's dorgqr / dsyevd exist only with 32-bit (rocblas_int) interfaces, and dsyevd reports
 // failures as row (n+1) + col in one rocblas_int; the dense eigen paths therefore keep every
-// n x n work array and 2 n^2 workspace under 2^31 elements: n <= 32767 samples.
-constexpr int64_t kDenseEigenMaxN = 32767;
+// n x n work array and the divide-and-conquer workspace (LAPACK's 1 + 6 n + 2 n^2 elements) under
+// 2^31 elements: n <= 32766 samples.
+constexpr int64_t kDenseEigenMaxN = 32766;
 inline void check_dense_eigen_size(int64_t n, const char* who) {
     if (n > kDenseEigenMaxN)
-        throw InvalidArgument(std::string(who) + ": the dense eigendecomposition path supports at most 32767 samples "
+        throw InvalidArgument(std::string(who) + ": the dense eigendecomposition path supports at most 32766 samples "
                               "(rocSOLVER's 32-bit indexing of the n x n work arrays); use the matrix-free path");
 }
 

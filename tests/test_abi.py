@@ -39,13 +39,13 @@ def test_error_path_without_gpu_calls():
 
 
 def test_dense_eigen_size_guard_without_gpu_calls():
-    """The dense eigen paths refuse n > 32767 before any device call (rocSOLVER's 32-bit
+    """The dense eigen paths refuse n > 32766 before any device call (rocSOLVER's 32-bit
     dorgqr / dsyevd interfaces cannot address larger n x n work arrays)."""
     import cosmicscanner_amd as csa
     n = 40000
     y = np.zeros(n)
     X = np.ones((n, 1), dtype=np.float32)
-    with pytest.raises(ValueError, match="32767"):
+    with pytest.raises(ValueError, match="32766"):
         csa.DenseLmm(y, X)
-    with pytest.raises(ValueError, match="32767"):
+    with pytest.raises(ValueError, match="32766"):
         csa.MvLmm(np.zeros((n, 2)), X)
