@@ -456,9 +456,19 @@ def main():
         name, kms, kbytes, key = "k_geno2s (Z W, sample-major)", phase[3], g_bytes, "k_geno2s"
     achieved = kbytes / (kms * 1e-3) / 1e9
     traffic = profiled_traffic(key) if (N, M, k) == (50_000, 500_000, 16) else None
+    # int8 MFMA work the exact formulation issues in that kernel: 2 products (x~, o) x L limbs of
+    # 16 x 16 x 64 MACs per 16 rows x 64 k per group, i.e. 2 * 2 L * 16 * rows * k_len ops
+    limbs = 3 if args.prec == "int8_l3" else 2
+    groups = (k + 15) // 16
+    rows_pad = ((M if key == "k_geno1s" else N) + 255) // 256 * 256
+    k_pad = ((N if key == "k_geno1s" else M) + 255) // 256 * 256
+    ops_issued = 2.0 * 2 * limbs * 16 * groups * float(rows_pad) * float(k_pad)
+    mfma_rate = ops_issued / (kms * 1e-3) / 1e12
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
-                    kernel_ms=round(kms, 4), apply_frac=round(gt_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+                    kernel_ms=round(kms, 4), apply_frac=round(gt_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    int8_mfma_tops_issued=round(mfma_rate, 1),
+                    int8_mfma_frac=round(mfma_rate / INT8_PEAK_TOPS, 4))
 
     # the same apply through the host-buffer entry point (PCIe transfers of X and Y included)
     Xh = np.asfortranarray(np.random.default_rng(7).standard_normal((N, k)))
