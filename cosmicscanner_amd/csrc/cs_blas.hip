@@ -1,4 +1,5 @@
-// FP64 products with a long inner dimension (split over the inner dimension).
+// FP64 helpers on rocBLAS: products with a long inner dimension (split over it) and the
+// covariate projection A^T G A by Householder reflectors.
 #include "cs_blas.h"
 
 #include <algorithm>
@@ -47,6 +48,37 @@ void gemm_tn_tall(rocblas_handle blas, hipStream_t st, const double* A, const do
     k_sum_parts<<<grid_for(s1 * s2), 256, 0, st>>>(part.as<double>(), nb, s1 * s2, C);
     CS_HIP(hipGetLastError());
     CS_HIP(hipStreamSynchronize(st));   // part is freed on return
+}
+
+void householder_project_sym(rocblas_handle blas, hipStream_t st, const double* V, const double* tau_h, int64_t n,
+                             int64_t c, double* G) {
+    if (c <= 0) return;
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    DevBuf v((size_t)n * 8), w((size_t)n * 8);
+    check_blas(rocblas_set_pointer_mode(blas, rocblas_pointer_mode_host), "householder_project_sym");
+    for (int64_t i = 0; i < c; ++i) {
+        const int64_t m = n - i;
+        const double tau = tau_h[i];
+        if (tau == 0.0) continue;   // H_i = I
+        // v = (1, V[i+1:, i])
+        if (m > 1)
+            CS_HIP(hipMemcpyAsync(v.as<double>() + 1, V + i * n + i + 1, (size_t)(m - 1) * 8, hipMemcpyDeviceToDevice,
+                                  st));
+        CS_HIP(hipMemcpyAsync(v.p, &one, 8, hipMemcpyHostToDevice, st));
+        double* Gi = G + i * n + i;
+        check_blas(rocblas_dsymv(blas, rocblas_fill_lower, (int)m, &tau, Gi, (int)n, v.as<double>(), 1, &zero,
+                                 w.as<double>(), 1),
+                   "householder_project_sym (symv)");
+        double wv = 0.0;
+        check_blas(rocblas_ddot(blas, (int)m, w.as<double>(), 1, v.as<double>(), 1, &wv), "householder_project_sym (dot)");
+        const double alpha = -0.5 * tau * wv;
+        check_blas(rocblas_daxpy(blas, (int)m, &alpha, v.as<double>(), 1, w.as<double>(), 1),
+                   "householder_project_sym (axpy)");
+        check_blas(rocblas_dsyr2(blas, rocblas_fill_lower, (int)m, &mone, v.as<double>(), 1, w.as<double>(), 1, Gi,
+                                 (int)n),
+                   "householder_project_sym (syr2)");
+    }
+    CS_HIP(hipStreamSynchronize(st));   // v, w are freed on return
 }
 
 }  // namespace csamd

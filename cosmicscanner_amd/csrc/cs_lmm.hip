@@ -1,8 +1,9 @@
 // Dense LMM null model on the device and the rotated per-SNP scan that consumes it.
 //
 //   EigenVce::prepare with covariate projection (libCosmicCore/src/eigen_vce.cpp:19-49):
-//     full Householder Q of X (n x c), A = Q[:, c:], G_reml = A^T G A, eigendecomposition
-//     G_reml = U diag(D) U^T (D clamped at 0), W_proj = A U, Uty = U^T A^T y.   rocSOLVER / rocBLAS.
+//     Householder QR of X (n x c), A = Q[:, c:], G_reml = A^T G A, eigendecomposition
+//     G_reml = U diag(D) U^T (D clamped at 0), W_proj = A U, Uty = U^T A^T y.   rocSOLVER / rocBLAS;
+//     A^T G A, A^T y and A U come from the c reflectors directly (O(c n^2), no explicit Q).
 //   VCESolver::solveDiagonal (libCosmicCore/src/vce.cpp:3073-3171), the default "brent" null
 //     solver: REML golden section on log10(delta) in [-5, 5] while |c - d| > 1e-5; one workgroup
 //     evaluates every likelihood with a fixed-order reduction.
@@ -17,6 +18,7 @@
 #include <memory>
 #include <vector>
 
+#include "cs_blas.h"
 #include "cs_common.h"
 #include "cs_ops.h"
 
@@ -156,11 +158,10 @@ struct cs_lmm {
     int64_t n = 0, c = 0, ne = 0;
     rocblas_handle blas = nullptr;
     hipStream_t st = nullptr;
-    DevBuf Q;        // n x n: full Householder Q of X (A = Q[:, c:]); unused when c == 0
+    DevBuf Q, tau;   // Householder reflectors of X (n x c, geqrf) and their scalars; unused when c == 0
+    std::vector<double> tau_h;
     DevBuf y;        // n
-    DevBuf G;        // staging of a host G (n x n)
-    DevBuf T;        // n x ne work
-    DevBuf U;        // ne x ne eigenvectors
+    DevBuf G;        // working copy of G (n x n): projected, then the eigenvectors at (c, c)
     DevBuf D, E, Uty, t, vinv, py;
     DevBuf R;        // n x (1 + ne): [W_proj P_y, W_proj]
     DevBuf info, res;
@@ -177,51 +178,37 @@ static void lmm_prepare(cs_lmm* h, const double* G, int64_t ldg, bool g_dev) {
     const int64_t n = h->n, ne = h->ne, c = h->c;
     if (ldg < n) throw RuntimeError("EigenVce::prepare: Dimension mismatch");
     const double one = 1.0, zero = 0.0;
-    const double* Gd = G;
-    if (!g_dev) {
-        h->G.ensure((size_t)(n * n) * 8);
-        CS_HIP(hipMemcpy2DAsync(h->G.p, (size_t)n * 8, G, (size_t)ldg * 8, (size_t)n * 8, (size_t)n,
-                                hipMemcpyHostToDevice, h->st));
-        Gd = h->G.as<double>();
-        ldg = n;
-    }
-    h->U.ensure((size_t)(ne * ne) * 8);
-    h->T.ensure((size_t)(n * ne) * 8);
+    h->G.ensure((size_t)(n * n) * 8);
+    double* Gw = h->G.as<double>();
+    CS_HIP(hipMemcpy2DAsync(Gw, (size_t)n * 8, G, (size_t)ldg * 8, (size_t)n * 8, (size_t)n,
+                            g_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->st));
     h->D.ensure((size_t)ne * 8);
     h->E.ensure((size_t)ne * 8);
     h->Uty.ensure((size_t)ne * 8);
-    h->t.ensure((size_t)ne * 8);
+    h->t.ensure((size_t)n * 8);
     h->R.ensure((size_t)(n * (ne + 1)) * 8);
-    double* U = h->U.as<double>();
     double* W = h->R.as<double>() + n;
+    CS_HIP(hipMemcpyAsync(h->t.p, h->y.p, (size_t)n * 8, hipMemcpyDeviceToDevice, h->st));
     if (c > 0) {
-        const double* A = h->Q.as<double>() + c * n;
-        // G_reml = A^T (G A)
-        CS_BLAS(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)ne,
-                              (int)n, &one, Gd, (int)ldg, A, (int)n, &zero, h->T.as<double>(), (int)n));
-        CS_BLAS(rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, (int)ne,
-                              (int)ne, (int)n, &one, A, (int)n, h->T.as<double>(), (int)n, &zero, U,
-                              (int)ne));
-        CS_BLAS(rocblas_dgemv(h->blas, rocblas_operation_transpose, (int)n, (int)ne, &one, A, (int)n,
-                              h->y.as<double>(), 1, &zero, h->t.as<double>(), 1));
-    } else {
-        CS_HIP(hipMemcpy2DAsync(U, (size_t)n * 8, Gd, (size_t)ldg * 8, (size_t)n * 8, (size_t)n,
-                                hipMemcpyDeviceToDevice, h->st));
-        CS_HIP(hipMemcpyAsync(h->t.p, h->y.p, (size_t)n * 8, hipMemcpyDeviceToDevice, h->st));
+        // G_reml = A^T G A (A = Q[:, c:]) in place from the reflectors; t = Q^T y (rows c: = A^T y)
+        householder_project_sym(h->blas, h->st, h->Q.as<double>(), h->tau_h.data(), n, c, Gw);
+        CS_BLAS(rocsolver_dormqr(h->blas, rocblas_side_left, rocblas_operation_transpose, (int)n, 1, (int)c,
+                                 h->Q.as<double>(), (int)n, h->tau.as<double>(), h->t.as<double>(), (int)n));
     }
-    CS_BLAS(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)ne, U, (int)ne,
+    double* U = Gw + c * n + c;   // ne x ne, ld n
+    CS_BLAS(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)ne, U, (int)n,
                              h->D.as<double>(), h->E.as<double>(), h->info.as<rocblas_int>()));
     k_clamp_nonneg<<<grid_for(ne), 256, 0, h->st>>>(h->D.as<double>(), ne);
     CS_HIP(hipGetLastError());
-    // Uty = U^T (A^T y); W_proj = A U
-    CS_BLAS(rocblas_dgemv(h->blas, rocblas_operation_transpose, (int)ne, (int)ne, &one, U, (int)ne,
-                          h->t.as<double>(), 1, &zero, h->Uty.as<double>(), 1));
+    // Uty = U^T (A^T y); W_proj = A U = Q [0; U]
+    CS_BLAS(rocblas_dgemv(h->blas, rocblas_operation_transpose, (int)ne, (int)ne, &one, U, (int)n,
+                          h->t.as<double>() + c, 1, &zero, h->Uty.as<double>(), 1));
+    if (c > 0) CS_HIP(hipMemset2DAsync(W, (size_t)n * 8, 0, (size_t)c * 8, (size_t)ne, h->st));
+    CS_HIP(hipMemcpy2DAsync(W + c, (size_t)n * 8, U, (size_t)n * 8, (size_t)ne * 8, (size_t)ne,
+                            hipMemcpyDeviceToDevice, h->st));
     if (c > 0)
-        CS_BLAS(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)ne,
-                              (int)ne, &one, h->Q.as<double>() + c * n, (int)n, U, (int)ne, &zero, W,
-                              (int)n));
-    else
-        CS_HIP(hipMemcpyAsync(W, U, (size_t)(n * n) * 8, hipMemcpyDeviceToDevice, h->st));
+        CS_BLAS(rocsolver_dormqr(h->blas, rocblas_side_left, rocblas_operation_none, (int)n, (int)ne, (int)c,
+                                 h->Q.as<double>(), (int)n, h->tau.as<double>(), W, (int)n));
     rocblas_int info = 0;
     CS_HIP(hipMemcpyAsync(&info, h->info.p, sizeof(info), hipMemcpyDeviceToHost, h->st));
     CS_HIP(hipStreamSynchronize(h->st));
@@ -254,12 +241,12 @@ int cs_lmm_create(const double* y, const float* X, int64_t ldx, int64_t n, int64
             std::vector<double> Xd((size_t)(n * c));
             for (int64_t j = 0; j < c; ++j)
                 for (int64_t i = 0; i < n; ++i) Xd[(size_t)(j * n + i)] = (double)X[j * ldx + i];
-            h->Q.alloc((size_t)(n * n) * 8);
-            DevBuf tau((size_t)c * 8);
+            h->Q.alloc((size_t)(n * c) * 8);
+            h->tau.alloc((size_t)c * 8);
             CS_HIP(hipMemcpyAsync(h->Q.p, Xd.data(), Xd.size() * 8, hipMemcpyHostToDevice, h->st));
-            CS_BLAS(rocsolver_dgeqrf(h->blas, (int)n, (int)c, h->Q.as<double>(), (int)n, tau.as<double>()));
-            CS_BLAS(rocsolver_dorgqr(h->blas, (int)n, (int)n, (int)c, h->Q.as<double>(), (int)n,
-                                     tau.as<double>()));
+            CS_BLAS(rocsolver_dgeqrf(h->blas, (int)n, (int)c, h->Q.as<double>(), (int)n, h->tau.as<double>()));
+            h->tau_h.resize((size_t)c);
+            CS_HIP(hipMemcpyAsync(h->tau_h.data(), h->tau.p, (size_t)c * 8, hipMemcpyDeviceToHost, h->st));
             CS_HIP(hipStreamSynchronize(h->st));
         }
         CS_HIP(hipStreamSynchronize(h->st));

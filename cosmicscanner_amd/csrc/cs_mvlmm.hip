@@ -2,8 +2,9 @@
 // 496-910, 1031-1066) and the fixed-null Wald scan of MvLmmEngine::scan(PlinkReader&)
 // (src/mvlmm_engine.cpp:210-394, 756-931), with covariate projection (the default).
 //
-// prepare: full Householder Q of X, A = Q[:, c:], G_reml = A^T G A, eigendecomposition (D clamped
-//   at 0), W_proj = A U, Uty = U^T A^T Y (ne x d); start values Vg = Ve = Cov(Y) / 2.
+// prepare: Householder QR of X, A = Q[:, c:], G_reml = A^T G A (from the c reflectors, no
+//   explicit Q), eigendecomposition (D clamped at 0), W_proj = A U, Uty = U^T A^T Y (ne x d);
+//   start values Vg = Ve = Cov(Y) / 2.
 // Null model: EM (updateEM) or EM warm-up + AI-REML with step halving (runAireml).  Every O(ne)
 //   sum -- the EM statistics, the AI matrix and score, the REML log-likelihood -- is one thread
 //   per eigen-sample on the GPU followed by a fixed-order column reduction; the d(d+1) x d(d+1)
@@ -24,6 +25,7 @@
 #include <memory>
 #include <vector>
 
+#include "cs_blas.h"
 #include "cs_common.h"
 #include "cs_ops.h"
 #include "cs_stats.h"
@@ -800,7 +802,8 @@ struct cs_mvlmm {
     int64_t n = 0, c = 0, ne = 0, d = 0;
     rocblas_handle blas = nullptr;
     hipStream_t st = nullptr;
-    DevBuf Q, Y, G, T, U, D, E, t, Uty, W, info, P, work, sums;
+    DevBuf Q, tau, Y, G, D, E, t, Uty, W, info, P, work, sums;   // Q: Householder reflectors of X (n x c)
+    std::vector<double> tau_h;              // their scalars on the host
     std::vector<double> Yh;                 // n x d host copy (Cov(Y) start values)
     std::vector<double> Vg, Ve, Vg_se, Ve_se;   // d x d row-major
     bool prepared = false, have_null = false, have_se = false;
@@ -1055,12 +1058,12 @@ int cs_mvlmm_create(const double* Y, int64_t ldy, int64_t n, int64_t d, const fl
             std::vector<double> Xd((size_t)(n * c));
             for (int64_t j = 0; j < c; ++j)
                 for (int64_t i = 0; i < n; ++i) Xd[(size_t)(j * n + i)] = (double)X[j * ldx + i];
-            h->Q.alloc((size_t)(n * n) * 8);
-            DevBuf tau((size_t)c * 8);
+            h->Q.alloc((size_t)(n * c) * 8);
+            h->tau.alloc((size_t)c * 8);
             CS_HIP(hipMemcpyAsync(h->Q.p, Xd.data(), Xd.size() * 8, hipMemcpyHostToDevice, h->st));
-            CS_MVBLAS(rocsolver_dgeqrf(h->blas, (int)n, (int)c, h->Q.as<double>(), (int)n, tau.as<double>()));
-            CS_MVBLAS(rocsolver_dorgqr(h->blas, (int)n, (int)n, (int)c, h->Q.as<double>(), (int)n,
-                                       tau.as<double>()));
+            CS_MVBLAS(rocsolver_dgeqrf(h->blas, (int)n, (int)c, h->Q.as<double>(), (int)n, h->tau.as<double>()));
+            h->tau_h.resize((size_t)c);
+            CS_HIP(hipMemcpyAsync(h->tau_h.data(), h->tau.p, (size_t)c * 8, hipMemcpyDeviceToHost, h->st));
         }
         CS_HIP(hipStreamSynchronize(h->st));
         *out = h.release();
@@ -1077,39 +1080,28 @@ int cs_mvlmm_prepare_dense(cs_mvlmm* h, const double* G, int64_t ldg, int g_on_d
         const int64_t n = h->n, ne = h->ne, c = h->c, d = h->d;
         if (ldg < n) throw RuntimeError("MvEigenVce::prepare: Dimension mismatch");
         const double one = 1.0, zero = 0.0;
-        const double* Gd = G;
-        if (!g_on_device) {
-            h->G.ensure((size_t)(n * n) * 8);
-            CS_HIP(hipMemcpy2DAsync(h->G.p, (size_t)n * 8, G, (size_t)ldg * 8, (size_t)n * 8, (size_t)n,
-                                    hipMemcpyHostToDevice, h->st));
-            Gd = h->G.as<double>();
-            ldg = n;
-        }
-        h->U.ensure((size_t)(ne * ne) * 8);
+        // working copy of G (n x n, ld n): projected in place, then overwritten by the eigenvectors
+        h->G.ensure((size_t)(n * n) * 8);
+        double* Gw = h->G.as<double>();
+        CS_HIP(hipMemcpy2DAsync(Gw, (size_t)n * 8, G, (size_t)ldg * 8, (size_t)n * 8, (size_t)n,
+                                g_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->st));
         h->D.ensure((size_t)ne * 8);
         h->E.ensure((size_t)ne * 8);
-        h->t.ensure((size_t)(ne * d) * 8);
+        h->t.ensure((size_t)(n * d) * 8);
         h->Uty.ensure((size_t)(ne * d) * 8);
         h->W.ensure((size_t)(n * ne) * 8);
-        double* U = h->U.as<double>();
+        CS_HIP(hipMemcpyAsync(h->t.p, h->Y.p, (size_t)(n * d) * 8, hipMemcpyDeviceToDevice, h->st));
         if (c > 0) {
-            const double* A = h->Q.as<double>() + c * n;
-            h->T.ensure((size_t)(n * ne) * 8);
-            CS_MVBLAS(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)ne,
-                                    (int)n, &one, Gd, (int)ldg, A, (int)n, &zero, h->T.as<double>(), (int)n));
-            CS_MVBLAS(rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, (int)ne,
-                                    (int)ne, (int)n, &one, A, (int)n, h->T.as<double>(), (int)n, &zero, U,
-                                    (int)ne));
-            CS_MVBLAS(rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, (int)ne,
-                                    (int)d, (int)n, &one, A, (int)n, h->Y.as<double>(), (int)n, &zero,
-                                    h->t.as<double>(), (int)ne));
-        } else {
-            CS_HIP(hipMemcpy2DAsync(U, (size_t)n * 8, Gd, (size_t)ldg * 8, (size_t)n * 8, (size_t)n,
-                                    hipMemcpyDeviceToDevice, h->st));
-            CS_HIP(hipMemcpyAsync(h->t.p, h->Y.p, (size_t)(n * d) * 8, hipMemcpyDeviceToDevice, h->st));
+            // A = Q[:, c:] for Q = H_1 ... H_c, the Householder reflectors of X: A^T G A in place
+            householder_project_sym(h->blas, h->st, h->Q.as<double>(), h->tau_h.data(), n, c, Gw);
+            // t = Q^T Y (rows c: are A^T Y)
+            CS_MVBLAS(rocsolver_dormqr(h->blas, rocblas_side_left, rocblas_operation_transpose, (int)n, (int)d,
+                                       (int)c, h->Q.as<double>(), (int)n, h->tau.as<double>(), h->t.as<double>(),
+                                       (int)n));
         }
-        h->G.release();
-        CS_MVBLAS(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)ne, U, (int)ne,
+        // eigendecomposition of A^T G A (ne x ne at offset (c, c), ld n); U overwrites it
+        double* U = Gw + c * n + c;
+        CS_MVBLAS(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)ne, U, (int)n,
                                    h->D.as<double>(), h->E.as<double>(), h->info.as<rocblas_int>()));
         std::vector<double> Dh((size_t)ne);
         CS_HIP(hipMemcpyAsync(Dh.data(), h->D.p, (size_t)ne * 8, hipMemcpyDeviceToHost, h->st));
@@ -1117,20 +1109,22 @@ int cs_mvlmm_prepare_dense(cs_mvlmm* h, const double* G, int64_t ldg, int g_on_d
         for (double& x : Dh)
             if (x < 0) x = 0;
         CS_HIP(hipMemcpyAsync(h->D.p, Dh.data(), (size_t)ne * 8, hipMemcpyHostToDevice, h->st));
-        // Uty = U^T (A^T Y) (ne x d); W_proj = A U (n x ne)
+        // Uty = U^T (A^T Y) (ne x d)
         CS_MVBLAS(rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, (int)ne, (int)d,
-                                (int)ne, &one, U, (int)ne, h->t.as<double>(), (int)ne, &zero, h->Uty.as<double>(),
+                                (int)ne, &one, U, (int)n, h->t.as<double>() + c, (int)n, &zero, h->Uty.as<double>(),
                                 (int)ne));
+        // W_proj = A U = Q [0; U] (n x ne)
+        double* W = h->W.as<double>();
+        if (c > 0) CS_HIP(hipMemset2DAsync(W, (size_t)n * 8, 0, (size_t)c * 8, (size_t)ne, h->st));
+        CS_HIP(hipMemcpy2DAsync(W + c, (size_t)n * 8, U, (size_t)n * 8, (size_t)ne * 8, (size_t)ne,
+                                hipMemcpyDeviceToDevice, h->st));
         if (c > 0)
-            CS_MVBLAS(rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, (int)n, (int)ne,
-                                    (int)ne, &one, h->Q.as<double>() + c * n, (int)n, U, (int)ne, &zero,
-                                    h->W.as<double>(), (int)n));
-        else
-            CS_HIP(hipMemcpyAsync(h->W.p, U, (size_t)(n * n) * 8, hipMemcpyDeviceToDevice, h->st));
+            CS_MVBLAS(rocsolver_dormqr(h->blas, rocblas_side_left, rocblas_operation_none, (int)n, (int)ne, (int)c,
+                                       h->Q.as<double>(), (int)n, h->tau.as<double>(), W, (int)n));
         rocblas_int info = 0;
         CS_HIP(hipMemcpyAsync(&info, h->info.p, sizeof(info), hipMemcpyDeviceToHost, h->st));
         CS_HIP(hipStreamSynchronize(h->st));
-        h->T.release();
+        h->G.release();
         if (info != 0) throw RuntimeError("MvEigenVce::prepare: eigendecomposition did not converge");
         // start values (eigen_vce.cpp:541-545): Cov(Y) with the divisor of the projected size
         const double nn = (double)(c > 0 ? ne : n);

@@ -324,10 +324,10 @@ int cs_server_release(cs_server* srv, int64_t id);
  *
  * cs_lmm_create: y (n) and covariates X (n x c, float32 column-major with leading dimension ldx,
  * intercept included -- univariate_model.cpp:368-372) over the kept samples.  With c > 0 the
- * covariates are projected out (EigenVce with use_covariate_projection, the default): the full
- * Householder Q of X is formed once and reused by every prepare.  n <= 32766: rocSOLVER's
- * dorgqr / dsyevd have only 32-bit interfaces, and larger n x n work arrays exceed them
- * (CS_ERR_INVALID_ARGUMENT). */
+ * covariates are projected out (EigenVce with use_covariate_projection, the default): the
+ * Householder QR of X is computed once, and every prepare applies its c reflectors to G.
+ * n <= 32766: rocSOLVER's dsyevd has only 32-bit interfaces, and larger n x n work arrays
+ * exceed them (CS_ERR_INVALID_ARGUMENT). */
 int cs_lmm_create(const double* y, const float* X, int64_t ldx, int64_t n, int64_t c, cs_lmm** out);
 int cs_lmm_destroy(cs_lmm* lmm);
 /* EigenVce::prepare (libCosmicCore/src/eigen_vce.cpp:19-49) for an n x n GRM G over the kept
@@ -418,7 +418,7 @@ int cs_gwas_matrix_free_run(const char* bfile, const double* y, const double* co
  * projection (the defaults of MultivariateModel, src/models/multivariate_model.cpp:870-1110).
  *
  * cs_mvlmm_create: Y (n x d, column-major, ldy) and covariates X (n x c float32, intercept
- * included) over the kept samples; the full Householder Q of X is formed once.  n <= 32766, as
+ * included) over the kept samples; the Householder QR of X is computed once.  n <= 32766, as
  * cs_lmm_create. */
 int cs_mvlmm_create(const double* Y, int64_t ldy, int64_t n, int64_t d, const float* X, int64_t ldx,
                     int64_t c, cs_mvlmm** out);
