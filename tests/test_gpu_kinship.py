@@ -177,3 +177,38 @@ def test_sparse_and_dense_missing_forms_bitwise(csa, miss):
     means, inv = oracle.stats(body, N, M)
     ref = oracle.kinship_apply(body, N, means, inv, keep, variants, X)
     assert colrel(outs[1], ref) < 2e-6
+
+
+def test_register_kernels_match_lds_kernels(csa, tmp_path):
+    """CSAMD_GENO_LDSB=0 selects the register-only kernels (no LDS sharing, two groups per wave
+    for several groups, no limb packing).  The integer products are exact, so every column must
+    be bitwise identical to the default LDS kernels' (run in a child process: the switch is read
+    once per process)."""
+    import os
+    import subprocess
+    import sys
+    N, M = 900, 2100
+    script = f"""
+import sys, numpy as np
+sys.path.insert(0, {repr(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))})
+import oracle, cosmicscanner_amd as csa
+body = oracle.gen_hwe_bed(79, {N}, {M}, 0.01, 0.5, 0.01)
+r = csa.PlinkReader.from_body(body, {N}, {M})
+keep = np.random.default_rng(3).choice({N}, 800, replace=True).astype(np.int32)
+op = csa.PlinkKinshipOperator(r, keep, None)
+X = np.random.default_rng(4).standard_normal((800, 41))
+out = {{}}
+for mode in (3, 4):
+    op.set_precision(mode)
+    for k in (1, 16, 41):
+        out[f"m{{mode}}k{{k}}"] = op.apply(X[:, :k])
+np.savez(sys.argv[1], **out)
+"""
+    ref_path, reg_path = str(tmp_path / "lds.npz"), str(tmp_path / "reg.npz")
+    print("child processes: default kernels, then CSAMD_GENO_LDSB=0", flush=True)
+    subprocess.run([sys.executable, "-c", script, ref_path], check=True, timeout=120)
+    env = dict(os.environ, CSAMD_GENO_LDSB="0")
+    subprocess.run([sys.executable, "-c", script, reg_path], check=True, timeout=120, env=env)
+    a, b = np.load(ref_path), np.load(reg_path)
+    for key in a.files:
+        assert np.array_equal(a[key], b[key]), key
