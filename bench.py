@@ -330,7 +330,9 @@ def mv_leg(csa, n=32_000, m=300_000, d=4, seed=20261207, n_check=32):
 def serving_leg(csa, n=100_000, m=1_000_000, n_requests=12, rate=5.0, seed=20261206):
     """Poisson stream of scan requests (rate req/s) over a resident n x m matrix; each request: a
     random cohort (log-uniform 10k..n), 1-64 traits, 0-10 covariates (+ intercept), h2 ~
-    U(0.1, 0.8), 16 calibration SNPs, a scan of all m SNPs.  Latency = completion - arrival."""
+    U(0.1, 0.8), 16 calibration SNPs, a scan of all m SNPs.  Latency = arrival to the client
+    holding the full results (freq, n_obs, pass, beta, SE, P, log10 P of every SNP and trait) on
+    the host; the time to completion on the device is reported beside it."""
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     reader = csa.PlinkReader.synthetic(seed, n, m, 0.05, 0.5, 0.005)
@@ -348,7 +350,7 @@ def serving_leg(csa, n=100_000, m=1_000_000, n_requests=12, rate=5.0, seed=20261
         h2 = float(rng.uniform(0.1, 0.8))
         cal = np.sort(rng.choice(m, 16, replace=False)).astype(np.int32)
         reqs.append(dict(Y=Y, X=X, sigma2_g=h2, sigma2_e=1.0 - h2, cal_snps=cal, keep=keep))
-    ids, submitted, done = {}, 0, {}
+    ids, submitted, done, done_dev = {}, 0, {}, {}
     genos = 0.0
     start = time.perf_counter()
     while len(done) < n_requests:
@@ -363,17 +365,23 @@ def serving_leg(csa, n=100_000, m=1_000_000, n_requests=12, rate=5.0, seed=20261
                 continue
             st, info = srv.status(rid)
             if st == 2:
+                done_dev[rid] = time.perf_counter() - start - arrivals[i]
+                res = srv.result(rid)
                 done[rid] = time.perf_counter() - start - arrivals[i]
+                assert res["log10p"].shape == (m, reqs[i]["Y"].shape[1])
+                del res
                 genos += reqs[i]["keep"].size * m * reqs[i]["Y"].shape[1]
                 srv.release(rid)
             elif st == -1:
                 raise RuntimeError(f"request {i} failed")
     wall = time.perf_counter() - start
     lat = np.array(sorted(done.values()))
+    lat_dev = np.array(sorted(done_dev.values()))
     srv.close()
     reader.close()
     return dict(workload=f"n={n} x m={m} resident, {n_requests} requests, Poisson {rate} req/s",
                 p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                p50_device_done_s=float(np.percentile(lat_dev, 50)),
                 wall_s=wall, trait_genotypes_per_s=genos / wall, setup_s=setup)
 
 

@@ -85,6 +85,7 @@ struct ScanOut {
     int64_t* n_obs = nullptr;
     int32_t* pass = nullptr;
     double *beta = nullptr, *se = nullptr, *p = nullptr, *log10p = nullptr;
+    bool device = false;   // the output pointers are device memory (no host copy)
 };
 void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const ScanOut& o);
 // Projected scan stages (cs_pscan.hip); need rocblas.h in the including file for the handle.
@@ -95,7 +96,8 @@ void pscan_prepare_rhs(cs_bed* bed, const int32_t* keep, int64_t n_keep, int64_t
 void pscan_finish(cs_bed* bed, const int32_t* keep, int64_t n_keep, int64_t n, int64_t T, int64_t c, int64_t L,
                   const double* B, const double* Z, const int32_t* snps, int64_t n_snps, double min_maf, int prec,
                   double* freq, int64_t* n_obs, int32_t* pass, double* beta, double* se, double* p, double* log10p,
-                  double* gamma_out, int64_t* ncal_out, hipStream_t st, rocblas_handle blas);
+                  double* gamma_out, int64_t* ncal_out, hipStream_t st, rocblas_handle blas,
+                  bool device_out = false);
 #endif
 // Prepared markers of `snps` over the keep list (NULL = all samples) into dG (n x ns, device).
 void markers_device(cs_bed* b, const int32_t* keep, int64_t n_keep, const int32_t* snps, int64_t ns,

@@ -103,7 +103,7 @@ void pscan_prepare_rhs(cs_bed* bed, const int32_t* keep, int64_t n_keep, int64_t
 void pscan_finish(cs_bed* bed, const int32_t* keep, int64_t n_keep, int64_t n, int64_t T, int64_t c, int64_t L,
                   const double* B, const double* Z, const int32_t* snps, int64_t n_snps, double min_maf, int prec,
                   double* freq, int64_t* n_obs, int32_t* pass, double* beta, double* se, double* p, double* log10p,
-                  double* gamma_out, int64_t* ncal_out, hipStream_t st, rocblas_handle blas) {
+                  double* gamma_out, int64_t* ncal_out, hipStream_t st, rocblas_handle blas, bool device_out) {
     const int64_t k = T + c + L;
     const double one = 1.0, mone = -1.0;
     const double* dX = B + T * n;
@@ -176,6 +176,7 @@ void pscan_finish(cs_bed* bed, const int32_t* keep, int64_t n_keep, int64_t n, i
     ScanOut o;
     o.freq = freq; o.n_obs = n_obs; o.pass = pass;
     o.beta = beta; o.se = se; o.p = p; o.log10p = log10p;
+    o.device = device_out;
     run_scan(bed, a, &m, o);
     if (gamma_out) *gamma_out = gamma;
     if (ncal_out) *ncal_out = gcnt;

@@ -234,7 +234,9 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
             CS_HIP(hipGetLastError());
         }
         auto d2h = [&](void* dst, const DevBuf& src, size_t bytes) {
-            if (dst) CS_HIP(hipMemcpyAsync(dst, src.p, bytes, hipMemcpyDeviceToHost, st));
+            if (dst)
+                CS_HIP(hipMemcpyAsync(dst, src.p, bytes, o.device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                      st));
         };
         d2h(o.freq, dfreq, (size_t)ns * 8);
         d2h(o.n_obs, dnobs, (size_t)ns * 8);

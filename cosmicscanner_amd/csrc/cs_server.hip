@@ -164,10 +164,9 @@ struct Request {
     int iterations = 0;
     int64_t bytes = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
-    // results
-    std::vector<double> freq, beta, se_out, p, log10p;
-    std::vector<int64_t> n_obs;
-    std::vector<int32_t> pass;
+    // results, in device memory until cs_server_result copies them out (or the request is released)
+    int64_t ns = 0;
+    DevBuf freq, n_obs, pass, beta, se_out, p, log10p;
     double gamma = 0;
     int64_t ncal = 0;
     std::string error;
@@ -293,17 +292,18 @@ void finish(cs_server* s, Request& q) {
     CS_HIP(hipGetLastError());
     CS_HIP(hipStreamSynchronize(s->st));
     const int64_t ns = q.snps.empty() ? s->bed->M : (int64_t)q.snps.size();
-    q.freq.resize((size_t)ns);
-    q.n_obs.resize((size_t)ns);
-    q.pass.resize((size_t)ns);
-    q.beta.resize((size_t)(ns * q.T));
-    q.se_out.resize((size_t)(ns * q.T));
-    q.p.resize((size_t)(ns * q.T));
-    q.log10p.resize((size_t)(ns * q.T));
+    q.ns = ns;
+    q.freq.alloc((size_t)ns * 8);
+    q.n_obs.alloc((size_t)ns * 8);
+    q.pass.alloc((size_t)ns * 4);
+    q.beta.alloc((size_t)(ns * q.T) * 8);
+    q.se_out.alloc((size_t)(ns * q.T) * 8);
+    q.p.alloc((size_t)(ns * q.T) * 8);
+    q.log10p.alloc((size_t)(ns * q.T) * 8);
     pscan_finish(s->bed, q.keep.data(), n, n, q.T, q.c, q.L, q.B.as<double>(), Zc.as<double>(),
-                 q.snps.empty() ? nullptr : q.snps.data(), ns, q.min_maf, q.prec, q.freq.data(), q.n_obs.data(),
-                 q.pass.data(), q.beta.data(), q.se_out.data(), q.p.data(), q.log10p.data(), &q.gamma, &q.ncal, s->st,
-                 s->blas);
+                 q.snps.empty() ? nullptr : q.snps.data(), ns, q.min_maf, q.prec, q.freq.as<double>(),
+                 q.n_obs.as<int64_t>(), q.pass.as<int32_t>(), q.beta.as<double>(), q.se_out.as<double>(),
+                 q.p.as<double>(), q.log10p.as<double>(), &q.gamma, &q.ncal, s->st, s->blas, true);
     // free the slots (P zeroed so idle columns add nothing to the shared apply)
     CS_HIP(hipMemsetAsync(s->P.as<double>() + q.slot0 * N, 0, (size_t)(N * k) * 8, s->st));
     CS_HIP(hipMemsetAsync(s->minv.as<double>() + q.slot0 * N, 0, (size_t)(N * k) * 8, s->st));
@@ -520,16 +520,18 @@ int cs_server_result(cs_server* s, int64_t id, int32_t* status, double* freq, in
         const bool wants = freq || n_obs || pass || beta || se || p || log10p;
         if (q.status == REQ_FAILED && wants) throw RuntimeError("request failed: " + q.error);
         if (q.status != REQ_DONE) return;
-        auto cp = [](auto* dst, const auto& v) {
-            if (dst) std::copy(v.begin(), v.end(), dst);
+        auto cp = [&](void* dst, const DevBuf& v, size_t bytes) {
+            if (dst && bytes) CS_HIP(hipMemcpyAsync(dst, v.p, bytes, hipMemcpyDeviceToHost, s->st));
         };
-        cp(freq, q.freq);
-        cp(n_obs, q.n_obs);
-        cp(pass, q.pass);
-        cp(beta, q.beta);
-        cp(se, q.se_out);
-        cp(p, q.p);
-        cp(log10p, q.log10p);
+        const size_t ns = (size_t)q.ns, nt = (size_t)(q.ns * q.T);
+        cp(freq, q.freq, ns * 8);
+        cp(n_obs, q.n_obs, ns * 8);
+        cp(pass, q.pass, ns * 4);
+        cp(beta, q.beta, nt * 8);
+        cp(se, q.se_out, nt * 8);
+        cp(p, q.p, nt * 8);
+        cp(log10p, q.log10p, nt * 8);
+        CS_HIP(hipStreamSynchronize(s->st));
     });
 }
 

@@ -313,7 +313,8 @@ int cs_server_submit(cs_server* srv, const cs_request* req, int64_t* id);
 int cs_server_step(cs_server* srv, int32_t* n_active, int32_t* n_pending, int32_t* n_completed);
 /* status: 0 pending, 1 active, 2 done, -1 failed.  Asking for outputs of a failed request gives
  * CS_ERR_RUNTIME with its message.  When done, outputs as cs_pcg_projected_scan (any may be
- * NULL). */
+ * NULL); a finished request keeps them in device memory (n_snps x (4 T + 3) x 8 bytes) until
+ * cs_server_release, and each call copies the asked-for arrays to the host buffers given. */
 int cs_server_result(cs_server* srv, int64_t id, int32_t* status, double* freq, int64_t* n_obs,
                      int32_t* pass, double* beta, double* se, double* p, double* log10p,
                      cs_request_stats* stats);
