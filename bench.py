@@ -47,11 +47,11 @@ def profiled_traffic(key):
     return None
 
 
-def cpu_baseline(N, M_slice, k, seed):
+def cpu_baseline(N, M_slice, k, seed, miss=0.005):
     """Oracle K.V on a slice of the same workload: all N samples, the first M_slice SNPs."""
     import oracle
     threads = min(os.cpu_count() or 1, 16)
-    body = oracle.gen_hwe_bed(seed, N, M_slice, 0.01, 0.5, 0.005, nthreads=threads)
+    body = oracle.gen_hwe_bed(seed, N, M_slice, 0.01, 0.5, miss, nthreads=threads)
     means, inv = oracle.stats(body, N, M_slice, nthreads=threads)
     X = np.random.default_rng(1).standard_normal((N, k))
     t0 = time.perf_counter()
@@ -394,6 +394,10 @@ def main():
     ap.add_argument("--m", type=int, default=500_000)
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--prec", choices=["int8_l3", "int8_l2"], default="int8_l3")
+    ap.add_argument("--miss", type=float, default=0.005,
+                    help="missing-genotype rate of the synthetic K.V genotypes (default 0.5 %%)")
+    ap.add_argument("--missing-mode", type=int, choices=[0, 1, 2], default=0,
+                    help="K.V missing-genotype form: 0 auto, 1 dense O products, 2 sparse missing lists")
     ap.add_argument("--cpu-slice-snps", type=int, default=250_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -432,10 +436,11 @@ def main():
     N, M, k = args.n, args.m, args.k
     seed = 20261204
     t0 = time.perf_counter()
-    reader = csa.PlinkReader.synthetic(seed, N, M, 0.01, 0.5, 0.005)
+    reader = csa.PlinkReader.synthetic(seed, N, M, 0.01, 0.5, args.miss)
     op = csa.PlinkKinshipOperator(reader)
     log(f"resident genotypes {reader.device_bytes() / 2**30:.2f} GiB, setup {time.perf_counter() - t0:.1f} s")
     op.set_precision(csa.CS_PREC_INT8_L3 if args.prec == "int8_l3" else csa.CS_PREC_INT8_L2)
+    op.set_missing_mode(args.missing_mode)
 
     X = csa.DeviceArray.from_host(np.random.default_rng(7).standard_normal((N, k)))
     Y = csa.DeviceArray(N, k)
@@ -556,7 +561,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(N, args.cpu_slice_snps, k, seed)
+        cpu = cpu_baseline(N, args.cpu_slice_snps, k, seed, args.miss)
         log(f"cpu baseline: {cpu['value']:.3e} genotypes/s on {cpu['cores']} threads")
 
     out = {
@@ -572,7 +577,7 @@ def main():
         "vs_baseline": None,
         "dtype": "int8 MFMA (exact genotypes, 3x8-bit fixed-point RHS limbs), int32 acc, fp64 epilogue"
         if args.prec == "int8_l3" else "int8 MFMA (2x8-bit limbs), int32 acc, fp64 epilogue",
-        "data": "synthetic HWE genotypes generated on device (p~U(0.01,0.5), 0.5% missing), random RHS",
+        "data": f"synthetic HWE genotypes generated on device (p~U(0.01,0.5), {100 * args.miss:g}% missing), random RHS",
         "config": {"workload": "GRM build + matrix-free K.v, n=50,000 x m=500,000 (K.V apply, k=16)",
                    "n": N, "m": M, "k": k, "precision": args.prec},
         "host_buffer_genotypes_per_s": host_rate,

@@ -467,6 +467,37 @@ static void build_lists(const uint8_t* A, int64_t KB, int64_t rows, int64_t limi
     CS_HIP(hipStreamSynchronize(st));
 }
 
+// Sum of the per-SNP missing counts (class 01 of the M x 4 counts), one int64 atomic per block.
+__global__ __launch_bounds__(256) void k_sum_missing(const int64_t* __restrict__ counts, int64_t M,
+                                                     unsigned long long* __restrict__ total) {
+    __shared__ long long red[256];
+    long long acc = 0;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < M; j += (int64_t)gridDim.x * blockDim.x)
+        acc += counts[j * 4 + 1];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(total, (unsigned long long)red[0]);
+}
+
+int64_t missing_total(cs_bed* b) {
+    if (b->miss_total >= 0) return b->miss_total;
+    ensure_stats(b);
+    DevBuf t(8);
+    CS_HIP(hipMemsetAsync(t.p, 0, 8, b->stream));
+    const unsigned nb = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (b->M + 255) / 256));
+    k_sum_missing<<<nb, 256, 0, b->stream>>>(b->counts.as<int64_t>(), b->M, t.as<unsigned long long>());
+    CS_HIP(hipGetLastError());
+    int64_t h = 0;
+    CS_HIP(hipMemcpyAsync(&h, t.p, 8, hipMemcpyDeviceToHost, b->stream));
+    CS_HIP(hipStreamSynchronize(b->stream));
+    b->miss_total = h;
+    return h;
+}
+
 void ensure_missing_lists(cs_bed* b) {
     if (b->miss_nnz >= 0) return;
     ensure_sample_major(b);

@@ -120,5 +120,6 @@ struct cs_bed {
     // the missing SNPs, both in increasing order; real rows and columns only.
     csamd::DevBuf miss_snp_off, miss_snp_idx, miss_smp_off, miss_smp_idx;
     int64_t miss_nnz = -1;     // -1: lists not built
+    int64_t miss_total = -1;   // missing genotypes from the class counts; -1: not yet summed
     hipStream_t stream = nullptr;
 };

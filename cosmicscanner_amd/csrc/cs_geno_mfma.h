@@ -142,9 +142,15 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
                 accX[m][j][l] = v4i{0, 0, 0, 0};
                 accO[m][j][l] = v4i{0, 0, 0, 0};
             }
+    // Group validity.  The LDS kernels (one group per workgroup) are launched with z = G or the
+    // flat mapping, so their group is always valid.  A compile-time true keeps the MFMA blocks
+    // of the m-subtiles in one basic block, which the compiler interleaves; it is used where the
+    // interleaved schedule still fits the register budget (the one-operand two-product form at
+    // three waves per SIMD would spill).
+    constexpr bool kAlwaysValid = PK || (LDSB && (TWO_B || ONLY_X));
     bool gok[GZ];
 #pragma unroll
-    for (int j = 0; j < GZ; ++j) gok[j] = PK || gz0 + j < a.G;   // packed: one group, always valid
+    for (int j = 0; j < GZ; ++j) gok[j] = kAlwaysValid || gz0 + j < a.G;
 
     if (kb0 < kb1) {
         const uint8_t* abase = a.A + ((tr * a.KB) << 12) + m0 * 1024 + lane * 16;
@@ -356,6 +362,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PK ? 4 : 3,
 k_geno1s(GenoArgs a) {
     __shared__ typename std::conditional<PK, v4i, uint4>::type ldsb[2 * GenoLds<PK ? 1 : L, false>::kChunks];
     geno_body<L, false, EPI, false, 4, 1, true, PK>(a, ldsb);
+}
+// Genotype-only product with the B fragments shared through LDS: one accumulator set of
+// 4 x L tiles, four waves per SIMD.
+template <int L, int EPI>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_genox_s(GenoArgs a) {
+    __shared__ uint4 ldsb[2 * GenoLds<L, false>::kChunks];
+    geno_body<L, false, EPI, true, 4, 1, true>(a, ldsb);
 }
 // Four limbs (two accumulator sets of 4 x 4 tiles) fit only at two waves per SIMD.
 template <int L, int EPI>

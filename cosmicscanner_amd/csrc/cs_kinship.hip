@@ -121,48 +121,150 @@ __global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, i
     }
 }
 
-// Exact column sums of a plain int32 [rows][K16] operand: one block per column, fixed order.
+// Exact column sums of a plain int32 [rows][K16] operand into S (zeroed by the caller): block
+// (x, y) sums a stride of the rows for columns 16 y .. 16 y + 15 (16 threads read one 64-byte row
+// piece) and adds its partial sums with one int64 atomic per column (integer addition, so the
+// result does not depend on the order).
 __global__ __launch_bounds__(256) void k_colsum_plain(const int32_t* __restrict__ P, int64_t rows, int64_t K16,
-                                                      long long* __restrict__ S) {
+                                                      unsigned long long* __restrict__ S) {
     __shared__ long long red[256];
-    const int64_t c = blockIdx.x;
+    const int c = threadIdx.x & 15, rr = threadIdx.x >> 4;
+    const int64_t col = (int64_t)blockIdx.y * 16 + c;
     long long acc = 0;
-    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) acc += P[r * K16 + c];
+    for (int64_t r = (int64_t)blockIdx.x * 16 + rr; r < rows; r += (int64_t)gridDim.x * 16) acc += P[r * K16 + col];
     red[threadIdx.x] = acc;
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (int off = 128; off >= 16; off >>= 1) {
         if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
-    if (threadIdx.x == 0) S[c] = red[0];
+    if (threadIdx.x < 16) atomicAdd(S + col, (unsigned long long)red[threadIdx.x]);
+}
+
+static void colsum_plain(const int32_t* P, int64_t rows, int64_t K16, long long* S, hipStream_t st) {
+    CS_HIP(hipMemsetAsync(S, 0, (size_t)K16 * 8, st));
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(1024, (rows + 15) / 16));
+    k_colsum_plain<<<dim3(gx, (unsigned)(K16 / 16)), 256, 0, st>>>(P, rows, K16,
+                                                                   reinterpret_cast<unsigned long long*>(S));
+    CS_HIP(hipGetLastError());
+}
+
+// Sum over the entries of one missing list of the rows of a plain int32 [.][K16] operand
+// (exact, int64).  One wave per list row; lane = e CW + cq
+// holds column quad `quad` (16 bytes) of list entry e of each batch of E = 64 / CW entries.  A
+// step gathers U batches and, before adding them, issues the index loads of the next U batches,
+// so a wave keeps U E row gathers and U index loads in flight.  The indices are read once
+// (non-temporal); the operand rows are cache hits.  The sums are reduced over e (valid in every
+// lane).
+template <int U, int CW>
+__device__ __forceinline__ void list_row_sums(const int32_t* __restrict__ idx, int64_t p0, int64_t p1,
+                                              const int4* __restrict__ P4, uint32_t q4, int64_t quad, int e,
+                                              long long acc[4]) {
+    constexpr int E = 64 / CW;
+    acc[0] = acc[1] = acc[2] = acc[3] = 0;
+    if (p1 == p0) return;   // empty row (p0, p1 are the same in every lane): nothing to reduce
+    const int4* col = P4 + quad;
+    const int32_t* ip = idx + p0 + e;
+    int64_t left = p1 - p0 - e;   // entries of this lane's stride still to read, from ip
+    int id[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) id[u] = E * u < left ? __builtin_nontemporal_load(ip + E * u) : -1;
+    while (left > 0) {
+        int4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = id[u] >= 0 ? col[(uint32_t)id[u] * q4] : make_int4(0, 0, 0, 0);
+        ip += E * U;
+        left -= E * U;
+#pragma unroll
+        for (int u = 0; u < U; ++u) id[u] = E * u < left ? __builtin_nontemporal_load(ip + E * u) : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            acc[0] += v[u].x;
+            acc[1] += v[u].y;
+            acc[2] += v[u].z;
+            acc[3] += v[u].w;
+        }
+    }
+    for (int s = CW; s < 64; s <<= 1)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] += __shfl_xor(acc[t], s);
+}
+
+// Row offsets of a block of R < 64 list rows starting at j0, loaded by one instruction (lane
+// l <= R holds off[min(j0 + l, rows)]); row r of the block spans [lane r, lane r + 1).
+template <int R>
+__device__ __forceinline__ int64_t block_offsets(const int64_t* __restrict__ off, int64_t j0, int64_t rows) {
+    static_assert(R > 0 && R < 64, "the end of the block sits in lane R");
+    const int lane = threadIdx.x & 63;
+    const int64_t j = j0 + lane;
+    return lane <= R ? off[j < rows ? j : rows] : 0;
 }
 
 // Observed-indicator product of the SNP rows from the missing lists:
 // qext[j][c] = sum_i o_ij Vq_i[c] = S[c] - sum_{i missing at j} Vq_i[c]   (exact integers).
-__global__ void k_sparse_q(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
-                           const int32_t* __restrict__ Vp, int64_t K16, const long long* __restrict__ S,
-                           int64_t rows, long long* __restrict__ qext) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < rows * K16;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t j = e / K16, c = e - j * K16;
-        long long acc = 0;
-        for (int64_t p = off[j]; p < off[j + 1]; ++p) acc += Vp[(int64_t)idx[p] * K16 + c];
-        qext[e] = S[c] - acc;
+// CW: a power of two dividing K16 / 4, at most 64.  Each wave takes blocks of R rows.
+template <int CW, int R>
+__global__ __launch_bounds__(256) void k_sparse_q(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
+                                                  const int32_t* __restrict__ Vp, int64_t K16,
+                                                  const long long* __restrict__ S, int64_t rows,
+                                                  long long* __restrict__ qext) {
+    const int lane = threadIdx.x & 63, e = lane / CW, cq = lane - e * CW;
+    const int64_t q4 = K16 / 4;
+    const int4* P4 = reinterpret_cast<const int4*>(Vp);
+    for (int64_t j0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * R; j0 < rows; j0 += (int64_t)gridDim.x * 4 * R) {
+        const int64_t o = block_offsets<R>(off, j0, rows);
+        const int nr = (int)(rows - j0 < R ? rows - j0 : R);
+        for (int r = 0; r < nr; ++r) {
+            const int64_t j = j0 + r, p0 = __shfl(o, r), p1 = __shfl(o, r + 1);
+            for (int64_t c0 = 0; c0 < q4; c0 += CW) {
+                const int64_t quad = c0 + cq;
+                long long acc[4];
+                list_row_sums<8, CW>(idx, p0, p1, P4, (uint32_t)q4, quad, e, acc);
+                if (e == 0)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) qext[j * K16 + 4 * quad + t] = S[4 * quad + t] - acc[t];
+            }
+        }
     }
 }
 
 // Z W with the sparse observed-indicator form: yint[i][c] (= X~ W1 from the MFMA kernel) minus
 // sum_j o_ij (mu W)_j[c] = S2[c] - sum_{j missing at i} (mu W)_j[c].
-__global__ void k_sparse_yadd(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
-                              const int32_t* __restrict__ B2p, int64_t K16, const long long* __restrict__ S2,
-                              int64_t rows, int64_t k, long long* __restrict__ yint) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < rows * k;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = e / k, c = e - i * k;
-        long long acc = 0;
-        for (int64_t p = off[i]; p < off[i + 1]; ++p) acc += B2p[(int64_t)idx[p] * K16 + c];
-        yint[e] += acc - S2[c];
+template <int CW, int R>
+__global__ __launch_bounds__(256) void k_sparse_yadd(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
+                                                     const int32_t* __restrict__ B2p, int64_t K16,
+                                                     const long long* __restrict__ S2, int64_t rows, int64_t k,
+                                                     long long* __restrict__ yint) {
+    const int lane = threadIdx.x & 63, e = lane / CW, cq = lane - e * CW;
+    const int64_t q4 = K16 / 4;
+    const int4* P4 = reinterpret_cast<const int4*>(B2p);
+    for (int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * R; i0 < rows; i0 += (int64_t)gridDim.x * 4 * R) {
+        const int64_t o = block_offsets<R>(off, i0, rows);
+        const int nr = (int)(rows - i0 < R ? rows - i0 : R);
+        for (int r = 0; r < nr; ++r) {
+            const int64_t i = i0 + r, p0 = __shfl(o, r), p1 = __shfl(o, r + 1);
+            for (int64_t c0 = 0; c0 < q4; c0 += CW) {
+                const int64_t quad = c0 + cq;
+                long long acc[4];
+                list_row_sums<8, CW>(idx, p0, p1, P4, (uint32_t)q4, quad, e, acc);
+                if (e == 0)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        const int64_t c = 4 * quad + t;
+                        if (c < k) yint[i * k + c] += acc[t] - S2[c];
+                    }
+            }
+        }
     }
+}
+
+// Column quads handled together by one wave of the list gathers: the largest power of two that
+// divides K16 / 4, at most 64.
+static int list_quads(int64_t K16) {
+    const int64_t q4 = K16 / 4;
+    int cw = 1;
+    while (cw < 64 && q4 % (2 * cw) == 0) cw *= 2;
+    return cw;
 }
 
 // y = scale_c * yint[keep[r]][c] into the column-major output.
@@ -365,6 +467,19 @@ bool geno_lds_enabled() {
     return on;
 }
 
+// Grid of the LDS kernels (one group per workgroup).  With several groups: a 1-D grid in
+// XCD-aware order, so the groups of one tile run together on one XCD and read its A tiles from
+// HBM about once.
+static dim3 lds_grid(dim3 grid, const GenoArgs& a, GenoArgs& f) {
+    f = a;
+    if (a.G < 2) return grid;
+    f.tiles_x = grid.x;
+    f.flat_tiles = (int64_t)grid.x * grid.y;
+    const int64_t blocks = (f.flat_tiles + 7) / 8 * 8 * a.G;
+    if (blocks > 0x7fffffff) throw RuntimeError("launch_geno: grid too large");
+    return dim3((unsigned)blocks);
+}
+
 template <int L, bool TWO_B, int EPI>
 static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a, bool packed) {
     // B shared by the workgroup through LDS unless CSAMD_GENO_LDSB=0
@@ -376,17 +491,8 @@ static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a, bool packed
         return;
     }
     if (ldsb) {
-        GenoArgs f = a;
-        dim3 g = grid;
-        if (a.G >= 2) {
-            // one group per workgroup, 1-D grid in XCD-aware order: the groups of one tile run
-            // together on one XCD and read its A tiles from HBM about once
-            f.tiles_x = grid.x;
-            f.flat_tiles = (int64_t)grid.x * grid.y;
-            const int64_t blocks = (f.flat_tiles + 7) / 8 * 8 * a.G;
-            if (blocks > 0x7fffffff) throw RuntimeError("launch_geno: grid too large");
-            g = dim3((unsigned)blocks);
-        }
+        GenoArgs f;
+        const dim3 g = lds_grid(grid, a, f);
         if constexpr (TWO_B) k_geno2s<L, EPI><<<g, 256, 0, st>>>(f);
         else if constexpr (L == 4) k_geno1s2<L, EPI><<<g, 256, 0, st>>>(f);
         else k_geno1s<L, EPI><<<g, 256, 0, st>>>(f);
@@ -412,7 +518,13 @@ void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const Ge
         if (two_b || packed) throw RuntimeError("launch_geno: the genotype-only product has one unpacked operand");
 #define CS_GENOX_CASE(LL, EE)                                \
         if (L == LL && epi == EE) {                          \
-            k_genox<LL, EE><<<grid, 256, 0, st>>>(a);        \
+            if (geno_lds_enabled()) {                        \
+                GenoArgs f;                                  \
+                const dim3 g = lds_grid(grid, a, f);         \
+                k_genox_s<LL, EE><<<g, 256, 0, st>>>(f);     \
+            } else {                                         \
+                k_genox<LL, EE><<<grid, 256, 0, st>>>(a);    \
+            }                                                \
             CS_HIP(hipGetLastError());                       \
             return;                                          \
         }
@@ -464,12 +576,20 @@ void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, cons
 
 // Sparse observed-indicator form when the missing genotypes are few: the MFMA kernels then run
 // only the genotype products, and O^T V / O (mu W) come from column sums and the missing lists.
-static bool use_sparse_missing(cs_kinship* op) {
+// Both forms give bitwise identical results.
+static bool use_sparse_missing(cs_kinship* op, int64_t k, int L) {
     cs_bed* b = op->bed;
-    // Measured at n = 50k, m = 500k, 0.5 % missing, k = 16: the genotype-only MFMA kernels take
-    // 1.77 + 1.88 ms against 2.09 + 2.84 ms for the dense pair, but the two list gathers take
-    // 2.7 + 3.5 ms, so auto keeps the dense form.
-    if (op->missing_mode != 2 || op->prec == CS_PREC_INT8_L4) return false;   // int32 plain operand
+    if (op->missing_mode == 1 || op->prec == CS_PREC_INT8_L4) return false;   // int32 plain operand
+    if (op->missing_mode == 0) {
+        // Measured at n = 50k, m = 500k, k = 16: the observed-indicator products cost 0.55 ms
+        // (Z^T V) + 0.75 ms (Z W) of the 1.9 + 2.0 ms dense kernels, and each list gather reads
+        // one 64-byte row per missing genotype per 16 columns at 6-8 TB/s (1.0 + 1.4 ms at 0.5 %
+        // missing): break-even near 0.3 % missing.  With k L <= 16 the dense products run with
+        // packed limbs and are bandwidth-bound, so there is nothing to save.
+        if (k * L <= 16) return false;
+        const double frac = (double)missing_total(b) / ((double)b->N * (double)b->M);
+        if (!(frac <= 0.0015)) return false;
+    }
     ensure_missing_lists(b);
     return true;
 }
@@ -480,7 +600,7 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     hipStream_t st = op->stream;
     const int L = prec_limbs(op->prec);
     const double qmax = limbs_qmax(L);
-    const bool sparse = use_sparse_missing(op);
+    const bool sparse = use_sparse_missing(op, k, L);
     const Tiling tl = tiling(k, L, !sparse);   // the genotype-only kernels take unpacked operands
     const int64_t K16 = 16 * (int64_t)tl.G;
     const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
@@ -505,10 +625,17 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
         CS_HIP(hipMemsetAsync(op->Vp.p, 0, (size_t)(b->Npad * K16) * 4, st));
         k_quant<<<grid_for(b->Npad * k), 256, 0, st>>>(op->Vt.as<double>(), b->Npad, k, L, tl.NT, qmax,
                                                       op->scl.as<double>() + k, nullptr, op->Vp.as<int32_t>(), K16);
-        k_colsum_plain<<<(unsigned)K16, 256, 0, st>>>(op->Vp.as<int32_t>(), b->Npad, K16, op->S1.as<long long>());
-        k_sparse_q<<<grid_for(b->M * K16), 256, 0, st>>>(b->miss_snp_off.as<int64_t>(), b->miss_snp_idx.as<int32_t>(),
-                                                        op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M,
-                                                        op->qext.as<long long>());
+        colsum_plain(op->Vp.as<int32_t>(), b->Npad, K16, op->S1.as<long long>(), st);
+        const unsigned gq = (unsigned)std::min<int64_t>(4096, (b->M + 127) / 128);   // 4 waves x 32 rows
+        const int64_t* qo = b->miss_snp_off.as<int64_t>();
+        const int32_t* qi = b->miss_snp_idx.as<int32_t>();
+        switch (list_quads(K16)) {
+            case 4: k_sparse_q<4, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+            case 8: k_sparse_q<8, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+            case 16: k_sparse_q<16, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+            case 32: k_sparse_q<32, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+            default: k_sparse_q<64, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+        }
         CS_HIP(hipGetLastError());
     }
     CS_HIP(hipEventRecord(op->ev[1], st));
@@ -551,7 +678,7 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
         CS_HIP(hipMemsetAsync(op->B2p.p, 0, (size_t)(b->Mpad * K16) * 4, st));
         k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, L, tl.NT, qmax,
                                                       sclW + k, nullptr, op->B2p.as<int32_t>(), K16);
-        k_colsum_plain<<<(unsigned)K16, 256, 0, st>>>(op->B2p.as<int32_t>(), b->Mpad, K16, op->S2.as<long long>());
+        colsum_plain(op->B2p.as<int32_t>(), b->Mpad, K16, op->S2.as<long long>(), st);
     } else {
         op->B2.ensure((size_t)(b->Mpad * tl.NT * 16));
         CS_HIP(hipMemsetAsync(op->B2.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
@@ -614,9 +741,16 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
                     tl.packed);
     CS_HIP(hipEventRecord(op->ev[4], st));
     if (sparse) {
-        k_sparse_yadd<<<grid_for(b->N * k), 256, 0, st>>>(b->miss_smp_off.as<int64_t>(), b->miss_smp_idx.as<int32_t>(),
-                                                         op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k,
-                                                         op->Yint.as<long long>());
+        const unsigned gy = (unsigned)std::min<int64_t>(4096, (b->N + 31) / 32);     // 4 waves x 8 rows
+        const int64_t* yo = b->miss_smp_off.as<int64_t>();
+        const int32_t* yi = b->miss_smp_idx.as<int32_t>();
+        switch (list_quads(K16)) {
+            case 4: k_sparse_yadd<4, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            case 8: k_sparse_yadd<8, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            case 16: k_sparse_yadd<16, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            case 32: k_sparse_yadd<32, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            default: k_sparse_yadd<64, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+        }
         CS_HIP(hipGetLastError());
     }
     k_finalize<<<grid_for(op->n * k), 256, 0, st>>>(op->Yint.as<long long>(), k, sclW,

@@ -63,6 +63,8 @@ void ensure_stats(cs_bed* b);
 void ensure_sample_major(cs_bed* b);
 // CSR lists of the missing genotypes (both orientations); idempotent.
 void ensure_missing_lists(cs_bed* b);
+// Number of missing genotypes over all samples and SNPs (from the class counts; cached).
+int64_t missing_total(cs_bed* b);
 // Per-SNP class counts weighted by the host multiplicities w[0..n_w) (zero beyond); counts is
 // M x 4 int64 on the device, by code 00, 01, 10, 11.  Synchronises st.
 void weighted_counts(cs_bed* b, const int32_t* w, int64_t n_w, int64_t* d_counts, hipStream_t st);

@@ -101,11 +101,12 @@ int cs_kinship_rows(const cs_kinship* op, int64_t* rows);
 int cs_kinship_num_variants(const cs_kinship* op, int64_t* m);
 int cs_kinship_set_precision(cs_kinship* op, int mode);
 /* How the int8 operator treats O = observed-genotype indicators (same results either way, both
- * exact integer sums): 0 auto (= dense; the list gathers cost more than they save at 0.5 %
- * missing, n = 50k, m = 500k), 1 dense (an O product on the
- * MFMA units next to each genotype product), 2 sparse (genotype products only; O^T V and
- * O (mu W) from column sums minus the missing entries, from lists built once per genotype
- * handle). */
+ * exact integer sums): 1 dense (an O product on the MFMA units next to each genotype product),
+ * 2 sparse (genotype products only; O^T V and O (mu W) from column sums minus the missing
+ * entries, from lists built once per genotype handle), 0 auto: sparse when at most 0.15 % of
+ * the genotypes are missing and k L > 16 (L = limbs), else dense.  The list gathers read one
+ * 64-byte row per missing genotype and 16 columns; at n = 50k, m = 500k, k = 16 they break even
+ * with the O products near 0.3 % missing. */
 int cs_kinship_set_missing_mode(cs_kinship* op, int mode);
 /* apply / apply_block / applyTranspose (kinship_operator.cpp:211-304): Y = K X for X n x k
  * column-major with leading dimension ldx; Y may alias X; k == 0 is allowed.  ldx < n or

@@ -157,10 +157,11 @@ def test_kinship_sketch_width_266(csa):
     assert colrel(op.apply(X), ref) < 2e-6
 
 
-@pytest.mark.parametrize("miss", [0.005, 0.05, 0.0])
+@pytest.mark.parametrize("miss", [0.005, 0.05, 0.001, 0.0])
 def test_sparse_and_dense_missing_forms_bitwise(csa, miss):
     # Both forms compute the same exact integer sums: the results must agree bit for bit, with
-    # keep lists (repeats, any order), a LOCO variant selection and a partial last byte.
+    # keep lists (repeats, any order), a LOCO variant selection and a partial last byte.  Auto
+    # (mode 0) takes the sparse form at 0.1 % and 0 % missing, the dense one above 0.15 %.
     N, M = 2999, 4100
     r = csa.PlinkReader.synthetic(8, N, M, 0.01, 0.5, miss)
     rng = np.random.default_rng(1)
@@ -169,10 +170,11 @@ def test_sparse_and_dense_missing_forms_bitwise(csa, miss):
     op = csa.PlinkKinshipOperator(r, keep, variants)
     X = rng.standard_normal((keep.size, 37))
     outs = []
-    for mode in (1, 2):
+    for mode in (1, 2, 0):
         op.set_missing_mode(mode)
         outs.append(op.apply(X))
     assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
     body = r.download()
     means, inv = oracle.stats(body, N, M)
     ref = oracle.kinship_apply(body, N, means, inv, keep, variants, X)
