@@ -183,9 +183,10 @@ def test_sparse_and_dense_missing_forms_bitwise(csa, miss):
 
 def test_register_kernels_match_lds_kernels(csa, tmp_path):
     """CSAMD_GENO_LDSB=0 selects the register-only kernels (no LDS sharing, two groups per wave
-    for several groups, no limb packing).  The integer products are exact, so every column must
-    be bitwise identical to the default LDS kernels' (run in a child process: the switch is read
-    once per process)."""
+    for several groups, no limb packing); CSAMD_GENO_DMA=0 stages the Z W operands through
+    registers instead of direct global-to-LDS loads.  The integer products are exact, so every
+    column must be bitwise identical to the default kernels' (each setting in a child process:
+    the switches are read once per process)."""
     import os
     import subprocess
     import sys
@@ -200,17 +201,20 @@ keep = np.random.default_rng(3).choice({N}, 800, replace=True).astype(np.int32)
 op = csa.PlinkKinshipOperator(r, keep, None)
 X = np.random.default_rng(4).standard_normal((800, 41))
 out = {{}}
-for mode in (3, 4):
+for mode in (2, 3, 4):
     op.set_precision(mode)
     for k in (1, 16, 41):
         out[f"m{{mode}}k{{k}}"] = op.apply(X[:, :k])
 np.savez(sys.argv[1], **out)
 """
-    ref_path, reg_path = str(tmp_path / "lds.npz"), str(tmp_path / "reg.npz")
-    print("child processes: default kernels, then CSAMD_GENO_LDSB=0", flush=True)
-    subprocess.run([sys.executable, "-c", script, ref_path], check=True, timeout=120)
-    env = dict(os.environ, CSAMD_GENO_LDSB="0")
-    subprocess.run([sys.executable, "-c", script, reg_path], check=True, timeout=120, env=env)
-    a, b = np.load(ref_path), np.load(reg_path)
-    for key in a.files:
-        assert np.array_equal(a[key], b[key]), key
+    paths = {name: str(tmp_path / f"{name}.npz") for name in ("lds", "reg", "nodma")}
+    envs = {"lds": {}, "reg": {"CSAMD_GENO_LDSB": "0"}, "nodma": {"CSAMD_GENO_DMA": "0"}}
+    for name in ("lds", "reg", "nodma"):
+        print(f"child process: {name} {envs[name]}", flush=True)
+        subprocess.run([sys.executable, "-c", script, paths[name]], check=True, timeout=120,
+                       env=dict(os.environ, **envs[name]))
+    a = np.load(paths["lds"])
+    for other in ("reg", "nodma"):
+        b = np.load(paths[other])
+        for key in a.files:
+            assert np.array_equal(a[key], b[key]), (other, key)
