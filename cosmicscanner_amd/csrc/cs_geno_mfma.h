@@ -106,7 +106,7 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
     // B staging type: native vectors for the two-operand and packed forms (a HIP_vector_type
     // array there is copied through a stack object), the HIP vector type for the one-operand
     // form (fewer registers at three waves per SIMD).
-    using ST = typename std::conditional<TWO_B || PK, v4i, uint4>::type;
+    using ST = typename std::conditional<TWO_B || PK || DMA, v4i, uint4>::type;
     ST* ldsb = static_cast<ST*>(lds_raw);
     // Z W needs only X~ W1 - O (mu W): with the second operand stored negated, both products
     // accumulate into one set of int32 registers (exact while |sum| < 2^31: 384 per k-element).
@@ -384,21 +384,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 // (two with four limbs).  Packed limbs (PK, k L <= 16) need a quarter of the registers and run
 // four waves per SIMD.
 //
-// DMA (two-operand form): the B fragments are staged by direct global-to-LDS loads, which frees
-// the 24 staging registers; measured 1.80 against 1.93 ms for Z W at n = 50k, m = 500k, k = 16.
-// (In the one-operand form the compiler waits for the loads in flight before every LDS read,
-// which costs more than it saves, so that form keeps the register staging.)
+// DMA: the B fragments are staged by direct global-to-LDS loads, which frees the staging
+// registers (24 for Z W, 12 for Z^T V).  The LDS array is a native vector type: with the HIP
+// uint4 struct the compiler could not tell the B reads from the in-flight LDS writes and waited
+// for every outstanding vector load before the first read of a block.
 template <int L, int EPI, bool PK = false, bool DMA = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PK ? 4 : 3, PK ? 4 : 3))) void
 k_geno2s(GenoArgs a) {
     __shared__ v4i ldsb[2 * GenoLds<PK ? 1 : L, true>::kChunks];
     geno_body<L, true, EPI, false, 4, 1, true, PK, DMA>(a, ldsb);
 }
-template <int L, int EPI, bool PK = false>
+template <int L, int EPI, bool PK = false, bool DMA = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PK ? 4 : 3, PK ? 4 : 3))) void
 k_geno1s(GenoArgs a) {
-    __shared__ typename std::conditional<PK, v4i, uint4>::type ldsb[2 * GenoLds<PK ? 1 : L, false>::kChunks];
-    geno_body<L, false, EPI, false, 4, 1, true, PK>(a, ldsb);
+    __shared__ typename std::conditional<PK || DMA, v4i, uint4>::type ldsb[2 * GenoLds<PK ? 1 : L, false>::kChunks];
+    geno_body<L, false, EPI, false, 4, 1, true, PK, DMA>(a, ldsb);
 }
 // Genotype-only product with the B fragments shared through LDS: one accumulator set of
 // 4 x L tiles, four waves per SIMD.

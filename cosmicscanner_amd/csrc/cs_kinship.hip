@@ -480,8 +480,8 @@ static dim3 lds_grid(dim3 grid, const GenoArgs& a, GenoArgs& f) {
     return dim3((unsigned)blocks);
 }
 
-// Z W's B operands staged by direct global-to-LDS loads unless CSAMD_GENO_DMA=0 (read once per
-// process).
+// B operands of the LDS kernels (three limbs or fewer) staged by direct global-to-LDS loads
+// unless CSAMD_GENO_DMA=0 (read once per process).
 static bool geno_dma_enabled() {
     static const bool on = [] {
         const char* e = std::getenv("CSAMD_GENO_DMA");
@@ -504,14 +504,15 @@ static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a, bool packed
         GenoArgs f;
         const dim3 g = lds_grid(grid, a, f);
         if constexpr (TWO_B) {
-            // B staged by direct global-to-LDS loads (three limbs or fewer: the four-limb kernel
-            // runs at two waves per SIMD with registers to spare)
+            // B staged by direct global-to-LDS loads (three limbs or fewer: the four-limb kernels
+            // run at two waves per SIMD with registers to spare)
             if (L != 4 && geno_dma_enabled()) k_geno2s<L, EPI, false, true><<<g, 256, 0, st>>>(f);
             else k_geno2s<L, EPI><<<g, 256, 0, st>>>(f);
         } else if constexpr (L == 4) {
             k_geno1s2<L, EPI><<<g, 256, 0, st>>>(f);
         } else {
-            k_geno1s<L, EPI><<<g, 256, 0, st>>>(f);
+            if (geno_dma_enabled()) k_geno1s<L, EPI, false, true><<<g, 256, 0, st>>>(f);
+            else k_geno1s<L, EPI><<<g, 256, 0, st>>>(f);
         }
         return;
     }

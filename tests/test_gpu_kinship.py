@@ -183,8 +183,8 @@ def test_sparse_and_dense_missing_forms_bitwise(csa, miss):
 
 def test_register_kernels_match_lds_kernels(csa, tmp_path):
     """CSAMD_GENO_LDSB=0 selects the register-only kernels (no LDS sharing, two groups per wave
-    for several groups, no limb packing); CSAMD_GENO_DMA=0 stages the Z W operands through
-    registers instead of direct global-to-LDS loads.  The integer products are exact, so every
+    for several groups, no limb packing); CSAMD_GENO_DMA=0 stages the B operands of the LDS
+    kernels through registers instead of direct global-to-LDS loads.  The integer products are exact, so every
     column must be bitwise identical to the default kernels' (each setting in a child process:
     the switches are read once per process)."""
     import os
