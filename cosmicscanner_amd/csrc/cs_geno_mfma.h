@@ -402,10 +402,10 @@ k_geno1s(GenoArgs a) {
 }
 // Genotype-only product with the B fragments shared through LDS: one accumulator set of
 // 4 x L tiles, four waves per SIMD.
-template <int L, int EPI>
+template <int L, int EPI, bool DMA = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_genox_s(GenoArgs a) {
-    __shared__ uint4 ldsb[2 * GenoLds<L, false>::kChunks];
-    geno_body<L, false, EPI, true, 4, 1, true>(a, ldsb);
+    __shared__ typename std::conditional<DMA, v4i, uint4>::type ldsb[2 * GenoLds<L, false>::kChunks];
+    geno_body<L, false, EPI, true, 4, 1, true, false, DMA>(a, ldsb);
 }
 // Four limbs (two accumulator sets of 4 x 4 tiles) fit only at two waves per SIMD.
 template <int L, int EPI>

@@ -539,7 +539,10 @@ void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const Ge
             if (geno_lds_enabled()) {                        \
                 GenoArgs f;                                  \
                 const dim3 g = lds_grid(grid, a, f);         \
-                k_genox_s<LL, EE><<<g, 256, 0, st>>>(f);     \
+                if (geno_dma_enabled())                      \
+                    k_genox_s<LL, EE, true><<<g, 256, 0, st>>>(f); \
+                else                                         \
+                    k_genox_s<LL, EE><<<g, 256, 0, st>>>(f);  \
             } else {                                         \
                 k_genox<LL, EE><<<grid, 256, 0, st>>>(a);    \
             }                                                \
