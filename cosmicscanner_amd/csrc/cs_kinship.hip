@@ -496,8 +496,14 @@ static void launch_one(dim3 grid, hipStream_t st, const GenoArgs& a, bool packed
     const bool ldsb = geno_lds_enabled();
     if (packed) {
         if (!ldsb || a.G != 1) throw RuntimeError("launch_geno: packed limbs need the LDS kernels and one group");
-        if (TWO_B) k_geno2s<L, EPI, true><<<grid, 256, 0, st>>>(a);
-        else k_geno1s<L, EPI, true><<<grid, 256, 0, st>>>(a);
+        const bool dma = geno_dma_enabled();
+        if constexpr (TWO_B) {
+            if (dma) k_geno2s<L, EPI, true, true><<<grid, 256, 0, st>>>(a);
+            else k_geno2s<L, EPI, true><<<grid, 256, 0, st>>>(a);
+        } else {
+            if (dma) k_geno1s<L, EPI, true, true><<<grid, 256, 0, st>>>(a);
+            else k_geno1s<L, EPI, true><<<grid, 256, 0, st>>>(a);
+        }
         return;
     }
     if (ldsb) {
