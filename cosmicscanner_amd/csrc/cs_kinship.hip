@@ -121,6 +121,55 @@ __global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, i
     }
 }
 
+// Limb l of the balanced base-256 digits of q (the same recurrence as k_quant).
+__device__ __forceinline__ int limb_digit(long long q, int l, int L) {
+    int d = 0;
+    for (int i = 0; i < L; ++i) {
+        d = (i == L - 1) ? (int)q : (int)(((q + 128) & 255) - 128);
+        if (i == l) break;
+        q = (q - d) >> 8;
+    }
+    return d;
+}
+
+// The two B operands of Z W in one pass: W1 and -W2 quantised with the common per-column
+// inverse scale inv, every byte of both written (zero digits in the padding columns and
+// limbs), and the int64 result rows cleared.  One thread per (row, tile column): CW = 16 G
+// columns (L limbs each), or with packed limbs 16 columns c L + l of the single tile.
+__global__ void k_quant_zw(const double* __restrict__ W1, const double* __restrict__ W2, int64_t rows, int64_t k,
+                           int L, int NT, int CW, double qmax, const double* __restrict__ inv,
+                           int8_t* __restrict__ B1, int8_t* __restrict__ B2, int packed,
+                           long long* __restrict__ yint, int64_t ny) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    for (int64_t e = first; e < ny; e += stride) yint[e] = 0;
+    const int64_t total = rows * CW;
+    for (int64_t e = first; e < total; e += stride) {
+        const int64_t row = e / CW;
+        const int j = (int)(e - row * CW);
+        const int c = packed ? j / L : j;
+        const bool ok = packed ? j < k * L : j < k;
+        long long q1 = 0, q2 = 0;
+        if (ok) {
+            double d1 = rint(W1[row * k + c] * inv[c]), d2 = rint(-W2[row * k + c] * inv[c]);
+            q1 = (long long)fmin(fmax(d1, -qmax), qmax);
+            q2 = (long long)fmin(fmax(d2, -qmax), qmax);
+        }
+        if (packed) {
+            const int l = j % L;
+            const int64_t o = bperm_offset(row, 0, j, NT);
+            B1[o] = (int8_t)(ok ? limb_digit(q1, l, L) : 0);
+            B2[o] = (int8_t)(ok ? limb_digit(q2, l, L) : 0);
+        } else {
+            for (int l = 0; l < L; ++l) {
+                const int64_t o = bperm_offset(row, (c >> 4) * L + l, c & 15, NT);
+                B1[o] = (int8_t)(ok ? limb_digit(q1, l, L) : 0);
+                B2[o] = (int8_t)(ok ? limb_digit(q2, l, L) : 0);
+            }
+        }
+    }
+}
+
 // Exact column sums of a plain int32 [rows][K16] operand into S (zeroed by the caller): block
 // (x, y) sums a stride of the rows for columns 16 y .. 16 y + 15 (16 threads read one 64-byte row
 // piece) and adds its partial sums with one int64 atomic per column (integer addition, so the
@@ -696,26 +745,28 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     double* sclW = op->scl.as<double>();  // the V scales are no longer needed
     k_scales<<<grid_for(k), 256, 0, st>>>(op->mxW.as<unsigned long long>(), k, qmax, sclW, sclW + k);
     op->B1.ensure((size_t)(b->Mpad * tl.NT * 16));
-    CS_HIP(hipMemsetAsync(op->B1.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
-    k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W1.as<double>(), b->Mpad, k, L, tl.NT, qmax,
-                                                  sclW + k, op->B1.as<int8_t>(), nullptr, 0, 1.0, tl.packed);
+    op->Yint.ensure((size_t)(b->Npad * k) * sizeof(long long));
     if (sparse) {
+        CS_HIP(hipMemsetAsync(op->B1.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
+        k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W1.as<double>(), b->Mpad, k, L, tl.NT, qmax,
+                                                      sclW + k, op->B1.as<int8_t>(), nullptr, 0, 1.0, tl.packed);
         op->B2p.ensure((size_t)(b->Mpad * K16) * 4);
         op->S2.ensure((size_t)K16 * 8);
         CS_HIP(hipMemsetAsync(op->B2p.p, 0, (size_t)(b->Mpad * K16) * 4, st));
         k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, L, tl.NT, qmax,
                                                       sclW + k, nullptr, op->B2p.as<int32_t>(), K16);
         colsum_plain(op->B2p.as<int32_t>(), b->Mpad, K16, op->S2.as<long long>(), st);
+        CS_HIP(hipMemsetAsync(op->Yint.p, 0, (size_t)(b->Npad * k) * sizeof(long long), st));
     } else {
         op->B2.ensure((size_t)(b->Mpad * tl.NT * 16));
-        CS_HIP(hipMemsetAsync(op->B2.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
-        // negated: the Z W kernel accumulates X~ W1 and O (-mu W) into the same registers
-        k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, L, tl.NT, qmax,
-                                                      sclW + k, op->B2.as<int8_t>(), nullptr, 0, -1.0, tl.packed);
+        // W1 and -(mu W) (negated: the Z W kernel accumulates X~ W1 and O (-mu W) into the same
+        // registers), every byte of both, and the cleared result rows, in one launch
+        const int cw = tl.packed ? 16 : 16 * tl.G;
+        k_quant_zw<<<grid_for(b->Mpad * cw), 256, 0, st>>>(
+            op->W1.as<double>(), op->W2.as<double>(), b->Mpad, k, L, tl.NT, cw, qmax, sclW + k,
+            op->B1.as<int8_t>(), op->B2.as<int8_t>(), tl.packed ? 1 : 0, op->Yint.as<long long>(), b->Npad * k);
     }
     CS_HIP(hipGetLastError());
-    op->Yint.ensure((size_t)(b->Npad * k) * sizeof(long long));
-    CS_HIP(hipMemsetAsync(op->Yint.p, 0, (size_t)(b->Npad * k) * sizeof(long long), st));
     CS_HIP(hipEventRecord(op->ev[3], st));
 
     // Z W over the sample-major copy, split along the SNPs, exact int64 accumulation.
