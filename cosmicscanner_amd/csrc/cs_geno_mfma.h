@@ -105,7 +105,8 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
     static_assert(!PK || GZ == 1, "packed limbs use one group");
     // B staging type: native vectors for the two-operand and packed forms (a HIP_vector_type
     // array there is copied through a stack object), the HIP vector type for the one-operand
-    // form (fewer registers at three waves per SIMD).
+    // form with register staging (fewer registers at three waves per SIMD).  With DMA there are
+    // no staging registers and the LDS array must be a native vector type (see k_geno1s).
     using ST = typename std::conditional<TWO_B || PK || DMA, v4i, uint4>::type;
     ST* ldsb = static_cast<ST*>(lds_raw);
     // Z W needs only X~ W1 - O (mu W): with the second operand stored negated, both products
