@@ -205,13 +205,17 @@ static void colsum_plain(const int32_t* P, int64_t rows, int64_t K16, long long*
 // so a wave keeps U E row gathers and U index loads in flight.  The indices are read once
 // (non-temporal); the operand rows are cache hits.  The sums are reduced over e (valid in every
 // lane).
-template <int U, int CW>
+// W = 64 (one row per wave) or 32 (one row per half-wave: two rows' gathers in flight per wave;
+// the reduction shuffles stay inside the half).
+template <int U, int CW, int W = 64>
 __device__ __forceinline__ void list_row_sums(const int32_t* __restrict__ idx, int64_t p0, int64_t p1,
                                               const int4* __restrict__ P4, uint32_t q4, int64_t quad, int e,
                                               long long acc[4]) {
-    constexpr int E = 64 / CW;
+    static_assert(W == 64 || W == 32, "a row takes a wave or a half-wave");
+    static_assert(CW <= W, "at least one entry per batch");
+    constexpr int E = W / CW;
     acc[0] = acc[1] = acc[2] = acc[3] = 0;
-    if (p1 == p0) return;   // empty row (p0, p1 are the same in every lane): nothing to reduce
+    if (p1 == p0) return;   // empty row (p0, p1 are the same in every lane of the row): nothing to reduce
     const int4* col = P4 + quad;
     const int32_t* ip = idx + p0 + e;
     int64_t left = p1 - p0 - e;   // entries of this lane's stride still to read, from ip
@@ -234,7 +238,7 @@ __device__ __forceinline__ void list_row_sums(const int32_t* __restrict__ idx, i
             acc[3] += v[u].w;
         }
     }
-    for (int s = CW; s < 64; s <<= 1)
+    for (int s = CW; s < W; s <<= 1)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[t] += __shfl_xor(acc[t], s);
 }
@@ -252,24 +256,28 @@ __device__ __forceinline__ int64_t block_offsets(const int64_t* __restrict__ off
 // Observed-indicator product of the SNP rows from the missing lists:
 // qext[j][c] = sum_i o_ij Vq_i[c] = S[c] - sum_{i missing at j} Vq_i[c]   (exact integers).
 // CW: a power of two dividing K16 / 4, at most 64.  Each wave takes blocks of R rows.
-template <int CW, int R>
+template <int CW, int R, int W>
 __global__ __launch_bounds__(256) void k_sparse_q(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
                                                   const int32_t* __restrict__ Vp, int64_t K16,
                                                   const long long* __restrict__ S, int64_t rows,
                                                   long long* __restrict__ qext) {
-    const int lane = threadIdx.x & 63, e = lane / CW, cq = lane - e * CW;
+    constexpr int H = 64 / W;   // rows in flight per wave
+    const int lane = threadIdx.x & 63, h = lane / W, e = (lane % W) / CW, cq = lane % CW;
     const int64_t q4 = K16 / 4;
     const int4* P4 = reinterpret_cast<const int4*>(Vp);
     for (int64_t j0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * R; j0 < rows; j0 += (int64_t)gridDim.x * 4 * R) {
         const int64_t o = block_offsets<R>(off, j0, rows);
         const int nr = (int)(rows - j0 < R ? rows - j0 : R);
-        for (int r = 0; r < nr; ++r) {
-            const int64_t j = j0 + r, p0 = __shfl(o, r), p1 = __shfl(o, r + 1);
+        for (int rp = 0; rp < nr; rp += H) {
+            // the offsets are read with uniform control flow; a row past the block end is empty
+            const int r = rp + h;
+            const int64_t j = j0 + r, p1 = __shfl(o, r + 1 < nr ? r + 1 : nr);
+            const int64_t p0 = r < nr ? __shfl(o, r) : p1;
             for (int64_t c0 = 0; c0 < q4; c0 += CW) {
                 const int64_t quad = c0 + cq;
                 long long acc[4];
-                list_row_sums<8, CW>(idx, p0, p1, P4, (uint32_t)q4, quad, e, acc);
-                if (e == 0)
+                list_row_sums<8, CW, W>(idx, p0, p1, P4, (uint32_t)q4, quad, e, acc);
+                if (e == 0 && r < nr)
 #pragma unroll
                     for (int t = 0; t < 4; ++t) qext[j * K16 + 4 * quad + t] = S[4 * quad + t] - acc[t];
             }
@@ -279,24 +287,27 @@ __global__ __launch_bounds__(256) void k_sparse_q(const int64_t* __restrict__ of
 
 // Z W with the sparse observed-indicator form: yint[i][c] (= X~ W1 from the MFMA kernel) minus
 // sum_j o_ij (mu W)_j[c] = S2[c] - sum_{j missing at i} (mu W)_j[c].
-template <int CW, int R>
+template <int CW, int R, int W>
 __global__ __launch_bounds__(256) void k_sparse_yadd(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
                                                      const int32_t* __restrict__ B2p, int64_t K16,
                                                      const long long* __restrict__ S2, int64_t rows, int64_t k,
                                                      long long* __restrict__ yint) {
-    const int lane = threadIdx.x & 63, e = lane / CW, cq = lane - e * CW;
+    constexpr int H = 64 / W;   // rows in flight per wave
+    const int lane = threadIdx.x & 63, h = lane / W, e = (lane % W) / CW, cq = lane % CW;
     const int64_t q4 = K16 / 4;
     const int4* P4 = reinterpret_cast<const int4*>(B2p);
     for (int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * R; i0 < rows; i0 += (int64_t)gridDim.x * 4 * R) {
         const int64_t o = block_offsets<R>(off, i0, rows);
         const int nr = (int)(rows - i0 < R ? rows - i0 : R);
-        for (int r = 0; r < nr; ++r) {
-            const int64_t i = i0 + r, p0 = __shfl(o, r), p1 = __shfl(o, r + 1);
+        for (int rp = 0; rp < nr; rp += H) {
+            const int r = rp + h;
+            const int64_t i = i0 + r, p1 = __shfl(o, r + 1 < nr ? r + 1 : nr);
+            const int64_t p0 = r < nr ? __shfl(o, r) : p1;
             for (int64_t c0 = 0; c0 < q4; c0 += CW) {
                 const int64_t quad = c0 + cq;
                 long long acc[4];
-                list_row_sums<8, CW>(idx, p0, p1, P4, (uint32_t)q4, quad, e, acc);
-                if (e == 0)
+                list_row_sums<8, CW, W>(idx, p0, p1, P4, (uint32_t)q4, quad, e, acc);
+                if (e == 0 && r < nr)
 #pragma unroll
                     for (int t = 0; t < 4; ++t) {
                         const int64_t c = 4 * quad + t;
@@ -657,11 +668,12 @@ static bool use_sparse_missing(cs_kinship* op, int64_t k, int L) {
     cs_bed* b = op->bed;
     if (op->missing_mode == 1 || op->prec == CS_PREC_INT8_L4) return false;   // int32 plain operand
     if (op->missing_mode == 0) {
-        // Measured at n = 50k, m = 500k, k = 16: the observed-indicator products cost 0.55 ms
-        // (Z^T V) + 0.75 ms (Z W) of the 1.9 + 2.0 ms dense kernels, and each list gather reads
-        // one 64-byte row per missing genotype per 16 columns at 6-8 TB/s (1.0 + 1.4 ms at 0.5 %
-        // missing): break-even near 0.3 % missing.  With k L <= 16 the dense products run with
-        // packed limbs and are bandwidth-bound, so there is nothing to save.
+        // Measured at n = 50k, m = 500k, k = 16: the genotype-only products take 1.3 + 1.2 ms
+        // against 1.8 + 1.9 ms for the dense kernels, and each list gather reads one 64-byte row
+        // per missing genotype per 16 columns (0.3 + 0.3 ms at 0.1 % missing, 1.0 + 1.4 ms at
+        // 0.5 %): the apply takes 2.9 ms at 0 %, 3.35 ms at 0.1 % against 3.8 ms dense, so
+        // break-even is near 0.2 % missing.  With k L <= 16 the dense products run with packed
+        // limbs and are bandwidth-bound, so there is nothing to save.
         if (k * L <= 16) return false;
         const double frac = (double)missing_total(b) / ((double)b->N * (double)b->M);
         if (!(frac <= 0.0015)) return false;
@@ -706,11 +718,11 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
         const int64_t* qo = b->miss_snp_off.as<int64_t>();
         const int32_t* qi = b->miss_snp_idx.as<int32_t>();
         switch (list_quads(K16)) {
-            case 4: k_sparse_q<4, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
-            case 8: k_sparse_q<8, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
-            case 16: k_sparse_q<16, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
-            case 32: k_sparse_q<32, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
-            default: k_sparse_q<64, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+            case 4: k_sparse_q<4, 32, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+            case 8: k_sparse_q<8, 32, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+            case 16: k_sparse_q<16, 32, 32><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+            case 32: k_sparse_q<32, 32, 64><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
+            default: k_sparse_q<64, 32, 64><<<gq, 256, 0, st>>>(qo, qi, op->Vp.as<int32_t>(), K16, op->S1.as<long long>(), b->M, op->qext.as<long long>()); break;
         }
         CS_HIP(hipGetLastError());
     }
@@ -823,11 +835,11 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
         const int64_t* yo = b->miss_smp_off.as<int64_t>();
         const int32_t* yi = b->miss_smp_idx.as<int32_t>();
         switch (list_quads(K16)) {
-            case 4: k_sparse_yadd<4, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
-            case 8: k_sparse_yadd<8, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
-            case 16: k_sparse_yadd<16, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
-            case 32: k_sparse_yadd<32, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
-            default: k_sparse_yadd<64, 8><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            case 4: k_sparse_yadd<4, 8, 32><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            case 8: k_sparse_yadd<8, 8, 32><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            case 16: k_sparse_yadd<16, 8, 32><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            case 32: k_sparse_yadd<32, 8, 64><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            default: k_sparse_yadd<64, 8, 64><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
         }
         CS_HIP(hipGetLastError());
     }

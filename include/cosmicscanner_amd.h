@@ -106,7 +106,7 @@ int cs_kinship_set_precision(cs_kinship* op, int mode);
  * entries, from lists built once per genotype handle), 0 auto: sparse when at most 0.15 % of
  * the genotypes are missing and k L > 16 (L = limbs), else dense.  The list gathers read one
  * 64-byte row per missing genotype and 16 columns; at n = 50k, m = 500k, k = 16 they break even
- * with the O products near 0.3 % missing. */
+ * with the O products near 0.2 % missing. */
 int cs_kinship_set_missing_mode(cs_kinship* op, int mode);
 /* apply / apply_block / applyTranspose (kinship_operator.cpp:211-304): Y = K X for X n x k
  * column-major with leading dimension ldx; Y may alias X; k == 0 is allowed.  ldx < n or
