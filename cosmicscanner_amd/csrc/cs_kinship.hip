@@ -121,51 +121,66 @@ __global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, i
     }
 }
 
-// Limb l of the balanced base-256 digits of q (the same recurrence as k_quant).
-__device__ __forceinline__ int limb_digit(long long q, int l, int L) {
-    int d = 0;
-    for (int i = 0; i < L; ++i) {
-        d = (i == L - 1) ? (int)q : (int)(((q + 128) & 255) - 128);
-        if (i == l) break;
-        q = (q - d) >> 8;
-    }
-    return d;
-}
-
 // The two B operands of Z W in one pass: W1 and -W2 quantised with the common per-column
-// inverse scale inv, every byte of both written (zero digits in the padding columns and
-// limbs), and the int64 result rows cleared.  One thread per (row, tile column): CW = 16 G
-// columns (L limbs each), or with packed limbs 16 columns c L + l of the single tile.
+// inverse scale inv, every byte of both written (zero digits in the padding columns and limbs),
+// and the int64 result rows cleared.  A thread builds whole 16-byte B chunks: the 16
+// consecutive k-elements (rows) 16 rg .. 16 rg + 15 of one tile column, whose bytes 4 s + q hold
+// row 16 rg + 4 q + s (bperm_offset).  Unpacked: thread (rg, column cc < 16 G) writes the L limb
+// tiles of cc; packed: thread (rg, j < 16) writes tile column j = c L + l.  Neighbouring threads
+// read neighbouring columns of W and store neighbouring chunks.
+template <int L, bool packed>
 __global__ void k_quant_zw(const double* __restrict__ W1, const double* __restrict__ W2, int64_t rows, int64_t k,
-                           int L, int NT, int CW, double qmax, const double* __restrict__ inv,
-                           int8_t* __restrict__ B1, int8_t* __restrict__ B2, int packed,
+                           int NT, int CW, double qmax, const double* __restrict__ inv,
+                           int8_t* __restrict__ B1, int8_t* __restrict__ B2,
                            long long* __restrict__ yint, int64_t ny) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     for (int64_t e = first; e < ny; e += stride) yint[e] = 0;
-    const int64_t total = rows * CW;
+    const int64_t total = (rows / 16) * CW;
     for (int64_t e = first; e < total; e += stride) {
-        const int64_t row = e / CW;
-        const int j = (int)(e - row * CW);
+        const int64_t rg = e / CW;
+        const int j = (int)(e - rg * CW);
         const int c = packed ? j / L : j;
+        const int l0 = packed ? j % L : 0;
         const bool ok = packed ? j < k * L : j < k;
-        long long q1 = 0, q2 = 0;
+        // chunk words: [limb][word s]; byte q of word s = row 16 rg + 4 q + s
+        constexpr int NL = packed ? 1 : L;
+        uint32_t w1[NL][4], w2[NL][4];
+#pragma unroll
+        for (int l = 0; l < NL; ++l)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) w1[l][s] = w2[l][s] = 0u;
         if (ok) {
-            double d1 = rint(W1[row * k + c] * inv[c]), d2 = rint(-W2[row * k + c] * inv[c]);
-            q1 = (long long)fmin(fmax(d1, -qmax), qmax);
-            q2 = (long long)fmin(fmax(d2, -qmax), qmax);
-        }
-        if (packed) {
-            const int l = j % L;
-            const int64_t o = bperm_offset(row, 0, j, NT);
-            B1[o] = (int8_t)(ok ? limb_digit(q1, l, L) : 0);
-            B2[o] = (int8_t)(ok ? limb_digit(q2, l, L) : 0);
-        } else {
-            for (int l = 0; l < L; ++l) {
-                const int64_t o = bperm_offset(row, (c >> 4) * L + l, c & 15, NT);
-                B1[o] = (int8_t)(ok ? limb_digit(q1, l, L) : 0);
-                B2[o] = (int8_t)(ok ? limb_digit(q2, l, L) : 0);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int64_t row = rg * 16 + i;
+                const double d1 = rint(W1[row * k + c] * inv[c]), d2 = rint(-W2[row * k + c] * inv[c]);
+                long long q1 = (long long)fmin(fmax(d1, -qmax), qmax);
+                long long q2 = (long long)fmin(fmax(d2, -qmax), qmax);
+                const int s = i & 3, sh = 8 * (i >> 2);
+#pragma unroll
+                for (int l = 0; l < L; ++l) {
+                    const int e1 = (l == L - 1) ? (int)q1 : (int)(((q1 + 128) & 255) - 128);
+                    const int e2 = (l == L - 1) ? (int)q2 : (int)(((q2 + 128) & 255) - 128);
+                    q1 = (q1 - e1) >> 8;
+                    q2 = (q2 - e2) >> 8;
+                    if (packed) {   // keep limb l0 only
+                        if (l == l0) {
+                            w1[0][s] |= (uint32_t)(uint8_t)e1 << sh;
+                            w2[0][s] |= (uint32_t)(uint8_t)e2 << sh;
+                        }
+                    } else {
+                        w1[packed ? 0 : l][s] |= (uint32_t)(uint8_t)e1 << sh;
+                        w2[packed ? 0 : l][s] |= (uint32_t)(uint8_t)e2 << sh;
+                    }
+                }
             }
+        }
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            const int64_t o = packed ? bperm_offset(rg * 16, 0, j, NT) : bperm_offset(rg * 16, (c >> 4) * L + l, c & 15, NT);
+            *reinterpret_cast<uint4*>(B1 + o) = make_uint4(w1[l][0], w1[l][1], w1[l][2], w1[l][3]);
+            *reinterpret_cast<uint4*>(B2 + o) = make_uint4(w2[l][0], w2[l][1], w2[l][2], w2[l][3]);
         }
     }
 }
@@ -774,9 +789,16 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
         // W1 and -(mu W) (negated: the Z W kernel accumulates X~ W1 and O (-mu W) into the same
         // registers), every byte of both, and the cleared result rows, in one launch
         const int cw = tl.packed ? 16 : 16 * tl.G;
-        k_quant_zw<<<grid_for(b->Mpad * cw), 256, 0, st>>>(
-            op->W1.as<double>(), op->W2.as<double>(), b->Mpad, k, L, tl.NT, cw, qmax, sclW + k,
-            op->B1.as<int8_t>(), op->B2.as<int8_t>(), tl.packed ? 1 : 0, op->Yint.as<long long>(), b->Npad * k);
+        const unsigned gq = grid_for(b->Mpad / 16 * cw);
+        const double *w1 = op->W1.as<double>(), *w2 = op->W2.as<double>();
+        int8_t *b1 = op->B1.as<int8_t>(), *b2 = op->B2.as<int8_t>();
+        long long* yi = op->Yint.as<long long>();
+#define CS_QZW(LL, PP)                                                                                 \
+        if (L == LL && tl.packed == PP)                                                                \
+            k_quant_zw<LL, PP><<<gq, 256, 0, st>>>(w1, w2, b->Mpad, k, tl.NT, cw, qmax, sclW + k, b1, b2, yi, \
+                                                   b->Npad * k);
+        CS_QZW(2, false) CS_QZW(3, false) CS_QZW(4, false) CS_QZW(2, true) CS_QZW(3, true) CS_QZW(4, true)
+#undef CS_QZW
     }
     CS_HIP(hipGetLastError());
     CS_HIP(hipEventRecord(op->ev[3], st));
