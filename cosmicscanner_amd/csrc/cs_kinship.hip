@@ -128,14 +128,16 @@ __global__ void k_quant(const double* __restrict__ W, int64_t rows, int64_t k, i
 // row 16 rg + 4 q + s (bperm_offset).  Unpacked: thread (rg, column cc < 16 G) writes the L limb
 // tiles of cc; packed: thread (rg, j < 16) writes tile column j = c L + l.  Neighbouring threads
 // read neighbouring columns of W and store neighbouring chunks.
-template <int L, bool packed>
+// TWO = false: W1 into B1 only (W2, B2 and yint unused), the B operand of any product.
+template <int L, bool packed, bool TWO = true>
 __global__ void k_quant_zw(const double* __restrict__ W1, const double* __restrict__ W2, int64_t rows, int64_t k,
                            int NT, int CW, double qmax, const double* __restrict__ inv,
                            int8_t* __restrict__ B1, int8_t* __restrict__ B2,
                            long long* __restrict__ yint, int64_t ny) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t first = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    for (int64_t e = first; e < ny; e += stride) yint[e] = 0;
+    if (TWO)
+        for (int64_t e = first; e < ny; e += stride) yint[e] = 0;
     const int64_t total = (rows / 16) * CW;
     for (int64_t e = first; e < total; e += stride) {
         const int64_t rg = e / CW;
@@ -154,7 +156,7 @@ __global__ void k_quant_zw(const double* __restrict__ W1, const double* __restri
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int64_t row = rg * 16 + i;
-                const double d1 = rint(W1[row * k + c] * inv[c]), d2 = rint(-W2[row * k + c] * inv[c]);
+                const double d1 = rint(W1[row * k + c] * inv[c]), d2 = TWO ? rint(-W2[row * k + c] * inv[c]) : 0.0;
                 long long q1 = (long long)fmin(fmax(d1, -qmax), qmax);
                 long long q2 = (long long)fmin(fmax(d2, -qmax), qmax);
                 const int s = i & 3, sh = 8 * (i >> 2);
@@ -180,7 +182,7 @@ __global__ void k_quant_zw(const double* __restrict__ W1, const double* __restri
         for (int l = 0; l < NL; ++l) {
             const int64_t o = packed ? bperm_offset(rg * 16, 0, j, NT) : bperm_offset(rg * 16, (c >> 4) * L + l, c & 15, NT);
             *reinterpret_cast<uint4*>(B1 + o) = make_uint4(w1[l][0], w1[l][1], w1[l][2], w1[l][3]);
-            *reinterpret_cast<uint4*>(B2 + o) = make_uint4(w2[l][0], w2[l][1], w2[l][2], w2[l][3]);
+            if (TWO) *reinterpret_cast<uint4*>(B2 + o) = make_uint4(w2[l][0], w2[l][1], w2[l][2], w2[l][3]);
         }
     }
 }
@@ -668,10 +670,18 @@ void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, cons
         k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(W[i], rows, k, mx.as<unsigned long long>());
     k_scales<<<grid_for(k), 256, 0, st>>>(mx.as<unsigned long long>(), k, qmax, scl.as<double>(),
                                          scl.as<double>() + k);
+    // every byte of each operand from whole 16-byte chunks (rows is a multiple of 256)
+    if (rows % 16) throw RuntimeError("quantize_cols: rows must be a multiple of 16");
+    const int cw = tl.packed ? 16 : 16 * tl.G;
+    const unsigned gq = grid_for(rows / 16 * cw);
+    const double* inv = scl.as<double>() + k;
     for (int i = 0; i < nW; ++i) {
-        CS_HIP(hipMemsetAsync(B[i], 0, (size_t)(rows * tl.NT * 16), st));
-        k_quant<<<grid_for(rows * k), 256, 0, st>>>(W[i], rows, k, tl.L, tl.NT, qmax,
-                                                   scl.as<double>() + k, B[i], nullptr, 0, 1.0, tl.packed);
+#define CS_QC(LL, PP)                                                                                      \
+        if (tl.L == LL && tl.packed == PP)                                                                 \
+            k_quant_zw<LL, PP, false><<<gq, 256, 0, st>>>(W[i], nullptr, rows, k, tl.NT, cw, qmax, inv, B[i], \
+                                                          nullptr, nullptr, 0);
+        CS_QC(2, false) CS_QC(3, false) CS_QC(4, false) CS_QC(2, true) CS_QC(3, true) CS_QC(4, true)
+#undef CS_QC
     }
     CS_HIP(hipGetLastError());
 }
