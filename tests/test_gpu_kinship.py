@@ -181,6 +181,26 @@ def test_sparse_and_dense_missing_forms_bitwise(csa, miss):
     assert colrel(outs[1], ref) < 2e-6
 
 
+@pytest.mark.parametrize("k", [120, 250])
+def test_sparse_missing_form_wide_blocks_bitwise(csa, k):
+    """Wide blocks take the full-wave list gathers: 120 columns (eight groups, 128 plain int32
+    columns = 32 quads per row) and 250 (sixteen groups, 64 quads).  Bitwise equal to the dense
+    form, and within the operator's bound of the oracle."""
+    N, M = 1500, 2300
+    r = csa.PlinkReader.synthetic(21, N, M, 0.01, 0.5, 0.01)
+    op = csa.PlinkKinshipOperator(r, None, None)
+    X = np.random.default_rng(5).standard_normal((N, k))
+    outs = []
+    for mode in (1, 2):
+        op.set_missing_mode(mode)
+        outs.append(op.apply(X))
+    assert np.array_equal(outs[0], outs[1])
+    body = r.download()
+    means, inv = oracle.stats(body, N, M)
+    ref = oracle.kinship_apply(body, N, means, inv, None, None, X)
+    assert colrel(outs[1], ref) < 2e-6
+
+
 def test_register_kernels_match_lds_kernels(csa, tmp_path):
     """CSAMD_GENO_LDSB=0 selects the register-only kernels (no LDS sharing, two groups per wave
     for several groups, no limb packing); CSAMD_GENO_DMA=0 stages the B operands of the LDS
