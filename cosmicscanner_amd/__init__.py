@@ -114,8 +114,9 @@ def _load():
                               ctypes.c_int),
         "cs_pcg_projected_scan": ([_vp, _vp, _i32p, _i64, _dp, _i64, _i64, _f32p, _i64, _i64,
                                    ctypes.c_double, ctypes.c_double, _i32p, _i64, _i32p, _i64,
-                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp,
-                                   _i64p, _i32p, _dp, _dp, _dp, _dp, ctypes.c_void_p], ctypes.c_int),
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, _dp, _i64p, _i32p, _dp, _dp, _dp, _dp, ctypes.c_void_p],
+                                  ctypes.c_int),
         "cs_lowrank_prepare": ([_vp, _dp, _f32p, _i64, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                 ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp),
                                 ctypes.c_void_p], ctypes.c_int),
@@ -720,7 +721,7 @@ class GwasMfOptions(ctypes.Structure):
                 ("oversample", ctypes.c_int), ("seed", ctypes.c_int64), ("trace_threshold", ctypes.c_double),
                 ("min_rank", ctypes.c_int), ("no_fallback", ctypes.c_int), ("error_tol", ctypes.c_double),
                 ("sigma2_g", ctypes.c_double), ("sigma2_e", ctypes.c_double), ("pcg_tol", ctypes.c_double),
-                ("pcg_max_iter", ctypes.c_int), ("n_calibration", ctypes.c_int)]
+                ("pcg_max_iter", ctypes.c_int), ("n_calibration", ctypes.c_int), ("exact_log10p", ctypes.c_double)]
 
 
 def _summary_dict(sm):
@@ -734,7 +735,8 @@ def _summary_dict(sm):
 def gwas_matrix_free_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=False, scan_chr=None,
                          mode="lowrank", max_rank=256, oversample=10, seed=42, trace_threshold=0.95,
                          min_rank=1, no_fallback=False, error_tol=1.0, sigma2_g=0.0, sigma2_e=0.0,
-                         pcg_tol=1e-6, pcg_max_iter=200, n_calibration=32, prec=CS_PREC_INT8_L3):
+                         pcg_tol=1e-6, pcg_max_iter=200, n_calibration=32, prec=CS_PREC_INT8_L3,
+                         exact_log10p=0.0):
     """The matrix-free GWAS (--fastlmm --fastlmm-matrix-free) on the GPU; mode "lowrank" (the
     reference path) or "pcg" (PCG-projected scan)."""
     y = np.ascontiguousarray(y, dtype=np.float64)
@@ -747,7 +749,8 @@ def gwas_matrix_free_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=F
     o = GwasMfOptions(float(min_maf), int(bool(no_loco)), None if scan_chr is None else str(scan_chr).encode(),
                       int(prec), {"lowrank": 0, "pcg": 1}[mode], int(max_rank), int(oversample), int(seed),
                       float(trace_threshold), int(min_rank), int(bool(no_fallback)), float(error_tol),
-                      float(sigma2_g), float(sigma2_e), float(pcg_tol), int(pcg_max_iter), int(n_calibration))
+                      float(sigma2_g), float(sigma2_e), float(pcg_tol), int(pcg_max_iter), int(n_calibration),
+                      float(exact_log10p))
     sm = GwasSummary()
     _check(lib().cs_gwas_matrix_free_run(str(bfile).encode(), _p(y, _dp), _p(covf, _dp), N, nc, ctypes.byref(o),
                                          str(out_prefix).encode(), ctypes.byref(sm)))
@@ -782,7 +785,10 @@ def gwas_dense_run(bfile, y, out_prefix, cov=None, min_maf=0.01, no_loco=False, 
 # --------------------------------------------------------------------- PCG-projected scan
 class PscanInfo(ctypes.Structure):
     _fields_ = [("gamma", ctypes.c_double), ("n_calibration", ctypes.c_int64),
-                ("pcg_iterations", ctypes.c_int), ("max_rel_residual", ctypes.c_double)]
+                ("pcg_iterations", ctypes.c_int), ("max_rel_residual", ctypes.c_double),
+                ("n_exact", ctypes.c_int64), ("exact_pcg_iterations", ctypes.c_int),
+                ("calibration_rel_sd", ctypes.c_double), ("exact_log10p", ctypes.c_double),
+                ("dlog10p_bound_below", ctypes.c_double)]
 
 
 def markers_device(reader, keep, snps, d_G, min_maf=0.01):
@@ -798,8 +804,11 @@ def markers_device(reader, keep, snps, d_G, min_maf=0.01):
 
 
 def pcg_projected_scan(op, reader, keep, Y, X, sigma_g, sigma_e, cal_snps, snps=None, min_maf=0.01,
-                       rel_tol=1e-6, max_iter=200, prec=CS_PREC_INT8_L3):
-    """PCG-projected GLS scan with kinship operator `op` (see cs_pcg_projected_scan)."""
+                       rel_tol=1e-6, max_iter=200, prec=CS_PREC_INT8_L3, exact_log10p=0.0):
+    """PCG-projected GLS scan with kinship operator `op` (see cs_pcg_projected_scan).
+
+    exact_log10p: threshold of the exact stage (0 = the default 2, negative = every tested SNP,
+    ``inf`` = off)."""
     keep = _opt_i32(keep)
     snps = _opt_i32(snps)
     n = op.rows()
@@ -817,12 +826,14 @@ def pcg_projected_scan(op, reader, keep, Y, X, sigma_g, sigma_e, cal_snps, snps=
                                        _p(Yf, _dp), n, T, _p(Xf, _f32p), n, c, float(sigma_g),
                                        float(sigma_e), _p(cal, _i32p), cal.size, _p(snps, _i32p),
                                        0 if snps is None else snps.size, float(min_maf), float(rel_tol),
-                                       int(max_iter), int(prec), _p(freq, _dp), _p(nobs, _i64p),
+                                       int(max_iter), int(prec), float(exact_log10p), _p(freq, _dp), _p(nobs, _i64p),
                                        _p(ps, _i32p), _p(out["beta"], _dp), _p(out["se"], _dp),
                                        _p(out["p"], _dp), _p(out["log10p"], _dp), ctypes.byref(info)))
     out.update(freq=freq, n_obs=nobs, passes=ps.astype(bool), gamma=info.gamma,
                n_calibration=info.n_calibration, pcg_iterations=info.pcg_iterations,
-               max_rel_residual=info.max_rel_residual)
+               max_rel_residual=info.max_rel_residual, n_exact=info.n_exact,
+               exact_pcg_iterations=info.exact_pcg_iterations, calibration_rel_sd=info.calibration_rel_sd,
+               exact_log10p=info.exact_log10p, dlog10p_bound_below=info.dlog10p_bound_below)
     return out
 
 
@@ -898,13 +909,14 @@ class _Request(ctypes.Structure):
                 ("sigma2_g", ctypes.c_double), ("sigma2_e", ctypes.c_double), ("cal_snps", _i32p),
                 ("n_cal", ctypes.c_int64), ("snps", _i32p), ("n_snps", ctypes.c_int64),
                 ("min_maf", ctypes.c_double), ("rel_tol", ctypes.c_double), ("max_iter", ctypes.c_int),
-                ("prec", ctypes.c_int)]
+                ("prec", ctypes.c_int), ("exact_log10p", ctypes.c_double)]
 
 
 class _ReqStats(ctypes.Structure):
     _fields_ = [("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("gamma", ctypes.c_double), ("n_calibration", ctypes.c_int64),
-                ("n_columns", ctypes.c_int64)]
+                ("n_columns", ctypes.c_int64), ("n_exact", ctypes.c_int64), ("exact_iterations", ctypes.c_int),
+                ("calibration_rel_sd", ctypes.c_double), ("dlog10p_bound_below", ctypes.c_double)]
 
 
 class ScanServer:
@@ -931,7 +943,7 @@ class ScanServer:
             pass
 
     def submit(self, Y, X, sigma2_g, sigma2_e, cal_snps, keep=None, snps=None, min_maf=0.01, rel_tol=1e-6,
-               max_iter=200, prec=CS_PREC_INT8_L3):
+               max_iter=200, prec=CS_PREC_INT8_L3, exact_log10p=0.0):
         keep_a = _opt_i32(keep)
         n = self.reader.num_samples if keep_a is None else keep_a.size
         Yf = np.asfortranarray(np.asarray(Y, dtype=np.float64).reshape(n, -1))
@@ -941,7 +953,7 @@ class ScanServer:
         r = _Request(_p(keep_a, _i32p), 0 if keep_a is None else keep_a.size, _p(Yf, _dp), n, Yf.shape[1],
                      _p(Xf, _f32p), n, Xf.shape[1], float(sigma2_g), float(sigma2_e), _p(cal, _i32p), cal.size,
                      _p(sn, _i32p), 0 if sn is None else sn.size, float(min_maf), float(rel_tol), int(max_iter),
-                     int(prec))
+                     int(prec), float(exact_log10p))
         rid = ctypes.c_int64()
         _check(lib().cs_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
         self._meta[rid.value] = (Yf.shape[1], self.reader.num_snps if sn is None else sn.size)

@@ -488,7 +488,7 @@ extern "C" int cs_gwas_matrix_free_run(const char* bfile, const double* y, const
                 chk(cs_pcg_projected_scan(K.h, in.bed.h, in.keep.data(), n, in.y.data(), n, 1, in.X.data(), n, in.c,
                                           pcg_sg, pcg_se, pool.data(), (int64_t)pool.size(),
                                           snps.empty() ? nullptr : snps.data(), ns, o.min_maf, o.pcg_tol, o.pcg_max_iter,
-                                          o.prec, freq.data(), nobs.data(), ps.data(), beta.data(), se.data(), p.data(),
+                                          o.prec, o.exact_log10p, freq.data(), nobs.data(), ps.data(), beta.data(), se.data(), p.data(),
                                           l10.data(), &info));
                 ms_scan += ms_since(t1);
                 model_labels.push_back(chrom.empty() ? "GLOBAL" : chrom);

@@ -95,12 +95,45 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
 void pscan_prepare_rhs(cs_bed* bed, const int32_t* keep, int64_t n_keep, int64_t n, const double* Y, int64_t ldy,
                        int64_t T, const float* X, int64_t ldx, int64_t c, const int32_t* cal_snps, int64_t n_cal,
                        double min_maf, DevBuf& B, int64_t& L, hipStream_t st);
-void pscan_finish(cs_bed* bed, const int32_t* keep, int64_t n_keep, int64_t n, int64_t T, int64_t c, int64_t L,
-                  const double* B, const double* Z, const int32_t* snps, int64_t n_snps, double min_maf, int prec,
-                  double* freq, int64_t* n_obs, int32_t* pass, double* beta, double* se, double* p, double* log10p,
-                  double* gamma_out, int64_t* ncal_out, hipStream_t st, rocblas_handle blas,
-                  bool device_out = false);
+// Projection state of one solve: R = [P y_1..T, X] (n x (T + c), device), ZX = V^-1 X (n x c,
+// device), C = (X^T V^-1 X)^-1, (X^T X)^-1 (c x c column-major), the calibration factor and the
+// relative spread of the calibration markers' ratios (NaN with fewer than two).
+struct PscanProj {
+    int64_t n = 0, T = 0, c = 0;
+    DevBuf R, ZX;
+    std::vector<double> C, XtXi;
+    double gamma = 0.0, rel_sd = 0.0;
+    int64_t ncal = 0;
+};
+void pscan_project(int64_t n, int64_t T, int64_t c, int64_t L, const double* B, const double* Z, PscanProj& pj,
+                   hipStream_t st, rocblas_handle blas);
+void pscan_scan(cs_bed* bed, const int32_t* keep, const PscanProj& pj, const int32_t* snps, int64_t n_snps,
+                double min_maf, int prec, double* freq, int64_t* n_obs, int32_t* pass, double* beta, double* se,
+                double* p, double* log10p, bool device_out);
+// Rows (indices into the scanned list, ascending) that pass and whose largest -log10 P over the
+// T traits reaches thr (thr < 0: every tested row; +inf: none); log10p / pass on the device when
+// `device`.
+std::vector<int64_t> pscan_select(const double* log10p, const int32_t* pass, int64_t ns, int64_t T, double thr,
+                                  bool device, hipStream_t st);
+// Exact statistics of S selected rows from their prepared markers dG and an approximate solution
+// dZg of V Z = dG (n x S, device), with dAux = V dZg (aux_is_residual false) or the residual
+// dG - V dZg (true).  g^T V^-1 g comes from the variational form g^T z + z^T r, whose error
+// -r^T V^-1 r is quadratic in the residual, so a loose solve tolerance suffices.  Overwrites
+// rows[s] of the row-major ns x T outputs (device memory when device_out, else host).
+void pscan_exact(const PscanProj& pj, int64_t S, const int64_t* rows, const double* dG, const double* dZg,
+                 const double* dAux, bool aux_is_residual, double* beta, double* se, double* p, double* log10p,
+                 bool device_out, hipStream_t st, rocblas_handle blas);
 #endif
+// Exact-stage threshold of the projected scans: 0 -> the default, < 0 -> -1 (every tested SNP),
+// +inf -> off; NaN is rejected.
+constexpr double kPscanExactDefault = 2.0;
+double pscan_threshold(double exact_log10p);
+// Solve tolerance of the exact stage from the request's PCG tolerance (the variational variance
+// is accurate to the square of the residual).
+inline double pscan_exact_tol(double rel_tol) { return rel_tol * 100.0 < 1e-4 ? rel_tol * 100.0 : 1e-4; }
+// Stated bound on |d(-log10 P)| of the calibrated statistics below threshold thr, from the
+// calibration markers' relative spread (0 when thr < 0; NaN when off or unknown).
+double pscan_bound_below(double thr, double rel_sd);
 // Prepared markers of `snps` over the keep list (NULL = all samples) into dG (n x ns, device).
 void markers_device(cs_bed* b, const int32_t* keep, int64_t n_keep, const int32_t* snps, int64_t ns,
                     double min_maf, double* dG, int64_t ldg, int32_t* pass_out, double* freq_out,

@@ -8,15 +8,19 @@
 // calibration markers) to one shared multi-RHS PCG: one K.V apply per iteration serves every
 // admitted request, each column keeping its own variance components, preconditioner, mask,
 // recurrence and convergence.  New requests are admitted at iteration boundaries as column slots
-// and the HBM budget free up (continuous batching); a request whose columns have all converged is
-// finished at once -- projection, calibration and the fused per-SNP scan over its cohort -- and
-// its slots are reused.
+// and the HBM budget free up (continuous batching).  A request whose columns have all converged
+// runs its projection, calibration and fused per-SNP scan at once and gives its slots back; the
+// SNPs its exact stage selects then return to the batch as a second phase of columns (their
+// prepared markers, in chunks that fit the free slots and budget), whose solutions give the exact
+// GLS variances.  The budget covers the solve buffers while held and the result arrays from
+// admission until the client releases them.
 //
 // A column's arithmetic is independent of the other columns in the batch (per-column
 // quantisation scales, exact integer products, per-column dots with a fixed reduction order), so a
 // request's results are the same alone and in any mix.
 #include <rocblas/rocblas.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <deque>
@@ -156,19 +160,24 @@ struct Request {
     std::vector<float> X;
     double sg = 0, se = 1;
     std::vector<int32_t> cal, snps;
-    double min_maf = 0.01, tol = 1e-6;
+    double min_maf = 0.01, tol = 1e-6, thr = kPscanExactDefault;
     int max_iter = 200, prec = CS_PREC_INT8_L3;
-    DevBuf B;                      // compact prepared RHS (n x k)
+    // phase 1: the main solve over [Y, X, G_cal]; phase 2: exact-stage chunks of selected markers
+    int phase = 1;
+    DevBuf B;                      // compact RHS of the columns in the slots (n x cols)
     DevBuf dkeep;
-    int64_t slot0 = -1;
-    int iterations = 0;
-    int64_t bytes = 0;
+    int64_t slot0 = -1, cols = 0;  // slots held (slot0 < 0: none)
+    int iterations = 0, chunk_iterations = 0, exact_iterations = 0;
+    int64_t bytes = 0;             // solve buffers charged to the budget while held
+    int64_t result_bytes = 0;      // result arrays, charged from admission until release
     double t_submit = 0, t_admit = 0, t_done = 0;
+    // projection state between the phases, selected rows, next chunk
+    PscanProj pj;
+    std::vector<int64_t> rows;
+    int64_t row_next = 0, chunk = 0;
     // results, in device memory until cs_server_result copies them out (or the request is released)
     int64_t ns = 0;
     DevBuf freq, n_obs, pass, beta, se_out, p, log10p;
-    double gamma = 0;
-    int64_t ncal = 0;
     std::string error;
 };
 
@@ -219,30 +228,25 @@ int64_t find_slots(cs_server* s, int64_t k) {
     return -1;
 }
 
-bool admit(cs_server* s, Request& q) {
-    // RHS in compact cohort rows, then into the slots
-    if (q.k == 0) {
-        DevBuf B;
-        int64_t L = 0;
-        pscan_prepare_rhs(s->bed, q.keep.data(), q.n, q.n, q.Y.data(), q.n, q.T, q.X.data(), q.n, q.c,
-                          q.cal.empty() ? nullptr : q.cal.data(), (int64_t)q.cal.size(), q.min_maf, B, L, s->st);
-        q.L = L;
-        q.k = q.T + q.c + L;
-        q.B = std::move(B);
-        q.bytes = (int64_t)q.B.bytes * 3;
+int64_t longest_free_run(const cs_server* s) {
+    int64_t run = 0, best = 0;
+    for (int64_t c = 0; c < s->cap; ++c) {
+        run = s->slot_req[(size_t)c] < 0 ? run + 1 : 0;
+        best = std::max(best, run);
     }
-    if (q.k > s->cap) throw InvalidArgument("server: request needs more columns than the server holds");
-    if (s->budget_used + q.bytes > s->budget) return false;
-    const int64_t s0 = find_slots(s, q.k);
-    if (s0 < 0) return false;
-    const int64_t N = s->N, k = q.k;
+    return best;
+}
+
+// Put the k compact cohort columns of q.B into the slots [s0, s0 + k): scatter into full-sample
+// columns, preconditioner / mask, variance components, PCG init, squared norms and first r.z.
+void place_columns(cs_server* s, Request& q, int64_t s0, int64_t k) {
+    const int64_t N = s->N;
     q.slot0 = s0;
-    q.dkeep.alloc((size_t)q.n * 4);
-    CS_HIP(hipMemcpyAsync(q.dkeep.p, q.keep.data(), (size_t)q.n * 4, hipMemcpyHostToDevice, s->st));
+    q.cols = k;
+    for (int64_t c = 0; c < k; ++c) s->slot_req[(size_t)(s0 + c)] = q.id;
     CS_HIP(hipMemsetAsync(s->Bs.as<double>() + s0 * N, 0, (size_t)(N * k) * 8, s->st));
     k_srv_scatter<<<grid_for(q.n * k), 256, 0, s->st>>>(q.B.as<double>(), q.n, q.dkeep.as<int32_t>(), N, k,
                                                        s->Bs.as<double>() + s0 * N);
-    // preconditioner / mask columns
     std::vector<double> mv((size_t)(N * k), 0.0);
     for (int64_t c = 0; c < k; ++c)
         for (int64_t r = 0; r < q.n; ++r) {
@@ -253,7 +257,6 @@ bool admit(cs_server* s, Request& q) {
     for (int64_t c = 0; c < k; ++c) {
         s->hsg[(size_t)(s0 + c)] = q.sg;
         s->hse[(size_t)(s0 + c)] = q.se;
-        s->slot_req[(size_t)(s0 + c)] = q.id;
     }
     CS_HIP(hipMemcpyAsync(s->dsg.p, s->hsg.data(), (size_t)s->cap * 8, hipMemcpyHostToDevice, s->st));
     CS_HIP(hipMemcpyAsync(s->dse.p, s->hse.data(), (size_t)s->cap * 8, hipMemcpyHostToDevice, s->st));
@@ -261,9 +264,6 @@ bool admit(cs_server* s, Request& q) {
                                                   s->Xs.as<double>(), s->R.as<double>(), s->Z.as<double>(),
                                                   s->P.as<double>());
     CS_HIP(hipGetLastError());
-    // squared norms and the first r.z of the new columns
-    std::vector<double> b2((size_t)k), r2((size_t)k);
-    DevBuf tmp((size_t)k * 8);
     const double* Bc = s->Bs.as<double>() + s0 * N;
     k_srv_dots<<<dim3(kSrvDotBlocks, (unsigned)k), 256, 0, s->st>>>(Bc, Bc, N, s->partial.as<double>());
     k_srv_reduce<<<grid_for(k), 256, 0, s->st>>>(s->partial.as<double>(), k, s->bb.as<double>() + s0);
@@ -278,46 +278,157 @@ bool admit(cs_server* s, Request& q) {
         s->hact[(size_t)(s0 + c)] = s->hbb[(size_t)(s0 + c)] > 0.0 ? 1 : 0;
     }
     s->used = std::max(s->used, s0 + k);
-    s->budget_used += q.bytes;
+    q.chunk_iterations = 0;
+}
+
+// Free the request's slots (P and the mask zeroed so idle columns add nothing to the shared
+// apply) and its solve buffers; safe to call on any path, including failures.
+void release_slots(cs_server* s, Request& q) {
+    if (q.slot0 >= 0) {
+        const int64_t N = s->N;
+        (void)hipMemsetAsync(s->P.as<double>() + q.slot0 * N, 0, (size_t)(N * q.cols) * 8, s->st);
+        (void)hipMemsetAsync(s->minv.as<double>() + q.slot0 * N, 0, (size_t)(N * q.cols) * 8, s->st);
+        (void)hipStreamSynchronize(s->st);
+        for (int64_t c = 0; c < q.cols; ++c) {
+            s->slot_req[(size_t)(q.slot0 + c)] = -1;
+            s->hact[(size_t)(q.slot0 + c)] = 0;
+        }
+        while (s->used > 0 && s->slot_req[(size_t)(s->used - 1)] < 0) --s->used;
+        q.slot0 = -1;
+        q.cols = 0;
+    }
+    s->budget_used -= q.bytes;
+    q.bytes = 0;
+    q.B.release();
+}
+
+void free_results(cs_server* s, Request& q) {
+    for (DevBuf* b : {&q.freq, &q.n_obs, &q.pass, &q.beta, &q.se_out, &q.p, &q.log10p}) b->release();
+    s->budget_used -= q.result_bytes;
+    q.result_bytes = 0;
+}
+
+// A request that failed gives back everything it holds.
+void fail(cs_server* s, Request& q, const std::string& msg) {
+    release_slots(s, q);
+    free_results(s, q);
+    q.dkeep.release();
+    q.pj = PscanProj();
+    q.status = REQ_FAILED;
+    q.error = msg;
+    q.t_done = s->now();
+}
+
+int64_t result_bytes_of(const cs_server* s, const Request& q) {
+    const int64_t ns = q.snps.empty() ? s->bed->M : (int64_t)q.snps.size();
+    return ns * (4 * q.T + 3) * 8;
+}
+
+bool admit(cs_server* s, Request& q) {
+    if (q.k == 0) {
+        DevBuf B;
+        int64_t L = 0;
+        pscan_prepare_rhs(s->bed, q.keep.data(), q.n, q.n, q.Y.data(), q.n, q.T, q.X.data(), q.n, q.c,
+                          q.cal.empty() ? nullptr : q.cal.data(), (int64_t)q.cal.size(), q.min_maf, B, L, s->st);
+        q.L = L;
+        q.k = q.T + q.c + L;
+        q.B = std::move(B);
+    }
+    if (q.k > s->cap) throw InvalidArgument("server: request needs more columns than the server holds");
+    // solve buffers (B, the gathered solution, the projection) and the result arrays
+    const int64_t need = (int64_t)q.B.bytes * 3, res = result_bytes_of(s, q);
+    if (s->budget_used + need + res > s->budget) return false;
+    const int64_t s0 = find_slots(s, q.k);
+    if (s0 < 0) return false;
+    q.dkeep.alloc((size_t)q.n * 4);
+    CS_HIP(hipMemcpyAsync(q.dkeep.p, q.keep.data(), (size_t)q.n * 4, hipMemcpyHostToDevice, s->st));
+    q.bytes = need;
+    q.result_bytes = res;
+    s->budget_used += need + res;
+    place_columns(s, q, s0, q.k);
     q.status = REQ_ACTIVE;
     q.t_admit = s->now();
     return true;
 }
 
-void finish(cs_server* s, Request& q) {
-    const int64_t N = s->N, k = q.k, n = q.n;
-    DevBuf Zc((size_t)(n * k) * 8);
-    k_srv_gather<<<grid_for(n * k), 256, 0, s->st>>>(s->Xs.as<double>() + q.slot0 * N, N, q.dkeep.as<int32_t>(), n, k,
-                                                    Zc.as<double>());
-    CS_HIP(hipGetLastError());
-    CS_HIP(hipStreamSynchronize(s->st));
-    const int64_t ns = q.snps.empty() ? s->bed->M : (int64_t)q.snps.size();
-    q.ns = ns;
-    q.freq.alloc((size_t)ns * 8);
-    q.n_obs.alloc((size_t)ns * 8);
-    q.pass.alloc((size_t)ns * 4);
-    q.beta.alloc((size_t)(ns * q.T) * 8);
-    q.se_out.alloc((size_t)(ns * q.T) * 8);
-    q.p.alloc((size_t)(ns * q.T) * 8);
-    q.log10p.alloc((size_t)(ns * q.T) * 8);
-    pscan_finish(s->bed, q.keep.data(), n, n, q.T, q.c, q.L, q.B.as<double>(), Zc.as<double>(),
-                 q.snps.empty() ? nullptr : q.snps.data(), ns, q.min_maf, q.prec, q.freq.as<double>(),
-                 q.n_obs.as<int64_t>(), q.pass.as<int32_t>(), q.beta.as<double>(), q.se_out.as<double>(),
-                 q.p.as<double>(), q.log10p.as<double>(), &q.gamma, &q.ncal, s->st, s->blas, true);
-    // free the slots (P zeroed so idle columns add nothing to the shared apply)
-    CS_HIP(hipMemsetAsync(s->P.as<double>() + q.slot0 * N, 0, (size_t)(N * k) * 8, s->st));
-    CS_HIP(hipMemsetAsync(s->minv.as<double>() + q.slot0 * N, 0, (size_t)(N * k) * 8, s->st));
-    CS_HIP(hipStreamSynchronize(s->st));
-    for (int64_t c = 0; c < k; ++c) {
-        s->slot_req[(size_t)(q.slot0 + c)] = -1;
-        s->hact[(size_t)(q.slot0 + c)] = 0;
+// Place the next exact-stage chunk of q's selected markers if slots and budget allow: as many
+// rows as fit the longest free slot run and the budget left (prepared markers, gathered solution
+// and residual: three n x chunk FP64 arrays).
+bool admit_chunk(cs_server* s, Request& q) {
+    const int64_t left = (int64_t)q.rows.size() - q.row_next;
+    const int64_t per_col = 3 * q.n * 8;
+    int64_t sb = std::min<int64_t>(left, longest_free_run(s));
+    sb = std::min<int64_t>(sb, (s->budget - s->budget_used) / per_col);
+    if (sb < 1) return false;
+    const int64_t s0 = find_slots(s, sb);
+    std::vector<int32_t> sel((size_t)sb);
+    for (int64_t i = 0; i < sb; ++i) {
+        const int64_t r = q.rows[(size_t)(q.row_next + i)];
+        sel[(size_t)i] = q.snps.empty() ? (int32_t)r : q.snps[(size_t)r];
     }
-    while (s->used > 0 && s->slot_req[(size_t)(s->used - 1)] < 0) --s->used;
-    s->budget_used -= q.bytes;
-    q.B.release();
-    q.dkeep.release();
-    q.status = REQ_DONE;
-    q.t_done = s->now();
+    q.B.alloc((size_t)(q.n * sb) * 8);
+    q.bytes = per_col * sb;
+    s->budget_used += q.bytes;
+    markers_device(s->bed, q.keep.data(), q.n, sel.data(), sb, q.min_maf, q.B.as<double>(), q.n, nullptr, nullptr,
+                   s->st);
+    q.chunk = sb;
+    place_columns(s, q, s0, sb);
+    return true;
+}
+
+// The request's columns in the slots have all converged: end of phase 1 (projection,
+// calibration, fused scan, selection of the exact rows) or of an exact-stage chunk.
+void advance(cs_server* s, Request& q) {
+    const int64_t N = s->N, n = q.n;
+    if (q.phase == 1) {
+        const int64_t k = q.k;
+        DevBuf Zc((size_t)(n * k) * 8);
+        k_srv_gather<<<grid_for(n * k), 256, 0, s->st>>>(s->Xs.as<double>() + q.slot0 * N, N, q.dkeep.as<int32_t>(),
+                                                        n, k, Zc.as<double>());
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipStreamSynchronize(s->st));
+        const int64_t ns = q.snps.empty() ? s->bed->M : (int64_t)q.snps.size();
+        q.ns = ns;
+        q.freq.alloc((size_t)ns * 8);
+        q.n_obs.alloc((size_t)ns * 8);
+        q.pass.alloc((size_t)ns * 4);
+        q.beta.alloc((size_t)(ns * q.T) * 8);
+        q.se_out.alloc((size_t)(ns * q.T) * 8);
+        q.p.alloc((size_t)(ns * q.T) * 8);
+        q.log10p.alloc((size_t)(ns * q.T) * 8);
+        pscan_project(n, q.T, q.c, q.L, q.B.as<double>(), Zc.as<double>(), q.pj, s->st, s->blas);
+        Zc.release();
+        release_slots(s, q);
+        pscan_scan(s->bed, q.keep.data(), q.pj, q.snps.empty() ? nullptr : q.snps.data(), ns, q.min_maf, q.prec,
+                   q.freq.as<double>(), q.n_obs.as<int64_t>(), q.pass.as<int32_t>(), q.beta.as<double>(),
+                   q.se_out.as<double>(), q.p.as<double>(), q.log10p.as<double>(), true);
+        q.rows = pscan_select(q.log10p.as<double>(), q.pass.as<int32_t>(), ns, q.T, q.thr, true, s->st);
+        q.row_next = 0;
+        q.phase = 2;
+    } else {
+        const int64_t sb = q.chunk;
+        DevBuf Zc((size_t)(n * sb) * 8), Rc((size_t)(n * sb) * 8);
+        k_srv_gather<<<grid_for(n * sb), 256, 0, s->st>>>(s->Xs.as<double>() + q.slot0 * N, N, q.dkeep.as<int32_t>(),
+                                                         n, sb, Zc.as<double>());
+        k_srv_gather<<<grid_for(n * sb), 256, 0, s->st>>>(s->R.as<double>() + q.slot0 * N, N, q.dkeep.as<int32_t>(),
+                                                         n, sb, Rc.as<double>());
+        CS_HIP(hipGetLastError());
+        // the recurrence residual stands for B - V Z (its drift from the true residual is at the
+        // operator's quantisation level)
+        pscan_exact(q.pj, sb, q.rows.data() + q.row_next, q.B.as<double>(), Zc.as<double>(), Rc.as<double>(), true,
+                    q.beta.as<double>(), q.se_out.as<double>(), q.p.as<double>(), q.log10p.as<double>(), true, s->st,
+                    s->blas);
+        q.exact_iterations += q.chunk_iterations;
+        q.row_next += sb;
+        release_slots(s, q);
+    }
+    if (q.row_next >= (int64_t)q.rows.size()) {
+        q.dkeep.release();
+        q.pj.R.release();
+        q.pj.ZX.release();
+        q.status = REQ_DONE;
+        q.t_done = s->now();
+    }
 }
 
 }  // namespace
@@ -407,6 +518,7 @@ int cs_server_submit(cs_server* s, const cs_request* r, int64_t* id) {
         q->tol = r->rel_tol > 0.0 ? r->rel_tol : 1e-6;
         q->max_iter = r->max_iter > 0 ? r->max_iter : 200;
         q->prec = r->prec ? r->prec : CS_PREC_INT8_L3;
+        q->thr = pscan_threshold(r->exact_log10p);
         q->t_submit = s->now();
         if (id) *id = q->id;
         s->pending.push_back(q->id);
@@ -418,6 +530,16 @@ int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t*
     return guarded([&] {
         if (!s) throw InvalidArgument("cs_server_step: null handle");
         int32_t completed = 0;
+        // exact-stage chunks of active requests first (they hold results already), in id order
+        for (auto& kv : s->reqs) {
+            Request& q = *kv.second;
+            if (q.status != REQ_ACTIVE || q.phase != 2 || q.slot0 >= 0) continue;
+            try {
+                (void)admit_chunk(s, q);
+            } catch (const std::exception& e) {
+                fail(s, q, e.what());
+            }
+        }
         // admission in arrival order (a request that does not fit waits; later ones may pass it)
         for (auto it = s->pending.begin(); it != s->pending.end();) {
             Request& q = *s->reqs[*it];
@@ -425,9 +547,7 @@ int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t*
             try {
                 ok = admit(s, q);
             } catch (const std::exception& e) {
-                q.status = REQ_FAILED;
-                q.error = e.what();
-                q.t_done = s->now();
+                fail(s, q, e.what());
                 it = s->pending.erase(it);
                 continue;
             }
@@ -463,33 +583,38 @@ int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t*
                 if (s->hact[(size_t)c]) rzo[(size_t)c] = rzn[(size_t)c];
             CS_HIP(hipMemcpyAsync(s->rz.p, rzo.data(), (size_t)k * 8, hipMemcpyHostToDevice, s->st));
             CS_HIP(hipStreamSynchronize(s->st));
-            // per-request iteration counts and convergence
+            // per-request iteration counts and convergence (the exact stage solves to its own
+            // tolerance)
             std::map<int64_t, bool> stepped;
             for (int64_t c = 0; c < k; ++c)
                 if (s->hact[(size_t)c] && s->slot_req[(size_t)c] >= 0) stepped[s->slot_req[(size_t)c]] = true;
-            for (auto& kv : stepped) s->reqs[kv.first]->iterations++;
+            for (auto& kv : stepped) {
+                Request& q = *s->reqs[kv.first];
+                if (q.phase == 1) q.iterations++;
+                else q.chunk_iterations++;
+            }
             for (int64_t c = 0; c < k; ++c) {
                 if (!s->hact[(size_t)c] || s->slot_req[(size_t)c] < 0) continue;
                 const Request& q = *s->reqs[s->slot_req[(size_t)c]];
                 const double b2 = s->hbb[(size_t)c];
-                if (b2 == 0.0 || std::sqrt(s->hrr[(size_t)c] / b2) <= q.tol || q.iterations >= q.max_iter)
+                const double tol = q.phase == 1 ? q.tol : pscan_exact_tol(q.tol);
+                const int its = q.phase == 1 ? q.iterations : q.chunk_iterations;
+                if (b2 == 0.0 || std::sqrt(s->hrr[(size_t)c] / b2) <= tol || its >= q.max_iter)
                     s->hact[(size_t)c] = 0;
             }
         }
-        // finish requests whose columns are all converged
+        // advance requests whose columns have all converged
         for (auto& kv : s->reqs) {
             Request& q = *kv.second;
-            if (q.status != REQ_ACTIVE) continue;
+            if (q.status != REQ_ACTIVE || q.slot0 < 0) continue;
             bool live = false;
-            for (int64_t c = 0; c < q.k; ++c) live = live || s->hact[(size_t)(q.slot0 + c)];
+            for (int64_t c = 0; c < q.cols; ++c) live = live || s->hact[(size_t)(q.slot0 + c)];
             if (live) continue;
             try {
-                finish(s, q);
-                ++completed;
+                advance(s, q);
+                if (q.status == REQ_DONE) ++completed;
             } catch (const std::exception& e) {
-                q.status = REQ_FAILED;
-                q.error = e.what();
-                q.t_done = s->now();
+                fail(s, q, e.what());
             }
         }
         int32_t na = 0;
@@ -513,9 +638,13 @@ int cs_server_result(cs_server* s, int64_t id, int32_t* status, double* freq, in
             stats->t_admit = q.t_admit;
             stats->t_done = q.t_done;
             stats->iterations = q.iterations;
-            stats->gamma = q.gamma;
-            stats->n_calibration = q.ncal;
+            stats->gamma = q.pj.gamma;
+            stats->n_calibration = q.pj.ncal;
             stats->n_columns = q.k;
+            stats->n_exact = (int64_t)q.rows.size();
+            stats->exact_iterations = q.exact_iterations;
+            stats->calibration_rel_sd = q.pj.rel_sd;
+            stats->dlog10p_bound_below = pscan_bound_below(q.thr, q.pj.rel_sd);
         }
         const bool wants = freq || n_obs || pass || beta || se || p || log10p;
         if (q.status == REQ_FAILED && wants) throw RuntimeError("request failed: " + q.error);
@@ -542,6 +671,7 @@ int cs_server_release(cs_server* s, int64_t id) {
         if (it == s->reqs.end()) return;
         if (it->second->status == REQ_ACTIVE || it->second->status == REQ_PENDING)
             throw InvalidArgument("cs_server_release: request still running");
+        free_results(s, *it->second);
         s->reqs.erase(it);
     });
 }

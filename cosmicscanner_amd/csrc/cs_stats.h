@@ -1,4 +1,4 @@
-// Upper tail of the chi-square distribution on the device, in FP64: gamma_q / chisq_prob of
+// Upper tail of the chi-square distribution in FP64 (device and host): gamma_q / chisq_prob of
 // src/lmm_engine.cpp:22-64 (the same routine is repeated in src/mvlmm_engine.cpp:12-52):
 // series for x < a + 1, Lentz continued fraction otherwise, relative tolerance 1e-7, at most 100
 // terms, 0 when neither converges; stat <= 0 gives 1.
@@ -7,7 +7,7 @@
 
 namespace csamd {
 
-__device__ inline double d_gamma_q(double a, double x) {
+__host__ __device__ inline double d_gamma_q(double a, double x) {
     if (x < 0.0 || a <= 0.0) return 1.0;
     if (x < a + 1.0) {
         double ap = a, sum = 1.0 / a, del = sum;
@@ -35,7 +35,7 @@ __device__ inline double d_gamma_q(double a, double x) {
     return 0.0;
 }
 
-__device__ inline double d_chisq_prob(double stat, int df) {
+__host__ __device__ inline double d_chisq_prob(double stat, int df) {
     if (stat <= 0) return 1.0;
     return d_gamma_q(df / 2.0, stat / 2.0);
 }

@@ -231,22 +231,35 @@ int cs_markers_device(cs_bed* bed, const int32_t* keep, int64_t n_keep, const in
  * P y = Z_Y - Z_X (X^T Z_X)^-1 X^T Z_Y (the GLS projection of LmmEngine::scan,
  * src/lmm_engine.cpp:1363-1525), calibration gamma = mean_cal g^T P g / g~^T g~ (g~ = g projected
  * off X), and per SNP and trait: score = g^T P y_t, var = gamma g~^T g~, beta, se, p, log10p
- * (n_snps x T row-major; NaN when not tested).  Y: n x T FP64 (ldy); X: n x c float32 (ldx, the
- * intercept included, 1 <= c <= 32).  cal_snps: calibration SNPs (those failing the filter are
- * skipped). */
+ * (n_snps x T row-major; NaN when not tested).  Exact stage: every tested SNP whose largest
+ * -log10 P over the traits reaches exact_log10p is recomputed with its exact GLS variance
+ * g^T P g (a second multi-RHS PCG solve over those markers, tolerance min(100 rel_tol, 1e-4), and
+ * the variational form g^T z + z^T r of g^T V^-1 g, accurate to the square of the residual) and
+ * score g^T P y_t in FP64.  exact_log10p: 0 -> 2 (the default), < 0 -> every tested SNP,
+ * +inf -> off.  Below the threshold the calibrated statistics stand; dlog10p_bound_below states
+ * their |d(-log10 P)| bound from the spread of the calibration markers' ratios (4 standard
+ * deviations of the variance ratio at the threshold statistic; 0 when every SNP is exact, NaN when
+ * the stage is off or fewer than two markers calibrate).  Y: n x T FP64 (ldy); X: n x c float32
+ * (ldx, the intercept included, 1 <= c <= 32).  cal_snps: calibration SNPs (those failing the
+ * filter are skipped). */
 typedef struct cs_pscan_info {
     double gamma;
     int64_t n_calibration;
     int pcg_iterations;
     double max_rel_residual;
+    int64_t n_exact;              /* SNPs recomputed by the exact stage */
+    int exact_pcg_iterations;     /* largest iteration count of its solves */
+    double calibration_rel_sd;    /* relative standard deviation of the calibration ratios */
+    double exact_log10p;          /* threshold used (-1: every SNP; inf: off) */
+    double dlog10p_bound_below;   /* stated bound below the threshold */
 } cs_pscan_info;
 int cs_pcg_projected_scan(cs_kinship* K, cs_bed* bed, const int32_t* keep, int64_t n_keep,
                           const double* Y, int64_t ldy, int64_t T, const float* X, int64_t ldx,
                           int64_t c, double sigma_g, double sigma_e, const int32_t* cal_snps,
                           int64_t n_cal, const int32_t* snps, int64_t n_snps, double min_maf,
-                          double rel_tol, int max_iter, int prec, double* freq, int64_t* n_obs,
-                          int32_t* pass, double* beta, double* se, double* p, double* log10p,
-                          cs_pscan_info* info);
+                          double rel_tol, int max_iter, int prec, double exact_log10p, double* freq,
+                          int64_t* n_obs, int32_t* pass, double* beta, double* se, double* p,
+                          double* log10p, cs_pscan_info* info);
 
 /* ---- Randomized low-rank null model on the kinship operator (the matrix-free GWAS path).
  * LmmEngine::prepareLowRankMatrixFree (src/lmm_engine.cpp:897-1095): Omega from
@@ -282,9 +295,10 @@ int cs_lowrank_scan(cs_lowrank* lr, cs_bed* bed, const int32_t* keep, int64_t n_
  * continuously batched multi-RHS PCG columns (one K.V apply per iteration for every admitted
  * request; the kinship matrix of a cohort is the restriction of the all-sample operator since the
  * standardisation uses all-sample statistics) and a PCG-projected fused scan per finished request
- * (cs_pcg_projected_scan's estimator).  A request's results do not depend on which other
- * requests share the batch.  No reference counterpart: the reference runs one phenotype per
- * process. */
+ * (cs_pcg_projected_scan's estimator, exact stage included: a request's selected markers come back
+ * into the batch as columns of a second phase, in chunks that fit the free slots).  A request's
+ * results do not depend on which other requests share the batch.  No reference counterpart: the
+ * reference runs one phenotype per process. */
 typedef struct cs_request {
     const int32_t* keep; int64_t n_keep;     /* cohort (distinct samples; NULL/0 = all) */
     const double* Y; int64_t ldy; int64_t n_traits;   /* n x T phenotypes over the cohort */
@@ -296,15 +310,21 @@ typedef struct cs_request {
     double rel_tol;                          /* PCG relative residual (<= 0: 1e-6) */
     int max_iter;                            /* (<= 0: 200) */
     int prec;                                /* (0: CS_PREC_INT8_L3) */
+    double exact_log10p;                     /* exact-stage threshold, as cs_pcg_projected_scan */
 } cs_request;
 typedef struct cs_request_stats {
     double t_submit, t_admit, t_done;        /* seconds since the server was created */
-    int iterations;
+    int iterations;                          /* PCG iterations of the main solve */
     double gamma;
     int64_t n_calibration, n_columns;
+    int64_t n_exact;                         /* SNPs recomputed by the exact stage */
+    int exact_iterations;                    /* PCG iterations of the exact stage, all chunks */
+    double calibration_rel_sd, dlog10p_bound_below;
 } cs_request_stats;
 /* max_columns: PCG column slots (N x max_columns x 8 doubles of HBM); hbm_budget_bytes: bound on
- * the per-request buffers of admitted requests (<= 0: unbounded). */
+ * the per-request device buffers (<= 0: unbounded): a request is admitted only when its solve
+ * buffers and its result arrays (n_snps x (4 T + 3) x 8 bytes) fit, the results stay charged until
+ * cs_server_release, and exact-stage chunks are sized to what is left. */
 int cs_server_create(cs_bed* bed, const int32_t* variants, int64_t n_variants, int64_t max_columns,
                      int64_t hbm_budget_bytes, cs_server** out);
 int cs_server_destroy(cs_server* srv);
@@ -411,6 +431,7 @@ typedef struct cs_gwas_mf_options {
     double pcg_tol;        /* mode 1: relative residual, 1e-6 */
     int pcg_max_iter;      /* mode 1: 200 */
     int n_calibration;     /* mode 1: calibration SNPs per chromosome, 32 */
+    double exact_log10p;   /* mode 1: exact-stage threshold (cs_pcg_projected_scan; 0 = 2) */
 } cs_gwas_mf_options;
 int cs_gwas_matrix_free_run(const char* bfile, const double* y, const double* cov, int64_t ldc,
                             int64_t n_cov, const cs_gwas_mf_options* opt, const char* out_prefix,

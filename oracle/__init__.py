@@ -467,10 +467,15 @@ def prepared_markers(bed, N, M, keep, snps, min_maf=0.01):
     return G, ok
 
 
-def pcg_projected_scan(bed, N, M, keep, variants, Y, X, sigma_g, sigma_e, cal_snps, min_maf=0.01):
+def pcg_projected_scan(bed, N, M, keep, variants, Y, X, sigma_g, sigma_e, cal_snps, min_maf=0.01,
+                       exact_log10p=None):
     """The projected scan with exact dense solves: V = sigma_g K + sigma_e I, P y, calibration
     gamma over cal_snps, var = gamma g~^T g~.  Returns (stats per trait dict of (M,) arrays,
-    gamma); SNPs failing the filter or var <= 1e-12 are NaN."""
+    gamma); SNPs failing the filter or var <= 1e-12 are NaN.
+
+    exact_log10p (None: no exact stage; 0 -> 2; < 0: every tested SNP; inf: off): SNPs that pass
+    and whose largest calibrated -log10 P over the traits reaches the threshold get the textbook
+    GLS statistics instead (score g^T P y, var g^T P g with the explicit P, as gls_scan)."""
     K = dense_kinship(bed, N, M, keep, variants)
     n = K.shape[0]
     Y = np.asarray(Y, dtype=np.float64).reshape(n, -1)
@@ -499,6 +504,25 @@ def pcg_projected_scan(bed, N, M, keep, variants, Y, X, sigma_g, sigma_e, cal_sn
         for key in ("beta", "se", "p", "log10p"):
             st[key] = np.where(bad, np.nan, st[key])
         out.append(st)
+    if exact_log10p is not None and not np.isinf(exact_log10p):
+        thr = 2.0 if exact_log10p == 0 else (-1.0 if exact_log10p < 0 else float(exact_log10p))
+        L = np.column_stack([o["log10p"] for o in out])
+        if thr < 0:
+            sel = ps.astype(bool) & ~np.isnan(L[:, 0])
+        else:
+            with np.errstate(invalid="ignore"):
+                sel = ps.astype(bool) & np.any(L >= thr, axis=1)
+        idx = np.nonzero(sel)[0]
+        if idx.size:
+            Gs, _ = prepared_markers(bed, N, M, keep, idx, min_maf)
+            Vi = np.linalg.inv(V)
+            XtVi = Xd.T @ Vi
+            P = Vi - XtVi.T @ np.linalg.solve(XtVi @ Xd, XtVi)
+            vs = np.einsum("ij,ij->j", Gs, P @ Gs)
+            for t in range(T):
+                ex = _stats_from(Gs.T @ (P @ Y[:, t]), vs)
+                for key in ("beta", "se", "p", "log10p"):
+                    out[t][key][idx] = ex[key]
     return out, gamma, freq
 
 
@@ -571,6 +595,50 @@ class StdNormal:
         mult = math.sqrt(-2.0 * math.log(r2) / r2)
         self.saved = x * mult
         return y * mult
+
+
+def std_uniform_int(eng, b):
+    """libstdc++'s std::uniform_int_distribution<unsigned long>{0, b} over a 64-bit engine:
+    Lemire's nearly-divisionless multiply-shift (ACM TOMACS 29(1), 2019) with a 128-bit product
+    of the engine's draw and b + 1."""
+    rng = b + 1
+    prod = eng() * rng
+    low = prod & ((1 << 64) - 1)
+    if low < rng:
+        thr = ((1 << 64) - rng) % rng
+        while low < thr:
+            prod = eng() * rng
+            low = prod & ((1 << 64) - 1)
+    return prod >> 64
+
+
+def std_shuffle(seq, seed):
+    """std::shuffle(first, last, std::mt19937_64(seed)) as libstdc++ does it for ranges whose
+    squared length fits the engine's range: an even length swaps the second element first with a
+    {0, 1} draw, then the elements go in pairs, both swap positions from one draw over the product
+    of the two ranges (x / (r + 1), x % (r + 1) for r = position + 1).  Used by the matrix-free
+    driver's calibration sample (cs_gwas.cpp).  Pinned by known answers of this image's standard
+    library (tests/golden/std_shuffle_mt64.txt, made by tests/golden/make_std_shuffle.cpp)."""
+    v = list(seq)
+    n = len(v)
+    if n < 2:
+        return v
+    assert n * n < (1 << 64)
+    eng = MT19937_64(seed)
+    i = 1
+    if n % 2 == 0:
+        j = std_uniform_int(eng, 1)
+        v[i], v[j] = v[j], v[i]
+        i += 1
+    while i < n:
+        r = i + 1                       # swap range of element i: 0..i
+        x = std_uniform_int(eng, r * (r + 1) - 1)
+        a, b = x // (r + 1), x % (r + 1)
+        v[i], v[a] = v[a], v[i]
+        i += 1
+        v[i], v[b] = v[b], v[i]
+        i += 1
+    return v
 
 
 def sketch_omega(seed, n, s):

@@ -258,14 +258,22 @@ def test_gwas_matrix_free_pcg(csa, synth_gwas, tmp_path):
     f = means / 2
     grm = [j for j in range(d["M"]) if inv[j] != 0 and min(f[j], 1 - f[j]) >= 0.01]
     got = {r[1]: float(r[10]) for r in rg}
+    got_beta = {r[1]: float(r[7]) for r in rg}
     for chrom in ("1", "10", "2"):
         snps = np.array([j for j in range(d["M"]) if d["chroms"][j] == chrom], dtype=np.int32)
         var = np.array([j for j in grm if d["chroms"][j] != chrom], dtype=np.int32)
-        # the driver draws its calibration SNPs with std::shuffle (not restated here), so gamma
-        # differs by the calibration sample: compare the ordering of the calibrated scan
+        # the driver's calibration sample: std::shuffle of the chromosome's SNPs with
+        # std::mt19937_64(seed = 42), the first n_calibration
+        cal = np.array(oracle.std_shuffle(snps.tolist(), 42)[:24], dtype=np.int32)
         st, gamma, _ = oracle.pcg_projected_scan(d["body"], d["N"], d["M"], keep, var, d["y"][keep], X, 0.4, 0.6,
-                                                 snps[:24])
+                                                 cal, exact_log10p=0.0)
         lo = st[0]["log10p"][snps]
-        lg = np.array([got.get(f"rs{j:06d}", np.nan) for j in snps])
-        m = ~np.isnan(lo) & ~np.isnan(lg)
-        assert np.corrcoef(lo[m], lg[m])[0, 1] > 0.999
+        ids = [f"rs{j:06d}" for j in snps]
+        lg = np.array([got.get(i, np.nan) for i in ids])
+        assert np.array_equal(np.isnan(lo), np.isnan(lg))
+        m = ~np.isnan(lo)
+        # .assoc.txt holds 6 significant digits
+        assert np.max(np.abs(lo[m] - lg[m]) / np.maximum(1.0, np.abs(lo[m]))) <= 1e-3
+        bo = st[0]["beta"][snps][m]
+        bg = np.array([got_beta[i] for i, mm in zip(ids, m) if mm])
+        assert np.max(np.abs(bo - bg) / st[0]["se"][snps][m]) <= 1e-3 + 1e-5

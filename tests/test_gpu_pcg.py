@@ -4,9 +4,14 @@
   solutions of (sigma_g K + sigma_e I) X = B against a dense FP64 solve of the oracle's K,
   relative error <= 2e-5 (the operator's 23-bit RHS quantisation, amplified by cond(V) < 10);
   the reported true residual meets the tolerance.
-* Projected scan: against the oracle's dense-solve restatement of the same estimator (exact V^-1,
-  same calibration markers): |d log10 P| <= 1e-3 (the stated bound for the mixed-precision path),
-  gamma relative 1e-4.  Its distance to exact GLS (var = g^T P g per SNP) is printed, not gated.
+* Projected scan, calibrated statistics (exact stage off): against the oracle's dense-solve
+  restatement of the same estimator (exact V^-1, same calibration markers): |d log10 P| <= 1e-3,
+  gamma relative 1e-4.
+* Projected scan with the exact stage: every SNP whose largest -log10 P reaches the threshold
+  (default 2) against textbook GLS with an explicit dense V^-1 (the definition of LmmEngine::scan,
+  src/lmm_engine.cpp:1363-1525): |d log10 P| <= 1e-3 and |d beta| <= 1e-3 SE, at n = 400 and at
+  n = 2000; below the threshold the distance stays within the reported bound.  With threshold -1
+  every tested SNP meets the 1e-3 bound.
 """
 import numpy as np
 import pytest
@@ -88,7 +93,9 @@ def test_projected_scan_vs_oracle(csa, data):
     Y = rng.standard_normal((n, 3)) + X[:, 1:2]
     cal = rng.choice(np.arange(2000, 2600), 24, replace=False).astype(np.int32)
     snps = np.arange(2000, 2600, dtype=np.int32)
-    out = csa.pcg_projected_scan(op, r, keep, Y, X, 0.45, 0.55, cal, snps=snps, rel_tol=1e-9, max_iter=500)
+    out = csa.pcg_projected_scan(op, r, keep, Y, X, 0.45, 0.55, cal, snps=snps, rel_tol=1e-9, max_iter=500,
+                                 exact_log10p=float("inf"))
+    assert out["n_exact"] == 0 and np.isnan(out["dlog10p_bound_below"])
     ref, gamma, freq = oracle.pcg_projected_scan(body, N, M, keep, loco, Y, X, 0.45, 0.55, cal)
     print(f"gamma {out['gamma']:.6f} (oracle {gamma:.6f}), {out['pcg_iterations']} PCG iterations")
     assert abs(out["gamma"] / gamma - 1) <= 1e-4
@@ -100,13 +107,81 @@ def test_projected_scan_vs_oracle(csa, data):
         assert np.max(np.abs(lp[m] - lr[m])) <= 1e-3
         b, br, sr = out["beta"][:, t][m], ref[t]["beta"][snps][m], ref[t]["se"][snps][m]
         assert np.max(np.abs(b - br) / sr) <= 1e-3
-    # distance of the calibrated variance to exact GLS (informational)
+    # the default exact stage against textbook GLS, and against the oracle's restatement of the
+    # combined estimator
+    ex_out = csa.pcg_projected_scan(op, r, keep, Y, X, 0.45, 0.55, cal, snps=snps, rel_tol=1e-9, max_iter=500)
     K = oracle.dense_kinship(body, N, M, keep, loco)
     Gm, okm = oracle.prepared_markers(body, N, M, keep, snps)
-    ex = oracle.gls_scan(0.45 * K + 0.55 * np.eye(n), X.astype(np.float64), Y[:, 0], Gm[:, okm])
-    lp = out["log10p"][:, 0][okm]
-    mm = ~np.isnan(lp) & ex["tested"]
-    print(f"calibrated vs exact GLS: max |d log10 P| = {np.max(np.abs(lp[mm] - ex['log10p'][mm])):.3e}")
+    check_exact_stage(ex_out, out, K, X, Y, Gm, okm, 0.45, 0.55, 2.0)
+    refx, _, _ = oracle.pcg_projected_scan(body, N, M, keep, loco, Y, X, 0.45, 0.55, cal, exact_log10p=0.0)
+    for t in range(3):
+        lp, lr = ex_out["log10p"][:, t], refx[t]["log10p"][snps]
+        m = ~np.isnan(lr)
+        assert np.max(np.abs(lp[m] - lr[m])) <= 1e-3
+
+
+def check_exact_stage(out, calib, K, X, Y, Gm, okm, sg, se, thr):
+    """Rows whose calibrated (exact stage off: `calib`) largest -log10 P over traits reaches thr
+    equal textbook GLS within 1e-3 in -log10 P and 1e-3 SE in beta; rows below keep their
+    calibrated values and stay within the reported bound (thr >= 0)."""
+    n = K.shape[0]
+    V = sg * K + se * np.eye(n)
+    T = Y.shape[1]
+    L = out["log10p"]
+    Lc = calib["log10p"]
+    with np.errstate(invalid="ignore"):
+        sel = calib["passes"] & (np.any(Lc >= thr, axis=1) if thr >= 0 else ~np.isnan(Lc[:, 0]))
+    assert np.array_equal(L[~sel], Lc[~sel], equal_nan=True)
+    assert out["n_exact"] == int(sel.sum())
+    worst_above, worst_below = 0.0, 0.0
+    for t in range(T):
+        ex = oracle.gls_scan(V, X.astype(np.float64), Y[:, t], Gm)
+        m = okm & ex["tested"] & ~np.isnan(L[:, t])
+        d = np.abs(L[:, t] - ex["log10p"])
+        db = np.abs(out["beta"][:, t] - ex["beta"]) / ex["se"]
+        if np.any(m & sel):
+            worst_above = max(worst_above, float(np.max(d[m & sel])))
+            assert np.max(db[m & sel]) <= 1e-3
+        if np.any(m & ~sel):
+            worst_below = max(worst_below, float(np.max(d[m & ~sel])))
+    print(f"exact stage: {int(sel.sum())} SNPs at or above {thr}, max |d log10 P| {worst_above:.3e}; "
+          f"below: {worst_below:.3e} (stated bound {out['dlog10p_bound_below']:.3e}, "
+          f"calibration spread {out['calibration_rel_sd']:.3e}), {out['exact_pcg_iterations']} PCG iterations",
+          flush=True)
+    assert worst_above <= 1e-3
+    if thr >= 0:
+        assert worst_below <= out["dlog10p_bound_below"]
+
+
+def test_projected_scan_exact_gls_n2000(csa):
+    """n = 2000 samples, 6000 LOCO kinship SNPs, 1500 scanned SNPs with 12 causal ones: the exact
+    stage at the default threshold and at -1 (every SNP) against textbook GLS."""
+    N, M = 2100, 7500
+    body = oracle.gen_hwe_bed(2026, N, M, 0.05, 0.5, 0.01)
+    rng = np.random.default_rng(21)
+    keep = np.sort(rng.choice(N, 2000, replace=False)).astype(np.int32)
+    n = keep.size
+    loco = np.arange(0, 6000, dtype=np.int32)
+    snps = np.arange(6000, 7500, dtype=np.int32)
+    r = csa.PlinkReader.from_body(body, N, M)
+    op = csa.PlinkKinshipOperator(r, keep, loco)
+    Gm, okm = oracle.prepared_markers(body, N, M, keep, snps)
+    X = np.column_stack([np.ones(n), rng.standard_normal((n, 2))]).astype(np.float32)
+    causal = rng.choice(np.nonzero(okm)[0], 12, replace=False)
+    Y = rng.standard_normal((n, 2)) + X[:, 1:2]
+    Y += Gm[:, causal] @ rng.normal(0.0, 0.25, (12, 2))
+    cal = rng.choice(snps, 24, replace=False).astype(np.int32)
+    K = oracle.dense_kinship(body, N, M, keep, loco)
+    calib = csa.pcg_projected_scan(op, r, keep, Y, X, 0.4, 0.6, cal, snps=snps, rel_tol=1e-6, max_iter=300,
+                                   exact_log10p=float("inf"))
+    for thr in (0.0, -1.0):
+        out = csa.pcg_projected_scan(op, r, keep, Y, X, 0.4, 0.6, cal, snps=snps, rel_tol=1e-6, max_iter=300,
+                                     exact_log10p=thr)
+        assert out["max_rel_residual"] <= 1e-6
+        check_exact_stage(out, calib, K, X, Y, Gm, okm, 0.4, 0.6, 2.0 if thr == 0 else -1.0)
+        assert out["n_exact"] >= 12
+    op.close()
+    r.close()
 
 
 def test_pcg_refines_below_operator_floor(csa, data):

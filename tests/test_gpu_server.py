@@ -4,9 +4,15 @@ A request's columns are numerically independent of the rest of the batch (per-co
 quantisation scales, exact integer products, per-column PCG recurrences and fixed-order dots), so
 its results must be BIT-IDENTICAL whether it runs alone or mixed with other requests that arrive
 while it is on the GPU -- also when a small column capacity forces requests to queue and be
-admitted as slots free up.  Against the standalone projected scan with a cohort-restricted
-operator (same estimator, compact rows, different dot partition) the bound is
-|d log10 P| <= 1e-5 (PCG tolerance 1e-9).
+admitted as slots free up.  The exact stage of a
+request (its selected markers solved as a second phase of batch columns, in chunks that depend
+on the free slots) keeps that property: its per-column dots run one workgroup per column.
+Against the standalone projected scan with a cohort-restricted operator (same estimator, compact
+rows, different dot partition) the bound is |d log10 P| <= 1e-5 (PCG tolerance 1e-9).
+
+Admission and failure: a request that fails after admission (no calibration marker passes the
+filter) gives its slots and budget back, so later requests still run; result arrays stay charged
+to the HBM budget until the client releases them.
 """
 import numpy as np
 import pytest
@@ -94,10 +100,11 @@ def test_server_matches_projected_scan(csa, world):
     m = ~np.isnan(ref["log10p"])
     assert np.array_equal(np.isnan(got["log10p"]), ~m)
     assert np.max(np.abs(got["log10p"][m] - ref["log10p"][m])) <= 1e-5
-    # and the oracle's dense-solve restatement of the estimator
+    assert got["stats"]["n_exact"] == ref["n_exact"] and ref["n_exact"] > 0
+    # and the oracle's dense-solve restatement of the estimator, exact stage included
     body = reader.download()
     st, gamma, _ = oracle.pcg_projected_scan(body, reader.num_samples, reader.num_snps, q["keep"], None, q["Y"], q["X"],
-                                             q["sigma2_g"], q["sigma2_e"], q["cal_snps"])
+                                             q["sigma2_g"], q["sigma2_e"], q["cal_snps"], exact_log10p=0.0)
     assert abs(got["stats"]["gamma"] / gamma - 1) <= 1e-4
     for t in range(q["Y"].shape[1]):
         lo = st[t]["log10p"][q["snps"]]
@@ -120,3 +127,58 @@ def test_server_rejects_bad_requests(csa, world):
     assert srv2.status(rid)[0] == -1
     with pytest.raises(RuntimeError, match="more columns"):
         srv2.result(rid)
+
+
+def test_failed_request_releases_slots(csa, world):
+    reader, reqs = world
+    q = dict(reqs[0])
+    T, c = q["Y"].shape[1], q["X"].shape[1]
+    k_good = T + c + 16
+    # capacity for one good request only: leaked slots of the failed one would block it for good
+    srv = csa.ScanServer(reader, max_columns=k_good)
+    bad = srv.submit(**dict(q, min_maf=0.6))        # every SNP fails the filter: no calibration marker
+    srv.step()
+    good = srv.submit(**q)
+    for _ in range(2000):
+        a, p, _ = srv.step()
+        if a == 0 and p == 0:
+            break
+    assert srv.status(bad)[0] == -1
+    with pytest.raises(RuntimeError, match="calibration"):
+        srv.result(bad)
+    assert srv.status(good)[0] == 2
+    same(srv.result(good), run_alone(csa, reader, q))
+    srv.release(bad)
+    srv.release(good)
+
+
+def test_budget_holds_unreleased_results(csa, world):
+    reader, reqs = world
+    N = reader.num_samples
+    rng = np.random.default_rng(5)
+    T, c = 2, 2
+    X = np.column_stack([np.ones(N), rng.standard_normal(N)]).astype(np.float32)
+    snps = np.arange(0, 700, dtype=np.int32)
+    base = dict(X=X, sigma2_g=0.5, sigma2_e=0.5, cal_snps=np.arange(1000, 1016, dtype=np.int32), snps=snps,
+                rel_tol=1e-8, max_iter=400)
+    qa = dict(base, Y=rng.standard_normal((N, T)))
+    qb = dict(base, Y=rng.standard_normal((N, T)))
+    k = T + c + 16
+    need = 3 * N * k * 8                   # solve buffers while admitted
+    res = snps.size * (4 * T + 3) * 8      # result arrays until release
+    srv = csa.ScanServer(reader, max_columns=64, hbm_budget_bytes=need + res + res // 2)
+    ia, ib = srv.submit(**qa), srv.submit(**qb)
+    for _ in range(2000):
+        srv.step()
+        if srv.status(ia)[0] == 2:
+            break
+    assert srv.status(ia)[0] == 2
+    for _ in range(30):
+        srv.step()
+    assert srv.status(ib)[0] == 0          # A's results still hold the budget
+    got_a = srv.result(ia)
+    srv.release(ia)
+    srv.run_until_idle()
+    assert srv.status(ib)[0] == 2
+    same(got_a, run_alone(csa, reader, qa))
+    same(srv.result(ib), run_alone(csa, reader, qb))

@@ -378,6 +378,23 @@ def test_std_normal_known_answers():
     assert om[0, 0] == blocks[42][0] and om[2, 0] == blocks[42][2] and om[0, 1] == blocks[42][3]
 
 
+def test_std_shuffle_known_answers():
+    # tests/golden/std_shuffle_mt64.txt: this image's libstdc++ std::shuffle over
+    # std::mt19937_64 (the matrix-free driver's calibration draw), made by make_std_shuffle.cpp;
+    # odd and even lengths, both seeds.
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "std_shuffle_mt64.txt")
+    lines = open(path).read().splitlines()
+    assert len(lines) == 28
+    for head, body in zip(lines[::2], lines[1::2]):
+        t = head.split()
+        seed, n = int(t[1]), int(t[3])
+        assert oracle.std_shuffle(range(n), seed) == [int(x) for x in body.split()]
+    # a permutation of the input whatever the values
+    v = list(range(100, 137))
+    assert sorted(oracle.std_shuffle(v, 3)) == v
+
+
 def test_lowrank_exact_when_rank_covers_kinship():
     # With K of rank < sketch size the sketch is exact: the leading eigenpairs are K's own, r0 = 0
     # beyond them, and the low-rank REML equals the dense REML on the same model.
