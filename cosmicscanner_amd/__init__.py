@@ -81,6 +81,7 @@ def _load():
         "cs_kinship_set_missing_mode": ([_vp, ctypes.c_int], ctypes.c_int),
         "cs_kinship_apply": ([_vp, _dp, _i64, _i64, _dp, _i64], ctypes.c_int),
         "cs_kinship_apply_device": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
+        "cs_kinship_apply_device_noalloc": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
         "cs_kinship_trace": ([_vp, _dp], ctypes.c_int),
         "cs_kinship_diag": ([_vp, _dp], ctypes.c_int),
         "cs_kinship_last_timing": ([_vp, _dp], ctypes.c_int),
@@ -398,6 +399,11 @@ class PlinkKinshipOperator:
     def apply_device(self, dX_ptr, ldx, k, dY_ptr, ldy):
         """Y = K X on device buffers (FP64, column-major) -- no host transfer."""
         _check(lib().cs_kinship_apply_device(self._h, dX_ptr, ldx, k, dY_ptr, ldy))
+
+    def apply_device_noalloc(self, dX_ptr, ldx, k, dY_ptr, ldy):
+        """Y = K X on device buffers through the entry point that allocates nothing (fixed
+        per-GPU workspace; see cs_kinship_apply_device_noalloc)."""
+        _check(lib().cs_kinship_apply_device_noalloc(self._h, dX_ptr, ldx, k, dY_ptr, ldy))
 
     def trace(self):
         t = ctypes.c_double()

@@ -15,6 +15,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <mutex>
+
 #include "cs_common.h"
 #include "cs_geno_mfma.h"
 #include "cs_ops.h"
@@ -662,19 +664,23 @@ void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const Ge
 
 void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, const Tiling& tl,
                    hipStream_t st, DevBuf& mx, DevBuf& scl, int8_t* const* B) {
-    const double qmax = limbs_qmax(tl.L);
     mx.ensure((size_t)k * sizeof(unsigned long long));
     scl.ensure((size_t)2 * k * sizeof(double));
-    CS_HIP(hipMemsetAsync(mx.p, 0, (size_t)k * sizeof(unsigned long long), st));
+    quantize_cols_into(W, nW, rows, k, tl, st, mx.as<unsigned long long>(), scl.as<double>(), B);
+}
+
+void quantize_cols_into(const double* const* W, int nW, int64_t rows, int64_t k, const Tiling& tl,
+                        hipStream_t st, unsigned long long* mxp, double* sclp, int8_t* const* B) {
+    const double qmax = limbs_qmax(tl.L);
+    CS_HIP(hipMemsetAsync(mxp, 0, (size_t)k * sizeof(unsigned long long), st));
     for (int i = 0; i < nW; ++i)
-        k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(W[i], rows, k, mx.as<unsigned long long>());
-    k_scales<<<grid_for(k), 256, 0, st>>>(mx.as<unsigned long long>(), k, qmax, scl.as<double>(),
-                                         scl.as<double>() + k);
+        k_colmaxabs<<<dim3(64, (unsigned)k), 256, 0, st>>>(W[i], rows, k, mxp);
+    k_scales<<<grid_for(k), 256, 0, st>>>(mxp, k, qmax, sclp, sclp + k);
     // every byte of each operand from whole 16-byte chunks (rows is a multiple of 256)
     if (rows % 16) throw RuntimeError("quantize_cols: rows must be a multiple of 16");
     const int cw = tl.packed ? 16 : 16 * tl.G;
     const unsigned gq = grid_for(rows / 16 * cw);
-    const double* inv = scl.as<double>() + k;
+    const double* inv = sclp + k;
     for (int i = 0; i < nW; ++i) {
 #define CS_QC(LL, PP)                                                                                      \
         if (tl.L == LL && tl.packed == PP)                                                                 \
@@ -707,28 +713,94 @@ static bool use_sparse_missing(cs_kinship* op, int64_t k, int L) {
     return true;
 }
 
+// Device workspace of one dense-form int8 apply with k columns (kv_work_layout gives the offsets
+// of its regions in one caller-provided block, each aligned to 256 bytes).
+struct KvWork {
+    double* Vt = nullptr;              // Npad x k scattered V
+    int8_t* BV = nullptr;              // V limbs (B operand of Z^T V)
+    unsigned long long* mx = nullptr;  // k column maxima
+    double* scl = nullptr;             // 2 k scales and inverses
+    double *W1 = nullptr, *W2 = nullptr;   // Mpad x k epilogue outputs w, mu w
+    unsigned long long* mxW = nullptr; // k column maxima of W1 / W2
+    int8_t *B1 = nullptr, *B2 = nullptr;   // W limbs (B operands of Z W)
+    long long* Yint = nullptr;         // Npad x k exact int64 result rows
+};
+
+// Byte offsets of the KvWork regions for k columns; returns the total size.
+static size_t kv_work_layout(const cs_bed* b, int64_t k, const Tiling& tl, size_t off[10]) {
+    const size_t sz[10] = {(size_t)(b->Npad * k) * 8, (size_t)(b->Npad * tl.NT * 16), (size_t)k * 8, (size_t)(2 * k) * 8,
+                           (size_t)(b->Mpad * k) * 8, (size_t)(b->Mpad * k) * 8, (size_t)k * 8,
+                           (size_t)(b->Mpad * tl.NT * 16), (size_t)(b->Mpad * tl.NT * 16), (size_t)(b->Npad * k) * 8};
+    size_t t = 0;
+    for (int i = 0; i < 10; ++i) {
+        off[i] = t;
+        t += (sz[i] + 255) / 256 * 256;
+    }
+    return t;
+}
+
+static KvWork kv_work_at(unsigned char* base, const size_t off[10]) {
+    KvWork w;
+    w.Vt = (double*)(base + off[0]);
+    w.BV = (int8_t*)(base + off[1]);
+    w.mx = (unsigned long long*)(base + off[2]);
+    w.scl = (double*)(base + off[3]);
+    w.W1 = (double*)(base + off[4]);
+    w.W2 = (double*)(base + off[5]);
+    w.mxW = (unsigned long long*)(base + off[6]);
+    w.B1 = (int8_t*)(base + off[7]);
+    w.B2 = (int8_t*)(base + off[8]);
+    w.Yint = (long long*)(base + off[9]);
+    return w;
+}
+
+// One int8 apply.  ws == NULL: the operator's own workspaces (grown on demand) and the
+// missing-genotype form of its mode; otherwise the dense form entirely in the caller's workspace,
+// with no allocation (both forms give bitwise identical results).
 static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
-                       int64_t ldy) {
+                       int64_t ldy, const KvWork* ws = nullptr) {
     cs_bed* b = op->bed;
     hipStream_t st = op->stream;
     const int L = prec_limbs(op->prec);
     const double qmax = limbs_qmax(L);
-    const bool sparse = use_sparse_missing(op, k, L);
+    const bool sparse = ws ? false : use_sparse_missing(op, k, L);
     const Tiling tl = tiling(k, L, !sparse);   // the genotype-only kernels take unpacked operands
     const int64_t K16 = 16 * (int64_t)tl.G;
     const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
     const int32_t* inv_idx = op->identity ? nullptr : op->d_inv_idx.as<int32_t>();
+    KvWork w;
+    if (ws) {
+        w = *ws;
+    } else {
+        op->Vt.ensure((size_t)(b->Npad * k) * sizeof(double));
+        op->BV.ensure((size_t)(b->Npad * tl.NT * 16));
+        op->mx.ensure((size_t)k * sizeof(unsigned long long));
+        op->scl.ensure((size_t)2 * k * sizeof(double));
+        op->W1.ensure((size_t)(b->Mpad * k) * sizeof(double));
+        op->W2.ensure((size_t)(b->Mpad * k) * sizeof(double));
+        op->mxW.ensure((size_t)k * 8);
+        op->B1.ensure((size_t)(b->Mpad * tl.NT * 16));
+        op->Yint.ensure((size_t)(b->Npad * k) * sizeof(long long));
+        if (!sparse) op->B2.ensure((size_t)(b->Mpad * tl.NT * 16));
+        w.Vt = op->Vt.as<double>();
+        w.BV = op->BV.as<int8_t>();
+        w.mx = op->mx.as<unsigned long long>();
+        w.scl = op->scl.as<double>();
+        w.W1 = op->W1.as<double>();
+        w.W2 = op->W2.as<double>();
+        w.mxW = op->mxW.as<unsigned long long>();
+        w.B1 = op->B1.as<int8_t>();
+        w.B2 = sparse ? nullptr : op->B2.as<int8_t>();
+        w.Yint = op->Yint.as<long long>();
+    }
 
     CS_HIP(hipEventRecord(op->ev[0], st));
-    op->Vt.ensure((size_t)(b->Npad * k) * sizeof(double));
-    k_scatter<<<grid_for(b->Npad * k), 256, 0, st>>>(dX, ldx, k, inv_off, inv_idx, b->N, b->Npad,
-                                                     op->Vt.as<double>());
+    k_scatter<<<grid_for(b->Npad * k), 256, 0, st>>>(dX, ldx, k, inv_off, inv_idx, b->N, b->Npad, w.Vt);
     CS_HIP(hipGetLastError());
-    op->BV.ensure((size_t)(b->Npad * tl.NT * 16));
     {
-        const double* Ws[1] = {op->Vt.as<double>()};
-        int8_t* Bs[1] = {op->BV.as<int8_t>()};
-        quantize_cols(Ws, 1, b->Npad, k, tl, st, op->mx, op->scl, Bs);
+        const double* Ws[1] = {w.Vt};
+        int8_t* Bs[1] = {w.BV};
+        quantize_cols_into(Ws, 1, b->Npad, k, tl, st, w.mx, w.scl, Bs);
     }
     if (sparse) {
         // plain quantised V, its column sums, and O^T V per SNP from the missing lists
@@ -736,8 +808,8 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
         op->S1.ensure((size_t)K16 * 8);
         op->qext.ensure((size_t)(b->Mpad * K16) * 8);
         CS_HIP(hipMemsetAsync(op->Vp.p, 0, (size_t)(b->Npad * K16) * 4, st));
-        k_quant<<<grid_for(b->Npad * k), 256, 0, st>>>(op->Vt.as<double>(), b->Npad, k, L, tl.NT, qmax,
-                                                      op->scl.as<double>() + k, nullptr, op->Vp.as<int32_t>(), K16);
+        k_quant<<<grid_for(b->Npad * k), 256, 0, st>>>(w.Vt, b->Npad, k, L, tl.NT, qmax, w.scl + k, nullptr,
+                                                      op->Vp.as<int32_t>(), K16);
         colsum_plain(op->Vp.as<int32_t>(), b->Npad, K16, op->S1.as<long long>(), st);
         const unsigned gq = (unsigned)std::min<int64_t>(4096, (b->M + 127) / 128);   // 4 waves x 32 rows
         const int64_t* qo = b->miss_snp_off.as<int64_t>();
@@ -754,55 +826,49 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     CS_HIP(hipEventRecord(op->ev[1], st));
 
     // Z^T V with the fused epilogue: W1 = w, W2 = mu w, and their per-column maxima.
-    op->W1.ensure((size_t)(b->Mpad * k) * sizeof(double));
-    op->W2.ensure((size_t)(b->Mpad * k) * sizeof(double));
-    op->mxW.ensure((size_t)k * 8);
-    CS_HIP(hipMemsetAsync(op->mxW.p, 0, (size_t)k * 8, st));
+    CS_HIP(hipMemsetAsync(w.mxW, 0, (size_t)k * 8, st));
     GenoArgs a{};
     a.A = b->snp.as<uint8_t>();
     a.KB = b->KBs;
     a.rows = b->M;
     a.kb_total = a.kb_per_split = b->KBs;
     a.G = tl.G;
-    a.Bx = a.Bo = op->BV.as<int8_t>();
+    a.Bx = a.Bo = w.BV;
     a.k = k;
-    a.scale = op->scl.as<double>();
+    a.scale = w.scl;
     a.means = b->means.as<double>();
     a.inv_stds = b->inv_stds.as<double>();
     a.mult = op->d_mult.as<int32_t>();
     a.inv_M = 1.0 / (double)op->Msel;
-    a.W1 = op->W1.as<double>();
-    a.W2 = op->W2.as<double>();
-    a.mx = op->mxW.as<unsigned long long>();
+    a.W1 = w.W1;
+    a.W2 = w.W2;
+    a.mx = w.mxW;
     a.qext = sparse ? op->qext.as<long long>() : nullptr;
     launch_geno(L, false, EPI_KV_GT, dim3((unsigned)(b->Mpad / 256), 1, (unsigned)tl.G), st, a, sparse, tl.packed);
     CS_HIP(hipEventRecord(op->ev[2], st));
 
     // common per-column scale of W1 and W2, their limbs
-    double* sclW = op->scl.as<double>();  // the V scales are no longer needed
-    k_scales<<<grid_for(k), 256, 0, st>>>(op->mxW.as<unsigned long long>(), k, qmax, sclW, sclW + k);
-    op->B1.ensure((size_t)(b->Mpad * tl.NT * 16));
-    op->Yint.ensure((size_t)(b->Npad * k) * sizeof(long long));
+    double* sclW = w.scl;  // the V scales are no longer needed
+    k_scales<<<grid_for(k), 256, 0, st>>>(w.mxW, k, qmax, sclW, sclW + k);
     if (sparse) {
-        CS_HIP(hipMemsetAsync(op->B1.p, 0, (size_t)(b->Mpad * tl.NT * 16), st));
-        k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W1.as<double>(), b->Mpad, k, L, tl.NT, qmax,
-                                                      sclW + k, op->B1.as<int8_t>(), nullptr, 0, 1.0, tl.packed);
+        CS_HIP(hipMemsetAsync(w.B1, 0, (size_t)(b->Mpad * tl.NT * 16), st));
+        k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(w.W1, b->Mpad, k, L, tl.NT, qmax, sclW + k, w.B1, nullptr, 0,
+                                                      1.0, tl.packed);
         op->B2p.ensure((size_t)(b->Mpad * K16) * 4);
         op->S2.ensure((size_t)K16 * 8);
         CS_HIP(hipMemsetAsync(op->B2p.p, 0, (size_t)(b->Mpad * K16) * 4, st));
-        k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(op->W2.as<double>(), b->Mpad, k, L, tl.NT, qmax,
-                                                      sclW + k, nullptr, op->B2p.as<int32_t>(), K16);
+        k_quant<<<grid_for(b->Mpad * k), 256, 0, st>>>(w.W2, b->Mpad, k, L, tl.NT, qmax, sclW + k, nullptr,
+                                                      op->B2p.as<int32_t>(), K16);
         colsum_plain(op->B2p.as<int32_t>(), b->Mpad, K16, op->S2.as<long long>(), st);
-        CS_HIP(hipMemsetAsync(op->Yint.p, 0, (size_t)(b->Npad * k) * sizeof(long long), st));
+        CS_HIP(hipMemsetAsync(w.Yint, 0, (size_t)(b->Npad * k) * sizeof(long long), st));
     } else {
-        op->B2.ensure((size_t)(b->Mpad * tl.NT * 16));
         // W1 and -(mu W) (negated: the Z W kernel accumulates X~ W1 and O (-mu W) into the same
         // registers), every byte of both, and the cleared result rows, in one launch
         const int cw = tl.packed ? 16 : 16 * tl.G;
         const unsigned gq = grid_for(b->Mpad / 16 * cw);
-        const double *w1 = op->W1.as<double>(), *w2 = op->W2.as<double>();
-        int8_t *b1 = op->B1.as<int8_t>(), *b2 = op->B2.as<int8_t>();
-        long long* yi = op->Yint.as<long long>();
+        const double *w1 = w.W1, *w2 = w.W2;
+        int8_t *b1 = w.B1, *b2 = w.B2;
+        long long* yi = w.Yint;
 #define CS_QZW(LL, PP)                                                                                 \
         if (L == LL && tl.packed == PP)                                                                \
             k_quant_zw<LL, PP><<<gq, 256, 0, st>>>(w1, w2, b->Mpad, k, tl.NT, cw, qmax, sclW + k, b1, b2, yi, \
@@ -852,10 +918,10 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     c.kb_total = kbt;
     c.kb_per_split = kbps;
     c.G = tl.G;
-    c.Bx = op->B1.as<int8_t>();
-    c.Bo = sparse ? op->B1.as<int8_t>() : op->B2.as<int8_t>();
+    c.Bx = w.B1;
+    c.Bo = sparse ? w.B1 : w.B2;
     c.k = k;
-    c.yint = op->Yint.as<long long>();
+    c.yint = w.Yint;
     if (sparse)
         launch_geno(L, false, EPI_KV_G, dim3((unsigned)rblocks, (unsigned)nsplit, (unsigned)tl.G), st, c, true);
     else
@@ -867,15 +933,15 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
         const int64_t* yo = b->miss_smp_off.as<int64_t>();
         const int32_t* yi = b->miss_smp_idx.as<int32_t>();
         switch (list_quads(K16)) {
-            case 4: k_sparse_yadd<4, 8, 32><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
-            case 8: k_sparse_yadd<8, 8, 32><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
-            case 16: k_sparse_yadd<16, 8, 32><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
-            case 32: k_sparse_yadd<32, 8, 64><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
-            default: k_sparse_yadd<64, 8, 64><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, op->Yint.as<long long>()); break;
+            case 4: k_sparse_yadd<4, 8, 32><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, w.Yint); break;
+            case 8: k_sparse_yadd<8, 8, 32><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, w.Yint); break;
+            case 16: k_sparse_yadd<16, 8, 32><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, w.Yint); break;
+            case 32: k_sparse_yadd<32, 8, 64><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, w.Yint); break;
+            default: k_sparse_yadd<64, 8, 64><<<gy, 256, 0, st>>>(yo, yi, op->B2p.as<int32_t>(), K16, op->S2.as<long long>(), b->N, k, w.Yint); break;
         }
         CS_HIP(hipGetLastError());
     }
-    k_finalize<<<grid_for(op->n * k), 256, 0, st>>>(op->Yint.as<long long>(), k, sclW,
+    k_finalize<<<grid_for(op->n * k), 256, 0, st>>>(w.Yint, k, sclW,
                                                    op->identity ? nullptr : op->d_keep.as<int32_t>(),
                                                    op->n, dY, ldy);
     CS_HIP(hipGetLastError());
@@ -932,6 +998,39 @@ static void apply_f32(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, 
     }
 }
 
+// Fixed workspace of cs_kinship_apply_device_noalloc for callers that cannot allocate (code under
+// graph capture, serving loops, harnesses that forbid hipMalloc).  Each device holds its own copy
+// from module load until the process ends.  In host code the array's name is not a device
+// address: kernels and copies take the per-device pointer from hipGetSymbolAddress.
+constexpr size_t kKvScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kKvScratchBytes];
+
+// The workspace's address on the current device, looked up once per device (first call there).
+static unsigned char* scratch_ptr() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static unsigned char* ptrs[kMaxDevices] = {};
+    int dev = 0;
+    CS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw RuntimeError("cs_kinship_apply_device_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        CS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw RuntimeError("cs_kinship_apply_device_noalloc: no device address for the workspace");
+        ptrs[dev] = (unsigned char*)p;
+    }
+    return ptrs[dev];
+}
+
+static void record_timings(cs_kinship* op) {
+    for (int p = 0; p < 5; ++p) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, op->ev[p], op->ev[p + 1]) != hipSuccess) ms = 0.f;
+        op->last_ms[p] = ms;
+    }
+}
+
 static void apply_device_impl(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
                               int64_t ldy) {
     if (k < 0) throw RuntimeError("PlinkKinshipOperator::apply: dimension mismatch");
@@ -940,11 +1039,7 @@ static void apply_device_impl(cs_kinship* op, const double* dX, int64_t ldx, int
     if (op->prec == CS_PREC_FP32) apply_f32(op, dX, ldx, k, dY, ldy);
     else apply_int8(op, dX, ldx, k, dY, ldy);
     CS_HIP(hipStreamSynchronize(op->stream));
-    for (int p = 0; p < 5; ++p) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, op->ev[p], op->ev[p + 1]) != hipSuccess) ms = 0.f;
-        op->last_ms[p] = ms;
-    }
+    record_timings(op);
 }
 
 }  // namespace csamd
@@ -1064,6 +1159,30 @@ int cs_kinship_apply_device(cs_kinship* op, const double* dX, int64_t ldx, int64
         if (!op) throw InvalidArgument("cs_kinship_apply_device: null handle");
         if (k > 0 && (!dX || !dY)) throw InvalidArgument("cs_kinship_apply_device: null buffer");
         apply_device_impl(op, dX, ldx, k, dY, ldy);
+    });
+}
+
+int cs_kinship_apply_device_noalloc(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
+                                    int64_t ldy) {
+    return guarded([&] {
+        if (!op) throw InvalidArgument("cs_kinship_apply_device_noalloc: null handle");
+        if (k < 0 || (k > 0 && (ldx < op->n || ldy < op->n)))
+            throw RuntimeError("PlinkKinshipOperator::apply: dimension mismatch");
+        if (k == 0) return;
+        if (!dX || !dY) throw InvalidArgument("cs_kinship_apply_device_noalloc: null buffer");
+        if (op->prec == CS_PREC_FP32)
+            throw InvalidArgument("cs_kinship_apply_device_noalloc: the FP32 comparator precision is not supported");
+        const Tiling tl = tiling(k, prec_limbs(op->prec), true);
+        size_t off[10];
+        const size_t need = kv_work_layout(op->bed, k, tl, off);
+        if (need > kKvScratchBytes)
+            throw InvalidArgument("cs_kinship_apply_device_noalloc: " + std::to_string(need) +
+                                  " bytes of workspace needed for this operator and k, the fixed workspace holds " +
+                                  std::to_string(kKvScratchBytes));
+        const KvWork w = kv_work_at(scratch_ptr(), off);
+        apply_int8(op, dX, ldx, k, dY, ldy, &w);
+        CS_HIP(hipStreamSynchronize(op->stream));
+        record_timings(op);
     });
 }
 

@@ -54,6 +54,10 @@ inline Tiling tiling(int64_t k, int L, bool allow_pack = true) {
 void quantize_cols(const double* const* W, int nW, int64_t rows, int64_t k, const Tiling& tl,
                    hipStream_t st, DevBuf& mx, DevBuf& scl, int8_t* const* B);
 
+// The same with caller-provided column maxima (k) and scales (2 k); allocates nothing.
+void quantize_cols_into(const double* const* W, int nW, int64_t rows, int64_t k, const Tiling& tl,
+                        hipStream_t st, unsigned long long* mx, double* scl, int8_t* const* B);
+
 // Launch k_geno<L, TWO_B, EPI> (cs_geno_mfma.h) on the given grid.
 struct GenoArgs;
 void launch_geno(int L, bool two_b, int epi, dim3 grid, hipStream_t st, const GenoArgs& a,

@@ -116,6 +116,20 @@ int cs_kinship_apply(cs_kinship* op, const double* X, int64_t ldx, int64_t k, do
 /* Same with X, Y in device memory (no host transfer); for device-resident solvers. */
 int cs_kinship_apply_device(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
                             int64_t ldy);
+/* No counterpart in the reference.  cs_kinship_apply_device for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): it allocates nothing and keeps every intermediate (scattered and quantised
+ * operands, the Z^T V epilogue output, the int64 result rows) in one fixed 256 MiB device
+ * workspace per GPU, held from library load until the process ends.  Runs the dense
+ * missing-genotype form, whose results are bitwise identical to the sparse one (so to
+ * cs_kinship_apply_device with any mode).  The workspace needs 16 Npad k + 16 Npad G L +
+ * 16 Mpad k + 32 Mpad G L bytes (Npad, Mpad: samples and SNPs rounded up to 256; G = ceil(k / 16)
+ * column groups, L limbs; packed tiles when k L <= 16), e.g. 190 MB at 50,000 x 500,000 with
+ * k = 16; larger requests give CS_ERR_INVALID_ARGUMENT before any device work, as does the FP32
+ * comparator precision.  Every call on a GPU shares that GPU's workspace: calls that overlap on
+ * one GPU (from several threads or operators) race and must be serialised by the caller. */
+int cs_kinship_apply_device_noalloc(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
+                                    double* dY, int64_t ldy);
 /* trace() and getDiagonal() (kinship_operator.cpp:306-368). */
 int cs_kinship_trace(cs_kinship* op, double* trace);
 int cs_kinship_diag(cs_kinship* op, double* diag);
