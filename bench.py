@@ -1,14 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X genotype hot path.
 
-Workload (one GPU): GRM build + matrix-free K.V at n = 50,000 samples x m = 500,000 SNPs, HWE
-genotypes with p ~ U(0.01, 0.5) and 0.5 % missing, generated on the device (seed 20261204).
-A step is one K.V apply with k = 16 right-hand sides over the resident packed matrix (both
-products, all epilogues).  value = genotypes/s = n*m / (time per step), inputs resident in HBM.
+Headline workload (one GPU): the LOCO matrix-free LMM genotypes, n = 200,000 samples x
+m = 600,000 SNPs, HWE genotypes with p ~ U(0.01, 0.5) and 0.5 % missing, generated on the device
+(seed 20261205).  A step is one K.V apply with k = 16 right-hand sides over all m SNPs of the
+resident packed matrix (both genotype products and all epilogues).  value = genotypes/s =
+n*m / (time per step), inputs resident in HBM.
 
-The JSON line also carries the plain FP32 GPU comparator's genotypes/s, the roofline of the
-dominant kernel (HIP events on the operator's own stream) and the CPU oracle's rate on a bounded
-slice of the same workload (cpu_baseline, kind "port").
+The same line carries the other legs: the LOCO PCG + scan over 22 chromosomes on these genotypes
+(16 phenotype replicates, relative residual 1e-6, exact GLS stage), the plain FP32 GPU
+comparator, K.V / exact and standardised GRM / projected scan at n = 50,000 x m = 500,000,
+a Poisson stream of serving requests over n = 100,000 x m = 1,000,000 resident genotypes, the
+roofline of the dominant kernel (HIP events on the operator's own stream) and the CPU rate of
+the reference's blocked K.V algorithm on a bounded slice of the headline workload
+(cpu_baseline, kind "port").
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -34,10 +39,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def profiled_traffic(key):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary (FETCH_SIZE +
-    WRITE_SIZE, calibrated against a plain copy kernel), or None if absent."""
-    path = os.path.join(ROOT, "profiles", "kv_apply_n50k_m500k_k16_hbm_traffic.md")
+def profiled_traffic(key, n, m, k):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC summary of this workload
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy kernel), or None if absent."""
+    path = os.path.join(ROOT, "profiles", f"kv_apply_n{n // 1000}k_m{m // 1000}k_k{k}_hbm_traffic.md")
     try:
         for line in open(path):
             if line.startswith(f"traffic_bytes_per_launch {key} "):
@@ -47,19 +52,117 @@ def profiled_traffic(key):
     return None
 
 
-def cpu_baseline(N, M_slice, k, seed, miss=0.005):
-    """Oracle K.V on a slice of the same workload: all N samples, the first M_slice SNPs."""
+def cpu_baseline(N, M_slice, k, seed, pmin=0.01, miss=0.005):
+    """The reference's K.V algorithm (kinship_operator.cpp:225-299: 512-SNP blocks decoded to FP64,
+    z^T X, then z (z^T X); the oracle's cache-blocked, threaded restatement) on a slice of the
+    headline workload: all N samples, the first M_slice SNPs of the same generator."""
     import oracle
     threads = min(os.cpu_count() or 1, 16)
-    body = oracle.gen_hwe_bed(seed, N, M_slice, 0.01, 0.5, miss, nthreads=threads)
+    body = oracle.gen_hwe_bed(seed, N, M_slice, pmin, 0.5, miss, nthreads=threads)
     means, inv = oracle.stats(body, N, M_slice, nthreads=threads)
     X = np.random.default_rng(1).standard_normal((N, k))
     t0 = time.perf_counter()
-    oracle.kinship_apply(body, N, means, inv, None, None, X, nthreads=threads)
+    oracle.kinship_apply_blocked(body, N, means, inv, None, None, X, nthreads=threads)
     dt = time.perf_counter() - t0
     return dict(value=N * M_slice / dt, unit="genotypes/s", cores=threads, kind="port",
-                sample=f"FP64 CPU oracle K.V apply, k={k}, all {N} samples x first {M_slice} SNPs "
-                       f"of the same generator ({dt:.1f} s)")
+                sample=f"FP64 CPU K.V apply in the reference's 512-SNP block form, k={k}, all {N} samples x "
+                       f"first {M_slice} SNPs of the same generator ({dt:.1f} s)")
+
+
+def kv_leg(csa, reader, N, M, k, steps, warmup, prec="int8_l3", missing_mode=0, extras=True):
+    """K.V applies with k device-resident right-hand sides over all SNPs: time per step, phases
+    (HIP events on the operator's stream), roofline of the dominant genotype product, and -- with
+    extras -- the host-buffer (PCIe) rate and the few-column applies."""
+    op = csa.PlinkKinshipOperator(reader)
+    op.set_precision(csa.CS_PREC_INT8_L3 if prec == "int8_l3" else csa.CS_PREC_INT8_L2)
+    op.set_missing_mode(missing_mode)
+    X = csa.DeviceArray.from_host(np.random.default_rng(7).standard_normal((N, k)))
+    Y = csa.DeviceArray(N, k)
+    for _ in range(warmup):
+        op.apply_device(X.ptr, N, k, Y.ptr, N)
+    csa.synchronize()
+    phase = np.zeros(5)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        op.apply_device(X.ptr, N, k, Y.ptr, N)
+        phase += op.last_timing_ms()
+    csa.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = dt * 1e3 / steps
+    phase /= steps
+    value = N * M / (dt / steps)
+    log(f"K.V n={N} m={M} k={k} {prec}: {ms_step:.3f} ms/step, phases ms {np.round(phase, 4).tolist()}")
+    # dominant kernel: the larger of the two genotype products (Z^T V on the SNP-major copy,
+    # Z W on the sample-major copy); algorithmic bytes = the packed matrix read once.
+    gt_bytes = M * ((N + 3) // 4)
+    g_bytes = N * ((M + 3) // 4)
+    if phase[1] >= phase[3]:
+        name, kms, kbytes, key = "k_geno1s (Z^T V, SNP-major)", phase[1], gt_bytes, "k_geno1s"
+    else:
+        name, kms, kbytes, key = "k_geno2s (Z W, sample-major)", phase[3], g_bytes, "k_geno2s"
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    # int8 MFMA work the exact formulation issues in that kernel: 2 products (x~, o) x L limbs of
+    # 16 x 16 x 64 MACs per 16 rows x 64 k per group, i.e. 2 * 2 L * 16 * rows * k_len ops
+    limbs = 3 if prec == "int8_l3" else 2
+    groups = (k + 15) // 16
+    rows_pad = ((M if key == "k_geno1s" else N) + 255) // 256 * 256
+    k_pad = ((N if key == "k_geno1s" else M) + 255) // 256 * 256
+    ops_issued = 2.0 * 2 * limbs * 16 * groups * float(rows_pad) * float(k_pad)
+    mfma_rate = ops_issued / (kms * 1e-3) / 1e12
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=profiled_traffic(key, N, M, k), kernel=name,
+                    kernel_ms=round(kms, 4), bytes_per_launch=kbytes,
+                    apply_frac=round(gt_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    apply_frac_two_reads=round(2 * gt_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    int8_mfma_tops_issued=round(mfma_rate, 1), int8_mfma_frac=round(mfma_rate / INT8_PEAK_TOPS, 4))
+    out = dict(ms_per_step=ms_step, genotypes_per_s=value, phases_ms=np.round(phase, 4).tolist(), roofline=roofline)
+    if extras:
+        # the same apply through the host-buffer entry point (PCIe transfers of X and Y included)
+        Xh = np.asfortranarray(np.random.default_rng(7).standard_normal((N, k)))
+        op.apply(Xh)
+        t1 = time.perf_counter()
+        for _ in range(3):
+            op.apply(Xh)
+        out["host_buffer_genotypes_per_s"] = N * M / ((time.perf_counter() - t1) / 3)
+        # few right-hand sides (limbs packed into one MFMA tile when k L <= 16), device buffers
+        small_k = {}
+        for ks in (1, 4):
+            if ks >= k:
+                continue
+            for _ in range(2):
+                op.apply_device(X.ptr, N, ks, Y.ptr, N)
+            csa.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(10):
+                op.apply_device(X.ptr, N, ks, Y.ptr, N)
+            csa.synchronize()
+            small_k[f"k{ks}_ms"] = (time.perf_counter() - t1) / 10 * 1e3
+        out["small_k_apply_ms"] = small_k or None
+    X.close()
+    Y.close()
+    op.close()
+    return out
+
+
+def fp32_leg(csa, reader, N, M, k, reps=2):
+    """The plain FP32 GPU comparator: the same K.V with genotypes decoded to FP32 and FP32 FMAs (no
+    MFMA, no split)."""
+    op = csa.PlinkKinshipOperator(reader)
+    op.set_precision(csa.CS_PREC_FP32)
+    X = csa.DeviceArray.from_host(np.random.default_rng(7).standard_normal((N, k)))
+    Y = csa.DeviceArray(N, k)
+    op.apply_device(X.ptr, N, k, Y.ptr, N)
+    csa.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        op.apply_device(X.ptr, N, k, Y.ptr, N)
+    csa.synchronize()
+    dt = (time.perf_counter() - t1) / reps
+    X.close()
+    Y.close()
+    op.close()
+    log(f"K.V plain FP32: {dt * 1e3:.2f} ms/step")
+    return dict(ms_per_step=dt * 1e3, genotypes_per_s=N * M / dt)
 
 
 def grm_leg(csa, reader, N, M):
@@ -109,8 +212,10 @@ def grm_std_leg(csa, reader, N, M):
 
 def scan_leg(csa, reader, N, M, T=16):
     """PCG-projected LMM scan of T phenotype replicates over the whole workload (one multi-RHS
-    PCG solve of T + 4 covariate + 16 calibration columns, then one fused scan pass), and the
-    fused scan pass alone.  genotypes/s = n m / time of one pass over all SNPs (all T traits)."""
+    PCG solve of T + 4 covariate + 16 calibration columns, one fused scan pass, the exact GLS stage
+    at the library's default level), and the fused scan pass alone with R and the statistics in HBM
+    (cs_lmm_scan_device).  genotypes/s = n m / time of one pass over all SNPs (all T traits); the
+    roofline of the genotype product of the pass (packed SNP-major bytes / its event time)."""
     rng = np.random.default_rng(17)
     X = np.column_stack([np.ones(N), rng.standard_normal((N, 3))]).astype(np.float32)
     Y = rng.standard_normal((N, T))
@@ -121,17 +226,31 @@ def scan_leg(csa, reader, N, M, T=16):
     out = csa.pcg_projected_scan(op, reader, None, Y, X, 0.4, 0.6, cal, rel_tol=1e-6, max_iter=200)
     csa.synchronize()
     total = time.perf_counter() - t0
-    # the fused scan pass alone: R = [P y_1..T, X]
-    R = np.asfortranarray(rng.standard_normal((N, T + 4)))
+    op.close()
+    # the fused scan pass alone: R = [P y_1..T, X] on the device, statistics left in HBM
+    dR = csa.DeviceArray.from_host(rng.standard_normal((N, T + 4)))
     cov_C = np.eye(4) * 1e-6
-    csa.lmm_scan(reader, None, R, n_score=T, alpha=1.0, cov=(T, T + 4), cov_C=cov_C)
+    kw = dict(n_score=T, alpha=1.0, cov=(T, T + 4), cov_C=cov_C)
+    res, _ = csa.lmm_scan_device(reader, None, dR, **kw)
+    csa.synchronize()
     t1 = time.perf_counter()
-    csa.lmm_scan(reader, None, R, n_score=T, alpha=1.0, cov=(T, T + 4), cov_C=cov_C)
+    res, (ms_counts, ms_prod, ms_epi, packed) = csa.lmm_scan_device(reader, None, dR, out=res, **kw)
+    csa.synchronize()
     scan_s = time.perf_counter() - t1
+    for a in [dR] + list(res.values()):
+        a.close()
+    gbytes = M * ((N + 3) // 4)
+    achieved = gbytes / (ms_prod * 1e-3) / 1e9
     return dict(traits=T, covariates=4, pcg_iterations=out["pcg_iterations"], gamma=out["gamma"],
-                max_rel_residual=out["max_rel_residual"], total_ms=total * 1e3, scan_pass_ms=scan_s * 1e3,
+                max_rel_residual=out["max_rel_residual"], n_exact=out["n_exact"],
+                calibration_rel_sd=out["calibration_rel_sd"], dlog10p_bound_below=out["dlog10p_bound_below"],
+                total_ms=total * 1e3, scan_pass_ms=scan_s * 1e3,
+                scan_device_ms=dict(class_counts=ms_counts, products=ms_prod, epilogue=ms_epi),
                 scan_genotypes_per_s=N * M / scan_s, scan_trait_genotypes_per_s=N * M * T / scan_s,
-                end_to_end_trait_genotypes_per_s=N * M * T / total)
+                end_to_end_trait_genotypes_per_s=N * M * T / total,
+                roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                              frac=round(achieved / HBM_PEAK_GBS, 4), kernel="k_geno<EPI_PQ> (g^T R)",
+                              kernel_ms=round(ms_prod, 4), bytes_per_launch=gbytes))
 
 
 # GRCh38 autosome lengths (Mb), the chromosome proportions of the LOCO workload
@@ -163,20 +282,20 @@ def c3_setup(csa, n, m, T, seed):
     means, inv_stds = reader.getMeans(), reader.getInvStds()
     f = means / 2.0
     grm_ok = (inv_stds != 0) & (np.minimum(f, 1 - f) >= 0.01)
-    return rng, reader, bounds, X, Y, grm_ok
+    return dict(rng=rng, reader=reader, bounds=bounds, X=X, Y=Y, grm_ok=grm_ok, n=n, m=m)
 
 
-def loco_leg(csa, n=200_000, m=600_000, T=16, seed=20261205):
-    """LOCO matrix-free PCG + LMM scan (the C3 workload, c3_setup): per chromosome one kinship
-    operator without that chromosome, one multi-RHS PCG solve (T traits + 3 covariates + 16
-    calibration SNPs, relative residual 1e-6) and the fused scan of that chromosome's SNPs.
-    genotypes/s = n m / total time."""
-    t0 = time.perf_counter()
-    rng, reader, bounds, X, Y, grm_ok = c3_setup(csa, n, m, T, seed)
-    setup = time.perf_counter() - t0
+def loco_leg(csa, ctx, T=16, exact_log10p=0.0):
+    """LOCO matrix-free PCG + LMM scan (c3_setup's genotypes and phenotypes): per chromosome one
+    kinship operator without that chromosome, one multi-RHS PCG solve (T traits + 3 covariates +
+    16 calibration SNPs, relative residual 1e-6), the fused scan of that chromosome's SNPs, and the
+    exact GLS stage for every SNP whose smallest P over the T traits, times T, reaches
+    10^-exact_log10p (0: the library default 5).  genotypes/s = n m / total time."""
+    rng, reader, bounds, X, Y, grm_ok = ctx["rng"], ctx["reader"], ctx["bounds"], ctx["X"], ctx["Y"], ctx["grm_ok"]
+    n, m = ctx["n"], ctx["m"]
     csa.synchronize()
     t0 = time.perf_counter()
-    iters, resid, gammas, tested = [], [], [], 0
+    iters, resid, gammas, rsd, bound, tested, n_exact, ex_it = [], [], [], [], [], 0, 0, []
     for c in range(22):
         lo, hi = bounds[c], bounds[c + 1]
         idx = np.arange(m)
@@ -184,35 +303,41 @@ def loco_leg(csa, n=200_000, m=600_000, T=16, seed=20261205):
         snps = np.arange(lo, hi, dtype=np.int32)
         cal = np.sort(rng.choice(snps, 16, replace=False)).astype(np.int32)
         op = csa.PlinkKinshipOperator(reader, None, loco)
-        out = csa.pcg_projected_scan(op, reader, None, Y, X, 0.4, 0.6, cal, snps=snps, rel_tol=1e-6,
-                                     max_iter=200)
+        out = csa.pcg_projected_scan(op, reader, None, Y[:, :T], X, 0.4, 0.6, cal, snps=snps, rel_tol=1e-6,
+                                     max_iter=200, exact_log10p=exact_log10p)
         op.close()
         iters.append(out["pcg_iterations"])
         resid.append(out["max_rel_residual"])
         gammas.append(out["gamma"])
+        rsd.append(out["calibration_rel_sd"])
+        bound.append(out["dlog10p_bound_below"])
         tested += int(np.sum(out["passes"]))
+        n_exact += int(out["n_exact"])
+        ex_it.append(out["exact_pcg_iterations"])
     csa.synchronize()
     total = time.perf_counter() - t0
-    reader.close()
-    res = dict(workload=f"n={n} x m={m}, 22 LOCO chromosomes, {T} traits, PCG rel. residual 1e-6",
-               setup_s=setup, total_s=total, genotypes_per_s=n * m / total,
+    res = dict(workload=f"n={n} x m={m}, 22 LOCO chromosomes, {T} traits, PCG rel. residual 1e-6, "
+                        f"exact GLS at level {exact_log10p or 5:g} (+ log10 {T})",
+               total_s=total, genotypes_per_s=n * m / total,
                trait_genotypes_per_s=n * m * T / total, pcg_iterations=iters,
                max_rel_residual=float(np.max(resid)), gamma_range=[float(np.min(gammas)), float(np.max(gammas))],
-               n_tested=tested)
+               n_tested=tested, exact_log10p=exact_log10p, n_exact=n_exact, exact_pcg_iterations=ex_it,
+               calibration_rel_sd_range=[float(np.nanmin(rsd)), float(np.nanmax(rsd))],
+               dlog10p_bound_below_max=float(np.nanmax(bound)))
     log(f"LOCO leg: {total:.1f} s for 22 chromosomes, {res['genotypes_per_s']:.3e} genotypes/s, "
-        f"PCG iterations {min(iters)}-{max(iters)}")
+        f"PCG iterations {min(iters)}-{max(iters)}, {n_exact} SNPs exact, calibration spread "
+        f"{res['calibration_rel_sd_range']}")
     return res
 
 
-def lowrank_leg(csa, n=200_000, m=600_000, seed=20261205, rank=256, oversample=10):
+def lowrank_leg(csa, ctx, rank=256, oversample=10):
     """The reference's live matrix-free GWAS path (--fastlmm-matrix-free, low-rank mode) on the C3
     genotypes (c3_setup, first phenotype replicate): per LOCO chromosome a kinship operator without
     that chromosome, the randomized low-rank null model (rank 256 + 10 oversampling, two power
     iterations, REML) and scanLowRank of that chromosome's SNPs.  genotypes/s = n m / total."""
-    t0 = time.perf_counter()
-    _, reader, bounds, X, Y, grm_ok = c3_setup(csa, n, m, 16, seed)
+    reader, bounds, X, Y, grm_ok = ctx["reader"], ctx["bounds"], ctx["X"], ctx["Y"], ctx["grm_ok"]
+    n, m = ctx["n"], ctx["m"]
     y = np.ascontiguousarray(Y[:, 0])
-    setup = time.perf_counter() - t0
     csa.synchronize()
     t0 = time.perf_counter()
     t_null, ranks, h2, tested = 0.0, [], [], 0
@@ -233,9 +358,8 @@ def lowrank_leg(csa, n=200_000, m=600_000, seed=20261205, rank=256, oversample=1
         op.close()
     csa.synchronize()
     total = time.perf_counter() - t0
-    reader.close()
     res = dict(workload=f"n={n} x m={m}, 22 LOCO chromosomes, 1 trait, low-rank rank {rank} + {oversample}",
-               setup_s=setup, total_s=total, null_model_s=t_null, genotypes_per_s=n * m / total,
+               total_s=total, null_model_s=t_null, genotypes_per_s=n * m / total,
                effective_rank=[min(ranks), max(ranks)], h2_range=[float(np.min(h2)), float(np.max(h2))],
                n_tested=tested)
     log(f"low-rank leg: {total:.1f} s for 22 chromosomes (null models {t_null:.1f} s), "
@@ -327,16 +451,18 @@ def mv_leg(csa, n=32_000, m=300_000, d=4, seed=20261207, n_check=32):
     return out
 
 
-def serving_leg(csa, n=100_000, m=1_000_000, n_requests=12, rate=5.0, seed=20261206):
-    """Poisson stream of scan requests (rate req/s) over a resident n x m matrix; each request: a
-    random cohort (log-uniform 10k..n), 1-64 traits, 0-10 covariates (+ intercept), h2 ~
-    U(0.1, 0.8), 16 calibration SNPs, a scan of all m SNPs.  Latency = arrival to the client
-    holding the full results (freq, n_obs, pass, beta, SE, P, log10 P of every SNP and trait) on
-    the host; the time to completion on the device is reported beside it."""
+def serving_leg(csa, n=100_000, m=1_000_000, n_requests=60, rate=5.0, seed=20261206, budget_gib=64.0):
+    """Poisson stream of scan requests (rate req/s) over a resident n x m matrix with an HBM budget
+    for the requests' buffers; each request: a random cohort (log-uniform 10k..n), 1-64 traits,
+    0-10 covariates (+ intercept), h2 ~ U(0.1, 0.8), 16 calibration SNPs, a scan of all m SNPs with
+    the exact GLS stage at its default threshold.  Latency = arrival to the client holding the full
+    results (freq, n_obs, pass, beta, SE, P, log10 P of every SNP and trait) on the host; p50 / p99
+    over all requests and over the second half of the arrivals (steady state), the service rate
+    (completions per second while requests were arriving or queued) beside the arrival rate."""
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     reader = csa.PlinkReader.synthetic(seed, n, m, 0.05, 0.5, 0.005)
-    srv = csa.ScanServer(reader, max_columns=512)
+    srv = csa.ScanServer(reader, max_columns=512, hbm_budget_bytes=int(budget_gib * 2**30))
     setup = time.perf_counter() - t0
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
     reqs = []
@@ -350,7 +476,7 @@ def serving_leg(csa, n=100_000, m=1_000_000, n_requests=12, rate=5.0, seed=20261
         h2 = float(rng.uniform(0.1, 0.8))
         cal = np.sort(rng.choice(m, 16, replace=False)).astype(np.int32)
         reqs.append(dict(Y=Y, X=X, sigma2_g=h2, sigma2_e=1.0 - h2, cal_snps=cal, keep=keep))
-    ids, submitted, done, done_dev = {}, 0, {}, {}
+    ids, submitted, done, done_dev, n_exact = {}, 0, {}, {}, 0
     genos = 0.0
     start = time.perf_counter()
     while len(done) < n_requests:
@@ -369,20 +495,27 @@ def serving_leg(csa, n=100_000, m=1_000_000, n_requests=12, rate=5.0, seed=20261
                 res = srv.result(rid)
                 done[rid] = time.perf_counter() - start - arrivals[i]
                 assert res["log10p"].shape == (m, reqs[i]["Y"].shape[1])
+                n_exact += int(res["stats"]["n_exact"])
                 del res
                 genos += reqs[i]["keep"].size * m * reqs[i]["Y"].shape[1]
                 srv.release(rid)
             elif st == -1:
                 raise RuntimeError(f"request {i} failed")
     wall = time.perf_counter() - start
-    lat = np.array(sorted(done.values()))
-    lat_dev = np.array(sorted(done_dev.values()))
+    order = sorted(ids.items(), key=lambda kv: kv[1])
+    lat = np.array([done[rid] for rid, _ in order])
+    lat_dev = np.array([done_dev[rid] for rid, _ in order])
+    half = lat[n_requests // 2:]
     srv.close()
     reader.close()
-    return dict(workload=f"n={n} x m={m} resident, {n_requests} requests, Poisson {rate} req/s",
+    return dict(workload=f"n={n} x m={m} resident, {n_requests} requests, Poisson {rate} req/s, "
+                         f"HBM budget {budget_gib:g} GiB",
                 p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                p50_latency_second_half_s=float(np.percentile(half, 50)),
+                p99_latency_second_half_s=float(np.percentile(half, 99)),
                 p50_device_done_s=float(np.percentile(lat_dev, 50)),
-                wall_s=wall, trait_genotypes_per_s=genos / wall, setup_s=setup)
+                arrival_rate_per_s=float(n_requests / arrivals[-1]), service_rate_per_s=float(n_requests / wall),
+                wall_s=wall, trait_genotypes_per_s=genos / wall, n_exact=n_exact, setup_s=setup)
 
 
 def main():
@@ -390,26 +523,30 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=50_000)
-    ap.add_argument("--m", type=int, default=500_000)
+    ap.add_argument("--n", type=int, default=200_000)
+    ap.add_argument("--m", type=int, default=600_000)
     ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--prec", choices=["int8_l3", "int8_l2"], default="int8_l3")
-    ap.add_argument("--miss", type=float, default=0.005,
-                    help="missing-genotype rate of the synthetic K.V genotypes (default 0.5 %%)")
     ap.add_argument("--missing-mode", type=int, choices=[0, 1, 2], default=0,
                     help="K.V missing-genotype form: 0 auto, 1 dense O products, 2 sparse missing lists")
-    ap.add_argument("--cpu-slice-snps", type=int, default=250_000)
+    ap.add_argument("--kv-only", action="store_true", help="only the headline K.V applies (for profiling)")
+    ap.add_argument("--cpu-slice-snps", type=int, default=160_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-loco", action="store_true")
+    ap.add_argument("--loco-exact-log10p", type=float, default=0.0,
+                    help="exact GLS stage level of the LOCO leg (-log10 of the trait-adjusted P; 0 = the "
+                         "library default 5)")
+    ap.add_argument("--no-c2", action="store_true", help="skip the n=50k x m=500k legs (K.V, GRMs, scan)")
     ap.add_argument("--no-grm", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--mv", action="store_true", help="also run the multi-trait LMM leg (C5 workload)")
-    ap.add_argument("--loco", action="store_true", help="also run the LOCO PCG + scan leg (C3 workload)")
+    ap.add_argument("--serving-requests", type=int, default=60)
+    ap.add_argument("--mv", action="store_true", help="also run the multi-trait LMM leg")
     ap.add_argument("--lowrank", action="store_true",
-                    help="also run the low-rank matrix-free GWAS leg (the reference path, C3 genotypes)")
+                    help="also run the low-rank matrix-free GWAS leg (the reference path, headline genotypes)")
     # the dense eigen path (MvEigenVce::prepare) is bounded by rocSOLVER's 32-bit interfaces at
-    # n <= 32766 individuals; the C5 workload names 50,000
+    # n <= 32766 individuals; the multi-trait workload names 50,000
     ap.add_argument("--mv-n", type=int, default=32_000)
     ap.add_argument("--mv-m", type=int, default=300_000)
     ap.add_argument("--copy-probe", action="store_true",
@@ -434,134 +571,61 @@ def main():
     import cosmicscanner_amd as csa
 
     N, M, k = args.n, args.m, args.k
-    seed = 20261204
+    seed = 20261205
     t0 = time.perf_counter()
-    reader = csa.PlinkReader.synthetic(seed, N, M, 0.01, 0.5, args.miss)
-    op = csa.PlinkKinshipOperator(reader)
-    log(f"resident genotypes {reader.device_bytes() / 2**30:.2f} GiB, setup {time.perf_counter() - t0:.1f} s")
-    op.set_precision(csa.CS_PREC_INT8_L3 if args.prec == "int8_l3" else csa.CS_PREC_INT8_L2)
-    op.set_missing_mode(args.missing_mode)
-
-    X = csa.DeviceArray.from_host(np.random.default_rng(7).standard_normal((N, k)))
-    Y = csa.DeviceArray(N, k)
-    for _ in range(args.warmup):
-        op.apply_device(X.ptr, N, k, Y.ptr, N)
-    csa.synchronize()
-    phase = np.zeros(5)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        op.apply_device(X.ptr, N, k, Y.ptr, N)
-        phase += op.last_timing_ms()
-    csa.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = dt * 1e3 / args.steps
-    phase /= args.steps
-    value = N * M / (dt / args.steps)
-    log(f"K.V {args.prec}: {ms_step:.3f} ms/step, phases ms {np.round(phase, 4).tolist()}")
-
-    # dominant kernel: the larger of the two genotype products (Z^T V on the SNP-major copy,
-    # Z W on the sample-major copy); algorithmic bytes = the packed matrix read once.
-    gt_bytes = M * ((N + 3) // 4)
-    g_bytes = N * ((M + 3) // 4)
-    if phase[1] >= phase[3]:
-        name, kms, kbytes, key = "k_geno1s (Z^T V, SNP-major)", phase[1], gt_bytes, "k_geno1s"
-    else:
-        name, kms, kbytes, key = "k_geno2s (Z W, sample-major)", phase[3], g_bytes, "k_geno2s"
-    achieved = kbytes / (kms * 1e-3) / 1e9
-    traffic = profiled_traffic(key) if (N, M, k) == (50_000, 500_000, 16) else None
-    # int8 MFMA work the exact formulation issues in that kernel: 2 products (x~, o) x L limbs of
-    # 16 x 16 x 64 MACs per 16 rows x 64 k per group, i.e. 2 * 2 L * 16 * rows * k_len ops
-    limbs = 3 if args.prec == "int8_l3" else 2
-    groups = (k + 15) // 16
-    rows_pad = ((M if key == "k_geno1s" else N) + 255) // 256 * 256
-    k_pad = ((N if key == "k_geno1s" else M) + 255) // 256 * 256
-    ops_issued = 2.0 * 2 * limbs * 16 * groups * float(rows_pad) * float(k_pad)
-    mfma_rate = ops_issued / (kms * 1e-3) / 1e12
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel=name,
-                    kernel_ms=round(kms, 4), apply_frac=round(gt_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    int8_mfma_tops_issued=round(mfma_rate, 1),
-                    int8_mfma_frac=round(mfma_rate / INT8_PEAK_TOPS, 4))
-
-    # the same apply through the host-buffer entry point (PCIe transfers of X and Y included)
-    Xh = np.asfortranarray(np.random.default_rng(7).standard_normal((N, k)))
-    op.apply(Xh)
-    t1 = time.perf_counter()
-    for _ in range(3):
-        op.apply(Xh)
-    host_rate = N * M / ((time.perf_counter() - t1) / 3)
-    log(f"K.V host buffers: {host_rate:.3e} genotypes/s")
-
-    # few right-hand sides (limbs packed into one MFMA tile when k L <= 16), device buffers
-    small_k = {}
-    for ks in (1, 4):
-        if ks >= k:
-            continue
-        for _ in range(2):
-            op.apply_device(X.ptr, N, ks, Y.ptr, N)
-        csa.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(10):
-            op.apply_device(X.ptr, N, ks, Y.ptr, N)
-        csa.synchronize()
-        small_k[f"k{ks}_ms"] = (time.perf_counter() - t1) / 10 * 1e3
-    if small_k:
-        log("K.V few columns: " + ", ".join(f"{key} {v:.3f}" for key, v in small_k.items()))
+    ctx = c3_setup(csa, N, M, 16, seed)
+    log(f"resident genotypes {ctx['reader'].device_bytes() / 2**30:.2f} GiB, setup {time.perf_counter() - t0:.1f} s")
+    kv = kv_leg(csa, ctx["reader"], N, M, k, args.steps, args.warmup, args.prec, args.missing_mode,
+                extras=not args.kv_only)
+    value, ms_step = kv["genotypes_per_s"], kv["ms_per_step"]
 
     fp32 = None
-    if not args.no_fp32:
-        op.set_precision(csa.CS_PREC_FP32)
-        op.apply_device(X.ptr, N, k, Y.ptr, N)
-        t1 = time.perf_counter()
-        reps = 2
-        for _ in range(reps):
-            op.apply_device(X.ptr, N, k, Y.ptr, N)
-        csa.synchronize()
-        fp32_dt = (time.perf_counter() - t1) / reps
-        fp32 = N * M / fp32_dt
-        log(f"K.V plain FP32: {fp32_dt * 1e3:.2f} ms/step")
-
-    grm = None
-    if not args.no_grm:
-        grm = grm_leg(csa, reader, N, M)
-        log(f"GRM: {grm['ms']:.1f} ms, {grm['tops_full_square']:.1f} TOPS (2 n^2 m), "
-            f"int8 products {grm['ms_products']:.1f} ms in {grm['launches']} launches")
-        grm["standardised"] = grm_std_leg(csa, reader, N, M)
-        gs = grm["standardised"]
-        log(f"standardised GRM: {gs['ms']:.1f} ms, {gs['tops_full_square']:.1f} TOPS (2 n^2 m), "
-            f"fp16 products {gs['ms_products']:.1f} ms = {gs['roofline']['achieved']} TFLOP/s issued")
-
-    scan = None
-    if not args.no_scan:
-        scan = scan_leg(csa, reader, N, M)
-        log(f"projected LMM scan: {scan['total_ms']:.1f} ms end to end ({scan['pcg_iterations']} PCG iterations), "
-            f"scan pass {scan['scan_pass_ms']:.1f} ms = {scan['scan_genotypes_per_s']:.3e} genotypes/s")
-
-    serving = None
-    if not args.no_serving:
-        op.close()
-        serving = serving_leg(csa)
-        log(f"serving: p50 {serving['p50_latency_s']:.2f} s, p99 {serving['p99_latency_s']:.2f} s, "
-            f"{serving['trait_genotypes_per_s']:.3e} trait-genotypes/s")
+    if not args.no_fp32 and not args.kv_only:
+        fp32 = fp32_leg(csa, ctx["reader"], N, M, k)
 
     locores = None
-    if args.loco:
-        op.close()
-        locores = loco_leg(csa)
+    if not args.no_loco and not args.kv_only:
+        locores = loco_leg(csa, ctx, exact_log10p=args.loco_exact_log10p)
 
     lowres = None
-    if args.lowrank:
-        op.close()
-        lowres = lowrank_leg(csa)
+    if args.lowrank and not args.kv_only:
+        lowres = lowrank_leg(csa, ctx)
+    ctx["reader"].close()
+    del ctx
+
+    kv_c2 = grm = scan = None
+    if not args.no_c2 and not args.kv_only:
+        N2, M2 = 50_000, 500_000
+        reader2 = csa.PlinkReader.synthetic(20261204, N2, M2, 0.01, 0.5, 0.005)
+        kv_c2 = kv_leg(csa, reader2, N2, M2, 16, args.steps, args.warmup, args.prec, args.missing_mode, extras=True)
+        if not args.no_grm:
+            grm = grm_leg(csa, reader2, N2, M2)
+            log(f"GRM: {grm['ms']:.1f} ms, {grm['tops_full_square']:.1f} TOPS (2 n^2 m), "
+                f"int8 products {grm['ms_products']:.1f} ms in {grm['launches']} launches")
+            grm["standardised"] = grm_std_leg(csa, reader2, N2, M2)
+            gs = grm["standardised"]
+            log(f"standardised GRM: {gs['ms']:.1f} ms, {gs['tops_full_square']:.1f} TOPS (2 n^2 m), "
+                f"fp16 products {gs['ms_products']:.1f} ms = {gs['roofline']['achieved']} TFLOP/s issued")
+        if not args.no_scan:
+            scan = scan_leg(csa, reader2, N2, M2)
+            log(f"projected LMM scan: {scan['total_ms']:.1f} ms end to end ({scan['pcg_iterations']} PCG "
+                f"iterations), scan pass {scan['scan_pass_ms']:.1f} ms = {scan['scan_genotypes_per_s']:.3e} genotypes/s")
+        reader2.close()
+
+    serving = None
+    if not args.no_serving and not args.kv_only:
+        serving = serving_leg(csa, n_requests=args.serving_requests)
+        log(f"serving: p50 {serving['p50_latency_s']:.2f} s, p99 {serving['p99_latency_s']:.2f} s "
+            f"(second half {serving['p50_latency_second_half_s']:.2f} / {serving['p99_latency_second_half_s']:.2f} s), "
+            f"service {serving['service_rate_per_s']:.2f} req/s, {serving['trait_genotypes_per_s']:.3e} trait-genotypes/s")
 
     mvres = None
-    if args.mv:
-        op.close()
+    if args.mv and not args.kv_only:
         mvres = mv_leg(csa, n=args.mv_n, m=args.mv_m)
 
     cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(N, args.cpu_slice_snps, k, seed, args.miss)
+    if not args.no_cpu and not args.kv_only:
+        cpu = cpu_baseline(N, args.cpu_slice_snps, k, seed)
         log(f"cpu baseline: {cpu['value']:.3e} genotypes/s on {cpu['cores']} threads")
 
     out = {
@@ -577,20 +641,22 @@ def main():
         "vs_baseline": None,
         "dtype": "int8 MFMA (exact genotypes, 3x8-bit fixed-point RHS limbs), int32 acc, fp64 epilogue"
         if args.prec == "int8_l3" else "int8 MFMA (2x8-bit limbs), int32 acc, fp64 epilogue",
-        "data": f"synthetic HWE genotypes generated on device (p~U(0.01,0.5), {100 * args.miss:g}% missing), random RHS",
-        "config": {"workload": "GRM build + matrix-free K.v, n=50,000 x m=500,000 (K.V apply, k=16)",
+        "data": "synthetic HWE genotypes generated on device (p~U(0.01,0.5), 0.5% missing), random RHS",
+        "config": {"workload": f"LOCO matrix-free LMM genotypes n={N:,} x m={M:,}: K.V apply, k={k} (all SNPs)",
                    "n": N, "m": M, "k": k, "precision": args.prec},
-        "host_buffer_genotypes_per_s": host_rate,
-        "small_k_apply_ms": small_k or None,
-        "fp32_gpu_genotypes_per_s": fp32,
-        "speedup_vs_fp32_gpu": (value / fp32) if fp32 else None,
-        "roofline": roofline,
+        "phases_ms": kv["phases_ms"],
+        "host_buffer_genotypes_per_s": kv.get("host_buffer_genotypes_per_s"),
+        "small_k_apply_ms": kv.get("small_k_apply_ms"),
+        "fp32_gpu_genotypes_per_s": fp32["genotypes_per_s"] if fp32 else None,
+        "speedup_vs_fp32_gpu": (value / fp32["genotypes_per_s"]) if fp32 else None,
+        "roofline": kv["roofline"],
+        "loco_scan": locores,
+        "lowrank_scan": lowres,
+        "kv_n50k_m500k": kv_c2,
         "grm": grm,
         "lmm_scan": scan,
         "serving": serving,
         "multivariate": mvres,
-        "loco_scan": locores,
-        "lowrank_scan": lowres,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

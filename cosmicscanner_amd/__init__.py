@@ -90,6 +90,10 @@ def _load():
         "cs_lmm_scan": ([_vp, _i32p, _i64, _dp, _i64, _i64, ctypes.c_double, _i32p, _i64,
                          ctypes.c_int, ctypes.c_void_p, _dp, _i64p, _i32p, _dp, _dp, _dp, _dp],
                         ctypes.c_int),
+        "cs_lmm_scan_device": ([_vp, _i32p, _i64, _vp, _i64, _i64, ctypes.c_double, _i32p, _i64,
+                                ctypes.c_int, ctypes.c_void_p, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                               ctypes.c_int),
+        "cs_scan_last_timing": ([_dp, _dp, _dp, _i64p], ctypes.c_int),
         "cs_grm_build": ([_vp, ctypes.c_int, ctypes.c_float, _fp, _dp, _dp, _i64p], ctypes.c_int),
         "cs_grm_build_device": ([_vp, ctypes.c_int, ctypes.c_float, _vp, _dp, _i64p], ctypes.c_int),
         "cs_grm_last_timing": ([_dp, _dp, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
@@ -498,6 +502,32 @@ def lmm_scan(reader, keep, R, n_score=1, alpha=0.0, quad=None, quad_w=None, cov=
     return out
 
 
+def lmm_scan_device(reader, keep, dR, n_score=1, alpha=0.0, quad=None, quad_w=None, cov=None, cov_C=None,
+                    min_maf=0.01, snps=None, prec=CS_PREC_INT8_L3, out=None):
+    """Fused per-SNP scan with R (a DeviceArray, n x nr) and the statistics in device memory.
+    out: dict of DeviceArrays to fill (freq, n_obs, pass, beta, se, p, log10p; made when None).
+    Returns (out, timing) with timing = (ms counts, ms products, ms epilogues, packed bytes)."""
+    keep = _opt_i32(keep)
+    snps = _opt_i32(snps)
+    ns = reader.num_snps if snps is None else snps.size
+    qb, qe = quad if quad is not None else (0, 0)
+    cb, ce = cov if cov is not None else (0, 0)
+    qw = np.ascontiguousarray(quad_w if quad_w is not None else np.zeros(0), dtype=np.float64)
+    cC = np.ascontiguousarray(cov_C if cov_C is not None else np.zeros(0), dtype=np.float64)
+    m = ScanModel(n_score, float(alpha), qb, qe, _p(qw, _dp), cb, ce, _p(cC, _dp))
+    if out is None:
+        out = {k: DeviceArray(ns, n_score) for k in ("beta", "se", "p", "log10p")}
+        out.update(freq=DeviceArray(ns, 1), n_obs=DeviceArray(ns, 1), passes=DeviceArray((ns + 1) // 2, 1))
+    _check(lib().cs_lmm_scan_device(reader._h, _p(keep, _i32p), 0 if keep is None else keep.size, dR.ptr, dR.n,
+                                    dR.k, float(min_maf), _p(snps, _i32p), 0 if snps is None else snps.size,
+                                    int(prec), ctypes.byref(m), out["freq"].ptr, out["n_obs"].ptr,
+                                    out["passes"].ptr, out["beta"].ptr, out["se"].ptr, out["p"].ptr,
+                                    out["log10p"].ptr))
+    a, b, c, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+    _check(lib().cs_scan_last_timing(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
+    return out, (a.value, b.value, c.value, d.value)
+
+
 # ----------------------------------------------------------------------------- GRM (kernel b)
 class GrmBuilder:
     """Dense GRM build (GrmBuilder, libCosmicCore/src/grm_builder.h:15-80; compute at
@@ -813,8 +843,8 @@ def pcg_projected_scan(op, reader, keep, Y, X, sigma_g, sigma_e, cal_snps, snps=
                        rel_tol=1e-6, max_iter=200, prec=CS_PREC_INT8_L3, exact_log10p=0.0):
     """PCG-projected GLS scan with kinship operator `op` (see cs_pcg_projected_scan).
 
-    exact_log10p: threshold of the exact stage (0 = the default 2, negative = every tested SNP,
-    ``inf`` = off)."""
+    exact_log10p: level of the exact stage, on the smallest P over the traits Bonferroni-adjusted
+    for their number (0 = the default 5, negative = every tested SNP, ``inf`` = off)."""
     keep = _opt_i32(keep)
     snps = _opt_i32(snps)
     n = op.rows()

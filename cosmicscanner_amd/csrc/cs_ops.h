@@ -128,10 +128,11 @@ void pscan_exact(const PscanProj& pj, int64_t S, const int64_t* rows, const doub
                  const double* dAux, bool aux_is_residual, double* beta, double* se, double* p, double* log10p,
                  bool device_out, hipStream_t st, rocblas_handle blas);
 #endif
-// Exact-stage threshold of the projected scans: 0 -> the default, < 0 -> -1 (every tested SNP),
-// +inf -> off; NaN is rejected.
-constexpr double kPscanExactDefault = 2.0;
-double pscan_threshold(double exact_log10p);
+// Exact-stage threshold of the projected scans on the largest -log10 P over T traits: the given
+// level (0 -> the default 5) plus log10 T, i.e. the smallest P Bonferroni-adjusted over the traits
+// reaches 10^-level; < 0 -> -1 (every tested SNP); +inf -> off; NaN is rejected.
+constexpr double kPscanExactDefault = 5.0;
+double pscan_threshold(double exact_log10p, int64_t T);
 // Solve tolerance of the exact stage from the request's PCG tolerance (the variational variance
 // is accurate to the square of the residual).
 inline double pscan_exact_tol(double rel_tol) { return rel_tol * 100.0 < 1e-4 ? rel_tol * 100.0 : 1e-4; }

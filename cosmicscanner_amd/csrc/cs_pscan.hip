@@ -207,10 +207,12 @@ void pscan_scan(cs_bed* bed, const int32_t* keep, const PscanProj& pj, const int
     run_scan(bed, a, &m, o);
 }
 
-double pscan_threshold(double exact_log10p) {
+double pscan_threshold(double exact_log10p, int64_t T) {
     if (std::isnan(exact_log10p)) throw InvalidArgument("projected scan: exact_log10p is NaN");
-    if (exact_log10p == 0.0) return kPscanExactDefault;
-    return exact_log10p < 0.0 ? -1.0 : exact_log10p;
+    if (std::isinf(exact_log10p) && exact_log10p > 0.0) return exact_log10p;
+    if (exact_log10p < 0.0) return -1.0;
+    const double level = exact_log10p == 0.0 ? kPscanExactDefault : exact_log10p;
+    return level + std::log10((double)std::max<int64_t>(1, T));
 }
 
 // flags[i] = 1 for rows that pass and whose largest -log10 P over the traits reaches thr
@@ -468,7 +470,7 @@ extern "C" int cs_pcg_projected_scan(cs_kinship* K, cs_bed* bed, const int32_t* 
         if (!freq || !n_obs || !pass || !beta || !se || !p || !log10p)
             throw InvalidArgument("cs_pcg_projected_scan: null output array");
         if (T < 1 || c < 1 || c > 32) throw InvalidArgument("cs_pcg_projected_scan: need T >= 1 and 1 <= c <= 32");
-        const double thr = pscan_threshold(exact_log10p);
+        const double thr = pscan_threshold(exact_log10p, T);
         int64_t n = 0;
         if (cs_kinship_rows(K, &n) != CS_OK) throw RuntimeError(cs_last_error());
         const int64_t nk = (keep && n_keep > 0) ? n_keep : bed->N;

@@ -106,6 +106,12 @@ __global__ void k_scan_epilogue(const int32_t* __restrict__ snps, int64_t n_snps
     }
 }
 
+// Event times of the last scan pass (any thread): class counts, genotype products, epilogues.
+static struct {
+    double ms_counts = 0.0, ms_products = 0.0, ms_epilogue = 0.0;
+    int64_t packed_bytes = 0;
+} g_scan_timing;
+
 void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const ScanOut& o) {
     if (a.nr < 1) throw InvalidArgument("scan: need at least one projected column");
     ensure_stats(b);
@@ -148,7 +154,12 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
         CS_HIP(hipMemcpyAsync(doff.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, st));
         if (!idx.empty()) CS_HIP(hipMemcpyAsync(didx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, st));
         DevBuf counts((size_t)M * 4 * sizeof(int64_t));
+        hipEvent_t evc[2];
+        CS_HIP(hipEventCreate(&evc[0]));
+        CS_HIP(hipEventCreate(&evc[1]));
+        CS_HIP(hipEventRecord(evc[0], st));
         weighted_counts(b, w.data(), (int64_t)w.size(), counts.as<int64_t>(), st);
+        CS_HIP(hipEventRecord(evc[1], st));
         // R into device (compact n x nr) then scatter to the full sample space
         DevBuf dR;
         const double* Rd = a.R;
@@ -194,26 +205,38 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
             md.quad_w = dqw.as<double>();
             md.cov_C = dC.as<double>();
         }
-        // outputs on device
-        DevBuf dsn((size_t)ns * 4), dfreq((size_t)ns * 8), dnobs((size_t)ns * 8), dpass((size_t)ns * 4);
+        // outputs on the device: the caller's arrays when they are device memory, else staging
+        DevBuf dsn((size_t)ns * 4), sfreq, snobs, spass, sdot, sgtg, sbeta, sse, sp, sl;
         CS_HIP(hipMemcpyAsync(dsn.p, snps, (size_t)ns * 4, hipMemcpyHostToDevice, st));
-        DevBuf ddot, dgtg, dbeta, dse, dp, dl;
-        if (o.dot) ddot.alloc((size_t)(ns * a.nr) * 8);
-        if (o.gtg) dgtg.alloc((size_t)ns * 8);
         const int64_t nsc = model ? model->n_score : 0;
-        if (model) {
-            dbeta.alloc((size_t)(ns * nsc) * 8);
-            dse.alloc((size_t)(ns * nsc) * 8);
-            dp.alloc((size_t)(ns * nsc) * 8);
-            dl.alloc((size_t)(ns * nsc) * 8);
-        }
+        auto dev_out = [&](void* user, DevBuf& stage, size_t bytes, bool wanted) -> void* {
+            if (!wanted) return nullptr;
+            if (o.device && user) return user;
+            stage.alloc(bytes);
+            return stage.p;
+        };
+        double* pfreq = (double*)dev_out(o.freq, sfreq, (size_t)ns * 8, true);
+        int64_t* pnobs = (int64_t*)dev_out(o.n_obs, snobs, (size_t)ns * 8, true);
+        int32_t* ppass = (int32_t*)dev_out(o.pass, spass, (size_t)ns * 4, true);
+        double* pdot = (double*)dev_out(o.dot, sdot, (size_t)(ns * a.nr) * 8, o.dot != nullptr);
+        double* pgtg = (double*)dev_out(o.gtg, sgtg, (size_t)ns * 8, o.gtg != nullptr);
+        double* pbeta = (double*)dev_out(o.beta, sbeta, (size_t)(ns * nsc) * 8, model != nullptr);
+        double* pse = (double*)dev_out(o.se, sse, (size_t)(ns * nsc) * 8, model != nullptr);
+        double* pp = (double*)dev_out(o.p, sp, (size_t)(ns * nsc) * 8, model != nullptr);
+        double* pl = (double*)dev_out(o.log10p, sl, (size_t)(ns * nsc) * 8, model != nullptr);
         // SNP-row chunks of the Z^T R product
         const int64_t row_lo0 = lo / kPad * kPad;
         const int64_t row_hi = round_up(hi + 1, kPad);
         const int64_t G16 = 16 * (int64_t)tl.G;
         int64_t chunk = std::max<int64_t>(kPad, ((int64_t)(256ll << 20) / (16ll * G16)) / kPad * kPad);
         DevBuf PQ((size_t)std::min<int64_t>(chunk, row_hi - row_lo0) * 2 * G16 * sizeof(long long));
+        std::vector<hipEvent_t> ev;
         for (int64_t r0 = row_lo0; r0 < row_hi; r0 += chunk) {
+            for (int e = 0; e < 3; ++e) {
+                ev.push_back(nullptr);
+                CS_HIP(hipEventCreate(&ev.back()));
+            }
+            CS_HIP(hipEventRecord(ev[ev.size() - 3], st));
             const int64_t rows = std::min<int64_t>(chunk, row_hi - r0);
             GenoArgs ga{};
             ga.A = b->snp.as<uint8_t>() + (((r0 / 64) * b->KBs) << 12);
@@ -225,31 +248,45 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
             ga.k = a.nr;
             ga.pq = PQ.as<long long>();
             launch_geno(L, false, EPI_PQ, dim3((unsigned)(rows / 256), 1, (unsigned)tl.G), st, ga, false, tl.packed);
+            CS_HIP(hipEventRecord(ev[ev.size() - 2], st));
             k_scan_epilogue<<<grid_for(ns), 256, 0, st>>>(
                 dsn.as<int32_t>(), ns, r0, rows, counts.as<int64_t>(), PQ.as<long long>(), G16, a.nr,
-                scl.as<double>(), a.min_maf, dfreq.as<double>(), dnobs.as<int64_t>(), dpass.as<int32_t>(),
-                o.dot ? ddot.as<double>() : nullptr, o.gtg ? dgtg.as<double>() : nullptr, md,
-                model ? dbeta.as<double>() : nullptr, model ? dse.as<double>() : nullptr,
-                model ? dp.as<double>() : nullptr, model ? dl.as<double>() : nullptr);
+                scl.as<double>(), a.min_maf, pfreq, pnobs, ppass, pdot, pgtg, md, pbeta, pse, pp, pl);
             CS_HIP(hipGetLastError());
+            CS_HIP(hipEventRecord(ev[ev.size() - 1], st));
         }
-        auto d2h = [&](void* dst, const DevBuf& src, size_t bytes) {
-            if (dst)
-                CS_HIP(hipMemcpyAsync(dst, src.p, bytes, o.device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+        // staged outputs to the caller (host copies, or device copies when the caller passed
+        // device memory only for some)
+        auto d2h = [&](void* dst, const DevBuf& stage, size_t bytes) {
+            if (dst && stage.p)
+                CS_HIP(hipMemcpyAsync(dst, stage.p, bytes, o.device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                                       st));
         };
-        d2h(o.freq, dfreq, (size_t)ns * 8);
-        d2h(o.n_obs, dnobs, (size_t)ns * 8);
-        d2h(o.pass, dpass, (size_t)ns * 4);
-        if (o.dot) d2h(o.dot, ddot, (size_t)(ns * a.nr) * 8);
-        if (o.gtg) d2h(o.gtg, dgtg, (size_t)ns * 8);
+        d2h(o.freq, sfreq, (size_t)ns * 8);
+        d2h(o.n_obs, snobs, (size_t)ns * 8);
+        d2h(o.pass, spass, (size_t)ns * 4);
+        d2h(o.dot, sdot, (size_t)(ns * a.nr) * 8);
+        d2h(o.gtg, sgtg, (size_t)ns * 8);
         if (model) {
-            d2h(o.beta, dbeta, (size_t)(ns * nsc) * 8);
-            d2h(o.se, dse, (size_t)(ns * nsc) * 8);
-            d2h(o.p, dp, (size_t)(ns * nsc) * 8);
-            d2h(o.log10p, dl, (size_t)(ns * nsc) * 8);
+            d2h(o.beta, sbeta, (size_t)(ns * nsc) * 8);
+            d2h(o.se, sse, (size_t)(ns * nsc) * 8);
+            d2h(o.p, sp, (size_t)(ns * nsc) * 8);
+            d2h(o.log10p, sl, (size_t)(ns * nsc) * 8);
         }
         CS_HIP(hipStreamSynchronize(st));
+        float ms = 0.f;
+        double mp = 0.0, me = 0.0;
+        for (size_t e = 0; e + 2 < ev.size(); e += 3) {
+            if (hipEventElapsedTime(&ms, ev[e], ev[e + 1]) == hipSuccess) mp += ms;
+            if (hipEventElapsedTime(&ms, ev[e + 1], ev[e + 2]) == hipSuccess) me += ms;
+        }
+        g_scan_timing.ms_counts = hipEventElapsedTime(&ms, evc[0], evc[1]) == hipSuccess ? ms : 0.0;
+        g_scan_timing.ms_products = mp;
+        g_scan_timing.ms_epilogue = me;
+        g_scan_timing.packed_bytes = (row_hi - row_lo0) * b->KBs * 64;
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(evc[0]);
+        (void)hipEventDestroy(evc[1]);
     } catch (...) {
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
@@ -384,6 +421,37 @@ int cs_lmm_scan(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* 
         o.freq = freq; o.n_obs = n_obs; o.pass = pass;
         o.beta = beta; o.se = se; o.p = p; o.log10p = log10p;
         run_scan(bed, a, model, o);
+    });
+}
+
+int cs_lmm_scan_device(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* dR, int64_t ldr,
+                       int64_t nr, double min_maf, const int32_t* snps, int64_t n_snps, int prec,
+                       const cs_scan_model* model, double* d_freq, int64_t* d_n_obs, int32_t* d_pass,
+                       double* d_beta, double* d_se, double* d_p, double* d_log10p) {
+    return guarded([&] {
+        if (!bed || !dR || !model) throw InvalidArgument("cs_lmm_scan_device: null argument");
+        ScanArgs a;
+        a.keep = (keep && n_keep > 0) ? keep : nullptr;
+        a.n = a.keep ? n_keep : bed->N;
+        a.R = dR; a.ldr = ldr; a.nr = nr; a.min_maf = min_maf;
+        a.snps = (snps && n_snps > 0) ? snps : nullptr;
+        a.n_snps = a.snps ? n_snps : bed->M;
+        a.prec = prec;
+        a.device_R = true;
+        ScanOut o;
+        o.freq = d_freq; o.n_obs = d_n_obs; o.pass = d_pass;
+        o.beta = d_beta; o.se = d_se; o.p = d_p; o.log10p = d_log10p;
+        o.device = true;
+        run_scan(bed, a, model, o);
+    });
+}
+
+int cs_scan_last_timing(double* ms_counts, double* ms_products, double* ms_epilogue, int64_t* packed_bytes) {
+    return guarded([&] {
+        if (ms_counts) *ms_counts = g_scan_timing.ms_counts;
+        if (ms_products) *ms_products = g_scan_timing.ms_products;
+        if (ms_epilogue) *ms_epilogue = g_scan_timing.ms_epilogue;
+        if (packed_bytes) *packed_bytes = g_scan_timing.packed_bytes;
     });
 }
 

@@ -160,7 +160,7 @@ struct Request {
     std::vector<float> X;
     double sg = 0, se = 1;
     std::vector<int32_t> cal, snps;
-    double min_maf = 0.01, tol = 1e-6, thr = kPscanExactDefault;
+    double min_maf = 0.01, tol = 1e-6, thr = kPscanExactDefault;   // effective (pscan_threshold)
     int max_iter = 200, prec = CS_PREC_INT8_L3;
     // phase 1: the main solve over [Y, X, G_cal]; phase 2: exact-stage chunks of selected markers
     int phase = 1;
@@ -499,6 +499,7 @@ int cs_server_submit(cs_server* s, const cs_request* r, int64_t* id) {
         q->n = (int64_t)q->keep.size();
         if (r->ldy < q->n || r->ldx < q->n) throw RuntimeError("server: dimension mismatch");
         q->T = r->n_traits;
+        q->thr = pscan_threshold(r->exact_log10p, q->T);
         q->c = r->n_cov;
         q->Y.resize((size_t)(q->n * q->T));
         for (int64_t t = 0; t < q->T; ++t)
@@ -518,7 +519,7 @@ int cs_server_submit(cs_server* s, const cs_request* r, int64_t* id) {
         q->tol = r->rel_tol > 0.0 ? r->rel_tol : 1e-6;
         q->max_iter = r->max_iter > 0 ? r->max_iter : 200;
         q->prec = r->prec ? r->prec : CS_PREC_INT8_L3;
-        q->thr = pscan_threshold(r->exact_log10p);
+
         q->t_submit = s->now();
         if (id) *id = q->id;
         s->pending.push_back(q->id);

@@ -173,6 +173,18 @@ int cs_lmm_scan(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* 
                 int64_t nr, double min_maf, const int32_t* snps, int64_t n_snps, int prec,
                 const cs_scan_model* model, double* freq, int64_t* n_obs, int32_t* pass,
                 double* beta, double* se, double* p, double* log10p);
+/* Same with R and every output in device memory (d_R n x nr with leading dimension ldr; outputs
+ * as cs_lmm_scan, each may be NULL): the statistics stay in HBM, written by the epilogue kernel
+ * directly, with no host transfer -- for device-resident pipelines and serving. */
+int cs_lmm_scan_device(cs_bed* bed, const int32_t* keep, int64_t n_keep, const double* d_R, int64_t ldr,
+                       int64_t nr, double min_maf, const int32_t* snps, int64_t n_snps, int prec,
+                       const cs_scan_model* model, double* d_freq, int64_t* d_n_obs, int32_t* d_pass,
+                       double* d_beta, double* d_se, double* d_p, double* d_log10p);
+/* Device times of the last scan pass in this process (HIP events on its stream, ms): the weighted
+ * class counts, the genotype products g^T R (summed over SNP chunks) and the fused epilogues;
+ * packed_bytes: the SNP-major genotype bytes the products read.  Any may be NULL. */
+int cs_scan_last_timing(double* ms_counts, double* ms_products, double* ms_epilogue,
+                        int64_t* packed_bytes);
 
 /* ---- Kernel (b): dense GRM build.
  * GrmBuilder::compute(const PlinkReader&) (libCosmicCore/src/grm_builder.cpp:63-399) for
@@ -245,15 +257,16 @@ int cs_markers_device(cs_bed* bed, const int32_t* keep, int64_t n_keep, const in
  * P y = Z_Y - Z_X (X^T Z_X)^-1 X^T Z_Y (the GLS projection of LmmEngine::scan,
  * src/lmm_engine.cpp:1363-1525), calibration gamma = mean_cal g^T P g / g~^T g~ (g~ = g projected
  * off X), and per SNP and trait: score = g^T P y_t, var = gamma g~^T g~, beta, se, p, log10p
- * (n_snps x T row-major; NaN when not tested).  Exact stage: every tested SNP whose largest
- * -log10 P over the traits reaches exact_log10p is recomputed with its exact GLS variance
- * g^T P g (a second multi-RHS PCG solve over those markers, tolerance min(100 rel_tol, 1e-4), and
- * the variational form g^T z + z^T r of g^T V^-1 g, accurate to the square of the residual) and
- * score g^T P y_t in FP64.  exact_log10p: 0 -> 2 (the default), < 0 -> every tested SNP,
- * +inf -> off.  Below the threshold the calibrated statistics stand; dlog10p_bound_below states
- * their |d(-log10 P)| bound from the spread of the calibration markers' ratios (4 standard
- * deviations of the variance ratio at the threshold statistic; 0 when every SNP is exact, NaN when
- * the stage is off or fewer than two markers calibrate).  Y: n x T FP64 (ldy); X: n x c float32
+ * (n_snps x T row-major; NaN when not tested).  Exact stage: every tested SNP whose smallest
+ * P over the T traits, Bonferroni-adjusted (x T), reaches 10^-exact_log10p -- i.e. whose largest
+ * -log10 P reaches exact_log10p + log10 T -- is recomputed with its exact GLS variance g^T P g (a
+ * second multi-RHS PCG solve over those markers, tolerance min(100 rel_tol, 1e-4), and the
+ * variational form g^T z + z^T r of g^T V^-1 g, accurate to the square of the residual) and score
+ * g^T P y_t in FP64.  exact_log10p: 0 -> 5 (the default: suggestive significance), < 0 -> every
+ * tested SNP, +inf -> off.  Below the threshold the calibrated statistics stand;
+ * dlog10p_bound_below states their |d(-log10 P)| bound from the spread of the calibration
+ * markers' ratios (4 standard deviations of the variance ratio at the threshold statistic; 0 when
+ * every SNP is exact, NaN when the stage is off or fewer than two markers calibrate).  Y: n x T FP64 (ldy); X: n x c float32
  * (ldx, the intercept included, 1 <= c <= 32).  cal_snps: calibration SNPs (those failing the
  * filter are skipped). */
 typedef struct cs_pscan_info {
@@ -264,7 +277,8 @@ typedef struct cs_pscan_info {
     int64_t n_exact;              /* SNPs recomputed by the exact stage */
     int exact_pcg_iterations;     /* largest iteration count of its solves */
     double calibration_rel_sd;    /* relative standard deviation of the calibration ratios */
-    double exact_log10p;          /* threshold used (-1: every SNP; inf: off) */
+    double exact_log10p;          /* threshold used on the largest -log10 P over the traits
+                                     (level + log10 T; -1: every SNP; inf: off) */
     double dlog10p_bound_below;   /* stated bound below the threshold */
 } cs_pscan_info;
 int cs_pcg_projected_scan(cs_kinship* K, cs_bed* bed, const int32_t* keep, int64_t n_keep,
@@ -445,7 +459,7 @@ typedef struct cs_gwas_mf_options {
     double pcg_tol;        /* mode 1: relative residual, 1e-6 */
     int pcg_max_iter;      /* mode 1: 200 */
     int n_calibration;     /* mode 1: calibration SNPs per chromosome, 32 */
-    double exact_log10p;   /* mode 1: exact-stage threshold (cs_pcg_projected_scan; 0 = 2) */
+    double exact_log10p;   /* mode 1: exact-stage threshold (cs_pcg_projected_scan; 0 = 5) */
 } cs_gwas_mf_options;
 int cs_gwas_matrix_free_run(const char* bfile, const double* y, const double* cov, int64_t ldc,
                             int64_t n_cov, const cs_gwas_mf_options* opt, const char* out_prefix,

@@ -77,6 +77,8 @@ def lib():
         L.orc_class_counts.argtypes = [_u8p, _i64, _i64, _i32p, _i64p, ctypes.c_int]
         L.orc_kinship_apply.argtypes = [_u8p, _i64, _dp, _dp, _i32p, _i64, _i32p, _i64, _dp, _i64,
                                         _dp, ctypes.c_int]
+        L.orc_kinship_apply_blocked.argtypes = [_u8p, _i64, _dp, _dp, _i32p, _i64, _i32p, _i64, _dp, _i64,
+                                                _dp, ctypes.c_int]
         L.orc_kinship_trace.argtypes = [_u8p, _i64, _dp, _dp, _i32p, _i64, _i32p, _i64]
         L.orc_kinship_trace.restype = ctypes.c_double
         L.orc_kinship_diag.argtypes = [_u8p, _i64, _dp, _dp, _i32p, _i64, _i32p, _i64, _dp]
@@ -178,6 +180,26 @@ def kinship_apply(bed, N, means, inv_stds, keep, variants, X, nthreads=None):
                             _p(np.ascontiguousarray(inv_stds), _dp), _p(keep, _i32p), n,
                             _p(variants, _i32p), Msel, _p(Xf, _dp), k, _p(Y, _dp),
                             _nthreads(nthreads))
+    return np.ascontiguousarray(Y)
+
+
+def kinship_apply_blocked(bed, N, means, inv_stds, keep, variants, X, nthreads=None):
+    """K X in the reference's own form (kinship_operator.cpp:225-299: 512-SNP blocks decoded to
+    FP64, z^T X then z (z^T X)), cache-blocked and threaded: the CPU baseline.  Pinned against
+    kinship_apply (the per-SNP form) in tests/test_oracle.py."""
+    bed = _u8(bed)
+    keep = _i32opt(keep)
+    variants = _i32opt(variants)
+    X = np.asfortranarray(np.asarray(X, dtype=np.float64))
+    n = N if keep is None else keep.size
+    Msel = means.size if variants is None else variants.size
+    if X.shape[0] != n:
+        raise RuntimeError("PlinkKinshipOperator::apply: dimension mismatch")
+    k = X.shape[1]
+    Y = np.empty((n, k), order="F")
+    lib().orc_kinship_apply_blocked(_p(bed, _u8p), N, _p(np.ascontiguousarray(means), _dp),
+                                    _p(np.ascontiguousarray(inv_stds), _dp), _p(keep, _i32p), n,
+                                    _p(variants, _i32p), Msel, _p(X, _dp), k, _p(Y, _dp), _nthreads(nthreads))
     return np.ascontiguousarray(Y)
 
 
@@ -473,9 +495,10 @@ def pcg_projected_scan(bed, N, M, keep, variants, Y, X, sigma_g, sigma_e, cal_sn
     gamma over cal_snps, var = gamma g~^T g~.  Returns (stats per trait dict of (M,) arrays,
     gamma); SNPs failing the filter or var <= 1e-12 are NaN.
 
-    exact_log10p (None: no exact stage; 0 -> 2; < 0: every tested SNP; inf: off): SNPs that pass
-    and whose largest calibrated -log10 P over the traits reaches the threshold get the textbook
-    GLS statistics instead (score g^T P y, var g^T P g with the explicit P, as gls_scan)."""
+    exact_log10p (None: no exact stage; 0 -> 5; < 0: every tested SNP; inf: off): SNPs that pass
+    and whose largest calibrated -log10 P over the T traits reaches exact_log10p + log10 T get the
+    textbook GLS statistics instead (score g^T P y, var g^T P g with the explicit P, as
+    gls_scan)."""
     K = dense_kinship(bed, N, M, keep, variants)
     n = K.shape[0]
     Y = np.asarray(Y, dtype=np.float64).reshape(n, -1)
@@ -505,7 +528,7 @@ def pcg_projected_scan(bed, N, M, keep, variants, Y, X, sigma_g, sigma_e, cal_sn
             st[key] = np.where(bad, np.nan, st[key])
         out.append(st)
     if exact_log10p is not None and not np.isinf(exact_log10p):
-        thr = 2.0 if exact_log10p == 0 else (-1.0 if exact_log10p < 0 else float(exact_log10p))
+        thr = -1.0 if exact_log10p < 0 else (5.0 if exact_log10p == 0 else float(exact_log10p)) + np.log10(T)
         L = np.column_stack([o["log10p"] for o in out])
         if thr < 0:
             sel = ps.astype(bool) & ~np.isnan(L[:, 0])

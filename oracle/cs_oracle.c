@@ -133,6 +133,149 @@ void orc_kinship_apply(const uint8_t* bed, int64_t N, const double* means,
     free(U);
 }
 
+/* A7 as the reference computes it (kinship_operator.cpp:225-299), the CPU baseline's form: blocks
+ * of 512 selected SNPs are decoded to FP64 z (n x 512, one column per SNP, inv_std == 0 -> zero
+ * column), projected = z^T X (512 x k), result += z projected, and finally result x 1/M.  The two
+ * products run over 256-row tiles with OpenMP (per-thread partial sums of z^T X, added in thread
+ * order), so the same inputs give the same bits for a given thread count.  Same values as
+ * orc_kinship_apply up to FP64 summation order. */
+#define ORC_KB 512
+#define ORC_RT 256
+typedef double orc_v4d __attribute__((vector_size(32)));
+
+__attribute__((target_clones("avx2", "default")))
+static void orc_blk_zt_x(const double* Z, int64_t n, int cnt, const double* X, int64_t k, double* part,
+                         int64_t r0, int64_t r1) {
+    /* part[t * k + c] += sum_{r in [r0, r1)} Z[t n + r] X[c n + r]: the X tile is transposed to
+     * [r][16 columns], and two SNP rows accumulate 16 columns each in registers, sequentially in r */
+    orc_v4d xt[ORC_RT * 4];
+    for (int64_t c0 = 0; c0 < k; c0 += 16) {
+        const int kc = (int)((k - c0) < 16 ? (k - c0) : 16);
+        for (int64_t a = r0; a < r1; a += ORC_RT) {
+            const int64_t b = a + ORC_RT < r1 ? a + ORC_RT : r1;
+            for (int64_t r = a; r < b; ++r)
+                for (int c = 0; c < 16; ++c) ((double*)xt)[(r - a) * 16 + c] = c < kc ? X[(c0 + c) * n + r] : 0.0;
+            for (int t = 0; t < cnt; t += 2) {
+                const double* z0 = Z + (int64_t)t * n;
+                const double* z1 = Z + (int64_t)(t + 1 < cnt ? t + 1 : t) * n;
+                orc_v4d a0[4] = {{0}}, a1[4] = {{0}};
+                for (int64_t r = a; r < b; ++r) {
+                    const orc_v4d* xr = xt + (r - a) * 4;
+                    const double s0 = z0[r], s1 = z1[r];
+                    for (int q = 0; q < 4; ++q) {
+                        a0[q] += s0 * xr[q];
+                        a1[q] += s1 * xr[q];
+                    }
+                }
+                for (int c = 0; c < kc; ++c) part[t * k + c0 + c] += a0[c / 4][c % 4];
+                if (t + 1 < cnt)
+                    for (int c = 0; c < kc; ++c) part[(t + 1) * k + c0 + c] += a1[c / 4][c % 4];
+            }
+        }
+    }
+}
+
+__attribute__((target_clones("avx2", "default")))
+static void orc_blk_z_p(const double* Z, int64_t n, int cnt, const double* P, int64_t k, double* R, int64_t r0,
+                        int64_t r1) {
+    /* R[c n + r] += sum_t Z[t n + r] P[t k + c] for r in [r0, r1): 8 rows x 8 columns held in
+     * registers over all t (t in order) */
+    const int64_t r8 = r0 + (r1 - r0) / 8 * 8;
+    for (int tb = 0; tb < cnt; tb += 64)   /* 64 SNP columns at a time: few pages per row chunk */
+    for (int64_t r = r0; r < r8; r += 8) {
+        const int te = tb + 64 < cnt ? tb + 64 : cnt;
+        for (int64_t c0 = 0; c0 < k; c0 += 8) {
+            const int kc = (int)((k - c0) < 8 ? (k - c0) : 8);
+            orc_v4d lo[8], hi[8];
+            for (int c = 0; c < 8; ++c) {
+                if (c < kc) {
+                    memcpy(&lo[c], R + (c0 + c) * n + r, 32);
+                    memcpy(&hi[c], R + (c0 + c) * n + r + 4, 32);
+                } else {
+                    lo[c] = hi[c] = (orc_v4d){0, 0, 0, 0};
+                }
+            }
+            for (int t = tb; t < te; ++t) {
+                const double* z = Z + (int64_t)t * n + r;
+                orc_v4d z0, z1;
+                memcpy(&z0, z, 32);
+                memcpy(&z1, z + 4, 32);
+                const double* pt = P + t * k + c0;
+                for (int c = 0; c < 8; ++c) {
+                    const double p = c < kc ? pt[c] : 0.0;
+                    lo[c] += z0 * p;
+                    hi[c] += z1 * p;
+                }
+            }
+            for (int c = 0; c < kc; ++c) {
+                memcpy(R + (c0 + c) * n + r, &lo[c], 32);
+                memcpy(R + (c0 + c) * n + r + 4, &hi[c], 32);
+            }
+        }
+    }
+    for (int64_t r = r8; r < r1; ++r)
+        for (int64_t c = 0; c < k; ++c) {
+            double acc = R[c * n + r];
+            for (int t = 0; t < cnt; ++t) acc += Z[(int64_t)t * n + r] * P[t * k + c];
+            R[c * n + r] = acc;
+        }
+}
+
+void orc_kinship_apply_blocked(const uint8_t* bed, int64_t N, const double* means, const double* inv_stds,
+                               const int32_t* keep, int64_t n, const int32_t* variants, int64_t Msel,
+                               const double* X, int64_t k, double* Y, int nthreads) {
+    const int64_t B = bytes_per_snp(N);
+    if (nthreads < 1) nthreads = 1;
+    double* Z = (double*)malloc((size_t)(n * ORC_KB) * sizeof(double));
+    double* P = (double*)malloc((size_t)(ORC_KB * (k > 0 ? k : 1)) * sizeof(double));
+    double* parts = (double*)malloc((size_t)(nthreads * ORC_KB * (k > 0 ? k : 1)) * sizeof(double));
+    double* R = (double*)calloc((size_t)(n * k > 0 ? n * k : 1), sizeof(double));
+    for (int64_t first = 0; first < Msel; first += ORC_KB) {
+        const int cnt = (int)((Msel - first) < ORC_KB ? (Msel - first) : ORC_KB);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int j = 0; j < cnt; ++j) {
+            const int64_t snp = variants ? variants[first + j] : first + j;
+            const double mean = means[snp], s = inv_stds[snp];
+            double* dest = Z + (int64_t)j * n;
+            if (s == 0.0) {
+                memset(dest, 0, (size_t)n * sizeof(double));
+                continue;
+            }
+            const double vals[4] = {(2.0 - mean) * s, 0.0, (1.0 - mean) * s, -mean * s};
+            const uint8_t* col = bed + snp * B;
+            for (int64_t r = 0; r < n; ++r) dest[r] = vals[code_at(col, keep ? keep[r] : r)];
+        }
+        memset(parts, 0, (size_t)(nthreads * ORC_KB * k) * sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+        {
+            int th = 0, nth = 1;
+#ifdef _OPENMP
+            th = omp_get_thread_num();
+            nth = omp_get_num_threads();
+#endif
+            const int64_t per = (n + nth - 1) / nth;
+            const int64_t r0 = th * per, r1 = (r0 + per) < n ? (r0 + per) : n;
+            if (r0 < r1) orc_blk_zt_x(Z, n, cnt, X, k, parts + (int64_t)th * ORC_KB * k, r0, r1);
+#pragma omp barrier
+#pragma omp single
+            {
+                for (int64_t e = 0; e < (int64_t)cnt * k; ++e) {
+                    double v = 0.0;
+                    for (int t2 = 0; t2 < nth; ++t2) v += parts[(int64_t)t2 * ORC_KB * k + e];
+                    P[e] = v;
+                }
+            }
+            if (r0 < r1) orc_blk_z_p(Z, n, cnt, P, k, R, r0, r1);
+        }
+    }
+    const double scaling = 1.0 / (double)Msel;
+    for (int64_t e = 0; e < n * k; ++e) Y[e] = R[e] * scaling;
+    free(R);
+    free(parts);
+    free(P);
+    free(Z);
+}
+
 /* A8: trace (kinship_operator.cpp:306-344) -- (1/M) sum_t sum_r z^2, missing skipped. */
 double orc_kinship_trace(const uint8_t* bed, int64_t N, const double* means,
                          const double* inv_stds, const int32_t* keep, int64_t n,

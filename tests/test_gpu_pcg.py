@@ -8,7 +8,7 @@
   restatement of the same estimator (exact V^-1, same calibration markers): |d log10 P| <= 1e-3,
   gamma relative 1e-4.
 * Projected scan with the exact stage: every SNP whose largest -log10 P reaches the threshold
-  (default 2) against textbook GLS with an explicit dense V^-1 (the definition of LmmEngine::scan,
+  (level 5 by default, 2 and -1 here, plus log10 of the number of traits) against textbook GLS with an explicit dense V^-1 (the definition of LmmEngine::scan,
   src/lmm_engine.cpp:1363-1525): |d log10 P| <= 1e-3 and |d beta| <= 1e-3 SE, at n = 400 and at
   n = 2000; below the threshold the distance stays within the reported bound.  With threshold -1
   every tested SNP meets the 1e-3 bound.
@@ -109,11 +109,12 @@ def test_projected_scan_vs_oracle(csa, data):
         assert np.max(np.abs(b - br) / sr) <= 1e-3
     # the default exact stage against textbook GLS, and against the oracle's restatement of the
     # combined estimator
-    ex_out = csa.pcg_projected_scan(op, r, keep, Y, X, 0.45, 0.55, cal, snps=snps, rel_tol=1e-9, max_iter=500)
+    ex_out = csa.pcg_projected_scan(op, r, keep, Y, X, 0.45, 0.55, cal, snps=snps, rel_tol=1e-9, max_iter=500,
+                                    exact_log10p=2.0)
     K = oracle.dense_kinship(body, N, M, keep, loco)
     Gm, okm = oracle.prepared_markers(body, N, M, keep, snps)
-    check_exact_stage(ex_out, out, K, X, Y, Gm, okm, 0.45, 0.55, 2.0)
-    refx, _, _ = oracle.pcg_projected_scan(body, N, M, keep, loco, Y, X, 0.45, 0.55, cal, exact_log10p=0.0)
+    check_exact_stage(ex_out, out, K, X, Y, Gm, okm, 0.45, 0.55, 2.0 + np.log10(3))
+    refx, _, _ = oracle.pcg_projected_scan(body, N, M, keep, loco, Y, X, 0.45, 0.55, cal, exact_log10p=2.0)
     for t in range(3):
         lp, lr = ex_out["log10p"][:, t], refx[t]["log10p"][snps]
         m = ~np.isnan(lr)
@@ -174,12 +175,14 @@ def test_projected_scan_exact_gls_n2000(csa):
     K = oracle.dense_kinship(body, N, M, keep, loco)
     calib = csa.pcg_projected_scan(op, r, keep, Y, X, 0.4, 0.6, cal, snps=snps, rel_tol=1e-6, max_iter=300,
                                    exact_log10p=float("inf"))
-    for thr in (0.0, -1.0):
+    # the default level (5), 2 and every SNP; the level applies to the smallest P over the two
+    # traits times 2
+    for level, thr in ((0.0, 5.0 + np.log10(2)), (2.0, 2.0 + np.log10(2)), (-1.0, -1.0)):
         out = csa.pcg_projected_scan(op, r, keep, Y, X, 0.4, 0.6, cal, snps=snps, rel_tol=1e-6, max_iter=300,
-                                     exact_log10p=thr)
-        assert out["max_rel_residual"] <= 1e-6
-        check_exact_stage(out, calib, K, X, Y, Gm, okm, 0.4, 0.6, 2.0 if thr == 0 else -1.0)
-        assert out["n_exact"] >= 12
+                                     exact_log10p=level)
+        assert out["max_rel_residual"] <= 1e-6 and out["exact_log10p"] == thr
+        check_exact_stage(out, calib, K, X, Y, Gm, okm, 0.4, 0.6, thr)
+        assert out["n_exact"] >= 8
     op.close()
     r.close()
 

@@ -105,3 +105,39 @@ def test_low_rank_woodbury_scan_multi_trait(csa):
         assert np.max(np.abs(out["log10p"][ok, t] - ref["log10p"][ok])) < 1e-4
         assert np.max(np.abs(out["beta"][ok, t] - ref["beta"][ok]) / ref["se"][ok]) < 1e-5
         assert np.max(np.abs(out["se"][ok, t] / ref["se"][ok] - 1)) < 1e-5
+
+
+def test_scan_device_outputs_equal_host(csa, ref_datasets):
+    """cs_lmm_scan_device (R and every statistic in HBM, written by the epilogue directly) gives
+    the same bits as cs_lmm_scan with host buffers, over a keep subset with repeats and a SNP list;
+    the pass timings are reported."""
+    body, N, M, _, _ = ref_datasets["gemma"]
+    reader = csa.PlinkReader.from_body(body, N, M)
+    rng = np.random.default_rng(5)
+    keep = rng.choice(N, int(N * 0.8), replace=False).astype(np.int32)
+    keep = np.concatenate([keep, keep[:3]])
+    T, c = 3, 2
+    R = rng.standard_normal((keep.size, T + c))
+    snps = np.sort(rng.choice(M, M // 2, replace=False)).astype(np.int32)
+    C = np.array([[0.02, 0.001], [0.001, 0.03]])
+    kw = dict(n_score=T, alpha=0.9, cov=(T, T + c), cov_C=C, min_maf=0.02, snps=snps)
+    host = csa.lmm_scan(reader, keep, R, **kw)
+    dR = csa.DeviceArray.from_host(R)
+    out, timing = csa.lmm_scan_device(reader, keep, dR, **kw)
+    ns = snps.size
+
+    def rows(d, width):
+        return d.download().reshape(-1, order="F")[: ns * width].reshape(ns, width)
+
+    for key in ("beta", "se", "p", "log10p"):
+        got = rows(out[key], T)
+        assert np.array_equal(got, host[key], equal_nan=True), key
+    assert np.array_equal(rows(out["freq"], 1)[:, 0], host["freq"])
+    assert np.array_equal(rows(out["n_obs"], 1)[:, 0].view(np.int64), host["n_obs"])
+    passes = out["passes"].download().reshape(-1, order="F").view(np.int32)[:ns]
+    assert np.array_equal(passes.astype(bool), host["passes"])
+    ms_counts, ms_prod, ms_epi, nbytes = timing
+    assert ms_prod > 0 and ms_epi > 0 and ms_counts > 0 and nbytes > 0
+    for a in [dR] + list(out.values()):
+        a.close()
+    reader.close()

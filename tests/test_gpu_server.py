@@ -45,7 +45,7 @@ def world(csa):
         cal = rng.choice(M, 16, replace=False).astype(np.int32)
         snps = np.sort(rng.choice(M, 700, replace=False)).astype(np.int32)
         reqs.append(dict(Y=Y, X=X, sigma2_g=h2, sigma2_e=1 - h2, cal_snps=cal, keep=keep, snps=snps, rel_tol=1e-9,
-                         max_iter=400))
+                         max_iter=400, exact_log10p=2.0))
     return reader, reqs
 
 
@@ -95,7 +95,7 @@ def test_server_matches_projected_scan(csa, world):
     got = run_alone(csa, reader, q)
     op = csa.PlinkKinshipOperator(reader, q["keep"])
     ref = csa.pcg_projected_scan(op, reader, q["keep"], q["Y"], q["X"], q["sigma2_g"], q["sigma2_e"], q["cal_snps"],
-                                 snps=q["snps"], rel_tol=1e-9, max_iter=400)
+                                 snps=q["snps"], rel_tol=1e-9, max_iter=400, exact_log10p=2.0)
     assert abs(got["stats"]["gamma"] / ref["gamma"] - 1) <= 1e-7
     m = ~np.isnan(ref["log10p"])
     assert np.array_equal(np.isnan(got["log10p"]), ~m)
@@ -104,7 +104,7 @@ def test_server_matches_projected_scan(csa, world):
     # and the oracle's dense-solve restatement of the estimator, exact stage included
     body = reader.download()
     st, gamma, _ = oracle.pcg_projected_scan(body, reader.num_samples, reader.num_snps, q["keep"], None, q["Y"], q["X"],
-                                             q["sigma2_g"], q["sigma2_e"], q["cal_snps"], exact_log10p=0.0)
+                                             q["sigma2_g"], q["sigma2_e"], q["cal_snps"], exact_log10p=2.0)
     assert abs(got["stats"]["gamma"] / gamma - 1) <= 1e-4
     for t in range(q["Y"].shape[1]):
         lo = st[t]["log10p"][q["snps"]]

@@ -97,6 +97,10 @@ def test_kinship_apply_matches_dense_all_selections(kb_fixture):
             X = rng.standard_normal((Z.shape[0], width))
             Y = oracle.kinship_apply(bed, N, means, inv, keep, variants, X)
             assert rel(Y, K @ X) < 1e-10
+            # the reference's 512-SNP block form (the CPU baseline), one and several threads
+            for th in (1, 3):
+                assert rel(oracle.kinship_apply_blocked(bed, N, means, inv, keep, variants, X, nthreads=th),
+                           K @ X) < 1e-10
         with pytest.raises(RuntimeError):
             oracle.kinship_apply(bed, N, means, inv, keep, variants, np.ones((Z.shape[0] + 1, 1)))
 
