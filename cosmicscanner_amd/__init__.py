@@ -82,6 +82,11 @@ def _load():
         "cs_kinship_apply": ([_vp, _dp, _i64, _i64, _dp, _i64], ctypes.c_int),
         "cs_kinship_apply_device": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
         "cs_kinship_apply_device_noalloc": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
+        "cs_validate_variance_components": ([_dp, _i64, ctypes.c_char_p], ctypes.c_int),
+        "cs_kinship_multiply_zt": ([_vp, _dp, _i64, _i64, _dp, _i64], ctypes.c_int),
+        "cs_kinship_multiply_z": ([_vp, _dp, _i64, _i64, _dp, _i64], ctypes.c_int),
+        "cs_kinship_multiply_zt_device": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
+        "cs_kinship_multiply_z_device": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
         "cs_kinship_trace": ([_vp, _dp], ctypes.c_int),
         "cs_kinship_diag": ([_vp, _dp], ctypes.c_int),
         "cs_kinship_last_timing": ([_vp, _dp], ctypes.c_int),
@@ -403,6 +408,25 @@ class PlinkKinshipOperator:
     def apply_device(self, dX_ptr, ldx, k, dY_ptr, ldy):
         """Y = K X on device buffers (FP64, column-major) -- no host transfer."""
         _check(lib().cs_kinship_apply_device(self._h, dX_ptr, ldx, k, dY_ptr, ldy))
+
+    def multiply_Zt_v(self, V):
+        """U = Z^T V over all SNPs (PlinkReader::multiply_Zt_v, keep-list form): V n x k or n."""
+        V = np.asarray(V, dtype=np.float64)
+        vec = V.ndim == 1
+        Vf = np.asfortranarray(V[:, None] if vec else V)
+        M = self.reader.num_snps
+        U = np.empty((M, Vf.shape[1]), order="F")
+        _check(lib().cs_kinship_multiply_zt(self._h, _p(Vf, _dp), Vf.shape[0], Vf.shape[1], _p(U, _dp), M))
+        return U[:, 0] if vec else np.ascontiguousarray(U)
+
+    def multiply_Z_v(self, U):
+        """Y = Z U over all SNPs (PlinkReader::multiply_Z_v, keep-list form): U M x k or M."""
+        U = np.asarray(U, dtype=np.float64)
+        vec = U.ndim == 1
+        Uf = np.asfortranarray(U[:, None] if vec else U)
+        Y = np.empty((self._rows, Uf.shape[1]), order="F")
+        _check(lib().cs_kinship_multiply_z(self._h, _p(Uf, _dp), Uf.shape[0], Uf.shape[1], _p(Y, _dp), self._rows))
+        return Y[:, 0] if vec else np.ascontiguousarray(Y)
 
     def apply_device_noalloc(self, dX_ptr, ldx, k, dY_ptr, ldy):
         """Y = K X on device buffers through the entry point that allocates nothing (fixed

@@ -77,6 +77,8 @@ struct GenoArgs {
     unsigned long long* mx;    // EPI_KV_GT: per-column max(|w|, |mu w|) as double bits
     long long* yint;           // EPI_KV_G: [row][k] int64, accumulated atomically over splits
     const long long* qext;     // ONLY_X: O^T B per [row][16 G] (exact, from the missing lists)
+    int out_u;                 // EPI_KV_GT: write u = (P - mu Q) s scale to W1 for every row with
+                               // inv_std != 0 (no multiplicity, W2 and maxima untouched)
 };
 
 // ONLY_X: only the genotype product X~^T B runs on MFMA.  The observed-indicator product O^T B
@@ -336,7 +338,12 @@ __device__ __forceinline__ void geno_body(const GenoArgs& a, void* lds_raw = nul
                         a.pq[(row * 2 + 1) * G16 + rhs] = Q;
                     }
                 } else if (EPI == EPI_KV_GT) {
-                    if (col_ok) {
+                    if (col_ok && a.out_u) {
+                        double u = 0.0;
+                        if (row < a.rows && a.inv_stds[row] != 0.0)
+                            u = ((double)P - a.means[row] * (double)Q) * a.inv_stds[row] * a.scale[rhs];
+                        a.W1[row * a.k + rhs] = u;
+                    } else if (col_ok) {
                         double w1 = 0.0, w2 = 0.0;
                         if (row < a.rows && a.mult[row] != 0 && a.inv_stds[row] != 0.0) {
                             const double s = a.inv_stds[row], mu = a.means[row];

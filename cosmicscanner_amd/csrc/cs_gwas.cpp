@@ -281,6 +281,32 @@ void fill_summary(cs_gwas_summary* s, const Inputs& in, int64_t nv, double S, bo
 
 }  // namespace
 
+// validate_variance_components (libCosmicCore/src/validation.h:351-366), run after every null
+// model (src/gwas_scanner.cpp:402-409): the first NaN / Inf or negative component fails with the
+// reference's report text (ValidationReport::summary, validation.h:103-116).
+extern "C" int cs_validate_variance_components(const double* v, int64_t n, const char* context) {
+    return csamd::guarded([&] {
+        if (n > 0 && !v) throw csamd::InvalidArgument("cs_validate_variance_components: null argument");
+        const char* code = nullptr;
+        const char* msg = nullptr;
+        for (int64_t i = 0; i < n && !code; ++i) {
+            if (!std::isfinite(v[i])) {
+                code = "VARIANCE_NONFINITE";
+                msg = "variance component is NaN or Inf";
+            } else if (v[i] < 0.0) {
+                code = "VARIANCE_NEGATIVE";
+                msg = "variance component is negative";
+            }
+        }
+        if (!code) return;
+        std::ostringstream os;
+        os << "Validation : FAIL\nChecks     : 0 passed, 0 warnings, 1 failures\n  [" << code << "] " << msg;
+        if (context && context[0]) os << " (" << context << ")";
+        os << "\n";
+        throw csamd::RuntimeError(os.str());
+    });
+}
+
 extern "C" int cs_gwas_dense_run(const char* bfile, const double* y, const double* cov, int64_t ldc,
                                  int64_t n_cov, const cs_gwas_options* opt, const char* out_prefix,
                                  cs_gwas_summary* summary) {
@@ -317,8 +343,8 @@ extern "C" int cs_gwas_dense_run(const char* bfile, const double* y, const doubl
             chk(cs_lmm_prepare_dense(lmm.h, gsub.d(), n, 1));
             double vg = 0, ve = 0;
             chk(cs_lmm_null_brent(lmm.h, &vg, &ve, nullptr));
-            if (!(vg >= 0.0) || !(ve >= 0.0) || !std::isfinite(vg) || !std::isfinite(ve))
-                throw RuntimeError("CosmicScanner null model: invalid variance components");
+            const double vc[2] = {vg, ve};
+            chk(cs_validate_variance_components(vc, 2, "CosmicScanner null model"));
             ms_null += ms_since(t1);
             model_labels.push_back(label.empty() ? "GLOBAL" : label);
             vg_all.push_back(vg);
@@ -420,6 +446,10 @@ extern "C" int cs_gwas_matrix_free_run(const char* bfile, const double* y, const
             chk(cs_lowrank_get(lr.h, nullptr, nullptr, nullptr, 0, nullptr, &pcg_sg, &pcg_se));
             ms_null += ms_since(t1);
         }
+        if (o.mode == 1) {
+            const double vc[2] = {pcg_sg, pcg_se};
+            chk(cs_validate_variance_components(vc, 2, "CosmicScanner null model"));
+        }
         auto run_chrom = [&](const std::string& chrom) {
             std::vector<int32_t> loco_vars;
             if (use_loco && !chrom.empty()) {
@@ -455,6 +485,8 @@ extern "C" int cs_gwas_matrix_free_run(const char* bfile, const double* y, const
                 double vg = 0, ve = 0;
                 chk(cs_lowrank_get(lr.h, nullptr, nullptr, nullptr, 0, nullptr, &vg, &ve));
                 ms_null += ms_since(t1);
+                const double vc[2] = {vg, ve};
+                chk(cs_validate_variance_components(vc, 2, "CosmicScanner null model"));
                 {
                     std::ofstream meta(out + ".low_rank.meta.tsv", std::ios::app);
                     meta << (chrom.empty() ? "GLOBAL" : chrom) << "\trandomized_subspace_iteration_tail_mean_woodbury"

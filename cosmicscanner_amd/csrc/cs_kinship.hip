@@ -757,13 +757,45 @@ static KvWork kv_work_at(unsigned char* base, const size_t off[10]) {
 // One int8 apply.  ws == NULL: the operator's own workspaces (grown on demand) and the
 // missing-genotype form of its mode; otherwise the dense form entirely in the caller's workspace,
 // with no allocation (both forms give bitwise identical results).
+// Modes of apply_int8: the full apply K X, Z^T V alone (PlinkReader::multiply_Zt_v), Z U alone
+// (multiply_Z_v).
+enum KvMode { KV_APPLY = 0, KV_ZT = 1, KV_Z = 2 };
+
+// U (M x k column-major, ldu) from the row-major [Mpad][k] epilogue output
+__global__ void k_rows_to_cols(const double* __restrict__ Wr, int64_t M, int64_t k, double* __restrict__ U,
+                               int64_t ldu) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < M * k; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e / k, c = e - j * k;
+        U[j + c * ldu] = Wr[e];
+    }
+}
+
+// Forward operands of Z U: W1[j][c] = s_j U[j][c], W2 = mu_j W1 (zero for inv_std == 0 and the
+// padding rows), and their per-column maxima.
+__global__ void k_w_from_u(const double* __restrict__ U, int64_t ldu, int64_t M, int64_t Mpad, int64_t k,
+                           const double* __restrict__ means, const double* __restrict__ inv_stds,
+                           double* __restrict__ W1, double* __restrict__ W2, unsigned long long* __restrict__ mx) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < Mpad * k; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = e / k, c = e - j * k;
+        double w = 0.0, w2 = 0.0;
+        if (j < M && inv_stds[j] != 0.0) {
+            w = inv_stds[j] * U[j + c * ldu];
+            w2 = means[j] * w;
+        }
+        W1[e] = w;
+        W2[e] = w2;
+        const double m = fmax(fabs(w), fabs(w2));
+        if (m > 0.0) atomicMax(&mx[c], (unsigned long long)__double_as_longlong(m));
+    }
+}
+
 static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
-                       int64_t ldy, const KvWork* ws = nullptr) {
+                       int64_t ldy, const KvWork* ws = nullptr, int mode = KV_APPLY) {
     cs_bed* b = op->bed;
     hipStream_t st = op->stream;
     const int L = prec_limbs(op->prec);
     const double qmax = limbs_qmax(L);
-    const bool sparse = ws ? false : use_sparse_missing(op, k, L);
+    const bool sparse = (ws || mode != KV_APPLY) ? false : use_sparse_missing(op, k, L);
     const Tiling tl = tiling(k, L, !sparse);   // the genotype-only kernels take unpacked operands
     const int64_t K16 = 16 * (int64_t)tl.G;
     const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
@@ -795,6 +827,15 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     }
 
     CS_HIP(hipEventRecord(op->ev[0], st));
+    if (mode == KV_Z) {
+        // U given (dX: M x k): the forward operands directly, then the second half
+        CS_HIP(hipMemsetAsync(w.mxW, 0, (size_t)k * 8, st));
+        k_w_from_u<<<grid_for(b->Mpad * k), 256, 0, st>>>(dX, ldx, b->M, b->Mpad, k, b->means.as<double>(),
+                                                         b->inv_stds.as<double>(), w.W1, w.W2, w.mxW);
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipEventRecord(op->ev[1], st));
+        CS_HIP(hipEventRecord(op->ev[2], st));
+    } else {
     k_scatter<<<grid_for(b->Npad * k), 256, 0, st>>>(dX, ldx, k, inv_off, inv_idx, b->N, b->Npad, w.Vt);
     CS_HIP(hipGetLastError());
     {
@@ -844,8 +885,17 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
     a.W2 = w.W2;
     a.mx = w.mxW;
     a.qext = sparse ? op->qext.as<long long>() : nullptr;
+    a.out_u = mode == KV_ZT ? 1 : 0;
     launch_geno(L, false, EPI_KV_GT, dim3((unsigned)(b->Mpad / 256), 1, (unsigned)tl.G), st, a, sparse, tl.packed);
     CS_HIP(hipEventRecord(op->ev[2], st));
+    if (mode == KV_ZT) {
+        // u over all SNPs into dY (M x k)
+        k_rows_to_cols<<<grid_for(b->M * k), 256, 0, st>>>(w.W1, b->M, k, dY, ldy);
+        CS_HIP(hipGetLastError());
+        for (int e = 3; e < 6; ++e) CS_HIP(hipEventRecord(op->ev[e], st));
+        return;
+    }
+    }
 
     // common per-column scale of W1 and W2, their limbs
     double* sclW = w.scl;  // the V scales are no longer needed
@@ -1183,6 +1233,69 @@ int cs_kinship_apply_device_noalloc(cs_kinship* op, const double* dX, int64_t ld
         apply_int8(op, dX, ldx, k, dY, ldy, &w);
         CS_HIP(hipStreamSynchronize(op->stream));
         record_timings(op);
+    });
+}
+
+// PlinkReader::multiply_Zt_v / multiply_Z_v (plink_reader.cpp:241-438) over the operator's rows
+// and all SNPs: the first or second half of the int8 apply.
+static void multiply_impl(cs_kinship* op, int mode, const double* dIn, int64_t ldi, int64_t k, double* dOut,
+                          int64_t ldo) {
+    if (op->prec == CS_PREC_FP32) throw InvalidArgument("multiply_Z: the FP32 comparator precision is not supported");
+    const int64_t rin = mode == KV_ZT ? op->n : op->bed->M, rout = mode == KV_ZT ? op->bed->M : op->n;
+    if (k < 0 || (k > 0 && (ldi < rin || ldo < rout)))
+        throw RuntimeError(mode == KV_ZT ? "Dimension mismatch in multiply_Zt_v" : "Dimension mismatch in multiply_Z_v");
+    if (k == 0) return;
+    ensure_stats(op->bed);
+    apply_int8(op, dIn, ldi, k, dOut, ldo, nullptr, mode);
+    CS_HIP(hipStreamSynchronize(op->stream));
+}
+
+static void multiply_host(cs_kinship* op, int mode, const double* In, int64_t ldi, int64_t k, double* Out,
+                          int64_t ldo) {
+    const int64_t rin = mode == KV_ZT ? op->n : op->bed->M, rout = mode == KV_ZT ? op->bed->M : op->n;
+    if (k < 0 || (k > 0 && (ldi < rin || ldo < rout)))
+        throw RuntimeError(mode == KV_ZT ? "Dimension mismatch in multiply_Zt_v" : "Dimension mismatch in multiply_Z_v");
+    if (k == 0) return;
+    if (!In || !Out) throw InvalidArgument("multiply_Z: null buffer");
+    op->dX.ensure((size_t)(rin * k) * sizeof(double));
+    op->dY.ensure((size_t)(rout * k) * sizeof(double));
+    CS_HIP(hipMemcpy2DAsync(op->dX.p, (size_t)rin * 8, In, (size_t)ldi * 8, (size_t)rin * 8, (size_t)k,
+                            hipMemcpyHostToDevice, op->stream));
+    multiply_impl(op, mode, op->dX.as<double>(), rin, k, op->dY.as<double>(), rout);
+    CS_HIP(hipMemcpy2DAsync(Out, (size_t)ldo * 8, op->dY.p, (size_t)rout * 8, (size_t)rout * 8, (size_t)k,
+                            hipMemcpyDeviceToHost, op->stream));
+    CS_HIP(hipStreamSynchronize(op->stream));
+}
+
+int cs_kinship_multiply_zt(cs_kinship* op, const double* V, int64_t ldv, int64_t k, double* U, int64_t ldu) {
+    return guarded([&] {
+        if (!op) throw InvalidArgument("cs_kinship_multiply_zt: null handle");
+        multiply_host(op, KV_ZT, V, ldv, k, U, ldu);
+    });
+}
+
+int cs_kinship_multiply_z(cs_kinship* op, const double* U, int64_t ldu, int64_t k, double* Y, int64_t ldy) {
+    return guarded([&] {
+        if (!op) throw InvalidArgument("cs_kinship_multiply_z: null handle");
+        multiply_host(op, KV_Z, U, ldu, k, Y, ldy);
+    });
+}
+
+int cs_kinship_multiply_zt_device(cs_kinship* op, const double* dV, int64_t ldv, int64_t k, double* dU,
+                                  int64_t ldu) {
+    return guarded([&] {
+        if (!op) throw InvalidArgument("cs_kinship_multiply_zt_device: null handle");
+        if (k > 0 && (!dV || !dU)) throw InvalidArgument("cs_kinship_multiply_zt_device: null buffer");
+        multiply_impl(op, KV_ZT, dV, ldv, k, dU, ldu);
+    });
+}
+
+int cs_kinship_multiply_z_device(cs_kinship* op, const double* dU, int64_t ldu, int64_t k, double* dY,
+                                 int64_t ldy) {
+    return guarded([&] {
+        if (!op) throw InvalidArgument("cs_kinship_multiply_z_device: null handle");
+        if (k > 0 && (!dU || !dY)) throw InvalidArgument("cs_kinship_multiply_z_device: null buffer");
+        multiply_impl(op, KV_Z, dU, ldu, k, dY, ldy);
     });
 }
 

@@ -116,6 +116,19 @@ int cs_kinship_apply(cs_kinship* op, const double* X, int64_t ldx, int64_t k, do
 /* Same with X, Y in device memory (no host transfer); for device-resident solvers. */
 int cs_kinship_apply_device(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
                             int64_t ldy);
+/* PlinkReader::multiply_Zt_v and multiply_Z_v (libCosmicCore/src/plink_reader.cpp:241-438, the
+ * keep-list overloads when the operator has a keep list): with z_ij = (x_ij - mean_j) inv_std_j
+ * over ALL SNPs of the genotypes (the operator's variant selection does not apply; missing and
+ * inv_std == 0 give 0), U = Z^T V (M x k from n x k) and Y = Z U (n x k from M x k), column-major
+ * with leading dimensions.  Same int8 products as the apply (exact genotypes, 23-bit operands at
+ * INT8_L3).  Host-buffer and device-buffer forms; wrong leading dimensions -> CS_ERR_RUNTIME
+ * "Dimension mismatch in multiply_Zt_v" / "... multiply_Z_v". */
+int cs_kinship_multiply_zt(cs_kinship* op, const double* V, int64_t ldv, int64_t k, double* U, int64_t ldu);
+int cs_kinship_multiply_z(cs_kinship* op, const double* U, int64_t ldu, int64_t k, double* Y, int64_t ldy);
+int cs_kinship_multiply_zt_device(cs_kinship* op, const double* dV, int64_t ldv, int64_t k, double* dU,
+                                  int64_t ldu);
+int cs_kinship_multiply_z_device(cs_kinship* op, const double* dU, int64_t ldu, int64_t k, double* dY,
+                                 int64_t ldy);
 /* No counterpart in the reference.  cs_kinship_apply_device for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc): it allocates nothing and keeps every intermediate (scattered and quantised
@@ -406,6 +419,12 @@ int cs_lmm_scan_rotated(cs_lmm* lmm, cs_bed* bed, const int32_t* keep, int64_t n
  * d_raw / d_raw_chr: N x N device; d_out: n_keep x n_keep device; keep NULL/0 = all samples. */
 int cs_grm_loco_subset(const double* d_raw, const double* d_raw_chr, int64_t N, const int32_t* keep,
                        int64_t n_keep, double S_global, double S_chr, double* d_out);
+
+/* validate_variance_components (libCosmicCore/src/validation.h:351-366), which both drivers run
+ * after every null model (src/gwas_scanner.cpp:402-409): CS_OK when every component is finite and
+ * non-negative, else CS_ERR_RUNTIME with the reference's report text ("[VARIANCE_NONFINITE]" or
+ * "[VARIANCE_NEGATIVE]", then the context in parentheses). */
+int cs_validate_variance_components(const double* v, int64_t n, const char* context);
 
 /* ---- GWAS driver: the default dense PLINK path of GwasScanner::runLocoScan
  * (src/gwas_scanner.cpp:74-525) end to end on the GPU.  Reads bfile.{bed,bim,fam}; y has one

@@ -203,6 +203,28 @@ def kinship_apply_blocked(bed, N, means, inv_stds, keep, variants, X, nthreads=N
     return np.ascontiguousarray(Y)
 
 
+def standardized_z(bed, N, M, keep=None):
+    """Z over the keep list (or all samples) and ALL SNPs as PlinkReader::multiply_Z_v /
+    multiply_Zt_v see it (plink_reader.cpp:241-438): z = (x - mean) inv_std with the all-sample
+    stats, missing -> 0, inv_std == 0 -> 0 (n x M, numpy)."""
+    means, inv = stats(bed, N, M)
+    D = decode_numpy(bed, N, M)
+    if keep is not None:
+        D = D[np.asarray(keep)]
+    Z = (D - means[None, :]) * inv[None, :]
+    return np.where(np.isnan(Z), 0.0, Z)
+
+
+def multiply_zt(bed, N, M, V, keep=None):
+    """multiply_Zt_v (plink_reader.cpp:351-438): U = Z^T V (M x k)."""
+    return standardized_z(bed, N, M, keep).T @ np.asarray(V, dtype=np.float64)
+
+
+def multiply_z(bed, N, M, U, keep=None):
+    """multiply_Z_v (plink_reader.cpp:241-349): Y = Z U (n x k)."""
+    return standardized_z(bed, N, M, keep) @ np.asarray(U, dtype=np.float64)
+
+
 def kinship_trace(bed, N, means, inv_stds, keep=None, variants=None):
     bed = _u8(bed)
     keep = _i32opt(keep)

@@ -49,3 +49,24 @@ def test_dense_eigen_size_guard_without_gpu_calls():
         csa.DenseLmm(y, X)
     with pytest.raises(ValueError, match="32766"):
         csa.MvLmm(np.zeros((n, 2)), X)
+
+
+def test_validate_variance_components():
+    # validation.h:351-366 through the C-ABI (no device work): finite non-negative components
+    # pass; the first NaN/Inf or negative one fails with the reference's report code and context
+    import cosmicscanner_amd as csa
+    L = csa.lib()
+
+    def run(vals):
+        a = np.asarray(vals, dtype=np.float64)
+        return L.cs_validate_variance_components(a.ctypes.data_as(csa._dp), a.size, b"CosmicScanner null model")
+
+    assert run([0.4, 0.6]) == csa.CS_OK and run([0.0, 0.0]) == csa.CS_OK
+    assert run([-1e-12, 0.6]) == csa.CS_ERR_RUNTIME
+    msg = L.cs_last_error().decode()
+    assert "Validation : FAIL" in msg and "[VARIANCE_NEGATIVE] variance component is negative" in msg
+    assert msg.rstrip().endswith("(CosmicScanner null model)")
+    for bad in ([np.nan, 0.5], [0.5, np.inf]):
+        assert run(bad) == csa.CS_ERR_RUNTIME
+        assert "[VARIANCE_NONFINITE] variance component is NaN or Inf" in L.cs_last_error().decode()
+    assert run([np.nan, -1.0]) == csa.CS_ERR_RUNTIME and "NONFINITE" in L.cs_last_error().decode()

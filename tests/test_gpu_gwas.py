@@ -157,6 +157,35 @@ def test_gwas_end_to_end_synthetic(csa, synth_gwas, tmp_path):
     meta = dict(ln.split("\t", 1) for ln in open(out + ".scan.meta.txt").read().splitlines())
     assert meta["LOCO"] == "true" and meta["N"] == str(sm["n_samples"])
     assert meta["COVARIATE_PROJECTION"] == "true" and meta["matrix_approximation"] == "full_eigen"
+    check_sumstats(out + ".cosmic.sumstats", rg, rows, [f"rs{j:06d}" for j in range(d["M"])], sm["n_samples"])
+
+
+def check_sumstats(path, assoc_rows, oracle_rows, bim_ids, n):
+    """.cosmic.sumstats (gwas_output.cpp:187-239): the v1.0 comment header, the canonical columns,
+    one row per .assoc.txt row in the same order with BETA / SE / P at 10 significant digits
+    (against the oracle: |d log10 P| <= 1e-4, |d beta| <= 1e-5 SE), AF and LOG10P at 6, MODEL LMM,
+    Z = BETA / SE, WEIGHT = 1 / SE^2 at 6 digits, INFO 1 and WARNING '.'."""
+    lines = open(path).read().splitlines()
+    assert lines[0] == "# CosmicScanner Summary Statistics v1.0"
+    assert lines[1].startswith("# Created: ") and lines[1].endswith(" UTC")
+    assert lines[2] == "# Model: LMM" and lines[3] == "#"
+    assert lines[4].split("\t") == ("CHR SNP POS A1 A2 BETA SE P N AF LOG10P MODEL Z WEIGHT INFO "
+                                    "WARNING").split()
+    body = [ln.split("\t") for ln in lines[5:]]
+    assert len(body) == len(assoc_rows) == len(oracle_rows)
+    for sr, ar, orow in zip(body, assoc_rows, oracle_rows):
+        j, freq, beta, se, p, l10 = orow
+        assert sr[:5] == ar[:5] and sr[1] == bim_ids[j]
+        b, s, pv = float(sr[5]), float(sr[6]), float(sr[7])
+        assert sr[8] == str(n) and sr[9] == f"{freq:.6g}" and sr[11] == "LMM"
+        assert abs(b - beta) <= 1e-5 * se and abs(s / se - 1) <= 1e-5
+        lp = -np.log10(pv) if pv > 0 else 300.0
+        assert abs(lp - l10) <= 1e-4 and abs(float(sr[10]) - l10) <= 1e-4 + 5e-6 * abs(l10)
+        # the .assoc.txt values are the same numbers at 6 significant digits
+        assert ar[7] == f"{b:.6g}" and ar[8] == f"{s:.6g}" and ar[9] == f"{pv:.6g}"
+        assert abs(float(sr[12]) - b / s) <= 1e-5 * abs(b / s) + 1e-12
+        assert abs(float(sr[13]) * s * s - 1.0) <= 1e-5
+        assert sr[14] == "1" and sr[15] == "."
 
 
 def test_gwas_options(csa, synth_gwas, tmp_path):

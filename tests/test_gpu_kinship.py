@@ -238,3 +238,35 @@ np.savez(sys.argv[1], **out)
         b = np.load(paths[other])
         for key in a.files:
             assert np.array_equal(a[key], b[key]), (other, key)
+
+
+def test_multiply_z_and_zt(csa):
+    """PlinkReader::multiply_Zt_v / multiply_Z_v (plink_reader.cpp:241-438) over all SNPs, with and
+    without a keep list, against the oracle's dense Z: the int8 products carry the 23-bit operand
+    quantisation (<= 2e-6 of each output column's maximum); wrong leading dimensions raise the
+    reference's dimension-mismatch errors."""
+    N, M = 613, 2900
+    body = oracle.gen_hwe_bed(404, N, M, 0.0, 0.5, 0.01)
+    reader = csa.PlinkReader.from_body(body, N, M)
+    rng = np.random.default_rng(8)
+    for keep in (None, np.sort(rng.choice(N, 300, replace=False)).astype(np.int32)):
+        op = csa.PlinkKinshipOperator(reader, keep, np.arange(0, M, 2, dtype=np.int32))   # selection ignored
+        n = N if keep is None else keep.size
+        for k in (1, 5, 37):
+            V = rng.standard_normal((n, k))
+            U = op.multiply_Zt_v(V)
+            ref = oracle.multiply_zt(body, N, M, V, keep)
+            assert np.max(np.abs(U - ref), axis=0).max() <= 2e-6 * np.max(np.abs(ref))
+            W = rng.standard_normal((M, k))
+            Y = op.multiply_Z_v(W)
+            ref2 = oracle.multiply_z(body, N, M, W, keep)
+            assert np.max(np.abs(Y - ref2)) <= 2e-6 * np.max(np.abs(ref2))
+        u1 = op.multiply_Zt_v(V[:, 0])
+        assert u1.shape == (M,)
+        L = csa.lib()
+        X = np.zeros((n, 1))
+        Yb = np.zeros((M, 1))
+        rc = L.cs_kinship_multiply_zt(op._h, X.ctypes.data_as(csa._dp), n - 1, 1, Yb.ctypes.data_as(csa._dp), M)
+        assert rc == csa.CS_ERR_RUNTIME and b"multiply_Zt_v" in L.cs_last_error()
+        op.close()
+    reader.close()

@@ -524,3 +524,41 @@ def test_mv_oracle_score_is_gradient_and_em_ascends():
         nl = oracle.mv_logl(D, Uty, Vg, Ve)
         assert nl >= ll - 1e-9 * abs(ll)
         ll = nl
+
+
+def test_multiply_z_zt_restatement():
+    # plink_reader.cpp:241-438 by a per-genotype loop over the packed bytes on a tiny case, and
+    # the identity Z (Z^T V) / M = K V against the kinship operator restatement (all SNPs)
+    N, M = 37, 23
+    bed = oracle.gen_hwe_bed(5, N, M, 0.0, 0.5, 0.05)
+    means, inv = oracle.stats(bed, N, M)
+    B = (N + 3) // 4
+    raw = np.asarray(bed, dtype=np.uint8)
+    lut = [2.0, None, 1.0, 0.0]
+    keep = np.array([4, 0, 36, 4, 17, 9], dtype=np.int32)
+    v = np.random.default_rng(1).standard_normal(keep.size)
+    u_loop = np.zeros(M)
+    for j in range(M):
+        if inv[j] == 0.0:
+            continue
+        acc = 0.0
+        for r, smp in enumerate(keep):
+            code = (raw[j * B + smp // 4] >> (2 * (smp % 4))) & 3
+            if code != 1:
+                acc += (lut[code] - means[j]) * v[r]
+        u_loop[j] = acc * inv[j]
+    assert np.max(np.abs(oracle.multiply_zt(bed, N, M, v[:, None], keep)[:, 0] - u_loop)) < 1e-12
+    w = np.random.default_rng(2).standard_normal(M)
+    y_loop = np.zeros(keep.size)
+    for j in range(M):
+        if inv[j] == 0.0:
+            continue
+        for r, smp in enumerate(keep):
+            code = (raw[j * B + smp // 4] >> (2 * (smp % 4))) & 3
+            if code != 1:
+                y_loop[r] += (lut[code] - means[j]) * w[j] * inv[j]
+    assert np.max(np.abs(oracle.multiply_z(bed, N, M, w[:, None], keep)[:, 0] - y_loop)) < 1e-12
+    X = np.random.default_rng(3).standard_normal((N, 3))
+    KX = oracle.kinship_apply(bed, N, means, inv, None, None, X)
+    ZZX = oracle.multiply_z(bed, N, M, oracle.multiply_zt(bed, N, M, X)) / M
+    assert rel(ZZX, KX) < 1e-12
