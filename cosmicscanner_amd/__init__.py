@@ -68,6 +68,7 @@ def _load():
         "cs_bed_destroy": ([_vp], ctypes.c_int),
         "cs_bed_dims": ([_vp, _i64p, _i64p], ctypes.c_int),
         "cs_bed_download": ([_vp, _u8p], ctypes.c_int),
+        "cs_bed_download_snps": ([_vp, _i32p, _i64, _u8p], ctypes.c_int),
         "cs_bed_build_sample_major": ([_vp], ctypes.c_int),
         "cs_bed_device_bytes": ([_vp, _i64p], ctypes.c_int),
         "cs_bed_compute_stats": ([_vp], ctypes.c_int),
@@ -325,6 +326,13 @@ class PlinkReader:
     def download(self):
         out = np.empty(self.num_snps * self.getBytesPerSnp(), dtype=np.uint8)
         _check(lib().cs_bed_download(self._h, _p(out, _u8p)))
+        return out
+
+    def download_snps(self, snps):
+        """Packed body of the listed SNPs (row i = SNP snps[i])."""
+        snps = np.ascontiguousarray(snps, dtype=np.int32)
+        out = np.empty(snps.size * self.getBytesPerSnp(), dtype=np.uint8)
+        _check(lib().cs_bed_download_snps(self._h, _p(snps, _i32p), snps.size, _p(out, _u8p)))
         return out
 
     def device_bytes(self):

@@ -61,6 +61,20 @@ __global__ void k_unlayout(const uint8_t* __restrict__ src, int64_t KB, int64_t 
     }
 }
 
+// The same for a list of SNPs: row i of raw is SNP snps[i].
+__global__ void k_unlayout_snps(const uint8_t* __restrict__ src, int64_t KB, const int32_t* __restrict__ snps,
+                                int64_t ns, int64_t Braw, int64_t N, uint8_t* __restrict__ raw) {
+    const int64_t total = ns * Braw;
+    const int tail = (int)(N & 3);
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / Braw, b = e - i * Braw;
+        uint8_t v = src[tiled_offset(snps[i], b, KB)];
+        if (tail && b == Braw - 1) v &= (uint8_t)((1u << (2 * tail)) - 1u);
+        raw[e] = v;
+    }
+}
+
 // ------------------------------------------------------------------------------ synthetic
 // Counter-based HWE generator: byte-identical to the oracle's CPU generator (same integer
 // hash, same IEEE operations; the library is compiled with -ffp-contract=off).
@@ -637,6 +651,22 @@ int cs_bed_download(cs_bed* bed, uint8_t* body) {
         CS_HIP(hipGetLastError());
         CS_HIP(hipMemcpyAsync(body, raw.p, (size_t)(bed->M * bed->Braw), hipMemcpyDeviceToHost,
                               bed->stream));
+        CS_HIP(hipStreamSynchronize(bed->stream));
+    });
+}
+
+int cs_bed_download_snps(cs_bed* bed, const int32_t* snps, int64_t n_snps, uint8_t* body) {
+    return guarded([&] {
+        if (!bed || (n_snps > 0 && (!snps || !body))) throw InvalidArgument("cs_bed_download_snps: null argument");
+        for (int64_t i = 0; i < n_snps; ++i)
+            if (snps[i] < 0 || snps[i] >= bed->M) throw RuntimeError("cs_bed_download_snps: variant index out of range");
+        if (n_snps == 0) return;
+        DevBuf dsn((size_t)n_snps * 4), raw((size_t)(n_snps * bed->Braw));
+        CS_HIP(hipMemcpyAsync(dsn.p, snps, (size_t)n_snps * 4, hipMemcpyHostToDevice, bed->stream));
+        k_unlayout_snps<<<grid_for(n_snps * bed->Braw), 256, 0, bed->stream>>>(
+            bed->snp.as<uint8_t>(), bed->KBs, dsn.as<int32_t>(), n_snps, bed->Braw, bed->N, raw.as<uint8_t>());
+        CS_HIP(hipGetLastError());
+        CS_HIP(hipMemcpyAsync(body, raw.p, (size_t)(n_snps * bed->Braw), hipMemcpyDeviceToHost, bed->stream));
         CS_HIP(hipStreamSynchronize(bed->stream));
     });
 }

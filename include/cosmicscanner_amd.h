@@ -74,6 +74,8 @@ int cs_bed_destroy(cs_bed* bed);
 int cs_bed_dims(const cs_bed* bed, int64_t* n_samples, int64_t* n_snps);
 /* Copy the packed body back (n_snps * ceil(n_samples/4) bytes; padding slots zeroed). */
 int cs_bed_download(cs_bed* bed, uint8_t* body);
+/* The same for the listed SNPs only: row i of body (ceil(n_samples/4) bytes) is SNP snps[i]. */
+int cs_bed_download_snps(cs_bed* bed, const int32_t* snps, int64_t n_snps, uint8_t* body);
 /* Build the resident sample-major copy (used by the K.V forward product and GRM); idempotent. */
 int cs_bed_build_sample_major(cs_bed* bed);
 /* Bytes of HBM held by the handle. */

@@ -89,6 +89,13 @@ def lib():
         L.orc_grm_exact_centred.argtypes = [_u8p, _i64, _i64, ctypes.c_float, _dp, _fp, _dp,
                                             _i64p, ctypes.c_int]
         L.orc_grm_exact_centred.restype = ctypes.c_int
+        L.orc_grm_exact_tile.argtypes = [_u8p, _i64, _i64, ctypes.c_float, _i32p, _i64, _i32p, _i64, _dp, _fp,
+                                         _dp, _i64p, ctypes.c_int]
+        L.orc_grm_exact_tile.restype = ctypes.c_int
+        L.orc_grm_exact_diag.argtypes = [_u8p, _i64, _i64, ctypes.c_float, _dp, _fp, ctypes.c_int]
+        L.orc_grm_exact_diag.restype = ctypes.c_int
+        L.orc_grm_std_tile.argtypes = [_u8p, _i64, _i64, ctypes.c_float, _i32p, _i64, _i32p, _i64, _dp, _i64p,
+                                       ctypes.c_int]
         L.orc_scan_sums.argtypes = [_u8p, _i64, _i64, _i32p, _i64, ctypes.c_double, _dp, _i64,
                                     _dp, _i64p, _i32p, _dp, _dp, ctypes.c_int]
         L.orc_chisq_prob.argtypes = [ctypes.c_double, ctypes.c_int]
@@ -284,6 +291,46 @@ def grm_exact_centred(bed, N, M, min_maf=0.01, nthreads=None):
     if rc != 0:
         raise ValueError("some SNP mean is not a multiple of 2^-29")
     return raw, nf, s.value, nv.value
+
+
+def grm_exact_tile(bed, N, M, rows, cols, min_maf=0.01, nthreads=None):
+    """Entries (rows x cols) of the exact centred GRM (alg 1) over all SNPs at any float mean:
+    (raw FP64, GCTA FP32, sum_2pq, n_valid), each correctly rounded from the exact integer."""
+    bed = _u8(bed)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    raw = np.empty((rows.size, cols.size))
+    nf = np.empty((rows.size, cols.size), dtype=np.float32)
+    s = ctypes.c_double()
+    nv = ctypes.c_int64()
+    sb = lib().orc_grm_exact_tile(_p(bed, _u8p), N, M, min_maf, _p(rows, _i32p), rows.size, _p(cols, _i32p),
+                                  cols.size, _p(raw, _dp), _p(nf, _fp), ctypes.byref(s), ctypes.byref(nv),
+                                  _nthreads(nthreads))
+    if sb < 0:
+        raise ValueError("exact GRM tile: scale beyond 40 bits or more than 2^22 SNPs")
+    return raw, nf, s.value, nv.value
+
+
+def grm_std_tile(bed, N, M, rows, cols, min_maf=0.01, nthreads=None):
+    """Entries (rows x cols) of the standardised GRM (alg 0) in FP64 over all SNPs."""
+    bed = _u8(bed)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    G = np.empty((rows.size, cols.size))
+    nv = ctypes.c_int64()
+    lib().orc_grm_std_tile(_p(bed, _u8p), N, M, min_maf, _p(rows, _i32p), rows.size, _p(cols, _i32p), cols.size,
+                           _p(G, _dp), ctypes.byref(nv), _nthreads(nthreads))
+    return G, nv.value
+
+
+def grm_exact_diag(bed, N, M, min_maf=0.01, nthreads=None):
+    """Diagonal of the exact centred GRM for every sample: (raw FP64, GCTA FP32)."""
+    bed = _u8(bed)
+    raw = np.empty(N)
+    nf = np.empty(N, dtype=np.float32)
+    if lib().orc_grm_exact_diag(_p(bed, _u8p), N, M, min_maf, _p(raw, _dp), _p(nf, _fp), _nthreads(nthreads)) < 0:
+        raise ValueError("exact GRM diagonal: scale beyond 40 bits or more than 2^22 SNPs")
+    return raw, nf
 
 
 def scan_sums(bed, N, M, keep, R, min_maf=0.01, nthreads=None):

@@ -454,8 +454,11 @@ static double i128_to_f64_rn(i128 v) {
     return neg ? -r : r;
 }
 
+/* Correctly rounded float of (v * 2^-sc) / S for a positive normal double S. */
+static float i128_div_f32_rn_sc(i128 v, double S, int sc);
 /* Correctly rounded float of (v * 2^-58) / S for a positive normal double S. */
-static float i128_div_f32_rn(i128 v, double S) {
+static float i128_div_f32_rn(i128 v, double S) { return i128_div_f32_rn_sc(v, S, 58); }
+static float i128_div_f32_rn_sc(i128 v, double S, int sc) {
     if (v == 0) return 0.0f;
     int neg = v < 0;
     unsigned __int128 a = neg ? -(unsigned __int128)v : (unsigned __int128)v;
@@ -479,7 +482,7 @@ static float i128_div_f32_rn(i128 v, double S) {
     unsigned __int128 half = ((unsigned __int128)1) << (drop - 1);
     if (rem > half || (rem == half && (sticky || (keepq & 1)))) keepq += 1;
     else if (rem == half && !sticky && !(keepq & 1)) { /* tie to even: keep */ }
-    int e2 = drop - sh - 58 - (se - 53);
+    int e2 = drop - sh - sc - (se - 53);
     float r = ldexpf((float)(uint64_t)keepq, e2);
     return neg ? -r : r;
 }
@@ -537,6 +540,175 @@ int orc_grm_exact_centred(const uint8_t* bed, int64_t N, int64_t M, float min_ma
     *n_valid_out = nv;
     free(U); free(idx); free(A); free(valid); free(mean_f); free(scale_f); free(pq2_f);
     return 0;
+}
+
+/* Exact centred GRM entries (alg 1 semantics, as orc_grm_exact_centred) at any float mean: with
+ * SB the smallest shift >= 29 that makes every valid mu_j 2^SB an integer, z_ij 2^SB =
+ * U_ij = o_ij (x_ij 2^SB - mu_j 2^SB) exactly, and I = sum_j U_ij U_kj in int128.  Returns SB
+ * (or -1 if SB would exceed 40 bits or M 2^22 SNPs, where int128 could overflow). */
+static int orc_grm_exact_prepare(const uint8_t* bed, int64_t N, int64_t M, float min_maf, int8_t* valid,
+                                 int64_t* A, double* sum_2pq, int64_t* nv, int nthreads) {
+    float* mean_f = (float*)malloc((size_t)M * sizeof(float));
+    float* scale_f = (float*)malloc((size_t)M * sizeof(float));
+    float* pq2_f = (float*)malloc((size_t)M * sizeof(float));
+    orc_grm_snp_params(bed, N, M, 1, min_maf, valid, mean_f, scale_f, pq2_f, sum_2pq, nv, nthreads);
+    int sb = 29;
+    for (int64_t j = 0; j < M; ++j) {
+        if (!valid[j]) continue;
+        while (sb <= 40) {
+            const double a = ldexp((double)mean_f[j], sb);
+            if (a == floor(a)) break;
+            sb++;
+        }
+    }
+    if (sb > 40 || M > ((int64_t)1 << 22)) sb = -1;
+    for (int64_t j = 0; j < M; ++j) A[j] = (sb > 0 && valid[j]) ? (int64_t)ldexp((double)mean_f[j], sb) : 0;
+    free(mean_f); free(scale_f); free(pq2_f);
+    return sb;
+}
+
+/* Tile of the exact centred GRM: rows x cols sample lists over all SNPs -> the correctly rounded
+ * FP64 raw value and the correctly rounded FP32 of raw / sum_2pq per entry (row-major nr x nc).
+ * Return: SB, or -1 (range, see orc_grm_exact_prepare). */
+int orc_grm_exact_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, const int32_t* rows, int64_t nr,
+                       const int32_t* cols, int64_t nc, double* raw_f64, float* norm_f32, double* sum_2pq_out,
+                       int64_t* n_valid_out, int nthreads) {
+    const int64_t Bs = bytes_per_snp(N);
+    int8_t* valid = (int8_t*)malloc((size_t)M);
+    int64_t* A = (int64_t*)malloc((size_t)M * sizeof(int64_t));
+    double sum_2pq;
+    int64_t nv;
+    const int sb = orc_grm_exact_prepare(bed, N, M, min_maf, valid, A, &sum_2pq, &nv, nthreads);
+    if (sb < 0) { free(valid); free(A); return -1; }
+    if (nthreads < 1) nthreads = 1;
+    i128* acc = (i128*)calloc((size_t)(nthreads * nr * nc), sizeof(i128));
+#pragma omp parallel num_threads(nthreads)
+    {
+        int th = 0, nth = 1;
+#ifdef _OPENMP
+        th = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        i128* my = acc + (int64_t)th * nr * nc;
+        int64_t* ur = (int64_t*)malloc((size_t)nr * sizeof(int64_t));
+        int64_t* uc = (int64_t*)malloc((size_t)nc * sizeof(int64_t));
+        for (int64_t j = th; j < M; j += nth) {
+            if (!valid[j]) continue;
+            const uint8_t* col = bed + j * Bs;
+            for (int64_t a = 0; a < nr; ++a) {
+                const int c = code_at(col, rows[a]);
+                ur[a] = kObs[c] ? ((int64_t)kDosI[c] << sb) - A[j] : 0;
+            }
+            for (int64_t b = 0; b < nc; ++b) {
+                const int c = code_at(col, cols[b]);
+                uc[b] = kObs[c] ? ((int64_t)kDosI[c] << sb) - A[j] : 0;
+            }
+            for (int64_t a = 0; a < nr; ++a) {
+                const i128 x = ur[a];
+                if (x == 0) continue;
+                i128* row = my + a * nc;
+                for (int64_t b = 0; b < nc; ++b) row[b] += x * (i128)uc[b];
+            }
+        }
+        free(ur);
+        free(uc);
+    }
+    for (int64_t e = 0; e < nr * nc; ++e) {
+        i128 v = 0;
+        for (int t = 0; t < nthreads; ++t) v += acc[(int64_t)t * nr * nc + e];
+        raw_f64[e] = ldexp(i128_to_f64_rn(v), -2 * sb);
+        norm_f32[e] = sum_2pq > 0 ? i128_div_f32_rn_sc(v, sum_2pq, 2 * sb) : (float)raw_f64[e];
+    }
+    *sum_2pq_out = sum_2pq;
+    *n_valid_out = nv;
+    free(acc); free(A); free(valid);
+    return sb;
+}
+
+/* Tile of the standardised GRM (alg 0 / 3, grm_builder.cpp:241-243, 300-304, 386-387) in FP64:
+ * z = (x - fl32 mean) fl32 scale over the valid SNPs, G = Z Z^T / n_valid, rows x cols. */
+void orc_grm_std_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, const int32_t* rows, int64_t nr,
+                      const int32_t* cols, int64_t nc, double* G, int64_t* n_valid_out, int nthreads) {
+    const int64_t Bs = bytes_per_snp(N);
+    int8_t* valid = (int8_t*)malloc((size_t)M);
+    float* mean_f = (float*)malloc((size_t)M * sizeof(float));
+    float* scale_f = (float*)malloc((size_t)M * sizeof(float));
+    float* pq2_f = (float*)malloc((size_t)M * sizeof(float));
+    double sum_2pq;
+    int64_t nv;
+    orc_grm_snp_params(bed, N, M, 0, min_maf, valid, mean_f, scale_f, pq2_f, &sum_2pq, &nv, nthreads);
+    if (nthreads < 1) nthreads = 1;
+    double* acc = (double*)calloc((size_t)(nthreads * nr * nc), sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+    {
+        int th = 0, nth = 1;
+#ifdef _OPENMP
+        th = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        double* my = acc + (int64_t)th * nr * nc;
+        double* zr = (double*)malloc((size_t)nr * sizeof(double));
+        double* zc = (double*)malloc((size_t)nc * sizeof(double));
+        for (int64_t j = th; j < M; j += nth) {
+            if (!valid[j]) continue;
+            const double mean = mean_f[j], scale = scale_f[j];
+            const double vm[4] = {(2.0 - mean) * scale, 0.0, (1.0 - mean) * scale, (0.0 - mean) * scale};
+            const uint8_t* col = bed + j * Bs;
+            for (int64_t a = 0; a < nr; ++a) zr[a] = vm[code_at(col, rows[a])];
+            for (int64_t b = 0; b < nc; ++b) zc[b] = vm[code_at(col, cols[b])];
+            for (int64_t a = 0; a < nr; ++a)
+                for (int64_t b = 0; b < nc; ++b) my[a * nc + b] += zr[a] * zc[b];
+        }
+        free(zr);
+        free(zc);
+    }
+    for (int64_t e = 0; e < nr * nc; ++e) {
+        double v = 0.0;
+        for (int t = 0; t < nthreads; ++t) v += acc[(int64_t)t * nr * nc + e];
+        G[e] = nv > 0 ? v / (double)nv : v;
+    }
+    *n_valid_out = nv;
+    free(acc); free(valid); free(mean_f); free(scale_f); free(pq2_f);
+}
+
+/* Diagonal of the exact centred GRM for every sample (as orc_grm_exact_tile with rows = cols). */
+int orc_grm_exact_diag(const uint8_t* bed, int64_t N, int64_t M, float min_maf, double* raw_f64, float* norm_f32,
+                       int nthreads) {
+    const int64_t Bs = bytes_per_snp(N);
+    int8_t* valid = (int8_t*)malloc((size_t)M);
+    int64_t* A = (int64_t*)malloc((size_t)M * sizeof(int64_t));
+    double sum_2pq;
+    int64_t nv;
+    const int sb = orc_grm_exact_prepare(bed, N, M, min_maf, valid, A, &sum_2pq, &nv, nthreads);
+    if (sb < 0) { free(valid); free(A); return -1; }
+    if (nthreads < 1) nthreads = 1;
+    i128* acc = (i128*)calloc((size_t)(nthreads * N), sizeof(i128));
+#pragma omp parallel num_threads(nthreads)
+    {
+        int th = 0, nth = 1;
+#ifdef _OPENMP
+        th = omp_get_thread_num();
+        nth = omp_get_num_threads();
+#endif
+        i128* my = acc + (int64_t)th * N;
+        for (int64_t j = th; j < M; j += nth) {
+            if (!valid[j]) continue;
+            const uint8_t* col = bed + j * Bs;
+            for (int64_t i = 0; i < N; ++i) {
+                const int c = code_at(col, i);
+                const int64_t u = kObs[c] ? ((int64_t)kDosI[c] << sb) - A[j] : 0;
+                my[i] += (i128)u * u;
+            }
+        }
+    }
+    for (int64_t i = 0; i < N; ++i) {
+        i128 v = 0;
+        for (int t = 0; t < nthreads; ++t) v += acc[(int64_t)t * N + i];
+        raw_f64[i] = ldexp(i128_to_f64_rn(v), -2 * sb);
+        norm_f32[i] = sum_2pq > 0 ? i128_div_f32_rn_sc(v, sum_2pq, 2 * sb) : (float)raw_f64[i];
+    }
+    free(acc); free(A); free(valid);
+    return sb;
 }
 
 /* ---------------------------------------------------------------------------------------

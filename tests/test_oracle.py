@@ -185,6 +185,60 @@ def test_grm_exact_matches_python_integers():
             assert nf[i, k] == f32
 
 
+def test_grm_exact_tile_and_diag_any_mean():
+    # rare SNPs (float means below 2^-6 that are not multiples of 2^-29): the tile and diagonal
+    # oracles against Python rationals with the float mean taken exactly; at the default MAF the
+    # tile equals the full exact matrix
+    N, M = 300, 40
+    bed = oracle.gen_hwe_bed(12, N, M, 0.0005, 0.02, 0.03)
+    valid, mean_f, _, _, _, _ = oracle.grm_snp_params(bed, N, M, 1, 0.0001)
+    assert np.any(valid & (mean_f < 2 ** -6) & (np.ldexp(mean_f.astype(np.float64), 29) % 1 != 0))
+    rows = np.array([3, 0, 299, 17, 3], dtype=np.int32)
+    cols = np.array([5, 3, 120], dtype=np.int32)
+    raw, nf, s2pq, nv = oracle.grm_exact_tile(bed, N, M, rows, cols, min_maf=0.0001)
+    draw, dnf = oracle.grm_exact_diag(bed, N, M, min_maf=0.0001)
+    D = oracle.decode_numpy(bed, N, M)
+    from fractions import Fraction
+    mu = [Fraction(float(m)) for m in mean_f]
+
+    def exact(i, k):
+        tot = Fraction(0)
+        for j in range(M):
+            if not valid[j] or np.isnan(D[i, j]) or np.isnan(D[k, j]):
+                continue
+            tot += (Fraction(int(D[i, j])) - mu[j]) * (Fraction(int(D[k, j])) - mu[j])
+        return tot
+
+    def check(r, f, i, k):
+        e = exact(i, k)
+        assert r == float(e)
+        q = e / Fraction(s2pq)
+        f32 = np.float32(float(q))
+        lo, hi = np.nextafter(f32, np.float32(-np.inf)), np.nextafter(f32, np.float32(np.inf))
+        d = abs(Fraction(float(f32)) - q)
+        assert d <= abs(Fraction(float(lo)) - q) and d <= abs(Fraction(float(hi)) - q)
+        assert f == f32
+
+    for a, i in enumerate(rows):
+        for b, k in enumerate(cols):
+            check(raw[a, b], nf[a, b], i, k)
+    for i in (0, 3, 150, 299):
+        check(draw[i], dnf[i], i, i)
+    full_raw, full_nf, s_full, _ = oracle.grm_exact_centred(bed, N, M, min_maf=0.02)
+    t_raw, t_nf, s_t, _ = oracle.grm_exact_tile(bed, N, M, rows, cols, min_maf=0.02)
+    assert s_t == s_full and np.array_equal(t_raw, full_raw[np.ix_(rows, cols)])
+    assert np.array_equal(t_nf, full_nf[np.ix_(rows, cols)])
+
+
+def test_grm_std_tile_matches_full(ref_datasets):
+    bed, N, M, _, _ = ref_datasets["gemma"]
+    G, _, _ = oracle.grm(bed, N, M, alg=0, min_maf=0.01)
+    rows = np.array([7, 0, N - 1], dtype=np.int32)
+    cols = np.array([N - 1, 3, 7, 9], dtype=np.int32)
+    T, nv = oracle.grm_std_tile(bed, N, M, rows, cols)
+    assert rel(T, G[np.ix_(rows, cols)]) < 1e-12
+
+
 def test_grm_exact_vs_fp64(ref_datasets):
     bed, N, M, _, _ = ref_datasets["gemma"]
     raw, nf, s2pq, nv = oracle.grm_exact_centred(bed, N, M, min_maf=0.01)
