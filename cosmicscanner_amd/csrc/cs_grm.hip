@@ -1,20 +1,21 @@
 // Kernel (b), GRM build: the dense centred GRM of GrmBuilder::compute, algorithm 1 (VanRaden,
-// libCosmicCore/src/grm_builder.cpp:63-399), computed exactly on int8 MFMA.
+// libCosmicCore/src/grm_builder.cpp:63-399), computed exactly in integers.
 //
 // The reference centres each valid SNP by its float32-rounded mean mu_j = fl32(2 p_j)
 // (grm_builder.cpp:240) and accumulates G += Z Z^T in FP64 (:345), then divides by the block-order
-// FP64 sum of float32 2pq (:266-272, :392).  With A_j = mu_j 2^29 an integer (true whenever
-// mu_j >= 2^-6, i.e. every SNP passing the default MAF filter), z_ij = o_ij (x_ij - mu_j) is
-// exactly U_ij 2^-29 with U_ij = o_ij (2^29 x~_ij - A_j), so G_raw = 2^-58 I with the exact
-// 128-bit integer
-//   I_ik = 2^58 T1 - 2^29 (R_i + R_k) + S + 2^29 (C_ik + C_ki) - Q_i - Q_k + D_ik
+// FP64 sum of float32 2pq (:266-272, :392).  With SB the smallest shift >= 29 that makes every
+// valid A_j = mu_j 2^SB an integer (29 whenever every mean is at least 2^-6; at most 40),
+// z_ij = o_ij (x_ij - mu_j) is exactly U_ij 2^-SB with U_ij = o_ij (2^SB x~_ij - A_j), so
+// G_raw = 2^-2SB I with the exact 128-bit integer
+//   I_ik = 2^2SB T1 - 2^SB (R_i + R_k) + S + 2^SB (C_ik + C_ki) - Q_i - Q_k + D_ik
 //   T1 = X~ X~^T,  R_i = sum_j A_j x~_ij,  S = sum_j A_j^2,  Q_i = sum_j A_j^2 m_ij,
 //   C_ik = sum_j x~_ij A_j m_kj,  D_ik = sum_j A_j^2 m_ij m_kj   (m = missing indicator).
-// T1 is one int8 MFMA product of packed genotypes with themselves; C and D (missing genotypes)
-// are int8 MFMA products with balanced signed 8-bit digits of A_j (4 digits) and A_j^2 (8
-// digits) folded into the missing-indicator operand; without missing genotypes they vanish.
-// The output is the correctly rounded FP64 of G_raw and the correctly rounded FP32 of
-// G_raw / sum_2pq (the GCTA .grm.bin value, grm_builder.cpp:1977-2026).
+// T1 is one int8 MFMA product of packed genotypes with themselves; C (missing genotypes against
+// genotypes) runs as int8 MFMA products with balanced signed 8-bit digits of A_j folded into the
+// missing-indicator operand, in both orders; D, nonzero only for pairs of samples missing at the
+// same SNP, is summed pair by pair from the per-SNP missing lists.  Without missing genotypes C
+// and D vanish.  The output is the correctly rounded FP64 of G_raw and the correctly rounded
+// FP32 of G_raw / sum_2pq (the GCTA .grm.bin value, grm_builder.cpp:1977-2026).
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -30,10 +31,9 @@ namespace csamd {
 typedef __int128 i128;
 typedef unsigned __int128 u128;
 
-enum { OP_X = 0, OP_M = 1, OP_MD = 2 };
-enum { GE_T1 = 0, GE_C = 1, GE_D = 2 };
+enum { OP_X = 0, OP_MD = 2 };
+enum { GE_T1 = 0, GE_C = 1 };
 
-constexpr uint32_t kLutM = 0x00000100u;    // code 01 -> 1
 constexpr uint32_t kLutMF = 0x0000FF00u;   // code 01 -> 0xFF
 constexpr int64_t kDigitChunkKB = 16384;   // 2^22 SNPs per digit-product launch
 
@@ -48,7 +48,6 @@ struct GrmArgs {
     int shift;             // 8 * limb
     int32_t* t1;           // panel accumulators, [P][Npad]
     long long* c;
-    i128* d;
 };
 
 template <int OP>
@@ -58,9 +57,6 @@ __device__ __forceinline__ v4i grm_operand(uint32_t w, uint4 dig) {
     if (OP == OP_X)
         return v4i{(int)__builtin_amdgcn_perm(0u, kLutX, d0), (int)__builtin_amdgcn_perm(0u, kLutX, d1),
                    (int)__builtin_amdgcn_perm(0u, kLutX, d2), (int)__builtin_amdgcn_perm(0u, kLutX, d3)};
-    if (OP == OP_M)
-        return v4i{(int)__builtin_amdgcn_perm(0u, kLutM, d0), (int)__builtin_amdgcn_perm(0u, kLutM, d1),
-                   (int)__builtin_amdgcn_perm(0u, kLutM, d2), (int)__builtin_amdgcn_perm(0u, kLutM, d3)};
     return v4i{(int)(__builtin_amdgcn_perm(0u, kLutMF, d0) & dig.x),
                (int)(__builtin_amdgcn_perm(0u, kLutMF, d1) & dig.y),
                (int)(__builtin_amdgcn_perm(0u, kLutMF, d2) & dig.z),
@@ -140,8 +136,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 if (i >= a.N || k >= a.N || k > i || i < a.I0 || i >= a.I0 + a.P) continue;
                 const int64_t idx = (i - a.I0) * a.Npad + k;
                 if (EPI == GE_T1) a.t1[idx] = acc[m][n][v];
-                else if (EPI == GE_C) a.c[idx] += (long long)acc[m][n][v] * (1LL << a.shift);
-                else a.d[idx] += (i128)acc[m][n][v] * ((i128)1 << a.shift);
+                else a.c[idx] += (long long)acc[m][n][v] * (1LL << a.shift);
             }
 }
 
@@ -210,6 +205,46 @@ __global__ __launch_bounds__(256) void k_grm_rowsums(const uint8_t* __restrict__
     }
 }
 
+// D_ik = sum_j A_j^2 m_ij m_kj for the panel rows i in [I0, I1), k <= i: one wave per SNP of
+// [j0, j1) walks the pairs of its missing list (samples in increasing order) and adds A_j^2, split
+// into 42-bit limbs, to the two unsigned accumulators with atomics (each limb sum stays below
+// 2^63 for up to 2^21 SNPs).
+constexpr int kDLimbBits = 42;
+__global__ __launch_bounds__(256) void k_grm_dpairs(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
+                                                    const long long* __restrict__ A, int64_t j0, int64_t j1,
+                                                    int64_t I0, int64_t I1, int64_t Npad,
+                                                    unsigned long long* __restrict__ dhi,
+                                                    unsigned long long* __restrict__ dlo) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t j = j0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); j < j1; j += (int64_t)gridDim.x * 4) {
+        const long long aj = A[j];
+        if (aj == 0) continue;
+        const int64_t o0 = off[j], o1 = off[j + 1];
+        if (o1 - o0 < 1) continue;
+        const u128 a2 = (u128)aj * (u128)aj;
+        const unsigned long long hi = (unsigned long long)(a2 >> kDLimbBits);
+        const unsigned long long lo = (unsigned long long)(a2 & ((((u128)1) << kDLimbBits) - 1));
+        // list positions of the panel rows: [p0, p1)
+        int64_t p0 = 0, p1 = 0;
+        for (int64_t q = o0; q < o1; q += 64) {
+            const int64_t e = q + lane;
+            const int32_t s = e < o1 ? idx[e] : 0x7fffffff;
+            p0 += __popcll(__ballot(s < I0));
+            p1 += __popcll(__ballot(s < I1));
+        }
+        for (int64_t a = o0 + p0; a < o0 + p1; ++a) {
+            const int64_t row = (int64_t)idx[a] - I0;
+            unsigned long long* rh = dhi + row * Npad;
+            unsigned long long* rl = dlo + row * Npad;
+            for (int64_t b = o0 + lane; b <= a; b += 64) {
+                const int32_t k = idx[b];
+                atomicAdd(rh + k, hi);
+                atomicAdd(rl + k, lo);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ int top_bit(u128 a) {
     const unsigned long long hi = (unsigned long long)(a >> 64), lo = (unsigned long long)a;
     return hi ? 127 - __builtin_clzll(hi) : 63 - __builtin_clzll(lo);
@@ -235,8 +270,8 @@ __device__ double i128_to_f64_rn(i128 v) {
     return neg ? -res : res;
 }
 
-// Correctly rounded float of (v 2^-58) / S with S = D 2^(se - 53), D a 53-bit integer.
-__device__ float i128_div_f32_rn(i128 v, unsigned long long D, int se) {
+// Correctly rounded float of (v 2^-sc) / S with S = D 2^(se - 53), D a 53-bit integer.
+__device__ float i128_div_f32_rn(i128 v, unsigned long long D, int se, int sc) {
     if (v == 0) return 0.0f;
     const bool neg = v < 0;
     const u128 a = neg ? (u128)0 - (u128)v : (u128)v;
@@ -250,16 +285,18 @@ __device__ float i128_div_f32_rn(i128 v, unsigned long long D, int se) {
     const u128 rem = Q & ((((u128)1) << drop) - 1);
     const u128 half = ((u128)1) << (drop - 1);
     if (rem > half || (rem == half && (sticky || (keepq & 1)))) keepq += 1;
-    const int e2 = drop - sh - 58 - (se - 53);
+    const int e2 = drop - sh - sc - (se - 53);
     const float res = ldexpf((float)(unsigned long long)keepq, e2);
     return neg ? -res : res;
 }
 
 struct CombineArgs {
     int64_t I0, P, N, Npad;
+    int sb;               // U = z 2^sb
     const int32_t* t1;
     const long long* c;
-    const i128* d;        // may be null (no missing genotypes)
+    const unsigned long long* dhi;   // D limbs (may be null: no missing genotypes)
+    const unsigned long long* dlo;
     const long long* R;
     const i128* Q;
     unsigned long long S_lo;
@@ -278,12 +315,14 @@ __global__ void k_grm_combine(CombineArgs a) {
         const int64_t li = e / a.Npad, k = e - li * a.Npad;
         const int64_t i = a.I0 + li;
         if (i >= a.N || k > i) continue;
-        i128 I = (i128)a.t1[e] * ((i128)1 << 58) - (i128)(a.R[i] + a.R[k]) * (1LL << 29) + S - a.Q[i] - a.Q[k];
-        if (a.c) I += (i128)a.c[e] * (1LL << 29);
-        if (a.d) I += a.d[e];
-        if (a.tri) a.tri[i * (i + 1) / 2 + k] = i128_div_f32_rn(I, a.D53, a.se);
+        const i128 one = 1;
+        i128 I = (i128)a.t1[e] * (one << (2 * a.sb)) - ((i128)a.R[i] + (i128)a.R[k]) * (one << a.sb) + S - a.Q[i] -
+                 a.Q[k];
+        if (a.c) I += (i128)a.c[e] * (one << a.sb);
+        if (a.dhi) I += (i128)a.dhi[e] * (one << kDLimbBits) + (i128)a.dlo[e];
+        if (a.tri) a.tri[i * (i + 1) / 2 + k] = i128_div_f32_rn(I, a.D53, a.se, 2 * a.sb);
         if (a.raw) {
-            const double g = ldexp(i128_to_f64_rn(I), -58);
+            const double g = ldexp(i128_to_f64_rn(I), -2 * a.sb);
             a.raw[i * a.N + k] = g;
             a.raw[k * a.N + i] = g;
         }
@@ -438,7 +477,8 @@ __global__ void k_grm_std_final(const double* __restrict__ acc, int64_t I0, int6
 // ------------------------------------------------------------------------------------------ host
 struct GrmParams {
     std::vector<uint8_t> valid;
-    std::vector<long long> A;      // mu_j 2^29 (0 for invalid SNPs), Mpad entries
+    std::vector<long long> A;      // mu_j 2^sb (0 for invalid SNPs), Mpad entries
+    int sb = 29;                   // smallest shift >= 29 with every valid mu_j 2^sb an integer
     double sum_2pq = 0.0;          // float 2pq summed in 2048-SNP block order (compute, :266-272)
     double sum_2pq_seq = 0.0;      // ... summed SNP by SNP (computeSplit's per-chromosome sums)
     int64_t n_valid = 0;
@@ -463,7 +503,7 @@ static GrmParams grm_params(cs_bed* b, float min_maf, const int32_t* snps, int64
             sel[(size_t)snps[i]] = 1;
         }
     }
-    std::vector<float> pq2((size_t)b->M, 0.0f);
+    std::vector<float> pq2((size_t)b->M, 0.0f), mean((size_t)b->M, 0.0f);
     for (int64_t j = 0; j < b->M; ++j) {
         if (snps && !sel[(size_t)j]) continue;
         const int64_t* c = &cnt[(size_t)j * 4];
@@ -475,13 +515,20 @@ static GrmParams grm_params(cs_bed* b, float min_maf, const int32_t* snps, int64
         if (maf < (double)min_maf || std::fabs(maf) < 1e-8) continue;
         pq2[(size_t)j] = (float)(1.0f * 2.0 * freq * (1.0 - freq));
         const float mean_f = (float)(freq * 2.0);
-        const double a = std::ldexp((double)mean_f, 29);
-        if (a != std::floor(a))
-            throw InvalidArgument("GRM: a SNP mean below 2^-6 is not a multiple of 2^-29; raise min_maf");
+        mean[(size_t)j] = mean_f;
+        while (p.sb <= 40 && std::ldexp((double)mean_f, p.sb) != std::floor(std::ldexp((double)mean_f, p.sb))) ++p.sb;
+        if (p.sb > 40) throw InvalidArgument("GRM: a SNP mean needs more than 40 fractional bits");
         p.valid[(size_t)j] = 1;
-        p.A[(size_t)j] = (long long)a;
         if (c[1] > 0) p.any_missing = true;
     }
+    // A_j at the common shift; R_i = sum A_j x~_ij stays below 2^62 in int64
+    long double asum = 0.0L;
+    for (int64_t j = 0; j < b->M; ++j)
+        if (p.valid[(size_t)j]) {
+            p.A[(size_t)j] = (long long)std::ldexp((double)mean[(size_t)j], p.sb);
+            asum += (long double)p.A[(size_t)j];
+        }
+    if (2.0L * asum >= std::ldexp(1.0L, 62)) throw InvalidArgument("GRM: too many SNPs for the exact row sums");
     double total = 0.0, seq = 0.0;
     int64_t lo = b->M, hi = -1;
     for (int64_t blk = 0; blk < b->M; blk += 2048) {
@@ -751,19 +798,49 @@ static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int6
         CS_HIP(hipGetLastError());
         i128 S = 0;
         for (long long a : prm.A) S += (i128)a * a;
-        const int nC = prm.any_missing ? 4 : 0, nD = prm.any_missing ? 8 : 0;
+        // balanced signed 8-bit digits of A_j for the C products (enough for the largest A_j), and
+        // the per-SNP missing lists for D
+        long long amax = 0;
+        for (long long a : prm.A) amax = std::max(amax, a);
+        int nC = 0;
+        if (prm.any_missing) {
+            // the fewest digits whose top digit stays in [-128, 127] for every A_j
+            for (nC = 1; nC < 16; ++nC) {
+                bool ok = true;
+                for (long long a : prm.A) {
+                    if (!a) continue;
+                    int8_t dd[16];
+                    digits128((i128)a, nC, dd);
+                    i128 v = 0;
+                    for (int l = nC - 1; l >= 0; --l) v = v * 256 + dd[l];
+                    if (v != (i128)a) {
+                        ok = false;
+                        break;
+                    }
+                }
+                if (ok) break;
+            }
+            ensure_missing_lists(b);
+            if (b->M > ((int64_t)1 << 21)) throw InvalidArgument("GRM: more than 2^21 SNPs with missing genotypes");
+            // C_ik in int64: 2 max A_j times the most missing genotypes of one sample
+            std::vector<int64_t> soff((size_t)N + 1);
+            CS_HIP(hipMemcpy(soff.data(), b->miss_smp_off.p, soff.size() * 8, hipMemcpyDeviceToHost));
+            int64_t mmax = 0;
+            for (int64_t i = 0; i < N; ++i) mmax = std::max(mmax, soff[(size_t)i + 1] - soff[(size_t)i]);
+            if ((long double)amax * 4.0L * (long double)mmax >= std::ldexp(1.0L, 62))
+                throw InvalidArgument("GRM: too many missing genotypes in one sample for the exact sums");
+        }
+        const bool useD = prm.any_missing;
         // digit arrays in the operand order [kb][t][g][16]: element k = 256 kb + 64 g + 16 t + 4 q + s
         // at byte 4 s + q.
-        std::vector<uint8_t> dig((size_t)(nC + nD) * Mpad, 0);
+        std::vector<uint8_t> dig((size_t)nC * Mpad, 0);
         for (int64_t j = 0; j < Mpad; ++j) {
             if (!prm.A[(size_t)j]) continue;
-            int8_t dd[8];
+            int8_t dd[16];
             const int64_t kb = j >> 8, rr = j & 255, g = rr >> 6, t = (rr >> 4) & 3, q = (rr >> 2) & 3, s = rr & 3;
             const int64_t off = (((kb * 4 + t) * 4 + g) << 4) + 4 * s + q;
-            digits128((i128)prm.A[(size_t)j], 4, dd);
+            digits128((i128)prm.A[(size_t)j], nC, dd);
             for (int l = 0; l < nC; ++l) dig[(size_t)l * Mpad + off] = (uint8_t)dd[l];
-            digits128((i128)prm.A[(size_t)j] * prm.A[(size_t)j], 8, dd);
-            for (int l = 0; l < nD; ++l) dig[(size_t)(nC + l) * Mpad + off] = (uint8_t)dd[l];
         }
         DevBuf ddig(std::max<size_t>(1, dig.size()));
         if (!dig.empty()) CS_HIP(hipMemcpyAsync(ddig.p, dig.data(), dig.size(), hipMemcpyHostToDevice, st));
@@ -788,11 +865,11 @@ static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int6
         // (CSAMD_GRM_PANEL_BYTES overrides the bound)
         size_t panel_bytes = 8ull << 30;
         if (const char* env = std::getenv("CSAMD_GRM_PANEL_BYTES")) panel_bytes = std::strtoull(env, nullptr, 10);
-        const size_t per_row = (size_t)Npad * (4 + (nC ? 8 : 0) + (nD ? 16 : 0));
+        const size_t per_row = (size_t)Npad * (4 + (nC ? 8 : 0) + (useD ? 16 : 0));
         int64_t P = std::max<int64_t>(128, (int64_t)(panel_bytes / per_row) / 128 * 128);
         P = std::min<int64_t>(P, round_up(Npad, 128));
         DevBuf t1((size_t)(P * Npad) * 4), cacc(nC ? (size_t)(P * Npad) * 8 : 1),
-            dacc(nD ? (size_t)(P * Npad) * 16 : 1);
+            dacc(useD ? (size_t)(P * Npad) * 16 : 1);
         float ms_pass = 0.f, ms_comb = 0.f;
         int passes = 0;
         double mfma_ops = 0;
@@ -805,7 +882,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int6
             CS_HIP(hipMemcpyAsync(dblocks.p, blocks.data(), blocks.size() * sizeof(int2),
                                   hipMemcpyHostToDevice, st));
             if (nC) CS_HIP(hipMemsetAsync(cacc.p, 0, (size_t)(rows * Npad) * 8, st));
-            if (nD) CS_HIP(hipMemsetAsync(dacc.p, 0, (size_t)(rows * Npad) * 16, st));
+            if (useD) CS_HIP(hipMemsetAsync(dacc.p, 0, (size_t)(rows * Npad) * 16, st));
             GrmArgs ga{};
             ga.smp = masked.as<uint8_t>();
             ga.KB = KB;
@@ -816,7 +893,6 @@ static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int6
             ga.Npad = Npad;
             ga.t1 = t1.as<int32_t>();
             ga.c = cacc.as<long long>();
-            ga.d = dacc.as<i128>();
             const unsigned nb = (unsigned)blocks.size();
             int64_t quadrants = 0;   // 64 x 64 wave tiles on or below the diagonal
             for (const int2& bl : blocks) quadrants += bl.x == bl.y ? 3 : 4;
@@ -839,13 +915,15 @@ static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int6
                     k_grm_pass<OP_MD, OP_X, GE_C><<<nb, 256, 0, st>>>(g2);
                     passes += 2;
                 }
-                for (int l = 0; l < nD; ++l) {
-                    GrmArgs g2 = gc;
-                    g2.shift = 8 * l;
-                    g2.digB = ddig.as<uint8_t>() + (size_t)(nC + l) * Mpad;
-                    k_grm_pass<OP_M, OP_MD, GE_D><<<nb, 256, 0, st>>>(g2);
-                    ++passes;
-                }
+            }
+            unsigned long long* dhi = dacc.as<unsigned long long>();
+            unsigned long long* dlo = dhi + rows * Npad;
+            if (useD) {
+                const int64_t j0 = prm.kb_lo * 256, j1 = std::min<int64_t>(b->M, prm.kb_hi * 256);
+                const unsigned gd = (unsigned)std::min<int64_t>(65536, (j1 - j0 + 3) / 4);
+                k_grm_dpairs<<<gd, 256, 0, st>>>(b->miss_snp_off.as<int64_t>(), b->miss_snp_idx.as<int32_t>(),
+                                                 dA.as<long long>(), j0, j1, I0, std::min<int64_t>(N, I0 + rows), Npad,
+                                                 dhi, dlo);
             }
             CS_HIP(hipGetLastError());
             CS_HIP(hipEventRecord(e1, st));
@@ -858,8 +936,10 @@ static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int6
             ca.N = N;
             ca.Npad = Npad;
             ca.t1 = t1.as<int32_t>();
+            ca.sb = prm.sb;
             ca.c = nC ? cacc.as<long long>() : nullptr;
-            ca.d = nD ? dacc.as<i128>() : nullptr;
+            ca.dhi = useD ? dhi : nullptr;
+            ca.dlo = useD ? dlo : nullptr;
             ca.R = dR.as<long long>();
             ca.Q = dQ.as<i128>();
             ca.S_lo = (unsigned long long)S;

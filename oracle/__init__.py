@@ -289,7 +289,7 @@ def grm_exact_centred(bed, N, M, min_maf=0.01, nthreads=None):
     rc = lib().orc_grm_exact_centred(_p(bed, _u8p), N, M, min_maf, _p(raw, _dp), _p(nf, _fp),
                                      ctypes.byref(s), ctypes.byref(nv), _nthreads(nthreads))
     if rc != 0:
-        raise ValueError("some SNP mean is not a multiple of 2^-29")
+        raise ValueError("exact GRM: scale beyond 40 bits or more than 2^22 SNPs")
     return raw, nf, s.value, nv.value
 
 

@@ -427,13 +427,13 @@ void orc_grm(const uint8_t* bed, int64_t N, int64_t M, int alg, float min_maf, d
 }
 
 /* ---------------------------------------------------------------------------------------
- * Exact centred GRM (alg 1 semantics).  With the float mean mu_j = A_j * 2^-29 (A_j an
- * integer whenever mu_j >= 2^-6 or mu_j*2^29 is integral), the reference's z_ij =
- * o_ij (x_ij - mu_j) is exactly U_ij * 2^-29 with the integer U_ij = o_ij (2^29 x_ij - A_j),
- * so G_raw = sum_j z z' = I * 2^-58 with the exact 128-bit integer I = sum_j U_ij U_kj.
- * Returns, per entry, the correctly rounded FP64 of G_raw and the correctly rounded FP32 of
- * G_raw / sum_2pq (the GCTA .grm.bin value, grm_builder.cpp:1977-2026, without the reference's
- * GEMM rounding).  Return value: 0 ok, -1 if some valid SNP mean is not a multiple of 2^-29. */
+ * Exact centred GRM (alg 1 semantics).  With the float mean mu_j = A_j * 2^-SB (A_j an integer
+ * for a shift SB >= 29 chosen per matrix, orc_grm_exact_prepare), the reference's z_ij =
+ * o_ij (x_ij - mu_j) is exactly U_ij * 2^-SB with the integer U_ij = o_ij (2^SB x_ij - A_j),
+ * so G_raw = sum_j z z' = I * 2^-2SB with the exact 128-bit integer I = sum_j U_ij U_kj.
+ * The functions below return, per entry, the correctly rounded FP64 of G_raw and the correctly
+ * rounded FP32 of G_raw / sum_2pq (the GCTA .grm.bin value, grm_builder.cpp:1977-2026, without
+ * the reference's GEMM rounding). */
 static double i128_to_f64_rn(i128 v) {
     if (v == 0) return 0.0;
     int neg = v < 0;
@@ -485,61 +485,6 @@ static float i128_div_f32_rn_sc(i128 v, double S, int sc) {
     int e2 = drop - sh - sc - (se - 53);
     float r = ldexpf((float)(uint64_t)keepq, e2);
     return neg ? -r : r;
-}
-
-int orc_grm_exact_centred(const uint8_t* bed, int64_t N, int64_t M, float min_maf,
-                          double* raw_f64, float* norm_f32, double* sum_2pq_out,
-                          int64_t* n_valid_out, int nthreads) {
-    const int64_t Bs = bytes_per_snp(N);
-    int8_t* valid = (int8_t*)malloc((size_t)M);
-    float* mean_f = (float*)malloc((size_t)M * sizeof(float));
-    float* scale_f = (float*)malloc((size_t)M * sizeof(float));
-    float* pq2_f = (float*)malloc((size_t)M * sizeof(float));
-    double sum_2pq;
-    int64_t nv;
-    orc_grm_snp_params(bed, N, M, 1, min_maf, valid, mean_f, scale_f, pq2_f, &sum_2pq, &nv,
-                       nthreads);
-    int64_t* A = (int64_t*)malloc((size_t)M * sizeof(int64_t));
-    int bad = 0;
-    for (int64_t j = 0; j < M; ++j) {
-        A[j] = 0;
-        if (!valid[j]) continue;
-        double a = ldexp((double)mean_f[j], 29);
-        if (a != floor(a)) { bad = 1; continue; }
-        A[j] = (int64_t)a;
-    }
-    if (bad) { free(valid); free(mean_f); free(scale_f); free(pq2_f); free(A); return -1; }
-    /* U as int64 per (sample, valid SNP), SNP-contiguous per sample. */
-    int64_t* idx = (int64_t*)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(int64_t));
-    int64_t t = 0;
-    for (int64_t j = 0; j < M; ++j) if (valid[j]) idx[t++] = j;
-    int64_t* U = (int64_t*)malloc((size_t)(N * (nv > 0 ? nv : 1)) * sizeof(int64_t));
-    for (int64_t q = 0; q < nv; ++q) {
-        const int64_t j = idx[q];
-        const uint8_t* col = bed + j * Bs;
-        for (int64_t i = 0; i < N; ++i) {
-            int c = code_at(col, i);
-            U[i * nv + q] = kObs[c] ? ((int64_t)kDosI[c] << 29) - A[j] : 0;
-        }
-    }
-#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 2)
-    for (int64_t i = 0; i < N; ++i) {
-        for (int64_t k = 0; k <= i; ++k) {
-            i128 acc = 0;
-            const int64_t* ui = U + i * nv;
-            const int64_t* uk = U + k * nv;
-            for (int64_t q = 0; q < nv; ++q) acc += (i128)ui[q] * (i128)uk[q];
-            double raw = i128_to_f64_rn(acc);
-            raw = ldexp(raw, -58);
-            float nf = sum_2pq > 0 ? i128_div_f32_rn(acc, sum_2pq) : (float)raw;
-            raw_f64[i * N + k] = raw; raw_f64[k * N + i] = raw;
-            norm_f32[i * N + k] = nf; norm_f32[k * N + i] = nf;
-        }
-    }
-    *sum_2pq_out = sum_2pq;
-    *n_valid_out = nv;
-    free(U); free(idx); free(A); free(valid); free(mean_f); free(scale_f); free(pq2_f);
-    return 0;
 }
 
 /* Exact centred GRM entries (alg 1 semantics, as orc_grm_exact_centred) at any float mean: with
@@ -709,6 +654,51 @@ int orc_grm_exact_diag(const uint8_t* bed, int64_t N, int64_t M, float min_maf, 
     }
     free(acc); free(A); free(valid);
     return sb;
+}
+
+/* The whole exact centred GRM (n x n, both triangles) at any float mean (orc_grm_exact_prepare):
+ * per entry the correctly rounded FP64 of G_raw and the correctly rounded FP32 of G_raw / sum_2pq.
+ * Return value: 0 ok, -1 out of range (orc_grm_exact_prepare). */
+int orc_grm_exact_centred(const uint8_t* bed, int64_t N, int64_t M, float min_maf,
+                          double* raw_f64, float* norm_f32, double* sum_2pq_out,
+                          int64_t* n_valid_out, int nthreads) {
+    const int64_t Bs = bytes_per_snp(N);
+    int8_t* valid = (int8_t*)malloc((size_t)M);
+    int64_t* A = (int64_t*)malloc((size_t)M * sizeof(int64_t));
+    double sum_2pq;
+    int64_t nv;
+    const int sb = orc_grm_exact_prepare(bed, N, M, min_maf, valid, A, &sum_2pq, &nv, nthreads);
+    if (sb < 0) { free(valid); free(A); return -1; }
+    /* U as int64 per (sample, valid SNP), SNP-contiguous per sample. */
+    int64_t* idx = (int64_t*)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(int64_t));
+    int64_t t = 0;
+    for (int64_t j = 0; j < M; ++j) if (valid[j]) idx[t++] = j;
+    int64_t* U = (int64_t*)malloc((size_t)(N * (nv > 0 ? nv : 1)) * sizeof(int64_t));
+    for (int64_t q = 0; q < nv; ++q) {
+        const int64_t j = idx[q];
+        const uint8_t* col = bed + j * Bs;
+        for (int64_t i = 0; i < N; ++i) {
+            int c = code_at(col, i);
+            U[i * nv + q] = kObs[c] ? ((int64_t)kDosI[c] << sb) - A[j] : 0;
+        }
+    }
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 2)
+    for (int64_t i = 0; i < N; ++i) {
+        for (int64_t k = 0; k <= i; ++k) {
+            i128 acc = 0;
+            const int64_t* ui = U + i * nv;
+            const int64_t* uk = U + k * nv;
+            for (int64_t q = 0; q < nv; ++q) acc += (i128)ui[q] * (i128)uk[q];
+            const double raw = ldexp(i128_to_f64_rn(acc), -2 * sb);
+            const float nf = sum_2pq > 0 ? i128_div_f32_rn_sc(acc, sum_2pq, 2 * sb) : (float)raw;
+            raw_f64[i * N + k] = raw; raw_f64[k * N + i] = raw;
+            norm_f32[i * N + k] = nf; norm_f32[k * N + i] = nf;
+        }
+    }
+    *sum_2pq_out = sum_2pq;
+    *n_valid_out = nv;
+    free(U); free(idx); free(A); free(valid);
+    return 0;
 }
 
 /* ---------------------------------------------------------------------------------------

@@ -90,6 +90,27 @@ def test_grm_min_maf_filter(csa):
         check_against_oracle(csa, r, body, N, M, min_maf=maf)
 
 
+@pytest.mark.parametrize("maf", [0.001, 0.0])
+def test_grm_low_maf(csa, maf):
+    # rare SNPs: float means below 2^-6 that are not multiples of 2^-29 need a larger common
+    # shift (more fractional bits, more digits); bit-exact all the same
+    N, M = 700, 2600
+    r = csa.PlinkReader.synthetic(404, N, M, 0.0002, 0.05, 0.01)
+    body = r.download()
+    valid, mean_f, _, _, _, _ = oracle.grm_snp_params(body, N, M, 1, maf)
+    assert np.any(valid & (np.ldexp(mean_f.astype(np.float64), 29) % 1 != 0))
+    check_against_oracle(csa, r, body, N, M, min_maf=maf)
+
+
+def test_grm_many_missing_pairs(csa, monkeypatch):
+    # 12 % missing: pairs of samples missing at the same SNP are common (the D sums), over several
+    # 128-row panels
+    N, M = 390, 1800
+    monkeypatch.setenv("CSAMD_GRM_PANEL_BYTES", "1")
+    r = csa.PlinkReader.synthetic(77, N, M, 0.01, 0.5, 0.12)
+    check_against_oracle(csa, r, r.download(), N, M)
+
+
 def test_grm_device_output(csa):
     N, M = 333, 1200
     r = csa.PlinkReader.synthetic(5, N, M, 0.01, 0.5, 0.005)

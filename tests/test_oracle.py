@@ -224,10 +224,13 @@ def test_grm_exact_tile_and_diag_any_mean():
             check(raw[a, b], nf[a, b], i, k)
     for i in (0, 3, 150, 299):
         check(draw[i], dnf[i], i, i)
-    full_raw, full_nf, s_full, _ = oracle.grm_exact_centred(bed, N, M, min_maf=0.02)
-    t_raw, t_nf, s_t, _ = oracle.grm_exact_tile(bed, N, M, rows, cols, min_maf=0.02)
-    assert s_t == s_full and np.array_equal(t_raw, full_raw[np.ix_(rows, cols)])
-    assert np.array_equal(t_nf, full_nf[np.ix_(rows, cols)])
+    for maf in (0.0001, 0.02):
+        full_raw, full_nf, s_full, _ = oracle.grm_exact_centred(bed, N, M, min_maf=maf)
+        t_raw, t_nf, s_t, _ = oracle.grm_exact_tile(bed, N, M, rows, cols, min_maf=maf)
+        assert s_t == s_full and np.array_equal(t_raw, full_raw[np.ix_(rows, cols)])
+        assert np.array_equal(t_nf, full_nf[np.ix_(rows, cols)])
+        d_raw, d_nf = oracle.grm_exact_diag(bed, N, M, min_maf=maf)
+        assert np.array_equal(d_raw, np.diag(full_raw)) and np.array_equal(d_nf, np.diag(full_nf))
 
 
 def test_grm_std_tile_matches_full(ref_datasets):
