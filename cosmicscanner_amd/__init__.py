@@ -100,11 +100,12 @@ def _load():
                                 ctypes.c_int, ctypes.c_void_p, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
                                ctypes.c_int),
         "cs_scan_last_timing": ([_dp, _dp, _dp, _i64p], ctypes.c_int),
-        "cs_grm_build": ([_vp, ctypes.c_int, ctypes.c_float, _fp, _dp, _dp, _i64p], ctypes.c_int),
-        "cs_grm_build_device": ([_vp, ctypes.c_int, ctypes.c_float, _vp, _dp, _i64p], ctypes.c_int),
+        "cs_grm_build": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, _fp, _dp, _dp, _i64p], ctypes.c_int),
+        "cs_grm_build_device": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, _vp, _dp, _i64p],
+                                ctypes.c_int),
         "cs_grm_last_timing": ([_dp, _dp, ctypes.POINTER(ctypes.c_int), _dp], ctypes.c_int),
-        "cs_grm_build_raw": ([_vp, ctypes.c_int, ctypes.c_float, _i32p, _i64, _vp, ctypes.c_int, _dp,
-                              _dp, _i64p], ctypes.c_int),
+        "cs_grm_build_raw": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, _i32p, _i64, _vp, ctypes.c_int,
+                              _dp, _dp, _i64p], ctypes.c_int),
         "cs_grm_loco_subset": ([_vp, _vp, _i64, _i32p, _i64, ctypes.c_double, ctypes.c_double, _vp],
                                ctypes.c_int),
         "cs_lmm_create": ([_dp, _f32p, _i64, _i64, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
@@ -570,8 +571,9 @@ class GrmBuilder:
     flushed to FP64 every 16384 SNPs."""
 
     class Options:
-        def __init__(self, min_maf=0.01, algorithm=1, thread_num=1, block_size=2048):
+        def __init__(self, min_maf=0.01, algorithm=1, thread_num=1, block_size=2048, min_hwe=0.0):
             self.min_maf = min_maf
+            self.min_hwe = min_hwe
             self.algorithm = algorithm
             self.thread_num = thread_num
             self.block_size = block_size
@@ -598,7 +600,8 @@ class GrmBuilder:
         s = ctypes.c_double()
         nv = ctypes.c_int64()
         _check(lib().cs_grm_build(reader._h, int(self.options.algorithm), float(self.options.min_maf),
-                                  _p(tri, _fp), _p(raw, _dp), ctypes.byref(s), ctypes.byref(nv)))
+                                  float(self.options.min_hwe), _p(tri, _fp), _p(raw, _dp), ctypes.byref(s),
+                                  ctypes.byref(nv)))
         self.tri, self.raw = tri, raw
         self.total_sum_2pq, self.total_valid_snps = s.value, nv.value
         self._fam = reader.fam
@@ -644,12 +647,12 @@ class GrmBuilder:
             out_prefix + ".grm.N.bin")
 
 
-def grm_build_device(reader, d_tri_ptr, min_maf=0.01, alg=1):
+def grm_build_device(reader, d_tri_ptr, min_maf=0.01, alg=1, min_hwe=0.0):
     """GRM lower triangle into a device buffer of n(n+1)/2 floats; returns (sum_2pq, n_valid)."""
     s = ctypes.c_double()
     nv = ctypes.c_int64()
-    _check(lib().cs_grm_build_device(reader._h, int(alg), float(min_maf), d_tri_ptr, ctypes.byref(s),
-                                     ctypes.byref(nv)))
+    _check(lib().cs_grm_build_device(reader._h, int(alg), float(min_maf), float(min_hwe), d_tri_ptr,
+                                     ctypes.byref(s), ctypes.byref(nv)))
     return s.value, nv.value
 
 
@@ -726,11 +729,11 @@ class DenseLmm:
         return out
 
 
-def grm_build_raw(reader, d_raw, min_maf=0.01, snps=None, alg=1):
+def grm_build_raw(reader, d_raw, min_maf=0.01, snps=None, alg=1, min_hwe=0.0):
     """Raw Z Z^T over a SNP subset into an N x N DeviceArray; returns (S_block, S_seq, n_valid)."""
     snps = _opt_i32(snps)
     a, b, nv = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
-    _check(lib().cs_grm_build_raw(reader._h, int(alg), float(min_maf), _p(snps, _i32p),
+    _check(lib().cs_grm_build_raw(reader._h, int(alg), float(min_maf), float(min_hwe), _p(snps, _i32p),
                                   0 if snps is None else snps.size, d_raw.ptr, 1, ctypes.byref(a),
                                   ctypes.byref(b), ctypes.byref(nv)))
     return a.value, b.value, nv.value

@@ -22,8 +22,11 @@
 #include <cstring>
 #include <vector>
 
+#include <thread>
+
 #include "cs_common.h"
 #include "cs_geno_mfma.h"
+#include "cs_hwe.h"
 #include "cs_ops.h"
 
 namespace csamd {
@@ -486,13 +489,33 @@ struct GrmParams {
     bool any_missing = false;
 };
 
+// The optional HWE filter of GrmBuilder::compute (grm_builder.cpp:220-223): keep[j] = 0 when the
+// exact HWE p-value of SNP j's all-sample genotype counts (code 00 = hom. A1, 10 = het., 11 =
+// hom. A2) is below min_hwe; all ones when min_hwe <= 0.  Threads over the SNPs.
+static std::vector<uint8_t> hwe_keep(const std::vector<int64_t>& cnt, int64_t M, float min_hwe) {
+    std::vector<uint8_t> keep((size_t)M, 1);
+    if (!(min_hwe > 0.0f)) return keep;
+    const int nt = (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            for (int64_t j = t; j < M; j += nt) {
+                const int64_t* c = &cnt[(size_t)j * 4];
+                if (hwe_exact_p(c[0], c[2], c[3]) < (double)min_hwe) keep[(size_t)j] = 0;
+            }
+        });
+    for (auto& x : th) x.join();
+    return keep;
+}
+
 // Per-SNP filter and float32 parameters of GrmBuilder::compute (grm_builder.cpp:167-272) from
 // the all-sample class counts; `snps` (optional) restricts the SNP set, as a chromosome does in
 // GrmBuilder::computeSplit (grm_builder.cpp:506-600, 684-713).
-static GrmParams grm_params(cs_bed* b, float min_maf, const int32_t* snps, int64_t n_snps) {
+static GrmParams grm_params(cs_bed* b, float min_maf, float min_hwe, const int32_t* snps, int64_t n_snps) {
     GrmParams p;
     std::vector<int64_t> cnt((size_t)b->M * 4);
     CS_HIP(hipMemcpy(cnt.data(), b->counts.p, cnt.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    const std::vector<uint8_t> hwe_ok = hwe_keep(cnt, b->M, min_hwe);
     p.valid.assign((size_t)b->Mpad, 0);
     p.A.assign((size_t)b->Mpad, 0);
     std::vector<uint8_t> sel;
@@ -513,6 +536,7 @@ static GrmParams grm_params(cs_bed* b, float min_maf, const int32_t* snps, int64
         const double freq = (double)sum_x / (2.0 * (double)n_obs);
         const double maf = std::min(freq, 1.0 - freq);
         if (maf < (double)min_maf || std::fabs(maf) < 1e-8) continue;
+        if (!hwe_ok[(size_t)j]) continue;
         pq2[(size_t)j] = (float)(1.0f * 2.0 * freq * (1.0 - freq));
         const float mean_f = (float)(freq * 2.0);
         mean[(size_t)j] = mean_f;
@@ -586,10 +610,11 @@ struct GrmOut {
 
 // Per-SNP FP16 hi/lo lookup tables of the standardised GRM (grm_builder.cpp:167-257, 283-305):
 // filter and float32 mean/scale as GrmBuilder::compute computes them for algorithms 0 and 3.
-static std::vector<uint4> grm_std_tables(cs_bed* b, float min_maf, const int32_t* snps, int64_t n_snps,
-                                         GrmOut& out, int64_t& s_lo, int64_t& s_hi) {
+static std::vector<uint4> grm_std_tables(cs_bed* b, float min_maf, float min_hwe, const int32_t* snps,
+                                         int64_t n_snps, GrmOut& out, int64_t& s_lo, int64_t& s_hi) {
     std::vector<int64_t> cnt((size_t)b->M * 4);
     CS_HIP(hipMemcpy(cnt.data(), b->counts.p, cnt.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    const std::vector<uint8_t> hwe_ok = hwe_keep(cnt, b->M, min_hwe);
     std::vector<uint8_t> sel;
     if (snps) {
         sel.assign((size_t)b->M, 0);
@@ -620,6 +645,7 @@ static std::vector<uint4> grm_std_tables(cs_bed* b, float min_maf, const int32_t
         const double freq = (double)sum_x / (2.0 * (double)n_obs);
         const double maf = std::min(freq, 1.0 - freq);
         if (maf < (double)min_maf || std::fabs(maf) < 1e-8) continue;
+        if (!hwe_ok[(size_t)j]) continue;
         const float pq2f = (float)(1.0f * 2.0 * freq * (1.0 - freq));
         block += pq2f;
         seq += pq2f;
@@ -649,11 +675,11 @@ static std::vector<uint4> grm_std_tables(cs_bed* b, float min_maf, const int32_t
     return tab;
 }
 
-static void grm_std_run(cs_bed* b, float min_maf, const int32_t* snps, int64_t n_snps, GrmOut& out) {
+static void grm_std_run(cs_bed* b, float min_maf, float min_hwe, const int32_t* snps, int64_t n_snps, GrmOut& out) {
     ensure_stats(b);
     ensure_sample_major(b);
     int64_t s_lo = 0, s_hi = 0;
-    std::vector<uint4> tab = grm_std_tables(b, min_maf, snps, n_snps, out, s_lo, s_hi);
+    std::vector<uint4> tab = grm_std_tables(b, min_maf, min_hwe, snps, n_snps, out, s_lo, s_hi);
     const int64_t N = b->N, Npad = b->Npad, KB = b->KBm;
     hipStream_t st;
     CS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -752,16 +778,18 @@ static void grm_std_run(cs_bed* b, float min_maf, const int32_t* snps, int64_t n
     CS_HIP(hipStreamDestroy(st));
 }
 
-static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int64_t n_snps, GrmOut& out) {
+static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int32_t* snps, int64_t n_snps,
+                    GrmOut& out) {
+    if (!(min_hwe >= 0.0f && min_hwe <= 1.0f)) throw InvalidArgument("GRM: min_hwe must be in [0, 1]");
     if (alg == 0 || alg == 3) {
-        grm_std_run(b, min_maf, snps, n_snps, out);
+        grm_std_run(b, min_maf, min_hwe, snps, n_snps, out);
         return;
     }
     if (alg != 1)
         throw InvalidArgument("GRM: algorithms 0/3 (standardised, Yang) and 1 (VanRaden) are built here");
     ensure_stats(b);
     ensure_sample_major(b);
-    GrmParams prm = grm_params(b, min_maf, snps, n_snps);
+    GrmParams prm = grm_params(b, min_maf, min_hwe, snps, n_snps);
     out.sum_2pq = prm.sum_2pq;
     out.sum_2pq_seq = prm.sum_2pq_seq;
     out.n_valid = prm.n_valid;
@@ -985,39 +1013,39 @@ static void grm_run(cs_bed* b, int alg, float min_maf, const int32_t* snps, int6
 
 extern "C" {
 
-int cs_grm_build(cs_bed* bed, int alg, float min_maf, float* grm_tri_f32, double* grm_raw_f64,
+int cs_grm_build(cs_bed* bed, int alg, float min_maf, float min_hwe, float* grm_tri_f32, double* grm_raw_f64,
                  double* sum_2pq, int64_t* n_valid) {
     return guarded([&] {
         if (!bed) throw InvalidArgument("cs_grm_build: null handle");
         GrmOut o;
         o.tri_host = grm_tri_f32;
         o.raw_host = grm_raw_f64;
-        grm_run(bed, alg, min_maf, nullptr, 0, o);
+        grm_run(bed, alg, min_maf, min_hwe, nullptr, 0, o);
         if (sum_2pq) *sum_2pq = o.sum_2pq;
         if (n_valid) *n_valid = o.n_valid;
     });
 }
 
-int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float* d_grm_tri_f32, double* sum_2pq,
-                        int64_t* n_valid) {
+int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float min_hwe, float* d_grm_tri_f32,
+                        double* sum_2pq, int64_t* n_valid) {
     return guarded([&] {
         if (!bed || !d_grm_tri_f32) throw InvalidArgument("cs_grm_build_device: null argument");
         GrmOut o;
         o.tri_dev = d_grm_tri_f32;
-        grm_run(bed, alg, min_maf, nullptr, 0, o);
+        grm_run(bed, alg, min_maf, min_hwe, nullptr, 0, o);
         if (sum_2pq) *sum_2pq = o.sum_2pq;
         if (n_valid) *n_valid = o.n_valid;
     });
 }
 
-int cs_grm_build_raw(cs_bed* bed, int alg, float min_maf, const int32_t* snps, int64_t n_snps,
+int cs_grm_build_raw(cs_bed* bed, int alg, float min_maf, float min_hwe, const int32_t* snps, int64_t n_snps,
                      double* raw_f64, int raw_on_device, double* sum_2pq_block,
                      double* sum_2pq_seq, int64_t* n_valid) {
     return guarded([&] {
         if (!bed || !raw_f64) throw InvalidArgument("cs_grm_build_raw: null argument");
         GrmOut o;
         (raw_on_device ? o.raw_dev : o.raw_host) = raw_f64;
-        grm_run(bed, alg, min_maf, (snps && n_snps > 0) ? snps : nullptr, n_snps, o);
+        grm_run(bed, alg, min_maf, min_hwe, (snps && n_snps > 0) ? snps : nullptr, n_snps, o);
         if (sum_2pq_block) *sum_2pq_block = o.sum_2pq;
         if (sum_2pq_seq) *sum_2pq_seq = o.sum_2pq_seq;
         if (n_valid) *n_valid = o.n_valid;

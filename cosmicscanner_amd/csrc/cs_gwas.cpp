@@ -329,7 +329,7 @@ extern "C" int cs_gwas_dense_run(const char* bfile, const double* y, const doubl
         double S_global = 0.0;
         int64_t nv = 0;
         const bool std_alg = o.grm_alg == 0 || o.grm_alg == 3;
-        chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, nullptr, 0, raw.d(), 1, &S_global, nullptr, &nv));
+        chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, 0.0f, nullptr, 0, raw.d(), 1, &S_global, nullptr, &nv));
         if (std_alg) S_global = (double)nv;
         ms_grm += ms_since(t0);
         LmmHandle lmm;
@@ -375,7 +375,7 @@ extern "C" int cs_gwas_dense_run(const char* bfile, const double* y, const doubl
                 auto t1 = std::chrono::steady_clock::now();
                 double S_chr = 0.0;
                 int64_t nv_chr = 0;
-                chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, snps.data(), (int64_t)snps.size(),
+                chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, 0.0f, snps.data(), (int64_t)snps.size(),
                                      raw_chr.d(), 1, nullptr, &S_chr, &nv_chr));
                 if (std_alg) S_chr = (double)nv_chr;
                 chk(cs_grm_loco_subset(raw.d(), raw_chr.d(), N, in.keep.data(), n, S_global, S_chr, gsub.d()));
@@ -617,7 +617,7 @@ extern "C" int cs_gwas_mv_run(const char* bfile, const double* Y, int64_t ldy, i
         DevMem raw(N * N * 8), gsub(n * n * 8);
         double S_global = 0.0;
         int64_t nv = 0;
-        chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, nullptr, 0, raw.d(), 1, &S_global, nullptr, &nv));
+        chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, 0.0f, nullptr, 0, raw.d(), 1, &S_global, nullptr, &nv));
         const double norm_global = std_alg ? (double)nv : S_global;
         ms_grm += ms_since(t0);
         write_mv_header(path, d);
@@ -673,7 +673,7 @@ extern "C" int cs_gwas_mv_run(const char* bfile, const double* Y, int64_t ldy, i
                 auto t1 = std::chrono::steady_clock::now();
                 double S_chr = 0.0;
                 int64_t nv_chr = 0;
-                chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, snps.data(), (int64_t)snps.size(),
+                chk(cs_grm_build_raw(in.bed.h, o.grm_alg, (float)o.min_maf, 0.0f, snps.data(), (int64_t)snps.size(),
                                      raw_chr.d(), 1, &S_chr, nullptr, &nv_chr));
                 const double norm_chr = std_alg ? (double)nv_chr : S_chr;
                 chk(cs_grm_loco_subset_scaled(raw.d(), raw_chr.d(), N, in.keep.data(), n, norm_global, norm_chr,

@@ -204,7 +204,9 @@ int cs_scan_last_timing(double* ms_counts, double* ms_products, double* ms_epilo
 /* ---- Kernel (b): dense GRM build.
  * GrmBuilder::compute(const PlinkReader&) (libCosmicCore/src/grm_builder.cpp:63-399) for
  * algorithm 1 (VanRaden, the default, --make-grm-alg 1): per-SNP filter on all samples
- * (min(f,1-f) >= min_maf and > 1e-8, :167-257), float32 mean 2p and float32 2pq, centred
+ * (min(f,1-f) >= min_maf and > 1e-8, :167-257; with min_hwe > 0 also the exact HWE p-value of
+ * the genotype counts >= min_hwe, :220-223, calculateHWE of qc_utils.cpp:9-77), float32 mean 2p
+ * and float32 2pq, centred
  * z = x - mean (missing -> 0), G = Z Z^T / sum_2pq with sum_2pq the FP64 sum of the float32 2pq in
  * 2048-SNP block order (:266-272, :392).  Computed exactly in integers on int8 MFMA:
  *   grm_tri_f32: n(n+1)/2 floats, lower triangle row-major (i, j <= i) -- the GCTA .grm.bin
@@ -213,17 +215,17 @@ int cs_scan_last_timing(double* ms_counts, double* ms_products, double* ms_epilo
  *   grm_raw_f64: n x n doubles, the correctly rounded FP64 of Z Z^T (before normalisation,
  *                symmetric).  May be NULL.
  *   sum_2pq / n_valid: the normaliser and the number of SNPs used (.grm.N.bin holds n_valid).
- * A kept SNP whose float32 mean is below 2^-6 and not a multiple of 2^-29 (only possible for
- * min_maf < 2^-7) -> CS_ERR_INVALID_ARGUMENT.
+ * Any float32 mean is exact (the integer scale grows to at most 2^40 for rare SNPs); a mean needing
+ * more fractional bits, or min_hwe outside [0, 1], -> CS_ERR_INVALID_ARGUMENT.
  * alg 0 or 3 (Yang / GCTA default, grm_builder.cpp:241-243, 386-387): z = (x - fl32(2p)) *
  * fl32(1/sqrt(2p(1-p))), G = Z Z^T / n_valid, computed as FP16 hi/lo split products on MFMA with
  * FP32 accumulation flushed to FP64 every 16384 SNPs (entries within ~1e-6 of the FP64 value
  * relative to the diagonal); grm_raw_f64 is then that FP64 Z Z^T and grm_tri_f32 fl32(raw / n_valid).
  * Other algorithms -> CS_ERR_INVALID_ARGUMENT. */
-int cs_grm_build(cs_bed* bed, int alg, float min_maf, float* grm_tri_f32, double* grm_raw_f64,
-                 double* sum_2pq, int64_t* n_valid);
+int cs_grm_build(cs_bed* bed, int alg, float min_maf, float min_hwe, float* grm_tri_f32,
+                 double* grm_raw_f64, double* sum_2pq, int64_t* n_valid);
 /* Same with the lower triangle written to device memory (n(n+1)/2 floats). */
-int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float* d_grm_tri_f32,
+int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float min_hwe, float* d_grm_tri_f32,
                         double* sum_2pq, int64_t* n_valid);
 /* Raw Z Z^T (the correctly rounded FP64 of the exact integer product, n x n, symmetric) over a
  * SNP subset -- a chromosome's per_chrom_raw in GrmBuilder::computeSplit
@@ -232,8 +234,8 @@ int cs_grm_build_device(cs_bed* bed, int alg, float min_maf, float* d_grm_tri_f3
  * raw_on_device != 0.  sum_2pq_block: the float 2pq of the selected valid SNPs summed in 2048-SNP
  * block order (the global normaliser, :266-272); sum_2pq_seq: the same summed SNP by SNP (the
  * per-chromosome per_chrom_sum_2pq, :718-724). */
-int cs_grm_build_raw(cs_bed* bed, int alg, float min_maf, const int32_t* snps, int64_t n_snps,
-                     double* raw_f64, int raw_on_device, double* sum_2pq_block,
+int cs_grm_build_raw(cs_bed* bed, int alg, float min_maf, float min_hwe, const int32_t* snps,
+                     int64_t n_snps, double* raw_f64, int raw_on_device, double* sum_2pq_block,
                      double* sum_2pq_seq, int64_t* n_valid);
 /* Kernel time of the last GRM build on this thread (all int8 products; the int128 combine), the
  * number of int8 product launches and the int8 MFMA ops they issued, from HIP events on the

@@ -82,19 +82,23 @@ def lib():
         L.orc_kinship_trace.argtypes = [_u8p, _i64, _dp, _dp, _i32p, _i64, _i32p, _i64]
         L.orc_kinship_trace.restype = ctypes.c_double
         L.orc_kinship_diag.argtypes = [_u8p, _i64, _dp, _dp, _i32p, _i64, _i32p, _i64, _dp]
-        L.orc_grm_snp_params.argtypes = [_u8p, _i64, _i64, ctypes.c_int, ctypes.c_float, _i8p,
+        L.orc_hwe_p.argtypes = [ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong]
+        L.orc_hwe_p.restype = ctypes.c_double
+        L.orc_grm_snp_params.argtypes = [_u8p, _i64, _i64, ctypes.c_int, ctypes.c_float, ctypes.c_float, _i8p,
                                          _fp, _fp, _fp, _dp, _i64p, ctypes.c_int]
-        L.orc_grm.argtypes = [_u8p, _i64, _i64, ctypes.c_int, ctypes.c_float, _dp, _dp, _i64p,
+        L.orc_grm.argtypes = [_u8p, _i64, _i64, ctypes.c_int, ctypes.c_float, ctypes.c_float, _dp, _dp, _i64p,
                               ctypes.c_int]
-        L.orc_grm_exact_centred.argtypes = [_u8p, _i64, _i64, ctypes.c_float, _dp, _fp, _dp,
+        L.orc_grm_exact_centred.argtypes = [_u8p, _i64, _i64, ctypes.c_float, ctypes.c_float, _dp, _fp, _dp,
                                             _i64p, ctypes.c_int]
         L.orc_grm_exact_centred.restype = ctypes.c_int
-        L.orc_grm_exact_tile.argtypes = [_u8p, _i64, _i64, ctypes.c_float, _i32p, _i64, _i32p, _i64, _dp, _fp,
+        L.orc_grm_exact_tile.argtypes = [_u8p, _i64, _i64, ctypes.c_float, ctypes.c_float, _i32p, _i64, _i32p, _i64,
+                                         _dp, _fp,
                                          _dp, _i64p, ctypes.c_int]
         L.orc_grm_exact_tile.restype = ctypes.c_int
-        L.orc_grm_exact_diag.argtypes = [_u8p, _i64, _i64, ctypes.c_float, _dp, _fp, ctypes.c_int]
+        L.orc_grm_exact_diag.argtypes = [_u8p, _i64, _i64, ctypes.c_float, ctypes.c_float, _dp, _fp, ctypes.c_int]
         L.orc_grm_exact_diag.restype = ctypes.c_int
-        L.orc_grm_std_tile.argtypes = [_u8p, _i64, _i64, ctypes.c_float, _i32p, _i64, _i32p, _i64, _dp, _i64p,
+        L.orc_grm_std_tile.argtypes = [_u8p, _i64, _i64, ctypes.c_float, ctypes.c_float, _i32p, _i64, _i32p, _i64,
+                                       _dp, _i64p,
                                        ctypes.c_int]
         L.orc_scan_sums.argtypes = [_u8p, _i64, _i64, _i32p, _i64, ctypes.c_double, _dp, _i64,
                                     _dp, _i64p, _i32p, _dp, _dp, ctypes.c_int]
@@ -256,7 +260,12 @@ def kinship_diag(bed, N, means, inv_stds, keep=None, variants=None):
     return d
 
 
-def grm_snp_params(bed, N, M, alg=1, min_maf=0.01, nthreads=None):
+def hwe_p(hom1, hets, hom2):
+    """Exact HWE p-value (calculateHWE, libCosmicCore/src/qc_utils.cpp:9-77)."""
+    return lib().orc_hwe_p(int(hom1), int(hets), int(hom2))
+
+
+def grm_snp_params(bed, N, M, alg=1, min_maf=0.01, nthreads=None, min_hwe=0.0):
     bed = _u8(bed)
     valid = np.empty(M, dtype=np.int8)
     mean_f = np.empty(M, dtype=np.float32)
@@ -264,36 +273,36 @@ def grm_snp_params(bed, N, M, alg=1, min_maf=0.01, nthreads=None):
     pq2_f = np.empty(M, dtype=np.float32)
     s = ctypes.c_double()
     nv = ctypes.c_int64()
-    lib().orc_grm_snp_params(_p(bed, _u8p), N, M, alg, min_maf, _p(valid, _i8p), _p(mean_f, _fp),
+    lib().orc_grm_snp_params(_p(bed, _u8p), N, M, alg, min_maf, min_hwe, _p(valid, _i8p), _p(mean_f, _fp),
                              _p(scale_f, _fp), _p(pq2_f, _fp), ctypes.byref(s), ctypes.byref(nv),
                              _nthreads(nthreads))
     return valid.astype(bool), mean_f, scale_f, pq2_f, s.value, nv.value
 
 
-def grm(bed, N, M, alg=1, min_maf=0.01, nthreads=None):
+def grm(bed, N, M, alg=1, min_maf=0.01, nthreads=None, min_hwe=0.0):
     bed = _u8(bed)
     G = np.empty((N, N))
     s = ctypes.c_double()
     nv = ctypes.c_int64()
-    lib().orc_grm(_p(bed, _u8p), N, M, alg, min_maf, _p(G, _dp), ctypes.byref(s),
+    lib().orc_grm(_p(bed, _u8p), N, M, alg, min_maf, min_hwe, _p(G, _dp), ctypes.byref(s),
                   ctypes.byref(nv), _nthreads(nthreads))
     return G, s.value, nv.value
 
 
-def grm_exact_centred(bed, N, M, min_maf=0.01, nthreads=None):
+def grm_exact_centred(bed, N, M, min_maf=0.01, nthreads=None, min_hwe=0.0):
     bed = _u8(bed)
     raw = np.empty((N, N))
     nf = np.empty((N, N), dtype=np.float32)
     s = ctypes.c_double()
     nv = ctypes.c_int64()
-    rc = lib().orc_grm_exact_centred(_p(bed, _u8p), N, M, min_maf, _p(raw, _dp), _p(nf, _fp),
+    rc = lib().orc_grm_exact_centred(_p(bed, _u8p), N, M, min_maf, min_hwe, _p(raw, _dp), _p(nf, _fp),
                                      ctypes.byref(s), ctypes.byref(nv), _nthreads(nthreads))
     if rc != 0:
         raise ValueError("exact GRM: scale beyond 40 bits or more than 2^22 SNPs")
     return raw, nf, s.value, nv.value
 
 
-def grm_exact_tile(bed, N, M, rows, cols, min_maf=0.01, nthreads=None):
+def grm_exact_tile(bed, N, M, rows, cols, min_maf=0.01, nthreads=None, min_hwe=0.0):
     """Entries (rows x cols) of the exact centred GRM (alg 1) over all SNPs at any float mean:
     (raw FP64, GCTA FP32, sum_2pq, n_valid), each correctly rounded from the exact integer."""
     bed = _u8(bed)
@@ -303,7 +312,7 @@ def grm_exact_tile(bed, N, M, rows, cols, min_maf=0.01, nthreads=None):
     nf = np.empty((rows.size, cols.size), dtype=np.float32)
     s = ctypes.c_double()
     nv = ctypes.c_int64()
-    sb = lib().orc_grm_exact_tile(_p(bed, _u8p), N, M, min_maf, _p(rows, _i32p), rows.size, _p(cols, _i32p),
+    sb = lib().orc_grm_exact_tile(_p(bed, _u8p), N, M, min_maf, min_hwe, _p(rows, _i32p), rows.size, _p(cols, _i32p),
                                   cols.size, _p(raw, _dp), _p(nf, _fp), ctypes.byref(s), ctypes.byref(nv),
                                   _nthreads(nthreads))
     if sb < 0:
@@ -311,24 +320,26 @@ def grm_exact_tile(bed, N, M, rows, cols, min_maf=0.01, nthreads=None):
     return raw, nf, s.value, nv.value
 
 
-def grm_std_tile(bed, N, M, rows, cols, min_maf=0.01, nthreads=None):
+def grm_std_tile(bed, N, M, rows, cols, min_maf=0.01, nthreads=None, min_hwe=0.0):
     """Entries (rows x cols) of the standardised GRM (alg 0) in FP64 over all SNPs."""
     bed = _u8(bed)
     rows = np.ascontiguousarray(rows, dtype=np.int32)
     cols = np.ascontiguousarray(cols, dtype=np.int32)
     G = np.empty((rows.size, cols.size))
     nv = ctypes.c_int64()
-    lib().orc_grm_std_tile(_p(bed, _u8p), N, M, min_maf, _p(rows, _i32p), rows.size, _p(cols, _i32p), cols.size,
+    lib().orc_grm_std_tile(_p(bed, _u8p), N, M, min_maf, min_hwe, _p(rows, _i32p), rows.size, _p(cols, _i32p),
+                           cols.size,
                            _p(G, _dp), ctypes.byref(nv), _nthreads(nthreads))
     return G, nv.value
 
 
-def grm_exact_diag(bed, N, M, min_maf=0.01, nthreads=None):
+def grm_exact_diag(bed, N, M, min_maf=0.01, nthreads=None, min_hwe=0.0):
     """Diagonal of the exact centred GRM for every sample: (raw FP64, GCTA FP32)."""
     bed = _u8(bed)
     raw = np.empty(N)
     nf = np.empty(N, dtype=np.float32)
-    if lib().orc_grm_exact_diag(_p(bed, _u8p), N, M, min_maf, _p(raw, _dp), _p(nf, _fp), _nthreads(nthreads)) < 0:
+    if lib().orc_grm_exact_diag(_p(bed, _u8p), N, M, min_maf, min_hwe, _p(raw, _dp), _p(nf, _fp),
+                                _nthreads(nthreads)) < 0:
         raise ValueError("exact GRM diagonal: scale beyond 40 bits or more than 2^22 SNPs")
     return raw, nf
 

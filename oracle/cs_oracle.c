@@ -332,19 +332,60 @@ void orc_kinship_diag(const uint8_t* bed, int64_t N, const double* means,
  * Per-SNP outputs (valid flag, float mean, float scale, float 2pq) are also returned because
  * the GPU path must reproduce exactly the same filtered SNP set.
  * G is N x N row-major FP64 (full square). */
-void orc_grm_snp_params(const uint8_t* bed, int64_t N, int64_t M, int alg, float min_maf,
+/* calculateHWE (libCosmicCore/src/qc_utils.cpp:9-77): the exact Hardy-Weinberg test of
+ * Wigginton, Cutler and Abecasis (2005) -- heterozygote distribution given the allele counts by
+ * the recurrences from its mode, p = mass of the outcomes no more likely than the observed one. */
+double orc_hwe_p(long long hom1, long long hets, long long hom2) {
+    const long long homr = hom1 < hom2 ? hom1 : hom2;
+    const long long rare = 2 * homr + hets, n = hom1 + hets + hom2;
+    if (n == 0) return 1.0;
+    long long mid = rare * (2 * n - rare) / (2 * n);
+    if ((mid % 2) != (rare % 2)) mid++;
+    double* pr = (double*)calloc((size_t)rare + 1, sizeof(double));
+    pr[mid] = 1.0;
+    double total = 1.0;
+    long long r = (rare - mid) / 2, c = n - mid - r;
+    for (long long h = mid; h >= 2; h -= 2) {
+        pr[h - 2] = pr[h] * h * (h - 1.0) / (4.0 * (r + 1.0) * (c + 1.0));
+        total += pr[h - 2];
+        r++;
+        c++;
+    }
+    r = (rare - mid) / 2;
+    c = n - mid - r;
+    for (long long h = mid; h <= rare - 2; h += 2) {
+        pr[h + 2] = pr[h] * 4.0 * r * c / ((h + 2.0) * (h + 1.0));
+        total += pr[h + 2];
+        r--;
+        c--;
+    }
+    const double target = pr[hets];
+    double p = 0.0;
+    for (long long h = 0; h <= rare; ++h) {
+        if ((h % 2) != (rare % 2)) continue;
+        if (pr[h] <= target) p += pr[h];
+    }
+    free(pr);
+    p /= total;
+    return p < 1.0 ? p : 1.0;
+}
+
+void orc_grm_snp_params(const uint8_t* bed, int64_t N, int64_t M, int alg, float min_maf, float min_hwe,
                         int8_t* valid, float* mean_f, float* scale_f, float* pq2_f,
                         double* sum_2pq_out, int64_t* n_valid_out, int nthreads) {
     const int64_t Bs = bytes_per_snp(N);
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t j = 0; j < M; ++j) {
         const uint8_t* col = bed + j * Bs;
-        long long sum_x = 0, n_obs = 0;
+        long long sum_x = 0, n_obs = 0, n_hom1 = 0, n_het = 0, n_hom2 = 0;
         for (int64_t i = 0; i < N; ++i) {
             int c = code_at(col, i);
             if (c == 1) continue;
             n_obs++;
             sum_x += kDosI[c];
+            if (c == 0) n_hom1++;
+            else if (c == 2) n_het++;
+            else n_hom2++;
         }
         valid[j] = 0;
         mean_f[j] = 0.0f;
@@ -354,6 +395,7 @@ void orc_grm_snp_params(const uint8_t* bed, int64_t N, int64_t M, int alg, float
         double freq = (double)sum_x / (2.0 * n_obs);
         double maf = freq < 1.0 - freq ? freq : 1.0 - freq;
         if (maf < (double)min_maf || fabs(maf) < 1e-8) continue;
+        if (min_hwe > 0.0f && orc_hwe_p(n_hom1, n_het, n_hom2) < (double)min_hwe) continue;
         pq2_f[j] = (float)(1.0f * 2.0 * freq * (1.0 - freq));
         mean_f[j] = (float)(freq * 2.0);
         if (alg == 0 || alg == 3) {
@@ -375,7 +417,7 @@ void orc_grm_snp_params(const uint8_t* bed, int64_t N, int64_t M, int alg, float
     *n_valid_out = nv;
 }
 
-void orc_grm(const uint8_t* bed, int64_t N, int64_t M, int alg, float min_maf, double* G,
+void orc_grm(const uint8_t* bed, int64_t N, int64_t M, int alg, float min_maf, float min_hwe, double* G,
              double* sum_2pq_out, int64_t* n_valid_out, int nthreads) {
     const int64_t Bs = bytes_per_snp(N);
     int8_t* valid = (int8_t*)malloc((size_t)M);
@@ -384,7 +426,7 @@ void orc_grm(const uint8_t* bed, int64_t N, int64_t M, int alg, float min_maf, d
     float* pq2_f = (float*)malloc((size_t)M * sizeof(float));
     double sum_2pq;
     int64_t nv;
-    orc_grm_snp_params(bed, N, M, alg, min_maf, valid, mean_f, scale_f, pq2_f, &sum_2pq, &nv,
+    orc_grm_snp_params(bed, N, M, alg, min_maf, min_hwe, valid, mean_f, scale_f, pq2_f, &sum_2pq, &nv,
                        nthreads);
     memset(G, 0, (size_t)(N * N) * sizeof(double));
     /* Z fill in FP64 from the float mean / scale (grm_builder.cpp:300-342), then G += Z Z^T. */
@@ -491,12 +533,12 @@ static float i128_div_f32_rn_sc(i128 v, double S, int sc) {
  * SB the smallest shift >= 29 that makes every valid mu_j 2^SB an integer, z_ij 2^SB =
  * U_ij = o_ij (x_ij 2^SB - mu_j 2^SB) exactly, and I = sum_j U_ij U_kj in int128.  Returns SB
  * (or -1 if SB would exceed 40 bits or M 2^22 SNPs, where int128 could overflow). */
-static int orc_grm_exact_prepare(const uint8_t* bed, int64_t N, int64_t M, float min_maf, int8_t* valid,
+static int orc_grm_exact_prepare(const uint8_t* bed, int64_t N, int64_t M, float min_maf, float min_hwe, int8_t* valid,
                                  int64_t* A, double* sum_2pq, int64_t* nv, int nthreads) {
     float* mean_f = (float*)malloc((size_t)M * sizeof(float));
     float* scale_f = (float*)malloc((size_t)M * sizeof(float));
     float* pq2_f = (float*)malloc((size_t)M * sizeof(float));
-    orc_grm_snp_params(bed, N, M, 1, min_maf, valid, mean_f, scale_f, pq2_f, sum_2pq, nv, nthreads);
+    orc_grm_snp_params(bed, N, M, 1, min_maf, min_hwe, valid, mean_f, scale_f, pq2_f, sum_2pq, nv, nthreads);
     int sb = 29;
     for (int64_t j = 0; j < M; ++j) {
         if (!valid[j]) continue;
@@ -515,7 +557,8 @@ static int orc_grm_exact_prepare(const uint8_t* bed, int64_t N, int64_t M, float
 /* Tile of the exact centred GRM: rows x cols sample lists over all SNPs -> the correctly rounded
  * FP64 raw value and the correctly rounded FP32 of raw / sum_2pq per entry (row-major nr x nc).
  * Return: SB, or -1 (range, see orc_grm_exact_prepare). */
-int orc_grm_exact_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, const int32_t* rows, int64_t nr,
+int orc_grm_exact_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, float min_hwe, const int32_t* rows,
+                       int64_t nr,
                        const int32_t* cols, int64_t nc, double* raw_f64, float* norm_f32, double* sum_2pq_out,
                        int64_t* n_valid_out, int nthreads) {
     const int64_t Bs = bytes_per_snp(N);
@@ -523,7 +566,7 @@ int orc_grm_exact_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, 
     int64_t* A = (int64_t*)malloc((size_t)M * sizeof(int64_t));
     double sum_2pq;
     int64_t nv;
-    const int sb = orc_grm_exact_prepare(bed, N, M, min_maf, valid, A, &sum_2pq, &nv, nthreads);
+    const int sb = orc_grm_exact_prepare(bed, N, M, min_maf, min_hwe, valid, A, &sum_2pq, &nv, nthreads);
     if (sb < 0) { free(valid); free(A); return -1; }
     if (nthreads < 1) nthreads = 1;
     i128* acc = (i128*)calloc((size_t)(nthreads * nr * nc), sizeof(i128));
@@ -572,7 +615,8 @@ int orc_grm_exact_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, 
 
 /* Tile of the standardised GRM (alg 0 / 3, grm_builder.cpp:241-243, 300-304, 386-387) in FP64:
  * z = (x - fl32 mean) fl32 scale over the valid SNPs, G = Z Z^T / n_valid, rows x cols. */
-void orc_grm_std_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, const int32_t* rows, int64_t nr,
+void orc_grm_std_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, float min_hwe, const int32_t* rows,
+                      int64_t nr,
                       const int32_t* cols, int64_t nc, double* G, int64_t* n_valid_out, int nthreads) {
     const int64_t Bs = bytes_per_snp(N);
     int8_t* valid = (int8_t*)malloc((size_t)M);
@@ -581,7 +625,7 @@ void orc_grm_std_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, c
     float* pq2_f = (float*)malloc((size_t)M * sizeof(float));
     double sum_2pq;
     int64_t nv;
-    orc_grm_snp_params(bed, N, M, 0, min_maf, valid, mean_f, scale_f, pq2_f, &sum_2pq, &nv, nthreads);
+    orc_grm_snp_params(bed, N, M, 0, min_maf, min_hwe, valid, mean_f, scale_f, pq2_f, &sum_2pq, &nv, nthreads);
     if (nthreads < 1) nthreads = 1;
     double* acc = (double*)calloc((size_t)(nthreads * nr * nc), sizeof(double));
 #pragma omp parallel num_threads(nthreads)
@@ -617,14 +661,15 @@ void orc_grm_std_tile(const uint8_t* bed, int64_t N, int64_t M, float min_maf, c
 }
 
 /* Diagonal of the exact centred GRM for every sample (as orc_grm_exact_tile with rows = cols). */
-int orc_grm_exact_diag(const uint8_t* bed, int64_t N, int64_t M, float min_maf, double* raw_f64, float* norm_f32,
+int orc_grm_exact_diag(const uint8_t* bed, int64_t N, int64_t M, float min_maf, float min_hwe, double* raw_f64,
+                       float* norm_f32,
                        int nthreads) {
     const int64_t Bs = bytes_per_snp(N);
     int8_t* valid = (int8_t*)malloc((size_t)M);
     int64_t* A = (int64_t*)malloc((size_t)M * sizeof(int64_t));
     double sum_2pq;
     int64_t nv;
-    const int sb = orc_grm_exact_prepare(bed, N, M, min_maf, valid, A, &sum_2pq, &nv, nthreads);
+    const int sb = orc_grm_exact_prepare(bed, N, M, min_maf, min_hwe, valid, A, &sum_2pq, &nv, nthreads);
     if (sb < 0) { free(valid); free(A); return -1; }
     if (nthreads < 1) nthreads = 1;
     i128* acc = (i128*)calloc((size_t)(nthreads * N), sizeof(i128));
@@ -659,7 +704,7 @@ int orc_grm_exact_diag(const uint8_t* bed, int64_t N, int64_t M, float min_maf, 
 /* The whole exact centred GRM (n x n, both triangles) at any float mean (orc_grm_exact_prepare):
  * per entry the correctly rounded FP64 of G_raw and the correctly rounded FP32 of G_raw / sum_2pq.
  * Return value: 0 ok, -1 out of range (orc_grm_exact_prepare). */
-int orc_grm_exact_centred(const uint8_t* bed, int64_t N, int64_t M, float min_maf,
+int orc_grm_exact_centred(const uint8_t* bed, int64_t N, int64_t M, float min_maf, float min_hwe,
                           double* raw_f64, float* norm_f32, double* sum_2pq_out,
                           int64_t* n_valid_out, int nthreads) {
     const int64_t Bs = bytes_per_snp(N);
@@ -667,7 +712,7 @@ int orc_grm_exact_centred(const uint8_t* bed, int64_t N, int64_t M, float min_ma
     int64_t* A = (int64_t*)malloc((size_t)M * sizeof(int64_t));
     double sum_2pq;
     int64_t nv;
-    const int sb = orc_grm_exact_prepare(bed, N, M, min_maf, valid, A, &sum_2pq, &nv, nthreads);
+    const int sb = orc_grm_exact_prepare(bed, N, M, min_maf, min_hwe, valid, A, &sum_2pq, &nv, nthreads);
     if (sb < 0) { free(valid); free(A); return -1; }
     /* U as int64 per (sample, valid SNP), SNP-contiguous per sample. */
     int64_t* idx = (int64_t*)malloc((size_t)(nv > 0 ? nv : 1) * sizeof(int64_t));

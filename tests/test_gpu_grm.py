@@ -40,15 +40,15 @@ def lower(a):
     return a[np.tril_indices(a.shape[0])]
 
 
-def check_against_oracle(csa, reader, body, N, M, min_maf=0.01):
-    b = csa.GrmBuilder(csa.GrmBuilder.Options(min_maf=min_maf))
+def check_against_oracle(csa, reader, body, N, M, min_maf=0.01, min_hwe=0.0):
+    b = csa.GrmBuilder(csa.GrmBuilder.Options(min_maf=min_maf, min_hwe=min_hwe))
     b.compute(reader)
-    raw, nf, s2pq, nv = oracle.grm_exact_centred(body, N, M, min_maf)
+    raw, nf, s2pq, nv = oracle.grm_exact_centred(body, N, M, min_maf, min_hwe=min_hwe)
     assert b.getSum2pq() == s2pq
     assert b.getTotalSnps() == nv
     assert np.array_equal(b.getRaw(), raw)
     assert np.array_equal(b.getLowerTriangle(), lower(nf))
-    G64, s64, nv64 = oracle.grm(body, N, M, 1, min_maf)
+    G64, s64, nv64 = oracle.grm(body, N, M, 1, min_maf, min_hwe=min_hwe)
     assert (s64, nv64) == (s2pq, nv)
     G = b.getGrm()
     assert np.max(np.abs(G - G64)) <= 1e-12 * np.max(np.abs(G64))
@@ -109,6 +109,32 @@ def test_grm_many_missing_pairs(csa, monkeypatch):
     monkeypatch.setenv("CSAMD_GRM_PANEL_BYTES", "1")
     r = csa.PlinkReader.synthetic(77, N, M, 0.01, 0.5, 0.12)
     check_against_oracle(csa, r, r.download(), N, M)
+
+
+def test_grm_hwe_filter(csa):
+    # GrmBuilder's optional exact-HWE SNP filter (grm_builder.cpp:220-223): SNPs with an excess of
+    # heterozygotes or homozygotes are dropped from both GRMs at min_hwe = 1e-4
+    N, M = 400, 1500
+    r0 = csa.PlinkReader.synthetic(55, N, M, 0.05, 0.5, 0.01)
+    body = r0.download().reshape(M, -1).copy()
+    rng = np.random.default_rng(6)
+    for j in rng.choice(M, 60, replace=False):
+        codes = np.where(rng.random(N) < 0.8, 2, rng.choice([0, 3], N))    # mostly heterozygous
+        codes[rng.random(N) < 0.01] = 1
+        pad = np.zeros(body.shape[1] * 4, dtype=np.uint8)
+        pad[:N] = codes
+        q = pad.reshape(-1, 4)
+        body[j] = q[:, 0] | (q[:, 1] << 2) | (q[:, 2] << 4) | (q[:, 3] << 6)
+    body = body.reshape(-1)
+    r = csa.PlinkReader.from_body(body, N, M)
+    b = check_against_oracle(csa, r, body, N, M, min_hwe=1e-4)
+    assert b.getTotalSnps() <= M - 55
+    b0 = csa.GrmBuilder(csa.GrmBuilder.Options(algorithm=0, min_hwe=1e-4))
+    b0.compute(r)
+    _, _, nv0 = oracle.grm(body, N, M, 0, 0.01, min_hwe=1e-4)
+    assert b0.getTotalSnps() == nv0 == b.getTotalSnps()
+    with pytest.raises(ValueError, match="min_hwe"):
+        csa.GrmBuilder(csa.GrmBuilder.Options(min_hwe=1.5)).compute(r)
 
 
 def test_grm_device_output(csa):

@@ -619,3 +619,31 @@ def test_multiply_z_zt_restatement():
     KX = oracle.kinship_apply(bed, N, means, inv, None, None, X)
     ZZX = oracle.multiply_z(bed, N, M, oracle.multiply_zt(bed, N, M, X)) / M
     assert rel(ZZX, KX) < 1e-12
+
+
+def test_hwe_exact_known_answers_and_enumeration():
+    # tests/test_qc_pipeline.cpp:166-167 holds calculateHWE(3, 0, 3) = 5/231; beyond it, the exact
+    # conditional distribution of the heterozygote count given the allele counts enumerated with
+    # rationals: P(h) ~ n! 2^h / (h! homr! homc!), p = sum of P(h') <= P(h_obs)
+    from fractions import Fraction
+    from math import factorial
+    assert abs(oracle.hwe_p(3, 0, 3) - 5.0 / 231) < 1e-14
+    assert oracle.hwe_p(0, 0, 0) == 1.0 and oracle.hwe_p(10, 0, 0) == 1.0
+
+    def exact(hom1, het, hom2):
+        n = hom1 + het + hom2
+        rare = 2 * min(hom1, hom2) + het
+        probs = {}
+        for h in range(rare % 2, rare + 1, 2):
+            r = (rare - h) // 2
+            c = n - h - r
+            if c < 0:
+                continue
+            probs[h] = Fraction(factorial(n) * 2 ** h, factorial(h) * factorial(r) * factorial(c))
+        tot = sum(probs.values())
+        return float(sum(v for v in probs.values() if v <= probs[het]) / tot)
+
+    for hom1, het, hom2 in [(3, 0, 3), (10, 20, 10), (0, 30, 0), (25, 2, 3), (1, 1, 40), (57, 3, 0),
+                            (100, 80, 20), (7, 0, 9), (0, 1, 0), (12, 13, 11)]:
+        assert abs(oracle.hwe_p(hom1, het, hom2) - exact(hom1, het, hom2)) <= 1e-12, (hom1, het, hom2)
+        assert oracle.hwe_p(hom1, het, hom2) == oracle.hwe_p(hom2, het, hom1)
