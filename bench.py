@@ -367,10 +367,11 @@ def lowrank_leg(csa, ctx, rank=256, oversample=10):
     return res
 
 
-def mv_leg(csa, n=32_000, m=300_000, d=4, seed=20261207, n_check=32):
-    """Multi-trait LMM (the C5 workload): d = 4 correlated traits, Vg ~ Wishart(4, 0.3 I),
+def mv_leg(csa, n=50_000, m=300_000, d=4, seed=20261207, n_check=32):
+    """Multi-trait LMM workload: d = 4 correlated traits, Vg ~ Wishart(4, 0.3 I),
     Ve ~ Wishart(4, 0.7 I), 3 records per individual averaged, 500 causal SNPs; standardised GRM
-    (alg 0, FP16-split MFMA), MvEigenVce prepare (rocSOLVER QR + dsyevd), AI-REML, and the
+    (alg 0, FP16-split MFMA), MvEigenVce prepare (QR + dense eigendecomposition: rocSOLVER dsyevd
+    up to n = 32766, the 64-bit tridiagonal divide-and-conquer path beyond), AI-REML, and the
     multivariate Wald scan of all m SNPs.  The scan is checked against an FP64 recomputation
     (W_proj^T g in FP64 on the host) for n_check SNPs."""
     rng = np.random.default_rng(seed)
@@ -406,7 +407,7 @@ def mv_leg(csa, n=32_000, m=300_000, d=4, seed=20261207, n_check=32):
     mv.prepare(G)
     G.close()
     out["prepare_s"] = time.perf_counter() - t0
-    log(f"mv: prepare (QR, A^T G A, dsyevd, W) {out['prepare_s']:.1f} s")
+    log(f"mv: prepare (QR, A^T G A, eigendecomposition, W) {out['prepare_s']:.1f} s")
     t0 = time.perf_counter()
     conv, em_it, ai_it = mv.runNullModel("aireml")
     out["null_s"] = time.perf_counter() - t0
@@ -547,7 +548,7 @@ def main():
                     help="also run the low-rank matrix-free GWAS leg (the reference path, headline genotypes)")
     # the dense eigen path (MvEigenVce::prepare) is bounded by rocSOLVER's 32-bit interfaces at
     # n <= 32766 individuals; the multi-trait workload names 50,000
-    ap.add_argument("--mv-n", type=int, default=32_000)
+    ap.add_argument("--mv-n", type=int, default=50_000)
     ap.add_argument("--mv-m", type=int, default=300_000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 2 GiB device copy 3 times (HBM counter calibration)")

@@ -100,6 +100,7 @@ def _load():
                                 ctypes.c_int, ctypes.c_void_p, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
                                ctypes.c_int),
         "cs_scan_last_timing": ([_dp, _dp, _dp, _i64p], ctypes.c_int),
+        "cs_sym_eigen_device": ([_vp, _i64, _i64, _vp, _i64], ctypes.c_int),
         "cs_grm_build": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, _fp, _dp, _dp, _i64p], ctypes.c_int),
         "cs_grm_build_device": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, _vp, _dp, _i64p],
                                 ctypes.c_int),
@@ -654,6 +655,17 @@ def grm_build_device(reader, d_tri_ptr, min_maf=0.01, alg=1, min_hwe=0.0):
     _check(lib().cs_grm_build_device(reader._h, int(alg), float(min_maf), float(min_hwe), d_tri_ptr,
                                      ctypes.byref(s), ctypes.byref(nv)))
     return s.value, nv.value
+
+
+def sym_eigen_device(dA, dD, leaf=32766):
+    """Eigendecomposition of the symmetric DeviceArray dA (n x n, lower triangle read) in place:
+    dA <- eigenvectors, dD (n x 1 DeviceArray) <- eigenvalues ascending -- the
+    SelfAdjointEigenSolver step of EigenVce::prepare (libCosmicCore/src/eigen_vce.cpp:34-38).
+    leaf = 0: rocSOLVER dsyevd (n <= 32766); leaf > 0: the 64-bit tridiagonal divide-and-conquer
+    path with blocks of at most `leaf` rows."""
+    if dA.k != dA.n or dD.n * dD.k < dA.n:
+        raise ValueError("sym_eigen_device: dimension mismatch")
+    _check(lib().cs_sym_eigen_device(dA.ptr, dA.n, dA.n, dD.ptr, int(leaf)))
 
 
 # ----------------------------------------------------------------------- dense null model + scan

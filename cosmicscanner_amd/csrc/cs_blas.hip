@@ -66,16 +66,15 @@ void householder_project_sym(rocblas_handle blas, hipStream_t st, const double* 
                                   st));
         CS_HIP(hipMemcpyAsync(v.p, &one, 8, hipMemcpyHostToDevice, st));
         double* Gi = G + i * n + i;
-        check_blas(rocblas_dsymv(blas, rocblas_fill_lower, (int)m, &tau, Gi, (int)n, v.as<double>(), 1, &zero,
-                                 w.as<double>(), 1),
+        check_blas(rocblas_dsymv_64(blas, rocblas_fill_lower, m, &tau, Gi, n, v.as<double>(), 1, &zero,
+                                    w.as<double>(), 1),
                    "householder_project_sym (symv)");
         double wv = 0.0;
-        check_blas(rocblas_ddot(blas, (int)m, w.as<double>(), 1, v.as<double>(), 1, &wv), "householder_project_sym (dot)");
+        check_blas(rocblas_ddot_64(blas, m, w.as<double>(), 1, v.as<double>(), 1, &wv), "householder_project_sym (dot)");
         const double alpha = -0.5 * tau * wv;
-        check_blas(rocblas_daxpy(blas, (int)m, &alpha, v.as<double>(), 1, w.as<double>(), 1),
+        check_blas(rocblas_daxpy_64(blas, m, &alpha, v.as<double>(), 1, w.as<double>(), 1),
                    "householder_project_sym (axpy)");
-        check_blas(rocblas_dsyr2(blas, rocblas_fill_lower, (int)m, &mone, v.as<double>(), 1, w.as<double>(), 1, Gi,
-                                 (int)n),
+        check_blas(rocblas_dsyr2_64(blas, rocblas_fill_lower, m, &mone, v.as<double>(), 1, w.as<double>(), 1, Gi, n),
                    "householder_project_sym (syr2)");
     }
     CS_HIP(hipStreamSynchronize(st));   // v, w are freed on return

@@ -196,17 +196,25 @@ static void lmm_prepare(cs_lmm* h, const double* G, int64_t ldg, bool g_dev) {
                                  h->Q.as<double>(), (int)n, h->tau.as<double>(), h->t.as<double>(), (int)n));
     }
     double* U = Gw + c * n + c;   // ne x ne, ld n
-    CS_BLAS(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)ne, U, (int)n,
-                             h->D.as<double>(), h->E.as<double>(), h->info.as<rocblas_int>()));
+    const bool large = eig_use_large(ne);
+    if (large) {
+        sym_eig_large(h->blas, h->st, ne, U, n, h->D.as<double>(), eig_large_leaf());
+        CS_HIP(hipMemsetAsync(h->info.p, 0, sizeof(rocblas_int), h->st));
+    } else {
+        CS_BLAS(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)ne, U, (int)n,
+                                 h->D.as<double>(), h->E.as<double>(), h->info.as<rocblas_int>()));
+    }
     k_clamp_nonneg<<<grid_for(ne), 256, 0, h->st>>>(h->D.as<double>(), ne);
     CS_HIP(hipGetLastError());
     // Uty = U^T (A^T y); W_proj = A U = Q [0; U]
-    CS_BLAS(rocblas_dgemv(h->blas, rocblas_operation_transpose, (int)ne, (int)ne, &one, U, (int)n,
-                          h->t.as<double>() + c, 1, &zero, h->Uty.as<double>(), 1));
+    CS_BLAS(rocblas_dgemv_64(h->blas, rocblas_operation_transpose, ne, ne, &one, U, n, h->t.as<double>() + c, 1,
+                             &zero, h->Uty.as<double>(), 1));
     if (c > 0) CS_HIP(hipMemset2DAsync(W, (size_t)n * 8, 0, (size_t)c * 8, (size_t)ne, h->st));
     CS_HIP(hipMemcpy2DAsync(W + c, (size_t)n * 8, U, (size_t)n * 8, (size_t)ne * 8, (size_t)ne,
                             hipMemcpyDeviceToDevice, h->st));
-    if (c > 0)
+    if (c > 0 && large)
+        apply_reflectors(h->blas, h->st, h->Q.as<double>(), n, h->tau_h.data(), n, c, false, W, n, ne);
+    else if (c > 0)
         CS_BLAS(rocsolver_dormqr(h->blas, rocblas_side_left, rocblas_operation_none, (int)n, (int)ne, (int)c,
                                  h->Q.as<double>(), (int)n, h->tau.as<double>(), W, (int)n));
     rocblas_int info = 0;
@@ -224,7 +232,6 @@ int cs_lmm_create(const double* y, const float* X, int64_t ldx, int64_t n, int64
         if (!y || !out || (c > 0 && !X)) throw InvalidArgument("cs_lmm_create: null argument");
         if (n < 2 || c < 0 || c >= n || (c > 0 && ldx < n))
             throw RuntimeError("EigenVce::prepare: Dimension mismatch");
-        check_dense_eigen_size(n, "cs_lmm_create");
         auto h = std::make_unique<cs_lmm>();
         h->n = n;
         h->c = c;
@@ -327,9 +334,8 @@ int cs_lmm_scan_rotated(cs_lmm* h, cs_bed* bed, const int32_t* keep, int64_t n_k
                                                 h->vinv.as<double>(), h->py.as<double>());
         CS_HIP(hipGetLastError());
         const double one = 1.0, zero = 0.0;
-        CS_BLAS(rocblas_dgemv(h->blas, rocblas_operation_none, (int)n, (int)ne, &one,
-                              h->R.as<double>() + n, (int)n, h->py.as<double>(), 1, &zero,
-                              h->R.as<double>(), 1));
+        CS_BLAS(rocblas_dgemv_64(h->blas, rocblas_operation_none, n, ne, &one, h->R.as<double>() + n, n,
+                                 h->py.as<double>(), 1, &zero, h->R.as<double>(), 1));
         std::vector<double> w((size_t)ne);
         CS_HIP(hipMemcpyAsync(w.data(), h->vinv.p, (size_t)ne * 8, hipMemcpyDeviceToHost, h->st));
         CS_HIP(hipStreamSynchronize(h->st));

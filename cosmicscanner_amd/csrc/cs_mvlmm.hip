@@ -1039,7 +1039,6 @@ int cs_mvlmm_create(const double* Y, int64_t ldy, int64_t n, int64_t d, const fl
         if (d < 1 || d > kMvMaxD) throw InvalidArgument("multivariate LMM: 1 to 6 traits are supported");
         if (n < 2 || c < 0 || c >= n || ldy < n || (c > 0 && ldx < n))
             throw RuntimeError("MvEigenVce::prepare: Dimension mismatch");
-        check_dense_eigen_size(n, "cs_mvlmm_create");
         auto h = std::make_unique<cs_mvlmm>();
         h->n = n;
         h->c = c;
@@ -1101,8 +1100,14 @@ int cs_mvlmm_prepare_dense(cs_mvlmm* h, const double* G, int64_t ldg, int g_on_d
         }
         // eigendecomposition of A^T G A (ne x ne at offset (c, c), ld n); U overwrites it
         double* U = Gw + c * n + c;
-        CS_MVBLAS(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)ne, U, (int)n,
-                                   h->D.as<double>(), h->E.as<double>(), h->info.as<rocblas_int>()));
+        const bool large = eig_use_large(ne);
+        if (large) {
+            sym_eig_large(h->blas, h->st, ne, U, n, h->D.as<double>(), eig_large_leaf());
+            CS_HIP(hipMemsetAsync(h->info.p, 0, sizeof(rocblas_int), h->st));
+        } else {
+            CS_MVBLAS(rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_lower, (int)ne, U, (int)n,
+                                       h->D.as<double>(), h->E.as<double>(), h->info.as<rocblas_int>()));
+        }
         std::vector<double> Dh((size_t)ne);
         CS_HIP(hipMemcpyAsync(Dh.data(), h->D.p, (size_t)ne * 8, hipMemcpyDeviceToHost, h->st));
         CS_HIP(hipStreamSynchronize(h->st));
@@ -1110,15 +1115,16 @@ int cs_mvlmm_prepare_dense(cs_mvlmm* h, const double* G, int64_t ldg, int g_on_d
             if (x < 0) x = 0;
         CS_HIP(hipMemcpyAsync(h->D.p, Dh.data(), (size_t)ne * 8, hipMemcpyHostToDevice, h->st));
         // Uty = U^T (A^T Y) (ne x d)
-        CS_MVBLAS(rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, (int)ne, (int)d,
-                                (int)ne, &one, U, (int)n, h->t.as<double>() + c, (int)n, &zero, h->Uty.as<double>(),
-                                (int)ne));
+        CS_MVBLAS(rocblas_dgemm_64(h->blas, rocblas_operation_transpose, rocblas_operation_none, ne, d, ne, &one, U,
+                                   n, h->t.as<double>() + c, n, &zero, h->Uty.as<double>(), ne));
         // W_proj = A U = Q [0; U] (n x ne)
         double* W = h->W.as<double>();
         if (c > 0) CS_HIP(hipMemset2DAsync(W, (size_t)n * 8, 0, (size_t)c * 8, (size_t)ne, h->st));
         CS_HIP(hipMemcpy2DAsync(W + c, (size_t)n * 8, U, (size_t)n * 8, (size_t)ne * 8, (size_t)ne,
                                 hipMemcpyDeviceToDevice, h->st));
-        if (c > 0)
+        if (c > 0 && large)
+            apply_reflectors(h->blas, h->st, h->Q.as<double>(), n, h->tau_h.data(), n, c, false, W, n, ne);
+        else if (c > 0)
             CS_MVBLAS(rocsolver_dormqr(h->blas, rocblas_side_left, rocblas_operation_none, (int)n, (int)ne, (int)c,
                                        h->Q.as<double>(), (int)n, h->tau.as<double>(), W, (int)n));
         rocblas_int info = 0;

@@ -19,18 +19,10 @@ inline int grid_for(int64_t total, int block = 256) {
     return (int)g;
 }
 
-// Column tiling of an int8 product: k right-hand sides in groups of 16 (G = ceil(k/16) groups,
-// one per grid.z slice), each group as L limb tiles of 16 columns (NT = G*L tiles in B).
-// rocSOLVER's dorgqr / dsyevd exist only with 32-bit (rocblas_int) interfaces, and dsyevd reports
-// failures as row (n+1) + col in one rocblas_int; the dense eigen paths therefore keep every
-// n x n work array and the divide-and-conquer workspace (LAPACK's 1 + 6 n + 2 n^2 elements) under
-// 2^31 elements: n <= 32766 samples.
+// rocSOLVER's dorgqr / dormqr / dsyevd exist only with 32-bit (rocblas_int) interfaces and keep
+// their n x n work arrays (LAPACK's 1 + 6 n + 2 n^2 elements for dsyevd) under 2^31 elements:
+// n <= 32766.  Larger dense eigendecompositions take sym_eig_large.
 constexpr int64_t kDenseEigenMaxN = 32766;
-inline void check_dense_eigen_size(int64_t n, const char* who) {
-    if (n > kDenseEigenMaxN)
-        throw InvalidArgument(std::string(who) + ": the dense eigendecomposition path supports at most 32766 samples "
-                              "(rocSOLVER's 32-bit indexing of the n x n work arrays); use the matrix-free path");
-}
 
 // Whether the genotype products run the LDS-shared kernels (CSAMD_GENO_LDSB unset or not 0).
 bool geno_lds_enabled();
@@ -127,7 +119,20 @@ std::vector<int64_t> pscan_select(const double* log10p, const int32_t* pass, int
 void pscan_exact(const PscanProj& pj, int64_t S, const int64_t* rows, const double* dG, const double* dZg,
                  const double* dAux, bool aux_is_residual, double* beta, double* se, double* p, double* log10p,
                  bool device_out, hipStream_t st, rocblas_handle blas);
+// Dense symmetric eigendecomposition on 64-bit interfaces (cs_eig.hip): A (n x n, ld lda, lower
+// triangle, device) is overwritten by the eigenvectors, D (device) by the eigenvalues in ascending
+// order; tridiagonal blocks of at most `leaf` rows go to rocSOLVER's dstedc.
+void sym_eig_large(rocblas_handle h, hipStream_t st, int64_t n, double* A, int64_t lda, double* D, int64_t leaf);
+// E (m x ncols) <- Q E or Q^T E for Q = H(0) ... H(c-1), H(i) = I - tau_i v_i v_i^T, v_i zero above
+// row i, 1 on it, V(r, i) below (the geqrf layout); tau on the host.
+void apply_reflectors(rocblas_handle h, hipStream_t st, const double* V, int64_t ldv, const double* tau_h, int64_t m,
+                      int64_t c, bool transpose, double* E, int64_t lde, int64_t ncols);
 #endif
+// Leaf size of the 64-bit eigen path forced on every dense eigendecomposition by
+// CSAMD_EIG_LARGE=<leaf> (0: unset -- the path runs only beyond rocSOLVER's limits).
+int64_t eig_large_leaf();
+// Whether a dense eigendecomposition of order n takes the 64-bit path.
+inline bool eig_use_large(int64_t n) { return n > kDenseEigenMaxN || eig_large_leaf() > 0; }
 // Exact-stage threshold of the projected scans on the largest -log10 P over T traits: the given
 // level (0 -> the default 5) plus log10 T, i.e. the smallest P Bonferroni-adjusted over the traits
 // reaches 10^-level; < 0 -> -1 (every tested SNP); +inf -> off; NaN is rejected.

@@ -393,9 +393,19 @@ int cs_server_release(cs_server* srv, int64_t id);
  * intercept included -- univariate_model.cpp:368-372) over the kept samples.  With c > 0 the
  * covariates are projected out (EigenVce with use_covariate_projection, the default): the
  * Householder QR of X is computed once, and every prepare applies its c reflectors to G.
- * n <= 32766: rocSOLVER's dsyevd has only 32-bit interfaces, and larger n x n work arrays
- * exceed them (CS_ERR_INVALID_ARGUMENT). */
+ * Up to n - c = 32766 the eigendecomposition is rocSOLVER's dsyevd; beyond (rocSOLVER's 32-bit
+ * interfaces) it is cs_sym_eigen_device's own 64-bit path, which needs about 6 n^2 doubles of
+ * device memory in all. */
 int cs_lmm_create(const double* y, const float* X, int64_t ldx, int64_t n, int64_t c, cs_lmm** out);
+/* Dense symmetric eigendecomposition of the n x n device matrix A (lower triangle, leading
+ * dimension lda): A is overwritten by the orthonormal eigenvectors, D (n doubles, device) by the
+ * eigenvalues in ascending order -- the SelfAdjointEigenSolver step of EigenVce::prepare and
+ * MvEigenVce::prepare (libCosmicCore/src/eigen_vce.cpp:34-38, 520-526).  leaf = 0: rocSOLVER's
+ * dsyevd (n <= 32766).  leaf in 1..32766: the 64-bit path -- blocked Householder tridiagonal
+ * reduction, divide and conquer over tridiagonal blocks of at most `leaf` rows (rocSOLVER's
+ * dstedc on the blocks, rank-one merges with deflation and Gu-Eisenstat eigenvectors), compact-WY
+ * back-transformation; any n that device memory holds (about 6 n^2 doubles).  Synchronous. */
+int cs_sym_eigen_device(double* A, int64_t n, int64_t lda, double* D, int64_t leaf);
 int cs_lmm_destroy(cs_lmm* lmm);
 /* EigenVce::prepare (libCosmicCore/src/eigen_vce.cpp:19-49) for an n x n GRM G over the kept
  * samples (host memory, or device memory when g_on_device != 0): G_reml = A^T G A,
@@ -492,8 +502,8 @@ int cs_gwas_matrix_free_run(const char* bfile, const double* y, const double* co
  * projection (the defaults of MultivariateModel, src/models/multivariate_model.cpp:870-1110).
  *
  * cs_mvlmm_create: Y (n x d, column-major, ldy) and covariates X (n x c float32, intercept
- * included) over the kept samples; the Householder QR of X is computed once.  n <= 32766, as
- * cs_lmm_create. */
+ * included) over the kept samples; the Householder QR of X is computed once.  The
+ * eigendecomposition takes the 64-bit path beyond n - c = 32766, as cs_lmm_create. */
 int cs_mvlmm_create(const double* Y, int64_t ldy, int64_t n, int64_t d, const float* X, int64_t ldx,
                     int64_t c, cs_mvlmm** out);
 int cs_mvlmm_destroy(cs_mvlmm* mv);

@@ -38,17 +38,18 @@ def test_error_path_without_gpu_calls():
     assert b"null" in L.cs_last_error()
 
 
-def test_dense_eigen_size_guard_without_gpu_calls():
-    """The dense eigen paths refuse n > 32766 before any device call (rocSOLVER's 32-bit
-    dorgqr / dsyevd interfaces cannot address larger n x n work arrays)."""
+def test_dense_eigen_argument_checks_without_gpu_calls():
+    """cs_sym_eigen_device validates its arguments before any device call: the rocSOLVER route
+    (leaf 0) is limited to n <= 32766, the 64-bit route takes leaf sizes 1..32766."""
     import cosmicscanner_amd as csa
-    n = 40000
-    y = np.zeros(n)
-    X = np.ones((n, 1), dtype=np.float32)
-    with pytest.raises(ValueError, match="32766"):
-        csa.DenseLmm(y, X)
-    with pytest.raises(ValueError, match="32766"):
-        csa.MvLmm(np.zeros((n, 2)), X)
+    L = csa.lib()
+    fake = ctypes.c_void_p(16)
+    assert L.cs_sym_eigen_device(None, 10, 10, None, 0) == csa.CS_ERR_INVALID_ARGUMENT
+    assert L.cs_sym_eigen_device(fake, 10, 9, fake, 0) == csa.CS_ERR_INVALID_ARGUMENT
+    assert L.cs_sym_eigen_device(fake, 10, 10, fake, -1) == csa.CS_ERR_INVALID_ARGUMENT
+    assert L.cs_sym_eigen_device(fake, 10, 10, fake, 40000) == csa.CS_ERR_INVALID_ARGUMENT
+    assert b"leaf" in L.cs_last_error()
+    assert L.cs_sym_eigen_device(fake, 0, 0, fake, 0) == 0     # empty: nothing to do
 
 
 def test_validate_variance_components():
