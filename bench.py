@@ -546,10 +546,10 @@ def main():
     ap.add_argument("--mv", action="store_true", help="also run the multi-trait LMM leg")
     ap.add_argument("--lowrank", action="store_true",
                     help="also run the low-rank matrix-free GWAS leg (the reference path, headline genotypes)")
-    # the dense eigen path (MvEigenVce::prepare) is bounded by rocSOLVER's 32-bit interfaces at
-    # n <= 32766 individuals; the multi-trait workload names 50,000
     ap.add_argument("--mv-n", type=int, default=50_000)
     ap.add_argument("--mv-m", type=int, default=300_000)
+    ap.add_argument("--grm-only", action="store_true",
+                    help="only the n=50k x m=500k GRM legs (exact and standardised; for counter runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 2 GiB device copy 3 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -570,6 +570,14 @@ def main():
         return 2
 
     import cosmicscanner_amd as csa
+
+    if args.grm_only:
+        reader2 = csa.PlinkReader.synthetic(20261204, 50_000, 500_000, 0.01, 0.5, 0.005)
+        grm = grm_leg(csa, reader2, 50_000, 500_000)
+        grm["standardised"] = grm_std_leg(csa, reader2, 50_000, 500_000)
+        reader2.close()
+        print(json.dumps({"grm": grm}), flush=True)
+        return 0
 
     N, M, k = args.n, args.m, args.k
     seed = 20261205
