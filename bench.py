@@ -145,8 +145,10 @@ def kv_leg(csa, reader, N, M, k, steps, warmup, prec="int8_l3", missing_mode=0, 
 
 
 def fp32_leg(csa, reader, N, M, k, reps=2):
-    """The plain FP32 GPU comparator: the same K.V with genotypes decoded to FP32 and FP32 FMAs (no
-    MFMA, no split)."""
+    """The plain FP32 GPU comparator: the same K.V with genotypes decoded to FP32 values and FP32
+    FMAs on the vector ALUs (no MFMA, no split): per-SNP value tables, right-hand sides staged in
+    LDS and reused over four rows per lane, one FMA per genotype and column in Z^T V, two in Z W.
+    Reported with its FP32 rate (3 n m k FMAs per apply)."""
     op = csa.PlinkKinshipOperator(reader)
     op.set_precision(csa.CS_PREC_FP32)
     X = csa.DeviceArray.from_host(np.random.default_rng(7).standard_normal((N, k)))
@@ -161,8 +163,9 @@ def fp32_leg(csa, reader, N, M, k, reps=2):
     X.close()
     Y.close()
     op.close()
-    log(f"K.V plain FP32: {dt * 1e3:.2f} ms/step")
-    return dict(ms_per_step=dt * 1e3, genotypes_per_s=N * M / dt)
+    tflops = 2.0 * 3.0 * N * M * k / dt / 1e12
+    log(f"K.V plain FP32: {dt * 1e3:.2f} ms/step, {tflops:.1f} TFLOP/s")
+    return dict(ms_per_step=dt * 1e3, genotypes_per_s=N * M / dt, fp32_tflops=tflops)
 
 
 def grm_leg(csa, reader, N, M):
@@ -658,6 +661,7 @@ def main():
         "small_k_apply_ms": kv.get("small_k_apply_ms"),
         "fp32_gpu_genotypes_per_s": fp32["genotypes_per_s"] if fp32 else None,
         "speedup_vs_fp32_gpu": (value / fp32["genotypes_per_s"]) if fp32 else None,
+        "fp32_gpu_tflops": fp32["fp32_tflops"] if fp32 else None,
         "roofline": kv["roofline"],
         "loco_scan": locores,
         "lowrank_scan": lowres,

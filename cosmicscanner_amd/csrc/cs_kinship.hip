@@ -32,7 +32,7 @@ struct cs_kinship {
     hipStream_t stream = nullptr;
     csamd::DevBuf d_keep, d_inv_off, d_inv_idx, d_mult;
     // workspaces (grown on demand)
-    csamd::DevBuf dX, dY, Vt, BV, W1, W2, B1, B2, Yint, mxW, mx, scl, Vf, PQf, Wf;
+    csamd::DevBuf dX, dY, Vt, BV, W1, W2, B1, B2, Yint, mxW, mx, scl, Vf, PQf, Wf, Yf;
     csamd::DevBuf Vp, B2p, S1, S2, qext;   // sparse-missing form: plain quantised operands, sums
     csamd::DevBuf diag_tab, diag_full;
     std::vector<double> diag_cache;  // diag(K) over the rows, once computed (fixed by keep / variants)
@@ -361,56 +361,196 @@ __global__ void k_finalize(const long long* __restrict__ yint, int64_t k,
 }
 
 // ------------------------------------------------------------------------------------------
-// Plain FP32 comparator: decode to FP32 and FMA on the vector ALUs (no MFMA, no split), over
-// the same tiled copies.  One wave per 64-row tile; lane (g, r) accumulates row 16m + r over its
-// 16-byte chunk g of every 256-k block, reading the right-hand sides (fp32 [kpad][KC]) from
-// cache; the four chunk lanes are reduced by shuffles.
+// Plain FP32 comparator: the same K.V with genotypes decoded to FP32 values and FP32 FMAs on the
+// vector ALUs (no MFMA, no fixed-point split), over the same tiled copies.  One wave per 64-row
+// tile; lane (g, r) holds rows 16 m + r (m = 0..3) and the 64 k positions of its 16-byte chunk g
+// of every 256-k block, so each right-hand-side value read from LDS feeds four rows.  The
+// right-hand sides of a k-block are staged in LDS (double-buffered) for the workgroup; the four
+// chunk lanes of a row are reduced by shuffles at the end.
+//
+// ZᵀV (rows = SNPs): a per-SNP table z[code] = {(2-mu)s, 0, (1-mu)s, -mu s} gives the
+// standardised value, one FMA per column.  Z W (rows = samples): y = X~ W1 - O W2 with
+// W1 = w, W2 = mu w per SNP, two FMAs per column; split along the SNPs into partial rows.
+typedef float f32x4n __attribute__((ext_vector_type(4)));   // native vector: stays in registers
+
 template <int KC>
-__global__ __launch_bounds__(256) void k_rows_f32(const uint8_t* __restrict__ A, int64_t KB,
-                                                  int64_t rows, const float* __restrict__ Bx,
-                                                  const float* __restrict__ Bo,
-                                                  float* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_f32_zt(const uint8_t* __restrict__ A, int64_t KB, int64_t rows,
+                                                const double* __restrict__ means,
+                                                const double* __restrict__ inv_stds,
+                                                const f32x4n* __restrict__ V, float* __restrict__ U) {
+    constexpr int Q = (KC + 3) / 4;                 // f32x4n per k position
+    constexpr int CH = 256 * Q;                     // f32x4n per k-block
+    __shared__ f32x4n lb[2][CH];
     const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
     const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const uint8_t* base = A + ((tr * KB) << 12) + lane * 16;
+    float t[4][4];
+#pragma unroll
     for (int m = 0; m < 4; ++m) {
-        float ax[KC], ao[KC];
+        const int64_t row = tr * 64 + 16 * m + r;
+        float mu = 0.f, sd = 0.f;
+        if (row < rows && inv_stds[row] != 0.0) {
+            mu = (float)means[row];
+            sd = (float)inv_stds[row];
+        }
+        t[m][0] = (2.f - mu) * sd;
+        t[m][1] = 0.f;
+        t[m][2] = (1.f - mu) * sd;
+        t[m][3] = -mu * sd;
+    }
+    float acc[4][4 * Q];
 #pragma unroll
-        for (int c = 0; c < KC; ++c) { ax[c] = 0.f; ao[c] = 0.f; }
-        for (int64_t kb = 0; kb < KB; ++kb) {
-            const uint4 w4 = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
-            const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
-            const int64_t k0 = kb * 256 + g * 64;
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+        for (int c = 0; c < 4 * Q; ++c) acc[m][c] = 0.f;
+    const uint8_t* base = A + ((tr * KB) << 12) + lane * 16;
+    f32x4n stg[(CH + 255) / 256];
+    auto load = [&](int64_t kb) {
+#pragma unroll
+        for (int i = 0; i < (CH + 255) / 256; ++i) {
+            const int e = (int)threadIdx.x + 256 * i;
+            if (e < CH) stg[i] = V[kb * CH + e];
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < (CH + 255) / 256; ++i) {
+            const int e = (int)threadIdx.x + 256 * i;
+            if (e < CH) lb[buf][e] = stg[i];
+        }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int64_t kb = 0; kb < KB; ++kb) {
+        const int cur = (int)(kb & 1);
+        if (kb + 1 < KB) load(kb + 1);
+        uint4 a[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a[m] = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t w[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) w[m] = q == 0 ? a[m].x : q == 1 ? a[m].y : q == 2 ? a[m].z : a[m].w;
 #pragma unroll 4
-                for (int u = 0; u < 16; ++u) {
-                    const uint32_t code = (ws[q] >> (2 * u)) & 3u;
-                    const float x = code == 0 ? 2.f : (code == 2 ? 1.f : 0.f);
-                    const float o = code == 1 ? 0.f : 1.f;
-                    const float* bx = Bx + (k0 + 16 * q + u) * KC;
-                    const float* bo = Bo + (k0 + 16 * q + u) * KC;
+            for (int u = 0; u < 16; ++u) {
+                const f32x4n* bp = &lb[cur][(64 * g + 16 * q + u) * Q];
+                f32x4n bv[Q];
 #pragma unroll
-                    for (int c = 0; c < KC; ++c) {
-                        ax[c] = __builtin_fmaf(x, bx[c], ax[c]);
-                        ao[c] = __builtin_fmaf(o, bo[c], ao[c]);
+                for (int c = 0; c < Q; ++c) bv[c] = bp[c];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const uint32_t code = (w[m] >> (2 * u)) & 3u;
+                    const float z = (code & 1u) ? ((code & 2u) ? t[m][3] : t[m][1]) : ((code & 2u) ? t[m][2] : t[m][0]);
+#pragma unroll
+                    for (int c = 0; c < Q; ++c) {
+                        acc[m][4 * c] = __builtin_fmaf(z, bv[c].x, acc[m][4 * c]);
+                        acc[m][4 * c + 1] = __builtin_fmaf(z, bv[c].y, acc[m][4 * c + 1]);
+                        acc[m][4 * c + 2] = __builtin_fmaf(z, bv[c].z, acc[m][4 * c + 2]);
+                        acc[m][4 * c + 3] = __builtin_fmaf(z, bv[c].w, acc[m][4 * c + 3]);
                     }
                 }
-        }
-#pragma unroll
-        for (int c = 0; c < KC; ++c)
-            for (int off = 16; off < 64; off <<= 1) {
-                ax[c] += __shfl_xor(ax[c], off);
-                ao[c] += __shfl_xor(ao[c], off);
             }
+        }
+        if (kb + 1 < KB) store(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            acc[m][c] += __shfl_xor(acc[m][c], 16);
+            acc[m][c] += __shfl_xor(acc[m][c], 32);
+        }
         const int64_t row = tr * 64 + 16 * m + r;
-        if (g == 0 && row < rows) {
+        if (g == 0)
 #pragma unroll
-            for (int c = 0; c < KC; ++c) {
-                out[(row * 2) * KC + c] = ax[c];
-                out[(row * 2 + 1) * KC + c] = ao[c];
+            for (int c = 0; c < KC; ++c) U[row * KC + c] = acc[m][c];
+    }
+}
+
+// Z W over SNP split blockIdx.y (kb_per_split k-blocks each): partial rows P[split][row][KC].
+template <int KC>
+__global__ __launch_bounds__(256) void k_f32_z(const uint8_t* __restrict__ A, int64_t KB, int64_t kb_per_split,
+                                               int64_t rows_pad, const f32x4n* __restrict__ W12,
+                                               float* __restrict__ P) {
+    constexpr int Q = (KC + 3) / 4;
+    constexpr int CH = 256 * 2 * Q;                 // W1 and W2 of 256 SNPs
+    __shared__ f32x4n lb[2][CH];
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t kb0 = (int64_t)blockIdx.y * kb_per_split;
+    const int64_t kb1 = kb0 + kb_per_split < KB ? kb0 + kb_per_split : KB;
+    float acc[4][4 * Q];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int c = 0; c < 4 * Q; ++c) acc[m][c] = 0.f;
+    const uint8_t* base = A + ((tr * KB) << 12) + lane * 16;
+    f32x4n stg[CH / 256];
+    auto load = [&](int64_t kb) {
+#pragma unroll
+        for (int i = 0; i < CH / 256; ++i) stg[i] = W12[kb * CH + (int)threadIdx.x + 256 * i];
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < CH / 256; ++i) lb[buf][(int)threadIdx.x + 256 * i] = stg[i];
+    };
+    if (kb0 < kb1) {
+        load(kb0);
+        store(0);
+    }
+    __syncthreads();
+    for (int64_t kb = kb0; kb < kb1; ++kb) {
+        const int cur = (int)((kb - kb0) & 1);
+        if (kb + 1 < kb1) load(kb + 1);
+        uint4 a[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a[m] = *reinterpret_cast<const uint4*>(base + (kb << 12) + m * 1024);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t w[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) w[m] = q == 0 ? a[m].x : q == 1 ? a[m].y : q == 2 ? a[m].z : a[m].w;
+#pragma unroll 2
+            for (int u = 0; u < 16; ++u) {
+                const f32x4n* bp = &lb[cur][(64 * g + 16 * q + u) * 2 * Q];
+                f32x4n b1[Q], b2[Q];
+#pragma unroll
+                for (int c = 0; c < Q; ++c) {
+                    b1[c] = bp[c];
+                    b2[c] = bp[Q + c];
+                }
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const uint32_t code = (w[m] >> (2 * u)) & 3u;
+                    const float x = code == 0u ? 2.f : (code == 2u ? 1.f : 0.f);
+                    const float o = code == 1u ? 0.f : -1.f;
+#pragma unroll
+                    for (int c = 0; c < Q; ++c) {
+                        acc[m][4 * c] = __builtin_fmaf(o, b2[c].x, __builtin_fmaf(x, b1[c].x, acc[m][4 * c]));
+                        acc[m][4 * c + 1] = __builtin_fmaf(o, b2[c].y, __builtin_fmaf(x, b1[c].y, acc[m][4 * c + 1]));
+                        acc[m][4 * c + 2] = __builtin_fmaf(o, b2[c].z, __builtin_fmaf(x, b1[c].z, acc[m][4 * c + 2]));
+                        acc[m][4 * c + 3] = __builtin_fmaf(o, b2[c].w, __builtin_fmaf(x, b1[c].w, acc[m][4 * c + 3]));
+                    }
+                }
             }
         }
+        if (kb + 1 < kb1) store(cur ^ 1);
+        __syncthreads();
+    }
+    float* out = P + (int64_t)blockIdx.y * rows_pad * KC;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            acc[m][c] += __shfl_xor(acc[m][c], 16);
+            acc[m][c] += __shfl_xor(acc[m][c], 32);
+        }
+        const int64_t row = tr * 64 + 16 * m + r;
+        if (g == 0)
+#pragma unroll
+            for (int c = 0; c < KC; ++c) out[row * KC + c] = acc[m][c];
     }
 }
 
@@ -425,38 +565,39 @@ __global__ void k_to_f32_cols(const double* __restrict__ src, int64_t rows, int6
     }
 }
 
-__global__ void k_epi_gt_f32(const float* __restrict__ PQ, int KC, int64_t M, int64_t Mpad,
-                             int64_t k, const double* __restrict__ means,
-                             const double* __restrict__ inv_stds, const int32_t* __restrict__ mult,
-                             float inv_M, float* __restrict__ W1, float* __restrict__ W2) {
+// W1 = mult s u / M and W2 = mu W1 per SNP, interleaved per 256-SNP block as [kb][j][W1 | W2].
+__global__ void k_epi_w_f32(const float* __restrict__ U, int KC, int64_t M, int64_t Mpad, int64_t k,
+                            const double* __restrict__ means, const double* __restrict__ inv_stds,
+                            const int32_t* __restrict__ mult, float inv_M, float* __restrict__ W12) {
     const int64_t total = Mpad * KC;
+    const int Qc = (KC + 3) / 4 * 4;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = e / KC;
         const int c = (int)(e - j * KC);
         float w1 = 0.f, w2 = 0.f;
         if (j < M && c < k && mult[j] != 0 && inv_stds[j] != 0.0) {
-            const float s = (float)inv_stds[j], mu = (float)means[j];
-            const float u = (PQ[(j * 2) * KC + c] - mu * PQ[(j * 2 + 1) * KC + c]) * s;
-            const float w = (float)mult[j] * s * u * inv_M;
-            w1 = w;
-            w2 = mu * w;
+            w1 = (float)mult[j] * (float)inv_stds[j] * U[j * KC + c] * inv_M;
+            w2 = (float)means[j] * w1;
         }
-        W1[e] = w1;
-        W2[e] = w2;
+        float* o = W12 + j * 2 * Qc;
+        o[c] = w1;
+        o[Qc + c] = w2;
     }
 }
 
-__global__ void k_epi_g_f32(const float* __restrict__ PQ, int KC, int64_t k,
-                            const int32_t* __restrict__ keep, int64_t n, double* __restrict__ Y,
-                            int64_t ldy) {
+// Y[r, c] = sum over the splits of P[split][keep[r]][c], in split order.
+__global__ void k_epi_y_f32(const float* __restrict__ P, int64_t nsplit, int64_t rows_pad, int KC, int64_t k,
+                            const int32_t* __restrict__ keep, int64_t n, double* __restrict__ Y, int64_t ldy) {
     const int64_t total = n * k;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = e / k;
         const int c = (int)(e - r * k);
         const int64_t s = keep ? keep[r] : r;
-        Y[r + c * ldy] = (double)(PQ[(s * 2) * KC + c] - PQ[(s * 2 + 1) * KC + c]);
+        float acc = 0.f;
+        for (int64_t p = 0; p < nsplit; ++p) acc += P[(p * rows_pad + s) * KC + c];
+        Y[r + c * ldy] = (double)acc;
     }
 }
 
@@ -999,29 +1140,41 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
 }
 
 template <int KC>
-static void apply_f32_cols(cs_kinship* op, int64_t k) {
+static void apply_f32_cols(cs_kinship* op, int64_t k, double* dY, int64_t ldy) {
     cs_bed* b = op->bed;
     hipStream_t st = op->stream;
-    op->Vf.ensure((size_t)(b->Npad * KC) * sizeof(float));
-    k_to_f32_cols<<<grid_for(b->Npad * KC), 256, 0, st>>>(op->Vt.as<double>(), b->Npad, k, KC,
-                                                         op->Vf.as<float>());
+    constexpr int Qc = (KC + 3) / 4 * 4;
+    op->Vf.ensure((size_t)(b->Npad * Qc) * sizeof(float));
+    k_to_f32_cols<<<grid_for(b->Npad * Qc), 256, 0, st>>>(op->Vt.as<double>(), b->Npad, k, Qc, op->Vf.as<float>());
     CS_HIP(hipEventRecord(op->ev[1], st));
-    op->PQf.ensure((size_t)(std::max(b->Mpad, b->Npad) * 2 * KC) * sizeof(float));
-    k_rows_f32<KC><<<(unsigned)(b->Mpad / 256), 256, 0, st>>>(
-        b->snp.as<uint8_t>(), b->KBs, b->M, op->Vf.as<float>(), op->Vf.as<float>(), op->PQf.as<float>());
+    op->Wf.ensure((size_t)(b->Mpad * 2 * Qc) * sizeof(float));
+    float* W12 = op->Wf.as<float>();   // per SNP [W1 | W2], Qc floats each
+    op->PQf.ensure((size_t)(b->Mpad * KC) * sizeof(float));   // u = Z^T V, Mpad x KC
+    k_f32_zt<KC><<<(unsigned)(b->Mpad / 256), 256, 0, st>>>(b->snp.as<uint8_t>(), b->KBs, b->M,
+                                                           b->means.as<double>(), b->inv_stds.as<double>(),
+                                                           reinterpret_cast<const f32x4n*>(op->Vf.as<float>()),
+                                                           op->PQf.as<float>());
     CS_HIP(hipGetLastError());
     CS_HIP(hipEventRecord(op->ev[2], st));
-    op->Wf.ensure((size_t)(2 * b->Mpad * KC) * sizeof(float));
-    float* W1 = op->Wf.as<float>();
-    float* W2 = W1 + b->Mpad * KC;
-    k_epi_gt_f32<<<grid_for(b->Mpad * KC), 256, 0, st>>>(
-        op->PQf.as<float>(), KC, b->M, b->Mpad, k, b->means.as<double>(), b->inv_stds.as<double>(),
-        op->d_mult.as<int32_t>(), (float)(1.0 / (double)op->Msel), W1, W2);
+    k_epi_w_f32<<<grid_for(b->Mpad * KC), 256, 0, st>>>(op->PQf.as<float>(), KC, b->M, b->Mpad, k,
+                                                       b->means.as<double>(), b->inv_stds.as<double>(),
+                                                       op->d_mult.as<int32_t>(), (float)(1.0 / (double)op->Msel), W12);
+    CS_HIP(hipGetLastError());
     CS_HIP(hipEventRecord(op->ev[3], st));
-    k_rows_f32<KC><<<(unsigned)(b->Npad / 256), 256, 0, st>>>(b->smp.as<uint8_t>(), b->KBm, b->N, W1,
-                                                              W2, op->PQf.as<float>());
+    // SNP splits: about 2048 workgroups, at least 8 k-blocks each
+    const int64_t rblocks = b->Npad / 256;
+    const int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>((2048 + rblocks - 1) / rblocks, b->KBm / 8));
+    const int64_t kbps = (b->KBm + nsplit - 1) / nsplit;
+    const int64_t ns = (b->KBm + kbps - 1) / kbps;
+    op->Yf.ensure((size_t)(ns * b->Npad * KC) * sizeof(float));
+    k_f32_z<KC><<<dim3((unsigned)rblocks, (unsigned)ns), 256, 0, st>>>(
+        b->smp.as<uint8_t>(), b->KBm, kbps, b->Npad, reinterpret_cast<const f32x4n*>(W12), op->Yf.as<float>());
     CS_HIP(hipGetLastError());
     CS_HIP(hipEventRecord(op->ev[4], st));
+    k_epi_y_f32<<<grid_for(op->n * k), 256, 0, st>>>(op->Yf.as<float>(), ns, b->Npad, KC, k,
+                                                    op->identity ? nullptr : op->d_keep.as<int32_t>(), op->n,
+                                                    dY, ldy);
+    CS_HIP(hipGetLastError());
 }
 
 static void apply_f32(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
@@ -1036,14 +1189,9 @@ static void apply_f32(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, 
         op->Vt.ensure((size_t)(b->Npad * kc) * sizeof(double));
         k_scatter<<<grid_for(b->Npad * kc), 256, 0, st>>>(dX + c0 * ldx, ldx, kc, inv_off, inv_idx,
                                                           b->N, b->Npad, op->Vt.as<double>());
-        int KC;
-        if (kc <= 1) { apply_f32_cols<1>(op, kc); KC = 1; }
-        else if (kc <= 4) { apply_f32_cols<4>(op, kc); KC = 4; }
-        else { apply_f32_cols<16>(op, kc); KC = 16; }
-        k_epi_g_f32<<<grid_for(op->n * kc), 256, 0, st>>>(
-            op->PQf.as<float>(), KC, kc, op->identity ? nullptr : op->d_keep.as<int32_t>(), op->n,
-            dY + c0 * ldy, ldy);
-        CS_HIP(hipGetLastError());
+        if (kc <= 1) apply_f32_cols<1>(op, kc, dY + c0 * ldy, ldy);
+        else if (kc <= 4) apply_f32_cols<4>(op, kc, dY + c0 * ldy, ldy);
+        else apply_f32_cols<16>(op, kc, dY + c0 * ldy, ldy);
         CS_HIP(hipEventRecord(op->ev[5], st));
     }
 }

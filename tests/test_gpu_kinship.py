@@ -123,6 +123,27 @@ def test_kinship_precision_modes_larger(csa, mode, tol):
         op.close()
 
 
+def test_kinship_fp32_comparator_widths(csa):
+    """The FP32 comparator's kernels for 1, 4 and 16 columns per pass (and several passes),
+    with a keep list that repeats samples and a variant selection, against the oracle."""
+    N, M = 1700, 3300
+    body = oracle.gen_hwe_bed(81, N, M, 0.01, 0.5, 0.01)
+    reader = csa.PlinkReader.from_body(body, N, M)
+    means, inv = oracle.stats(body, N, M)
+    rng = np.random.default_rng(9)
+    keep = rng.choice(N, 1400, replace=True).astype(np.int32)
+    snps = np.sort(rng.choice(M, 2900, replace=False)).astype(np.int32)
+    op = csa.PlinkKinshipOperator(reader, keep, snps)
+    op.set_precision(csa.CS_PREC_FP32)
+    for k in (1, 3, 5, 20):
+        X = rng.standard_normal((op.rows(), k))
+        ref = oracle.kinship_apply(body, N, means, inv, keep, snps, X)
+        err = colrel(op.apply(X), ref)
+        print(f"FP32 k {k}: column-relative error {err:.2e}", flush=True)
+        assert err < 2e-5
+    op.close()
+
+
 @pytest.mark.parametrize("mode,tol", [(2, 1e-3), (3, 2e-6), (4, 3e-8)])
 def test_kinship_packed_limbs_small_k(csa, mode, tol):
     """Up to 16 / L right-hand sides share one MFMA tile (limb l of column c in tile column
