@@ -20,6 +20,7 @@
 
 #include "cs_blas.h"
 #include "cs_common.h"
+#include "cs_golden.h"
 #include "cs_ops.h"
 
 namespace csamd {
@@ -76,28 +77,8 @@ __global__ __launch_bounds__(256) void k_reml_golden(const double* __restrict__ 
         ++evals;
         return 0.5 * (slog + n_eff * log(squad / n_eff) + n_eff);
     };
-    double lo = -5.0, hi = 5.0;
-    const double gr = (sqrt(5.0) + 1.0) / 2.0;
-    double c = hi - (hi - lo) / gr;
-    double d = lo + (hi - lo) / gr;
-    double fc = negll(pow(10.0, c));
-    double fd = negll(pow(10.0, d));
-    while (fabs(c - d) > 1e-5) {
-        if (fc < fd) {
-            hi = d;
-            d = c;
-            fd = fc;
-            c = hi - (hi - lo) / gr;
-            fc = negll(pow(10.0, c));
-        } else {
-            lo = c;
-            c = d;
-            fc = fd;
-            d = lo + (hi - lo) / gr;
-            fd = negll(pow(10.0, d));
-        }
-    }
-    const double best = pow(10.0, (lo + hi) / 2.0);
+    const GoldenSpec spec{(sqrt(5.0) + 1.0) / 2.0, true, 1e-5, -1, true};
+    const double best = pow(10.0, golden_section_min([&](double x) { return negll(pow(10.0, x)); }, -5.0, 5.0, spec));
     double slog, squad;
     reml_sums(D, Uty, ne, best, sh, slog, squad);
     if (threadIdx.x == 0) {

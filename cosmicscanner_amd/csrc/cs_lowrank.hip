@@ -33,6 +33,7 @@
 
 #include "cs_blas.h"
 #include "cs_common.h"
+#include "cs_golden.h"
 #include "cs_ops.h"
 
 namespace csamd {
@@ -374,19 +375,9 @@ int cs_lowrank_prepare(cs_kinship* K, const double* y, const float* X, int64_t l
         }
         double left = -12.0 + 24.0 * std::max(0, best_index - 1) / (double)(grid - 1);
         double right = -12.0 + 24.0 * std::min(grid - 1, best_index + 1) / (double)(grid - 1);
-        const double golden = (std::sqrt(5.0) - 1.0) / 2.0;
-        double cc = right - golden * (right - left), dd = left + golden * (right - left);
-        double fc = lowrank_objective(h.get(), cc, nullptr), fd = lowrank_objective(h.get(), dd, nullptr);
-        for (int it = 0; it < 80 && right - left > 1e-8; ++it) {
-            if (fc < fd) {
-                right = dd; dd = cc; fd = fc; cc = right - golden * (right - left);
-                fc = lowrank_objective(h.get(), cc, nullptr);
-            } else {
-                left = cc; cc = dd; fc = fd; dd = left + golden * (right - left);
-                fd = lowrank_objective(h.get(), dd, nullptr);
-            }
-        }
-        const double best_log_eta = 0.5 * (left + right);
+        const GoldenSpec spec{(std::sqrt(5.0) - 1.0) / 2.0, false, 1e-8, 80, false};
+        const double best_log_eta =
+            golden_section_min([&](double x) { return lowrank_objective(h.get(), x, nullptr); }, left, right, spec);
         double quadratic = 0.0;
         best = lowrank_objective(h.get(), best_log_eta, &quadratic);
         if (!std::isfinite(best)) throw RuntimeError("Low-rank REML optimization failed");
