@@ -202,6 +202,33 @@ def test_sparse_and_dense_missing_forms_bitwise(csa, miss):
     assert colrel(outs[1], ref) < 2e-6
 
 
+@pytest.mark.parametrize("prec,k,tol", [(3, 64, 2e-6), (3, 20, 2e-6), (2, 20, 1e-3), (2, 64, 1e-3)])
+def test_sparse_missing_form_precisions_bitwise(csa, prec, k, tol):
+    """The 16-quad list gathers (64 columns: 16 plain int32 quads per row) and the two-limb
+    genotype-only products (INT8_L2): sparse and dense forms bitwise equal, auto (mode 0, sparse
+    at 0.1 % missing) equal to both, and within the precision's bound of the oracle."""
+    N, M = 1800, 3100
+    r = csa.PlinkReader.synthetic(31, N, M, 0.01, 0.5, 0.001)
+    rng = np.random.default_rng(12)
+    keep = rng.choice(N, 1500, replace=True).astype(np.int32)
+    op = csa.PlinkKinshipOperator(r, keep, None)
+    op.set_precision(prec)
+    X = rng.standard_normal((keep.size, k))
+    outs = []
+    for mode in (1, 2, 0):
+        op.set_missing_mode(mode)
+        outs.append(op.apply(X))
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
+    body = r.download()
+    means, inv = oracle.stats(body, N, M)
+    ref = oracle.kinship_apply(body, N, means, inv, keep, None, X)
+    err = colrel(outs[1], ref)
+    print(f"prec {prec} k {k}: column-relative error {err:.2e}", flush=True)
+    assert err < tol
+    op.close()
+
+
 @pytest.mark.parametrize("k", [120, 250])
 def test_sparse_missing_form_wide_blocks_bitwise(csa, k):
     """Wide blocks take the full-wave list gathers: 120 columns (eight groups, 128 plain int32
