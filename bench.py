@@ -455,7 +455,7 @@ def mv_leg(csa, n=50_000, m=300_000, d=4, seed=20261207, n_check=32):
     return out
 
 
-def serving_leg(csa, n=100_000, m=1_000_000, n_requests=60, rate=5.0, seed=20261206, budget_gib=64.0):
+def serving_leg(csa, n=100_000, m=1_000_000, n_requests=200, rate=5.0, seed=20261206, budget_gib=64.0):
     """Poisson stream of scan requests (rate req/s) over a resident n x m matrix with an HBM budget
     for the requests' buffers; each request: a random cohort (log-uniform 10k..n), 1-64 traits,
     0-10 covariates (+ intercept), h2 ~ U(0.1, 0.8), 16 calibration SNPs, a scan of all m SNPs with
@@ -545,7 +545,7 @@ def main():
     ap.add_argument("--no-grm", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serving-requests", type=int, default=60)
+    ap.add_argument("--serving-requests", type=int, default=200)
     ap.add_argument("--mv", action="store_true", help="also run the multi-trait LMM leg")
     ap.add_argument("--lowrank", action="store_true",
                     help="also run the low-rank matrix-free GWAS leg (the reference path, headline genotypes)")
