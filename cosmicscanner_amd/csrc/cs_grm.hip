@@ -38,7 +38,7 @@ enum { OP_X = 0, OP_MD = 2 };
 enum { GE_T1 = 0, GE_C = 1 };
 
 constexpr uint32_t kLutMF = 0x0000FF00u;   // code 01 -> 0xFF
-constexpr int64_t kDigitChunkKB = 16384;   // 2^22 SNPs per digit-product launch
+constexpr int64_t kDigitChunkKB = 8192;   // 2^21 SNPs per fused digit-product launch (both orders)
 
 struct GrmArgs {
     const uint8_t* smp;    // masked tiled sample-major genotypes
@@ -140,6 +140,91 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 const int64_t idx = (i - a.I0) * a.Npad + k;
                 if (EPI == GE_T1) a.t1[idx] = acc[m][n][v];
                 else a.c[idx] += (long long)acc[m][n][v] * (1LL << a.shift);
+            }
+}
+
+// Both orders of one C digit product in one pass: acc += X~_i (M_k d) + (M_i d) X~_k, which is
+// the C_ik + C_ki the combine needs.  Each wave decodes its 64-row tile once per 64-k step into
+// both operand forms (the code extraction shared, then one perm per form), so a block decodes
+// each tile once for 32 MFMAs per wave instead of twice for 16; LDS holds both forms
+// (double-buffered, 64 KiB: two workgroups per CU).  With two products summed into one int32
+// accumulator (|x~ d| <= 256 each), a launch covers at most 2^21 SNPs.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_grm_cpass(GrmArgs a) {
+    __shared__ v4i lds[2][2][4][4][64];   // [buffer][form: X~, M d][tile][m-subtile][lane]
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, r = lane & 15;
+    const int2 blk = a.blocks[blockIdx.x];
+    const int wi = wave >> 1, wk = wave & 1;
+    const int64_t i0 = (int64_t)blk.x * 128 + 64 * wi;
+    const int64_t k0 = (int64_t)blk.y * 128 + 64 * wk;
+    const bool active = k0 < i0 + 64;
+    const bool sideA = wave < 2;
+    const int64_t trow = sideA ? (int64_t)blk.x * 128 + 64 * wave : (int64_t)blk.y * 128 + 64 * (wave - 2);
+    const uint8_t* tbase = a.smp + (((trow >> 6) * a.KB) << 12) + lane * 16;
+    v4i acc[4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[m][n] = v4i{0, 0, 0, 0};
+    uint4 cv[4], cn[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) cv[m] = *reinterpret_cast<const uint4*>(tbase + (a.kb0 << 12) + m * 1024);
+    int buf = 0;
+    for (int64_t kb = a.kb0; kb < a.kb1; ++kb) {
+        const int64_t kn = kb + 1 < a.kb1 ? kb + 1 : kb;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) cn[m] = *reinterpret_cast<const uint4*>(tbase + (kn << 12) + m * 1024);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint4 d = *reinterpret_cast<const uint4*>(a.digA + (((kb * 4 + t) * 4 + g) << 4));
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const uint32_t w = t == 0 ? cv[m].x : t == 1 ? cv[m].y : t == 2 ? cv[m].z : cv[m].w;
+                const uint32_t d0 = w & 0x03030303u, d1 = (w >> 2) & 0x03030303u;
+                const uint32_t d2 = (w >> 4) & 0x03030303u, d3 = (w >> 6) & 0x03030303u;
+                lds[buf][0][wave][m][lane] =
+                    v4i{(int)__builtin_amdgcn_perm(0u, kLutX, d0), (int)__builtin_amdgcn_perm(0u, kLutX, d1),
+                        (int)__builtin_amdgcn_perm(0u, kLutX, d2), (int)__builtin_amdgcn_perm(0u, kLutX, d3)};
+                lds[buf][1][wave][m][lane] = v4i{(int)(__builtin_amdgcn_perm(0u, kLutMF, d0) & d.x),
+                                                 (int)(__builtin_amdgcn_perm(0u, kLutMF, d1) & d.y),
+                                                 (int)(__builtin_amdgcn_perm(0u, kLutMF, d2) & d.z),
+                                                 (int)(__builtin_amdgcn_perm(0u, kLutMF, d3) & d.w)};
+            }
+            __syncthreads();
+            if (active) {
+                v4i BX[4], BM[4];
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    BX[n] = lds[buf][0][2 + wk][n][lane];
+                    BM[n] = lds[buf][1][2 + wk][n][lane];
+                }
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const v4i AX = lds[buf][0][wi][m][lane];
+                    const v4i AM = lds[buf][1][wi][m][lane];
+#pragma unroll
+                    for (int n = 0; n < 4; ++n) {
+                        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(AX, BM[n], acc[m][n], 0, 0, 0);
+                        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(AM, BX[n], acc[m][n], 0, 0, 0);
+                    }
+                }
+            }
+            buf ^= 1;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) cv[m] = cn[m];
+    }
+    if (!active) return;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 4; ++n)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int64_t i = i0 + 16 * m + 4 * g + v;
+                const int64_t k = k0 + 16 * n + r;
+                if (i >= a.N || k >= a.N || k > i || i < a.I0 || i >= a.I0 + a.P) continue;
+                a.c[(i - a.I0) * a.Npad + k] += (long long)acc[m][n][v] * (1LL << a.shift);
             }
 }
 
@@ -899,7 +984,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
         DevBuf t1((size_t)(P * Npad) * 4), cacc(nC ? (size_t)(P * Npad) * 8 : 1),
             dacc(useD ? (size_t)(P * Npad) * 16 : 1);
         float ms_pass = 0.f, ms_comb = 0.f;
-        int passes = 0;
+        int passes = 0, launches = 0;   // products (for the MFMA op count) and kernel launches
         double mfma_ops = 0;
         for (int64_t I0 = 0; I0 < Npad; I0 += P) {
             const int64_t rows = std::min<int64_t>(P, Npad - I0);
@@ -930,7 +1015,9 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             ga.kb1 = prm.kb_hi;
             k_grm_pass<OP_X, OP_X, GE_T1><<<nb, 256, 0, st>>>(ga);
             ++passes;
-            // digit products accumulate |x~ d| <= 256 per SNP in int32: chunks of <= 2^22 SNPs
+            ++launches;
+            // digit products, both orders in one int32 accumulator: |x~ d| <= 256 per SNP and
+            // order, so chunks of <= 2^21 SNPs
             for (int64_t c0 = prm.kb_lo; c0 < prm.kb_hi; c0 += kDigitChunkKB) {
                 GrmArgs gc = ga;
                 gc.kb0 = c0;
@@ -939,9 +1026,9 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
                     GrmArgs g2 = gc;
                     g2.shift = 8 * l;
                     g2.digA = g2.digB = ddig.as<uint8_t>() + (size_t)l * Mpad;
-                    k_grm_pass<OP_X, OP_MD, GE_C><<<nb, 256, 0, st>>>(g2);
-                    k_grm_pass<OP_MD, OP_X, GE_C><<<nb, 256, 0, st>>>(g2);
-                    passes += 2;
+                    k_grm_cpass<<<nb, 256, 0, st>>>(g2);
+                    passes += 2;   // two products' MFMA work
+                    ++launches;
                 }
             }
             unsigned long long* dhi = dacc.as<unsigned long long>();
@@ -995,7 +1082,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
         CS_HIP(hipStreamSynchronize(st));
         g_grm_timing.ms_passes = ms_pass;
         g_grm_timing.ms_combine = ms_comb;
-        g_grm_timing.passes = passes;
+        g_grm_timing.passes = launches;
         g_grm_timing.mfma_ops = mfma_ops;
     } catch (...) {
         (void)hipStreamSynchronize(st);
