@@ -34,8 +34,6 @@ namespace csamd {
 typedef __int128 i128;
 typedef unsigned __int128 u128;
 
-enum { OP_X = 0, OP_MD = 2 };
-enum { GE_T1 = 0, GE_C = 1 };
 
 constexpr uint32_t kLutMF = 0x0000FF00u;   // code 01 -> 0xFF
 constexpr int64_t kDigitChunkKB = 8192;   // 2^21 SNPs per fused digit-product launch (both orders)
@@ -46,33 +44,19 @@ struct GrmArgs {
     int64_t kb0, kb1;      // SNP-block range of this launch
     const int2* blocks;    // (bi, bk) 128-row block pairs, bk <= bi
     int64_t I0, P, N, Npad;
-    const uint8_t* digA;   // digit arrays [kb][t][g][16] (OP_MD operands)
-    const uint8_t* digB;
+    const uint8_t* digA;   // digit array [kb][t][g][16] of the C pass (the missing-indicator operand)
     int shift;             // 8 * limb
     int32_t* t1;           // panel accumulators, [P][Npad]
     long long* c;
 };
 
-template <int OP>
-__device__ __forceinline__ v4i grm_operand(uint32_t w, uint4 dig) {
-    const uint32_t d0 = w & 0x03030303u, d1 = (w >> 2) & 0x03030303u;
-    const uint32_t d2 = (w >> 4) & 0x03030303u, d3 = (w >> 6) & 0x03030303u;
-    if (OP == OP_X)
-        return v4i{(int)__builtin_amdgcn_perm(0u, kLutX, d0), (int)__builtin_amdgcn_perm(0u, kLutX, d1),
-                   (int)__builtin_amdgcn_perm(0u, kLutX, d2), (int)__builtin_amdgcn_perm(0u, kLutX, d3)};
-    return v4i{(int)(__builtin_amdgcn_perm(0u, kLutMF, d0) & dig.x),
-               (int)(__builtin_amdgcn_perm(0u, kLutMF, d1) & dig.y),
-               (int)(__builtin_amdgcn_perm(0u, kLutMF, d2) & dig.z),
-               (int)(__builtin_amdgcn_perm(0u, kLutMF, d3) & dig.w)};
-}
 
-// One 128 x 128 lower-triangle block per workgroup; wave (wi, wk) owns the 64 x 64 quadrant
+// T1 = X~ X~^T.  One 128 x 128 lower-triangle block per workgroup; wave (wi, wk) owns the 64 x 64 quadrant
 // rows i0 = 128 bi + 64 wi, columns k0 = 128 bk + 64 wk (4 x 4 MFMA tiles per step).  The block
 // needs four 64-row operand tiles (A rows of bi, B rows of bk); wave w decodes tile w once per
 // 64-k step into LDS (double-buffered, one barrier per step) and every wave reads its A and B
 // operands from there, so each tile is loaded and decoded once instead of twice.  Quadrants
 // above the diagonal still decode their tile but skip the MFMAs and the epilogue.
-template <int AM, int BM, int EPI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_grm_pass(GrmArgs a) {
     __shared__ v4i lds[2][4][4][64];   // [buffer][tile][m-subtile][lane], 32 KiB
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -85,8 +69,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     const bool sideA = wave < 2;
     const int64_t trow = sideA ? (int64_t)blk.x * 128 + 64 * wave : (int64_t)blk.y * 128 + 64 * (wave - 2);
     const uint8_t* tbase = a.smp + (((trow >> 6) * a.KB) << 12) + lane * 16;
-    const uint8_t* dig = sideA ? a.digA : a.digB;
-    const bool use_dig = sideA ? AM == OP_MD : BM == OP_MD;
     v4i acc[4][4];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -102,12 +84,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
         for (int m = 0; m < 4; ++m) cn[m] = *reinterpret_cast<const uint4*>(tbase + (kn << 12) + m * 1024);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            uint4 d = uint4{0, 0, 0, 0};
-            if (use_dig) d = *reinterpret_cast<const uint4*>(dig + (((kb * 4 + t) * 4 + g) << 4));
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const uint32_t w = t == 0 ? cv[m].x : t == 1 ? cv[m].y : t == 2 ? cv[m].z : cv[m].w;
-                lds[buf][wave][m][lane] = sideA ? grm_operand<AM>(w, d) : grm_operand<BM>(w, d);
+                const uint32_t d0 = w & 0x03030303u, d1 = (w >> 2) & 0x03030303u;
+                const uint32_t d2 = (w >> 4) & 0x03030303u, d3 = (w >> 6) & 0x03030303u;
+                lds[buf][wave][m][lane] =
+                    v4i{(int)__builtin_amdgcn_perm(0u, kLutX, d0), (int)__builtin_amdgcn_perm(0u, kLutX, d1),
+                        (int)__builtin_amdgcn_perm(0u, kLutX, d2), (int)__builtin_amdgcn_perm(0u, kLutX, d3)};
             }
             __syncthreads();
             if (active) {
@@ -137,9 +121,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
                 const int64_t i = i0 + 16 * m + 4 * g + v;
                 const int64_t k = k0 + 16 * n + r;
                 if (i >= a.N || k >= a.N || k > i || i < a.I0 || i >= a.I0 + a.P) continue;
-                const int64_t idx = (i - a.I0) * a.Npad + k;
-                if (EPI == GE_T1) a.t1[idx] = acc[m][n][v];
-                else a.c[idx] += (long long)acc[m][n][v] * (1LL << a.shift);
+                a.t1[(i - a.I0) * a.Npad + k] = acc[m][n][v];
             }
 }
 
@@ -1013,7 +995,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             CS_HIP(hipEventRecord(e0, st));
             ga.kb0 = prm.kb_lo;
             ga.kb1 = prm.kb_hi;
-            k_grm_pass<OP_X, OP_X, GE_T1><<<nb, 256, 0, st>>>(ga);
+            k_grm_pass<<<nb, 256, 0, st>>>(ga);
             ++passes;
             ++launches;
             // digit products, both orders in one int32 accumulator: |x~ d| <= 256 per SNP and
@@ -1025,7 +1007,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
                 for (int l = 0; l < nC; ++l) {
                     GrmArgs g2 = gc;
                     g2.shift = 8 * l;
-                    g2.digA = g2.digB = ddig.as<uint8_t>() + (size_t)l * Mpad;
+                    g2.digA = ddig.as<uint8_t>() + (size_t)l * Mpad;
                     k_grm_cpass<<<nb, 256, 0, st>>>(g2);
                     passes += 2;   // two products' MFMA work
                     ++launches;
