@@ -582,12 +582,13 @@ __global__ void k_build_v(const double* __restrict__ V, int64_t ldv, int64_t p, 
 }  // namespace
 
 // E (m x ncols, ld lde) <- Q E (transpose false) or Q^T E (true) for Q = H(0) ... H(c-1),
-// H(i) = I - tau_i v_i v_i^T with v_i as in k_build_v.  Blocks of 64 reflectors, each applied as
+// H(i) = I - tau_i v_i v_i^T with v_i as in k_build_v.  Blocks of 256 reflectors (with 64 the
+// GEMMs' inner and outer widths left them skinny, about 20 TFLOP/s), each applied as
 // E(p:, :) -= V T' (V^T E(p:, :)) with T the compact-WY factor of LAPACK's dlarft (forward,
 // column-wise) and T' = T or T^T: three FP64 GEMMs per block on the 64-bit rocBLAS interface.
 void apply_reflectors(rocblas_handle h, hipStream_t st, const double* V, int64_t ldv, const double* tau_h, int64_t m,
                       int64_t c, bool transpose, double* E, int64_t lde, int64_t ncols) {
-    constexpr int64_t nb = 64;
+    constexpr int64_t nb = 256;
     if (c <= 0 || ncols <= 0) return;
     DevBuf Vb((size_t)(m * std::min(nb, c)) * 8), Gb((size_t)(nb * nb) * 8), Tb((size_t)(nb * nb) * 8),
         Wb((size_t)(nb * ncols) * 8), W2b((size_t)(nb * ncols) * 8);
