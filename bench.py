@@ -52,6 +52,22 @@ def profiled_traffic(key, n, m, k):
     return None
 
 
+def copy_rate(csa, gib=4, reps=5):
+    """On-box HBM reference: a plain device copy of `gib` GiB (k_copy16), read + write bytes over
+    the fastest of `reps` synchronous calls, in GB/s."""
+    nbytes = gib << 30
+    src = csa.DeviceArray(nbytes // 8)
+    dst = csa.DeviceArray(nbytes // 8)
+    best = float("inf")
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        csa._check(csa.lib().cs_dev_copy(dst.ptr, src.ptr, nbytes))
+        best = min(best, time.perf_counter() - t0)
+    src.close()
+    dst.close()
+    return 2.0 * nbytes / best / 1e9
+
+
 def cpu_baseline(N, M_slice, k, seed, pmin=0.01, miss=0.005):
     """The reference's K.V algorithm (kinship_operator.cpp:225-299: 512-SNP blocks decoded to FP64,
     z^T X, then z (z^T X); the oracle's cache-blocked, threaded restatement) on a slice of the
@@ -590,6 +606,13 @@ def main():
     kv = kv_leg(csa, ctx["reader"], N, M, k, args.steps, args.warmup, args.prec, args.missing_mode,
                 extras=not args.kv_only)
     value, ms_step = kv["genotypes_per_s"], kv["ms_per_step"]
+    if not args.kv_only:
+        # context for the roofline: the same kernel against a measured plain copy (read + write)
+        cr = copy_rate(csa)
+        kv["roofline"]["copy_gbs_measured"] = round(cr, 1)
+        kv["roofline"]["frac_of_copy"] = round(kv["roofline"]["achieved"] / cr, 4)
+        log(f"plain device copy: {cr:.0f} GB/s (read + write); dominant K.V kernel at "
+            f"{kv['roofline']['frac_of_copy']:.3f} of it")
 
     fp32 = None
     if not args.no_fp32 and not args.kv_only:
