@@ -68,6 +68,61 @@ def copy_rate(csa, gib=4, reps=5):
     return 2.0 * nbytes / best / 1e9
 
 
+def measured_traffic(args):
+    """HBM bytes per launch of the two genotype products of this K.V workload, measured in this run:
+    two rocprofv3 --pmc child runs of `bench.py --kv-only` (FETCH_SIZE, then WRITE_SIZE), each
+    scaled by what the same counter reads for the plain 2 GiB copy kernel (`--copy-probe`, two
+    more child runs).  Returns {"k_geno1s": bytes, "k_geno2s": bytes, "fetch_scale": .,
+    "write_scale": .} or None when rocprofv3 is absent or a run fails (the caller then falls back
+    to the committed profile)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3") else None)
+    if exe is None:
+        return None
+    tmp = tempfile.mkdtemp(prefix="csamd_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+
+    def run(counter, extra, tag):
+        out = os.path.join(tmp, tag)
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", tag, "--",
+               sys.executable, os.path.abspath(__file__)] + extra
+        subprocess.run(cmd, cwd="/tmp", env=env, timeout=240, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL, check=True)
+        vals = {}
+        for root, _, files in os.walk(out):
+            for fn in files:
+                if fn.endswith("counter_collection.csv"):
+                    for r in csv.DictReader(open(os.path.join(root, fn))):
+                        if r.get("Counter_Name") == counter:
+                            vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return {kname: sum(v) / len(v) for kname, v in vals.items()}
+
+    try:
+        kv = ["--kv-only", "--steps", "1", "--warmup", "0", "--no-cpu", "--n", str(args.n), "--m", str(args.m),
+              "--k", str(args.k), "--prec", args.prec, "--missing-mode", str(args.missing_mode)]
+        f, w = run("FETCH_SIZE", kv, "f"), run("WRITE_SIZE", kv, "w")
+        cf, cw = run("FETCH_SIZE", ["--copy-probe"], "cf"), run("WRITE_SIZE", ["--copy-probe"], "cw")
+        copy_kib = (2 << 30) / 1024.0
+        cfk = [v for kname, v in cf.items() if "k_copy16" in kname]
+        cwk = [v for kname, v in cw.items() if "k_copy16" in kname]
+        fs, ws = copy_kib / cfk[0], copy_kib / cwk[0]
+        out = dict(fetch_scale=round(fs, 4), write_scale=round(ws, 4))
+        for key in ("k_geno1s", "k_geno2s"):
+            fk = [v for kname, v in f.items() if f"csamd::{key}<" in kname and ", false, " in kname]
+            wk = [v for kname, v in w.items() if f"csamd::{key}<" in kname and ", false, " in kname]
+            if fk and wk:
+                out[key] = (max(fk) * fs + max(wk) * ws) * 1024.0
+        return out if "k_geno1s" in out and "k_geno2s" in out else None
+    except Exception as e:   # a missing tool or a failed counter run leaves the committed profile
+        log(f"traffic measurement skipped: {e}")
+        return None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
 def cpu_baseline(N, M_slice, k, seed, pmin=0.01, miss=0.005):
     """The reference's K.V algorithm (kinship_operator.cpp:225-299: 512-SNP blocks decoded to FP64,
     z^T X, then z (z^T X); the oracle's cache-blocked, threaded restatement) on a slice of the
@@ -567,6 +622,9 @@ def main():
                     help="also run the low-rank matrix-free GWAS leg (the reference path, headline genotypes)")
     ap.add_argument("--mv-n", type=int, default=50_000)
     ap.add_argument("--mv-m", type=int, default=300_000)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE child runs (roofline.traffic then comes "
+                         "from the committed profile)")
     ap.add_argument("--grm-only", action="store_true",
                     help="only the n=50k x m=500k GRM legs (exact and standardised; for counter runs)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -606,6 +664,18 @@ def main():
     kv = kv_leg(csa, ctx["reader"], N, M, k, args.steps, args.warmup, args.prec, args.missing_mode,
                 extras=not args.kv_only)
     value, ms_step = kv["genotypes_per_s"], kv["ms_per_step"]
+    if not args.kv_only and not args.no_traffic:
+        tr = measured_traffic(args)
+        key = "k_geno2s" if "k_geno2s" in kv["roofline"]["kernel"] else "k_geno1s"
+        if tr is not None:
+            kv["roofline"]["traffic"] = round(tr[key], -6)
+            kv["roofline"]["traffic_source"] = (f"rocprofv3 FETCH_SIZE x{tr['fetch_scale']} + WRITE_SIZE "
+                                               f"x{tr['write_scale']} in this run (plain-copy calibration)")
+            kv["roofline"]["traffic_other_product"] = round(tr["k_geno1s" if key == "k_geno2s" else "k_geno2s"], -6)
+            log(f"measured HBM traffic per launch: k_geno1s {tr['k_geno1s'] / 1e9:.2f} GB, "
+                f"k_geno2s {tr['k_geno2s'] / 1e9:.2f} GB")
+        elif kv["roofline"]["traffic"] is not None:
+            kv["roofline"]["traffic_source"] = "committed rocprofv3 summary in profiles/"
     if not args.kv_only:
         # context for the roofline: the same kernel against a measured plain copy (read + write)
         cr = copy_rate(csa)
