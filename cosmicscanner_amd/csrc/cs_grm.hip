@@ -866,12 +866,15 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
     const int64_t N = b->N, Npad = b->Npad, Mpad = b->Mpad, KB = b->KBm;
     if (Mpad > (int64_t(1) << 29))   // x~ x~ products reach 4 M in int32
         throw InvalidArgument("GRM: more than 2^29 SNPs");
-    hipStream_t st;
+    hipStream_t st, st2;   // st2: the D pairs beside the MFMA passes
     CS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    hipEvent_t e0, e1, e2;
+    CS_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+    hipEvent_t e0, e1, e2, ed0, ed1;
     CS_HIP(hipEventCreate(&e0));
     CS_HIP(hipEventCreate(&e1));
     CS_HIP(hipEventCreate(&e2));
+    CS_HIP(hipEventCreateWithFlags(&ed0, hipEventDisableTiming));
+    CS_HIP(hipEventCreateWithFlags(&ed1, hipEventDisableTiming));
     try {
         // keep mask and masked copy
         std::vector<uint32_t> keep((size_t)(Mpad / 16), 0);
@@ -977,7 +980,22 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             CS_HIP(hipMemcpyAsync(dblocks.p, blocks.data(), blocks.size() * sizeof(int2),
                                   hipMemcpyHostToDevice, st));
             if (nC) CS_HIP(hipMemsetAsync(cacc.p, 0, (size_t)(rows * Npad) * 8, st));
-            if (useD) CS_HIP(hipMemsetAsync(dacc.p, 0, (size_t)(rows * Npad) * 16, st));
+            unsigned long long* dhi = dacc.as<unsigned long long>();
+            unsigned long long* dlo = dhi + rows * Npad;
+            if (useD) {
+                // D from the missing lists on st2 (latency-bound atomics), concurrent with the
+                // MFMA passes on st; the combine waits for both
+                CS_HIP(hipMemsetAsync(dacc.p, 0, (size_t)(rows * Npad) * 16, st));
+                CS_HIP(hipEventRecord(ed0, st));
+                CS_HIP(hipStreamWaitEvent(st2, ed0, 0));
+                const int64_t j0 = prm.kb_lo * 256, j1 = std::min<int64_t>(b->M, prm.kb_hi * 256);
+                const unsigned gd = (unsigned)std::min<int64_t>(65536, (j1 - j0 + 3) / 4);
+                k_grm_dpairs<<<gd, 256, 0, st2>>>(b->miss_snp_off.as<int64_t>(), b->miss_snp_idx.as<int32_t>(),
+                                                  dA.as<long long>(), j0, j1, I0, std::min<int64_t>(N, I0 + rows),
+                                                  Npad, dhi, dlo);
+                CS_HIP(hipGetLastError());
+                CS_HIP(hipEventRecord(ed1, st2));
+            }
             GrmArgs ga{};
             ga.smp = masked.as<uint8_t>();
             ga.KB = KB;
@@ -1013,17 +1031,9 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
                     ++launches;
                 }
             }
-            unsigned long long* dhi = dacc.as<unsigned long long>();
-            unsigned long long* dlo = dhi + rows * Npad;
-            if (useD) {
-                const int64_t j0 = prm.kb_lo * 256, j1 = std::min<int64_t>(b->M, prm.kb_hi * 256);
-                const unsigned gd = (unsigned)std::min<int64_t>(65536, (j1 - j0 + 3) / 4);
-                k_grm_dpairs<<<gd, 256, 0, st>>>(b->miss_snp_off.as<int64_t>(), b->miss_snp_idx.as<int32_t>(),
-                                                 dA.as<long long>(), j0, j1, I0, std::min<int64_t>(N, I0 + rows), Npad,
-                                                 dhi, dlo);
-            }
             CS_HIP(hipGetLastError());
             CS_HIP(hipEventRecord(e1, st));
+            if (useD) CS_HIP(hipStreamWaitEvent(st, ed1, 0));
             // int8 MFMA ops issued: T1 over all SNP blocks plus each digit product over all blocks
             mfma_ops += (double)(passes - passes0) * (double)quadrants * 64.0 * 64.0 *
                         (double)((prm.kb_hi - prm.kb_lo) * 256) * 2.0;
@@ -1068,16 +1078,23 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
         g_grm_timing.mfma_ops = mfma_ops;
     } catch (...) {
         (void)hipStreamSynchronize(st);
+        (void)hipStreamSynchronize(st2);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipEventDestroy(e2);
+        (void)hipEventDestroy(ed0);
+        (void)hipEventDestroy(ed1);
         (void)hipStreamDestroy(st);
+        (void)hipStreamDestroy(st2);
         throw;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(e2);
+    (void)hipEventDestroy(ed0);
+    (void)hipEventDestroy(ed1);
     CS_HIP(hipStreamDestroy(st));
+    CS_HIP(hipStreamDestroy(st2));
 }
 
 extern "C" {
