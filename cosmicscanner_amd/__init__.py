@@ -80,6 +80,8 @@ def _load():
         "cs_kinship_num_variants": ([_vp, _i64p], ctypes.c_int),
         "cs_kinship_set_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "cs_kinship_set_missing_mode": ([_vp, ctypes.c_int], ctypes.c_int),
+        "cs_kinship_compact_rows": ([_vp], ctypes.c_int),
+        "cs_kinship_device_bytes": ([_vp, _i64p], ctypes.c_int),
         "cs_kinship_apply": ([_vp, _dp, _i64, _i64, _dp, _i64], ctypes.c_int),
         "cs_kinship_apply_device": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
         "cs_kinship_apply_device_noalloc": ([_vp, _vp, _i64, _i64, _vp, _i64], ctypes.c_int),
@@ -395,6 +397,16 @@ class PlinkKinshipOperator:
     def set_missing_mode(self, mode):
         """0 auto, 1 dense observed-indicator products, 2 sparse missing lists (same results)."""
         _check(lib().cs_kinship_set_missing_mode(self._h, int(mode)))
+
+    def compact_rows(self):
+        """Run the genotype products on a compact copy of the cohort's rows (same results; the
+        copy costs 2 * roundup(n, 256) * roundup(M, 256) / 4 bytes of HBM)."""
+        _check(lib().cs_kinship_compact_rows(self._h))
+
+    def device_bytes(self):
+        b = ctypes.c_int64()
+        _check(lib().cs_kinship_device_bytes(self._h, ctypes.byref(b)))
+        return b.value
 
     def apply(self, X, n_threads=1):
         X = np.asarray(X, dtype=np.float64)
@@ -999,7 +1011,8 @@ class _ReqStats(ctypes.Structure):
     _fields_ = [("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("gamma", ctypes.c_double), ("n_calibration", ctypes.c_int64),
                 ("n_columns", ctypes.c_int64), ("n_exact", ctypes.c_int64), ("exact_iterations", ctypes.c_int),
-                ("calibration_rel_sd", ctypes.c_double), ("dlog10p_bound_below", ctypes.c_double)]
+                ("calibration_rel_sd", ctypes.c_double), ("dlog10p_bound_below", ctypes.c_double),
+                ("admit_bytes", ctypes.c_int64), ("compact_rows", ctypes.c_int)]
 
 
 class ScanServer:

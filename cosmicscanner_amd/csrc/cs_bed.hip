@@ -258,6 +258,29 @@ static void upload(cs_bed* b, const uint8_t* body) {
     CS_HIP(hipHostFree(pinned));
 }
 
+// Row gather of a tiled matrix: destination row r (< rows_pad) takes source row keep[r] when
+// r < n, code 01 (missing) beyond.  One lane per 16-byte chunk, in destination address order:
+// chunk e = ((rt KB + kb) << 8) + (m 64 + g 16 + rr) holds k-block kb, chunk g of row 64 rt + 16 m + rr.
+__global__ __launch_bounds__(256) void k_gather_rows(const uint4* __restrict__ src, int64_t KB,
+                                                     const int32_t* __restrict__ keep, int64_t n,
+                                                     int64_t rows_pad, uint4* __restrict__ dst) {
+    const int64_t chunks = rows_pad / 64 * KB * 256;
+    const uint32_t miss = 0x55555555u;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < chunks;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t tile = e >> 8;
+        const int w = (int)(e & 255);
+        const int64_t rt = tile / KB, kb = tile - rt * KB;
+        const int64_t row = rt * 64 + ((w >> 6) << 4) + (w & 15);
+        uint4 v = make_uint4(miss, miss, miss, miss);
+        if (row < n) {
+            const int64_t s = keep[row];
+            v = src[(((s >> 6) * KB + kb) << 8) + (((s >> 4) & 3) << 6) + (w & 48) + (s & 15)];
+        }
+        dst[e] = v;
+    }
+}
+
 void ensure_sample_major(cs_bed* b) {
     if (b->smp.p) return;
     b->smp.alloc((size_t)(b->Npad * b->Mpad / 4));
@@ -510,6 +533,56 @@ int64_t missing_total(cs_bed* b) {
     CS_HIP(hipStreamSynchronize(b->stream));
     b->miss_total = h;
     return h;
+}
+
+int64_t bed_rows_copy_bytes(const cs_bed* parent, int64_t n) {
+    const int64_t npad = round_up(n, kPad);
+    return 2 * (npad * parent->Mpad / 4) + parent->M * 48;
+}
+
+cs_bed* bed_rows_copy(cs_bed* parent, const int32_t* keep, int64_t n) {
+    if (n <= 0) throw InvalidArgument("cohort copy: empty cohort");
+    for (int64_t r = 0; r < n; ++r)
+        if (keep[r] < 0 || keep[r] >= parent->N) throw RuntimeError("cohort copy: sample index out of range");
+    ensure_stats(parent);
+    ensure_sample_major(parent);
+    auto* b = new cs_bed();
+    try {
+        CS_HIP(hipGetDevice(&b->device));
+        b->N = n;
+        b->M = parent->M;
+        b->Braw = (n + 3) / 4;
+        b->Npad = round_up(n, kPad);
+        b->Mpad = parent->Mpad;
+        b->KBs = b->Npad / 256;
+        b->KBm = parent->KBm;
+        CS_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        DevBuf dkeep((size_t)n * 4);
+        CS_HIP(hipMemcpyAsync(dkeep.p, keep, (size_t)n * 4, hipMemcpyHostToDevice, b->stream));
+        b->smp.alloc((size_t)(b->Npad * b->Mpad / 4));
+        k_gather_rows<<<grid_for(b->Npad / 64 * b->KBm * 256), 256, 0, b->stream>>>(
+            parent->smp.as<uint4>(), b->KBm, dkeep.as<int32_t>(), n, b->Npad, b->smp.as<uint4>());
+        CS_HIP(hipGetLastError());
+        // SNP-major copy by the tiled transpose (input rows: the cohort's samples)
+        b->snp.alloc((size_t)(b->Mpad * b->Npad / 4));
+        k_transpose<<<dim3((unsigned)(b->Mpad / 256), (unsigned)(b->Npad / 256)), 256, 0, b->stream>>>(
+            b->smp.as<uint8_t>(), b->KBm, b->snp.as<uint8_t>(), b->KBs);
+        CS_HIP(hipGetLastError());
+        // the cohort's own class counts (missing fraction), the parent's all-sample standardisation
+        ensure_stats(b);
+        CS_HIP(hipMemcpyAsync(b->means.p, parent->means.p, (size_t)b->M * 8, hipMemcpyDeviceToDevice, b->stream));
+        CS_HIP(hipMemcpyAsync(b->inv_stds.p, parent->inv_stds.p, (size_t)b->M * 8, hipMemcpyDeviceToDevice,
+                              b->stream));
+        CS_HIP(hipStreamSynchronize(b->stream));
+    } catch (...) {
+        if (b->stream) {
+            (void)hipStreamSynchronize(b->stream);
+            (void)hipStreamDestroy(b->stream);
+        }
+        delete b;
+        throw;
+    }
+    return b;
 }
 
 void ensure_missing_lists(cs_bed* b) {

@@ -35,12 +35,18 @@ struct cs_kinship {
     csamd::DevBuf dX, dY, Vt, BV, W1, W2, B1, B2, Yint, mxW, mx, scl, Vf, PQf, Wf, Yf;
     csamd::DevBuf Vp, B2p, S1, S2, qext;   // sparse-missing form: plain quantised operands, sums
     csamd::DevBuf diag_tab, diag_full;
+    cs_bed* rows = nullptr;         // compact copy of the cohort's genotype rows (owned), or null
     std::vector<double> diag_cache;  // diag(K) over the rows, once computed (fixed by keep / variants)
     hipEvent_t ev[6] = {};
     float last_ms[5] = {0, 0, 0, 0, 0};
 };
 
 namespace csamd {
+
+// The matrix the genotype products read (the cohort's compact copy when there is one) and
+// whether its rows are the operator's rows in order (no scatter / gather through keep).
+static cs_bed* row_bed(const cs_kinship* op) { return op->rows ? op->rows : op->bed; }
+static bool rows_direct(const cs_kinship* op) { return op->identity || op->rows != nullptr; }
 
 // ------------------------------------------------------------------------------------------
 __global__ void k_scatter(const double* __restrict__ V, int64_t ldv, int64_t k,
@@ -837,7 +843,7 @@ void quantize_cols_into(const double* const* W, int nW, int64_t rows, int64_t k,
 // only the genotype products, and O^T V / O (mu W) come from column sums and the missing lists.
 // Both forms give bitwise identical results.
 static bool use_sparse_missing(cs_kinship* op, int64_t k, int L) {
-    cs_bed* b = op->bed;
+    cs_bed* b = row_bed(op);
     if (op->missing_mode == 1 || op->prec == CS_PREC_INT8_L4) return false;   // int32 plain operand
     if (op->missing_mode == 0) {
         // Measured at n = 50k, m = 500k, k = 16: the genotype-only products take 1.3 + 1.2 ms
@@ -868,10 +874,10 @@ struct KvWork {
 };
 
 // Byte offsets of the KvWork regions for k columns; returns the total size.
-static size_t kv_work_layout(const cs_bed* b, int64_t k, const Tiling& tl, size_t off[10]) {
-    const size_t sz[10] = {(size_t)(b->Npad * k) * 8, (size_t)(b->Npad * tl.NT * 16), (size_t)k * 8, (size_t)(2 * k) * 8,
-                           (size_t)(b->Mpad * k) * 8, (size_t)(b->Mpad * k) * 8, (size_t)k * 8,
-                           (size_t)(b->Mpad * tl.NT * 16), (size_t)(b->Mpad * tl.NT * 16), (size_t)(b->Npad * k) * 8};
+static size_t kv_work_layout(int64_t Npad, int64_t Mpad, int64_t k, const Tiling& tl, size_t off[10]) {
+    const size_t sz[10] = {(size_t)(Npad * k) * 8, (size_t)(Npad * tl.NT * 16), (size_t)k * 8, (size_t)(2 * k) * 8,
+                           (size_t)(Mpad * k) * 8, (size_t)(Mpad * k) * 8, (size_t)k * 8,
+                           (size_t)(Mpad * tl.NT * 16), (size_t)(Mpad * tl.NT * 16), (size_t)(Npad * k) * 8};
     size_t t = 0;
     for (int i = 0; i < 10; ++i) {
         off[i] = t;
@@ -932,15 +938,16 @@ __global__ void k_w_from_u(const double* __restrict__ U, int64_t ldu, int64_t M,
 
 static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
                        int64_t ldy, const KvWork* ws = nullptr, int mode = KV_APPLY) {
-    cs_bed* b = op->bed;
+    cs_bed* b = row_bed(op);
+    const bool direct = rows_direct(op);
     hipStream_t st = op->stream;
     const int L = prec_limbs(op->prec);
     const double qmax = limbs_qmax(L);
     const bool sparse = (ws || mode != KV_APPLY) ? false : use_sparse_missing(op, k, L);
     const Tiling tl = tiling(k, L, !sparse);   // the genotype-only kernels take unpacked operands
     const int64_t K16 = 16 * (int64_t)tl.G;
-    const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
-    const int32_t* inv_idx = op->identity ? nullptr : op->d_inv_idx.as<int32_t>();
+    const int32_t* inv_off = direct ? nullptr : op->d_inv_off.as<int32_t>();
+    const int32_t* inv_idx = direct ? nullptr : op->d_inv_idx.as<int32_t>();
     KvWork w;
     if (ws) {
         w = *ws;
@@ -1132,8 +1139,7 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
         }
         CS_HIP(hipGetLastError());
     }
-    k_finalize<<<grid_for(op->n * k), 256, 0, st>>>(w.Yint, k, sclW,
-                                                   op->identity ? nullptr : op->d_keep.as<int32_t>(),
+    k_finalize<<<grid_for(op->n * k), 256, 0, st>>>(w.Yint, k, sclW, direct ? nullptr : op->d_keep.as<int32_t>(),
                                                    op->n, dY, ldy);
     CS_HIP(hipGetLastError());
     CS_HIP(hipEventRecord(op->ev[5], st));
@@ -1141,7 +1147,7 @@ static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k,
 
 template <int KC>
 static void apply_f32_cols(cs_kinship* op, int64_t k, double* dY, int64_t ldy) {
-    cs_bed* b = op->bed;
+    cs_bed* b = row_bed(op);
     hipStream_t st = op->stream;
     constexpr int Qc = (KC + 3) / 4 * 4;
     op->Vf.ensure((size_t)(b->Npad * Qc) * sizeof(float));
@@ -1172,17 +1178,17 @@ static void apply_f32_cols(cs_kinship* op, int64_t k, double* dY, int64_t ldy) {
     CS_HIP(hipGetLastError());
     CS_HIP(hipEventRecord(op->ev[4], st));
     k_epi_y_f32<<<grid_for(op->n * k), 256, 0, st>>>(op->Yf.as<float>(), ns, b->Npad, KC, k,
-                                                    op->identity ? nullptr : op->d_keep.as<int32_t>(), op->n,
+                                                    rows_direct(op) ? nullptr : op->d_keep.as<int32_t>(), op->n,
                                                     dY, ldy);
     CS_HIP(hipGetLastError());
 }
 
 static void apply_f32(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
                       int64_t ldy) {
-    cs_bed* b = op->bed;
+    cs_bed* b = row_bed(op);
     hipStream_t st = op->stream;
-    const int32_t* inv_off = op->identity ? nullptr : op->d_inv_off.as<int32_t>();
-    const int32_t* inv_idx = op->identity ? nullptr : op->d_inv_idx.as<int32_t>();
+    const int32_t* inv_off = rows_direct(op) ? nullptr : op->d_inv_off.as<int32_t>();
+    const int32_t* inv_idx = rows_direct(op) ? nullptr : op->d_inv_idx.as<int32_t>();
     for (int64_t c0 = 0; c0 < k; c0 += 16) {
         const int64_t kc = std::min<int64_t>(16, k - c0);
         CS_HIP(hipEventRecord(op->ev[0], st));
@@ -1309,8 +1315,31 @@ int cs_kinship_destroy(cs_kinship* op) {
         if (op->stream) (void)hipStreamSynchronize(op->stream);
         for (auto& e : op->ev) if (e) (void)hipEventDestroy(e);
         hipStream_t s = op->stream;
+        cs_bed* rows = op->rows;
         delete op;
+        if (rows) (void)cs_bed_destroy(rows);
         if (s) (void)hipStreamDestroy(s);
+    });
+}
+
+int cs_kinship_compact_rows(cs_kinship* op) {
+    return guarded([&] {
+        if (!op) throw InvalidArgument("cs_kinship_compact_rows: null handle");
+        kinship_compact_rows(op);
+    });
+}
+
+int cs_kinship_device_bytes(const cs_kinship* op, int64_t* bytes) {
+    return guarded([&] {
+        if (!op || !bytes) throw InvalidArgument("cs_kinship_device_bytes: null argument");
+        int64_t t = 0;
+        for (const DevBuf* b : {&op->d_keep, &op->d_inv_off, &op->d_inv_idx, &op->d_mult, &op->dX, &op->dY, &op->Vt,
+                                &op->BV, &op->W1, &op->W2, &op->B1, &op->B2, &op->Yint, &op->mxW, &op->mx, &op->scl,
+                                &op->Vf, &op->PQf, &op->Wf, &op->Yf, &op->Vp, &op->B2p, &op->S1, &op->S2, &op->qext,
+                                &op->diag_tab, &op->diag_full})
+            t += (int64_t)b->bytes;
+        if (op->rows) t += bed_rows_copy_bytes(op->bed, op->n);
+        *bytes = t;
     });
 }
 
@@ -1340,6 +1369,45 @@ int cs_kinship_set_precision(cs_kinship* op, int mode) {
 }  // extern "C"
 
 int csamd::kinship_get_precision(const cs_kinship* op) { return op->prec; }
+
+// The compact copy holds the cohort's rows in the operator's order, so V needs no scatter and
+// K V no gather.  Every per-column quantity of the products is unchanged: V's column maxima (the
+// scattered copy only adds zero rows), the exact integer sums over the samples, W's per-SNP
+// values and maxima, and the per-row results.  With repeated samples the scatter would sum V's
+// rows before quantising, so such cohorts stay on the full-sample products.
+void csamd::kinship_compact_rows(cs_kinship* op) {
+    if (op->identity || op->rows) return;
+    std::vector<uint8_t> seen((size_t)op->bed->N, 0);
+    for (int32_t s : op->keep)
+        if (seen[(size_t)s]++) throw InvalidArgument("PlinkKinshipOperator: a compact cohort lists a sample twice");
+    CS_HIP(hipStreamSynchronize(op->stream));
+    op->rows = bed_rows_copy(op->bed, op->keep.data(), op->n);
+    kinship_release_workspace(op);   // sized for the full-sample rows
+}
+
+bool csamd::kinship_is_compact(const cs_kinship* op) { return op->rows != nullptr; }
+
+int64_t csamd::kinship_work_bytes(int64_t Npad, int64_t Mpad, int prec, int64_t k) {
+    if (k <= 0) return 0;
+    const int L = prec_limbs(prec);
+    const Tiling tl = tiling(k, L, true), tu = tiling(k, L, false);
+    size_t off[10];
+    // dense form, plus what the sparse missing form adds (plain operands, sums, per-SNP terms)
+    const int64_t K16 = 16 * (int64_t)tu.G;
+    int64_t t = (int64_t)std::max(kv_work_layout(Npad, Mpad, k, tl, off), kv_work_layout(Npad, Mpad, k, tu, off));
+    t += Npad * K16 * 4 + Mpad * K16 * 12 + 2 * K16 * 8;
+    // FP32 comparator: V and W in FP32, u, and the per-split partial rows (about 2048 workgroups)
+    if (prec == CS_PREC_FP32) t += Npad * 16 * 4 + Mpad * 16 * 12 + std::max<int64_t>(2 * 2048 * 256, 2 * Npad) * 16 * 4;
+    return t;
+}
+
+void csamd::kinship_release_workspace(cs_kinship* op) {
+    CS_HIP(hipStreamSynchronize(op->stream));
+    for (DevBuf* b : {&op->dX, &op->dY, &op->Vt, &op->BV, &op->W1, &op->W2, &op->B1, &op->B2, &op->Yint, &op->mxW,
+                      &op->mx, &op->scl, &op->Vf, &op->PQf, &op->Wf, &op->Yf, &op->Vp, &op->B2p, &op->S1, &op->S2,
+                      &op->qext})
+        b->release();
+}
 
 extern "C" {
 
@@ -1372,7 +1440,7 @@ int cs_kinship_apply_device_noalloc(cs_kinship* op, const double* dX, int64_t ld
             throw InvalidArgument("cs_kinship_apply_device_noalloc: the FP32 comparator precision is not supported");
         const Tiling tl = tiling(k, prec_limbs(op->prec), true);
         size_t off[10];
-        const size_t need = kv_work_layout(op->bed, k, tl, off);
+        const size_t need = kv_work_layout(row_bed(op)->Npad, row_bed(op)->Mpad, k, tl, off);
         if (need > kKvScratchBytes)
             throw InvalidArgument("cs_kinship_apply_device_noalloc: " + std::to_string(need) +
                                   " bytes of workspace needed for this operator and k, the fixed workspace holds " +

@@ -59,6 +59,22 @@ void ensure_stats(cs_bed* b);
 void ensure_sample_major(cs_bed* b);
 // CSR lists of the missing genotypes (both orientations); idempotent.
 void ensure_missing_lists(cs_bed* b);
+// A new resident matrix of the cohort keep[0..n) (host indices, caller's order): both tiled
+// copies gathered from the parent's rows, the cohort's own class counts, and the parent's
+// all-sample means and inverse standard deviations.  Freed with cs_bed_destroy.
+cs_bed* bed_rows_copy(cs_bed* parent, const int32_t* keep, int64_t n);
+// Device bytes of such a copy.
+int64_t bed_rows_copy_bytes(const cs_bed* parent, int64_t n);
+// Keep the kinship operator's genotype products on a compact copy of its cohort's rows (no-op
+// for the identity cohort or when already compact); results are bitwise those of the
+// full-sample products.  Needs a keep list without repeated samples.
+void kinship_compact_rows(cs_kinship* op);
+bool kinship_is_compact(const cs_kinship* op);
+// Upper bound of the device workspace one apply of k columns grows in an operator whose products
+// read Npad sample rows and Mpad SNPs (both padded to 256) at precision prec.
+int64_t kinship_work_bytes(int64_t Npad, int64_t Mpad, int prec, int64_t k);
+// Free the operator's grown workspaces (the next apply grows them again).
+void kinship_release_workspace(cs_kinship* op);
 // Number of missing genotypes over all samples and SNPs (from the class counts; cached).
 int64_t missing_total(cs_bed* b);
 // Per-SNP class counts weighted by the host multiplicities w[0..n_w) (zero beyond); counts is

@@ -1,23 +1,27 @@
 // Serving loop for scan requests over a resident genotype matrix.
 //
-// One kinship operator K over all samples (and the chosen variants) stays resident.  Because K's
-// standardisation uses all-sample statistics (kinship_operator.cpp:64-67, plink_reader.cpp:187-239),
-// the kinship matrix of a cohort S is the restriction P_S K P_S^T: a request's system
-// (sigma_g K_S + sigma_e I) x = b is solved in the full sample space with b zero outside S and
-// the operator masked to S.  Every request contributes columns (its traits, its covariates and its
-// calibration markers) to one shared multi-RHS PCG: one K.V apply per iteration serves every
-// admitted request, each column keeping its own variance components, preconditioner, mask,
-// recurrence and convergence.  New requests are admitted at iteration boundaries as column slots
-// and the HBM budget free up (continuous batching).  A request whose columns have all converged
-// runs its projection, calibration and fused per-SNP scan at once and gives its slots back; the
-// SNPs its exact stage selects then return to the batch as a second phase of columns (their
-// prepared markers, in chunks that fit the free slots and budget), whose solutions give the exact
-// GLS variances.  The budget covers the solve buffers while held and the result arrays from
+// Because K's standardisation uses all-sample statistics (kinship_operator.cpp:64-67,
+// plink_reader.cpp:187-239), the kinship matrix of a cohort S is the restriction P_S K P_S^T.
+// Each admitted request owns a kinship operator over its cohort.  When the cohort is at most 3/4
+// of the samples and the HBM budget holds it, the operator's products run on a compact copy of the
+// cohort's genotype rows (gathered once at admission), so a K.V apply reads and multiplies n_S
+// instead of N sample rows; otherwise they run on the full matrix through the operator's keep
+// list.  A request's columns (its traits, covariates and calibration markers) form one multi-RHS
+// PCG over n_S rows, each column with its own recurrence and convergence; one server step
+// advances every active request by one iteration (one K.V apply each).  New requests are
+// admitted at step boundaries as column capacity and the HBM budget free up (continuous
+// batching).  A request whose columns have all converged runs its projection, calibration and
+// fused per-SNP scan at once and gives its solve buffers back; the SNPs its exact stage selects
+// then return as a second phase of columns (their prepared markers, in chunks that fit the free
+// capacity and budget), whose solutions give the exact GLS variances.  The budget covers a
+// request's compact copy, operator lists, preconditioner and projection from admission to
+// completion, its solve buffers and operator workspace while held, and its result arrays from
 // admission until the client releases them.
 //
-// A column's arithmetic is independent of the other columns in the batch (per-column
-// quantisation scales, exact integer products, per-column dots with a fixed reduction order), so a
-// request's results are the same alone and in any mix.
+// A column's arithmetic depends only on its own request (per-column quantisation scales, exact
+// integer products, per-column dots with a fixed reduction order, and an operator whose choice of
+// compact rows depends only on the cohort and the server's budget), so a request's results are
+// the same alone and in any mix.
 #include <rocblas/rocblas.h>
 
 #include <algorithm>
@@ -25,7 +29,9 @@
 #include <cmath>
 #include <deque>
 #include <map>
+#include <cstring>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "cs_common.h"
@@ -42,6 +48,10 @@ namespace csamd {
     } while (0)
 
 constexpr int kSrvDotBlocks = 128;
+// Pinned staging for results delivered to the host: two buffers of this size.
+constexpr size_t kStageBytes = (size_t)64 << 20;
+// Cohorts up to this fraction of the samples run on a compact copy of their rows.
+constexpr double kCompactFrac = 0.75;
 
 // partial[c * kSrvDotBlocks + blk] = sum over the block's rows of A[:, c] B[:, c]
 __global__ __launch_bounds__(256) void k_srv_dots(const double* __restrict__ A, const double* __restrict__ B,
@@ -69,30 +79,25 @@ __global__ void k_srv_reduce(const double* __restrict__ partial, int64_t k, doub
     out[c] = s;
 }
 
-// AP = sigma_g[c] mask(K P) + sigma_e[c] P   (mask: minv != 0 on the column's cohort)
-__global__ void k_srv_apply(const double* __restrict__ KP, const double* __restrict__ P,
-                            const double* __restrict__ minv, const double* __restrict__ sg,
-                            const double* __restrict__ se, int64_t n, int64_t k, double* __restrict__ AP) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * k;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = e / n;
-        AP[e] = (minv[e] != 0.0 ? sg[c] * KP[e] : 0.0) + se[c] * P[e];
-    }
+// AP = sigma_g K P + sigma_e P
+__global__ void k_srv_apply(const double* __restrict__ KP, const double* __restrict__ P, double sg, double se,
+                            int64_t total, double* __restrict__ AP) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x)
+        AP[e] = sg * KP[e] + se * P[e];
 }
 
-// column init: X = 0, R = B, Z = Minv R, P = Z on columns [c0, c0 + k)
-__global__ void k_srv_init(const double* __restrict__ Bm, const double* __restrict__ minv, int64_t n, int64_t c0,
-                           int64_t k, double* __restrict__ X, double* __restrict__ R, double* __restrict__ Z,
+// PCG init of k columns: X = 0, R = B, Z = Minv R, P = Z (Minv: one diagonal for all columns)
+__global__ void k_srv_init(const double* __restrict__ Bm, const double* __restrict__ minv, int64_t n, int64_t k,
+                           double* __restrict__ X, double* __restrict__ R, double* __restrict__ Z,
                            double* __restrict__ P) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * k;
          e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t g = c0 * n + e;
-        const double r = Bm[g];
-        X[g] = 0.0;
-        R[g] = r;
-        const double z = minv[g] * r;
-        Z[g] = z;
-        P[g] = z;
+        const double r = Bm[e];
+        X[e] = 0.0;
+        R[e] = r;
+        const double z = minv[e % n] * r;
+        Z[e] = z;
+        P[e] = z;
     }
 }
 
@@ -109,7 +114,7 @@ __global__ void k_srv_step(double* __restrict__ X, double* __restrict__ R, doubl
         X[e] += alpha * P[e];
         const double r = R[e] - alpha * AP[e];
         R[e] = r;
-        Z[e] = minv[e] * r;
+        Z[e] = minv[e - c * n] * r;
     }
 }
 
@@ -124,23 +129,11 @@ __global__ void k_srv_dir(double* __restrict__ P, const double* __restrict__ Z, 
     }
 }
 
-// dst[c * n + keep[r]] = src[c * nk + r]  (scatter compact cohort rows into full-sample columns)
-__global__ void k_srv_scatter(const double* __restrict__ src, int64_t nk, const int32_t* __restrict__ keep,
-                              int64_t n, int64_t k, double* __restrict__ dst) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nk * k;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = e / nk, r = e - c * nk;
-        dst[c * n + keep[r]] = src[e];
-    }
-}
-
-__global__ void k_srv_gather(const double* __restrict__ src, int64_t n, const int32_t* __restrict__ keep,
-                             int64_t nk, int64_t k, double* __restrict__ dst) {
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nk * k;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = e / nk, r = e - c * nk;
-        dst[e] = src[c * n + keep[r]];
-    }
+// rz <- rz_new on the active columns
+__global__ void k_srv_rz_keep(double* __restrict__ rz, const double* __restrict__ rz_new,
+                              const int32_t* __restrict__ active, int64_t k) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c < k && active[c]) rz[c] = rz_new[c];
 }
 
 }  // namespace csamd
@@ -151,10 +144,20 @@ namespace {
 
 enum { REQ_PENDING = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_FAILED = -1 };
 
+// PCG state of the columns a request holds (n x cols each, column-major).
+struct Solve {
+    int64_t cols = 0;
+    int64_t bytes = 0;   // charged to the budget while held (with the operator's workspace)
+    DevBuf X, R, Z, P, KP, AP, partial, rz, rzn, pap, rr, bb, act;
+    std::vector<double> hbb, hrr;
+    std::vector<int32_t> hact;
+    bool act_dirty = true;
+};
+
 struct Request {
     int64_t id = 0;
     int status = REQ_PENDING;
-    std::vector<int32_t> keep;     // sorted-order not required; the cohort in the caller's order
+    std::vector<int32_t> keep;     // the cohort in the caller's order
     int64_t n = 0, T = 0, c = 0, L = 0, k = 0;
     std::vector<double> Y;
     std::vector<float> X;
@@ -164,11 +167,14 @@ struct Request {
     int max_iter = 200, prec = CS_PREC_INT8_L3;
     // phase 1: the main solve over [Y, X, G_cal]; phase 2: exact-stage chunks of selected markers
     int phase = 1;
-    DevBuf B;                      // compact RHS of the columns in the slots (n x cols)
-    DevBuf dkeep;
-    int64_t slot0 = -1, cols = 0;  // slots held (slot0 < 0: none)
+    cs_kinship* K = nullptr;       // the request's operator (admission to completion)
+    bool compact = false;          // its products run on a compact copy of the cohort's rows
+    DevBuf B;                      // right-hand sides of the columns held (n x cols)
+    DevBuf minv;                   // Jacobi preconditioner 1 / (sigma_g diag(K) + sigma_e), n
+    Solve sv;
     int iterations = 0, chunk_iterations = 0, exact_iterations = 0;
-    int64_t bytes = 0;             // solve buffers charged to the budget while held
+    int64_t hold_bytes = 0;        // compact copy, lists, preconditioner, projection: admission to completion
+    int64_t admit_bytes = 0;       // hold + solve buffers + workspace at admission (results excluded)
     int64_t result_bytes = 0;      // result arrays, charged from admission until release
     double t_submit = 0, t_admit = 0, t_done = 0;
     // projection state between the phases, selected rows, next chunk
@@ -179,27 +185,35 @@ struct Request {
     int64_t ns = 0;
     DevBuf freq, n_obs, pass, beta, se_out, p, log10p;
     std::string error;
+
+    ~Request() {
+        if (K) (void)cs_kinship_destroy(K);
+    }
 };
 
 }  // namespace
 
 struct cs_server {
     cs_bed* bed = nullptr;
-    cs_kinship* K = nullptr;
     int64_t N = 0, cap = 0, used = 0;
     int64_t budget = 0, budget_used = 0;
+    std::vector<int32_t> variants;
     hipStream_t st = nullptr;
     rocblas_handle blas = nullptr;
-    DevBuf Xs, R, Z, P, KP, AP, Bs, minv, dsg, dse, act, partial, rz, rzn, pap, rr, bb;
-    std::vector<double> diag, hsg, hse, hbb, hrr;
-    std::vector<int32_t> hact;
-    std::vector<int64_t> slot_req;
+    std::vector<double> diag;
     std::deque<int64_t> pending;
     std::map<int64_t, std::unique_ptr<Request>> reqs;
     int64_t next_id = 1;
     std::chrono::steady_clock::time_point t0;
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
     ~cs_server() {
-        if (K) cs_kinship_destroy(K);
+        if (st) (void)hipStreamSynchronize(st);
+        reqs.clear();
+        for (int b = 0; b < 2; ++b) {
+            if (stage[b]) (void)hipHostFree(stage[b]);
+            if (stage_ev[b]) (void)hipEventDestroy(stage_ev[b]);
+        }
         if (blas) (void)rocblas_destroy_handle(blas);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -208,10 +222,60 @@ struct cs_server {
 
 namespace {
 
-void dots(cs_server* s, const double* A, const double* B, int64_t k, DevBuf& out, std::vector<double>* host) {
-    if (k == 0) return;
-    k_srv_dots<<<dim3(kSrvDotBlocks, (unsigned)k), 256, 0, s->st>>>(A, B, s->N, s->partial.as<double>());
-    k_srv_reduce<<<grid_for(k), 256, 0, s->st>>>(s->partial.as<double>(), k, out.as<double>());
+// memcpy over up to 8 host threads (a fresh destination's page faults are taken in parallel too)
+void host_copy(char* dst, const char* src, size_t bytes) {
+    const size_t min_part = (size_t)8 << 20;
+    const unsigned nt = (unsigned)std::max<size_t>(1, std::min<size_t>(8, bytes / min_part));
+    if (nt == 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = (bytes + nt - 1) / nt;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) {
+        const size_t a = t * part, b = std::min(bytes, a + part);
+        if (a < b) th.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
+    }
+    for (auto& t : th) t.join();
+}
+
+// Device arrays to host arrays through the two pinned stages: the copy engine fills one stage
+// while the host copies the other out.
+struct Download {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+void deliver(cs_server* s, const std::vector<Download>& list) {
+    struct Chunk {
+        char* dst;
+        const char* src;
+        size_t len;
+    };
+    std::vector<Chunk> ch;
+    for (const Download& d : list)
+        if (d.dst && d.bytes)
+            for (size_t o = 0; o < d.bytes; o += kStageBytes)
+                ch.push_back({(char*)d.dst + o, (const char*)d.src + o, std::min(kStageBytes, d.bytes - o)});
+    for (size_t i = 0; i <= ch.size(); ++i) {
+        if (i < ch.size()) {
+            const int b = (int)(i & 1);
+            CS_HIP(hipMemcpyAsync(s->stage[b], ch[i].src, ch[i].len, hipMemcpyDeviceToHost, s->st));
+            CS_HIP(hipEventRecord(s->stage_ev[b], s->st));
+        }
+        if (i > 0) {
+            const int b = (int)((i - 1) & 1);
+            CS_HIP(hipEventSynchronize(s->stage_ev[b]));
+            host_copy(ch[i - 1].dst, (const char*)s->stage[b], ch[i - 1].len);
+        }
+    }
+}
+
+void dots(cs_server* s, Solve& v, const double* A, const double* B, int64_t n, DevBuf& out,
+          std::vector<double>* host) {
+    const int64_t k = v.cols;
+    k_srv_dots<<<dim3(kSrvDotBlocks, (unsigned)k), 256, 0, s->st>>>(A, B, n, v.partial.as<double>());
+    k_srv_reduce<<<grid_for(k), 256, 0, s->st>>>(v.partial.as<double>(), k, out.as<double>());
     CS_HIP(hipGetLastError());
     if (host) {
         CS_HIP(hipMemcpyAsync(host->data(), out.p, (size_t)k * 8, hipMemcpyDeviceToHost, s->st));
@@ -219,87 +283,54 @@ void dots(cs_server* s, const double* A, const double* B, int64_t k, DevBuf& out
     }
 }
 
-int64_t find_slots(cs_server* s, int64_t k) {
-    int64_t run = 0;
-    for (int64_t c = 0; c < s->cap; ++c) {
-        run = s->slot_req[(size_t)c] < 0 ? run + 1 : 0;
-        if (run == k) return c - k + 1;
-    }
-    return -1;
+int64_t result_bytes_of(const cs_server* s, const Request& q) {
+    const int64_t ns = q.snps.empty() ? s->bed->M : (int64_t)q.snps.size();
+    return ns * (4 * q.T + 3) * 8;
 }
 
-int64_t longest_free_run(const cs_server* s) {
-    int64_t run = 0, best = 0;
-    for (int64_t c = 0; c < s->cap; ++c) {
-        run = s->slot_req[(size_t)c] < 0 ? run + 1 : 0;
-        best = std::max(best, run);
-    }
-    return best;
+// Solve buffers of k columns over n rows: X, R, Z, P, K P, A P, the right-hand sides, dot partials.
+int64_t solve_bytes(int64_t n, int64_t k) { return 7 * n * k * 8 + k * (kSrvDotBlocks + 8) * 8; }
+
+// Rows the request's products read (padded), and the workspace of an apply of k columns.
+int64_t work_bytes(const cs_server* s, int64_t n, bool compact, int64_t k) {
+    return kinship_work_bytes(compact ? round_up(n, kPad) : s->bed->Npad, s->bed->Mpad, CS_PREC_INT8_L3, k);
 }
 
-// Put the k compact cohort columns of q.B into the slots [s0, s0 + k): scatter into full-sample
-// columns, preconditioner / mask, variance components, PCG init, squared norms and first r.z.
-void place_columns(cs_server* s, Request& q, int64_t s0, int64_t k) {
-    const int64_t N = s->N;
-    q.slot0 = s0;
-    q.cols = k;
-    for (int64_t c = 0; c < k; ++c) s->slot_req[(size_t)(s0 + c)] = q.id;
-    CS_HIP(hipMemsetAsync(s->Bs.as<double>() + s0 * N, 0, (size_t)(N * k) * 8, s->st));
-    k_srv_scatter<<<grid_for(q.n * k), 256, 0, s->st>>>(q.B.as<double>(), q.n, q.dkeep.as<int32_t>(), N, k,
-                                                       s->Bs.as<double>() + s0 * N);
-    std::vector<double> mv((size_t)(N * k), 0.0);
-    for (int64_t c = 0; c < k; ++c)
-        for (int64_t r = 0; r < q.n; ++r) {
-            const int64_t i = q.keep[(size_t)r];
-            mv[(size_t)(c * N + i)] = 1.0 / (q.sg * s->diag[(size_t)i] + q.se);
-        }
-    CS_HIP(hipMemcpyAsync(s->minv.as<double>() + s0 * N, mv.data(), mv.size() * 8, hipMemcpyHostToDevice, s->st));
-    for (int64_t c = 0; c < k; ++c) {
-        s->hsg[(size_t)(s0 + c)] = q.sg;
-        s->hse[(size_t)(s0 + c)] = q.se;
-    }
-    CS_HIP(hipMemcpyAsync(s->dsg.p, s->hsg.data(), (size_t)s->cap * 8, hipMemcpyHostToDevice, s->st));
-    CS_HIP(hipMemcpyAsync(s->dse.p, s->hse.data(), (size_t)s->cap * 8, hipMemcpyHostToDevice, s->st));
-    k_srv_init<<<grid_for(N * k), 256, 0, s->st>>>(s->Bs.as<double>(), s->minv.as<double>(), N, s0, k,
-                                                  s->Xs.as<double>(), s->R.as<double>(), s->Z.as<double>(),
-                                                  s->P.as<double>());
+// The n x k right-hand sides in q.B become the request's PCG columns: init, squared norms, first r.z.
+void place_columns(cs_server* s, Request& q, int64_t k) {
+    const int64_t n = q.n;
+    Solve& v = q.sv;
+    v.cols = k;
+    s->used += k;
+    for (DevBuf* b : {&v.X, &v.R, &v.Z, &v.P, &v.KP, &v.AP}) b->alloc((size_t)(n * k) * 8);
+    v.partial.alloc((size_t)(k * kSrvDotBlocks) * 8);
+    for (DevBuf* b : {&v.rz, &v.rzn, &v.pap, &v.rr, &v.bb}) b->alloc((size_t)k * 8);
+    v.act.alloc((size_t)k * 4);
+    v.hbb.assign((size_t)k, 0.0);
+    v.hrr.assign((size_t)k, 0.0);
+    v.hact.assign((size_t)k, 0);
+    k_srv_init<<<grid_for(n * k), 256, 0, s->st>>>(q.B.as<double>(), q.minv.as<double>(), n, k, v.X.as<double>(),
+                                                  v.R.as<double>(), v.Z.as<double>(), v.P.as<double>());
     CS_HIP(hipGetLastError());
-    const double* Bc = s->Bs.as<double>() + s0 * N;
-    k_srv_dots<<<dim3(kSrvDotBlocks, (unsigned)k), 256, 0, s->st>>>(Bc, Bc, N, s->partial.as<double>());
-    k_srv_reduce<<<grid_for(k), 256, 0, s->st>>>(s->partial.as<double>(), k, s->bb.as<double>() + s0);
-    k_srv_dots<<<dim3(kSrvDotBlocks, (unsigned)k), 256, 0, s->st>>>(s->R.as<double>() + s0 * N, s->Z.as<double>() + s0 * N,
-                                                                   N, s->partial.as<double>());
-    k_srv_reduce<<<grid_for(k), 256, 0, s->st>>>(s->partial.as<double>(), k, s->rz.as<double>() + s0);
-    CS_HIP(hipGetLastError());
-    CS_HIP(hipMemcpyAsync(s->hbb.data() + s0, s->bb.as<double>() + s0, (size_t)k * 8, hipMemcpyDeviceToHost, s->st));
-    CS_HIP(hipStreamSynchronize(s->st));
+    dots(s, v, v.R.as<double>(), v.Z.as<double>(), n, v.rz, nullptr);
+    dots(s, v, q.B.as<double>(), q.B.as<double>(), n, v.bb, &v.hbb);
     for (int64_t c = 0; c < k; ++c) {
-        s->hrr[(size_t)(s0 + c)] = s->hbb[(size_t)(s0 + c)];
-        s->hact[(size_t)(s0 + c)] = s->hbb[(size_t)(s0 + c)] > 0.0 ? 1 : 0;
+        v.hrr[(size_t)c] = v.hbb[(size_t)c];
+        v.hact[(size_t)c] = v.hbb[(size_t)c] > 0.0 ? 1 : 0;
     }
-    s->used = std::max(s->used, s0 + k);
+    v.act_dirty = true;
     q.chunk_iterations = 0;
 }
 
-// Free the request's slots (P and the mask zeroed so idle columns add nothing to the shared
-// apply) and its solve buffers; safe to call on any path, including failures.
-void release_slots(cs_server* s, Request& q) {
-    if (q.slot0 >= 0) {
-        const int64_t N = s->N;
-        (void)hipMemsetAsync(s->P.as<double>() + q.slot0 * N, 0, (size_t)(N * q.cols) * 8, s->st);
-        (void)hipMemsetAsync(s->minv.as<double>() + q.slot0 * N, 0, (size_t)(N * q.cols) * 8, s->st);
-        (void)hipStreamSynchronize(s->st);
-        for (int64_t c = 0; c < q.cols; ++c) {
-            s->slot_req[(size_t)(q.slot0 + c)] = -1;
-            s->hact[(size_t)(q.slot0 + c)] = 0;
-        }
-        while (s->used > 0 && s->slot_req[(size_t)(s->used - 1)] < 0) --s->used;
-        q.slot0 = -1;
-        q.cols = 0;
-    }
-    s->budget_used -= q.bytes;
-    q.bytes = 0;
+// Give back the columns, their buffers and the operator's workspace; safe on any path.
+void release_solve(cs_server* s, Request& q) {
+    Solve& v = q.sv;
+    if (v.cols > 0 || v.bytes > 0) (void)hipStreamSynchronize(s->st);
+    s->used -= v.cols;
+    s->budget_used -= v.bytes;
+    v = Solve();
     q.B.release();
+    if (q.K) kinship_release_workspace(q.K);
 }
 
 void free_results(cs_server* s, Request& q) {
@@ -308,24 +339,47 @@ void free_results(cs_server* s, Request& q) {
     q.result_bytes = 0;
 }
 
+// What a request holds from admission to completion: operator, compact copy, preconditioner,
+// projection.
+void release_hold(cs_server* s, Request& q) {
+    if (q.K) {
+        (void)cs_kinship_destroy(q.K);
+        q.K = nullptr;
+    }
+    q.minv.release();
+    q.pj.R.release();
+    q.pj.ZX.release();
+    s->budget_used -= q.hold_bytes;
+    q.hold_bytes = 0;
+}
+
 // A request that failed gives back everything it holds.
 void fail(cs_server* s, Request& q, const std::string& msg) {
-    release_slots(s, q);
+    release_solve(s, q);
     free_results(s, q);
-    q.dkeep.release();
+    release_hold(s, q);
     q.pj = PscanProj();
     q.status = REQ_FAILED;
     q.error = msg;
     q.t_done = s->now();
 }
 
-int64_t result_bytes_of(const cs_server* s, const Request& q) {
-    const int64_t ns = q.snps.empty() ? s->bed->M : (int64_t)q.snps.size();
-    return ns * (4 * q.T + 3) * 8;
-}
-
 bool admit(cs_server* s, Request& q) {
+    const int64_t n = q.n, res = result_bytes_of(s, q);
+    // operator lists (variant multiplicities, keep and its inverse), preconditioner, projection
+    const int64_t base = s->bed->M * 4 + (s->N + 1) * 4 + 2 * n * 4 + n * 8 + (q.T + 2 * q.c) * n * 8;
+    auto need_of = [&](bool compact, int64_t k) {
+        return base + (compact ? bed_rows_copy_bytes(s->bed, n) : 0) + solve_bytes(n, k) +
+               work_bytes(s, n, compact, k) + res;
+    };
+    const bool want_compact = n < s->N && (double)n <= kCompactFrac * (double)s->N;
     if (q.k == 0) {
+        // the right-hand sides are prepared only once the request could fit (at most T + c +
+        // n_cal columns)
+        const int64_t k_est = std::min<int64_t>(q.T + q.c + (int64_t)q.cal.size(), s->cap);
+        if (s->used + k_est > s->cap) return false;
+        const bool cpt = want_compact && need_of(true, k_est) <= s->budget;
+        if (need_of(cpt, k_est) <= s->budget && s->budget_used + need_of(cpt, k_est) > s->budget) return false;
         DevBuf B;
         int64_t L = 0;
         pscan_prepare_rhs(s->bed, q.keep.data(), q.n, q.n, q.Y.data(), q.n, q.T, q.X.data(), q.n, q.c,
@@ -335,58 +389,109 @@ bool admit(cs_server* s, Request& q) {
         q.B = std::move(B);
     }
     if (q.k > s->cap) throw InvalidArgument("server: request needs more columns than the server holds");
-    // solve buffers (B, the gathered solution, the projection) and the result arrays
-    const int64_t need = (int64_t)q.B.bytes * 3, res = result_bytes_of(s, q);
-    if (s->budget_used + need + res > s->budget) return false;
-    const int64_t s0 = find_slots(s, q.k);
-    if (s0 < 0) return false;
-    q.dkeep.alloc((size_t)q.n * 4);
-    CS_HIP(hipMemcpyAsync(q.dkeep.p, q.keep.data(), (size_t)q.n * 4, hipMemcpyHostToDevice, s->st));
-    q.bytes = need;
+    if (s->used + q.k > s->cap) return false;
+    const bool compact = want_compact && need_of(true, q.k) <= s->budget;   // else the copy never fits
+    const int64_t need = need_of(compact, q.k);
+    if (need > s->budget) throw InvalidArgument("server: request needs more HBM than the budget holds");
+    if (s->budget_used + need > s->budget) return false;
+    const int rc = cs_kinship_create(s->bed, q.keep.data(), n, s->variants.empty() ? nullptr : s->variants.data(),
+                                     (int64_t)s->variants.size(), &q.K);
+    if (rc != CS_OK) throw RuntimeError(cs_last_error());
+    q.compact = compact;
+    q.hold_bytes = base + (compact ? bed_rows_copy_bytes(s->bed, n) : 0);
+    q.sv.bytes = solve_bytes(n, q.k) + work_bytes(s, n, compact, q.k);
     q.result_bytes = res;
-    s->budget_used += need + res;
-    place_columns(s, q, s0, q.k);
+    q.admit_bytes = q.hold_bytes + q.sv.bytes;
+    s->budget_used += need;
+    if (compact) kinship_compact_rows(q.K);
+    std::vector<double> mv((size_t)n);
+    for (int64_t r = 0; r < n; ++r) mv[(size_t)r] = 1.0 / (q.sg * s->diag[(size_t)q.keep[(size_t)r]] + q.se);
+    q.minv.alloc((size_t)n * 8);
+    CS_HIP(hipMemcpyAsync(q.minv.p, mv.data(), (size_t)n * 8, hipMemcpyHostToDevice, s->st));
     q.status = REQ_ACTIVE;
     q.t_admit = s->now();
+    place_columns(s, q, q.k);
     return true;
 }
 
-// Place the next exact-stage chunk of q's selected markers if slots and budget allow: as many
-// rows as fit the longest free slot run and the budget left (prepared markers, gathered solution
-// and residual: three n x chunk FP64 arrays).
+// Place the next exact-stage chunk of q's selected markers if capacity and budget allow: the
+// largest chunk that fits both.
 bool admit_chunk(cs_server* s, Request& q) {
     const int64_t left = (int64_t)q.rows.size() - q.row_next;
-    const int64_t per_col = 3 * q.n * 8;
-    int64_t sb = std::min<int64_t>(left, longest_free_run(s));
-    sb = std::min<int64_t>(sb, (s->budget - s->budget_used) / per_col);
-    if (sb < 1) return false;
-    const int64_t s0 = find_slots(s, sb);
+    const int64_t maxc = std::min<int64_t>(left, s->cap - s->used);
+    const int64_t avail = s->budget - s->budget_used;
+    auto cost = [&](int64_t sb) { return solve_bytes(q.n, sb) + work_bytes(s, q.n, q.compact, sb); };
+    if (maxc < 1 || cost(1) > avail) return false;
+    int64_t lo = 1, hi = maxc;   // cost(lo) fits
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo + 1) / 2;
+        if (cost(mid) <= avail) lo = mid;
+        else hi = mid - 1;
+    }
+    const int64_t sb = lo;
     std::vector<int32_t> sel((size_t)sb);
     for (int64_t i = 0; i < sb; ++i) {
         const int64_t r = q.rows[(size_t)(q.row_next + i)];
         sel[(size_t)i] = q.snps.empty() ? (int32_t)r : q.snps[(size_t)r];
     }
+    q.sv.bytes = cost(sb);
+    s->budget_used += q.sv.bytes;
     q.B.alloc((size_t)(q.n * sb) * 8);
-    q.bytes = per_col * sb;
-    s->budget_used += q.bytes;
     markers_device(s->bed, q.keep.data(), q.n, sel.data(), sb, q.min_maf, q.B.as<double>(), q.n, nullptr, nullptr,
                    s->st);
     q.chunk = sb;
-    place_columns(s, q, s0, sb);
+    place_columns(s, q, sb);
     return true;
 }
 
-// The request's columns in the slots have all converged: end of phase 1 (projection,
-// calibration, fused scan, selection of the exact rows) or of an exact-stage chunk.
+// One PCG iteration over the request's columns.
+void iterate(cs_server* s, Request& q) {
+    Solve& v = q.sv;
+    const int64_t n = q.n, k = v.cols;
+    if (v.act_dirty) {
+        CS_HIP(hipMemcpyAsync(v.act.p, v.hact.data(), (size_t)k * 4, hipMemcpyHostToDevice, s->st));
+        v.act_dirty = false;
+    }
+    CS_HIP(hipStreamSynchronize(s->st));
+    const int rc = cs_kinship_apply_device(q.K, v.P.as<double>(), n, k, v.KP.as<double>(), n);
+    if (rc != CS_OK) throw RuntimeError(cs_last_error());
+    k_srv_apply<<<grid_for(n * k), 256, 0, s->st>>>(v.KP.as<double>(), v.P.as<double>(), q.sg, q.se, n * k,
+                                                   v.AP.as<double>());
+    dots(s, v, v.P.as<double>(), v.AP.as<double>(), n, v.pap, nullptr);
+    k_srv_step<<<grid_for(n * k), 256, 0, s->st>>>(v.X.as<double>(), v.R.as<double>(), v.Z.as<double>(),
+                                                  v.P.as<double>(), v.AP.as<double>(), q.minv.as<double>(),
+                                                  v.rz.as<double>(), v.pap.as<double>(), v.act.as<int32_t>(), n, k);
+    dots(s, v, v.R.as<double>(), v.Z.as<double>(), n, v.rzn, nullptr);
+    k_srv_dir<<<grid_for(n * k), 256, 0, s->st>>>(v.P.as<double>(), v.Z.as<double>(), v.rzn.as<double>(),
+                                                 v.rz.as<double>(), v.act.as<int32_t>(), n, k);
+    k_srv_rz_keep<<<grid_for(k), 256, 0, s->st>>>(v.rz.as<double>(), v.rzn.as<double>(), v.act.as<int32_t>(), k);
+    CS_HIP(hipGetLastError());
+    dots(s, v, v.R.as<double>(), v.R.as<double>(), n, v.rr, &v.hrr);
+    int& its = q.phase == 1 ? q.iterations : q.chunk_iterations;
+    ++its;
+    // convergence per column (the exact stage solves to its own tolerance)
+    const double tol = q.phase == 1 ? q.tol : pscan_exact_tol(q.tol);
+    for (int64_t c = 0; c < k; ++c) {
+        if (!v.hact[(size_t)c]) continue;
+        const double b2 = v.hbb[(size_t)c];
+        if (b2 == 0.0 || std::sqrt(v.hrr[(size_t)c] / b2) <= tol || its >= q.max_iter) {
+            v.hact[(size_t)c] = 0;
+            v.act_dirty = true;
+        }
+    }
+}
+
+bool live(const Request& q) {
+    for (int32_t a : q.sv.hact)
+        if (a) return true;
+    return false;
+}
+
+// The request's columns have all converged: end of phase 1 (projection, calibration, fused scan,
+// selection of the exact rows) or of an exact-stage chunk.
 void advance(cs_server* s, Request& q) {
-    const int64_t N = s->N, n = q.n;
+    const int64_t n = q.n;
     if (q.phase == 1) {
-        const int64_t k = q.k;
-        DevBuf Zc((size_t)(n * k) * 8);
-        k_srv_gather<<<grid_for(n * k), 256, 0, s->st>>>(s->Xs.as<double>() + q.slot0 * N, N, q.dkeep.as<int32_t>(),
-                                                        n, k, Zc.as<double>());
-        CS_HIP(hipGetLastError());
-        CS_HIP(hipStreamSynchronize(s->st));
         const int64_t ns = q.snps.empty() ? s->bed->M : (int64_t)q.snps.size();
         q.ns = ns;
         q.freq.alloc((size_t)ns * 8);
@@ -396,9 +501,8 @@ void advance(cs_server* s, Request& q) {
         q.se_out.alloc((size_t)(ns * q.T) * 8);
         q.p.alloc((size_t)(ns * q.T) * 8);
         q.log10p.alloc((size_t)(ns * q.T) * 8);
-        pscan_project(n, q.T, q.c, q.L, q.B.as<double>(), Zc.as<double>(), q.pj, s->st, s->blas);
-        Zc.release();
-        release_slots(s, q);
+        pscan_project(n, q.T, q.c, q.L, q.B.as<double>(), q.sv.X.as<double>(), q.pj, s->st, s->blas);
+        release_solve(s, q);
         pscan_scan(s->bed, q.keep.data(), q.pj, q.snps.empty() ? nullptr : q.snps.data(), ns, q.min_maf, q.prec,
                    q.freq.as<double>(), q.n_obs.as<int64_t>(), q.pass.as<int32_t>(), q.beta.as<double>(),
                    q.se_out.as<double>(), q.p.as<double>(), q.log10p.as<double>(), true);
@@ -407,25 +511,18 @@ void advance(cs_server* s, Request& q) {
         q.phase = 2;
     } else {
         const int64_t sb = q.chunk;
-        DevBuf Zc((size_t)(n * sb) * 8), Rc((size_t)(n * sb) * 8);
-        k_srv_gather<<<grid_for(n * sb), 256, 0, s->st>>>(s->Xs.as<double>() + q.slot0 * N, N, q.dkeep.as<int32_t>(),
-                                                         n, sb, Zc.as<double>());
-        k_srv_gather<<<grid_for(n * sb), 256, 0, s->st>>>(s->R.as<double>() + q.slot0 * N, N, q.dkeep.as<int32_t>(),
-                                                         n, sb, Rc.as<double>());
-        CS_HIP(hipGetLastError());
         // the recurrence residual stands for B - V Z (its drift from the true residual is at the
         // operator's quantisation level)
-        pscan_exact(q.pj, sb, q.rows.data() + q.row_next, q.B.as<double>(), Zc.as<double>(), Rc.as<double>(), true,
-                    q.beta.as<double>(), q.se_out.as<double>(), q.p.as<double>(), q.log10p.as<double>(), true, s->st,
-                    s->blas);
+        pscan_exact(q.pj, sb, q.rows.data() + q.row_next, q.B.as<double>(), q.sv.X.as<double>(),
+                    q.sv.R.as<double>(), true, q.beta.as<double>(), q.se_out.as<double>(), q.p.as<double>(),
+                    q.log10p.as<double>(), true, s->st, s->blas);
         q.exact_iterations += q.chunk_iterations;
         q.row_next += sb;
-        release_slots(s, q);
+        release_solve(s, q);
     }
     if (q.row_next >= (int64_t)q.rows.size()) {
-        q.dkeep.release();
-        q.pj.R.release();
-        q.pj.ZX.release();
+        CS_HIP(hipStreamSynchronize(s->st));
+        release_hold(s, q);
         q.status = REQ_DONE;
         q.t_done = s->now();
     }
@@ -446,28 +543,22 @@ int cs_server_create(cs_bed* bed, const int32_t* variants, int64_t n_variants, i
         s->N = bed->N;
         s->cap = max_columns;
         s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)1 << 40;
-        const int rc = cs_kinship_create(bed, nullptr, 0, variants, n_variants, &s->K);
-        if (rc != CS_OK) throw RuntimeError(cs_last_error());
+        if (variants && n_variants > 0) s->variants.assign(variants, variants + n_variants);
         CS_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        for (int b = 0; b < 2; ++b) {
+            CS_HIP(hipHostMalloc(&s->stage[b], kStageBytes, hipHostMallocDefault));
+            CS_HIP(hipEventCreateWithFlags(&s->stage_ev[b], hipEventDisableTiming));
+        }
         CS_BLAS4(rocblas_create_handle(&s->blas));
         CS_BLAS4(rocblas_set_stream(s->blas, s->st));
-        const size_t cols = (size_t)(s->N * s->cap) * 8;
-        for (DevBuf* b : {&s->Xs, &s->R, &s->Z, &s->P, &s->KP, &s->AP, &s->Bs, &s->minv}) {
-            b->alloc(cols);
-            CS_HIP(hipMemsetAsync(b->p, 0, cols, s->st));
-        }
-        for (DevBuf* b : {&s->dsg, &s->dse, &s->rz, &s->rzn, &s->pap, &s->rr, &s->bb}) b->alloc((size_t)s->cap * 8);
-        s->act.alloc((size_t)s->cap * 4);
-        s->partial.alloc((size_t)(s->cap * kSrvDotBlocks) * 8);
+        // diag(K) over all samples (the requests' Jacobi preconditioners gather from it)
+        cs_kinship* K = nullptr;
+        if (cs_kinship_create(bed, nullptr, 0, variants, n_variants, &K) != CS_OK) throw RuntimeError(cs_last_error());
         s->diag.resize((size_t)s->N);
-        if (cs_kinship_diag(s->K, s->diag.data()) != CS_OK) throw RuntimeError(cs_last_error());
-        s->hsg.assign((size_t)s->cap, 0.0);
-        s->hse.assign((size_t)s->cap, 1.0);
-        s->hbb.assign((size_t)s->cap, 0.0);
-        s->hrr.assign((size_t)s->cap, 0.0);
-        s->hact.assign((size_t)s->cap, 0);
-        s->slot_req.assign((size_t)s->cap, -1);
-        CS_HIP(hipStreamSynchronize(s->st));
+        const int rc = cs_kinship_diag(K, s->diag.data());
+        const std::string err = rc == CS_OK ? "" : cs_last_error();
+        (void)cs_kinship_destroy(K);
+        if (rc != CS_OK) throw RuntimeError(err);
         *out = s.release();
     });
 }
@@ -534,7 +625,7 @@ int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t*
         // exact-stage chunks of active requests first (they hold results already), in id order
         for (auto& kv : s->reqs) {
             Request& q = *kv.second;
-            if (q.status != REQ_ACTIVE || q.phase != 2 || q.slot0 >= 0) continue;
+            if (q.status != REQ_ACTIVE || q.phase != 2 || q.sv.cols > 0) continue;
             try {
                 (void)admit_chunk(s, q);
             } catch (const std::exception& e) {
@@ -555,62 +646,19 @@ int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t*
             if (ok) it = s->pending.erase(it);
             else ++it;
         }
-        const int64_t N = s->N, k = s->used;
-        bool any = false;
-        for (int64_t c = 0; c < k; ++c) any = any || s->hact[(size_t)c];
-        if (any) {
-            CS_HIP(hipMemcpyAsync(s->act.p, s->hact.data(), (size_t)k * 4, hipMemcpyHostToDevice, s->st));
-            CS_HIP(hipStreamSynchronize(s->st));
-            const int rc = cs_kinship_apply_device(s->K, s->P.as<double>(), N, k, s->KP.as<double>(), N);
-            if (rc != CS_OK) throw RuntimeError(cs_last_error());
-            k_srv_apply<<<grid_for(N * k), 256, 0, s->st>>>(s->KP.as<double>(), s->P.as<double>(), s->minv.as<double>(),
-                                                           s->dsg.as<double>(), s->dse.as<double>(), N, k,
-                                                           s->AP.as<double>());
-            dots(s, s->P.as<double>(), s->AP.as<double>(), k, s->pap, nullptr);
-            k_srv_step<<<grid_for(N * k), 256, 0, s->st>>>(s->Xs.as<double>(), s->R.as<double>(), s->Z.as<double>(),
-                                                          s->P.as<double>(), s->AP.as<double>(), s->minv.as<double>(),
-                                                          s->rz.as<double>(), s->pap.as<double>(), s->act.as<int32_t>(),
-                                                          N, k);
-            dots(s, s->R.as<double>(), s->Z.as<double>(), k, s->rzn, nullptr);
-            k_srv_dir<<<grid_for(N * k), 256, 0, s->st>>>(s->P.as<double>(), s->Z.as<double>(), s->rzn.as<double>(),
-                                                         s->rz.as<double>(), s->act.as<int32_t>(), N, k);
-            CS_HIP(hipGetLastError());
-            // rz <- rz_new on the active columns only
-            std::vector<double> rzn((size_t)k), rzo((size_t)k);
-            CS_HIP(hipMemcpyAsync(rzn.data(), s->rzn.p, (size_t)k * 8, hipMemcpyDeviceToHost, s->st));
-            CS_HIP(hipMemcpyAsync(rzo.data(), s->rz.p, (size_t)k * 8, hipMemcpyDeviceToHost, s->st));
-            dots(s, s->R.as<double>(), s->R.as<double>(), k, s->rr, &s->hrr);
-            for (int64_t c = 0; c < k; ++c)
-                if (s->hact[(size_t)c]) rzo[(size_t)c] = rzn[(size_t)c];
-            CS_HIP(hipMemcpyAsync(s->rz.p, rzo.data(), (size_t)k * 8, hipMemcpyHostToDevice, s->st));
-            CS_HIP(hipStreamSynchronize(s->st));
-            // per-request iteration counts and convergence (the exact stage solves to its own
-            // tolerance)
-            std::map<int64_t, bool> stepped;
-            for (int64_t c = 0; c < k; ++c)
-                if (s->hact[(size_t)c] && s->slot_req[(size_t)c] >= 0) stepped[s->slot_req[(size_t)c]] = true;
-            for (auto& kv : stepped) {
-                Request& q = *s->reqs[kv.first];
-                if (q.phase == 1) q.iterations++;
-                else q.chunk_iterations++;
-            }
-            for (int64_t c = 0; c < k; ++c) {
-                if (!s->hact[(size_t)c] || s->slot_req[(size_t)c] < 0) continue;
-                const Request& q = *s->reqs[s->slot_req[(size_t)c]];
-                const double b2 = s->hbb[(size_t)c];
-                const double tol = q.phase == 1 ? q.tol : pscan_exact_tol(q.tol);
-                const int its = q.phase == 1 ? q.iterations : q.chunk_iterations;
-                if (b2 == 0.0 || std::sqrt(s->hrr[(size_t)c] / b2) <= tol || its >= q.max_iter)
-                    s->hact[(size_t)c] = 0;
-            }
-        }
-        // advance requests whose columns have all converged
+        // one PCG iteration of every request with live columns, then advance the converged ones
         for (auto& kv : s->reqs) {
             Request& q = *kv.second;
-            if (q.status != REQ_ACTIVE || q.slot0 < 0) continue;
-            bool live = false;
-            for (int64_t c = 0; c < q.cols; ++c) live = live || s->hact[(size_t)(q.slot0 + c)];
-            if (live) continue;
+            if (q.status != REQ_ACTIVE || q.sv.cols == 0 || !live(q)) continue;
+            try {
+                iterate(s, q);
+            } catch (const std::exception& e) {
+                fail(s, q, e.what());
+            }
+        }
+        for (auto& kv : s->reqs) {
+            Request& q = *kv.second;
+            if (q.status != REQ_ACTIVE || q.sv.cols == 0 || live(q)) continue;
             try {
                 advance(s, q);
                 if (q.status == REQ_DONE) ++completed;
@@ -646,22 +694,20 @@ int cs_server_result(cs_server* s, int64_t id, int32_t* status, double* freq, in
             stats->exact_iterations = q.exact_iterations;
             stats->calibration_rel_sd = q.pj.rel_sd;
             stats->dlog10p_bound_below = pscan_bound_below(q.thr, q.pj.rel_sd);
+            stats->admit_bytes = q.admit_bytes;
+            stats->compact_rows = q.compact ? 1 : 0;
         }
         const bool wants = freq || n_obs || pass || beta || se || p || log10p;
         if (q.status == REQ_FAILED && wants) throw RuntimeError("request failed: " + q.error);
         if (q.status != REQ_DONE) return;
-        auto cp = [&](void* dst, const DevBuf& v, size_t bytes) {
-            if (dst && bytes) CS_HIP(hipMemcpyAsync(dst, v.p, bytes, hipMemcpyDeviceToHost, s->st));
-        };
         const size_t ns = (size_t)q.ns, nt = (size_t)(q.ns * q.T);
-        cp(freq, q.freq, ns * 8);
-        cp(n_obs, q.n_obs, ns * 8);
-        cp(pass, q.pass, ns * 4);
-        cp(beta, q.beta, nt * 8);
-        cp(se, q.se_out, nt * 8);
-        cp(p, q.p, nt * 8);
-        cp(log10p, q.log10p, nt * 8);
-        CS_HIP(hipStreamSynchronize(s->st));
+        deliver(s, {{freq, q.freq.p, ns * 8},
+                    {n_obs, q.n_obs.p, ns * 8},
+                    {pass, q.pass.p, ns * 4},
+                    {beta, q.beta.p, nt * 8},
+                    {se, q.se_out.p, nt * 8},
+                    {p, q.p.p, nt * 8},
+                    {log10p, q.log10p.p, nt * 8}});
     });
 }
 

@@ -110,6 +110,15 @@ int cs_kinship_set_precision(cs_kinship* op, int mode);
  * 64-byte row per missing genotype and 16 columns; at n = 50k, m = 500k, k = 16 they break even
  * with the O products near 0.2 % missing. */
 int cs_kinship_set_missing_mode(cs_kinship* op, int mode);
+/* No counterpart in the reference (an HBM-for-work trade of this library): run the operator's
+ * genotype products on a compact copy of its cohort's rows, gathered once from the resident
+ * matrix (2 * roundup(n, 256) * roundup(M, 256) / 4 bytes more), so that each apply reads and
+ * multiplies n instead of N sample rows.  The copy keeps the all-sample standardisation, and
+ * results are bitwise those of the full-sample products.  No-op for the all-sample cohort;
+ * CS_ERR_INVALID_ARGUMENT when keep repeats a sample. */
+int cs_kinship_compact_rows(cs_kinship* op);
+/* Device bytes the operator holds: its lists, grown workspaces and compact copy. */
+int cs_kinship_device_bytes(const cs_kinship* op, int64_t* bytes);
 /* apply / apply_block / applyTranspose (kinship_operator.cpp:211-304): Y = K X for X n x k
  * column-major with leading dimension ldx; Y may alias X; k == 0 is allowed.  ldx < n or
  * n_threads-like misuse -> CS_ERR_RUNTIME "dimension mismatch". */
@@ -365,17 +374,24 @@ typedef struct cs_request_stats {
     int64_t n_exact;                         /* SNPs recomputed by the exact stage */
     int exact_iterations;                    /* PCG iterations of the exact stage, all chunks */
     double calibration_rel_sd, dlog10p_bound_below;
+    int64_t admit_bytes;                     /* device bytes charged at admission, results excluded */
+    int compact_rows;                        /* 1: products on a compact copy of the cohort's rows */
 } cs_request_stats;
-/* max_columns: PCG column slots (N x max_columns x 8 doubles of HBM); hbm_budget_bytes: bound on
- * the per-request device buffers (<= 0: unbounded): a request is admitted only when its solve
- * buffers and its result arrays (n_snps x (4 T + 3) x 8 bytes) fit, the results stay charged until
- * cs_server_release, and exact-stage chunks are sized to what is left. */
+/* max_columns: PCG columns held at once over all active requests; hbm_budget_bytes: bound on the
+ * per-request device memory (<= 0: unbounded).  Each admitted request owns a kinship operator over
+ * its cohort, whose products run on a compact copy of the cohort's genotype rows (2 roundup(n,
+ * 256) roundup(M, 256) / 4 bytes, cs_kinship_compact_rows) when n <= 3/4 N and the budget can hold
+ * it, else on the full matrix.  A request is admitted only when its columns fit and its device
+ * memory fits the budget: the compact copy, operator lists, preconditioner and projection (held
+ * to completion), the solve buffers and operator workspace (held while its columns are), and its
+ * result arrays (n_snps x (4 T + 3) x 8 bytes, held until cs_server_release); exact-stage chunks
+ * are sized to what is left.  A request that can never fit fails with CS_ERR_INVALID_ARGUMENT. */
 int cs_server_create(cs_bed* bed, const int32_t* variants, int64_t n_variants, int64_t max_columns,
                      int64_t hbm_budget_bytes, cs_server** out);
 int cs_server_destroy(cs_server* srv);
 int cs_server_submit(cs_server* srv, const cs_request* req, int64_t* id);
-/* One scheduler iteration: admit pending requests that fit, one PCG iteration over all active
- * columns, finish the requests whose columns converged. */
+/* One scheduler iteration: admit pending requests that fit, one PCG iteration (one K.V apply) of
+ * every active request, finish the requests whose columns converged. */
 int cs_server_step(cs_server* srv, int32_t* n_active, int32_t* n_pending, int32_t* n_completed);
 /* status: 0 pending, 1 active, 2 done, -1 failed.  Asking for outputs of a failed request gives
  * CS_ERR_RUNTIME with its message.  When done, outputs as cs_pcg_projected_scan (any may be

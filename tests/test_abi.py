@@ -36,6 +36,10 @@ def test_error_path_without_gpu_calls():
     # null handle -> invalid argument, message set, no device work involved
     assert L.cs_kinship_rows(None, None) == csa.CS_ERR_INVALID_ARGUMENT
     assert b"null" in L.cs_last_error()
+    assert L.cs_kinship_compact_rows(None) == csa.CS_ERR_INVALID_ARGUMENT
+    assert b"cs_kinship_compact_rows" in L.cs_last_error()
+    b = ctypes.c_int64()
+    assert L.cs_kinship_device_bytes(None, ctypes.byref(b)) == csa.CS_ERR_INVALID_ARGUMENT
 
 
 def test_dense_eigen_argument_checks_without_gpu_calls():

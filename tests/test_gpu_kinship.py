@@ -318,3 +318,66 @@ def test_multiply_z_and_zt(csa):
         assert rc == csa.CS_ERR_RUNTIME and b"multiply_Zt_v" in L.cs_last_error()
         op.close()
     reader.close()
+
+
+@pytest.mark.parametrize("miss", [0.001, 0.02])
+def test_compact_rows_bitwise(csa, miss):
+    """cs_kinship_compact_rows: the genotype products on a compact copy of the cohort's rows give
+    bitwise the full-sample products' results -- unsorted and sorted cohorts, a variant
+    selection, INT8_L2/L3/L4, dense / sparse / auto missing forms (the copy has its own missing
+    fraction), packed (k L <= 16) and grouped operands, Z^T V and Z U alone, and the
+    no-allocation entry point.  FP32 (summation order changes) within its bound of the oracle;
+    trace and diag unchanged; repeated samples refused."""
+    N, M = 1800, 3100
+    r = csa.PlinkReader.synthetic(41, N, M, 0.01, 0.5, miss)
+    rng = np.random.default_rng(9)
+    variants = np.sort(rng.choice(M, 2400, replace=False)).astype(np.int32)
+    body = r.download()
+    means, inv = oracle.stats(body, N, M)
+    for keep in (rng.permutation(N)[:700].astype(np.int32), np.sort(rng.choice(N, 1200, replace=False)).astype(np.int32)):
+        n = keep.size
+        print(f"compact rows: miss={miss} n={n}", flush=True)
+        full = csa.PlinkKinshipOperator(r, keep, variants)
+        comp = csa.PlinkKinshipOperator(r, keep, variants)
+        b0 = comp.device_bytes()
+        comp.compact_rows()
+        comp.compact_rows()                       # idempotent
+        npad, mpad = -(-n // 256) * 256, -(-M // 256) * 256
+        assert comp.device_bytes() - b0 >= 2 * npad * mpad // 4
+        assert abs(comp.trace() / full.trace() - 1) <= 1e-12
+        assert np.array_equal(comp.getDiagonal(), full.getDiagonal())
+        for prec in (csa.CS_PREC_INT8_L3, csa.CS_PREC_INT8_L2, csa.CS_PREC_INT8_L4):
+            full.set_precision(prec)
+            comp.set_precision(prec)
+            for k in (3, 20):
+                X = rng.standard_normal((n, k))
+                for mode in (1, 2, 0):
+                    full.set_missing_mode(mode)
+                    comp.set_missing_mode(mode)
+                    assert np.array_equal(comp.apply(X), full.apply(X)), (prec, k, mode)
+        full.set_precision(csa.CS_PREC_INT8_L3)
+        comp.set_precision(csa.CS_PREC_INT8_L3)
+        X = rng.standard_normal((n, 9))
+        U = rng.standard_normal((M, 9))
+        assert np.array_equal(comp.multiply_Zt_v(X), full.multiply_Zt_v(X))
+        assert np.array_equal(comp.multiply_Z_v(U), full.multiply_Z_v(U))
+        dX = csa.DeviceArray.from_host(X)
+        dY = csa.DeviceArray(n, 9)
+        comp.apply_device_noalloc(dX.ptr, n, 9, dY.ptr, n)
+        assert np.array_equal(dY.download(), full.apply(X))
+        dX.close()
+        dY.close()
+        comp.set_precision(csa.CS_PREC_FP32)
+        ref = oracle.kinship_apply(body, N, means, inv, keep, variants, X)
+        assert colrel(comp.apply(X), ref) < 2e-5
+        full.close()
+        comp.close()
+    dup = csa.PlinkKinshipOperator(r, np.array([3, 5, 3, 9], dtype=np.int32), None)
+    with pytest.raises(ValueError, match="twice"):
+        dup.compact_rows()
+    dup.close()
+    allrows = csa.PlinkKinshipOperator(r, None, None)
+    b0 = allrows.device_bytes()
+    allrows.compact_rows()                        # the all-sample cohort: nothing to copy
+    assert allrows.device_bytes() == b0
+    allrows.close()

@@ -163,9 +163,15 @@ def test_budget_holds_unreleased_results(csa, world):
                 rel_tol=1e-8, max_iter=400)
     qa = dict(base, Y=rng.standard_normal((N, T)))
     qb = dict(base, Y=rng.standard_normal((N, T)))
-    k = T + c + 16
-    need = 3 * N * k * 8                   # solve buffers while admitted
-    res = snps.size * (4 * T + 3) * 8      # result arrays until release
+    # what a request holds from admission (operator, preconditioner, projection, solve buffers and
+    # workspace; reported in its stats) and its result arrays until release
+    alone = csa.ScanServer(reader, max_columns=64)
+    rid = alone.submit(**qa)
+    alone.run_until_idle()
+    st = alone.status(rid)[1]
+    need = st["admit_bytes"]
+    assert st["compact_rows"] == 0 and need > 10 * N * (T + c + 16) * 8
+    res = snps.size * (4 * T + 3) * 8
     srv = csa.ScanServer(reader, max_columns=64, hbm_budget_bytes=need + res + res // 2)
     ia, ib = srv.submit(**qa), srv.submit(**qb)
     for _ in range(2000):
