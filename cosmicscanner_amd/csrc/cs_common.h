@@ -56,22 +56,36 @@ int guarded(F&& f) {
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 // ---------------------------------------------------------------------------------------------
-// Device buffer (RAII over hipMalloc).
+// Device memory with a per-device cache of large blocks (cs_alloc.cpp): dev_alloc returns a block
+// of at least n bytes and its capacity, dev_free takes it back (large blocks are kept for reuse),
+// dev_cache_release frees every cached block.
+void* dev_alloc(size_t n, size_t* cap);
+void dev_free(void* p, size_t cap);
+void dev_cache_release();
+
+// Device buffer (RAII over dev_alloc).  `bytes` is the size asked for; `cap` the block's capacity.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    size_t cap = 0;
     DevBuf() = default;
     explicit DevBuf(size_t n) { alloc(n); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes), cap(o.cap) {
+        o.p = nullptr;
+        o.bytes = 0;
+        o.cap = 0;
+    }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
             release();
             p = o.p;
             bytes = o.bytes;
+            cap = o.cap;
             o.p = nullptr;
             o.bytes = 0;
+            o.cap = 0;
         }
         return *this;
     }
@@ -79,16 +93,22 @@ struct DevBuf {
     void alloc(size_t n) {
         release();
         if (n == 0) n = 1;
-        CS_HIP(hipMalloc(&p, n));
+        p = dev_alloc(n, &cap);
         bytes = n;
     }
     void ensure(size_t n) {
-        if (n > bytes) alloc(n);
+        if (n <= bytes) return;
+        if (p && n <= cap) {
+            bytes = n;
+            return;
+        }
+        alloc(n);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) dev_free(p, cap);
         p = nullptr;
         bytes = 0;
+        cap = 0;
     }
     template <class T>
     T* as() const { return static_cast<T*>(p); }
