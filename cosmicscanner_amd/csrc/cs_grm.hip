@@ -10,12 +10,18 @@
 //   I_ik = 2^2SB T1 - 2^SB (R_i + R_k) + S + 2^SB (C_ik + C_ki) - Q_i - Q_k + D_ik
 //   T1 = X~ X~^T,  R_i = sum_j A_j x~_ij,  S = sum_j A_j^2,  Q_i = sum_j A_j^2 m_ij,
 //   C_ik = sum_j x~_ij A_j m_kj,  D_ik = sum_j A_j^2 m_ij m_kj   (m = missing indicator).
-// T1 is one int8 MFMA product of packed genotypes with themselves; C (missing genotypes against
-// genotypes) runs as int8 MFMA products with balanced signed 8-bit digits of A_j folded into the
-// missing-indicator operand, in both orders; D, nonzero only for pairs of samples missing at the
-// same SNP, is summed pair by pair from the per-SNP missing lists.  Without missing genotypes C
-// and D vanish.  The output is the correctly rounded FP64 of G_raw and the correctly rounded
-// FP32 of G_raw / sum_2pq (the GCTA .grm.bin value, grm_builder.cpp:1977-2026).
+// T1 is one int8 MFMA product of packed genotypes with themselves.  The C terms come from
+// products of one operand with itself: with A_j = sum_l 256^l d_lj in balanced signed 8-bit
+// digits, E_l holds x~_ij where genotype (i, j) is observed and d_lj where it is missing, so
+//   E_l E_l^T = T1 + C_l + C_l^T + D_l,   D_l,ik = sum_j d_lj^2 m_ij m_kj,
+// and  sum_l 256^l E_l E_l^T = W T1 + (C + C^T) + sum_j w_j m_ij m_kj  with W = sum_l 256^l,
+// w_j = sum_l 256^l d_lj^2.  Hence, with c = sum_l 256^l E_l E_l^T,
+//   I_ik = (2^2SB - 2^SB W) T1 - 2^SB (R_i + R_k) + S + 2^SB c - Q_i - Q_k + D^v_ik,
+//   D^v_ik = sum_j v_j m_ij m_kj,   v_j = A_j^2 - 2^SB w_j,
+// where D^v, nonzero only for pairs of samples missing at the same SNP, is summed from the
+// missing lists.  Without missing genotypes c and D^v vanish.  The output is the correctly
+// rounded FP64 of G_raw and the correctly rounded FP32 of G_raw / sum_2pq (the GCTA .grm.bin
+// value, grm_builder.cpp:1977-2026).
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -36,7 +42,6 @@ typedef unsigned __int128 u128;
 
 
 constexpr uint32_t kLutMF = 0x0000FF00u;   // code 01 -> 0xFF
-constexpr int64_t kDigitChunkKB = 8192;   // 2^21 SNPs per fused digit-product launch (both orders)
 
 struct GrmArgs {
     const uint8_t* smp;    // masked tiled sample-major genotypes
@@ -44,10 +49,11 @@ struct GrmArgs {
     int64_t kb0, kb1;      // SNP-block range of this launch
     const int2* blocks;    // (bi, bk) 128-row block pairs, bk <= bi
     int64_t I0, P, N, Npad;
-    const uint8_t* digA;   // digit array [kb][t][g][16] of the C pass (the missing-indicator operand)
-    int shift;             // 8 * limb
+    const uint8_t* digA;   // digit arrays [l][kb][t][g][16] of the E passes (l = digit, stride dstride)
+    int64_t dstride;
+    int shift;             // 8 * (first digit of the pass)
     int32_t* t1;           // panel accumulators, [P][Npad]
-    long long* c;
+    i128* c;
 };
 
 
@@ -125,14 +131,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
             }
 }
 
-// Both orders of one C digit product in one pass: acc += X~_i (M_k d) + (M_i d) X~_k, which is
-// the C_ik + C_ki the combine needs.  Each wave decodes its 64-row tile once per 64-k step into
-// both operand forms (the code extraction shared, then one perm per form), so a block decodes
-// each tile once for 32 MFMAs per wave instead of twice for 16; LDS holds both forms
-// (double-buffered, 64 KiB: two workgroups per CU).  With two products summed into one int32
-// accumulator (|x~ d| <= 256 each), a launch covers at most 2^21 SNPs.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_grm_cpass(GrmArgs a) {
-    __shared__ v4i lds[2][2][4][4][64];   // [buffer][form: X~, M d][tile][m-subtile][lane]
+// E passes: ND digits of c = sum_l 256^l E_l E_l^T per launch.  Each wave decodes its 64-row
+// tile once per 64-k step (the code extraction and the two perms shared by the digits, one
+// and-or per digit) into LDS (double-buffered, ND x 32 KiB) and multiplies the block's E_l tiles,
+// 16 MFMAs per digit and step.  Every term of E_l E_l^T is bounded by 4 (both observed), 256 (one
+// missing) or 16384 (both missing), so a launch covers a SNP range whose bound stays in int32
+// (chosen on the host from the largest per-sample missing count).
+template <int ND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_grm_epass(GrmArgs a) {
+    __shared__ v4i lds[2][ND][4][4][64];   // [buffer][digit][tile][m-subtile][lane]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
     const int2 blk = a.blocks[blockIdx.x];
@@ -143,11 +150,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const bool sideA = wave < 2;
     const int64_t trow = sideA ? (int64_t)blk.x * 128 + 64 * wave : (int64_t)blk.y * 128 + 64 * (wave - 2);
     const uint8_t* tbase = a.smp + (((trow >> 6) * a.KB) << 12) + lane * 16;
-    v4i acc[4][4];
+    v4i acc[ND][4][4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
+    for (int l = 0; l < ND; ++l)
 #pragma unroll
-        for (int n = 0; n < 4; ++n) acc[m][n] = v4i{0, 0, 0, 0};
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int n = 0; n < 4; ++n) acc[l][m][n] = v4i{0, 0, 0, 0};
     uint4 cv[4], cn[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) cv[m] = *reinterpret_cast<const uint4*>(tbase + (a.kb0 << 12) + m * 1024);
@@ -158,36 +167,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int m = 0; m < 4; ++m) cn[m] = *reinterpret_cast<const uint4*>(tbase + (kn << 12) + m * 1024);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            const uint4 d = *reinterpret_cast<const uint4*>(a.digA + (((kb * 4 + t) * 4 + g) << 4));
+            uint4 d[ND];
+#pragma unroll
+            for (int l = 0; l < ND; ++l)
+                d[l] = *reinterpret_cast<const uint4*>(a.digA + l * a.dstride + (((kb * 4 + t) * 4 + g) << 4));
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
                 const uint32_t w = t == 0 ? cv[m].x : t == 1 ? cv[m].y : t == 2 ? cv[m].z : cv[m].w;
-                const uint32_t d0 = w & 0x03030303u, d1 = (w >> 2) & 0x03030303u;
-                const uint32_t d2 = (w >> 4) & 0x03030303u, d3 = (w >> 6) & 0x03030303u;
-                lds[buf][0][wave][m][lane] =
-                    v4i{(int)__builtin_amdgcn_perm(0u, kLutX, d0), (int)__builtin_amdgcn_perm(0u, kLutX, d1),
-                        (int)__builtin_amdgcn_perm(0u, kLutX, d2), (int)__builtin_amdgcn_perm(0u, kLutX, d3)};
-                lds[buf][1][wave][m][lane] = v4i{(int)(__builtin_amdgcn_perm(0u, kLutMF, d0) & d.x),
-                                                 (int)(__builtin_amdgcn_perm(0u, kLutMF, d1) & d.y),
-                                                 (int)(__builtin_amdgcn_perm(0u, kLutMF, d2) & d.z),
-                                                 (int)(__builtin_amdgcn_perm(0u, kLutMF, d3) & d.w)};
+                const uint32_t s0 = w & 0x03030303u, s1 = (w >> 2) & 0x03030303u;
+                const uint32_t s2 = (w >> 4) & 0x03030303u, s3 = (w >> 6) & 0x03030303u;
+                const uint32_t x0 = __builtin_amdgcn_perm(0u, kLutX, s0), x1 = __builtin_amdgcn_perm(0u, kLutX, s1);
+                const uint32_t x2 = __builtin_amdgcn_perm(0u, kLutX, s2), x3 = __builtin_amdgcn_perm(0u, kLutX, s3);
+                const uint32_t f0 = __builtin_amdgcn_perm(0u, kLutMF, s0), f1 = __builtin_amdgcn_perm(0u, kLutMF, s1);
+                const uint32_t f2 = __builtin_amdgcn_perm(0u, kLutMF, s2), f3 = __builtin_amdgcn_perm(0u, kLutMF, s3);
+#pragma unroll
+                for (int l = 0; l < ND; ++l)
+                    lds[buf][l][wave][m][lane] = v4i{(int)((f0 & d[l].x) | x0), (int)((f1 & d[l].y) | x1),
+                                                     (int)((f2 & d[l].z) | x2), (int)((f3 & d[l].w) | x3)};
             }
             __syncthreads();
             if (active) {
-                v4i BX[4], BM[4];
 #pragma unroll
-                for (int n = 0; n < 4; ++n) {
-                    BX[n] = lds[buf][0][2 + wk][n][lane];
-                    BM[n] = lds[buf][1][2 + wk][n][lane];
-                }
+                for (int l = 0; l < ND; ++l) {
+                    v4i B[4];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const v4i AX = lds[buf][0][wi][m][lane];
-                    const v4i AM = lds[buf][1][wi][m][lane];
+                    for (int n = 0; n < 4; ++n) B[n] = lds[buf][l][2 + wk][n][lane];
 #pragma unroll
-                    for (int n = 0; n < 4; ++n) {
-                        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(AX, BM[n], acc[m][n], 0, 0, 0);
-                        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(AM, BX[n], acc[m][n], 0, 0, 0);
+                    for (int m = 0; m < 4; ++m) {
+                        const v4i A = lds[buf][l][wi][m][lane];
+#pragma unroll
+                        for (int n = 0; n < 4; ++n)
+                            acc[l][m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[n], acc[l][m][n], 0, 0, 0);
                     }
                 }
             }
@@ -206,7 +216,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const int64_t i = i0 + 16 * m + 4 * g + v;
                 const int64_t k = k0 + 16 * n + r;
                 if (i >= a.N || k >= a.N || k > i || i < a.I0 || i >= a.I0 + a.P) continue;
-                a.c[(i - a.I0) * a.Npad + k] += (long long)acc[m][n][v] * (1LL << a.shift);
+                i128 add = 0;
+#pragma unroll
+                for (int l = 0; l < ND; ++l) add += (i128)acc[l][m][n][v] << (a.shift + 8 * l);
+                a.c[(i - a.I0) * a.Npad + k] += add;
             }
 }
 
@@ -275,43 +288,74 @@ __global__ __launch_bounds__(256) void k_grm_rowsums(const uint8_t* __restrict__
     }
 }
 
-// D_ik = sum_j A_j^2 m_ij m_kj for the panel rows i in [I0, I1), k <= i: one wave per SNP of
-// [j0, j1) walks the pairs of its missing list (samples in increasing order) and adds A_j^2, split
-// into 42-bit limbs, to the two unsigned accumulators with atomics (each limb sum stays below
-// 2^63 for up to 2^21 SNPs).
+// D^v_ik = sum_j v_j m_ij m_kj for the panel rows i in [row0, row1) and k <= i, from the missing
+// lists: one workgroup per row i walks the row's missing SNPs j and, for each, the samples k
+// missing at j, adding v_j (split into three 42-bit limbs, the top one signed) to an LDS
+// accumulator of kDCols columns; chunks of columns are taken in order, each SNP keeping its list
+// position (cursor) in LDS from one chunk to the next, so every pair is visited once.  A chunk is
+// stored to the three output planes [row][Npad] (added to them for rows whose list is longer
+// than one cursor batch).  Each limb sum stays below 2^63 for up to 2^21 SNPs.
 constexpr int kDLimbBits = 42;
-__global__ __launch_bounds__(256) void k_grm_dpairs(const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
-                                                    const long long* __restrict__ A, int64_t j0, int64_t j1,
-                                                    int64_t I0, int64_t I1, int64_t Npad,
-                                                    unsigned long long* __restrict__ dhi,
-                                                    unsigned long long* __restrict__ dlo) {
-    const int lane = threadIdx.x & 63;
-    for (int64_t j = j0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); j < j1; j += (int64_t)gridDim.x * 4) {
-        const long long aj = A[j];
-        if (aj == 0) continue;
-        const int64_t o0 = off[j], o1 = off[j + 1];
-        if (o1 - o0 < 1) continue;
-        const u128 a2 = (u128)aj * (u128)aj;
-        const unsigned long long hi = (unsigned long long)(a2 >> kDLimbBits);
-        const unsigned long long lo = (unsigned long long)(a2 & ((((u128)1) << kDLimbBits) - 1));
-        // list positions of the panel rows: [p0, p1)
-        int64_t p0 = 0, p1 = 0;
-        for (int64_t q = o0; q < o1; q += 64) {
-            const int64_t e = q + lane;
-            const int32_t s = e < o1 ? idx[e] : 0x7fffffff;
-            p0 += __popcll(__ballot(s < I0));
-            p1 += __popcll(__ballot(s < I1));
-        }
-        for (int64_t a = o0 + p0; a < o0 + p1; ++a) {
-            const int64_t row = (int64_t)idx[a] - I0;
-            unsigned long long* rh = dhi + row * Npad;
-            unsigned long long* rl = dlo + row * Npad;
-            for (int64_t b = o0 + lane; b <= a; b += 64) {
-                const int32_t k = idx[b];
-                atomicAdd(rh + k, hi);
-                atomicAdd(rl + k, lo);
+constexpr int kDCols = 1024;
+constexpr int kDBatch = 4096;
+
+__global__ __launch_bounds__(256) void k_grm_dlds(const int64_t* __restrict__ soff, const int32_t* __restrict__ sidx,
+                                                  const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
+                                                  const unsigned long long* __restrict__ v3, int64_t row0,
+                                                  int64_t row1, int64_t I0, int64_t Npad,
+                                                  unsigned long long* __restrict__ d0,
+                                                  unsigned long long* __restrict__ d1,
+                                                  unsigned long long* __restrict__ d2) {
+    __shared__ unsigned long long acc[3][kDCols];
+    __shared__ int32_t cur[kDBatch];
+    const int64_t i = row0 + blockIdx.x;
+    if (i >= row1) return;   // whole workgroup, before any barrier
+    const int tid = threadIdx.x;
+    const int64_t s0 = soff[i], s1 = soff[i + 1];
+    const int64_t nch = i / kDCols + 1;
+    unsigned long long* o0 = d0 + (i - I0) * Npad;
+    unsigned long long* o1 = d1 + (i - I0) * Npad;
+    unsigned long long* o2 = d2 + (i - I0) * Npad;
+    for (int64_t b0 = s0;; b0 += kDBatch) {
+        const int nb = (int)min<int64_t>(kDBatch, s1 - b0);
+        const bool first = b0 == s0;
+        for (int q = tid; q < nb; q += 256) cur[q] = 0;
+        for (int64_t ch = 0; ch < nch; ++ch) {
+            const int64_t c0 = ch * kDCols;
+            const int64_t c1 = min<int64_t>(c0 + kDCols, i + 1);
+            for (int e = tid; e < kDCols; e += 256) acc[0][e] = acc[1][e] = acc[2][e] = 0ull;
+            __syncthreads();
+            for (int q = tid; q < nb; q += 256) {
+                const int64_t j = sidx[b0 + q];
+                const unsigned long long v0 = v3[3 * j], v1 = v3[3 * j + 1], v2 = v3[3 * j + 2];
+                if ((v0 | v1 | v2) == 0ull) continue;
+                const int64_t l0 = off[j], l1 = off[j + 1];
+                int64_t p = l0 + cur[q];
+                for (; p < l1; ++p) {
+                    const int64_t k = idx[p];
+                    if (k >= c1) break;
+                    const int e = (int)(k - c0);
+                    atomicAdd(&acc[0][e], v0);
+                    atomicAdd(&acc[1][e], v1);
+                    atomicAdd(&acc[2][e], v2);
+                }
+                cur[q] = (int32_t)(p - l0);
             }
+            __syncthreads();
+            for (int e = tid; e < (int)(c1 - c0); e += 256) {
+                if (first) {
+                    o0[c0 + e] = acc[0][e];
+                    o1[c0 + e] = acc[1][e];
+                    o2[c0 + e] = acc[2][e];
+                } else {
+                    o0[c0 + e] += acc[0][e];
+                    o1[c0 + e] += acc[1][e];
+                    o2[c0 + e] += acc[2][e];
+                }
+            }
+            __syncthreads();
         }
+        if (b0 + kDBatch >= s1) break;
     }
 }
 
@@ -364,9 +408,12 @@ struct CombineArgs {
     int64_t I0, P, N, Npad;
     int sb;               // U = z 2^sb
     const int32_t* t1;
-    const long long* c;
-    const unsigned long long* dhi;   // D limbs (may be null: no missing genotypes)
-    const unsigned long long* dlo;
+    unsigned long long t1c_lo;       // coefficient of T1: 2^2sb - 2^sb W
+    long long t1c_hi;
+    const i128* c;                   // sum_l 256^l E_l E_l^T (null: no missing genotypes)
+    const unsigned long long* d0;    // D^v limbs (null: no missing genotypes)
+    const unsigned long long* d1;
+    const unsigned long long* d2;
     const long long* R;
     const i128* Q;
     unsigned long long S_lo;
@@ -386,10 +433,12 @@ __global__ void k_grm_combine(CombineArgs a) {
         const int64_t i = a.I0 + li;
         if (i >= a.N || k > i) continue;
         const i128 one = 1;
-        i128 I = (i128)a.t1[e] * (one << (2 * a.sb)) - ((i128)a.R[i] + (i128)a.R[k]) * (one << a.sb) + S - a.Q[i] -
-                 a.Q[k];
-        if (a.c) I += (i128)a.c[e] * (one << a.sb);
-        if (a.dhi) I += (i128)a.dhi[e] * (one << kDLimbBits) + (i128)a.dlo[e];
+        const i128 t1c = ((i128)a.t1c_hi << 64) + (i128)a.t1c_lo;
+        i128 I = (i128)a.t1[e] * t1c - ((i128)a.R[i] + (i128)a.R[k]) * (one << a.sb) + S - a.Q[i] - a.Q[k];
+        if (a.c) I += a.c[e] * (one << a.sb);
+        if (a.d0)
+            I += (i128)(long long)a.d2[e] * (one << (2 * kDLimbBits)) + (i128)a.d1[e] * (one << kDLimbBits) +
+                 (i128)a.d0[e];
         if (a.tri) a.tri[i * (i + 1) / 2 + k] = i128_div_f32_rn(I, a.D53, a.se, 2 * a.sb);
         if (a.raw) {
             const double g = ldexp(i128_to_f64_rn(I), -2 * a.sb);
@@ -901,6 +950,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
         long long amax = 0;
         for (long long a : prm.A) amax = std::max(amax, a);
         int nC = 0;
+        int64_t mmax = 0;   // most missing genotypes of one sample
         if (prm.any_missing) {
             // the fewest digits whose top digit stays in [-128, 127] for every A_j
             for (nC = 1; nC < 16; ++nC) {
@@ -920,28 +970,52 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             }
             ensure_missing_lists(b);
             if (b->M > ((int64_t)1 << 21)) throw InvalidArgument("GRM: more than 2^21 SNPs with missing genotypes");
-            // C_ik in int64: 2 max A_j times the most missing genotypes of one sample
             std::vector<int64_t> soff((size_t)N + 1);
             CS_HIP(hipMemcpy(soff.data(), b->miss_smp_off.p, soff.size() * 8, hipMemcpyDeviceToHost));
-            int64_t mmax = 0;
             for (int64_t i = 0; i < N; ++i) mmax = std::max(mmax, soff[(size_t)i + 1] - soff[(size_t)i]);
-            if ((long double)amax * 4.0L * (long double)mmax >= std::ldexp(1.0L, 62))
-                throw InvalidArgument("GRM: too many missing genotypes in one sample for the exact sums");
         }
         const bool useD = prm.any_missing;
         // digit arrays in the operand order [kb][t][g][16]: element k = 256 kb + 64 g + 16 t + 4 q + s
-        // at byte 4 s + q.
+        // at byte 4 s + q; per SNP the pair value v_j = A_j^2 - 2^SB w_j (w_j = sum_l 256^l d_lj^2)
+        // as three 42-bit limbs, the top one signed
         std::vector<uint8_t> dig((size_t)nC * Mpad, 0);
+        std::vector<unsigned long long> v3(useD ? (size_t)Mpad * 3 : 0, 0ull);
+        const i128 lmask = (((i128)1) << kDLimbBits) - 1;
         for (int64_t j = 0; j < Mpad; ++j) {
             if (!prm.A[(size_t)j]) continue;
             int8_t dd[16];
             const int64_t kb = j >> 8, rr = j & 255, g = rr >> 6, t = (rr >> 4) & 3, q = (rr >> 2) & 3, s = rr & 3;
             const int64_t off = (((kb * 4 + t) * 4 + g) << 4) + 4 * s + q;
             digits128((i128)prm.A[(size_t)j], nC, dd);
+            i128 w = 0;
+            for (int l = nC - 1; l >= 0; --l) w = w * 256 + (i128)((int)dd[l] * (int)dd[l]);
             for (int l = 0; l < nC; ++l) dig[(size_t)l * Mpad + off] = (uint8_t)dd[l];
+            if (useD) {
+                const i128 v = (i128)prm.A[(size_t)j] * (i128)prm.A[(size_t)j] - (w << prm.sb);
+                v3[(size_t)j * 3] = (unsigned long long)(v & lmask);
+                v3[(size_t)j * 3 + 1] = (unsigned long long)((v >> kDLimbBits) & lmask);
+                v3[(size_t)j * 3 + 2] = (unsigned long long)(long long)(v >> (2 * kDLimbBits));
+            }
         }
-        DevBuf ddig(std::max<size_t>(1, dig.size()));
+        DevBuf ddig(std::max<size_t>(1, dig.size())), dv3(std::max<size_t>(8, v3.size() * 8));
         if (!dig.empty()) CS_HIP(hipMemcpyAsync(ddig.p, dig.data(), dig.size(), hipMemcpyHostToDevice, st));
+        if (!v3.empty()) CS_HIP(hipMemcpyAsync(dv3.p, v3.data(), v3.size() * 8, hipMemcpyHostToDevice, st));
+        // coefficient of T1: 2^2SB - 2^SB W, W = sum_{l < nC} 256^l
+        i128 Wsum = 0;
+        for (int l = nC - 1; l >= 0; --l) Wsum = Wsum * 256 + 1;
+        const i128 t1c = (((i128)1) << (2 * prm.sb)) - (Wsum << prm.sb);
+        // E-pass SNP ranges: |E_l E_l^T| <= 4 L + 256 (m_i + m_k) + 16384 min(m_i, m_k) over a range
+        // of L SNPs in which samples i and k miss m_i and m_k genotypes, so ranges of L SNPs with
+        // 4 L + 16896 min(L, mmax) < 2^31 keep every int32 accumulator exact
+        int64_t chunk_kb = prm.kb_hi - prm.kb_lo;
+        if (useD) {
+            while (chunk_kb > 1 && 4.0 * 256.0 * (double)chunk_kb +
+                                       16896.0 * (double)std::min<int64_t>(256 * chunk_kb, mmax) >= 2147483647.0)
+                chunk_kb = (chunk_kb + 1) / 2;
+            if (4.0 * 256.0 + 16896.0 * 256.0 >= 2147483647.0) throw RuntimeError("GRM: E-pass bound");
+        }
+        if (const char* env = std::getenv("CSAMD_GRM_EPASS_KB")) chunk_kb = std::min<int64_t>(chunk_kb, std::atoll(env));
+        chunk_kb = std::max<int64_t>(1, chunk_kb);
         // S2pq as D * 2^(se-53)
         int se = 0;
         const double fm = std::frexp(prm.sum_2pq > 0 ? prm.sum_2pq : 1.0, &se);
@@ -963,13 +1037,13 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
         // (CSAMD_GRM_PANEL_BYTES overrides the bound)
         size_t panel_bytes = 8ull << 30;
         if (const char* env = std::getenv("CSAMD_GRM_PANEL_BYTES")) panel_bytes = std::strtoull(env, nullptr, 10);
-        const size_t per_row = (size_t)Npad * (4 + (nC ? 8 : 0) + (useD ? 16 : 0));
+        const size_t per_row = (size_t)Npad * (4 + (nC ? 16 : 0) + (useD ? 24 : 0));
         int64_t P = std::max<int64_t>(128, (int64_t)(panel_bytes / per_row) / 128 * 128);
         P = std::min<int64_t>(P, round_up(Npad, 128));
-        DevBuf t1((size_t)(P * Npad) * 4), cacc(nC ? (size_t)(P * Npad) * 8 : 1),
-            dacc(useD ? (size_t)(P * Npad) * 16 : 1);
+        DevBuf t1((size_t)(P * Npad) * 4), cacc(nC ? (size_t)(P * Npad) * 16 : 1),
+            dacc(useD ? (size_t)(P * Npad) * 24 : 1);
         float ms_pass = 0.f, ms_comb = 0.f;
-        int passes = 0, launches = 0;   // products (for the MFMA op count) and kernel launches
+        int launches = 0;
         double mfma_ops = 0;
         for (int64_t I0 = 0; I0 < Npad; I0 += P) {
             const int64_t rows = std::min<int64_t>(P, Npad - I0);
@@ -979,20 +1053,19 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             DevBuf dblocks(blocks.size() * sizeof(int2));
             CS_HIP(hipMemcpyAsync(dblocks.p, blocks.data(), blocks.size() * sizeof(int2),
                                   hipMemcpyHostToDevice, st));
-            if (nC) CS_HIP(hipMemsetAsync(cacc.p, 0, (size_t)(rows * Npad) * 8, st));
-            unsigned long long* dhi = dacc.as<unsigned long long>();
-            unsigned long long* dlo = dhi + rows * Npad;
-            if (useD) {
-                // D from the missing lists on st2 (latency-bound atomics), concurrent with the
-                // MFMA passes on st; the combine waits for both
-                CS_HIP(hipMemsetAsync(dacc.p, 0, (size_t)(rows * Npad) * 16, st));
+            if (nC) CS_HIP(hipMemsetAsync(cacc.p, 0, (size_t)(rows * Npad) * 16, st));
+            unsigned long long* dp0 = dacc.as<unsigned long long>();
+            unsigned long long* dp1 = dp0 + rows * Npad;
+            unsigned long long* dp2 = dp1 + rows * Npad;
+            if (useD && I0 < N) {
+                // D^v from the missing lists on st2, concurrent with the MFMA passes on st; every
+                // entry (i, k <= i) of the panel's real rows is written; the combine waits for both
                 CS_HIP(hipEventRecord(ed0, st));
                 CS_HIP(hipStreamWaitEvent(st2, ed0, 0));
-                const int64_t j0 = prm.kb_lo * 256, j1 = std::min<int64_t>(b->M, prm.kb_hi * 256);
-                const unsigned gd = (unsigned)std::min<int64_t>(65536, (j1 - j0 + 3) / 4);
-                k_grm_dpairs<<<gd, 256, 0, st2>>>(b->miss_snp_off.as<int64_t>(), b->miss_snp_idx.as<int32_t>(),
-                                                  dA.as<long long>(), j0, j1, I0, std::min<int64_t>(N, I0 + rows),
-                                                  Npad, dhi, dlo);
+                const int64_t r1 = std::min<int64_t>(N, I0 + rows);
+                k_grm_dlds<<<(unsigned)(r1 - I0), 256, 0, st2>>>(
+                    b->miss_smp_off.as<int64_t>(), b->miss_smp_idx.as<int32_t>(), b->miss_snp_off.as<int64_t>(),
+                    b->miss_snp_idx.as<int32_t>(), dv3.as<unsigned long long>(), I0, r1, I0, Npad, dp0, dp1, dp2);
                 CS_HIP(hipGetLastError());
                 CS_HIP(hipEventRecord(ed1, st2));
             }
@@ -1005,37 +1078,35 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             ga.N = N;
             ga.Npad = Npad;
             ga.t1 = t1.as<int32_t>();
-            ga.c = cacc.as<long long>();
+            ga.c = cacc.as<i128>();
+            ga.dstride = Mpad;
             const unsigned nb = (unsigned)blocks.size();
             int64_t quadrants = 0;   // 64 x 64 wave tiles on or below the diagonal
             for (const int2& bl : blocks) quadrants += bl.x == bl.y ? 3 : 4;
-            const int passes0 = passes;
             CS_HIP(hipEventRecord(e0, st));
             ga.kb0 = prm.kb_lo;
             ga.kb1 = prm.kb_hi;
             k_grm_pass<<<nb, 256, 0, st>>>(ga);
-            ++passes;
             ++launches;
-            // digit products, both orders in one int32 accumulator: |x~ d| <= 256 per SNP and
-            // order, so chunks of <= 2^21 SNPs
-            for (int64_t c0 = prm.kb_lo; c0 < prm.kb_hi; c0 += kDigitChunkKB) {
+            // E passes, two digits per launch, over SNP ranges that keep int32 exact
+            for (int64_t c0 = prm.kb_lo; c0 < prm.kb_hi; c0 += chunk_kb) {
                 GrmArgs gc = ga;
                 gc.kb0 = c0;
-                gc.kb1 = std::min<int64_t>(prm.kb_hi, c0 + kDigitChunkKB);
-                for (int l = 0; l < nC; ++l) {
+                gc.kb1 = std::min<int64_t>(prm.kb_hi, c0 + chunk_kb);
+                for (int l = 0; l < nC; l += 2) {
                     GrmArgs g2 = gc;
                     g2.shift = 8 * l;
                     g2.digA = ddig.as<uint8_t>() + (size_t)l * Mpad;
-                    k_grm_cpass<<<nb, 256, 0, st>>>(g2);
-                    passes += 2;   // two products' MFMA work
+                    if (l + 1 < nC) k_grm_epass<2><<<nb, 256, 0, st>>>(g2);
+                    else k_grm_epass<1><<<nb, 256, 0, st>>>(g2);
                     ++launches;
                 }
             }
             CS_HIP(hipGetLastError());
             CS_HIP(hipEventRecord(e1, st));
-            if (useD) CS_HIP(hipStreamWaitEvent(st, ed1, 0));
-            // int8 MFMA ops issued: T1 over all SNP blocks plus each digit product over all blocks
-            mfma_ops += (double)(passes - passes0) * (double)quadrants * 64.0 * 64.0 *
+            if (useD && I0 < N) CS_HIP(hipStreamWaitEvent(st, ed1, 0));
+            // int8 MFMA ops issued: T1 and each digit's E product over all SNP blocks
+            mfma_ops += (double)(1 + nC) * (double)quadrants * 64.0 * 64.0 *
                         (double)((prm.kb_hi - prm.kb_lo) * 256) * 2.0;
             CombineArgs ca{};
             ca.I0 = I0;
@@ -1043,10 +1114,13 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             ca.N = N;
             ca.Npad = Npad;
             ca.t1 = t1.as<int32_t>();
+            ca.t1c_lo = (unsigned long long)t1c;
+            ca.t1c_hi = (long long)(t1c >> 64);
             ca.sb = prm.sb;
-            ca.c = nC ? cacc.as<long long>() : nullptr;
-            ca.dhi = useD ? dhi : nullptr;
-            ca.dlo = useD ? dlo : nullptr;
+            ca.c = nC ? cacc.as<i128>() : nullptr;
+            ca.d0 = useD ? dp0 : nullptr;
+            ca.d1 = useD ? dp1 : nullptr;
+            ca.d2 = useD ? dp2 : nullptr;
             ca.R = dR.as<long long>();
             ca.Q = dQ.as<i128>();
             ca.S_lo = (unsigned long long)S;

@@ -111,6 +111,15 @@ def test_grm_many_missing_pairs(csa, monkeypatch):
     check_against_oracle(csa, r, r.download(), N, M)
 
 
+def test_grm_long_missing_lists_and_snp_ranges(csa, monkeypatch):
+    # 50 % missing over 9000 SNPs: every sample misses more SNPs than one cursor batch of the pair
+    # sums holds (4096), and E passes forced onto SNP ranges of 8 blocks (2048 SNPs)
+    N, M = 190, 9000
+    monkeypatch.setenv("CSAMD_GRM_EPASS_KB", "8")
+    r = csa.PlinkReader.synthetic(5150, N, M, 0.01, 0.5, 0.5)
+    check_against_oracle(csa, r, r.download(), N, M)
+
+
 def test_grm_hwe_filter(csa):
     # GrmBuilder's optional exact-HWE SNP filter (grm_builder.cpp:220-223): SNPs with an excess of
     # heterozygotes or homozygotes are dropped from both GRMs at min_hwe = 1e-4
