@@ -20,6 +20,7 @@ Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 from __future__ import annotations
 
 import argparse
+from concurrent.futures import ThreadPoolExecutor
 import json
 import os
 import sys
@@ -551,34 +552,48 @@ def serving_leg(csa, n=100_000, m=1_000_000, n_requests=200, rate=5.0, seed=2026
         h2 = float(rng.uniform(0.1, 0.8))
         cal = np.sort(rng.choice(m, 16, replace=False)).astype(np.int32)
         reqs.append(dict(Y=Y, X=X, sigma2_g=h2, sigma2_e=1.0 - h2, cal_snps=cal, keep=keep))
-    ids, submitted, done, done_dev, n_exact = {}, 0, {}, {}, 0
-    genos = 0.0
+    ids, submitted, done, done_dev, fetch = {}, 0, {}, {}, {}
     start = time.perf_counter()
-    while len(done) < n_requests:
-        now = time.perf_counter() - start
-        while submitted < n_requests and arrivals[submitted] <= now:
-            ids[srv.submit(**reqs[submitted])] = submitted
-            submitted += 1
-        if srv.step() == (0, 0, 0) and submitted < n_requests:
-            time.sleep(max(0.0, min(0.01, arrivals[submitted] - (time.perf_counter() - start))))
-        for rid, i in list(ids.items()):
-            if rid in done:
-                continue
-            st, info = srv.status(rid)
-            if st == 2:
-                done_dev[rid] = time.perf_counter() - start - arrivals[i]
-                res = srv.result(rid)
-                done[rid] = time.perf_counter() - start - arrivals[i]
-                assert res["log10p"].shape == (m, reqs[i]["Y"].shape[1])
-                n_exact += int(res["stats"]["n_exact"])
-                del res
-                genos += reqs[i]["keep"].size * m * reqs[i]["Y"].shape[1]
-                srv.release(rid)
-            elif st == -1:
-                raise RuntimeError(f"request {i} failed")
+
+    def fetch_one(rid, i):
+        # the client thread holding the results: the copy to host runs beside the server's steps
+        res = srv.result(rid)
+        t = time.perf_counter() - start - arrivals[i]
+        assert res["log10p"].shape == (m, reqs[i]["Y"].shape[1])
+        ne = int(res["stats"]["n_exact"])
+        del res
+        srv.release(rid)
+        return t, ne
+
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        while len(done) < n_requests:
+            now = time.perf_counter() - start
+            while submitted < n_requests and arrivals[submitted] <= now:
+                ids[srv.submit(**reqs[submitted])] = submitted
+                submitted += 1
+            if srv.step() == (0, 0, 0):
+                wait = arrivals[submitted] - (time.perf_counter() - start) if submitted < n_requests else 0.01
+                time.sleep(max(0.0, min(0.01, wait)))
+            for rid, i in list(ids.items()):
+                if rid in fetch:
+                    if rid not in done and fetch[rid].done():
+                        done[rid] = fetch[rid].result()
+                    continue
+                st, info = srv.status(rid)
+                if st == 2:
+                    done_dev[rid] = time.perf_counter() - start - arrivals[i]
+                    fetch[rid] = pool.submit(fetch_one, rid, i)
+                elif st == -1:
+                    raise RuntimeError(f"request {i} failed")
+            if len(fetch) == n_requests and submitted == n_requests:
+                for rid, f in fetch.items():
+                    if rid not in done:
+                        done[rid] = f.result()
+    n_exact = sum(ne for _, ne in done.values())
+    genos = float(sum(reqs[i]["keep"].size * m * reqs[i]["Y"].shape[1] for i in ids.values()))
     wall = time.perf_counter() - start
     order = sorted(ids.items(), key=lambda kv: kv[1])
-    lat = np.array([done[rid] for rid, _ in order])
+    lat = np.array([done[rid][0] for rid, _ in order])
     lat_dev = np.array([done_dev[rid] for rid, _ in order])
     half = lat[n_requests // 2:]
     srv.close()

@@ -31,6 +31,7 @@
 #include <map>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -185,6 +186,7 @@ struct Request {
     int64_t ns = 0;
     DevBuf freq, n_obs, pass, beta, se_out, p, log10p;
     std::string error;
+    int fetching = 0;   // cs_server_result calls copying this request's arrays (guarded by mu)
 
     ~Request() {
         if (K) (void)cs_kinship_destroy(K);
@@ -207,8 +209,14 @@ struct cs_server {
     std::chrono::steady_clock::time_point t0;
     void* stage[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    // mu guards the scheduler state (requests, budget, columns); cs_server_result holds it only to
+    // read a request and takes dmu for the copy itself, on its own stream, so one client thread
+    // can fetch finished results while another runs the steps.
+    std::mutex mu, dmu;
+    hipStream_t dst = nullptr;   // result delivery
     ~cs_server() {
         if (st) (void)hipStreamSynchronize(st);
+        if (dst) (void)hipStreamSynchronize(dst);
         reqs.clear();
         for (int b = 0; b < 2; ++b) {
             if (stage[b]) (void)hipHostFree(stage[b]);
@@ -216,6 +224,7 @@ struct cs_server {
         }
         if (blas) (void)rocblas_destroy_handle(blas);
         if (st) (void)hipStreamDestroy(st);
+        if (dst) (void)hipStreamDestroy(dst);
     }
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
@@ -260,8 +269,8 @@ void deliver(cs_server* s, const std::vector<Download>& list) {
     for (size_t i = 0; i <= ch.size(); ++i) {
         if (i < ch.size()) {
             const int b = (int)(i & 1);
-            CS_HIP(hipMemcpyAsync(s->stage[b], ch[i].src, ch[i].len, hipMemcpyDeviceToHost, s->st));
-            CS_HIP(hipEventRecord(s->stage_ev[b], s->st));
+            CS_HIP(hipMemcpyAsync(s->stage[b], ch[i].src, ch[i].len, hipMemcpyDeviceToHost, s->dst));
+            CS_HIP(hipEventRecord(s->stage_ev[b], s->dst));
         }
         if (i > 0) {
             const int b = (int)((i - 1) & 1);
@@ -545,6 +554,7 @@ int cs_server_create(cs_bed* bed, const int32_t* variants, int64_t n_variants, i
         s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)1 << 40;
         if (variants && n_variants > 0) s->variants.assign(variants, variants + n_variants);
         CS_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        CS_HIP(hipStreamCreateWithFlags(&s->dst, hipStreamNonBlocking));
         for (int b = 0; b < 2; ++b) {
             CS_HIP(hipHostMalloc(&s->stage[b], kStageBytes, hipHostMallocDefault));
             CS_HIP(hipEventCreateWithFlags(&s->stage_ev[b], hipEventDisableTiming));
@@ -564,12 +574,20 @@ int cs_server_create(cs_bed* bed, const int32_t* variants, int64_t n_variants, i
 }
 
 int cs_server_destroy(cs_server* s) {
-    return guarded([&] { delete s; });
+    return guarded([&] {
+        if (s) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (auto& kv : s->reqs)
+                if (kv.second->fetching) throw InvalidArgument("cs_server_destroy: results are being fetched");
+        }
+        delete s;
+    });
 }
 
 int cs_server_submit(cs_server* s, const cs_request* r, int64_t* id) {
     return guarded([&] {
         if (!s || !r || !r->Y || (r->n_cov > 0 && !r->X)) throw InvalidArgument("cs_server_submit: null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
         if (r->n_traits < 1 || r->n_cov < 1 || r->n_cov > 32)
             throw InvalidArgument("cs_server_submit: need n_traits >= 1 and 1 <= n_cov <= 32");
         if (!(r->sigma2_e > 0.0) || !(r->sigma2_g >= 0.0))
@@ -621,6 +639,7 @@ int cs_server_submit(cs_server* s, const cs_request* r, int64_t* id) {
 int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t* n_completed) {
     return guarded([&] {
         if (!s) throw InvalidArgument("cs_server_step: null handle");
+        std::lock_guard<std::mutex> lk(s->mu);
         int32_t completed = 0;
         // exact-stage chunks of active requests first (they hold results already), in id order
         for (auto& kv : s->reqs) {
@@ -678,46 +697,63 @@ int cs_server_result(cs_server* s, int64_t id, int32_t* status, double* freq, in
                      double* beta, double* se, double* p, double* log10p, cs_request_stats* stats) {
     return guarded([&] {
         if (!s) throw InvalidArgument("cs_server_result: null handle");
-        auto it = s->reqs.find(id);
-        if (it == s->reqs.end()) throw InvalidArgument("cs_server_result: unknown request");
-        const Request& q = *it->second;
-        if (status) *status = q.status;
-        if (stats) {
-            stats->t_submit = q.t_submit;
-            stats->t_admit = q.t_admit;
-            stats->t_done = q.t_done;
-            stats->iterations = q.iterations;
-            stats->gamma = q.pj.gamma;
-            stats->n_calibration = q.pj.ncal;
-            stats->n_columns = q.k;
-            stats->n_exact = (int64_t)q.rows.size();
-            stats->exact_iterations = q.exact_iterations;
-            stats->calibration_rel_sd = q.pj.rel_sd;
-            stats->dlog10p_bound_below = pscan_bound_below(q.thr, q.pj.rel_sd);
-            stats->admit_bytes = q.admit_bytes;
-            stats->compact_rows = q.compact ? 1 : 0;
+        std::vector<Download> list;
+        Request* qp = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            auto it = s->reqs.find(id);
+            if (it == s->reqs.end()) throw InvalidArgument("cs_server_result: unknown request");
+            Request& q = *it->second;
+            if (status) *status = q.status;
+            if (stats) {
+                stats->t_submit = q.t_submit;
+                stats->t_admit = q.t_admit;
+                stats->t_done = q.t_done;
+                stats->iterations = q.iterations;
+                stats->gamma = q.pj.gamma;
+                stats->n_calibration = q.pj.ncal;
+                stats->n_columns = q.k;
+                stats->n_exact = (int64_t)q.rows.size();
+                stats->exact_iterations = q.exact_iterations;
+                stats->calibration_rel_sd = q.pj.rel_sd;
+                stats->dlog10p_bound_below = pscan_bound_below(q.thr, q.pj.rel_sd);
+                stats->admit_bytes = q.admit_bytes;
+                stats->compact_rows = q.compact ? 1 : 0;
+            }
+            const bool wants = freq || n_obs || pass || beta || se || p || log10p;
+            if (q.status == REQ_FAILED && wants) throw RuntimeError("request failed: " + q.error);
+            if (q.status != REQ_DONE || !wants) return;
+            const size_t ns = (size_t)q.ns, nt = (size_t)(q.ns * q.T);
+            list = {{freq, q.freq.p, ns * 8},   {n_obs, q.n_obs.p, ns * 8}, {pass, q.pass.p, ns * 4},
+                    {beta, q.beta.p, nt * 8},   {se, q.se_out.p, nt * 8},  {p, q.p.p, nt * 8},
+                    {log10p, q.log10p.p, nt * 8}};
+            ++q.fetching;
+            qp = &q;
         }
-        const bool wants = freq || n_obs || pass || beta || se || p || log10p;
-        if (q.status == REQ_FAILED && wants) throw RuntimeError("request failed: " + q.error);
-        if (q.status != REQ_DONE) return;
-        const size_t ns = (size_t)q.ns, nt = (size_t)(q.ns * q.T);
-        deliver(s, {{freq, q.freq.p, ns * 8},
-                    {n_obs, q.n_obs.p, ns * 8},
-                    {pass, q.pass.p, ns * 4},
-                    {beta, q.beta.p, nt * 8},
-                    {se, q.se_out.p, nt * 8},
-                    {p, q.p.p, nt * 8},
-                    {log10p, q.log10p.p, nt * 8}});
+        // a finished request's arrays are read only here, and it stays in the map while fetching
+        // (cs_server_release refuses it), so the copy runs outside the scheduler lock
+        struct Done {
+            cs_server* s;
+            Request* q;
+            ~Done() {
+                std::lock_guard<std::mutex> lk(s->mu);
+                --q->fetching;
+            }
+        } done{s, qp};
+        std::lock_guard<std::mutex> dl(s->dmu);
+        deliver(s, list);
     });
 }
 
 int cs_server_release(cs_server* s, int64_t id) {
     return guarded([&] {
         if (!s) throw InvalidArgument("cs_server_release: null handle");
+        std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->reqs.find(id);
         if (it == s->reqs.end()) return;
         if (it->second->status == REQ_ACTIVE || it->second->status == REQ_PENDING)
             throw InvalidArgument("cs_server_release: request still running");
+        if (it->second->fetching) throw InvalidArgument("cs_server_release: results are being fetched");
         free_results(s, *it->second);
         s->reqs.erase(it);
     });

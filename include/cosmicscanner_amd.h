@@ -396,7 +396,11 @@ int cs_server_step(cs_server* srv, int32_t* n_active, int32_t* n_pending, int32_
 /* status: 0 pending, 1 active, 2 done, -1 failed.  Asking for outputs of a failed request gives
  * CS_ERR_RUNTIME with its message.  When done, outputs as cs_pcg_projected_scan (any may be
  * NULL); a finished request keeps them in device memory (n_snps x (4 T + 3) x 8 bytes) until
- * cs_server_release, and each call copies the asked-for arrays to the host buffers given. */
+ * cs_server_release, and each call copies the asked-for arrays to the host buffers given.
+ * Threads: the server calls may come from several host threads.  cs_server_result copies a
+ * finished request's arrays on its own stream without holding the scheduler, so a client thread
+ * can fetch results while another thread runs cs_server_step; cs_server_release and
+ * cs_server_destroy refuse (CS_ERR_INVALID_ARGUMENT) while a fetch of the request is running. */
 int cs_server_result(cs_server* srv, int64_t id, int32_t* status, double* freq, int64_t* n_obs,
                      int32_t* pass, double* beta, double* se, double* p, double* log10p,
                      cs_request_stats* stats);

@@ -188,3 +188,27 @@ def test_budget_holds_unreleased_results(csa, world):
     assert srv.status(ib)[0] == 2
     same(got_a, run_alone(csa, reader, qa))
     same(srv.result(ib), run_alone(csa, reader, qb))
+
+
+def test_fetch_on_another_thread_while_stepping(csa, world):
+    # a client thread copies finished results out while the main thread keeps running steps; the
+    # arrays are those of a request run alone, and release / destroy wait for no lock of their own
+    from concurrent.futures import ThreadPoolExecutor
+    reader, reqs = world
+    alone = [run_alone(csa, reader, q) for q in reqs[:4]]
+    srv = csa.ScanServer(reader, max_columns=128)
+    ids = [srv.submit(**q) for q in reqs[:4]]
+    futures = {}
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        for _ in range(100000):
+            a, p, _ = srv.step()
+            for rid in ids:
+                if rid not in futures and srv.status(rid)[0] == 2:
+                    futures[rid] = pool.submit(srv.result, rid)
+            if a == 0 and p == 0 and len(futures) == len(ids):
+                break
+        got = {rid: f.result() for rid, f in futures.items()}
+    for rid, ref in zip(ids, alone):
+        same(got[rid], ref)
+        srv.release(rid)
+    srv.close()
