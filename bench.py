@@ -261,7 +261,7 @@ def grm_leg(csa, reader, N, M):
                 sum_2pq=s2pq, n_valid=nv, tops_full_square=2.0 * N * N * M / dt / 1e12,
                 roofline=dict(bound="mfma", achieved=round(achieved, 1), peak=INT8_PEAK_TOPS,
                               unit="TOP/s (int8)", frac=round(achieved / INT8_PEAK_TOPS, 4),
-                              kernel="k_grm_pass (all launches)"))
+                              kernel="k_grm_xpass (all launches)"))
 
 
 def grm_std_leg(csa, reader, N, M):

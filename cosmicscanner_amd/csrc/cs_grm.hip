@@ -57,89 +57,24 @@ struct GrmArgs {
 };
 
 
-// T1 = X~ X~^T.  One 128 x 128 lower-triangle block per workgroup; wave (wi, wk) owns the 64 x 64 quadrant
-// rows i0 = 128 bi + 64 wi, columns k0 = 128 bk + 64 wk (4 x 4 MFMA tiles per step).  The block
-// needs four 64-row operand tiles (A rows of bi, B rows of bk); wave w decodes tile w once per
-// 64-k step into LDS (double-buffered, one barrier per step) and every wave reads its A and B
-// operands from there, so each tile is loaded and decoded once instead of twice.  Quadrants
-// above the diagonal still decode their tile but skip the MFMAs and the epilogue.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_grm_pass(GrmArgs a) {
-    __shared__ v4i lds[2][4][4][64];   // [buffer][tile][m-subtile][lane], 32 KiB
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int g = lane >> 4, r = lane & 15;
-    const int2 blk = a.blocks[blockIdx.x];
-    const int wi = wave >> 1, wk = wave & 1;
-    const int64_t i0 = (int64_t)blk.x * 128 + 64 * wi;
-    const int64_t k0 = (int64_t)blk.y * 128 + 64 * wk;
-    const bool active = k0 < i0 + 64;   // quadrant on or below the diagonal
-    const bool sideA = wave < 2;
-    const int64_t trow = sideA ? (int64_t)blk.x * 128 + 64 * wave : (int64_t)blk.y * 128 + 64 * (wave - 2);
-    const uint8_t* tbase = a.smp + (((trow >> 6) * a.KB) << 12) + lane * 16;
-    v4i acc[4][4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int n = 0; n < 4; ++n) acc[m][n] = v4i{0, 0, 0, 0};
-    uint4 cv[4], cn[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) cv[m] = *reinterpret_cast<const uint4*>(tbase + (a.kb0 << 12) + m * 1024);
-    int buf = 0;
-    for (int64_t kb = a.kb0; kb < a.kb1; ++kb) {
-        const int64_t kn = kb + 1 < a.kb1 ? kb + 1 : kb;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) cn[m] = *reinterpret_cast<const uint4*>(tbase + (kn << 12) + m * 1024);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const uint32_t w = t == 0 ? cv[m].x : t == 1 ? cv[m].y : t == 2 ? cv[m].z : cv[m].w;
-                const uint32_t d0 = w & 0x03030303u, d1 = (w >> 2) & 0x03030303u;
-                const uint32_t d2 = (w >> 4) & 0x03030303u, d3 = (w >> 6) & 0x03030303u;
-                lds[buf][wave][m][lane] =
-                    v4i{(int)__builtin_amdgcn_perm(0u, kLutX, d0), (int)__builtin_amdgcn_perm(0u, kLutX, d1),
-                        (int)__builtin_amdgcn_perm(0u, kLutX, d2), (int)__builtin_amdgcn_perm(0u, kLutX, d3)};
-            }
-            __syncthreads();
-            if (active) {
-                v4i B[4];
-#pragma unroll
-                for (int n = 0; n < 4; ++n) B[n] = lds[buf][2 + wk][n][lane];
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const v4i A = lds[buf][wi][m][lane];
-#pragma unroll
-                    for (int n = 0; n < 4; ++n)
-                        acc[m][n] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[n], acc[m][n], 0, 0, 0);
-                }
-            }
-            buf ^= 1;
-        }
-#pragma unroll
-        for (int m = 0; m < 4; ++m) cv[m] = cn[m];
-    }
-    if (!active) return;
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int n = 0; n < 4; ++n)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int64_t i = i0 + 16 * m + 4 * g + v;
-                const int64_t k = k0 + 16 * n + r;
-                if (i >= a.N || k >= a.N || k > i || i < a.I0 || i >= a.I0 + a.P) continue;
-                a.t1[(i - a.I0) * a.Npad + k] = acc[m][n][v];
-            }
-}
-
-// E passes: ND digits of c = sum_l 256^l E_l E_l^T per launch.  Each wave decodes its 64-row
-// tile once per 64-k step (the code extraction and the two perms shared by the digits, one
-// and-or per digit) into LDS (double-buffered, ND x 32 KiB) and multiplies the block's E_l tiles,
-// 16 MFMAs per digit and step.  Every term of E_l E_l^T is bounded by 4 (both observed), 256 (one
-// missing) or 16384 (both missing), so a launch covers a SNP range whose bound stays in int32
-// (chosen on the host from the largest per-sample missing count).
-template <int ND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_grm_epass(GrmArgs a) {
-    __shared__ v4i lds[2][ND][4][4][64];   // [buffer][digit][tile][m-subtile][lane]
+// Products of the packed sample-major genotypes with themselves on int8 MFMA, one 128 x 128
+// lower-triangle block per workgroup; wave (wi, wk) owns the 64 x 64 quadrant rows
+// i0 = 128 bi + 64 wi, columns k0 = 128 bk + 64 wk (4 x 4 MFMA tiles per step).  The block needs
+// four 64-row operand tiles (A rows of bi, B rows of bk); wave w decodes tile w once per 64-k step
+// into LDS and every wave reads its A and B operands from there.
+//   DIG = false: T1 = X~ X~^T (one form, x~), stored as int32.
+//   DIG = true: ND digits of c = sum_l 256^l E_l E_l^T, E_l = x~ where observed and the digit d_lj
+//   where missing (the code extraction and the two perms shared by the digits, one and-or per
+//   digit), added to the int128 panel.  Every term of E_l E_l^T is bounded by 4 (both observed),
+//   256 (one missing) or 16384 (both missing), so a launch covers a SNP range whose bound stays in
+//   int32 (chosen on the host from the largest per-sample missing count).
+// LDS is double-buffered (one barrier per step).  Quadrants above the diagonal decode their tile
+// but skip the MFMAs and the epilogue.  (Decoding step s + 1 beside the MFMAs of step s, in one
+// basic block, was slower: 3.45 against 3.20 s at 50k x 500k.)
+template <int ND, bool DIG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(DIG ? 2 : 3, DIG ? 2 : 3))) void
+k_grm_xpass(GrmArgs a) {
+    __shared__ v4i lds[2][ND][4][4][64];   // [buffer][form][tile][m-subtile][lane]
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r = lane & 15;
     const int2 blk = a.blocks[blockIdx.x];
@@ -157,6 +92,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int m = 0; m < 4; ++m)
 #pragma unroll
             for (int n = 0; n < 4; ++n) acc[l][m][n] = v4i{0, 0, 0, 0};
+    // decode step (kb, t) of the packed words cw into LDS buffer bw
+    auto decode = [&](const uint4 (&cw)[4], int64_t kb, int t, int bw) {
+        uint4 d[ND];
+        if (DIG) {
+#pragma unroll
+            for (int l = 0; l < ND; ++l)
+                d[l] = *reinterpret_cast<const uint4*>(a.digA + l * a.dstride + (((kb * 4 + t) * 4 + g) << 4));
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const uint32_t w = t == 0 ? cw[m].x : t == 1 ? cw[m].y : t == 2 ? cw[m].z : cw[m].w;
+            const uint32_t s0 = w & 0x03030303u, s1 = (w >> 2) & 0x03030303u;
+            const uint32_t s2 = (w >> 4) & 0x03030303u, s3 = (w >> 6) & 0x03030303u;
+            const uint32_t x0 = __builtin_amdgcn_perm(0u, kLutX, s0), x1 = __builtin_amdgcn_perm(0u, kLutX, s1);
+            const uint32_t x2 = __builtin_amdgcn_perm(0u, kLutX, s2), x3 = __builtin_amdgcn_perm(0u, kLutX, s3);
+            if (!DIG) {
+                lds[bw][0][wave][m][lane] = v4i{(int)x0, (int)x1, (int)x2, (int)x3};
+            } else {
+                const uint32_t f0 = __builtin_amdgcn_perm(0u, kLutMF, s0), f1 = __builtin_amdgcn_perm(0u, kLutMF, s1);
+                const uint32_t f2 = __builtin_amdgcn_perm(0u, kLutMF, s2), f3 = __builtin_amdgcn_perm(0u, kLutMF, s3);
+#pragma unroll
+                for (int l = 0; l < ND; ++l)
+                    lds[bw][l][wave][m][lane] = v4i{(int)((f0 & d[l].x) | x0), (int)((f1 & d[l].y) | x1),
+                                                    (int)((f2 & d[l].z) | x2), (int)((f3 & d[l].w) | x3)};
+            }
+        }
+    };
     uint4 cv[4], cn[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) cv[m] = *reinterpret_cast<const uint4*>(tbase + (a.kb0 << 12) + m * 1024);
@@ -167,24 +129,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         for (int m = 0; m < 4; ++m) cn[m] = *reinterpret_cast<const uint4*>(tbase + (kn << 12) + m * 1024);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            uint4 d[ND];
-#pragma unroll
-            for (int l = 0; l < ND; ++l)
-                d[l] = *reinterpret_cast<const uint4*>(a.digA + l * a.dstride + (((kb * 4 + t) * 4 + g) << 4));
-#pragma unroll
-            for (int m = 0; m < 4; ++m) {
-                const uint32_t w = t == 0 ? cv[m].x : t == 1 ? cv[m].y : t == 2 ? cv[m].z : cv[m].w;
-                const uint32_t s0 = w & 0x03030303u, s1 = (w >> 2) & 0x03030303u;
-                const uint32_t s2 = (w >> 4) & 0x03030303u, s3 = (w >> 6) & 0x03030303u;
-                const uint32_t x0 = __builtin_amdgcn_perm(0u, kLutX, s0), x1 = __builtin_amdgcn_perm(0u, kLutX, s1);
-                const uint32_t x2 = __builtin_amdgcn_perm(0u, kLutX, s2), x3 = __builtin_amdgcn_perm(0u, kLutX, s3);
-                const uint32_t f0 = __builtin_amdgcn_perm(0u, kLutMF, s0), f1 = __builtin_amdgcn_perm(0u, kLutMF, s1);
-                const uint32_t f2 = __builtin_amdgcn_perm(0u, kLutMF, s2), f3 = __builtin_amdgcn_perm(0u, kLutMF, s3);
-#pragma unroll
-                for (int l = 0; l < ND; ++l)
-                    lds[buf][l][wave][m][lane] = v4i{(int)((f0 & d[l].x) | x0), (int)((f1 & d[l].y) | x1),
-                                                     (int)((f2 & d[l].z) | x2), (int)((f3 & d[l].w) | x3)};
-            }
+            decode(cv, kb, t, buf);
             __syncthreads();
             if (active) {
 #pragma unroll
@@ -216,10 +161,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const int64_t i = i0 + 16 * m + 4 * g + v;
                 const int64_t k = k0 + 16 * n + r;
                 if (i >= a.N || k >= a.N || k > i || i < a.I0 || i >= a.I0 + a.P) continue;
-                i128 add = 0;
+                if (!DIG) {
+                    a.t1[(i - a.I0) * a.Npad + k] = acc[0][m][n][v];
+                } else {
+                    i128 add = 0;
 #pragma unroll
-                for (int l = 0; l < ND; ++l) add += (i128)acc[l][m][n][v] << (a.shift + 8 * l);
-                a.c[(i - a.I0) * a.Npad + k] += add;
+                    for (int l = 0; l < ND; ++l) add += (i128)acc[l][m][n][v] << (a.shift + 8 * l);
+                    a.c[(i - a.I0) * a.Npad + k] += add;
+                }
             }
 }
 
@@ -291,13 +240,15 @@ __global__ __launch_bounds__(256) void k_grm_rowsums(const uint8_t* __restrict__
 // D^v_ik = sum_j v_j m_ij m_kj for the panel rows i in [row0, row1) and k <= i, from the missing
 // lists: one workgroup per row i walks the row's missing SNPs j and, for each, the samples k
 // missing at j, adding v_j (split into three 42-bit limbs, the top one signed) to an LDS
-// accumulator of kDCols columns; chunks of columns are taken in order, each SNP keeping its list
-// position (cursor) in LDS from one chunk to the next, so every pair is visited once.  A chunk is
-// stored to the three output planes [row][Npad] (added to them for rows whose list is longer
-// than one cursor batch).  Each limb sum stays below 2^63 for up to 2^21 SNPs.
+// accumulator of kDCols columns.  Chunks of columns are taken in order; each thread holds kDPer of
+// the row's SNPs (value limbs and list position) in registers from one chunk to the next, so
+// every pair is visited once, and reads a SNP's list four entries at a time.  A chunk is stored to
+// the three output planes [row][Npad] (added to them for rows whose list is longer than one batch
+// of 256 kDPer SNPs).  Each limb sum stays below 2^63 for up to 2^21 SNPs.
 constexpr int kDLimbBits = 42;
-constexpr int kDCols = 1024;
-constexpr int kDBatch = 4096;
+constexpr int kDCols = 2048;
+constexpr int kDPer = 8;
+constexpr int kDBatch = 256 * kDPer;
 
 __global__ __launch_bounds__(256) void k_grm_dlds(const int64_t* __restrict__ soff, const int32_t* __restrict__ sidx,
                                                   const int64_t* __restrict__ off, const int32_t* __restrict__ idx,
@@ -307,7 +258,6 @@ __global__ __launch_bounds__(256) void k_grm_dlds(const int64_t* __restrict__ so
                                                   unsigned long long* __restrict__ d1,
                                                   unsigned long long* __restrict__ d2) {
     __shared__ unsigned long long acc[3][kDCols];
-    __shared__ int32_t cur[kDBatch];
     const int64_t i = row0 + blockIdx.x;
     if (i >= row1) return;   // whole workgroup, before any barrier
     const int tid = threadIdx.x;
@@ -317,29 +267,50 @@ __global__ __launch_bounds__(256) void k_grm_dlds(const int64_t* __restrict__ so
     unsigned long long* o1 = d1 + (i - I0) * Npad;
     unsigned long long* o2 = d2 + (i - I0) * Npad;
     for (int64_t b0 = s0;; b0 += kDBatch) {
-        const int nb = (int)min<int64_t>(kDBatch, s1 - b0);
         const bool first = b0 == s0;
-        for (int q = tid; q < nb; q += 256) cur[q] = 0;
+        int64_t p[kDPer], pe[kDPer];
+        unsigned long long v0[kDPer], v1[kDPer], v2[kDPer];
+#pragma unroll
+        for (int u = 0; u < kDPer; ++u) {
+            const int64_t q = b0 + tid + 256 * u;
+            p[u] = pe[u] = 0;
+            v0[u] = v1[u] = v2[u] = 0ull;
+            if (q < s1) {
+                const int64_t j = sidx[q];
+                v0[u] = v3[3 * j];
+                v1[u] = v3[3 * j + 1];
+                v2[u] = v3[3 * j + 2];
+                if ((v0[u] | v1[u] | v2[u]) != 0ull) {
+                    p[u] = off[j];
+                    pe[u] = off[j + 1];
+                }
+            }
+        }
         for (int64_t ch = 0; ch < nch; ++ch) {
             const int64_t c0 = ch * kDCols;
             const int64_t c1 = min<int64_t>(c0 + kDCols, i + 1);
             for (int e = tid; e < kDCols; e += 256) acc[0][e] = acc[1][e] = acc[2][e] = 0ull;
             __syncthreads();
-            for (int q = tid; q < nb; q += 256) {
-                const int64_t j = sidx[b0 + q];
-                const unsigned long long v0 = v3[3 * j], v1 = v3[3 * j + 1], v2 = v3[3 * j + 2];
-                if ((v0 | v1 | v2) == 0ull) continue;
-                const int64_t l0 = off[j], l1 = off[j + 1];
-                int64_t p = l0 + cur[q];
-                for (; p < l1; ++p) {
-                    const int64_t k = idx[p];
-                    if (k >= c1) break;
-                    const int e = (int)(k - c0);
-                    atomicAdd(&acc[0][e], v0);
-                    atomicAdd(&acc[1][e], v1);
-                    atomicAdd(&acc[2][e], v2);
+#pragma unroll
+            for (int u = 0; u < kDPer; ++u) {
+                while (p[u] < pe[u]) {
+                    int64_t kk[4];
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) kk[w] = p[u] + w < pe[u] ? (int64_t)idx[p[u] + w] : INT64_MAX;
+                    int taken = 0;
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) {
+                        if (taken == w && kk[w] < c1) {   // the list is sorted: stop at the first k >= c1
+                            const int e = (int)(kk[w] - c0);
+                            atomicAdd(&acc[0][e], v0[u]);
+                            atomicAdd(&acc[1][e], v1[u]);
+                            atomicAdd(&acc[2][e], v2[u]);
+                            ++taken;
+                        }
+                    }
+                    p[u] += taken;
+                    if (taken < 4) break;
                 }
-                cur[q] = (int32_t)(p - l0);
             }
             __syncthreads();
             for (int e = tid; e < (int)(c1 - c0); e += 256) {
@@ -1086,7 +1057,7 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             CS_HIP(hipEventRecord(e0, st));
             ga.kb0 = prm.kb_lo;
             ga.kb1 = prm.kb_hi;
-            k_grm_pass<<<nb, 256, 0, st>>>(ga);
+            k_grm_xpass<1, false><<<nb, 256, 0, st>>>(ga);
             ++launches;
             // E passes, two digits per launch, over SNP ranges that keep int32 exact
             for (int64_t c0 = prm.kb_lo; c0 < prm.kb_hi; c0 += chunk_kb) {
@@ -1097,8 +1068,8 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
                     GrmArgs g2 = gc;
                     g2.shift = 8 * l;
                     g2.digA = ddig.as<uint8_t>() + (size_t)l * Mpad;
-                    if (l + 1 < nC) k_grm_epass<2><<<nb, 256, 0, st>>>(g2);
-                    else k_grm_epass<1><<<nb, 256, 0, st>>>(g2);
+                    if (l + 1 < nC) k_grm_xpass<2, true><<<nb, 256, 0, st>>>(g2);
+                    else k_grm_xpass<1, true><<<nb, 256, 0, st>>>(g2);
                     ++launches;
                 }
             }
