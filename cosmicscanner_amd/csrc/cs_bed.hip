@@ -308,11 +308,18 @@ void ensure_stats(cs_bed* b) {
 // for a 16-genotype word with low bits lo = w & 0x55.. and high bits hi = (w >> 1) & 0x55..:
 // n11 = popc(hi & lo & pm), n10 = popc(hi & pm) - n11, n01 = popc(lo & pm) - n11,
 // n00 = popc(pm) - n01 - n10 - n11, each weighted by 2^plane.
+// tiles (optional): the 64-SNP row tiles to count, ntiles of them (wave w of block b takes entry
+// 4 b + w); without a list every row tile is counted.
 __global__ __launch_bounds__(256) void k_wcounts_pc(const uint8_t* __restrict__ snp, int64_t KB, int64_t M,
                                                     const uint32_t* __restrict__ planes, int nplanes,
-                                                    int64_t* __restrict__ counts) {
+                                                    int64_t* __restrict__ counts, const int32_t* __restrict__ tiles,
+                                                    int64_t ntiles) {
     const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-    const int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    int64_t tr = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tiles) {
+        if (tr >= ntiles) return;
+        tr = tiles[tr];
+    }
     if (tr * 64 >= M) return;
     const uint8_t* base = snp + ((tr * KB) << 12) + lane * 16;
     long long c[4][4];
@@ -364,7 +371,9 @@ __global__ __launch_bounds__(256) void k_wcounts_pc(const uint8_t* __restrict__ 
     }
 }
 
-void weighted_counts(cs_bed* b, const int32_t* w, int64_t n_w, int64_t* d_counts, hipStream_t st) {
+// Weight planes of the class-count kernel: planes[p][kb][g][q], word q of chunk g of k-block kb
+// covers samples 256 kb + 64 g + 16 q + u; returns the plane count (0: every weight is 0).
+static int weight_planes(cs_bed* b, const int32_t* w, int64_t n_w, std::vector<uint32_t>& planes) {
     if (n_w > b->Npad) throw InvalidArgument("class counts: more weights than samples");
     int32_t wmax = 0;
     for (int64_t i = 0; i < n_w; ++i) {
@@ -373,13 +382,8 @@ void weighted_counts(cs_bed* b, const int32_t* w, int64_t n_w, int64_t* d_counts
     }
     int np = 0;
     while (np < 31 && (wmax >> np) > 0) ++np;
-    if (np == 0) {
-        CS_HIP(hipMemsetAsync(d_counts, 0, (size_t)b->M * 32, st));
-        return;
-    }
-    // planes[p][kb][g][q]: word q of chunk g of k-block kb covers samples 256 kb + 64 g + 16 q + u
     const int64_t KB = b->KBs;
-    std::vector<uint32_t> planes((size_t)(np * KB * 16), 0u);
+    planes.assign((size_t)(np * KB * 16), 0u);
     for (int64_t i = 0; i < n_w; ++i) {
         const int32_t wi = w[i];
         if (!wi) continue;
@@ -387,11 +391,54 @@ void weighted_counts(cs_bed* b, const int32_t* w, int64_t n_w, int64_t* d_counts
         for (int p = 0; p < np; ++p)
             if ((wi >> p) & 1) planes[(size_t)(((p * KB + kb) * 4 + g) * 4 + q)] |= 1u << (2 * u);
     }
+    return np;
+}
+
+void weighted_counts(cs_bed* b, const int32_t* w, int64_t n_w, int64_t* d_counts, hipStream_t st) {
+    std::vector<uint32_t> planes;
+    const int np = weight_planes(b, w, n_w, planes);
+    if (np == 0) {
+        CS_HIP(hipMemsetAsync(d_counts, 0, (size_t)b->M * 32, st));
+        return;
+    }
     DevBuf dp(planes.size() * 4);
     CS_HIP(hipMemcpyAsync(dp.p, planes.data(), planes.size() * 4, hipMemcpyHostToDevice, st));
-    k_wcounts_pc<<<(unsigned)(b->Mpad / 256), 256, 0, st>>>(b->snp.as<uint8_t>(), KB, b->M, dp.as<uint32_t>(), np,
-                                                           d_counts);
+    k_wcounts_pc<<<(unsigned)(b->Mpad / 256), 256, 0, st>>>(b->snp.as<uint8_t>(), b->KBs, b->M, dp.as<uint32_t>(), np,
+                                                           d_counts, nullptr, 0);
     CS_HIP(hipGetLastError());
+    CS_HIP(hipStreamSynchronize(st));
+}
+
+__global__ void k_gather_counts(const int64_t* __restrict__ counts, const int32_t* __restrict__ snps, int64_t ns,
+                                int64_t* __restrict__ out) {
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e < ns * 4) out[e] = counts[(int64_t)snps[e >> 2] * 4 + (e & 3)];
+}
+
+void weighted_counts_of(cs_bed* b, const int32_t* w, int64_t n_w, const int32_t* snps, int64_t ns, int64_t* h_counts,
+                        hipStream_t st) {
+    if (ns <= 0) return;
+    std::vector<uint32_t> planes;
+    const int np = weight_planes(b, w, n_w, planes);
+    if (np == 0) {
+        std::fill(h_counts, h_counts + ns * 4, (int64_t)0);
+        return;
+    }
+    std::vector<int32_t> tiles;
+    for (int64_t i = 0; i < ns; ++i) tiles.push_back(snps[i] / 64);
+    std::sort(tiles.begin(), tiles.end());
+    tiles.erase(std::unique(tiles.begin(), tiles.end()), tiles.end());
+    const int64_t nt = (int64_t)tiles.size();
+    DevBuf dp(planes.size() * 4), dt((size_t)nt * 4), dsn((size_t)ns * 4), counts((size_t)b->M * 32),
+        sel((size_t)ns * 32);
+    CS_HIP(hipMemcpyAsync(dp.p, planes.data(), planes.size() * 4, hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(dt.p, tiles.data(), (size_t)nt * 4, hipMemcpyHostToDevice, st));
+    CS_HIP(hipMemcpyAsync(dsn.p, snps, (size_t)ns * 4, hipMemcpyHostToDevice, st));
+    k_wcounts_pc<<<(unsigned)((nt + 3) / 4), 256, 0, st>>>(b->snp.as<uint8_t>(), b->KBs, b->M, dp.as<uint32_t>(), np,
+                                                          counts.as<int64_t>(), dt.as<int32_t>(), nt);
+    k_gather_counts<<<grid_for(ns * 4), 256, 0, st>>>(counts.as<int64_t>(), dsn.as<int32_t>(), ns, sel.as<int64_t>());
+    CS_HIP(hipGetLastError());
+    CS_HIP(hipMemcpyAsync(h_counts, sel.p, (size_t)ns * 32, hipMemcpyDeviceToHost, st));
     CS_HIP(hipStreamSynchronize(st));
 }
 

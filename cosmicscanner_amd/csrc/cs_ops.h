@@ -80,6 +80,9 @@ int64_t missing_total(cs_bed* b);
 // Per-SNP class counts weighted by the host multiplicities w[0..n_w) (zero beyond); counts is
 // M x 4 int64 on the device, by code 00, 01, 10, 11.  Synchronises st.
 void weighted_counts(cs_bed* b, const int32_t* w, int64_t n_w, int64_t* d_counts, hipStream_t st);
+// The same class counts for the SNPs `snps` only, into host memory ([ns][4]).
+void weighted_counts_of(cs_bed* b, const int32_t* w, int64_t n_w, const int32_t* snps, int64_t ns, int64_t* h_counts,
+                        hipStream_t st);
 
 // Per-SNP scan over the keep list (cs_scan.hip): marker preparation, g^T R in int8 MFMA, and --
 // with a model -- the fused statistics.  R is host memory, or device memory when device_R.

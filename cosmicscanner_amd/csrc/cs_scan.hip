@@ -329,16 +329,15 @@ void markers_device(cs_bed* b, const int32_t* keep, int64_t n_keep, const int32_
     if (ns == 0) return;
     std::vector<int32_t> w((size_t)b->Npad, 0);
     for (int64_t r = 0; r < n; ++r) w[(size_t)(keep ? keep[r] : r)] += 1;
-    DevBuf counts((size_t)M * 32), dk((size_t)std::max<int64_t>(1, keep ? n : 1) * 4),
-        dsn((size_t)ns * 4), dmean((size_t)ns * 8), dpass((size_t)ns * 4);
-    weighted_counts(b, w.data(), (int64_t)w.size(), counts.as<int64_t>(), st);
-    std::vector<int64_t> cnt((size_t)M * 4);
-    CS_HIP(hipMemcpyAsync(cnt.data(), counts.p, cnt.size() * 8, hipMemcpyDeviceToHost, st));
-    CS_HIP(hipStreamSynchronize(st));
+    DevBuf dk((size_t)std::max<int64_t>(1, keep ? n : 1) * 4), dsn((size_t)ns * 4), dmean((size_t)ns * 8),
+        dpass((size_t)ns * 4);
+    // class counts of the selected SNPs only (their 64-SNP row tiles)
+    std::vector<int64_t> cnt((size_t)ns * 4);
+    weighted_counts_of(b, w.data(), (int64_t)w.size(), snps, ns, cnt.data(), st);
     std::vector<double> mean((size_t)ns, 0.0);
     std::vector<int32_t> ps((size_t)ns, 0);
     for (int64_t i = 0; i < ns; ++i) {
-        const int64_t* c = &cnt[(size_t)snps[i] * 4];
+        const int64_t* c = &cnt[(size_t)i * 4];
         const int64_t obs = c[0] + c[2] + c[3];
         const double sx = 2.0 * (double)c[0] + (double)c[2];
         double f = 0.0;
