@@ -886,15 +886,12 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
     const int64_t N = b->N, Npad = b->Npad, Mpad = b->Mpad, KB = b->KBm;
     if (Mpad > (int64_t(1) << 29))   // x~ x~ products reach 4 M in int32
         throw InvalidArgument("GRM: more than 2^29 SNPs");
-    hipStream_t st, st2;   // st2: the D pairs beside the MFMA passes
+    hipStream_t st;
     CS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    CS_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
-    hipEvent_t e0, e1, e2, ed0, ed1;
+    hipEvent_t e0, e1, e2;
     CS_HIP(hipEventCreate(&e0));
     CS_HIP(hipEventCreate(&e1));
     CS_HIP(hipEventCreate(&e2));
-    CS_HIP(hipEventCreateWithFlags(&ed0, hipEventDisableTiming));
-    CS_HIP(hipEventCreateWithFlags(&ed1, hipEventDisableTiming));
     try {
         // keep mask and masked copy
         std::vector<uint32_t> keep((size_t)(Mpad / 16), 0);
@@ -1029,16 +1026,14 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             unsigned long long* dp1 = dp0 + rows * Npad;
             unsigned long long* dp2 = dp1 + rows * Npad;
             if (useD && I0 < N) {
-                // D^v from the missing lists on st2, concurrent with the MFMA passes on st; every
-                // entry (i, k <= i) of the panel's real rows is written; the combine waits for both
-                CS_HIP(hipEventRecord(ed0, st));
-                CS_HIP(hipStreamWaitEvent(st2, ed0, 0));
+                // D^v from the missing lists; every entry (i, k <= i) of the panel's real rows is
+                // written.  (On a second stream beside the MFMA passes it took 0.44 s instead of
+                // 0.11 s alone and the GRM 3.16 s in all: the two compete for the CUs.)
                 const int64_t r1 = std::min<int64_t>(N, I0 + rows);
-                k_grm_dlds<<<(unsigned)(r1 - I0), 256, 0, st2>>>(
+                k_grm_dlds<<<(unsigned)(r1 - I0), 256, 0, st>>>(
                     b->miss_smp_off.as<int64_t>(), b->miss_smp_idx.as<int32_t>(), b->miss_snp_off.as<int64_t>(),
                     b->miss_snp_idx.as<int32_t>(), dv3.as<unsigned long long>(), I0, r1, I0, Npad, dp0, dp1, dp2);
                 CS_HIP(hipGetLastError());
-                CS_HIP(hipEventRecord(ed1, st2));
             }
             GrmArgs ga{};
             ga.smp = masked.as<uint8_t>();
@@ -1075,7 +1070,6 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
             }
             CS_HIP(hipGetLastError());
             CS_HIP(hipEventRecord(e1, st));
-            if (useD && I0 < N) CS_HIP(hipStreamWaitEvent(st, ed1, 0));
             // int8 MFMA ops issued: T1 and each digit's E product over all SNP blocks
             mfma_ops += (double)(1 + nC) * (double)quadrants * 64.0 * 64.0 *
                         (double)((prm.kb_hi - prm.kb_lo) * 256) * 2.0;
@@ -1123,23 +1117,16 @@ static void grm_run(cs_bed* b, int alg, float min_maf, float min_hwe, const int3
         g_grm_timing.mfma_ops = mfma_ops;
     } catch (...) {
         (void)hipStreamSynchronize(st);
-        (void)hipStreamSynchronize(st2);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipEventDestroy(e2);
-        (void)hipEventDestroy(ed0);
-        (void)hipEventDestroy(ed1);
         (void)hipStreamDestroy(st);
-        (void)hipStreamDestroy(st2);
         throw;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(e2);
-    (void)hipEventDestroy(ed0);
-    (void)hipEventDestroy(ed1);
     CS_HIP(hipStreamDestroy(st));
-    CS_HIP(hipStreamDestroy(st2));
 }
 
 extern "C" {
