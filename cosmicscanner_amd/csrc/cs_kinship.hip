@@ -1401,6 +1401,30 @@ int64_t csamd::kinship_work_bytes(int64_t Npad, int64_t Mpad, int prec, int64_t 
     return t;
 }
 
+// The K.V apply enqueued on the caller's stream, with no wait on the host: the serving loop puts
+// every active request's iteration on one stream and waits once per step.  (The FP32 comparator
+// precision runs synchronously on the operator's stream.)
+void csamd::kinship_apply_async(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY, int64_t ldy,
+                                hipStream_t st) {
+    if (op->prec == CS_PREC_FP32) {
+        CS_HIP(hipStreamSynchronize(st));
+        apply_device_impl(op, dX, ldx, k, dY, ldy);
+        return;
+    }
+    if (k < 0 || (k > 0 && (ldx < op->n || ldy < op->n)))
+        throw RuntimeError("PlinkKinshipOperator::apply: dimension mismatch");
+    if (k == 0) return;
+    hipStream_t own = op->stream;
+    op->stream = st;
+    try {
+        apply_int8(op, dX, ldx, k, dY, ldy);
+    } catch (...) {
+        op->stream = own;
+        throw;
+    }
+    op->stream = own;
+}
+
 void csamd::kinship_release_workspace(cs_kinship* op) {
     CS_HIP(hipStreamSynchronize(op->stream));
     for (DevBuf* b : {&op->dX, &op->dY, &op->Vt, &op->BV, &op->W1, &op->W2, &op->B1, &op->B2, &op->Yint, &op->mxW,

@@ -75,6 +75,9 @@ bool kinship_is_compact(const cs_kinship* op);
 int64_t kinship_work_bytes(int64_t Npad, int64_t Mpad, int prec, int64_t k);
 // Free the operator's grown workspaces (the next apply grows them again).
 void kinship_release_workspace(cs_kinship* op);
+// K.V (dY = K dX, device buffers) enqueued on st without a host wait.
+void kinship_apply_async(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY, int64_t ldy,
+                         hipStream_t st);
 // Number of missing genotypes over all samples and SNPs (from the class counts; cached).
 int64_t missing_total(cs_bed* b);
 // Per-SNP class counts weighted by the host multiplicities w[0..n_w) (zero beyond); counts is

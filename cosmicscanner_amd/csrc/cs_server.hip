@@ -146,13 +146,36 @@ namespace {
 enum { REQ_PENDING = 0, REQ_ACTIVE = 1, REQ_DONE = 2, REQ_FAILED = -1 };
 
 // PCG state of the columns a request holds (n x cols each, column-major).
+// Pinned host memory for a request's per-iteration exchange with the device (the squared
+// residual norms read back, the active-column flags sent), so both copies stay asynchronous.
+struct Pinned {
+    void* p = nullptr;
+    Pinned() = default;
+    Pinned(const Pinned&) = delete;
+    Pinned& operator=(const Pinned&) = delete;
+    Pinned& operator=(Pinned&& o) noexcept {
+        if (this != &o) {
+            if (p) (void)hipHostFree(p);
+            p = o.p;
+            o.p = nullptr;
+        }
+        return *this;
+    }
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 struct Solve {
     int64_t cols = 0;
     int64_t bytes = 0;   // charged to the budget while held (with the operator's workspace)
     DevBuf X, R, Z, P, KP, AP, partial, rz, rzn, pap, rr, bb, act;
     std::vector<double> hbb, hrr;
     std::vector<int32_t> hact;
+    Pinned pin;          // cols doubles (residual norms), then cols int32 (active flags)
     bool act_dirty = true;
+    double* pin_rr() const { return static_cast<double*>(pin.p); }
+    int32_t* pin_act() const { return reinterpret_cast<int32_t*>(static_cast<double*>(pin.p) + cols); }
 };
 
 struct Request {
@@ -315,6 +338,7 @@ void place_columns(cs_server* s, Request& q, int64_t k) {
     v.partial.alloc((size_t)(k * kSrvDotBlocks) * 8);
     for (DevBuf* b : {&v.rz, &v.rzn, &v.pap, &v.rr, &v.bb}) b->alloc((size_t)k * 8);
     v.act.alloc((size_t)k * 4);
+    CS_HIP(hipHostMalloc(&v.pin.p, (size_t)k * 12, hipHostMallocDefault));
     v.hbb.assign((size_t)k, 0.0);
     v.hrr.assign((size_t)k, 0.0);
     v.hact.assign((size_t)k, 0);
@@ -453,17 +477,18 @@ bool admit_chunk(cs_server* s, Request& q) {
     return true;
 }
 
-// One PCG iteration over the request's columns.
-void iterate(cs_server* s, Request& q) {
+// One PCG iteration over the request's columns, enqueued on the server stream: the K.V apply,
+// the vector updates and the squared residual norms (copied to pinned memory).  The step waits
+// for the stream once, after every active request's iteration is queued.
+void iterate_launch(cs_server* s, Request& q) {
     Solve& v = q.sv;
     const int64_t n = q.n, k = v.cols;
     if (v.act_dirty) {
-        CS_HIP(hipMemcpyAsync(v.act.p, v.hact.data(), (size_t)k * 4, hipMemcpyHostToDevice, s->st));
+        std::copy(v.hact.begin(), v.hact.end(), v.pin_act());
+        CS_HIP(hipMemcpyAsync(v.act.p, v.pin_act(), (size_t)k * 4, hipMemcpyHostToDevice, s->st));
         v.act_dirty = false;
     }
-    CS_HIP(hipStreamSynchronize(s->st));
-    const int rc = cs_kinship_apply_device(q.K, v.P.as<double>(), n, k, v.KP.as<double>(), n);
-    if (rc != CS_OK) throw RuntimeError(cs_last_error());
+    kinship_apply_async(q.K, v.P.as<double>(), n, k, v.KP.as<double>(), n, s->st);
     k_srv_apply<<<grid_for(n * k), 256, 0, s->st>>>(v.KP.as<double>(), v.P.as<double>(), q.sg, q.se, n * k,
                                                    v.AP.as<double>());
     dots(s, v, v.P.as<double>(), v.AP.as<double>(), n, v.pap, nullptr);
@@ -475,7 +500,15 @@ void iterate(cs_server* s, Request& q) {
                                                  v.rz.as<double>(), v.act.as<int32_t>(), n, k);
     k_srv_rz_keep<<<grid_for(k), 256, 0, s->st>>>(v.rz.as<double>(), v.rzn.as<double>(), v.act.as<int32_t>(), k);
     CS_HIP(hipGetLastError());
-    dots(s, v, v.R.as<double>(), v.R.as<double>(), n, v.rr, &v.hrr);
+    dots(s, v, v.R.as<double>(), v.R.as<double>(), n, v.rr, nullptr);
+    CS_HIP(hipMemcpyAsync(v.pin_rr(), v.rr.p, (size_t)k * 8, hipMemcpyDeviceToHost, s->st));
+}
+
+// After the stream wait: the iteration count and each column's convergence.
+void iterate_finish(cs_server* s, Request& q) {
+    Solve& v = q.sv;
+    const int64_t k = v.cols;
+    std::copy(v.pin_rr(), v.pin_rr() + k, v.hrr.begin());
     int& its = q.phase == 1 ? q.iterations : q.chunk_iterations;
     ++its;
     // convergence per column (the exact stage solves to its own tolerance)
@@ -665,16 +698,21 @@ int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t*
             if (ok) it = s->pending.erase(it);
             else ++it;
         }
-        // one PCG iteration of every request with live columns, then advance the converged ones
+        // one PCG iteration of every request with live columns, queued back to back on the
+        // server stream and waited for once, then advance the converged ones
+        std::vector<Request*> ran;
         for (auto& kv : s->reqs) {
             Request& q = *kv.second;
             if (q.status != REQ_ACTIVE || q.sv.cols == 0 || !live(q)) continue;
             try {
-                iterate(s, q);
+                iterate_launch(s, q);
+                ran.push_back(&q);
             } catch (const std::exception& e) {
                 fail(s, q, e.what());
             }
         }
+        if (!ran.empty()) CS_HIP(hipStreamSynchronize(s->st));
+        for (Request* q : ran) iterate_finish(s, *q);
         for (auto& kv : s->reqs) {
             Request& q = *kv.second;
             if (q.status != REQ_ACTIVE || q.sv.cols == 0 || live(q)) continue;
