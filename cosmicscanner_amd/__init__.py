@@ -57,6 +57,7 @@ def _load():
         "cs_version": ([], ctypes.c_int),
         "cs_dev_alloc": ([_i64, ctypes.POINTER(_vp)], ctypes.c_int),
         "cs_dev_free": ([_vp], ctypes.c_int),
+        "cs_dev_release_cache": ([], ctypes.c_int),
         "cs_dev_upload": ([_vp, _vp, _i64], ctypes.c_int),
         "cs_dev_download": ([_vp, _vp, _i64], ctypes.c_int),
         "cs_dev_synchronize": ([], ctypes.c_int),

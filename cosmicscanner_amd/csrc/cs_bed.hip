@@ -751,7 +751,12 @@ int cs_bed_destroy(cs_bed* bed) {
         hipStream_t s = bed->stream;
         delete bed;
         if (s) (void)hipStreamDestroy(s);
+        dev_cache_release();   // the blocks of this matrix's work are sized for it
     });
+}
+
+int cs_dev_release_cache(void) {
+    return guarded([&] { dev_cache_release(); });
 }
 
 int cs_bed_dims(const cs_bed* bed, int64_t* n_samples, int64_t* n_snps) {

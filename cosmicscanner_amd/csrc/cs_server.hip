@@ -614,6 +614,7 @@ int cs_server_destroy(cs_server* s) {
                 if (kv.second->fetching) throw InvalidArgument("cs_server_destroy: results are being fetched");
         }
         delete s;
+        dev_cache_release();   // the requests' compact copies and result arrays
     });
 }
 

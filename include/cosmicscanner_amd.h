@@ -56,6 +56,10 @@ int cs_dev_download(void* dst, const void* src, int64_t bytes);
 int cs_dev_synchronize(void);
 /* Plain device-to-device copy kernel (also the calibration reference for HBM byte counters). */
 int cs_dev_copy(void* dst, const void* src, int64_t bytes);
+/* Frees the library's cache of released device blocks (blocks of 8 MB and more are kept for
+ * reuse, up to 64 GiB per process or CSAMD_DEVICE_CACHE_BYTES; destroying a genotype matrix or a
+ * server also frees it).  No reference counterpart. */
+int cs_dev_release_cache(void);
 
 /* ---- Genotypes (replaces PlinkReader::load/openBed, libCosmicCore/src/plink_reader.cpp:28-92,
  * with the packed matrix resident in HBM).  `body` is the .bed content after the 3 magic bytes:
