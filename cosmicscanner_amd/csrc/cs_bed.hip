@@ -632,6 +632,13 @@ cs_bed* bed_rows_copy(cs_bed* parent, const int32_t* keep, int64_t n) {
     return b;
 }
 
+void bed_free(cs_bed* b) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    hipStream_t s = b->stream;
+    delete b;
+    if (s) (void)hipStreamDestroy(s);
+}
+
 void ensure_missing_lists(cs_bed* b) {
     if (b->miss_nnz >= 0) return;
     ensure_sample_major(b);
@@ -747,10 +754,7 @@ int cs_bed_create_synthetic(uint64_t seed, int64_t n_samples, int64_t n_snps, do
 int cs_bed_destroy(cs_bed* bed) {
     return guarded([&] {
         if (!bed) return;
-        if (bed->stream) (void)hipStreamSynchronize(bed->stream);
-        hipStream_t s = bed->stream;
-        delete bed;
-        if (s) (void)hipStreamDestroy(s);
+        bed_free(bed);
         dev_cache_release();   // the blocks of this matrix's work are sized for it
     });
 }

@@ -938,6 +938,18 @@ __global__ void k_w_from_u(const double* __restrict__ U, int64_t ldu, int64_t M,
 
 static void apply_int8(cs_kinship* op, const double* dX, int64_t ldx, int64_t k, double* dY,
                        int64_t ldy, const KvWork* ws = nullptr, int mode = KV_APPLY) {
+    // A last partial group of r columns with r L <= 16 runs as its own apply with packed limbs
+    // (one MFMA tile, bandwidth-bound) instead of a dense group of L tiles.  Every column's
+    // quantisation scale and integer products are its own, so the results are bitwise those of
+    // one apply of all k columns.
+    if (!ws && mode == KV_APPLY && k > 16) {
+        const int64_t r = k % 16;
+        if (r > 0 && r * prec_limbs(op->prec) <= 16) {
+            apply_int8(op, dX, ldx, k - r, dY, ldy, nullptr, mode);
+            apply_int8(op, dX + (k - r) * ldx, ldx, r, dY + (k - r) * ldy, ldy, nullptr, mode);
+            return;
+        }
+    }
     cs_bed* b = row_bed(op);
     const bool direct = rows_direct(op);
     hipStream_t st = op->stream;
@@ -1317,7 +1329,7 @@ int cs_kinship_destroy(cs_kinship* op) {
         hipStream_t s = op->stream;
         cs_bed* rows = op->rows;
         delete op;
-        if (rows) (void)cs_bed_destroy(rows);
+        if (rows) bed_free(rows);
         if (s) (void)hipStreamDestroy(s);
     });
 }

@@ -59,6 +59,9 @@ void ensure_stats(cs_bed* b);
 void ensure_sample_major(cs_bed* b);
 // CSR lists of the missing genotypes (both orientations); idempotent.
 void ensure_missing_lists(cs_bed* b);
+// Frees a genotype handle (no release of the device-memory cache: for the row copies that
+// kinship operators own, whose blocks the next cohort reuses).
+void bed_free(cs_bed* b);
 // A new resident matrix of the cohort keep[0..n) (host indices, caller's order): both tiled
 // copies gathered from the parent's rows, the cohort's own class counts, and the parent's
 // all-sample means and inverse standard deviations.  Freed with cs_bed_destroy.

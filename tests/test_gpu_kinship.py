@@ -165,6 +165,31 @@ def test_kinship_packed_limbs_small_k(csa, mode, tol):
     op.close()
 
 
+@pytest.mark.parametrize("mode", [2, 3, 4])
+def test_partial_last_group_bitwise(csa, mode):
+    """k = 16 g + r with r L <= 16 runs the last r columns as their own packed-limb apply; every
+    column is quantised and multiplied on its own, so the columns equal the same columns of a
+    wider apply that needs no split (bitwise), and the oracle within the precision's bound."""
+    N, M = 1400, 2700
+    body = oracle.gen_hwe_bed(79, N, M, 0.01, 0.5, 0.01)
+    reader = csa.PlinkReader.from_body(body, N, M)
+    means, inv = oracle.stats(body, N, M)
+    rng = np.random.default_rng(10)
+    keep = np.sort(rng.choice(N, 1100, replace=False)).astype(np.int32)
+    op = csa.PlinkKinshipOperator(reader, keep, None)
+    op.set_precision(mode)
+    r = 16 // mode
+    k = 32 + r
+    X = rng.standard_normal((op.rows(), 48))
+    wide = op.apply(X)                      # 48 columns: three whole groups, no split
+    split = op.apply(X[:, :k])              # two groups + r packed-limb columns
+    assert np.array_equal(split, wide[:, :k])
+    ref = oracle.kinship_apply(body, N, means, inv, keep, None, X[:, :k])
+    tol = {2: 1e-3, 3: 2e-6, 4: 3e-8}[mode]
+    assert colrel(split, ref) < tol
+    op.close()
+
+
 def test_kinship_sketch_width_266(csa):
     # The reference's randomized low-rank path applies K to n x (256 + 10) sketches
     # (lmm_engine.cpp:915-946); widths above 256 columns exercise the per-column scale grid.
