@@ -325,7 +325,14 @@ def scan_leg(csa, reader, N, M, T=16):
                 end_to_end_trait_genotypes_per_s=N * M * T / total,
                 roofline=dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                               frac=round(achieved / HBM_PEAK_GBS, 4), kernel="k_geno<EPI_PQ> (g^T R)",
-                              kernel_ms=round(ms_prod, 4), bytes_per_launch=gbytes))
+                              kernel_ms=round(ms_prod, 4), bytes_per_launch=gbytes),
+                # kernel (a): the 2-bit decode and per-SNP class counts of the pass, one read of the
+                # packed SNP-major matrix
+                decode_roofline=dict(bound="hbm", achieved=round(gbytes / (ms_counts * 1e-3) / 1e9, 1),
+                                     peak=HBM_PEAK_GBS, unit="GB/s",
+                                     frac=round(gbytes / (ms_counts * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                     kernel="k_wcounts_pc (decode + class counts)", kernel_ms=round(ms_counts, 4),
+                                     bytes_per_launch=gbytes))
 
 
 # GRCh38 autosome lengths (Mb), the chromosome proportions of the LOCO workload
