@@ -406,3 +406,26 @@ def test_compact_rows_bitwise(csa, miss):
     allrows.compact_rows()                        # the all-sample cohort: nothing to copy
     assert allrows.device_bytes() == b0
     allrows.close()
+
+
+def test_device_cache_release_between_applies(csa):
+    """A cohort operator's compact row copies (two layouts of 18.5 MB here) go back to the device
+    cache when the operator is destroyed and are handed to the next one; cs_dev_release_cache
+    frees them.  Each apply matches the uncompacted operator bitwise, whichever path its
+    memory came from."""
+    N, M, n = 7000, 12000, 6000
+    reader = csa.PlinkReader.synthetic(515, N, M, 0.01, 0.5, 0.01)
+    rng = np.random.default_rng(12)
+    keep = np.sort(rng.choice(N, n, replace=False)).astype(np.int32)
+    X = rng.standard_normal((n, 24))
+    plain = csa.PlinkKinshipOperator(reader, keep, None)
+    want = plain.apply(X)
+    plain.close()
+    for release in (False, False, True, False):
+        op = csa.PlinkKinshipOperator(reader, keep, None)
+        op.compact_rows()
+        assert np.array_equal(op.apply(X), want), release
+        op.close()
+        if release:
+            csa._check(csa.lib().cs_dev_release_cache())
+    reader.close()
