@@ -554,8 +554,10 @@ def serving_leg(csa, n=100_000, m=1_000_000, n_requests=200, rate=5.0, seed=2026
         keep = np.sort(rng.choice(n, nc, replace=False)).astype(np.int32)
         T = int(rng.integers(1, 65))
         c = int(rng.integers(0, 11)) + 1
-        X = np.column_stack([np.ones(nc)] + [rng.standard_normal(nc) for _ in range(c - 1)]).astype(np.float32)
-        Y = rng.standard_normal((nc, T))
+        # column-major, the layout the server's submit takes (no conversion copy at submission)
+        X = np.asfortranarray(np.column_stack([np.ones(nc)] + [rng.standard_normal(nc) for _ in range(c - 1)]),
+                              dtype=np.float32)
+        Y = np.asfortranarray(rng.standard_normal((nc, T)))
         h2 = float(rng.uniform(0.1, 0.8))
         cal = np.sort(rng.choice(m, 16, replace=False)).astype(np.int32)
         reqs.append(dict(Y=Y, X=X, sigma2_g=h2, sigma2_e=1.0 - h2, cal_snps=cal, keep=keep))

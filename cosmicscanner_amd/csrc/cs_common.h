@@ -56,12 +56,40 @@ int guarded(F&& f) {
 inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 // ---------------------------------------------------------------------------------------------
-// Device memory with a per-device cache of large blocks (cs_alloc.cpp): dev_alloc returns a block
-// of at least n bytes and its capacity, dev_free takes it back (large blocks are kept for reuse),
-// dev_cache_release frees every cached block.
+// Device memory with a per-device cache of blocks (cs_alloc.cpp): dev_alloc returns a block of at
+// least n bytes and its capacity, dev_free takes it back (large blocks, and inside the scopes
+// below all blocks, are kept for reuse), dev_cache_release frees every cached block.
 void* dev_alloc(size_t n, size_t* cap);
 void dev_free(void* p, size_t cap);
 void dev_cache_release();
+
+// A section of the calling thread whose device work runs on `st` and `st2` (optional), plus other
+// streams that the section's functions synchronise before they return.  Blocks released inside
+// it are held back and enter the cache at its end, after both streams have been synchronised, as
+// idle blocks that are reused with no device-wide wait.  sync_device() makes the end wait for the
+// whole device instead (for error paths, which may leave other streams' work in flight).
+// Sections nest; the outermost one ends it.
+class DeferFrees {
+public:
+    explicit DeferFrees(hipStream_t st, hipStream_t st2 = nullptr);
+    ~DeferFrees();
+    void sync_device();
+    DeferFrees(const DeferFrees&) = delete;
+    DeferFrees& operator=(const DeferFrees&) = delete;
+};
+
+// A section of the calling thread that releases only blocks no work in flight uses (their
+// kernels and copies have been waited for): they enter the cache as idle blocks at once.
+class IdleFrees {
+public:
+    IdleFrees();
+    ~IdleFrees();
+    IdleFrees(const IdleFrees&) = delete;
+    IdleFrees& operator=(const IdleFrees&) = delete;
+
+private:
+    int prev_;
+};
 
 // Device buffer (RAII over dev_alloc).  `bytes` is the size asked for; `cap` the block's capacity.
 struct DevBuf {

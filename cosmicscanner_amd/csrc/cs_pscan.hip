@@ -95,13 +95,18 @@ void pscan_prepare_rhs(cs_bed* bed, const int32_t* keep, int64_t n_keep, int64_t
         if (cpass[(size_t)i]) cidx.push_back(i);
     L = (int64_t)cidx.size();
     const int64_t k = T + c + L;
-    std::vector<double> host((size_t)(n * (T + c)));
-    for (int64_t t = 0; t < T; ++t)
-        for (int64_t i = 0; i < n; ++i) host[(size_t)(t * n + i)] = Y[t * ldy + i];
+    // B = [Y, X, G_cal]: Y's columns straight from the caller's array, X widened to doubles
+    std::vector<double> hx((size_t)(n * c));
     for (int64_t a = 0; a < c; ++a)
-        for (int64_t i = 0; i < n; ++i) host[(size_t)((T + a) * n + i)] = (double)X[a * ldx + i];
+        for (int64_t i = 0; i < n; ++i) hx[(size_t)(a * n + i)] = (double)X[a * ldx + i];
     B.alloc((size_t)(n * k) * 8);
-    CS_HIP(hipMemcpyAsync(B.p, host.data(), host.size() * 8, hipMemcpyHostToDevice, st));
+    if (ldy == n) {
+        CS_HIP(hipMemcpyAsync(B.p, Y, (size_t)(n * T) * 8, hipMemcpyHostToDevice, st));
+    } else {
+        CS_HIP(hipMemcpy2DAsync(B.p, (size_t)n * 8, Y, (size_t)ldy * 8, (size_t)n * 8, (size_t)T,
+                                hipMemcpyHostToDevice, st));
+    }
+    if (c > 0) CS_HIP(hipMemcpyAsync(B.as<double>() + T * n, hx.data(), hx.size() * 8, hipMemcpyHostToDevice, st));
     for (int64_t l = 0; l < L; ++l)
         CS_HIP(hipMemcpyAsync(B.as<double>() + (T + c + l) * n, Gc.as<double>() + cidx[(size_t)l] * n,
                               (size_t)n * 8, hipMemcpyDeviceToDevice, st));

@@ -11,12 +11,17 @@
 // advances every active request by one iteration (one K.V apply each).  New requests are
 // admitted at step boundaries as column capacity and the HBM budget free up (continuous
 // batching).  A request whose columns have all converged runs its projection, calibration and
-// fused per-SNP scan at once and gives its solve buffers back; the SNPs its exact stage selects
+// fused per-SNP scan in the next step and gives its solve buffers back; the SNPs its exact stage selects
 // then return as a second phase of columns (their prepared markers, in chunks that fit the free
 // capacity and budget), whose solutions give the exact GLS variances.  The budget covers a
 // request's compact copy, operator lists, preconditioner and projection from admission to
 // completion, its solve buffers and operator workspace while held, and its result arrays from
 // admission until the client releases them.
+//
+// A step first queues the iterations on the server stream, then does the host-heavy work of
+// admissions and finished solves (right-hand sides, row copies, projections, scans, exact-stage
+// chunks) with its device work on a second stream, so that this work and its host waits overlap
+// the iterations instead of leaving the device idle; the step ends with both streams waited for.
 //
 // A column's arithmetic depends only on its own request (per-column quantisation scales, exact
 // integer products, per-column dots with a fixed reduction order, and an operator whose choice of
@@ -173,6 +178,7 @@ struct Solve {
     std::vector<double> hbb, hrr;
     std::vector<int32_t> hact;
     Pinned pin;          // cols doubles (residual norms), then cols int32 (active flags)
+    size_t pin_cap = 0;  // its capacity (the block comes from, and goes back to, the server's pool)
     bool act_dirty = true;
     double* pin_rr() const { return static_cast<double*>(pin.p); }
     int32_t* pin_act() const { return reinterpret_cast<int32_t*>(static_cast<double*>(pin.p) + cols); }
@@ -223,8 +229,12 @@ struct cs_server {
     int64_t N = 0, cap = 0, used = 0;
     int64_t budget = 0, budget_used = 0;
     std::vector<int32_t> variants;
-    hipStream_t st = nullptr;
-    rocblas_handle blas = nullptr;
+    hipStream_t st = nullptr;     // the requests' iterations
+    hipStream_t side = nullptr;   // admissions and finished solves, beside the iterations
+    rocblas_handle blas = nullptr;   // on the side stream
+    // pinned blocks for the iterations' residual readbacks, reused across requests (pinned memory
+    // is slow to allocate and free, and freeing it waits for the device)
+    std::vector<std::pair<size_t, void*>> pin_pool;
     std::vector<double> diag;
     std::deque<int64_t> pending;
     std::map<int64_t, std::unique_ptr<Request>> reqs;
@@ -239,14 +249,17 @@ struct cs_server {
     hipStream_t dst = nullptr;   // result delivery
     ~cs_server() {
         if (st) (void)hipStreamSynchronize(st);
+        if (side) (void)hipStreamSynchronize(side);
         if (dst) (void)hipStreamSynchronize(dst);
         reqs.clear();
+        for (auto& b : pin_pool) (void)hipHostFree(b.second);
         for (int b = 0; b < 2; ++b) {
             if (stage[b]) (void)hipHostFree(stage[b]);
             if (stage_ev[b]) (void)hipEventDestroy(stage_ev[b]);
         }
         if (blas) (void)rocblas_destroy_handle(blas);
         if (st) (void)hipStreamDestroy(st);
+        if (side) (void)hipStreamDestroy(side);
         if (dst) (void)hipStreamDestroy(dst);
     }
     double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
@@ -303,16 +316,39 @@ void deliver(cs_server* s, const std::vector<Download>& list) {
     }
 }
 
-void dots(cs_server* s, Solve& v, const double* A, const double* B, int64_t n, DevBuf& out,
-          std::vector<double>* host) {
+void dots(Solve& v, const double* A, const double* B, int64_t n, DevBuf& out, std::vector<double>* host,
+          hipStream_t st) {
     const int64_t k = v.cols;
-    k_srv_dots<<<dim3(kSrvDotBlocks, (unsigned)k), 256, 0, s->st>>>(A, B, n, v.partial.as<double>());
-    k_srv_reduce<<<grid_for(k), 256, 0, s->st>>>(v.partial.as<double>(), k, out.as<double>());
+    k_srv_dots<<<dim3(kSrvDotBlocks, (unsigned)k), 256, 0, st>>>(A, B, n, v.partial.as<double>());
+    k_srv_reduce<<<grid_for(k), 256, 0, st>>>(v.partial.as<double>(), k, out.as<double>());
     CS_HIP(hipGetLastError());
     if (host) {
-        CS_HIP(hipMemcpyAsync(host->data(), out.p, (size_t)k * 8, hipMemcpyDeviceToHost, s->st));
-        CS_HIP(hipStreamSynchronize(s->st));
+        CS_HIP(hipMemcpyAsync(host->data(), out.p, (size_t)k * 8, hipMemcpyDeviceToHost, st));
+        CS_HIP(hipStreamSynchronize(st));
     }
+}
+
+// A pinned block of at least `bytes` from the server's pool, and its return.
+void pin_take(cs_server* s, Solve& v, size_t bytes) {
+    size_t best = s->pin_pool.size();
+    for (size_t i = 0; i < s->pin_pool.size(); ++i)
+        if (s->pin_pool[i].first >= bytes && (best == s->pin_pool.size() || s->pin_pool[i].first < s->pin_pool[best].first))
+            best = i;
+    if (best < s->pin_pool.size()) {
+        v.pin_cap = s->pin_pool[best].first;
+        v.pin.p = s->pin_pool[best].second;
+        s->pin_pool.erase(s->pin_pool.begin() + (std::ptrdiff_t)best);
+        return;
+    }
+    const size_t cap = std::max<size_t>(4096, (bytes + 4095) / 4096 * 4096);
+    CS_HIP(hipHostMalloc(&v.pin.p, cap, hipHostMallocDefault));
+    v.pin_cap = cap;
+}
+void pin_give(cs_server* s, Solve& v) {
+    if (!v.pin.p) return;
+    s->pin_pool.push_back({v.pin_cap, v.pin.p});
+    v.pin.p = nullptr;
+    v.pin_cap = 0;
 }
 
 int64_t result_bytes_of(const cs_server* s, const Request& q) {
@@ -338,15 +374,15 @@ void place_columns(cs_server* s, Request& q, int64_t k) {
     v.partial.alloc((size_t)(k * kSrvDotBlocks) * 8);
     for (DevBuf* b : {&v.rz, &v.rzn, &v.pap, &v.rr, &v.bb}) b->alloc((size_t)k * 8);
     v.act.alloc((size_t)k * 4);
-    CS_HIP(hipHostMalloc(&v.pin.p, (size_t)k * 12, hipHostMallocDefault));
+    pin_take(s, v, (size_t)k * 12);
     v.hbb.assign((size_t)k, 0.0);
     v.hrr.assign((size_t)k, 0.0);
     v.hact.assign((size_t)k, 0);
-    k_srv_init<<<grid_for(n * k), 256, 0, s->st>>>(q.B.as<double>(), q.minv.as<double>(), n, k, v.X.as<double>(),
+    k_srv_init<<<grid_for(n * k), 256, 0, s->side>>>(q.B.as<double>(), q.minv.as<double>(), n, k, v.X.as<double>(),
                                                   v.R.as<double>(), v.Z.as<double>(), v.P.as<double>());
     CS_HIP(hipGetLastError());
-    dots(s, v, v.R.as<double>(), v.Z.as<double>(), n, v.rz, nullptr);
-    dots(s, v, q.B.as<double>(), q.B.as<double>(), n, v.bb, &v.hbb);
+    dots(v, v.R.as<double>(), v.Z.as<double>(), n, v.rz, nullptr, s->side);
+    dots(v, q.B.as<double>(), q.B.as<double>(), n, v.bb, &v.hbb, s->side);
     for (int64_t c = 0; c < k; ++c) {
         v.hrr[(size_t)c] = v.hbb[(size_t)c];
         v.hact[(size_t)c] = v.hbb[(size_t)c] > 0.0 ? 1 : 0;
@@ -355,10 +391,12 @@ void place_columns(cs_server* s, Request& q, int64_t k) {
     q.chunk_iterations = 0;
 }
 
-// Give back the columns, their buffers and the operator's workspace; safe on any path.
+// Give back the columns, their buffers and the operator's workspace.  The request has no device
+// work in flight: its last iteration was waited for in the step that saw it converge, and a
+// failure waits for both streams first.
 void release_solve(cs_server* s, Request& q) {
     Solve& v = q.sv;
-    if (v.cols > 0 || v.bytes > 0) (void)hipStreamSynchronize(s->st);
+    pin_give(s, v);
     s->used -= v.cols;
     s->budget_used -= v.bytes;
     v = Solve();
@@ -386,8 +424,11 @@ void release_hold(cs_server* s, Request& q) {
     q.hold_bytes = 0;
 }
 
-// A request that failed gives back everything it holds.
+// A request that failed gives back everything it holds (after its work in flight, on either
+// stream, has ended).
 void fail(cs_server* s, Request& q, const std::string& msg) {
+    (void)hipStreamSynchronize(s->st);
+    (void)hipStreamSynchronize(s->side);
     release_solve(s, q);
     free_results(s, q);
     release_hold(s, q);
@@ -416,7 +457,7 @@ bool admit(cs_server* s, Request& q) {
         DevBuf B;
         int64_t L = 0;
         pscan_prepare_rhs(s->bed, q.keep.data(), q.n, q.n, q.Y.data(), q.n, q.T, q.X.data(), q.n, q.c,
-                          q.cal.empty() ? nullptr : q.cal.data(), (int64_t)q.cal.size(), q.min_maf, B, L, s->st);
+                          q.cal.empty() ? nullptr : q.cal.data(), (int64_t)q.cal.size(), q.min_maf, B, L, s->side);
         q.L = L;
         q.k = q.T + q.c + L;
         q.B = std::move(B);
@@ -440,7 +481,7 @@ bool admit(cs_server* s, Request& q) {
     std::vector<double> mv((size_t)n);
     for (int64_t r = 0; r < n; ++r) mv[(size_t)r] = 1.0 / (q.sg * s->diag[(size_t)q.keep[(size_t)r]] + q.se);
     q.minv.alloc((size_t)n * 8);
-    CS_HIP(hipMemcpyAsync(q.minv.p, mv.data(), (size_t)n * 8, hipMemcpyHostToDevice, s->st));
+    CS_HIP(hipMemcpyAsync(q.minv.p, mv.data(), (size_t)n * 8, hipMemcpyHostToDevice, s->side));
     q.status = REQ_ACTIVE;
     q.t_admit = s->now();
     place_columns(s, q, q.k);
@@ -471,7 +512,7 @@ bool admit_chunk(cs_server* s, Request& q) {
     s->budget_used += q.sv.bytes;
     q.B.alloc((size_t)(q.n * sb) * 8);
     markers_device(s->bed, q.keep.data(), q.n, sel.data(), sb, q.min_maf, q.B.as<double>(), q.n, nullptr, nullptr,
-                   s->st);
+                   s->side);
     q.chunk = sb;
     place_columns(s, q, sb);
     return true;
@@ -491,16 +532,16 @@ void iterate_launch(cs_server* s, Request& q) {
     kinship_apply_async(q.K, v.P.as<double>(), n, k, v.KP.as<double>(), n, s->st);
     k_srv_apply<<<grid_for(n * k), 256, 0, s->st>>>(v.KP.as<double>(), v.P.as<double>(), q.sg, q.se, n * k,
                                                    v.AP.as<double>());
-    dots(s, v, v.P.as<double>(), v.AP.as<double>(), n, v.pap, nullptr);
+    dots(v, v.P.as<double>(), v.AP.as<double>(), n, v.pap, nullptr, s->st);
     k_srv_step<<<grid_for(n * k), 256, 0, s->st>>>(v.X.as<double>(), v.R.as<double>(), v.Z.as<double>(),
                                                   v.P.as<double>(), v.AP.as<double>(), q.minv.as<double>(),
                                                   v.rz.as<double>(), v.pap.as<double>(), v.act.as<int32_t>(), n, k);
-    dots(s, v, v.R.as<double>(), v.Z.as<double>(), n, v.rzn, nullptr);
+    dots(v, v.R.as<double>(), v.Z.as<double>(), n, v.rzn, nullptr, s->st);
     k_srv_dir<<<grid_for(n * k), 256, 0, s->st>>>(v.P.as<double>(), v.Z.as<double>(), v.rzn.as<double>(),
                                                  v.rz.as<double>(), v.act.as<int32_t>(), n, k);
     k_srv_rz_keep<<<grid_for(k), 256, 0, s->st>>>(v.rz.as<double>(), v.rzn.as<double>(), v.act.as<int32_t>(), k);
     CS_HIP(hipGetLastError());
-    dots(s, v, v.R.as<double>(), v.R.as<double>(), n, v.rr, nullptr);
+    dots(v, v.R.as<double>(), v.R.as<double>(), n, v.rr, nullptr, s->st);
     CS_HIP(hipMemcpyAsync(v.pin_rr(), v.rr.p, (size_t)k * 8, hipMemcpyDeviceToHost, s->st));
 }
 
@@ -543,12 +584,12 @@ void advance(cs_server* s, Request& q) {
         q.se_out.alloc((size_t)(ns * q.T) * 8);
         q.p.alloc((size_t)(ns * q.T) * 8);
         q.log10p.alloc((size_t)(ns * q.T) * 8);
-        pscan_project(n, q.T, q.c, q.L, q.B.as<double>(), q.sv.X.as<double>(), q.pj, s->st, s->blas);
+        pscan_project(n, q.T, q.c, q.L, q.B.as<double>(), q.sv.X.as<double>(), q.pj, s->side, s->blas);
         release_solve(s, q);
         pscan_scan(s->bed, q.keep.data(), q.pj, q.snps.empty() ? nullptr : q.snps.data(), ns, q.min_maf, q.prec,
                    q.freq.as<double>(), q.n_obs.as<int64_t>(), q.pass.as<int32_t>(), q.beta.as<double>(),
                    q.se_out.as<double>(), q.p.as<double>(), q.log10p.as<double>(), true);
-        q.rows = pscan_select(q.log10p.as<double>(), q.pass.as<int32_t>(), ns, q.T, q.thr, true, s->st);
+        q.rows = pscan_select(q.log10p.as<double>(), q.pass.as<int32_t>(), ns, q.T, q.thr, true, s->side);
         q.row_next = 0;
         q.phase = 2;
     } else {
@@ -557,13 +598,13 @@ void advance(cs_server* s, Request& q) {
         // operator's quantisation level)
         pscan_exact(q.pj, sb, q.rows.data() + q.row_next, q.B.as<double>(), q.sv.X.as<double>(),
                     q.sv.R.as<double>(), true, q.beta.as<double>(), q.se_out.as<double>(), q.p.as<double>(),
-                    q.log10p.as<double>(), true, s->st, s->blas);
+                    q.log10p.as<double>(), true, s->side, s->blas);
         q.exact_iterations += q.chunk_iterations;
         q.row_next += sb;
         release_solve(s, q);
     }
     if (q.row_next >= (int64_t)q.rows.size()) {
-        CS_HIP(hipStreamSynchronize(s->st));
+        CS_HIP(hipStreamSynchronize(s->side));
         release_hold(s, q);
         q.status = REQ_DONE;
         q.t_done = s->now();
@@ -587,13 +628,14 @@ int cs_server_create(cs_bed* bed, const int32_t* variants, int64_t n_variants, i
         s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)1 << 40;
         if (variants && n_variants > 0) s->variants.assign(variants, variants + n_variants);
         CS_HIP(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+        CS_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
         CS_HIP(hipStreamCreateWithFlags(&s->dst, hipStreamNonBlocking));
         for (int b = 0; b < 2; ++b) {
             CS_HIP(hipHostMalloc(&s->stage[b], kStageBytes, hipHostMallocDefault));
             CS_HIP(hipEventCreateWithFlags(&s->stage_ev[b], hipEventDisableTiming));
         }
         CS_BLAS4(rocblas_create_handle(&s->blas));
-        CS_BLAS4(rocblas_set_stream(s->blas, s->st));
+        CS_BLAS4(rocblas_set_stream(s->blas, s->side));
         // diag(K) over all samples (the requests' Jacobi preconditioners gather from it)
         cs_kinship* K = nullptr;
         if (cs_kinship_create(bed, nullptr, 0, variants, n_variants, &K) != CS_OK) throw RuntimeError(cs_last_error());
@@ -674,33 +716,15 @@ int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t*
     return guarded([&] {
         if (!s) throw InvalidArgument("cs_server_step: null handle");
         std::lock_guard<std::mutex> lk(s->mu);
+        // The step's device work runs on the server stream and the side stream; the operator,
+        // row-copy and scan functions it calls synchronise their own streams before they return.
+        // Device memory released during the step is therefore reused after one wait on the two
+        // streams at its end, with no device-wide wait (which would also wait for a client's result
+        // copies).
+        DeferFrees defer(s->st, s->side);
         int32_t completed = 0;
-        // exact-stage chunks of active requests first (they hold results already), in id order
-        for (auto& kv : s->reqs) {
-            Request& q = *kv.second;
-            if (q.status != REQ_ACTIVE || q.phase != 2 || q.sv.cols > 0) continue;
-            try {
-                (void)admit_chunk(s, q);
-            } catch (const std::exception& e) {
-                fail(s, q, e.what());
-            }
-        }
-        // admission in arrival order (a request that does not fit waits; later ones may pass it)
-        for (auto it = s->pending.begin(); it != s->pending.end();) {
-            Request& q = *s->reqs[*it];
-            bool ok = false;
-            try {
-                ok = admit(s, q);
-            } catch (const std::exception& e) {
-                fail(s, q, e.what());
-                it = s->pending.erase(it);
-                continue;
-            }
-            if (ok) it = s->pending.erase(it);
-            else ++it;
-        }
-        // one PCG iteration of every request with live columns, queued back to back on the
-        // server stream and waited for once, then advance the converged ones
+        // 1. one PCG iteration of every request with live columns, queued back to back on the
+        //    server stream and waited for once (step 3)
         std::vector<Request*> ran;
         for (auto& kv : s->reqs) {
             Request& q = *kv.second;
@@ -709,11 +733,15 @@ int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t*
                 iterate_launch(s, q);
                 ran.push_back(&q);
             } catch (const std::exception& e) {
+                defer.sync_device();
                 fail(s, q, e.what());
             }
         }
-        if (!ran.empty()) CS_HIP(hipStreamSynchronize(s->st));
-        for (Request* q : ran) iterate_finish(s, *q);
+        // 2. beside the iterations, with device work on the side stream: requests whose columns
+        //    converged in an earlier step (projection, calibration, scan; exact-stage results),
+        //    then exact-stage chunks of active requests (they hold results already) in id order,
+        //    then admission in arrival order (a request that does not fit waits; later ones may
+        //    pass it)
         for (auto& kv : s->reqs) {
             Request& q = *kv.second;
             if (q.status != REQ_ACTIVE || q.sv.cols == 0 || live(q)) continue;
@@ -721,9 +749,39 @@ int cs_server_step(cs_server* s, int32_t* n_active, int32_t* n_pending, int32_t*
                 advance(s, q);
                 if (q.status == REQ_DONE) ++completed;
             } catch (const std::exception& e) {
+                defer.sync_device();
                 fail(s, q, e.what());
             }
         }
+        for (auto& kv : s->reqs) {
+            Request& q = *kv.second;
+            if (q.status != REQ_ACTIVE || q.phase != 2 || q.sv.cols > 0) continue;
+            try {
+                (void)admit_chunk(s, q);
+            } catch (const std::exception& e) {
+                defer.sync_device();
+                fail(s, q, e.what());
+            }
+        }
+        for (auto it = s->pending.begin(); it != s->pending.end();) {
+            Request& q = *s->reqs[*it];
+            bool ok = false;
+            try {
+                ok = admit(s, q);
+            } catch (const std::exception& e) {
+                defer.sync_device();
+                fail(s, q, e.what());
+                it = s->pending.erase(it);
+                continue;
+            }
+            if (ok) it = s->pending.erase(it);
+            else ++it;
+        }
+        // 3. the iterations' residual norms and each column's convergence (converged requests
+        //    advance in the next step); the columns placed in step 2 iterate from the next step
+        if (!ran.empty()) CS_HIP(hipStreamSynchronize(s->st));
+        for (Request* q : ran) iterate_finish(s, *q);
+        CS_HIP(hipStreamSynchronize(s->side));
         int32_t na = 0;
         for (auto& kv : s->reqs) na += kv.second->status == REQ_ACTIVE ? 1 : 0;
         if (n_active) *n_active = na;
@@ -793,6 +851,9 @@ int cs_server_release(cs_server* s, int64_t id) {
         if (it->second->status == REQ_ACTIVE || it->second->status == REQ_PENDING)
             throw InvalidArgument("cs_server_release: request still running");
         if (it->second->fetching) throw InvalidArgument("cs_server_release: results are being fetched");
+        // a finished request's arrays were written before it completed (the server stream was
+        // waited for) and its fetches have ended: nothing in flight uses them
+        IdleFrees idle;
         free_results(s, *it->second);
         s->reqs.erase(it);
     });
