@@ -57,8 +57,8 @@ int cs_dev_synchronize(void);
 /* Plain device-to-device copy kernel (also the calibration reference for HBM byte counters). */
 int cs_dev_copy(void* dst, const void* src, int64_t bytes);
 /* Frees the library's cache of released device blocks (blocks of 8 MB and more are kept for
- * reuse, up to 64 GiB per process or CSAMD_DEVICE_CACHE_BYTES; destroying a genotype matrix or a
- * server also frees it).  No reference counterpart. */
+ * reuse, up to half of the device's memory per process or CSAMD_DEVICE_CACHE_BYTES; destroying a
+ * genotype matrix or a server also frees it).  No reference counterpart. */
 int cs_dev_release_cache(void);
 
 /* ---- Genotypes (replaces PlinkReader::load/openBed, libCosmicCore/src/plink_reader.cpp:28-92,
