@@ -21,7 +21,8 @@
 //    synchronised at its end.  An idle block has no work in flight and is handed out with no wait.
 //  - IdleFrees: the caller states that no work in flight uses the blocks it releases (their
 //    readers and writers have been waited for); they enter the cache as idle at once.
-// Inside either scope, small blocks are cached as well (rounded to a class, at least kSmallMin).
+// Inside either scope, small blocks are cached as well (rounded to a class, at least kSmallMin;
+// at most kSmallLimit of them in all, the oldest freed first).
 // A device-wide wait (hipFree, hipDeviceSynchronize) waits for every stream's work, including a
 // concurrent thread's copies, so removing them from a serving step keeps the device fed.
 // When hipMalloc runs out of memory, the device's cached blocks are freed and the allocation is

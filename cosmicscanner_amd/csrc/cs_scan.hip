@@ -42,7 +42,7 @@ __global__ void k_scan_epilogue(const int32_t* __restrict__ snps, int64_t n_snps
                                 double* __restrict__ log10p) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n_snps) return;
-    const int64_t j = snps[i];
+    const int64_t j = snps ? snps[i] : i;   // no list: every SNP in order
     if (j < row_lo || j >= row_lo + rows) return;
     const int64_t* c = counts + j * 4;
     const int64_t obs = c[0] + c[2] + c[3];
@@ -121,17 +121,11 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
     if (a.keep)
         for (int64_t r = 0; r < n; ++r)
             if (a.keep[r] < 0 || a.keep[r] >= N) throw RuntimeError("scan: sample index out of range");
-    std::vector<int32_t> all;
+    // no SNP list: every SNP in order, with no index list built, checked or uploaded
     const int32_t* snps = a.snps;
-    int64_t ns = a.n_snps;
-    if (!snps) {
-        all.resize((size_t)M);
-        for (int64_t j = 0; j < M; ++j) all[(size_t)j] = (int32_t)j;
-        snps = all.data();
-        ns = M;
-    }
-    int64_t lo = M, hi = -1;
-    for (int64_t i = 0; i < ns; ++i) {
+    const int64_t ns = snps ? a.n_snps : M;
+    int64_t lo = snps ? M : 0, hi = snps ? -1 : M - 1;
+    for (int64_t i = 0; snps && i < ns; ++i) {
         if (snps[i] < 0 || snps[i] >= M) throw RuntimeError("scan: variant index out of range");
         lo = std::min<int64_t>(lo, snps[i]);
         hi = std::max<int64_t>(hi, snps[i]);
@@ -206,8 +200,11 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
             md.cov_C = dC.as<double>();
         }
         // outputs on the device: the caller's arrays when they are device memory, else staging
-        DevBuf dsn((size_t)ns * 4), sfreq, snobs, spass, sdot, sgtg, sbeta, sse, sp, sl;
-        CS_HIP(hipMemcpyAsync(dsn.p, snps, (size_t)ns * 4, hipMemcpyHostToDevice, st));
+        DevBuf dsn, sfreq, snobs, spass, sdot, sgtg, sbeta, sse, sp, sl;
+        if (snps) {
+            dsn.alloc((size_t)ns * 4);
+            CS_HIP(hipMemcpyAsync(dsn.p, snps, (size_t)ns * 4, hipMemcpyHostToDevice, st));
+        }
         const int64_t nsc = model ? model->n_score : 0;
         auto dev_out = [&](void* user, DevBuf& stage, size_t bytes, bool wanted) -> void* {
             if (!wanted) return nullptr;
