@@ -30,11 +30,15 @@ struct ModelDev {
 };
 
 // One thread per requested SNP whose row falls in [row_lo, row_lo + rows): prepared-marker
-// statistics, the nr dot products, and (stats mode) the association statistics.
+// statistics, the nr dot products, and (stats mode) the association statistics.  Columns below
+// `split` come from PQ (rows of 2 x G16) with scales `scale`, the others from PQ2 (a packed-limb
+// product of at most 16 columns: rows of 2 x 16) with scales `scale2`.
 __global__ void k_scan_epilogue(const int32_t* __restrict__ snps, int64_t n_snps, int64_t row_lo,
                                 int64_t rows, const int64_t* __restrict__ counts,
                                 const long long* __restrict__ PQ, int64_t G16, int64_t nr,
-                                const double* __restrict__ scale, double min_maf,
+                                const double* __restrict__ scale, int64_t split,
+                                const long long* __restrict__ PQ2, const double* __restrict__ scale2,
+                                double min_maf,
                                 double* __restrict__ freq, int64_t* __restrict__ n_obs,
                                 int32_t* __restrict__ pass, double* __restrict__ dot,
                                 double* __restrict__ gtg, ModelDev md, double* __restrict__ beta,
@@ -63,8 +67,11 @@ __global__ void k_scan_epilogue(const int32_t* __restrict__ snps, int64_t n_snps
     pass[i] = ps;
     if (gtg) gtg[i] = g2;
     const long long* row = PQ + (j - row_lo) * 2 * G16;
+    const long long* row2 = PQ2 ? PQ2 + (j - row_lo) * 32 : nullptr;
     auto dotc = [&](int64_t col) -> double {
-        return ((double)row[col] - mean * (double)row[G16 + col]) * scale[col];
+        if (col < split) return ((double)row[col] - mean * (double)row[G16 + col]) * scale[col];
+        const int64_t c2 = col - split;
+        return ((double)row2[c2] - mean * (double)row2[16 + c2]) * scale2[c2];
     };
     if (dot) {
         for (int64_t col = 0; col < nr; ++col) dot[i * nr + col] = obs > 0 ? dotc(col) : 0.0;
@@ -165,17 +172,36 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
             Rd = dR.as<double>();
             ldr = n;
         }
-        DevBuf Rt((size_t)(b->Npad * a.nr) * sizeof(double));
-        k_scatter<<<grid_for(b->Npad * a.nr), 256, 0, st>>>(Rd, ldr, a.nr, doff.as<int32_t>(),
-                                                            didx.as<int32_t>(), N, b->Npad,
-                                                            Rt.as<double>());
+        // A last partial group of r columns with r L <= 16 runs as a packed-limb product of its own
+        // (2 MFMAs per genotype fragment instead of 6): the columns' scales and integer products
+        // do not depend on their neighbours, so the results are the same bit for bit.  Each part
+        // is scattered to the full sample space as its own row-major [Npad][columns] matrix.
+        const int64_t rr = a.nr % 16;
+        const bool split = a.nr > 16 && rr > 0 && tiling(rr, L).packed;   // rr L <= 16, LDS kernels on
+        const int64_t k1 = split ? a.nr - rr : a.nr;
+        DevBuf Rt((size_t)(b->Npad * k1) * sizeof(double)), Rt2;
+        k_scatter<<<grid_for(b->Npad * k1), 256, 0, st>>>(Rd, ldr, k1, doff.as<int32_t>(), didx.as<int32_t>(), N,
+                                                          b->Npad, Rt.as<double>());
+        if (split) {
+            Rt2.alloc((size_t)(b->Npad * rr) * sizeof(double));
+            k_scatter<<<grid_for(b->Npad * rr), 256, 0, st>>>(Rd + k1 * ldr, ldr, rr, doff.as<int32_t>(),
+                                                              didx.as<int32_t>(), N, b->Npad, Rt2.as<double>());
+        }
         CS_HIP(hipGetLastError());
-        const Tiling tl = tiling(a.nr, L);
+        const Tiling tl = tiling(k1, L);
         DevBuf mx, scl, B((size_t)(b->Npad * tl.NT * 16));
         {
             const double* Ws[1] = {Rt.as<double>()};
             int8_t* Bs[1] = {B.as<int8_t>()};
-            quantize_cols(Ws, 1, b->Npad, a.nr, tl, st, mx, scl, Bs);
+            quantize_cols(Ws, 1, b->Npad, k1, tl, st, mx, scl, Bs);
+        }
+        const Tiling tl2 = tiling(split ? rr : 1, L);
+        DevBuf mx2, scl2, B2;
+        if (split) {
+            B2.alloc((size_t)(b->Npad * tl2.NT * 16));
+            const double* Ws[1] = {Rt2.as<double>()};
+            int8_t* Bs[1] = {B2.as<int8_t>()};
+            quantize_cols(Ws, 1, b->Npad, rr, tl2, st, mx2, scl2, Bs);
         }
         // model to device
         ModelDev md{};
@@ -226,7 +252,8 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
         const int64_t row_hi = round_up(hi + 1, kPad);
         const int64_t G16 = 16 * (int64_t)tl.G;
         int64_t chunk = std::max<int64_t>(kPad, ((int64_t)(256ll << 20) / (16ll * G16)) / kPad * kPad);
-        DevBuf PQ((size_t)std::min<int64_t>(chunk, row_hi - row_lo0) * 2 * G16 * sizeof(long long));
+        DevBuf PQ((size_t)std::min<int64_t>(chunk, row_hi - row_lo0) * 2 * G16 * sizeof(long long)), PQ2;
+        if (split) PQ2.alloc((size_t)std::min<int64_t>(chunk, row_hi - row_lo0) * 2 * 16 * sizeof(long long));
         std::vector<hipEvent_t> ev;
         for (int64_t r0 = row_lo0; r0 < row_hi; r0 += chunk) {
             for (int e = 0; e < 3; ++e) {
@@ -242,13 +269,22 @@ void run_scan(cs_bed* b, const ScanArgs& a, const cs_scan_model* model, const Sc
             ga.kb_total = ga.kb_per_split = b->KBs;
             ga.G = tl.G;
             ga.Bx = ga.Bo = B.as<int8_t>();
-            ga.k = a.nr;
+            ga.k = k1;
             ga.pq = PQ.as<long long>();
             launch_geno(L, false, EPI_PQ, dim3((unsigned)(rows / 256), 1, (unsigned)tl.G), st, ga, false, tl.packed);
+            if (split) {
+                GenoArgs g2 = ga;
+                g2.G = tl2.G;
+                g2.Bx = g2.Bo = B2.as<int8_t>();
+                g2.k = rr;
+                g2.pq = PQ2.as<long long>();
+                launch_geno(L, false, EPI_PQ, dim3((unsigned)(rows / 256), 1, 1), st, g2, false, true);
+            }
             CS_HIP(hipEventRecord(ev[ev.size() - 2], st));
             k_scan_epilogue<<<grid_for(ns), 256, 0, st>>>(
                 dsn.as<int32_t>(), ns, r0, rows, counts.as<int64_t>(), PQ.as<long long>(), G16, a.nr,
-                scl.as<double>(), a.min_maf, pfreq, pnobs, ppass, pdot, pgtg, md, pbeta, pse, pp, pl);
+                scl.as<double>(), k1, split ? PQ2.as<long long>() : nullptr, split ? scl2.as<double>() : nullptr,
+                a.min_maf, pfreq, pnobs, ppass, pdot, pgtg, md, pbeta, pse, pp, pl);
             CS_HIP(hipGetLastError());
             CS_HIP(hipEventRecord(ev[ev.size() - 1], st));
         }

@@ -39,6 +39,31 @@ def test_scan_sums_keep_subset_and_snp_list(csa, ref_datasets):
         assert np.max(np.max(np.abs(dot - d0[snps]), axis=0) / den) < 2e-6
 
 
+@pytest.mark.parametrize("nr", [20, 21, 36])
+def test_scan_partial_last_group_bitwise(csa, nr):
+    """A last partial group of at most five columns (three limbs) runs as a packed-limb product of
+    its own: every column's dot products equal those of the same column scanned in a block of its
+    own, bit for bit, and match the oracle.  (21 columns: five past 16, the largest split; 36: four
+    past 32, after two full groups.)"""
+    N, M = 1100, 2600
+    reader = csa.PlinkReader.synthetic(77, N, M, 0.02, 0.5, 0.01)
+    body = reader.download()
+    rng = np.random.default_rng(5)
+    keep = rng.choice(N, 900, replace=False).astype(np.int32)
+    R = rng.standard_normal((keep.size, nr))
+    freq, nobs, ps, dot, gtg = csa.scan_sums(reader, keep, R, 0.05, None)
+    k1 = nr - nr % 16
+    _, _, _, d1, _ = csa.scan_sums(reader, keep, np.ascontiguousarray(R[:, :k1]), 0.05, None)
+    _, _, _, d2, _ = csa.scan_sums(reader, keep, np.ascontiguousarray(R[:, k1:]), 0.05, None)
+    assert np.array_equal(dot[:, :k1], d1)
+    assert np.array_equal(dot[:, k1:], d2)
+    f0, n0, p0, d0, g0 = oracle.scan_sums(body, N, M, keep, R, min_maf=0.05)
+    assert np.array_equal(freq, f0) and np.array_equal(nobs, n0) and np.array_equal(ps, p0)
+    den = np.max(np.abs(d0), axis=0)
+    assert np.max(np.max(np.abs(dot - d0), axis=0) / den) < 2e-6
+    reader.close()
+
+
 def _small_problem(seed=21, N=400, M=1500):
     body = oracle.gen_hwe_bed(seed, N, M, 0.02, 0.5, 0.01)
     G, _, _ = oracle.grm(body, N, M, alg=1, min_maf=0.01)
