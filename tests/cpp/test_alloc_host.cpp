@@ -78,7 +78,38 @@ static void worker(int seed, int rounds, long* failures) {
     }
 }
 
+// The reuse rule of a DeferFrees section: a block released inside it is not handed out again
+// before the section ends (its work may still be in flight), and is reused after it (small blocks
+// only by allocations inside a section, as the next step of a serving loop makes them).
+static int check_deferred_reuse() {
+    int bad = 0;
+    for (size_t n : {(size_t)3000, (size_t)1 << 20, (size_t)12 << 20}) {
+        void* first = nullptr;
+        {
+            csamd::DeferFrees d(nullptr);
+            DevBuf a(n);
+            first = a.p;
+            a.release();
+            DevBuf b(n);   // same size class: must not be the block released in this section
+            if (b.p == first) {
+                std::printf("block of %zu bytes reused inside its DeferFrees section\n", n);
+                ++bad;
+            }
+        }
+        {
+            csamd::DeferFrees next(nullptr);
+            DevBuf c(n), e(n);   // after the section the two held blocks are idle and come back
+            if (c.p != first && e.p != first) {
+                std::printf("block of %zu bytes not reused after its DeferFrees section\n", n);
+                ++bad;
+            }
+        }
+    }
+    return bad;
+}
+
 int main(int argc, char** argv) {
+    if (check_deferred_reuse() != 0) return 2;
     const int threads = argc > 1 ? std::atoi(argv[1]) : 4;
     const int rounds = argc > 2 ? std::atoi(argv[2]) : 200;
     if (argc > 3) g_stub_capacity = std::atol(argv[3]);

@@ -4,7 +4,9 @@ The allocator is compiled with g++ against a host-only stand-in of the HIP calls
 (tests/cpp/hip_host_stub) and the address and undefined-behaviour sanitizers, then driven by
 several threads through random allocations, growth and releases inside and outside the deferred
 and idle release sections, with and without a capped runtime (the out-of-memory retry).  Any use
-of a released block, double release or leak fails the run.
+of a released block, double release or leak fails the run.  Before that, the reuse rule of a
+deferred section is checked directly: a block released inside it is not handed out again until
+the section has ended (its device work may still run), and is handed out after.
 """
 import os
 import shutil
