@@ -637,6 +637,9 @@ def main():
                     help="exact GLS stage level of the LOCO leg (-log10 of the trait-adjusted P; 0 = the "
                          "library default 5)")
     ap.add_argument("--no-c2", action="store_true", help="skip the n=50k x m=500k legs (K.V, GRMs, scan)")
+    ap.add_argument("--miss", type=float, default=0.005,
+                    help="missing-genotype rate of the n=50k x m=500k genotypes (the sparse K.V form runs at "
+                         "0.15 %% or less)")
     ap.add_argument("--no-grm", action="store_true")
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -725,7 +728,7 @@ def main():
     kv_c2 = grm = scan = None
     if not args.no_c2 and not args.kv_only:
         N2, M2 = 50_000, 500_000
-        reader2 = csa.PlinkReader.synthetic(20261204, N2, M2, 0.01, 0.5, 0.005)
+        reader2 = csa.PlinkReader.synthetic(20261204, N2, M2, 0.01, 0.5, args.miss)
         kv_c2 = kv_leg(csa, reader2, N2, M2, 16, args.steps, args.warmup, args.prec, args.missing_mode, extras=True)
         if not args.no_grm:
             grm = grm_leg(csa, reader2, N2, M2)
