@@ -1399,6 +1399,8 @@ void csamd::kinship_compact_rows(cs_kinship* op) {
 
 bool csamd::kinship_is_compact(const cs_kinship* op) { return op->rows != nullptr; }
 
+cs_bed* csamd::kinship_rows_bed(cs_kinship* op) { return op->rows; }
+
 int64_t csamd::kinship_work_bytes(int64_t Npad, int64_t Mpad, int prec, int64_t k) {
     if (k <= 0) return 0;
     const int L = prec_limbs(prec);

@@ -73,6 +73,8 @@ int64_t bed_rows_copy_bytes(const cs_bed* parent, int64_t n);
 // full-sample products.  Needs a keep list without repeated samples.
 void kinship_compact_rows(cs_kinship* op);
 bool kinship_is_compact(const cs_kinship* op);
+// The operator's compact copy of its cohort's rows (rows in the operator's order), or null.
+cs_bed* kinship_rows_bed(cs_kinship* op);
 // Upper bound of the device workspace one apply of k columns grows in an operator whose products
 // read Npad sample rows and Mpad SNPs (both padded to 256) at precision prec.
 int64_t kinship_work_bytes(int64_t Npad, int64_t Mpad, int prec, int64_t k);

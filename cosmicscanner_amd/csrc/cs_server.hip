@@ -511,8 +511,11 @@ bool admit_chunk(cs_server* s, Request& q) {
     q.sv.bytes = cost(sb);
     s->budget_used += q.sv.bytes;
     q.B.alloc((size_t)(q.n * sb) * 8);
-    markers_device(s->bed, q.keep.data(), q.n, sel.data(), sb, q.min_maf, q.B.as<double>(), q.n, nullptr, nullptr,
-                   s->side);
+    // on the request's compact row copy when it has one (the cohort's rows in order): the same
+    // markers from a cohort-sized read
+    cs_bed* const rb = q.compact && q.K ? kinship_rows_bed(q.K) : nullptr;
+    markers_device(rb ? rb : s->bed, rb ? nullptr : q.keep.data(), q.n, sel.data(), sb, q.min_maf, q.B.as<double>(),
+                   q.n, nullptr, nullptr, s->side);
     q.chunk = sb;
     place_columns(s, q, sb);
     return true;
@@ -586,7 +589,11 @@ void advance(cs_server* s, Request& q) {
         q.log10p.alloc((size_t)(ns * q.T) * 8);
         pscan_project(n, q.T, q.c, q.L, q.B.as<double>(), q.sv.X.as<double>(), q.pj, s->side, s->blas);
         release_solve(s, q);
-        pscan_scan(s->bed, q.keep.data(), q.pj, q.snps.empty() ? nullptr : q.snps.data(), ns, q.min_maf, q.prec,
+        // the scan reads the request's compact row copy when it has one (the cohort's rows in the
+        // order of R): class counts and g^T R products from a cohort-sized matrix, the same
+        // integers as through the keep list over all samples
+        cs_bed* const rb = q.compact && q.K ? kinship_rows_bed(q.K) : nullptr;
+        pscan_scan(rb ? rb : s->bed, rb ? nullptr : q.keep.data(), q.pj, q.snps.empty() ? nullptr : q.snps.data(), ns, q.min_maf, q.prec,
                    q.freq.as<double>(), q.n_obs.as<int64_t>(), q.pass.as<int32_t>(), q.beta.as<double>(),
                    q.se_out.as<double>(), q.p.as<double>(), q.log10p.as<double>(), true);
         q.rows = pscan_select(q.log10p.as<double>(), q.pass.as<int32_t>(), ns, q.T, q.thr, true, s->side);

@@ -152,6 +152,43 @@ def test_failed_request_releases_slots(csa, world):
     srv.release(good)
 
 
+def test_compact_and_full_paths_bitwise(csa):
+    """A cohort's request on its compact row copy (products, scan and exact-stage markers read the
+    cohort's rows) and the same request on the full matrix through its keep list (a budget one
+    byte short of the copy) give the same results bit for bit, exact stage included.  (Many SNPs
+    per sample, so the copy outweighs the full-matrix workspace the second server needs.)"""
+    N, M = 1000, 12000
+    reader = csa.PlinkReader.synthetic(99, N, M, 0.05, 0.5, 0.01)
+    rng = np.random.default_rng(23)
+    n = int(0.3 * N)
+    keep = np.sort(rng.choice(N, n, replace=False)).astype(np.int32)
+    T, c = 3, 2
+    q = dict(Y=rng.standard_normal((n, T)), X=np.column_stack([np.ones(n), rng.standard_normal(n)]).astype(np.float32),
+             sigma2_g=0.6, sigma2_e=0.4, cal_snps=np.sort(rng.choice(M, 16, replace=False)).astype(np.int32),
+             keep=keep, exact_log10p=1.0)
+    big = csa.ScanServer(reader, max_columns=128)
+    rid = big.submit(**q)
+    big.run_until_idle()
+    got_c = big.result(rid)
+    st = got_c["stats"]
+    assert st["compact_rows"] == 1 and st["n_exact"] > 0
+    res = M * (4 * T + 3) * 8
+    tight = csa.ScanServer(reader, max_columns=128, hbm_budget_bytes=st["admit_bytes"] + res - 1)
+    rid2 = tight.submit(**q)
+    tight.run_until_idle()
+    got_f = tight.result(rid2)
+    assert got_f["stats"]["compact_rows"] == 0
+    same(got_c, got_f)
+    for k in ("n_obs", "passes"):
+        assert np.array_equal(got_c[k], got_f[k]), k
+    assert got_c["stats"]["n_exact"] == got_f["stats"]["n_exact"]
+    # (exact_iterations sums over the exact-stage chunks, whose size follows the free budget: the
+    # tight server runs more, smaller chunks; each column's arithmetic is its own either way)
+    big.close()
+    tight.close()
+    reader.close()
+
+
 def test_budget_holds_unreleased_results(csa, world):
     reader, reqs = world
     N = reader.num_samples
